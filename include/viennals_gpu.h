@@ -1,0 +1,153 @@
+/*
+ * viennals_gpu.h -- C ABI of the MI355X-native level-set advection library
+ * (libviennals_gpu.so).  Plain pointers and sizes only.
+ *
+ * Each entry point replaces one operation of the reference ViennaLS v5.8.3
+ * (header-only C++, include/viennals/...); the reference interface is cited
+ * beside each declaration.  All functions return 0 on success and a negative
+ * code on error; lsg_last_error() returns the message of the last error of
+ * the calling thread (the role of ViennaCore's Logger error in the
+ * reference, e.g. lsAdvect.hpp:1019-1027).
+ *
+ * Level sets live in GPU memory.  A level set is exchanged with the host as
+ * its run-start list in HRLE iteration order (x fastest): every defined grid
+ * point with its value, and every undefined grid point that starts a run of
+ * a new sign, with value +DBL_MAX or -DBL_MAX (Domain::POS_VALUE/NEG_VALUE,
+ * lsDomain.hpp:51-52).  Entry 0 is the grid minimum point.
+ */
+#ifndef VIENNALS_GPU_H
+#define VIENNALS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lsg_domain lsg_domain;
+typedef struct lsg_advect lsg_advect;
+
+/* Boundary conditions: viennahrle::BoundaryType numbering
+ * (python/viennals/_core.pyi:26-30). */
+enum { LSG_REFLECTIVE = 0, LSG_INFINITE = 1, LSG_PERIODIC = 2, LSG_POS_INFINITE = 3, LSG_NEG_INFINITE = 4 };
+
+/* Spatial schemes: SpatialSchemeEnum (lsAdvectIntegrationSchemes.hpp:10-23). */
+enum {
+  LSG_ENGQUIST_OSHER_1ST_ORDER = 0,
+  LSG_ENGQUIST_OSHER_2ND_ORDER = 1,
+  LSG_LAX_FRIEDRICHS_1ST_ORDER = 2,
+  LSG_LAX_FRIEDRICHS_2ND_ORDER = 3,
+  LSG_LOCAL_LAX_FRIEDRICHS_ANALYTICAL_1ST_ORDER = 4,
+  LSG_LOCAL_LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 5,
+  LSG_LOCAL_LOCAL_LAX_FRIEDRICHS_2ND_ORDER = 6,
+  LSG_LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 7,
+  LSG_LOCAL_LAX_FRIEDRICHS_2ND_ORDER = 8,
+  LSG_STENCIL_LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 9,
+  LSG_WENO_3RD_ORDER = 10,
+  LSG_WENO_5TH_ORDER = 11
+};
+
+/* Temporal schemes: TemporalSchemeEnum (lsAdvectIntegrationSchemes.hpp:31-35). */
+enum { LSG_FORWARD_EULER = 0, LSG_RUNGE_KUTTA_2ND_ORDER = 1, LSG_RUNGE_KUTTA_3RD_ORDER = 2 };
+
+/* BooleanOperationEnum (lsBooleanOperation.hpp:27-33). */
+enum { LSG_INTERSECT = 0, LSG_UNION = 1, LSG_RELATIVE_COMPLEMENT = 2 };
+
+/* Built-in velocity functors (evaluated on the GPU), replacing the virtual
+ * VelocityField<T> (lsVelocityField.hpp:11-41):
+ *   getScalarVelocity = b                                   (LSG_VEL_CONSTANT)
+ *                     = b + (n[dir] > 0 ? aPos : aNeg) * |n[dir]|^k
+ *                                                           (LSG_VEL_NORMAL_POWER)
+ *   getVectorVelocity = (vec[0], vec[1], vec[2])            (both kinds) */
+enum { LSG_VEL_CONSTANT = 0, LSG_VEL_NORMAL_POWER = 1 };
+
+const char *lsg_last_error(void);
+
+/* ---- Domain<T,D> (lsDomain.hpp:27-269) ---------------------------------- */
+
+/* Domain(bounds, boundaryConditions, gridDelta) (lsDomain.hpp:68-78); gmin/gmax
+ * are the index bounds floor(b0/dx), ceil(b1/dx) per dimension. D is 2 or 3. */
+int lsg_domain_create(int D, const int32_t *gmin, const int32_t *gmax, const int *bc, double grid_delta,
+                      lsg_domain **out);
+int lsg_domain_destroy(lsg_domain *d);
+/* Domain::deepCopy (lsDomain.hpp:124-129). */
+int lsg_domain_deep_copy(lsg_domain *dst, const lsg_domain *src);
+/* Domain::insertPoints (lsDomain.hpp:135-138): n defined points idx[n*D], val[n]. */
+int lsg_domain_insert_points(lsg_domain *d, int64_t n, const int32_t *idx, const double *val, int width);
+/* Set / get the run-start list (see top of file). */
+int lsg_domain_set_runs(lsg_domain *d, int64_t n, const int32_t *idx, const double *val, int width);
+int lsg_domain_num_runs(lsg_domain *d, int64_t *n);
+int lsg_domain_get_runs(lsg_domain *d, int32_t *idx, double *val, int64_t cap);
+/* Domain::getNumberOfPoints (lsDomain.hpp:156), getLevelSetWidth (:158). */
+int lsg_domain_num_points(lsg_domain *d, int64_t *n);
+int lsg_domain_width(lsg_domain *d, int *width);
+
+/* ---- MakeGeometry (lsMakeGeometry.hpp) --------------------------------- */
+/* makeSphere (lsMakeGeometry.hpp:177-241). */
+int lsg_make_sphere(lsg_domain *d, const double *origin, double radius);
+/* makePlane (lsMakeGeometry.hpp:245-360): signed Euclidean distance band. */
+int lsg_make_plane(lsg_domain *d, const double *origin, const double *normal);
+/* makeBox (lsMakeGeometry.hpp:363-419): signed Euclidean distance band. */
+int lsg_make_box(lsg_domain *d, const double *min_corner, const double *max_corner);
+
+/* ---- band maintenance ----------------------------------------------------- */
+/* Expand<T,D>(ls, width).apply() (lsExpand.hpp:44-172). */
+int lsg_expand(lsg_domain *d, int width);
+/* Reduce<T,D>(ls, width).apply() (lsReduce.hpp:55-134). */
+int lsg_reduce(lsg_domain *d, int width);
+/* Prune<T,D>(ls) with setRemoveStrayZeros (lsPrune.hpp:74-194). */
+int lsg_prune(lsg_domain *d, int remove_stray_zeros);
+/* BooleanOperation<T,D>(a, b, op).apply() (lsBooleanOperation.hpp:59-213,
+ * 329-366); the result is pruned as with the default pruneResult. */
+int lsg_boolean(lsg_domain *a, const lsg_domain *b, int op);
+/* Advect::rebuildLS (lsAdvect.hpp:238-429) on a single level set. */
+int lsg_rebuild(lsg_domain *d);
+
+/* ---- Advect<T,D> (lsAdvect.hpp:54-1044) ---------------------------------- */
+int lsg_advect_create(lsg_advect **out);
+int lsg_advect_destroy(lsg_advect *a);
+/* insertNextLevelSet (lsAdvect.hpp:856), clearLevelSets (:860). */
+int lsg_advect_insert_next_level_set(lsg_advect *a, lsg_domain *d);
+int lsg_advect_clear_level_sets(lsg_advect *a);
+/* setVelocityField (lsAdvect.hpp:864) with a built-in functor. */
+int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, double b, double a_pos, double a_neg,
+                                    const double *vec3);
+int lsg_advect_set_spatial_scheme(lsg_advect *a, int scheme);     /* :936 */
+int lsg_advect_set_temporal_scheme(lsg_advect *a, int scheme);    /* :949 */
+int lsg_advect_set_advection_time(lsg_advect *a, double time);    /* :873 */
+int lsg_advect_set_time_step_ratio(lsg_advect *a, double cfl);    /* :885 */
+int lsg_advect_set_single_step(lsg_advect *a, int single);        /* :879 */
+int lsg_advect_set_dissipation_alpha(lsg_advect *a, double alpha);/* :955 */
+int lsg_advect_set_check_dissipation(lsg_advect *a, int check);   /* :958 */
+int lsg_advect_set_calculate_normal_vectors(lsg_advect *a, int c);/* :891 */
+int lsg_advect_set_ignore_voids(lsg_advect *a, int iv);           /* :899; only 0 supported */
+int lsg_advect_set_adaptive_time_stepping(lsg_advect *a, int on, unsigned subdivisions); /* :904; no effect on one level set */
+int lsg_advect_set_update_point_data(lsg_advect *a, int update);  /* :962; point data is not carried */
+/* Advect::prepareLS (lsAdvect.hpp:973-1015). */
+int lsg_advect_prepare_ls(lsg_advect *a);
+/* Advect::apply (lsAdvect.hpp:1017-1043). */
+int lsg_advect_apply(lsg_advect *a);
+int lsg_advect_get_advected_time(lsg_advect *a, double *t);       /* :920 */
+int lsg_advect_get_current_time_step(lsg_advect *a, double *dt);  /* :923 */
+int lsg_advect_get_number_of_time_steps(lsg_advect *a, unsigned *n); /* :926 */
+/* No counterpart in the reference: active-point updates of the last apply()
+ * (sum over stages of points with |phi| <= 0.5) and, when profiling is on,
+ * the summed duration (ms), launch count and algorithmic bytes
+ * (8*band entries + 8*active points per launch) of the rate kernel. */
+int lsg_advect_get_point_updates(lsg_advect *a, double *n);
+int lsg_advect_set_profiling(lsg_advect *a, int on);
+int lsg_advect_get_rate_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes);
+
+/* Finite-difference kernels FiniteDifferences::weno3/weno5
+ * (lsFiniteDifferences.hpp:58-123) evaluated on the GPU for n stencils:
+ * x[n*(2r+1)] -> out[n]. */
+int lsg_weno_eval(int order, int64_t n, const double *x, double dx, int plus, double *out);
+
+/* Device synchronisation (for timing by callers). */
+int lsg_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

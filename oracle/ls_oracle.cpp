@@ -1,0 +1,1323 @@
+// CPU oracle for the ViennaLS level-set advection path (test infrastructure
+// only; see ls_oracle.hpp for the model and its parity status).
+//
+// Every function below cites the reference file:line it restates.  Floating
+// point expressions keep the reference's operation order; the file is built
+// with -ffp-contract=off so no FMA contraction changes a rounding.
+#include "ls_oracle.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <sstream>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace lso {
+
+static inline double pow2(double v) { return v * v; }
+// std::max / std::min semantics (first argument wins on ties and NaN)
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+static constexpr double EPS = std::numeric_limits<double>::epsilon();
+
+// ----------------------------------------------------------------- Band ---
+
+template <int D> void Band<D>::buildRows() {
+  rkeys.clear();
+  rstart.clear();
+  for (size_t i = 0; i < idx.size(); ++i) {
+    int64_t k = rowKey<D>(idx[i]);
+    if (rkeys.empty() || rkeys.back() != k) {
+      rkeys.push_back(k);
+      rstart.push_back((int64_t)i);
+    }
+  }
+  rstart.push_back((int64_t)idx.size());
+}
+
+template <int D> size_t Band<D>::numDefined() const {
+  size_t n = 0;
+  for (double v : val) n += !isUndef(v);
+  return n;
+}
+
+struct RowRef {
+  int64_t start, end;  // entries of the row; start==end if absent (start = insertion point)
+};
+
+template <int D> static RowRef findRow(const Band<D> &b, int64_t key) {
+  auto it = std::lower_bound(b.rkeys.begin(), b.rkeys.end(), key);
+  size_t r = (size_t)(it - b.rkeys.begin());
+  if (it != b.rkeys.end() && *it == key) return {b.rstart[r], b.rstart[r + 1]};
+  int64_t s = b.rstart[r];
+  return {s, s};
+}
+
+// Value of the run containing x in a row: the exact entry, else the last
+// entry before x (within the row, else before the row).
+template <int D> static inline double valueInRow(const Band<D> &b, const RowRef &r, int32_t x) {
+  int64_t lo = r.start, hi = r.end;  // find last entry with idx.x <= x
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (b.idx[mid][0] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  int64_t i = lo - 1;  // may be r.start-1 (predecessor before the row)
+  if (i < 0) i = 0;
+  return b.val[i];
+}
+
+template <int D> double Band<D>::value(const Idx<D> &p) const {
+  RowRef r = findRow(*this, rowKey<D>(p));
+  return valueInRow(*this, r, p[0]);
+}
+
+// Compress a candidate list into the canonical run-start list: keep every
+// defined point and every undefined point whose value differs from the
+// previous candidate's value (or follows a defined point).
+template <int D>
+static void compressInto(Band<D> &out, const std::vector<Idx<D>> &c, const std::vector<double> &v) {
+  out.idx.clear();
+  out.val.clear();
+  out.idx.reserve(c.size());
+  out.val.reserve(c.size());
+  for (size_t i = 0; i < c.size(); ++i) {
+    if (i == 0 || !isUndef(v[i]) || v[i] != v[i - 1]) {
+      out.idx.push_back(c[i]);
+      out.val.push_back(v[i]);
+    }
+  }
+  out.buildRows();
+}
+
+template <int D> static Idx<D> rowBase(const Idx<D> &p) { Idx<D> q = p; q[0] = 0; return q; }
+
+// Lexicographically next row inside the grid; false if none.
+template <int D> static bool nextRow(const Grid<D> &g, Idx<D> &r) {
+  for (int d = 1; d < D; ++d) {
+    if (r[d] < g.hi[d]) { r[d]++; return true; }
+    r[d] = g.lo[d];
+  }
+  return false;
+}
+
+// Generic per-point operation over a candidate set.  Mode STAR: candidates are
+// all grid points whose star neighbourhood (radius 1, boundary-mapped)
+// contains a run start of `in`, plus lexicographic successors -- the visit
+// set of HRLE's ConstSparseStarIterator<.,1>.  Mode SELF: candidates are the
+// run starts of `in` (ConstSparseIterator visit set).  `func(p, center,
+// nb[2D])` gets neighbours in the star-iterator order: nb[i] = f(p+e_i),
+// nb[D+i] = f(p-e_i).
+enum OpMode { STAR, SELF };
+
+template <int D, class F>
+static void pointOp(const Band<D> &in, Band<D> &out, OpMode mode, F &&func) {
+  const Grid<D> &g = in.g;
+  // candidate rows
+  std::vector<Idx<D>> rows;
+  const size_t R = in.rkeys.size();
+  rows.reserve(R * (mode == STAR ? (2 * D) : 1));
+  for (size_t r = 0; r < R; ++r) {
+    Idx<D> base = rowBase<D>(in.idx[in.rstart[r]]);
+    rows.push_back(base);
+    if (mode == STAR) {
+      for (int k = 1; k < D; ++k)
+        for (int s = -1; s <= 1; s += 2) {
+          Idx<D> q = base;
+          q[k] = g.map(k, (int64_t)base[k] + s);
+          rows.push_back(q);
+        }
+    }
+  }
+  if (mode == STAR && g.bounded(0)) {
+    size_t m = rows.size();
+    for (size_t i = 0; i < m; ++i) {
+      Idx<D> q = rows[i];
+      if (nextRow<D>(g, q)) rows.push_back(q);
+    }
+  }
+  std::sort(rows.begin(), rows.end(), idxLess<D>);
+  rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+
+  const size_t NR = rows.size();
+  std::vector<std::vector<Idx<D>>> candRow(NR);
+  std::vector<std::vector<double>> valRow(NR);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (size_t ri = 0; ri < NR; ++ri) {
+    const Idx<D> &p0 = rows[ri];
+    RowRef self = findRow(in, rowKey<D>(p0));
+    RowRef nbr[2 * D];
+    for (int k = 1; k < D; ++k) {
+      Idx<D> q = p0;
+      q[k] = g.map(k, (int64_t)p0[k] + 1);
+      nbr[k] = findRow(in, rowKey<D>(q));
+      q[k] = g.map(k, (int64_t)p0[k] - 1);
+      nbr[D + k] = findRow(in, rowKey<D>(q));
+    }
+    std::vector<int32_t> xs;
+    for (int64_t e = self.start; e < self.end; ++e) {
+      int32_t x = in.idx[e][0];
+      xs.push_back(x);
+      if (mode == STAR) {
+        xs.push_back(g.map(0, (int64_t)x - 1));
+        xs.push_back(g.map(0, (int64_t)x + 1));
+      }
+    }
+    if (mode == STAR) {
+      for (int k = 1; k < D; ++k)
+        for (int s = 0; s < 2; ++s) {
+          const RowRef &rr = nbr[s * D + k];
+          for (int64_t e = rr.start; e < rr.end; ++e) xs.push_back(in.idx[e][0]);
+        }
+      if (g.bounded(0)) xs.push_back(g.lo[0]);
+      std::sort(xs.begin(), xs.end());
+      xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
+      size_t m = xs.size();
+      for (size_t i = 0; i < m; ++i)
+        if (xs[i] < g.hi[0]) xs.push_back(xs[i] + 1);
+    }
+    std::sort(xs.begin(), xs.end());
+    xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
+
+    auto &cr = candRow[ri];
+    auto &vr = valRow[ri];
+    cr.reserve(xs.size());
+    vr.reserve(xs.size());
+    double nb[2 * D];
+    for (int32_t x : xs) {
+      Idx<D> p = p0;
+      p[0] = x;
+      double c = valueInRow(in, self, x);
+      nb[0] = valueInRow(in, self, g.map(0, (int64_t)x + 1));
+      nb[D] = valueInRow(in, self, g.map(0, (int64_t)x - 1));
+      for (int k = 1; k < D; ++k) {
+        nb[k] = valueInRow(in, nbr[k], x);
+        nb[D + k] = valueInRow(in, nbr[D + k], x);
+      }
+      cr.push_back(p);
+      vr.push_back(func(p, c, nb));
+    }
+  }
+  std::vector<Idx<D>> cand;
+  std::vector<double> vals;
+  size_t tot = 0;
+  for (auto &c : candRow) tot += c.size();
+  cand.reserve(tot);
+  vals.reserve(tot);
+  for (size_t ri = 0; ri < NR; ++ri) {
+    cand.insert(cand.end(), candRow[ri].begin(), candRow[ri].end());
+    vals.insert(vals.end(), valRow[ri].begin(), valRow[ri].end());
+  }
+  out.g = g;
+  compressInto<D>(out, cand, vals);
+}
+
+// ------------------------------------------------------------ band ops ---
+
+// Expand<T,D>::apply (lsExpand.hpp:44-172)
+template <int D> void expand(Band<D> &ls, int width) {
+  if (width <= ls.width) return;      // :52-53
+  if (ls.numDefined() == 0) return;   // :55-56
+  const double totalLimit = width * 0.5;  // :58
+  const int startWidth = ls.width;
+  const int cycles = width - startWidth;
+  for (int cyc = 0; cyc < cycles; ++cyc) {
+    const double limit = (startWidth + cyc + 1) * 0.5;  // :67
+    Band<D> out;
+    pointOp<D>(ls, out, STAR, [&](const Idx<D> &, double c, const double *nb) -> double {
+      if (std::fabs(c) <= totalLimit) return c;  // :108-112
+      if (c > -EPS) {                             // :114-134
+        double dist = POS;
+        for (int i = 0; i < 2 * D; ++i) {
+          double nv = nb[i] + 1.0;
+          if (dist > nv) dist = nv;
+        }
+        return (dist <= limit) ? dist : POS;
+      } else {                                    // :135-156
+        double dist = NEG;
+        for (int i = 0; i < 2 * D; ++i) {
+          double nv = nb[i] - 1.0;
+          if (dist < nv) dist = nv;
+        }
+        return (dist >= -limit) ? dist : NEG;
+      }
+    });
+    out.width = ls.width;
+    ls = std::move(out);
+  }
+  ls.width = width;  // :171
+}
+
+// Reduce<T,D>::apply (lsReduce.hpp:55-134)
+template <int D> void reduce(Band<D> &ls, int width) {
+  if (width >= ls.width) return;  // :63-64
+  const double valueLimit = width * 0.5;
+  Band<D> out;
+  pointOp<D>(ls, out, SELF, [&](const Idx<D> &, double c, const double *) -> double {
+    if (!isUndef(c) && std::fabs(c) <= valueLimit) return c;  // :107-111
+    return (c < 0) ? NEG : POS;                                // :113-117
+  });
+  out.width = width;
+  ls = std::move(out);
+}
+
+// Prune<T,D>::apply (lsPrune.hpp:74-194)
+static inline bool pIsNeg(double a) { return a <= -EPS; }                 // :22-24
+static inline bool pSignDiff(double a, double b) { return pIsNeg(a) ^ pIsNeg(b); }  // :27-29
+static inline bool pSignDiffOrZero(double a, double b) {                 // :32-36
+  if (a == 0. || b == 0.) return true;
+  return pSignDiff(a, b);
+}
+static inline bool pMonotone(double a, double c) { return a == 0. || c == 0. || pSignDiff(a, c); }  // :50-52
+
+template <int D> void prune(Band<D> &ls, bool removeStrayZeros) {
+  if (ls.numDefined() == 0) return;  // :81-83
+  Band<D> out;
+  pointOp<D>(ls, out, SELF, [&](const Idx<D> &, double c, const double *nb) -> double {
+    bool centerSign = pIsNeg(c);
+    if (!isUndef(c)) {
+      bool keep = true;
+      if (std::fabs(c) != 0.) {  // :126-133
+        keep = false;
+        for (int i = 0; i < 2 * D; ++i) {
+          if (removeStrayZeros ? pSignDiffOrZero(c, nb[i]) : pSignDiff(c, nb[i])) { keep = true; break; }
+        }
+      }
+      if (removeStrayZeros && std::fabs(c) == 0.) {  // :135-161
+        for (int i = 0; i < D; ++i) {
+          double negVal = nb[i];
+          double posVal = nb[D + i];
+          if (!pMonotone(negVal, posVal)) {
+            return std::fabs(negVal) < std::fabs(posVal) ? negVal : posVal;
+          }
+        }
+      }
+      if (keep) return c;
+    }
+    return centerSign ? NEG : POS;  // :168-179
+  });
+  out.width = 2;  // :193
+  ls = std::move(out);
+}
+
+// BooleanOperation::booleanOpInternal (lsBooleanOperation.hpp:59-213) on the
+// union of both run-start lists; comp(a, b) returns the new value.
+template <int D, class C> void booleanOp(Band<D> &A, const Band<D> &B, C &&comp, bool pruneResult = true) {
+  const Grid<D> &g = A.g;
+  std::vector<Idx<D>> cand;
+  cand.reserve(A.idx.size() + B.idx.size());
+  std::merge(A.idx.begin(), A.idx.end(), B.idx.begin(), B.idx.end(), std::back_inserter(cand), idxLess<D>);
+  cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+  std::vector<double> vals(cand.size());
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < cand.size(); ++i) {
+    double a = A.value(cand[i]);
+    double b = B.value(cand[i]);
+    double r = comp(a, b);
+    if (isUndef(r)) r = (r < 0) ? NEG : POS;  // :115-118
+    vals[i] = r;
+  }
+  Band<D> out;
+  out.g = g;
+  compressInto<D>(out, cand, vals);
+  out.width = A.width;  // :201
+  if (pruneResult) {    // :203-210
+    prune(out, true);
+    prune(out, false);
+  }
+  A = std::move(out);
+}
+
+// minComp / maxComp / relativeComplementComp (lsBooleanOperation.hpp:250-269)
+inline double minComp(double a, double b) { return (a < b) ? a : b; }
+inline double maxComp(double a, double b) { return (a > b) ? a : b; }
+
+// Advect::rebuildLS (lsAdvect.hpp:238-429), cutoff 1.0 / width 2 (non-SLLF)
+template <int D> void rebuild(Band<D> &ls, double cutoff = 1.0, int finalWidth = 2) {
+  Band<D> out;
+  pointOp<D>(ls, out, STAR, [&](const Idx<D> &, double c, const double *nb) -> double {
+    if (std::fabs(c) <= 1.0) {  // :306
+      int k = 0;
+      for (; k < 2 * D; k++)    // :309-312
+        if (std::signbit(nb[k] - 1e-7) != std::signbit(c + 1e-7)) break;
+      if (k != 2 * D) {
+        if (c > 0.5) {          // :316-333
+          int j = 0;
+          for (; j < 2 * D; j++)
+            if (std::fabs(nb[j]) <= 1.0 && nb[j] < -0.5) break;
+          return (j == 2 * D) ? c : 0.5;
+        } else if (c < -0.5) {  // :334-352
+          int j = 0;
+          for (; j < 2 * D; j++)
+            if (std::fabs(nb[j]) <= 1.0 && nb[j] > 0.5) break;
+          return (j == 2 * D) ? c : -0.5;
+        }
+        return c;               // :353-358
+      }
+      return (c < 0) ? NEG : POS;  // :359-364
+    }
+    if (c >= 0) {               // :367-389
+      double dist = POS;
+      for (int i = 0; i < 2 * D; i++) {
+        double v = nb[i];
+        if (std::fabs(v) <= 1.0 && (v < 0.)) {
+          if (dist > v + 1.0) dist = v + 1.0;
+        }
+      }
+      return (dist <= cutoff) ? dist : POS;
+    }
+    double dist = NEG;          // :390-413
+    for (int i = 0; i < 2 * D; i++) {
+      double v = nb[i];
+      if (std::fabs(v) <= 1.0 && (v > 0)) {
+        if (dist < v - 1.0) dist = v - 1.0;
+      }
+    }
+    return (dist >= -cutoff) ? dist : NEG;
+  });
+  out.width = finalWidth;
+  ls = std::move(out);
+}
+
+// Check<T,D>::apply (lsCheck.hpp:78-154); returns number of faults.
+static int statusFromDistance(double value) {  // lsCheck.hpp:51-58
+  int x = static_cast<int>(value);
+  if (value >= 0.) return (value <= static_cast<double>(x) + 0.5) ? x : x + 1;
+  return (value >= static_cast<double>(x) - 0.5) ? x : x - 1;
+}
+template <int D> int check(const Band<D> &ls, std::string *msg) {
+  Band<D> scratch;
+  int faults = 0;
+  std::ostringstream oss;
+  pointOp<D>(ls, scratch, STAR, [&](const Idx<D> &p, double c, const double *nb) -> double {
+    int f = 0;
+    if (!isUndef(c)) {
+      for (int i = 0; i < 2 * D; ++i) {
+        if (!isUndef(nb[i])) {
+          if (std::abs(statusFromDistance(c) - statusFromDistance(nb[i])) > 1) ++f;
+        } else if (nb[i] >= 0.) {
+          if (c < -0.5) ++f;
+        } else if (c > 0.5) {
+          ++f;
+        }
+      }
+    } else {
+      for (int i = 0; i < 2 * D; ++i)
+        if (isUndef(nb[i]) && ((c <= -EPS) != (nb[i] <= -EPS))) ++f;
+    }
+    if (f) {
+#pragma omp critical
+      {
+        faults += f;
+        if (msg && faults < 20) {
+          oss << "fault at (";
+          for (int d = 0; d < D; ++d) oss << p[d] << (d + 1 < D ? "," : ")");
+          oss << " value " << c << "\n";
+        }
+      }
+    }
+    return c;
+  });
+  if (msg) *msg = oss.str();
+  return faults;
+}
+
+// Canonical run list from defined points (restatement of the role of
+// HRLE's FillDomainWithSignedDistance, lsDomain.hpp:135-138): the run after a
+// defined point is negative iff that point's value is negative; the run
+// before the first defined point takes the first point's sign.  (Parity
+// unpinned: HRLE source absent.)
+template <int D>
+void fromPoints(Band<D> &ls, std::vector<std::pair<Idx<D>, double>> pts, int width) {
+  const Grid<D> &g = ls.g;
+  std::sort(pts.begin(), pts.end(), [](const auto &a, const auto &b) { return idxLess<D>(a.first, b.first); });
+  std::vector<Idx<D>> c;
+  std::vector<double> v;
+  Idx<D> gmin = g.minPoint();
+  double first = pts.empty() ? POS : ((pts[0].second < 0) ? NEG : POS);
+  c.push_back(gmin);
+  v.push_back(first);
+  for (size_t i = 0; i < pts.size(); ++i) {
+    const Idx<D> &p = pts[i].first;
+    if (i > 0 && p == pts[i - 1].first) continue;
+    if (p == gmin) { v[0] = pts[i].second; }
+    else { c.push_back(p); v.push_back(pts[i].second); }
+    // successor
+    Idx<D> s = p;
+    bool ok = true;
+    int d = 0;
+    for (; d < D; ++d) {
+      if (s[d] < g.hi[d]) { s[d]++; break; }
+      s[d] = g.lo[d];
+    }
+    if (d == D) ok = false;
+    if (ok && (i + 1 >= pts.size() || idxLess<D>(s, pts[i + 1].first))) {
+      c.push_back(s);
+      v.push_back(pts[i].second < 0 ? NEG : POS);
+    }
+  }
+  ls.width = width;
+  compressInto<D>(ls, c, v);
+}
+
+// MakeGeometry::makeSphere (lsMakeGeometry.hpp:177-241)
+template <int D> void makeSphere(Band<D> &ls, const double *origin, double radius) {
+  const double gridDelta = ls.g.delta;
+  Idx<D> index, endIndex;
+  for (int i = 0; i < D; ++i) {
+    index[i] = (int32_t)((origin[i] - radius) / gridDelta - 1);
+    endIndex[i] = (int32_t)((origin[i] + radius) / gridDelta + 1);
+  }
+  const double initialWidth = 2.;
+  const double valueLimit = initialWidth * 0.5 * gridDelta + 1e-5;
+  const double radius2 = radius * radius;
+  std::vector<std::pair<Idx<D>, double>> pts;
+  const Idx<D> minIndex = index;
+  auto lessIdx = [](const Idx<D> &a, const Idx<D> &b) { return idxLess<D>(a, b); };
+  while (lessIdx(index, endIndex)) {
+    double distance = std::numeric_limits<double>::max();
+    for (int i = 0; i < D; ++i) {
+      double y = (index[(i + 1) % D] * gridDelta) - origin[(i + 1) % D];
+      double z = 0;
+      if (D == 3) z = (index[(i + 2) % D] * gridDelta) - origin[(i + 2) % D];
+      double x = radius2 - y * y - z * z;
+      if (x < 0.) continue;
+      double dirRadius = std::fabs((index[i] * gridDelta) - origin[i]) - std::sqrt(x);
+      if (std::fabs(dirRadius) < std::fabs(distance)) distance = dirRadius;
+    }
+    if (std::fabs(distance) <= valueLimit) pts.push_back({index, distance / gridDelta});
+    int dim = 0;
+    for (; dim < D - 1; ++dim) {
+      if (index[dim] < endIndex[dim]) break;
+      index[dim] = minIndex[dim];
+    }
+    ++index[dim];
+  }
+  for (auto &pp : pts)
+    for (int j = 0; j < D; ++j)
+      if (ls.g.bc[j] == PERIODIC) pp.first[j] = ls.g.map(j, pp.first[j]);
+  fromPoints<D>(ls, pts, 2);
+}
+
+// ------------------------------------------------------------- schemes ---
+
+// FiniteDifferences::weno3 (lsFiniteDifferences.hpp:58-83)
+double weno3(const double *x, double delta, bool plus, double eps = 1e-6) {
+  double d[3];
+  if (plus) {
+    d[0] = x[2] - x[1];
+    d[1] = x[3] - x[2];
+    d[2] = x[4] - x[3];
+    double N = eps + pow2(d[2] - d[1]);
+    double Dn = eps + pow2(d[1] - d[0]);
+    double D2 = Dn * Dn;
+    double wp = D2 / (D2 + 2 * N * N);
+    return (d[0] + d[1] - wp * (d[2] - 2 * d[1] + d[0])) / (2 * delta);
+  }
+  d[0] = x[1] - x[0];
+  d[1] = x[2] - x[1];
+  d[2] = x[3] - x[2];
+  double N = eps + pow2(d[1] - d[0]);
+  double Dn = eps + pow2(d[2] - d[1]);
+  double D2 = Dn * Dn;
+  double wp = D2 / (D2 + 2 * N * N);
+  return (d[1] + d[2] - wp * (d[0] - 2 * d[1] + d[2])) / (2 * delta);
+}
+
+// FiniteDifferences::weno5 (lsFiniteDifferences.hpp:88-123)
+double weno5(const double *x, double dx, bool plus, double eps = 1e-6) {
+  double d[5];
+  if (!plus) {
+    for (int i = 0; i < 5; ++i) d[i] = x[i + 1] - x[i];
+  } else {
+    for (int i = 0; i < 5; ++i) d[i] = x[6 - i] - x[5 - i];
+  }
+  double s1 = (13.0 / 12.0) * pow2(d[0] - 2 * d[1] + d[2]) + (1.0 / 4.0) * pow2(d[0] - 4 * d[1] + 3 * d[2]);
+  double s2 = (13.0 / 12.0) * pow2(d[1] - 2 * d[2] + d[3]) + (1.0 / 4.0) * pow2(d[1] - d[3]);
+  double s3 = (13.0 / 12.0) * pow2(d[2] - 2 * d[3] + d[4]) + (1.0 / 4.0) * pow2(3 * d[2] - 4 * d[3] + d[4]);
+  double eps_scaled = eps * dx * dx;
+  double al1 = 0.1 / (eps_scaled + s1);
+  double al2 = 0.6 / (eps_scaled + s2);
+  double al3 = 0.3 / (eps_scaled + s3);
+  double alsum = al1 + al2 + al3;
+  double p1 = (1.0 / 6.0) * (2 * d[0] - 7 * d[1] + 11 * d[2]);
+  double p2 = (1.0 / 6.0) * (-d[1] + 5 * d[2] + 2 * d[3]);
+  double p3 = (1.0 / 6.0) * (2 * d[2] + 5 * d[3] - d[4]);
+  return (al1 * p1 + al2 * p2 + al3 * p3) / (alsum * dx);
+}
+
+// Per-point scheme evaluation.  Returns (H, dissipation).  `alphaAcc`
+// accumulates the per-direction alpha maxima (LLF/LLLF finalAlphas).
+template <int D> struct SchemeCtx {
+  const Band<D> *ls;
+  int spatial;
+  double dx;
+  bool calcNormals;
+  double alphaFactor;
+  double globalAlphas[3];  // LF only
+  const Velocity *vel;
+  double f(const Idx<D> &c, int dim, int off) const {
+    Idx<D> p = c;
+    p[dim] = ls->g.map(dim, (int64_t)c[dim] + off);
+    return ls->value(p);
+  }
+};
+
+// EO / LF / LLF / LLLF first/second order one-sided differences
+// (lsEngquistOsher.hpp:59-111, lsLaxFriedrichs.hpp:66-119,
+//  lsLocalLaxFriedrichs.hpp:85-143, lsLocalLocalLaxFriedrichs.hpp:62-115)
+template <int D>
+static void upwindDiffs(const SchemeCtx<D> &s, const Idx<D> &c, int i, int order, double phi0, double &diffPos,
+                        double &diffNeg) {
+  const double gridDelta = s.dx;
+  const double deltaPos = gridDelta;
+  const double deltaNeg = -gridDelta;
+  const double phiPos = s.f(c, i, 1);
+  const double phiNeg = s.f(c, i, -1);
+  diffPos = (phiPos - phi0) / deltaPos;
+  diffNeg = (phiNeg - phi0) / deltaNeg;
+  if (order == 2) {
+    const double deltaPosPos = 2 * gridDelta;
+    const double deltaNegNeg = -2 * gridDelta;
+    const double phiPosPos = s.f(c, i, 2);
+    const double phiNegNeg = s.f(c, i, -2);
+    const double diff00 = (((deltaNeg * phiPos - deltaPos * phiNeg) / (deltaPos - deltaNeg) + phi0)) / (deltaPos * deltaNeg);
+    const double diffNegNeg =
+        (((deltaNeg * phiNegNeg - deltaNegNeg * phiNeg) / (deltaNegNeg - deltaNeg) + phi0)) / (deltaNegNeg * deltaNeg);
+    const double diffPosPos =
+        (((deltaPos * phiPosPos - deltaPosPos * phiPos) / (deltaPosPos - deltaPos) + phi0)) / (deltaPosPos * deltaPos);
+    if (std::signbit(diff00) == std::signbit(diffPosPos)) {
+      if (std::fabs(diffPosPos * deltaPos) < std::fabs(diff00 * deltaNeg)) diffPos -= deltaPos * diffPosPos;
+      else diffPos += deltaNeg * diff00;
+    }
+    if (std::signbit(diff00) == std::signbit(diffNegNeg)) {
+      if (std::fabs(diffNegNeg * deltaNeg) < std::fabs(diff00 * deltaPos)) diffNeg -= deltaNeg * diffNegNeg;
+      else diffNeg += deltaPos * diff00;
+    }
+  }
+}
+
+// EngquistOsher::operator() (lsEngquistOsher.hpp:42-162)
+template <int D> static void schemeEO(const SchemeCtx<D> &s, const Idx<D> &c, int order, double &H, double &Dn) {
+  double gradPos[D], gradNeg[D];
+  double gradPosTotal = 0, gradNegTotal = 0;
+  const double phi0 = s.ls->value(c);
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs<D>(s, c, i, order, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    gradPosTotal += pow2(smax(diffNeg, 0.0)) + pow2(smin(diffPos, 0.0));
+    gradNegTotal += pow2(smin(diffNeg, 0.0)) + pow2(smax(diffPos, 0.0));
+  }
+  double vel_grad = 0.;
+  double n[3] = {0, 0, 0};
+  if (s.calcNormals) {  // :124-138 (no zero guard)
+    double denominator = 0;
+    for (int i = 0; i < D; i++) {
+      double pos = s.f(c, i, 1) - phi0;
+      double neg = phi0 - s.f(c, i, -1);
+      n[i] = (pos + neg) * 0.5;
+      denominator += n[i] * n[i];
+    }
+    denominator = 1. / std::sqrt(denominator);
+    for (int i = 0; i < D; ++i) n[i] *= denominator;
+  }
+  double sv = s.vel->scalar(n);
+  const double *vv = s.vel->vec;
+  if (sv > 0) vel_grad += std::sqrt(gradPosTotal) * sv;
+  else vel_grad += std::sqrt(gradNegTotal) * sv;
+  for (int w = 0; w < D; w++) {
+    if (vv[w] > 0.) vel_grad += vv[w] * gradPos[w];
+    else vel_grad += vv[w] * gradNeg[w];
+  }
+  H = vel_grad;
+  Dn = 0.;
+}
+
+// WENO::operator() (lsWENO.hpp:53-161)
+template <int D> static void schemeWENO(const SchemeCtx<D> &s, const Idx<D> &c, int order, double &H, double &Dn) {
+  const int r = (order + 1) / 2;
+  double gradPosTotal = 0, gradNegTotal = 0;
+  double gm[D], gp[D];
+  double st[7];
+  const double phi0 = s.ls->value(c);
+  for (int i = 0; i < D; i++) {
+    st[r] = phi0;
+    for (int k = 1; k <= r; ++k) {
+      st[r + k] = s.f(c, i, k);
+      st[r - k] = s.f(c, i, -k);
+    }
+    if (order == 3) {
+      gm[i] = weno3(st, s.dx, false);
+      gp[i] = weno3(st, s.dx, true);
+    } else {
+      gm[i] = weno5(st, s.dx, false);
+      gp[i] = weno5(st, s.dx, true);
+    }
+    gradPosTotal += pow2(smax(gm[i], 0.0)) + pow2(smin(gp[i], 0.0));
+    gradNegTotal += pow2(smin(gm[i], 0.0)) + pow2(smax(gp[i], 0.0));
+  }
+  double vel_grad = 0.;
+  double n[3] = {0, 0, 0};
+  if (s.calcNormals) {  // :113-129 (with zero guard)
+    double denominator = 0;
+    for (int i = 0; i < D; i++) {
+      double pos = s.f(c, i, 1);
+      double neg = s.f(c, i, -1);
+      n[i] = pos - neg;
+      denominator += n[i] * n[i];
+    }
+    if (denominator > 0) {
+      denominator = 1. / std::sqrt(denominator);
+      for (int i = 0; i < D; ++i) n[i] *= denominator;
+    }
+  }
+  double sv = s.vel->scalar(n);
+  const double *vv = s.vel->vec;
+  if (sv > 0) vel_grad += std::sqrt(gradPosTotal) * sv;
+  else vel_grad += std::sqrt(gradNegTotal) * sv;
+  for (int w = 0; w < D; w++) {
+    if (vv[w] > 0.) vel_grad += vv[w] * gm[w];
+    else vel_grad += vv[w] * gp[w];
+  }
+  H = vel_grad;
+  Dn = 0.;
+}
+
+// LaxFriedrichs::operator() (lsLaxFriedrichs.hpp:46-158)
+template <int D> static void schemeLF(const SchemeCtx<D> &s, const Idx<D> &c, int order, double &H, double &Dn) {
+  double gradPos[D], gradNeg[D];
+  double grad = 0., dissipation = 0.;
+  double n[3] = {0, 0, 0};
+  double normalModulus = 0;
+  const double phi0 = s.ls->value(c);
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs<D>(s, c, i, order, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    if (s.calcNormals) {
+      n[i] = (diffNeg + diffPos) * 0.5;
+      normalModulus += n[i] * n[i];
+    }
+    grad += pow2((diffNeg + diffPos) * 0.5);
+    dissipation += s.alphaFactor * s.globalAlphas[i] * (diffPos - diffNeg) * 0.5;
+  }
+  if (s.calcNormals) {
+    normalModulus = 1. / std::sqrt(normalModulus);
+    for (int i = 0; i < D; ++i) n[i] *= normalModulus;
+  }
+  double sv = s.vel->scalar(n);
+  const double *vv = s.vel->vec;
+  double totalGrad = 0.;
+  if (sv != 0.) totalGrad = sv * std::sqrt(grad);
+  for (int w = 0; w < D; w++) {
+    if (vv[w] > 0.) totalGrad += vv[w] * gradPos[w];
+    else totalGrad += vv[w] * gradNeg[w];
+  }
+  H = totalGrad;
+  Dn = (totalGrad != 0.) ? dissipation : 0;
+}
+
+// LocalLocalLaxFriedrichs::operator() (lsLocalLocalLaxFriedrichs.hpp:42-165)
+template <int D>
+static void schemeLLLF(const SchemeCtx<D> &s, const Idx<D> &c, int order, double &H, double &Dn, double *alphaAcc) {
+  double gradPos[D], gradNeg[D];
+  double grad = 0., dissipation = 0.;
+  double n[3] = {0, 0, 0};
+  double normalModulus = 0;
+  const double phi0 = s.ls->value(c);
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs<D>(s, c, i, order, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    n[i] = (diffNeg + diffPos) * 0.5;
+    normalModulus += n[i] * n[i];
+    grad += pow2((diffNeg + diffPos) * 0.5);
+  }
+  normalModulus = 1. / std::sqrt(normalModulus);
+  for (int i = 0; i < D; ++i) n[i] *= normalModulus;
+  double sv = s.vel->scalar(n);
+  const double *vv = s.vel->vec;
+  double totalGrad = 0.;
+  if (sv != 0.) totalGrad = sv * std::sqrt(grad);
+  for (int w = 0; w < D; w++) {
+    if (vv[w] > 0.) totalGrad += vv[w] * gradPos[w];
+    else totalGrad += vv[w] * gradNeg[w];
+  }
+  if (totalGrad == 0.) { H = 0; Dn = 0; return; }
+  for (int i = 0; i < D; ++i) {
+    double alpha = std::fabs((sv + vv[i]) * n[i]);
+    alphaAcc[i] = smax(alphaAcc[i], alpha);
+    dissipation += s.alphaFactor * alpha * (gradNeg[i] - gradPos[i]) * 0.5;
+  }
+  H = totalGrad;
+  Dn = dissipation;
+}
+
+// LocalLaxFriedrichs::operator() (lsLocalLaxFriedrichs.hpp:65-241)
+template <int D>
+static void schemeLLF(const SchemeCtx<D> &s, const Idx<D> &c, int order, double &H, double &Dn, double *alphaAcc) {
+  double gradPos[D], gradNeg[D];
+  double grad = 0., dissipation = 0.;
+  double n[3] = {0, 0, 0};
+  double normalModulus = 0;
+  const double phi0 = s.ls->value(c);
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs<D>(s, c, i, order, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    n[i] = (diffNeg + diffPos) * 0.5;
+    normalModulus += n[i] * n[i];
+    grad += pow2((diffNeg + diffPos) * 0.5);
+  }
+  normalModulus = 1. / std::sqrt(normalModulus);
+  for (int i = 0; i < D; ++i) n[i] *= normalModulus;
+  double sv = s.vel->scalar(n);
+  const double *vv = s.vel->vec;
+  double totalGrad = 0.;
+  if (sv != 0.) totalGrad = sv * std::sqrt(grad);
+  for (int w = 0; w < D; w++) {
+    if (vv[w] > 0.) totalGrad += vv[w] * gradPos[w];
+    else totalGrad += vv[w] * gradNeg[w];
+  }
+  if (totalGrad == 0.) { H = 0; Dn = 0; return; }
+  double alpha[D];
+  for (int i = 0; i < D; ++i) alpha[i] = 0;
+  int ni[D];
+  for (int d = 0; d < D; ++d) ni[d] = -1;
+  int numNeighbors = 1;
+  for (int d = 0; d < D; ++d) numNeighbors *= 3;
+  for (int q = 0; q < numNeighbors; ++q) {  // :193-231
+    Idx<D> nc;
+    for (int d = 0; d < D; ++d) nc[d] = s.ls->g.map(d, (int64_t)c[d] + ni[d]);
+    // neighbour values around the box point (box iterator offsets, mapped
+    // from the centre so that boundary handling matches the star reads)
+    auto fo = [&](int dir, int off) {
+      Idx<D> p;
+      for (int d = 0; d < D; ++d) p[d] = s.ls->g.map(d, (int64_t)c[d] + ni[d] + (d == dir ? off : 0));
+      return s.ls->value(p);
+    };
+    double normal[3] = {0, 0, 0};
+    double nm = 0.;
+    double center = s.ls->value(nc);
+    for (int dir = 0; dir < D; ++dir) {
+      double neg = fo(dir, -1);
+      double pos = fo(dir, 1);
+      double dP = (pos - center) / s.dx;
+      double dN = (center - neg) / s.dx;
+      normal[dir] = (dP + dN) * 0.5;
+      nm += normal[dir] * normal[dir];
+    }
+    nm = 1. / std::sqrt(nm);
+    for (int dir = 0; dir < D; ++dir) normal[dir] *= nm;
+    double scaVel = s.vel->scalar(normal);
+    const double *vecVel = s.vel->vec;
+    for (int dir = 0; dir < D; ++dir) {
+      double tempAlpha = std::fabs((scaVel + vecVel[dir]) * normal[dir]);
+      alpha[dir] = smax(alpha[dir], tempAlpha);
+      alphaAcc[dir] = smax(alphaAcc[dir], tempAlpha);
+    }
+    int dim = 0;  // incrementIndices (:38-48)
+    for (; dim < D - 1; ++dim) {
+      if (ni[dim] < 1) break;
+      ni[dim] = -1;
+    }
+    ++ni[dim];
+  }
+  for (int i = 0; i < D; ++i) dissipation += s.alphaFactor * alpha[i] * (gradNeg[i] - gradPos[i]) * 0.5;
+  H = totalGrad;
+  Dn = dissipation;
+}
+
+// scheme stencil width for prepareLS (lsAdvect.hpp:973-1015 and the
+// scheme prepareLS functions)
+static int prepareWidth(int spatial) {
+  switch (spatial) {
+  case EO1: return 3;   // lsEngquistOsher.hpp:31 (2*order+1)
+  case EO2: return 5;
+  case LF1: return 3;   // lsLaxFriedrichs.hpp:35
+  case LF2: return 5;
+  case LLLF1: return 3; // lsLocalLocalLaxFriedrichs.hpp:32
+  case LLLF2: return 5;
+  case LLF1: return 7;  // lsLocalLaxFriedrichs.hpp:55 (2*(order+2)+1)
+  case LLF2: return 9;
+  case WENO3: return 5; // lsWENO.hpp:44 (2*r+1)
+  case WENO5: return 7;
+  default: return -1;
+  }
+}
+static int schemeOrder(int spatial) {
+  switch (spatial) {
+  case EO2: case LF2: case LLLF2: case LLF2: return 2;
+  case WENO3: return 3;
+  case WENO5: return 5;
+  default: return 1;
+  }
+}
+
+// --------------------------------------------------------------- advect ---
+
+template <int D> struct Advector {
+  AdvectParams P;
+  std::vector<double> rates;  // per entry of the expanded band: final velocity (after checkDissipation)
+  double currentTimeStep = -1;
+  double pointUpdates = 0;
+  std::string err;
+
+  // Advect::findGlobalAlphas (lsAdvect.hpp:90-195), single level set
+  void globalAlphas(const Band<D> &ls, double *alphas) {
+    for (int i = 0; i < 3; ++i) alphas[i] = 0;
+    const size_t n = ls.idx.size();
+    double acc[3] = {0, 0, 0};
+#pragma omp parallel
+    {
+      double loc[3] = {0, 0, 0};
+#pragma omp for schedule(static)
+      for (size_t e = 0; e < n; ++e) {
+        double v = ls.val[e];
+        if (isUndef(v) || std::fabs(v) > 0.5) continue;
+        const Idx<D> &c = ls.idx[e];
+        double normal[3] = {0, 0, 0};
+        double nm = 0.;
+        for (int i = 0; i < D; ++i) {
+          Idx<D> pp = c, pn = c;
+          pp[i] = ls.g.map(i, (int64_t)c[i] + 1);
+          pn[i] = ls.g.map(i, (int64_t)c[i] - 1);
+          normal[i] = ls.value(pp) - ls.value(pn);
+          nm += normal[i] * normal[i];
+        }
+        nm = 1. / std::sqrt(nm);
+        for (int i = 0; i < D; ++i) normal[i] *= nm;
+        double sv = P.vel.scalar(normal);
+        for (int i = 0; i < D; ++i) {
+          double t = std::fabs((sv + P.vel.vec[i]) * normal[i]);
+          loc[i] = smax(loc[i], t);
+        }
+      }
+#pragma omp critical
+      for (int i = 0; i < D; ++i) acc[i] = smax(acc[i], loc[i]);
+    }
+    for (int i = 0; i < D; ++i) alphas[i] = acc[i];
+  }
+
+  // Advect::computeRates + integrateTime (lsAdvect.hpp:435-682), single LS.
+  // Returns the time step; fills rates for every entry (0 for inactive).
+  double computeRates(Band<D> &ls, double maxTimeStep) {
+    int w = prepareWidth(P.spatial);
+    if (w < 0) { err = "Advect: Discretization scheme not found."; return -1.; }
+    expand(ls, w);
+    SchemeCtx<D> s;
+    s.ls = &ls;
+    s.spatial = P.spatial;
+    s.dx = ls.g.delta;
+    s.calcNormals = P.calcNormals != 0;
+    s.alphaFactor = P.dissipationAlpha;
+    s.vel = &P.vel;
+    s.globalAlphas[0] = s.globalAlphas[1] = s.globalAlphas[2] = 0;
+    if (P.spatial == LF1 || P.spatial == LF2) globalAlphas(ls, s.globalAlphas);
+    // WENO receives dissipationAlpha as its calcNormal flag (lsAdvect.hpp:670-676)
+    if (P.spatial == WENO3 || P.spatial == WENO5) s.calcNormals = (P.dissipationAlpha != 0.0);
+    const int order = schemeOrder(P.spatial);
+    const size_t n = ls.idx.size();
+    rates.assign(n, 0.0);
+    double tempMax = maxTimeStep;
+    double alphaAcc[3] = {0, 0, 0};
+    double active = 0;
+#pragma omp parallel
+    {
+      double locMax = maxTimeStep;
+      double locAlpha[3] = {0, 0, 0};
+      double locActive = 0;
+#pragma omp for schedule(dynamic, 256)
+      for (size_t e = 0; e < n; ++e) {
+        double value = ls.val[e];
+        if (isUndef(value) || std::fabs(value) > 0.5) continue;  // :498-499
+        locActive += 1;
+        double H = 0, Dn = 0;
+        const Idx<D> &c = ls.idx[e];
+        switch (P.spatial) {
+        case EO1: case EO2: schemeEO<D>(s, c, order, H, Dn); break;
+        case LF1: case LF2: schemeLF<D>(s, c, order, H, Dn); break;
+        case LLLF1: case LLLF2: schemeLLLF<D>(s, c, order, H, Dn, locAlpha); break;
+        case LLF1: case LLF2: schemeLLF<D>(s, c, order, H, Dn, locAlpha); break;
+        case WENO3: case WENO5: schemeWENO<D>(s, c, order, H, Dn); break;
+        }
+        // :530-588, single level set (no material below)
+        double velocity = H - Dn;
+        double maxStepTime = 0;
+        const double cfl = P.cfl;
+        if (velocity > 0.) maxStepTime += cfl / velocity;
+        else if (velocity == 0.) maxStepTime = std::numeric_limits<double>::max();
+        else maxStepTime -= cfl / velocity;
+        if (maxStepTime < locMax) locMax = maxStepTime;  // :591-592
+        // updateLevelSet velocity rule (lsAdvect.hpp:738-744), precedence kept
+        double v = velocity;
+        if ((P.checkDissipation && (H < 0 && v > 0)) || (H > 0 && v < 0)) v = 0;
+        rates[e] = v;
+      }
+#pragma omp critical
+      {
+        if (locMax < tempMax) tempMax = locMax;
+        for (int i = 0; i < 3; ++i) alphaAcc[i] = smax(alphaAcc[i], locAlpha[i]);
+        active += locActive;
+      }
+    }
+    pointUpdates += active;
+    // reduceTimeStepHamiltonJacobi (LF: lsLaxFriedrichs.hpp:160-172; LLF:
+    // lsLocalLaxFriedrichs.hpp:243-254; LLLF: lsLocalLocalLaxFriedrichs.hpp:167-178)
+    const double *fa = nullptr;
+    if (P.spatial == LF1 || P.spatial == LF2) fa = s.globalAlphas;
+    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2) fa = alphaAcc;
+    if (fa) {
+      double timeStep = 0;
+      for (int i = 0; i < D; ++i) timeStep += fa[i] / ls.g.delta;
+      timeStep = 1.0 / timeStep;
+      tempMax = smin(timeStep, tempMax);
+    }
+    if (tempMax < maxTimeStep) maxTimeStep = tempMax;  // :604-605
+    return maxTimeStep;
+  }
+
+  // Advect::updateLevelSet (lsAdvect.hpp:686-800), single LS:
+  // Reduce(1) then value -= dt*velocity for the kept points.
+  void updateLevelSet(Band<D> &ls, double dt) {
+    const double cut = 0.5;
+    Band<D> out;
+    std::vector<double> vals(ls.idx.size());
+#pragma omp parallel for schedule(static)
+    for (size_t e = 0; e < ls.idx.size(); ++e) {
+      double c = ls.val[e];
+      if (!isUndef(c) && std::fabs(c) <= cut) {
+        double rate = dt * rates[e];
+        vals[e] = c - rate;
+      } else {
+        vals[e] = (c < 0) ? NEG : POS;
+      }
+    }
+    out.g = ls.g;
+    compressInto<D>(out, ls.idx, vals);
+    out.width = 1;
+    ls = std::move(out);
+  }
+
+  // AdvectTimeIntegration::evolveForwardEuler (lsAdvectIntegrationSchemes.hpp:46-59)
+  double evolveFE(Band<D> &ls, double maxTimeStep) {
+    currentTimeStep = computeRates(ls, maxTimeStep);
+    if (currentTimeStep < 0) return currentTimeStep;
+    updateLevelSet(ls, currentTimeStep);
+    rebuild(ls);
+    return currentTimeStep;
+  }
+
+  // Advect::combineLevelSets (lsAdvect.hpp:199-236)
+  void combine(Band<D> &ls, Band<D> &init, double wTarget, double wSource) {
+    int steps = 1;
+    if (P.temporal == RK2) steps = 2;
+    else if (P.temporal == RK3) steps = 3;
+    int ew = (int)std::ceil(2.0 * steps * P.cfl + 1);
+    expand(ls, ew);
+    expand(init, ew);
+    booleanOp<D>(ls, init, [wTarget, wSource](double a, double b) -> double {
+      if (!isUndef(a) && !isUndef(b)) return wSource * a + wTarget * b;
+      if (isUndef(a)) return a;
+      return b;
+    });
+    rebuild(ls);
+  }
+
+  // lsAdvectIntegrationSchemes.hpp:61-108
+  double evolveRK2(Band<D> &ls, double maxTimeStep) {
+    Band<D> init = ls;
+    double dt1 = evolveFE(ls, maxTimeStep);
+    if (dt1 <= 0.) return (dt1 < 0) ? dt1 : 0.;
+    double dt2 = evolveFE(ls, dt1);
+    combine(ls, init, 0.5, 0.5);
+    return 0.5 * dt1 + 0.5 * dt2;
+  }
+
+  // lsAdvectIntegrationSchemes.hpp:110-168
+  double evolveRK3(Band<D> &ls, double maxTimeStep) {
+    Band<D> init = ls;
+    double dt1 = evolveFE(ls, maxTimeStep);
+    if (dt1 <= 0.) return (dt1 < 0) ? dt1 : 0.;
+    double dt2 = evolveFE(ls, dt1);
+    combine(ls, init, 0.75, 0.25);
+    double dt3 = evolveFE(ls, dt1);
+    combine(ls, init, 1.0 / 3.0, 2.0 / 3.0);
+    return (dt1 + dt2 + 4.0 * dt3) / 6.0;
+  }
+
+  // Advect::advect (lsAdvect.hpp:817-830)
+  double advect(Band<D> &ls, double maxTimeStep) {
+    switch (P.temporal) {
+    case RK2: return evolveRK2(ls, maxTimeStep);
+    case RK3: return evolveRK3(ls, maxTimeStep);
+    default: return evolveFE(ls, maxTimeStep);
+    }
+  }
+
+  // Advect::apply (lsAdvect.hpp:1017-1043)
+  AdvectResult apply(Band<D> &ls) {
+    AdvectResult r;
+    pointUpdates = 0;
+    if (P.advectionTime == 0.) {
+      r.advectedTime = advect(ls, std::numeric_limits<double>::max());
+      r.numSteps = 1;
+    } else {
+      double t = 0.0;
+      r.numSteps = 0;
+      while (t < P.advectionTime) {
+        double dt = advect(ls, P.advectionTime - t);
+        if (!(dt > 0.)) { r.status = dt < 0 ? -1 : -2; break; }  // guard: no progress
+        t += dt;
+        ++r.numSteps;
+        if (P.singleStep) break;
+      }
+      r.advectedTime = t;
+    }
+    if (!err.empty()) r.status = -1;
+    r.currentTimeStep = currentTimeStep;
+    r.pointUpdates = pointUpdates;
+    return r;
+  }
+};
+
+}  // namespace lso
+
+// ================================================================ C ABI ===
+
+using namespace lso;
+
+struct lso_domain {
+  int D;
+  Band<2> b2;
+  Band<3> b3;
+};
+
+template <int D> static void getBand(const Band<D> &b, int32_t *idx, double *val) {
+  for (size_t i = 0; i < b.idx.size(); ++i) {
+    for (int k = 0; k < D; ++k) idx[i * D + k] = b.idx[i][k];
+    val[i] = b.val[i];
+  }
+}
+
+template <int D> static void setBand(Band<D> &b, int64_t n, const int32_t *idx, const double *val, int width) {
+  b.idx.resize(n);
+  b.val.resize(n);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < D; ++k) b.idx[i][k] = idx[i * D + k];
+    b.val[i] = val[i];
+  }
+  b.width = width;
+  b.buildRows();
+}
+
+template <int D> static void fromPts(Band<D> &b, int64_t n, const int32_t *idx, const double *val, int width) {
+  std::vector<std::pair<Idx<D>, double>> pts(n);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < D; ++k) pts[i].first[k] = idx[i * D + k];
+    pts[i].second = val[i];
+  }
+  fromPoints<D>(b, pts, width);
+}
+
+extern "C" {
+
+lso_domain *lso_domain_new(int D, const int32_t *gmin, const int32_t *gmax, const int *bc, double delta) {
+  if (D != 2 && D != 3) return nullptr;
+  auto *d = new lso_domain;
+  d->D = D;
+  if (D == 2) {
+    d->b2.g.init(gmin, gmax, bc, delta);
+    d->b2.idx = {d->b2.g.minPoint()};
+    d->b2.val = {POS};
+    d->b2.buildRows();
+  } else {
+    d->b3.g.init(gmin, gmax, bc, delta);
+    d->b3.idx = {d->b3.g.minPoint()};
+    d->b3.val = {POS};
+    d->b3.buildRows();
+  }
+  return d;
+}
+
+void lso_domain_free(lso_domain *d) { delete d; }
+
+lso_domain *lso_domain_copy(const lso_domain *s) { return new lso_domain(*s); }
+
+int64_t lso_domain_size(const lso_domain *d) { return d->D == 2 ? (int64_t)d->b2.idx.size() : (int64_t)d->b3.idx.size(); }
+int lso_domain_width(const lso_domain *d) { return d->D == 2 ? d->b2.width : d->b3.width; }
+
+void lso_domain_get(const lso_domain *d, int32_t *idx, double *val) {
+  if (d->D == 2) getBand(d->b2, idx, val);
+  else getBand(d->b3, idx, val);
+}
+
+// Set the raw canonical run-start list (as downloaded from either
+// implementation).  Entry 0 must be the grid minimum point.
+void lso_domain_set(lso_domain *d, int64_t n, const int32_t *idx, const double *val, int width) {
+  if (d->D == 2) setBand(d->b2, n, idx, val, width);
+  else setBand(d->b3, n, idx, val, width);
+}
+
+// Build from defined points (index, value) like Domain::insertPoints.
+void lso_domain_from_points(lso_domain *d, int64_t n, const int32_t *idx, const double *val, int width) {
+  if (d->D == 2) fromPts(d->b2, n, idx, val, width);
+  else fromPts(d->b3, n, idx, val, width);
+}
+
+void lso_make_sphere(lso_domain *d, const double *origin, double radius) {
+  if (d->D == 2) makeSphere<2>(d->b2, origin, radius);
+  else makeSphere<3>(d->b3, origin, radius);
+}
+
+void lso_expand(lso_domain *d, int width) {
+  if (d->D == 2) expand<2>(d->b2, width);
+  else expand<3>(d->b3, width);
+}
+void lso_reduce(lso_domain *d, int width) {
+  if (d->D == 2) reduce<2>(d->b2, width);
+  else reduce<3>(d->b3, width);
+}
+void lso_prune(lso_domain *d, int removeStrayZeros) {
+  if (d->D == 2) prune<2>(d->b2, removeStrayZeros != 0);
+  else prune<3>(d->b3, removeStrayZeros != 0);
+}
+void lso_rebuild(lso_domain *d) {
+  if (d->D == 2) rebuild<2>(d->b2);
+  else rebuild<3>(d->b3);
+}
+// op: 0 INTERSECT, 1 UNION, 2 RELATIVE_COMPLEMENT (lsBooleanOperation.hpp:27-33)
+int lso_boolean(lso_domain *a, const lso_domain *b, int op) {
+  auto run = [&](auto &A, const auto &B) {
+    switch (op) {
+    case 0: booleanOp(A, B, [](double x, double y) { return maxComp(x, y); }); break;
+    case 1: booleanOp(A, B, [](double x, double y) { return minComp(x, y); }); break;
+    case 2: booleanOp(A, B, [](double x, double y) { return maxComp(x, -y); }); break;
+    }
+  };
+  if (op < 0 || op > 2 || a->D != b->D) return -1;
+  if (a->D == 2) run(a->b2, b->b2);
+  else run(a->b3, b->b3);
+  return 0;
+}
+// CUSTOM comparator used by Advect::combineLevelSets (lsAdvect.hpp:218-230)
+void lso_combine(lso_domain *a, const lso_domain *b, double wTarget, double wSource) {
+  auto comp = [wTarget, wSource](double x, double y) -> double {
+    if (!isUndef(x) && !isUndef(y)) return wSource * x + wTarget * y;
+    if (isUndef(x)) return x;
+    return y;
+  };
+  if (a->D == 2) booleanOp(a->b2, b->b2, comp);
+  else booleanOp(a->b3, b->b3, comp);
+}
+
+int lso_check(const lso_domain *d, char *msg, int msglen) {
+  std::string m;
+  int f = d->D == 2 ? check<2>(d->b2, &m) : check<3>(d->b3, &m);
+  if (msg && msglen > 0) {
+    std::strncpy(msg, m.c_str(), msglen - 1);
+    msg[msglen - 1] = 0;
+  }
+  return f;
+}
+
+double lso_weno5(const double *x, double dx, int plus) { return weno5(x, dx, plus != 0); }
+double lso_weno3(const double *x, double dx, int plus) { return weno3(x, dx, plus != 0); }
+
+// Flat parameter block shared with the test harness (ctypes).
+struct lso_params {
+  int spatial, temporal;
+  double cfl, advectionTime;
+  int singleStep;
+  double dissipationAlpha;
+  int calcNormals, checkDissipation;
+  int velKind, velDir, velK;
+  double velB, velAPos, velANeg;
+  double vec[3];
+};
+
+static AdvectParams toParams(const lso_params *p) {
+  AdvectParams P;
+  P.spatial = p->spatial;
+  P.temporal = p->temporal;
+  P.cfl = p->cfl;
+  P.advectionTime = p->advectionTime;
+  P.singleStep = p->singleStep;
+  P.dissipationAlpha = p->dissipationAlpha;
+  P.calcNormals = p->calcNormals;
+  P.checkDissipation = p->checkDissipation;
+  P.vel.kind = p->velKind;
+  P.vel.dir = p->velDir;
+  P.vel.k = p->velK;
+  P.vel.b = p->velB;
+  P.vel.aPos = p->velAPos;
+  P.vel.aNeg = p->velANeg;
+  for (int i = 0; i < 3; ++i) P.vel.vec[i] = p->vec[i];
+  return P;
+}
+
+// Advect::apply; out[0]=advectedTime, out[1]=currentTimeStep,
+// out[2]=numSteps, out[3]=point updates.  Returns status.
+int lso_advect(lso_domain *d, const lso_params *p, double *out) {
+  AdvectResult r;
+  if (d->D == 2) {
+    Advector<2> a;
+    a.P = toParams(p);
+    r = a.apply(d->b2);
+  } else {
+    Advector<3> a;
+    a.P = toParams(p);
+    r = a.apply(d->b3);
+  }
+  if (out) {
+    out[0] = r.advectedTime;
+    out[1] = r.currentTimeStep;
+    out[2] = r.numSteps;
+    out[3] = r.pointUpdates;
+  }
+  return r.status;
+}
+
+// Per-point rates of one computeRates pass (prepareLS + integrateTime) on
+// the band; returns dt.  rates has one entry per entry of the expanded band.
+double lso_compute_rates(lso_domain *d, const lso_params *p, double maxTimeStep, double *rates, int64_t cap) {
+  double dt;
+  if (d->D == 2) {
+    Advector<2> a;
+    a.P = toParams(p);
+    dt = a.computeRates(d->b2, maxTimeStep);
+    for (int64_t i = 0; i < (int64_t)a.rates.size() && i < cap; ++i) rates[i] = a.rates[i];
+  } else {
+    Advector<3> a;
+    a.P = toParams(p);
+    dt = a.computeRates(d->b3, maxTimeStep);
+    for (int64_t i = 0; i < (int64_t)a.rates.size() && i < cap; ++i) rates[i] = a.rates[i];
+  }
+  return dt;
+}
+
+int lso_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void lso_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+// CPU oracle for the ViennaLS level-set advection path.
+//
+// TEST INFRASTRUCTURE ONLY.  This is a from-scratch CPU restatement of the
+// reference algorithm (ViennaLS v5.8.3, include/viennals/lsAdvect.hpp and the
+// files it calls) used as the parity checker for the HIP implementation.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it; the product library (viennals_amd) never links or calls it.
+//
+// Level-set model.  The reference stores a band in ViennaHRLE run-length form
+// (external dependency ViennaHRLE v1.1.2, absent from the reference tree).
+// Every band operation on the advection path (Expand, Reduce, Prune,
+// BooleanOperation, Advect::rebuildLS, the Hamiltonians) reads only the value
+// field f(p) at a point and its star neighbours, where f(p) is the defined
+// value or +-DBL_MAX (lsDomain.hpp:51-52) for an undefined run.  The oracle
+// therefore stores a band as its canonical run-start list in HRLE iteration
+// order (x fastest, lsMakeGeometry.hpp:198-225,
+// tests/BooleanOperationExactZero/result.hpp): every defined point, plus
+// every undefined point whose value differs from the previous grid point's
+// value or whose predecessor is defined.  The first entry is always the grid
+// minimum point.  f(p) for a point not in the list is the value of the last
+// entry before it.  An operation evaluates the pointwise rule at every point
+// whose star neighbourhood contains a run start (plus lexicographic
+// successors), then compresses.
+//
+// Parity status (see the function comments for each citation):
+//   pinned by closed forms / invariants / fixture: weno3/weno5, EO/LF/LLF/LLLF
+//   and WENO Hamiltonians on planes, Expand/Reduce/Prune/Boolean/rebuild,
+//   run order and sentinel (BooleanOperationExactZero fixture).
+//   parity unpinned: reflective/periodic mirroring convention of HRLE's Grid
+//   (HRLE absent), FillDomainWithSignedDistance undefined-run signs.
+#pragma once
+
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace lso {
+
+constexpr double POS = DBL_MAX;   // Domain::POS_VALUE  (lsDomain.hpp:51)
+constexpr double NEG = -DBL_MAX;  // Domain::NEG_VALUE  (lsDomain.hpp:52)
+constexpr int32_t INF_IDX = 715827882;  // HRLE infinite-grid extent, INT32_MAX/3
+                                        // (tests/BooleanOperationExactZero/result.hpp:4)
+
+inline bool isUndef(double v) { return v == POS || v == NEG; }
+
+// viennahrle::BoundaryType numbering (python/viennals/_core.pyi:26-30)
+enum BC : int { REFLECTIVE = 0, INFINITE = 1, PERIODIC = 2, POS_INFINITE = 3, NEG_INFINITE = 4 };
+
+// lsAdvectIntegrationSchemes.hpp:10-23
+enum Spatial : int {
+  EO1 = 0, EO2 = 1, LF1 = 2, LF2 = 3, LLFA1 = 4, LLLF1 = 5, LLLF2 = 6,
+  LLF1 = 7, LLF2 = 8, SLLF1 = 9, WENO3 = 10, WENO5 = 11
+};
+// lsAdvectIntegrationSchemes.hpp:31-35
+enum Temporal : int { FE = 0, RK2 = 1, RK3 = 2 };
+
+template <int D> using Idx = std::array<int32_t, D>;
+
+template <int D> struct Grid {
+  int32_t lo[D], hi[D];  // inclusive index range of grid points
+  int bc[D];
+  double delta = 1.0;
+
+  // gmin/gmax are the index bounds lsDomain computes from physical bounds
+  // (floor(b0/dx), ceil(b1/dx), lsDomain.hpp:71-74).
+  void init(const int32_t *gmin, const int32_t *gmax, const int *bcs, double dx) {
+    delta = dx;
+    for (int d = 0; d < D; ++d) {
+      bc[d] = bcs[d];
+      switch (bcs[d]) {
+      case INFINITE: lo[d] = -INF_IDX; hi[d] = INF_IDX; break;
+      case PERIODIC: lo[d] = gmin[d]; hi[d] = gmax[d] - 1; break;  // period gmax-gmin
+      case POS_INFINITE: lo[d] = gmin[d]; hi[d] = INF_IDX; break;
+      case NEG_INFINITE: lo[d] = -INF_IDX; hi[d] = gmax[d]; break;
+      default: lo[d] = gmin[d]; hi[d] = gmax[d]; break;  // REFLECTIVE: both ends
+      }
+    }
+  }
+  bool bounded(int d) const { return bc[d] == REFLECTIVE || bc[d] == PERIODIC; }
+  // Map an index that left the grid back into it: periodic wrap, mirror about
+  // the boundary point for reflective ends, clamp for infinite ends.
+  int32_t map(int d, int64_t c) const {
+    if (c >= lo[d] && c <= hi[d]) return (int32_t)c;
+    if (bc[d] == PERIODIC) {
+      int64_t L = (int64_t)hi[d] - lo[d] + 1;
+      int64_t m = (c - lo[d]) % L;
+      if (m < 0) m += L;
+      return (int32_t)(lo[d] + m);
+    }
+    bool reflLo = bc[d] == REFLECTIVE || bc[d] == POS_INFINITE;
+    bool reflHi = bc[d] == REFLECTIVE || bc[d] == NEG_INFINITE;
+    for (int it = 0; it < 8; ++it) {
+      if (c < lo[d]) c = reflLo ? 2 * (int64_t)lo[d] - c : lo[d];
+      else if (c > hi[d]) c = reflHi ? 2 * (int64_t)hi[d] - c : hi[d];
+      else break;
+    }
+    if (c < lo[d]) c = lo[d];
+    if (c > hi[d]) c = hi[d];
+    return (int32_t)c;
+  }
+  Idx<D> minPoint() const { Idx<D> p; for (int d = 0; d < D; ++d) p[d] = lo[d]; return p; }
+};
+
+// Lexicographic order, highest dimension most significant (HRLE order).
+template <int D> inline bool idxLess(const Idx<D> &a, const Idx<D> &b) {
+  for (int d = D - 1; d >= 0; --d) {
+    if (a[d] != b[d]) return a[d] < b[d];
+  }
+  return false;
+}
+template <int D> inline int64_t rowKey(const Idx<D> &p) {
+  if constexpr (D == 2) return p[1];
+  else return ((int64_t)p[2] << 32) + ((int64_t)p[1] + 2147483648LL);
+}
+
+template <int D> struct Band {
+  Grid<D> g;
+  std::vector<Idx<D>> idx;
+  std::vector<double> val;
+  int width = 1;  // Domain::levelSetWidth (lsDomain.hpp:42)
+  // row index: one entry per distinct row (all dims but x), in order
+  std::vector<int64_t> rkeys;
+  std::vector<int64_t> rstart;  // size rows+1
+
+  void buildRows();
+  size_t numDefined() const;
+  // f(p) for p inside the grid (already mapped)
+  double value(const Idx<D> &p) const;
+  double valueMapped(Idx<D> p) const {
+    for (int d = 0; d < D; ++d) p[d] = g.map(d, p[d]);
+    return value(p);
+  }
+};
+
+// Built-in velocity functors (the device-functor set of the HIP library).
+// scalar(n) = b                                           (kind 0)
+//           = b + (n[dir] > 0 ? aPos : aNeg) * |n[dir]|^k (kind 1)
+// vector    = vec (constant)
+struct Velocity {
+  int kind = 0;
+  int dir = 2;
+  int k = 1;
+  double b = 0, aPos = 0, aNeg = 0;
+  double vec[3] = {0, 0, 0};
+  double scalar(const double n[3]) const {
+    if (kind == 0) return b;
+    double c = n[dir];
+    double a = (c > 0) ? aPos : aNeg;
+    double ac = std::fabs(c);
+    double m = 1.0;
+    if (k >= 1) {
+      m = ac;
+      for (int j = 1; j < k; ++j) m = m * ac;
+    }
+    return b + a * m;
+  }
+};
+
+struct AdvectParams {
+  int spatial = EO1;
+  int temporal = FE;
+  double cfl = 0.4999;           // lsAdvect.hpp:66
+  double advectionTime = 0.0;    // lsAdvect.hpp:70
+  int singleStep = 0;
+  double dissipationAlpha = 1.0; // lsAdvect.hpp:67
+  int calcNormals = 1;           // lsAdvect.hpp:68
+  int checkDissipation = 1;      // lsAdvect.hpp:76
+  Velocity vel;
+};
+
+struct AdvectResult {
+  double advectedTime = 0;
+  double currentTimeStep = -1;
+  unsigned numSteps = 0;
+  int status = 0;
+  double pointUpdates = 0;  // sum over stages of active points
+};
+
+}  // namespace lso

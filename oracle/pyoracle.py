@@ -1,0 +1,229 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+never by the viennals_amd package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liblsoracle.so")
+_lib = None
+
+POS = np.finfo(np.float64).max
+NEG = -POS
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("spatial", ctypes.c_int), ("temporal", ctypes.c_int),
+        ("cfl", ctypes.c_double), ("advectionTime", ctypes.c_double),
+        ("singleStep", ctypes.c_int), ("dissipationAlpha", ctypes.c_double),
+        ("calcNormals", ctypes.c_int), ("checkDissipation", ctypes.c_int),
+        ("velKind", ctypes.c_int), ("velDir", ctypes.c_int), ("velK", ctypes.c_int),
+        ("velB", ctypes.c_double), ("velAPos", ctypes.c_double), ("velANeg", ctypes.c_double),
+        ("vec", ctypes.c_double * 3),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.lso_domain_new.restype = P
+        L.lso_domain_new.argtypes = [ctypes.c_int, i32p, i32p, ip, ctypes.c_double]
+        L.lso_domain_free.argtypes = [P]
+        L.lso_domain_copy.restype = P
+        L.lso_domain_copy.argtypes = [P]
+        L.lso_domain_size.restype = ctypes.c_int64
+        L.lso_domain_size.argtypes = [P]
+        L.lso_domain_width.restype = ctypes.c_int
+        L.lso_domain_width.argtypes = [P]
+        L.lso_domain_get.argtypes = [P, i32p, dp]
+        L.lso_domain_set.argtypes = [P, ctypes.c_int64, i32p, dp, ctypes.c_int]
+        L.lso_domain_from_points.argtypes = [P, ctypes.c_int64, i32p, dp, ctypes.c_int]
+        L.lso_make_sphere.argtypes = [P, dp, ctypes.c_double]
+        L.lso_expand.argtypes = [P, ctypes.c_int]
+        L.lso_reduce.argtypes = [P, ctypes.c_int]
+        L.lso_prune.argtypes = [P, ctypes.c_int]
+        L.lso_rebuild.argtypes = [P]
+        L.lso_boolean.argtypes = [P, P, ctypes.c_int]
+        L.lso_boolean.restype = ctypes.c_int
+        L.lso_combine.argtypes = [P, P, ctypes.c_double, ctypes.c_double]
+        L.lso_check.argtypes = [P, ctypes.c_char_p, ctypes.c_int]
+        L.lso_check.restype = ctypes.c_int
+        L.lso_weno5.argtypes = [dp, ctypes.c_double, ctypes.c_int]
+        L.lso_weno5.restype = ctypes.c_double
+        L.lso_weno3.argtypes = [dp, ctypes.c_double, ctypes.c_int]
+        L.lso_weno3.restype = ctypes.c_double
+        L.lso_advect.argtypes = [P, ctypes.POINTER(Params), dp]
+        L.lso_advect.restype = ctypes.c_int
+        L.lso_compute_rates.argtypes = [P, ctypes.POINTER(Params), ctypes.c_double, dp, ctypes.c_int64]
+        L.lso_compute_rates.restype = ctypes.c_double
+        L.lso_num_threads.restype = ctypes.c_int
+        L.lso_set_num_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Domain:
+    """Oracle level set: canonical run-start list (see ls_oracle.hpp)."""
+
+    def __init__(self, D, gmin, gmax, bc, delta, _handle=None):
+        L = lib()
+        self.D = D
+        self.gmin = list(gmin)
+        self.gmax = list(gmax)
+        self.bc = list(bc)
+        self.delta = float(delta)
+        if _handle is not None:
+            self.h = _handle
+        else:
+            a, pa = _i32(gmin)
+            b, pb = _i32(gmax)
+            c = (ctypes.c_int * D)(*bc)
+            self.h = L.lso_domain_new(D, pa, pb, c, float(delta))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lso_domain_free(self.h)
+            self.h = None
+
+    def copy(self):
+        return Domain(self.D, self.gmin, self.gmax, self.bc, self.delta, _handle=lib().lso_domain_copy(self.h))
+
+    @property
+    def width(self):
+        return lib().lso_domain_width(self.h)
+
+    def get(self):
+        n = lib().lso_domain_size(self.h)
+        idx = np.zeros((n, self.D), dtype=np.int32)
+        val = np.zeros(n, dtype=np.float64)
+        lib().lso_domain_get(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                             val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return idx, val
+
+    def set(self, idx, val, width):
+        a, pa = _i32(idx)
+        b, pb = _f64(val)
+        lib().lso_domain_set(self.h, len(b), pa, pb, int(width))
+
+    def from_points(self, idx, val, width=2):
+        a, pa = _i32(idx)
+        b, pb = _f64(val)
+        lib().lso_domain_from_points(self.h, len(b), pa, pb, int(width))
+
+    def make_sphere(self, origin, radius):
+        o, po = _f64(list(origin) + [0.0] * (3 - len(origin)))
+        lib().lso_make_sphere(self.h, po, float(radius))
+
+    def expand(self, w):
+        lib().lso_expand(self.h, int(w))
+
+    def reduce(self, w):
+        lib().lso_reduce(self.h, int(w))
+
+    def prune(self, remove_stray_zeros=False):
+        lib().lso_prune(self.h, int(bool(remove_stray_zeros)))
+
+    def rebuild(self):
+        lib().lso_rebuild(self.h)
+
+    def boolean(self, other, op):
+        return lib().lso_boolean(self.h, other.h, int(op))
+
+    def combine(self, other, w_target, w_source):
+        lib().lso_combine(self.h, other.h, float(w_target), float(w_source))
+
+    def check(self):
+        buf = ctypes.create_string_buffer(4096)
+        n = lib().lso_check(self.h, buf, 4096)
+        return n, buf.value.decode()
+
+    def defined(self):
+        idx, val = self.get()
+        m = np.abs(val) != POS
+        return idx[m], val[m]
+
+
+def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
+                dissipation_alpha=1.0, calc_normals=True, check_dissipation=True,
+                vel=None):
+    """vel: dict(kind, dir, k, b, a_pos, a_neg, vec) of the built-in velocity."""
+    p = Params()
+    p.spatial = spatial
+    p.temporal = temporal
+    p.cfl = cfl
+    p.advectionTime = time
+    p.singleStep = int(single_step)
+    p.dissipationAlpha = dissipation_alpha
+    p.calcNormals = int(calc_normals)
+    p.checkDissipation = int(check_dissipation)
+    vel = vel or {}
+    p.velKind = vel.get("kind", 0)
+    p.velDir = vel.get("dir", 2)
+    p.velK = vel.get("k", 1)
+    p.velB = vel.get("b", 0.0)
+    p.velAPos = vel.get("a_pos", 0.0)
+    p.velANeg = vel.get("a_neg", 0.0)
+    v = vel.get("vec", (0.0, 0.0, 0.0))
+    for i in range(3):
+        p.vec[i] = v[i]
+    return p
+
+
+def advect(dom, params):
+    out = (ctypes.c_double * 4)()
+    st = lib().lso_advect(dom.h, ctypes.byref(params), out)
+    return st, dict(advected_time=out[0], time_step=out[1], steps=int(out[2]), point_updates=out[3])
+
+
+def compute_rates(dom, params, max_dt=np.finfo(np.float64).max):
+    cap = lib().lso_domain_size(dom.h) * 4 + 64
+    buf = np.zeros(cap, dtype=np.float64)
+    dt = lib().lso_compute_rates(dom.h, ctypes.byref(params), float(max_dt),
+                                 buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), cap)
+    n = lib().lso_domain_size(dom.h)
+    return dt, buf[:n]
+
+
+def weno5(x, dx, plus):
+    a, pa = _f64(x)
+    return lib().lso_weno5(pa, float(dx), int(plus))
+
+
+def weno3(x, dx, plus):
+    a, pa = _f64(x)
+    return lib().lso_weno3(pa, float(dx), int(plus))
+
+
+def set_threads(n):
+    lib().lso_set_num_threads(int(n))
+
+
+def num_threads():
+    return lib().lso_num_threads()

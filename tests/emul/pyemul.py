@@ -1,0 +1,96 @@
+"""ctypes binding of the host build of the GPU per-item logic (test only)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "liblsgemul.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        P = ctypes.c_void_p
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.emu_domain_new.restype = P
+        L.emu_domain_new.argtypes = [ctypes.c_int, i32p, i32p, ctypes.POINTER(ctypes.c_int), ctypes.c_double]
+        L.emu_domain_free.argtypes = [P]
+        L.emu_domain_set.argtypes = [P, ctypes.c_int64, i32p, dp, ctypes.c_int]
+        L.emu_domain_size.restype = ctypes.c_int64
+        L.emu_domain_size.argtypes = [P]
+        L.emu_domain_width.argtypes = [P]
+        L.emu_domain_get.argtypes = [P, i32p, dp]
+        for f in ("emu_expand", "emu_reduce", "emu_prune"):
+            getattr(L, f).argtypes = [P, ctypes.c_int]
+        L.emu_rebuild.argtypes = [P]
+        L.emu_boolean.argtypes = [P, P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.emu_advect.argtypes = [P, ctypes.c_void_p, dp]
+        _lib = L
+    return _lib
+
+
+class Domain:
+    def __init__(self, D, gmin, gmax, bc, delta):
+        self.D = D
+        g0 = np.ascontiguousarray(gmin, dtype=np.int32)
+        g1 = np.ascontiguousarray(gmax, dtype=np.int32)
+        c = (ctypes.c_int * D)(*bc)
+        self.h = lib().emu_domain_new(D, g0.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                      g1.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c, float(delta))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().emu_domain_free(self.h)
+            self.h = None
+
+    def set(self, idx, val, width):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        r = lib().emu_domain_set(self.h, len(val), idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                 val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(width))
+        assert r == 0
+
+    def get(self):
+        n = lib().emu_domain_size(self.h)
+        idx = np.zeros((n, self.D), dtype=np.int32)
+        val = np.zeros(n, dtype=np.float64)
+        lib().emu_domain_get(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                             val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return idx, val
+
+    @property
+    def width(self):
+        return lib().emu_domain_width(self.h)
+
+    def expand(self, w):
+        assert lib().emu_expand(self.h, int(w)) == 0
+
+    def reduce(self, w):
+        assert lib().emu_reduce(self.h, int(w)) == 0
+
+    def prune(self, rsz=False):
+        assert lib().emu_prune(self.h, int(bool(rsz))) == 0
+
+    def rebuild(self):
+        assert lib().emu_rebuild(self.h) == 0
+
+    def boolean(self, other, kind, wt=0.0, ws=0.0):
+        assert lib().emu_boolean(self.h, other.h, int(kind), float(wt), float(ws)) == 0
+
+
+def advect(dom, params):
+    """params: the oracle's Params structure (same layout)."""
+    out = (ctypes.c_double * 4)()
+    st = lib().emu_advect(dom.h, ctypes.byref(params), out)
+    return st, dict(advected_time=out[0], steps=int(out[2]), point_updates=out[3])

@@ -1,0 +1,181 @@
+// Device-side level-set band layout for the MI355X advection path.
+//
+// A band is the canonical run-start list of the level-set value field (every
+// defined point plus every undefined point that starts a run of a new sign,
+// in HRLE order: x fastest, highest dimension most significant; see
+// include/viennals/lsDomain.hpp:51-52 for the +-DBL_MAX undefined values).
+// Entries inside a rectangular "frame" are indexed by a bitmap with one bit
+// per grid point (rows padded to a power of two) and a per-word exclusive
+// prefix count, so the rank of any grid point -- the index of the run that
+// contains it -- costs one 64-bit word load, one prefix load and a popcount.
+// Entry 0 is always the grid minimum point; when it lies outside the frame
+// (any infinite dimension) it is kept separately and `base` is 1.
+#pragma once
+
+#ifndef LSG_HOST_ONLY
+#include <hip/hip_runtime.h>
+#define LSG_HD __host__ __device__
+#else
+#define LSG_HD
+#endif
+
+#include <cmath>
+#include <cstdint>
+
+namespace lsg {
+
+#ifdef LSG_HOST_ONLY
+using std::fabs;
+using std::signbit;
+using std::sqrt;
+#endif
+
+constexpr double POS = 1.7976931348623157e308;   // Domain::POS_VALUE
+constexpr double NEG = -1.7976931348623157e308;  // Domain::NEG_VALUE
+constexpr int32_t INF_IDX = 715827882;          // HRLE infinite extent (INT32_MAX/3)
+constexpr double DEPS = 2.220446049250313e-16;   // numeric_limits<double>::epsilon()
+
+enum BC : int { REFLECTIVE = 0, INFINITE = 1, PERIODIC = 2, POS_INFINITE = 3, NEG_INFINITE = 4 };
+
+LSG_HD inline bool isUndef(double v) { return v == POS || v == NEG; }
+
+struct GridInfo {
+  int32_t lo[3], hi[3];
+  int bc[3];
+  int D;
+  double dx;
+
+  LSG_HD bool bounded(int d) const { return bc[d] == REFLECTIVE || bc[d] == PERIODIC; }
+  // periodic wrap / mirror about the boundary point / clamp (same convention
+  // as the CPU restatement)
+  LSG_HD int32_t map(int d, int64_t c) const {
+    if (c >= lo[d] && c <= hi[d]) return (int32_t)c;
+    if (bc[d] == PERIODIC) {
+      int64_t L = (int64_t)hi[d] - lo[d] + 1;
+      int64_t m = (c - lo[d]) % L;
+      if (m < 0) m += L;
+      return (int32_t)(lo[d] + m);
+    }
+    bool reflLo = bc[d] == REFLECTIVE || bc[d] == POS_INFINITE;
+    bool reflHi = bc[d] == REFLECTIVE || bc[d] == NEG_INFINITE;
+    for (int it = 0; it < 8; ++it) {
+      if (c < lo[d]) c = reflLo ? 2 * (int64_t)lo[d] - c : lo[d];
+      else if (c > hi[d]) c = reflHi ? 2 * (int64_t)hi[d] - c : hi[d];
+      else break;
+    }
+    if (c < lo[d]) c = lo[d];
+    if (c > hi[d]) c = hi[d];
+    return (int32_t)c;
+  }
+};
+
+struct Frame {
+  int32_t o[3];    // absolute index of frame-relative 0 per dim
+  int32_t ext[3];  // valid extent per dim (ext[2] == 1 in 2D)
+  int lx;          // log2(bits per row), >= 6
+  int ly;          // log2(rows per z-slab)
+  int base;        // 1 if entry 0 (grid minimum) lies outside the frame
+  int64_t nwords;  // words in the bitmap (one zero pad word follows)
+  LSG_HD int wpr() const { return 1 << (lx - 6); }
+  LSG_HD uint32_t encode(int32_t xr, int32_t yr, int32_t zr) const {
+    return (uint32_t)(((((uint64_t)zr << ly) + (uint64_t)yr) << lx) + (uint64_t)xr);
+  }
+  LSG_HD void decode(uint32_t p, int32_t &xr, int32_t &yr, int32_t &zr) const {
+    xr = (int32_t)(p & ((1u << lx) - 1));
+    uint32_t row = p >> lx;
+    yr = (int32_t)(row & ((1u << ly) - 1));
+    zr = (int32_t)(row >> ly);
+  }
+};
+
+// Read-only view of a device band.
+struct BandView {
+  const uint64_t *bits;  // nwords + 1
+  const uint32_t *pre;   // nwords + 1 (exclusive prefix of popcounts, in-frame entries)
+  const double *val;     // entries
+  const uint32_t *pos;   // entries (frame bit index; pos[0] unused when base == 1)
+  Frame f;
+  GridInfo g;
+
+  // Absolute coordinates of entry/candidate position p.
+  LSG_HD void coords(uint32_t p, int32_t c[3]) const {
+    int32_t xr, yr, zr;
+    f.decode(p, xr, yr, zr);
+    c[0] = xr + f.o[0];
+    c[1] = yr + f.o[1];
+    c[2] = zr + f.o[2];
+  }
+
+  // Value of the level set at an absolute, in-grid (already mapped) point.
+  LSG_HD double value(int32_t x, int32_t y, int32_t z) const {
+    int64_t xr = (int64_t)x - f.o[0];
+    int64_t yr = (int64_t)y - f.o[1];
+    int64_t zr = (int64_t)z - f.o[2];
+    uint32_t rank;
+    bool inside = false;
+    if (zr < 0) {
+      return val[0];
+    } else if (zr >= f.ext[2]) {
+      rank = (uint32_t)f.base + pre[f.nwords];
+      return val[rank - 1];
+    } else {
+      uint64_t yc, xc;
+      if (yr < 0) { yc = 0; xc = 0; }
+      else if (yr >= f.ext[1]) { yc = (uint64_t)1 << f.ly; xc = 0; }
+      else {
+        yc = (uint64_t)yr;
+        if (xr < 0) xc = 0;
+        else if (xr >= f.ext[0]) xc = (uint64_t)1 << f.lx;
+        else { xc = (uint64_t)xr; inside = true; }
+      }
+      uint64_t lin = ((((uint64_t)zr << f.ly) + yc) << f.lx) + xc;
+      uint64_t w = lin >> 6;
+      uint32_t bt = (uint32_t)(lin & 63);
+      uint64_t word = bits[w];
+      rank = (uint32_t)f.base + pre[w] + (uint32_t)__builtin_popcountll(word & ((bt == 0) ? 0ull : (~0ull >> (64 - bt))));
+      if (inside && ((word >> bt) & 1ull)) return val[rank];
+      return val[rank - 1];
+    }
+  }
+  // Value at centre c displaced by off along dimension d (boundary mapped).
+  LSG_HD double at(const int32_t c[3], int d, int off) const {
+    int32_t p[3] = {c[0], c[1], c[2]};
+    p[d] = g.map(d, (int64_t)c[d] + off);
+    return value(p[0], p[1], p[2]);
+  }
+};
+
+// Parameters of the rate (Hamiltonian) pass.
+struct RateParams {
+  int spatial;
+  int order;
+  int calcNormals;
+  int checkDissipation;
+  double cfl;
+  double alphaFactor;
+  // built-in velocity
+  int velKind, velDir, velK;
+  double velB, velAPos, velANeg;
+  double vec[3];
+};
+
+// Pointwise rules of the band operations.
+enum OpKind : int {
+  OP_EXPAND = 0,    // Expand cycle (limit, totalLimit)
+  OP_REBUILD = 1,   // Advect::rebuildLS
+  OP_REDUCE = 2,    // Reduce(width)
+  OP_PRUNE = 3,     // Prune
+  OP_PRUNE_RSZ = 4, // Prune with removeStrayZeros
+  OP_UPDATE = 5,    // Reduce(1) + value -= dt * rate
+  OP_CHECK = 6
+};
+enum BoolKind : int { B_INTERSECT = 0, B_UNION = 1, B_RELCOMP = 2, B_COMBINE = 3 };
+
+struct OpParams {
+  int kind;
+  double a, b;  // expand: limit, totalLimit; reduce: valueLimit; rebuild: cutoff; combine: wTarget, wSource
+  const double *rates;
+  const double *dt;  // device pointer (update)
+};
+
+}  // namespace lsg

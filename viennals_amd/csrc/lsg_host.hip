@@ -1,0 +1,1058 @@
+// Host orchestration and C ABI of the MI355X level-set advection library.
+//
+// A level set (lsg_domain) owns device buffers for its canonical run-start
+// list and the frame bitmap index.  All work is enqueued on one HIP stream;
+// an advection step runs entirely on the device and the host synchronises
+// once per step to read the advanced time (Advect::apply's loop,
+// lsAdvect.hpp:1029-1042), the overflow flag and the band bounding box.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/viennals_gpu.h"
+#include "lsg_device.hpp"
+#include "lsg_geometry.hpp"
+#include "lsg_kernels.hpp"
+
+namespace lsg {
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+#define LSG_HIP(x)                                                                 \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return fail(-100, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+static constexpr int MARGIN = 24;        // frame margin (cells) in infinite dimensions
+static constexpr int REFRAME_SLACK = 12; // re-frame when the band comes closer to the frame edge
+
+struct WsStore;
+struct Ctx {
+  bool init = false;
+  hipStream_t s = nullptr;
+  DevScalars *sc = nullptr;
+  uint32_t *stat = nullptr;  // [0] count, [1..6] bbox (int32)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+static Ctx g_ctx;
+static WsStore *g_ws = nullptr;
+
+static int ctxInit() {
+  if (g_ctx.init) return 0;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-101, "no HIP device available");
+  LSG_HIP(hipStreamCreateWithFlags(&g_ctx.s, hipStreamNonBlocking));
+  LSG_HIP(hipMalloc(&g_ctx.sc, sizeof(DevScalars)));
+  LSG_HIP(hipMemset(g_ctx.sc, 0, sizeof(DevScalars)));
+  LSG_HIP(hipMalloc(&g_ctx.stat, 8 * sizeof(uint32_t)));
+  LSG_HIP(hipEventCreate(&g_ctx.ev0));
+  LSG_HIP(hipEventCreate(&g_ctx.ev1));
+  g_ctx.init = true;
+  return 0;
+}
+
+static int ceilLog2(int64_t v) {
+  int l = 0;
+  while (((int64_t)1 << l) < v) ++l;
+  return l;
+}
+
+// Device band storage sized for a frame (words) and an entry capacity.
+struct Store {
+  BandBufs b{};
+  int64_t nwAlloc = 0;
+  void release() {
+    if (b.bits) (void)hipFree(b.bits);
+    if (b.pre) (void)hipFree(b.pre);
+    if (b.val) (void)hipFree(b.val);
+    if (b.pos) (void)hipFree(b.pos);
+    if (b.n) (void)hipFree(b.n);
+    b = BandBufs{};
+    nwAlloc = 0;
+  }
+  int alloc(int64_t nw, size_t cap) {
+    release();
+    LSG_HIP(hipMalloc(&b.bits, (nw + 1) * sizeof(uint64_t)));
+    LSG_HIP(hipMalloc(&b.pre, (nw + 1) * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&b.val, cap * sizeof(double)));
+    LSG_HIP(hipMalloc(&b.pos, cap * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&b.n, sizeof(uint32_t)));
+    b.cap = cap;
+    nwAlloc = nw;
+    return 0;
+  }
+};
+
+struct WsStore {
+  Workspace w{};
+  int64_t nwAlloc = 0;
+  void release() {
+    (void)hipFree(w.cbits);
+    (void)hipFree(w.cnt);
+    (void)hipFree(w.cpre);
+    (void)hipFree(w.cpos);
+    (void)hipFree(w.out);
+    (void)hipFree(w.ks);
+    (void)hipFree(w.partials);
+    (void)hipFree(w.ncand);
+    w = Workspace{};
+    nwAlloc = 0;
+  }
+  int alloc(int64_t nw, size_t ccap) {
+    release();
+    LSG_HIP(hipMalloc(&w.cbits, (nw + 1) * sizeof(uint64_t)));
+    LSG_HIP(hipMalloc(&w.cnt, (nw + 1) * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&w.cpre, (nw + 1) * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&w.cpos, ccap * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&w.out, ccap * sizeof(double)));
+    LSG_HIP(hipMalloc(&w.ks, (ccap + 1) * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&w.partials, (SCAN_BLOCKS + 1) * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&w.ncand, sizeof(uint32_t)));
+    w.ccap = ccap;
+    nwAlloc = nw;
+    return 0;
+  }
+};
+
+}  // namespace lsg
+
+using namespace lsg;
+
+struct lsg_domain {
+  int D = 3;
+  GridInfo g{};
+  Frame f{};
+  int width = 1;
+  Store cur, tmp;  // current band and the output buffer of the next op
+  uint32_t hostN = 0;
+  bool framed = false;
+
+  ~lsg_domain() {
+    cur.release();
+    tmp.release();
+  }
+};
+
+namespace lsg {
+
+// ------------------------------------------------------------- frames ---
+
+static bool gridMinInFrame(const lsg_domain *d, const Frame &f) {
+  for (int k = 0; k < d->D; ++k)
+    if (f.o[k] != d->g.lo[k]) return false;
+  return true;
+}
+
+// Frame covering the given absolute bounding box (bounded dimensions span
+// the grid; infinite dimensions get MARGIN cells on each side).
+static int makeFrame(const lsg_domain *d, const int32_t *bbmin, const int32_t *bbmax, Frame &f) {
+  f = Frame{};
+  for (int k = 0; k < 3; ++k) {
+    if (k >= d->D) {
+      f.o[k] = 0;
+      f.ext[k] = 1;
+      continue;
+    }
+    if (d->g.bounded(k)) {
+      f.o[k] = d->g.lo[k];
+      f.ext[k] = d->g.hi[k] - d->g.lo[k] + 1;
+    } else {
+      int64_t lo = (int64_t)bbmin[k] - MARGIN, hi = (int64_t)bbmax[k] + MARGIN;
+      lo = std::max<int64_t>(lo, d->g.lo[k]);
+      hi = std::min<int64_t>(hi, d->g.hi[k]);
+      f.o[k] = (int32_t)lo;
+      f.ext[k] = (int32_t)(hi - lo + 1);
+    }
+  }
+  f.lx = std::max(6, ceilLog2(f.ext[0]));
+  f.ly = ceilLog2(f.ext[1]);
+  int64_t bitsTotal = ((int64_t)f.ext[2] << f.ly) << f.lx;
+  if (bitsTotal >= ((int64_t)1 << 32) - 64)
+    return fail(-10, "level set frame exceeds 2^32 grid points; reduce the domain or band extent");
+  f.nwords = bitsTotal >> 6;
+  f.base = gridMinInFrame(d, f) ? 0 : 1;
+  return 0;
+}
+
+static size_t capFor(size_t n) { return std::max<size_t>((size_t)1 << 20, n * 8 + 4096); }
+
+static int ensureStores(lsg_domain *d, size_t cap) {
+  const int64_t nw = d->f.nwords;
+  if (d->cur.b.cap < cap || d->cur.nwAlloc < nw) {
+    // preserve current contents
+    Store n;
+    if (int e = n.alloc(std::max(nw, d->cur.nwAlloc), std::max(cap, d->cur.b.cap))) return e;
+    if (d->cur.b.val) {
+      LSG_HIP(hipMemcpyAsync(n.b.val, d->cur.b.val, d->cur.b.cap * sizeof(double), hipMemcpyDeviceToDevice, g_ctx.s));
+      LSG_HIP(hipMemcpyAsync(n.b.pos, d->cur.b.pos, d->cur.b.cap * sizeof(uint32_t), hipMemcpyDeviceToDevice, g_ctx.s));
+      LSG_HIP(hipMemcpyAsync(n.b.bits, d->cur.b.bits, (d->cur.nwAlloc + 1) * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                             g_ctx.s));
+      LSG_HIP(hipMemcpyAsync(n.b.pre, d->cur.b.pre, (d->cur.nwAlloc + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                             g_ctx.s));
+      LSG_HIP(hipMemcpyAsync(n.b.n, d->cur.b.n, sizeof(uint32_t), hipMemcpyDeviceToDevice, g_ctx.s));
+      LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    }
+    d->cur.release();
+    d->cur = n;
+    n.b = BandBufs{};
+  }
+  if (d->tmp.b.cap < d->cur.b.cap || d->tmp.nwAlloc < d->cur.nwAlloc)
+    if (int e = d->tmp.alloc(d->cur.nwAlloc, d->cur.b.cap)) return e;
+  if (!g_ws) g_ws = new WsStore();
+  if (g_ws->w.ccap < d->cur.b.cap || g_ws->nwAlloc < d->cur.nwAlloc) {
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    if (int e = g_ws->alloc(std::max(g_ws->nwAlloc, d->cur.nwAlloc), std::max(g_ws->w.ccap, d->cur.b.cap))) return e;
+  }
+  return 0;
+}
+
+// Allocate only the current-band storage (copies kept for retries).
+static int ensureCur(lsg_domain *d, int64_t nw, size_t cap) {
+  if (d->cur.b.cap >= cap && d->cur.nwAlloc >= nw) return 0;
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  return d->cur.alloc(std::max(nw, d->cur.nwAlloc), std::max(cap, d->cur.b.cap));
+}
+
+// Upload a canonical run-start list (entry 0 = grid minimum point).
+static int uploadRuns(lsg_domain *d, const std::vector<int32_t> &idx, const std::vector<double> &val, int width,
+                      const Frame *forced = nullptr) {
+  if (int e = ctxInit()) return e;
+  const int D = d->D;
+  const size_t n = val.size();
+  if (n == 0) return fail(-11, "empty run list");
+  for (int k = 0; k < D; ++k)
+    if (idx[k] != d->g.lo[k]) return fail(-12, "run list must start at the grid minimum point");
+  int32_t bbmin[3] = {0, 0, 0}, bbmax[3] = {0, 0, 0};
+  bool any = false;
+  for (size_t i = 1; i < n; ++i) {
+    for (int k = 0; k < D; ++k) {
+      int32_t c = idx[i * D + k];
+      if (c < d->g.lo[k] || c > d->g.hi[k]) return fail(-13, "run list point outside the grid");
+      if (!any || c < bbmin[k]) bbmin[k] = c;
+      if (!any || c > bbmax[k]) bbmax[k] = c;
+    }
+    any = true;
+  }
+  if (!any)
+    for (int k = 0; k < D; ++k) bbmin[k] = bbmax[k] = (d->g.bounded(k) ? d->g.lo[k] : 0);
+  Frame f;
+  if (forced) f = *forced;
+  else if (int e = makeFrame(d, bbmin, bbmax, f)) return e;
+  d->f = f;
+  d->framed = true;
+  // host-side encode
+  std::vector<uint32_t> pos(n, 0xFFFFFFFFu);
+  std::vector<uint64_t> bits(f.nwords + 1, 0);
+  for (size_t i = 0; i < n; ++i) {
+    if (i == 0 && f.base) continue;
+    int32_t r[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) r[k] = idx[i * D + k] - f.o[k];
+    if (r[0] < 0 || r[0] >= f.ext[0] || r[1] < 0 || r[1] >= f.ext[1] || r[2] < 0 || r[2] >= f.ext[2])
+      return fail(-14, "internal: entry outside frame");
+    uint32_t p = f.encode(r[0], r[1], r[2]);
+    if (i > (size_t)f.base && p <= pos[i - 1]) return fail(-15, "run list is not strictly sorted in HRLE order");
+    pos[i] = p;
+    bits[p >> 6] |= 1ull << (p & 63);
+  }
+  std::vector<uint32_t> pre(f.nwords + 1, 0);
+  uint32_t acc = 0;
+  for (int64_t w = 0; w <= f.nwords; ++w) {
+    pre[w] = acc;
+    if (w < f.nwords) acc += (uint32_t)__builtin_popcountll(bits[w]);
+  }
+  if (int e = ensureStores(d, capFor(n))) return e;
+  BandBufs &b = d->cur.b;
+  uint32_t nn = (uint32_t)n;
+  LSG_HIP(hipMemcpyAsync(b.val, val.data(), n * sizeof(double), hipMemcpyHostToDevice, g_ctx.s));
+  LSG_HIP(hipMemcpyAsync(b.pos, pos.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, g_ctx.s));
+  LSG_HIP(hipMemcpyAsync(b.bits, bits.data(), (f.nwords + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, g_ctx.s));
+  LSG_HIP(hipMemcpyAsync(b.pre, pre.data(), (f.nwords + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, g_ctx.s));
+  LSG_HIP(hipMemcpyAsync(b.n, &nn, sizeof(uint32_t), hipMemcpyHostToDevice, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  d->hostN = nn;
+  d->width = width;
+  return 0;
+}
+
+static int downloadRuns(lsg_domain *d, std::vector<int32_t> &idx, std::vector<double> &val) {
+  if (int e = ctxInit()) return e;
+  uint32_t n = 0;
+  LSG_HIP(hipMemcpyAsync(&n, d->cur.b.n, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  std::vector<uint32_t> pos(n);
+  val.resize(n);
+  LSG_HIP(hipMemcpyAsync(val.data(), d->cur.b.val, n * sizeof(double), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipMemcpyAsync(pos.data(), d->cur.b.pos, n * sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  const int D = d->D;
+  idx.resize((size_t)n * D);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (i == 0 && d->f.base) {
+      for (int k = 0; k < D; ++k) idx[k] = d->g.lo[k];
+      continue;
+    }
+    int32_t r[3];
+    d->f.decode(pos[i], r[0], r[1], r[2]);
+    for (int k = 0; k < D; ++k) idx[(size_t)i * D + k] = r[k] + d->f.o[k];
+  }
+  d->hostN = n;
+  return 0;
+}
+
+static bool idxLessD(int D, const int32_t *a, const int32_t *b) {
+  for (int k = D - 1; k >= 0; --k)
+    if (a[k] != b[k]) return a[k] < b[k];
+  return false;
+}
+
+// Canonical run list from defined points: the run after a defined point is
+// negative iff the point's value is negative; the run before the first
+// point takes its sign (the role of HRLE's FillDomainWithSignedDistance
+// behind Domain::insertPoints, lsDomain.hpp:135-138).
+static void runsFromPoints(const lsg_domain *d, std::vector<int32_t> pidx, std::vector<double> pval,
+                           std::vector<int32_t> &idx, std::vector<double> &val) {
+  const int D = d->D;
+  const size_t n = pval.size();
+  std::vector<size_t> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return idxLessD(D, &pidx[a * D], &pidx[b * D]); });
+  idx.clear();
+  val.clear();
+  int32_t gmin[3] = {d->g.lo[0], d->g.lo[1], d->g.lo[2]};
+  for (int k = 0; k < D; ++k) idx.push_back(gmin[k]);
+  val.push_back(n == 0 ? POS : (pval[order[0]] < 0 ? NEG : POS));
+  std::vector<int32_t> cidx;
+  std::vector<double> cval;
+  for (size_t j = 0; j < n; ++j) {
+    const int32_t *p = &pidx[order[j] * D];
+    if (j > 0 && !idxLessD(D, &pidx[order[j - 1] * D], p)) continue;  // duplicate
+    bool isMin = true;
+    for (int k = 0; k < D; ++k) isMin = isMin && p[k] == gmin[k];
+    if (isMin) val[0] = pval[order[j]];
+    else {
+      for (int k = 0; k < D; ++k) idx.push_back(p[k]);
+      val.push_back(pval[order[j]]);
+    }
+    int32_t s[3] = {p[0], D > 1 ? p[1] : 0, D > 2 ? p[2] : 0};
+    int k = 0;
+    for (; k < D; ++k) {
+      if (s[k] < d->g.hi[k]) {
+        s[k]++;
+        break;
+      }
+      s[k] = d->g.lo[k];
+    }
+    if (k == D) continue;
+    bool beforeNext = (j + 1 >= n) || idxLessD(D, s, &pidx[order[j + 1] * D]);
+    if (beforeNext) {
+      for (int q = 0; q < D; ++q) idx.push_back(s[q]);
+      val.push_back(pval[order[j]] < 0 ? NEG : POS);
+    }
+  }
+  // compress to the canonical form
+  std::vector<int32_t> oi;
+  std::vector<double> ov;
+  for (size_t i = 0; i < val.size(); ++i) {
+    if (i == 0 || !isUndef(val[i]) || val[i] != val[i - 1]) {
+      for (int q = 0; q < D; ++q) oi.push_back(idx[i * D + q]);
+      ov.push_back(val[i]);
+    }
+  }
+  idx.swap(oi);
+  val.swap(ov);
+}
+
+// ------------------------------------------------------------- ops ----
+
+static void swapCur(lsg_domain *d) { std::swap(d->cur, d->tmp); }
+
+static void opStar(lsg_domain *d, const OpParams &op) {
+  launchStarOp(g_ctx.s, d->D, d->cur.b, d->tmp.b, d->f, d->g, g_ws->w, op, g_ctx.sc);
+  swapCur(d);
+}
+static void opSelf(lsg_domain *d, const OpParams &op) {
+  launchSelfOp(g_ctx.s, d->D, d->cur.b, d->tmp.b, d->f, d->g, g_ws->w, op, g_ctx.sc);
+  swapCur(d);
+}
+
+// Expand<T,D>::apply (lsExpand.hpp:44-172); the early exit for a band
+// without defined points is not taken (its result is then unchanged).
+static void doExpand(lsg_domain *d, int width) {
+  if (width <= d->width) return;
+  const double totalLimit = width * 0.5;
+  const int startWidth = d->width;
+  for (int cyc = 0; cyc < width - startWidth; ++cyc) {
+    OpParams op{};
+    op.kind = OP_EXPAND;
+    op.a = (startWidth + cyc + 1) * 0.5;
+    op.b = totalLimit;
+    opStar(d, op);
+  }
+  d->width = width;
+}
+static void doReduce(lsg_domain *d, int width) {  // lsReduce.hpp:55-134
+  if (width >= d->width) return;
+  OpParams op{};
+  op.kind = OP_REDUCE;
+  op.a = width * 0.5;
+  opSelf(d, op);
+  d->width = width;
+}
+static void doPrune(lsg_domain *d, bool rsz) {  // lsPrune.hpp:74-194
+  OpParams op{};
+  op.kind = rsz ? OP_PRUNE_RSZ : OP_PRUNE;
+  opSelf(d, op);
+  d->width = 2;
+}
+static void doRebuild(lsg_domain *d) {  // lsAdvect.hpp:238-429
+  OpParams op{};
+  op.kind = OP_REBUILD;
+  op.a = 1.0;
+  opStar(d, op);
+  d->width = 2;
+}
+// booleanOpInternal + pruning (lsBooleanOperation.hpp:59-213)
+static void doBoolean(lsg_domain *a, const lsg_domain *b, int kind, double wT, double wS) {
+  launchBoolean(g_ctx.s, a->D, a->cur.b, b->cur.b, a->tmp.b, a->f, a->g, g_ws->w, kind, wT, wS, g_ctx.sc);
+  swapCur(a);
+  doPrune(a, true);
+  doPrune(a, false);
+}
+
+static int readOverflow(bool reset) {
+  uint32_t ov = 0;
+  LSG_HIP(hipMemcpyAsync(&ov, &g_ctx.sc->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  if (ov && reset) LSG_HIP(hipMemsetAsync(&g_ctx.sc->overflow, 0, sizeof(uint32_t), g_ctx.s));
+  return ov ? 1 : 0;
+}
+
+// Band statistics: defined count and absolute bounding box of in-frame entries.
+static int bandStats(lsg_domain *d, uint32_t &ndef, int32_t bbmin[3], int32_t bbmax[3], bool &any) {
+  launchBandStats(g_ctx.s, d->cur.b, d->f, g_ctx.stat, (int32_t *)(g_ctx.stat + 1));
+  uint32_t h[7];
+  LSG_HIP(hipMemcpyAsync(h, g_ctx.stat, 7 * sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  ndef = h[0];
+  const int32_t *bb = (const int32_t *)(h + 1);
+  any = bb[0] <= bb[3];
+  for (int k = 0; k < 3; ++k) {
+    bbmin[k] = bb[k] + d->f.o[k];
+    bbmax[k] = bb[3 + k] + d->f.o[k];
+  }
+  return 0;
+}
+
+// Move the frame when the band approached its edge in an infinite dimension.
+static int maybeReframe(lsg_domain *d, bool force) {
+  uint32_t ndef;
+  int32_t bbmin[3], bbmax[3];
+  bool any;
+  if (int e = bandStats(d, ndef, bbmin, bbmax, any)) return e;
+  if (!any) return 0;
+  bool need = force;
+  for (int k = 0; k < d->D; ++k) {
+    if (d->g.bounded(k)) continue;
+    if (bbmin[k] - d->f.o[k] < REFRAME_SLACK || (d->f.o[k] + d->f.ext[k] - 1) - bbmax[k] < REFRAME_SLACK)
+      need = true;
+  }
+  if (!need) return 0;
+  std::vector<int32_t> idx;
+  std::vector<double> val;
+  if (int e = downloadRuns(d, idx, val)) return e;
+  int w = d->width;
+  return uploadRuns(d, idx, val, w);
+}
+
+// Bring two level sets onto a common frame (BooleanOperation inputs).
+static int commonFrame(lsg_domain *a, lsg_domain *b) {
+  if (a->D != b->D) return fail(-20, "level sets of different dimension");
+  for (int k = 0; k < 3; ++k)
+    if (a->g.lo[k] != b->g.lo[k] || a->g.hi[k] != b->g.hi[k] || a->g.bc[k] != b->g.bc[k])
+      return fail(-21, "level sets on different grids");
+  if (std::memcmp(&a->f, &b->f, sizeof(Frame)) == 0 && a->cur.nwAlloc == b->cur.nwAlloc) return 0;
+  uint32_t nd;
+  int32_t amin[3], amax[3], bmin[3], bmax[3];
+  bool aa, ba;
+  if (int e = bandStats(a, nd, amin, amax, aa)) return e;
+  if (int e = bandStats(b, nd, bmin, bmax, ba)) return e;
+  int32_t mn[3], mx[3];
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = aa ? (ba ? std::min(amin[k], bmin[k]) : amin[k]) : bmin[k];
+    mx[k] = aa ? (ba ? std::max(amax[k], bmax[k]) : amax[k]) : bmax[k];
+  }
+  if (!aa && !ba)
+    for (int k = 0; k < 3; ++k) mn[k] = mx[k] = 0;
+  Frame f;
+  if (int e = makeFrame(a, mn, mx, f)) return e;
+  for (lsg_domain *d : {a, b}) {
+    std::vector<int32_t> idx;
+    std::vector<double> val;
+    if (int e = downloadRuns(d, idx, val)) return e;
+    if (int e = uploadRuns(d, idx, val, d->width, &f)) return e;
+  }
+  return 0;
+}
+
+}  // namespace lsg
+
+// =============================================================== Advect ===
+
+struct lsg_advect {
+  std::vector<lsg_domain *> levelSets;
+  int spatial = 0, temporal = 0;
+  double cfl = 0.4999, advectionTime = 0.0, dissipationAlpha = 1.0;
+  int singleStep = 0, calcNormals = 1, checkDissipation = 1, ignoreVoids = 0;
+  bool velSet = false;
+  RateParams rp{};
+  double advectedTime = 0, currentTimeStep = -1, pointUpdates = 0;
+  unsigned numSteps = 0;
+  lsg_domain *init = nullptr;  // RK initial level set
+  lsg_domain *save = nullptr;  // band at the start of a step (capacity-overflow retry)
+  double *rates = nullptr;
+  size_t ratesCap = 0;
+  bool profiling = false;
+  std::vector<hipEvent_t> evs;
+  double rateMs = 0, rateBytes = 0;
+  int64_t rateLaunches = 0;
+  std::vector<uint32_t> rateN;  // band entries per profiled launch
+  ~lsg_advect() {
+    if (init) delete init;
+    if (save) delete save;
+    if (rates) (void)hipFree(rates);
+    for (auto e : evs) (void)hipEventDestroy(e);
+  }
+};
+
+namespace lsg {
+
+static int prepareWidth(int spatial) {
+  switch (spatial) {
+  case 0: return 3;   // EO1 (lsEngquistOsher.hpp:31)
+  case 1: return 5;   // EO2
+  case 2: return 3;   // LF1 (lsLaxFriedrichs.hpp:35)
+  case 3: return 5;   // LF2
+  case 5: return 3;   // LLLF1 (lsLocalLocalLaxFriedrichs.hpp:32)
+  case 6: return 5;   // LLLF2
+  case 7: return 7;   // LLF1 (lsLocalLaxFriedrichs.hpp:55)
+  case 8: return 9;   // LLF2
+  case 10: return 5;  // WENO3 (lsWENO.hpp:44)
+  case 11: return 7;  // WENO5
+  default: return -1;
+  }
+}
+static int schemeOrder(int spatial) {
+  switch (spatial) {
+  case 1: case 3: case 6: case 8: return 2;
+  case 10: return 3;
+  case 11: return 5;
+  default: return 1;
+  }
+}
+
+static int ensureRates(lsg_advect *a, size_t cap) {
+  if (a->ratesCap >= cap) return 0;
+  if (a->rates) (void)hipFree(a->rates);
+  LSG_HIP(hipMalloc(&a->rates, cap * sizeof(double)));
+  a->ratesCap = cap;
+  return 0;
+}
+
+// One forward-Euler stage (lsAdvectIntegrationSchemes.hpp:46-59):
+// computeRates (prepareLS + integrateTime), updateLevelSet, rebuildLS.
+static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *dtOut) {
+  doExpand(d, prepareWidth(a->spatial));  // Advect::prepareLS (lsAdvect.hpp:973-1015)
+  RateParams rp = a->rp;
+  if (a->spatial == 2 || a->spatial == 3) launchGlobalAlphas(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, g_ctx.sc);
+  if (a->profiling) {
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, g_ctx.s);
+    launchRates(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, a->rates, g_ctx.sc, maxDt, dtOut);
+    (void)hipEventRecord(e1, g_ctx.s);
+    a->evs.push_back(e0);
+    a->evs.push_back(e1);
+  } else {
+    launchRates(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, a->rates, g_ctx.sc, maxDt, dtOut);
+  }
+  // updateLevelSet: Reduce(1) + update (lsAdvect.hpp:686-800)
+  OpParams op{};
+  op.kind = OP_UPDATE;
+  op.rates = a->rates;
+  op.dt = dtOut;
+  opSelf(d, op);
+  d->width = 1;
+  doRebuild(d);
+}
+
+// Advect::combineLevelSets (lsAdvect.hpp:199-236)
+static void combine(lsg_advect *a, lsg_domain *d, lsg_domain *init, double wTarget, double wSource) {
+  int steps = 1;
+  if (a->temporal == 1) steps = 2;
+  else if (a->temporal == 2) steps = 3;
+  int ew = (int)std::ceil(2.0 * steps * a->cfl + 1);
+  doExpand(d, ew);
+  doExpand(init, ew);
+  doBoolean(d, init, B_COMBINE, wTarget, wSource);
+  doRebuild(d);
+}
+
+static int copyDomain(lsg_domain *dst, const lsg_domain *src) {
+  dst->D = src->D;
+  dst->g = src->g;
+  dst->f = src->f;
+  dst->framed = src->framed;
+  dst->width = src->width;
+  if (int e = ensureCur(dst, src->cur.nwAlloc, src->cur.b.cap)) return e;
+  launchCopyBand(g_ctx.s, src->cur.b, dst->cur.b, src->f);
+  dst->hostN = src->hostN;
+  return 0;
+}
+
+// One time step on the device (Advect::advect, lsAdvect.hpp:817-830).
+static void enqueueStep(lsg_advect *a, lsg_domain *d) {
+  DevScalars *sc = g_ctx.sc;
+  if (a->temporal == 0) {
+    stageFE(a, d, &sc->maxTimeStep, &sc->dt[0]);
+  } else if (a->temporal == 1) {  // evolveRungeKutta2 (lsAdvectIntegrationSchemes.hpp:61-108)
+    copyDomain(a->init, d);
+    stageFE(a, d, &sc->maxTimeStep, &sc->dt[0]);
+    stageFE(a, d, &sc->dt[0], &sc->dt[1]);
+    combine(a, d, a->init, 0.5, 0.5);
+  } else {  // evolveRungeKutta3 (lsAdvectIntegrationSchemes.hpp:110-168)
+    copyDomain(a->init, d);
+    stageFE(a, d, &sc->maxTimeStep, &sc->dt[0]);
+    stageFE(a, d, &sc->dt[0], &sc->dt[1]);
+    combine(a, d, a->init, 0.75, 0.25);
+    stageFE(a, d, &sc->dt[0], &sc->dt[2]);
+    combine(a, d, a->init, 1.0 / 3.0, 2.0 / 3.0);
+  }
+  launchStepTime(g_ctx.s, sc, a->temporal);
+}
+
+struct StepOut {
+  double stepTime, dt1;
+  unsigned long long active;
+};
+
+static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &out) {
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    // keep a copy of the band to retry after a capacity overflow
+    if (!a->save) a->save = new lsg_domain();
+    lsg_domain *save = a->save;
+    if (int e = copyDomain(save, d)) return e;
+    int widthBefore = d->width;
+    LSG_HIP(hipMemcpyAsync(&g_ctx.sc->maxTimeStep, &maxTimeStep, sizeof(double), hipMemcpyHostToDevice, g_ctx.s));
+    LSG_HIP(hipMemsetAsync(&g_ctx.sc->overflow, 0, sizeof(uint32_t), g_ctx.s));
+    unsigned long long act0 = 0;
+    LSG_HIP(hipMemcpyAsync(&act0, &g_ctx.sc->totalActive, sizeof(act0), hipMemcpyDeviceToHost, g_ctx.s));
+    enqueueStep(a, d);
+    DevScalars h;
+    LSG_HIP(hipMemcpyAsync(&h, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    if (!h.overflow) {
+      out.stepTime = h.stepTime;
+      out.dt1 = h.dt[0];
+      out.active = h.totalActive - act0;
+      return maybeReframe(d, false);
+    }
+    // grow and retry from the saved band
+    size_t cap = d->cur.b.cap * 2;
+    if (int e = copyDomain(d, save)) return e;
+    d->width = widthBefore;
+    if (int e = ensureStores(d, cap)) return e;
+    if (a->init && a->init->framed && (a->init->cur.b.cap < cap))
+      if (int e = ensureStores(a->init, cap)) return e;
+    if (int e = ensureRates(a, cap)) return e;
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  }
+  return fail(-30, "band capacity overflow persists after retries");
+}
+
+static int advectApply(lsg_advect *a) {
+  if (a->levelSets.empty()) return fail(-1, "No level sets passed to Advect. Not advecting.");
+  if (!a->velSet) return fail(-2, "No velocity field passed to Advect. Not advecting.");
+  if (a->levelSets.size() > 1) return fail(-3, "Advect: more than one level set (multi-material) is not supported");
+  if (prepareWidth(a->spatial) < 0)
+    return fail(-4, "Advect: Discretization scheme not supported on the GPU (schemes 4 and 9 are not implemented)");
+  if (a->ignoreVoids) return fail(-5, "Advect: setIgnoreVoids(true) is not supported");
+  if (!(a->cfl > 0)) return fail(-6, "Advect: time step ratio must be positive");
+  if (int e = ctxInit()) return e;
+  lsg_domain *d = a->levelSets.back();
+  if (!d->framed) return fail(-7, "Advect: level set has no data");
+  if (a->cfl >= 0.5) fprintf(stderr, "[viennals_gpu] Integration time step ratio should be smaller than 0.5. Advection might fail!\n");
+  a->rp.spatial = a->spatial;
+  a->rp.order = schemeOrder(a->spatial);
+  a->rp.cfl = a->cfl;
+  a->rp.alphaFactor = a->dissipationAlpha;
+  a->rp.checkDissipation = a->checkDissipation;
+  // WENO receives dissipationAlpha as its calcNormal flag (lsAdvect.hpp:670-676)
+  a->rp.calcNormals = (a->spatial == 10 || a->spatial == 11) ? (a->dissipationAlpha != 0.0) : a->calcNormals;
+  if (int e = ensureStores(d, d->cur.b.cap)) return e;
+  if (int e = ensureRates(a, d->cur.b.cap)) return e;
+  if (a->temporal != 0) {
+    if (!a->init) a->init = new lsg_domain();
+    if (int e = copyDomain(a->init, d)) return e;
+    if (int e = ensureStores(a->init, d->cur.b.cap)) return e;
+  }
+  for (auto e : a->evs) (void)hipEventDestroy(e);
+  a->evs.clear();
+  a->pointUpdates = 0;
+  StepOut so{};
+  if (a->advectionTime == 0.) {
+    if (int e = runStep(a, d, std::numeric_limits<double>::max(), so)) return e;
+    a->advectedTime = so.stepTime;
+    a->numSteps = 1;
+    a->pointUpdates += (double)so.active;
+  } else {
+    double t = 0.0;
+    a->numSteps = 0;
+    while (t < a->advectionTime) {
+      if (int e = runStep(a, d, a->advectionTime - t, so)) return e;
+      a->pointUpdates += (double)so.active;
+      if (!(so.stepTime > 0.)) return fail(-8, "Advect: time step is not positive; advection stalled");
+      t += so.stepTime;
+      ++a->numSteps;
+      if (a->singleStep) break;
+    }
+    a->advectedTime = t;
+  }
+  a->currentTimeStep = so.dt1;
+  if (a->temporal == 0) a->currentTimeStep = so.stepTime;
+  if (a->profiling) {
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    double rb = 0;
+    LSG_HIP(hipMemcpy(&rb, &g_ctx.sc->rateBytes, sizeof(double), hipMemcpyDeviceToHost));
+    a->rateBytes = rb;
+    for (size_t i = 0; i + 1 < a->evs.size(); i += 2) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, a->evs[i], a->evs[i + 1]);
+      a->rateMs += ms;
+      a->rateLaunches += 1;
+    }
+    for (auto e : a->evs) (void)hipEventDestroy(e);
+    a->evs.clear();
+  }
+  return 0;
+}
+
+}  // namespace lsg
+
+// ================================================================ C ABI ===
+
+extern "C" {
+
+const char *lsg_last_error(void) { return g_err.c_str(); }
+
+int lsg_domain_create(int D, const int32_t *gmin, const int32_t *gmax, const int *bc, double grid_delta,
+                      lsg_domain **out) {
+  if (D != 2 && D != 3) return fail(-1, "dimension must be 2 or 3");
+  if (!(grid_delta > 0)) return fail(-1, "gridDelta must be positive");
+  if (int e = ctxInit()) return e;
+  auto *d = new lsg_domain();
+  d->D = D;
+  for (int k = 0; k < 3; ++k) {
+    d->g.lo[k] = d->g.hi[k] = 0;
+    d->g.bc[k] = REFLECTIVE;
+  }
+  for (int k = 0; k < D; ++k) {
+    int b = bc[k];
+    d->g.bc[k] = b;
+    switch (b) {
+    case INFINITE: d->g.lo[k] = -INF_IDX; d->g.hi[k] = INF_IDX; break;
+    case PERIODIC: d->g.lo[k] = gmin[k]; d->g.hi[k] = gmax[k] - 1; break;
+    case POS_INFINITE: d->g.lo[k] = gmin[k]; d->g.hi[k] = INF_IDX; break;
+    case NEG_INFINITE: d->g.lo[k] = -INF_IDX; d->g.hi[k] = gmax[k]; break;
+    case REFLECTIVE: d->g.lo[k] = gmin[k]; d->g.hi[k] = gmax[k]; break;
+    default: delete d; return fail(-1, "unknown boundary condition");
+    }
+  }
+  d->g.D = D;
+  d->g.dx = grid_delta;
+  // empty level set: one positive undefined run (Domain constructor, lsDomain.hpp:76-77)
+  std::vector<int32_t> idx(D);
+  for (int k = 0; k < D; ++k) idx[k] = d->g.lo[k];
+  std::vector<double> val(1, POS);
+  if (int e = uploadRuns(d, idx, val, 1)) {
+    delete d;
+    return e;
+  }
+  *out = d;
+  return 0;
+}
+
+int lsg_domain_destroy(lsg_domain *d) {
+  delete d;
+  return 0;
+}
+
+int lsg_domain_deep_copy(lsg_domain *dst, const lsg_domain *src) {
+  if (int e = copyDomain(dst, src)) return e;
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  return 0;
+}
+
+int lsg_domain_insert_points(lsg_domain *d, int64_t n, const int32_t *idx, const double *val, int width) {
+  std::vector<int32_t> pi(idx, idx + n * d->D);
+  std::vector<double> pv(val, val + n);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < d->D; ++k)
+      if (pi[i * d->D + k] < d->g.lo[k] || pi[i * d->D + k] > d->g.hi[k]) return fail(-1, "point outside the grid");
+  std::vector<int32_t> ri;
+  std::vector<double> rv;
+  runsFromPoints(d, pi, pv, ri, rv);
+  return uploadRuns(d, ri, rv, width);
+}
+
+int lsg_domain_set_runs(lsg_domain *d, int64_t n, const int32_t *idx, const double *val, int width) {
+  std::vector<int32_t> ri(idx, idx + n * d->D);
+  std::vector<double> rv(val, val + n);
+  return uploadRuns(d, ri, rv, width);
+}
+
+int lsg_domain_num_runs(lsg_domain *d, int64_t *n) {
+  uint32_t h = 0;
+  LSG_HIP(hipMemcpyAsync(&h, d->cur.b.n, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  *n = h;
+  return 0;
+}
+
+int lsg_domain_get_runs(lsg_domain *d, int32_t *idx, double *val, int64_t cap) {
+  std::vector<int32_t> ri;
+  std::vector<double> rv;
+  if (int e = downloadRuns(d, ri, rv)) return e;
+  if ((int64_t)rv.size() > cap) return fail(-1, "buffer too small");
+  std::memcpy(idx, ri.data(), ri.size() * sizeof(int32_t));
+  std::memcpy(val, rv.data(), rv.size() * sizeof(double));
+  return 0;
+}
+
+int lsg_domain_num_points(lsg_domain *d, int64_t *n) {
+  uint32_t nd;
+  int32_t a[3], b[3];
+  bool any;
+  if (int e = bandStats(d, nd, a, b, any)) return e;
+  *n = nd;
+  return 0;
+}
+
+int lsg_domain_width(lsg_domain *d, int *w) {
+  *w = d->width;
+  return 0;
+}
+
+int lsg_make_sphere(lsg_domain *d, const double *origin, double radius) {
+  std::vector<int32_t> pi;
+  std::vector<double> pv;
+  sphereBand(d->D, d->g, origin, radius, pi, pv);
+  std::vector<int32_t> ri;
+  std::vector<double> rv;
+  runsFromPoints(d, pi, pv, ri, rv);
+  return uploadRuns(d, ri, rv, 2);
+}
+
+int lsg_make_plane(lsg_domain *d, const double *origin, const double *normal) {
+  std::vector<int32_t> pi;
+  std::vector<double> pv;
+  int e = planeBand(d->D, d->g, origin, normal, pi, pv);
+  if (e == -1) return fail(-1, "MakeGeometry: plane normal is zero");
+  if (e == -2) return fail(-1, "Planes can only be created with one Infinite Boundary Condition. More than one found!");
+  if (e == -3) return fail(-1, "MakeGeometry: Plane cannot be parallel to Infinite Boundary direction!");
+  std::vector<int32_t> ri;
+  std::vector<double> rv;
+  runsFromPoints(d, pi, pv, ri, rv);
+  return uploadRuns(d, ri, rv, 2);
+}
+
+int lsg_make_box(lsg_domain *d, const double *mn, const double *mx) {
+  std::vector<int32_t> pi;
+  std::vector<double> pv;
+  boxBand(d->D, d->g, mn, mx, pi, pv);
+  std::vector<int32_t> ri;
+  std::vector<double> rv;
+  runsFromPoints(d, pi, pv, ri, rv);
+  return uploadRuns(d, ri, rv, 2);
+}
+
+static int afterOp(lsg_domain *d) {
+  if (readOverflow(true)) return fail(-40, "band capacity overflow");
+  return maybeReframe(d, false);
+}
+
+int lsg_expand(lsg_domain *d, int width) {
+  if (int e = ensureStores(d, d->cur.b.cap)) return e;
+  doExpand(d, width);
+  return afterOp(d);
+}
+int lsg_reduce(lsg_domain *d, int width) {
+  if (int e = ensureStores(d, d->cur.b.cap)) return e;
+  doReduce(d, width);
+  return afterOp(d);
+}
+int lsg_prune(lsg_domain *d, int rsz) {
+  if (int e = ensureStores(d, d->cur.b.cap)) return e;
+  doPrune(d, rsz != 0);
+  return afterOp(d);
+}
+int lsg_rebuild(lsg_domain *d) {
+  if (int e = ensureStores(d, d->cur.b.cap)) return e;
+  doRebuild(d);
+  return afterOp(d);
+}
+int lsg_boolean(lsg_domain *a, const lsg_domain *b, int op) {
+  if (op < 0 || op > 2) return fail(-1, "unsupported boolean operation");
+  if (int e = commonFrame(a, const_cast<lsg_domain *>(b))) return e;
+  if (int e = ensureStores(a, std::max(a->cur.b.cap, b->cur.b.cap))) return e;
+  doBoolean(a, b, op, 0, 0);
+  return afterOp(a);
+}
+
+int lsg_advect_create(lsg_advect **out) {
+  *out = new lsg_advect();
+  return 0;
+}
+int lsg_advect_destroy(lsg_advect *a) {
+  delete a;
+  return 0;
+}
+int lsg_advect_insert_next_level_set(lsg_advect *a, lsg_domain *d) {
+  a->levelSets.push_back(d);
+  return 0;
+}
+int lsg_advect_clear_level_sets(lsg_advect *a) {
+  a->levelSets.clear();
+  return 0;
+}
+int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, double b, double aPos, double aNeg,
+                                    const double *vec3) {
+  if (kind != 0 && kind != 1) return fail(-1, "unknown velocity kind");
+  if (dir < 0 || dir > 2) return fail(-1, "velocity direction must be 0, 1 or 2");
+  a->rp.velKind = kind;
+  a->rp.velDir = dir;
+  a->rp.velK = k;
+  a->rp.velB = b;
+  a->rp.velAPos = aPos;
+  a->rp.velANeg = aNeg;
+  for (int i = 0; i < 3; ++i) a->rp.vec[i] = vec3 ? vec3[i] : 0.0;
+  a->velSet = true;
+  return 0;
+}
+int lsg_advect_set_spatial_scheme(lsg_advect *a, int s) {
+  if (s < 0 || s > 11) return fail(-1, "Advect: Discretization scheme not found.");
+  a->spatial = s;
+  return 0;
+}
+int lsg_advect_set_temporal_scheme(lsg_advect *a, int s) {
+  a->temporal = (s == 1 || s == 2) ? s : 0;
+  return 0;
+}
+int lsg_advect_set_advection_time(lsg_advect *a, double t) {
+  a->advectionTime = t;
+  return 0;
+}
+int lsg_advect_set_time_step_ratio(lsg_advect *a, double c) {
+  a->cfl = c;
+  return 0;
+}
+int lsg_advect_set_single_step(lsg_advect *a, int s) {
+  a->singleStep = s;
+  return 0;
+}
+int lsg_advect_set_dissipation_alpha(lsg_advect *a, double v) {
+  a->dissipationAlpha = v;
+  return 0;
+}
+int lsg_advect_set_check_dissipation(lsg_advect *a, int c) {
+  a->checkDissipation = c;
+  return 0;
+}
+int lsg_advect_set_calculate_normal_vectors(lsg_advect *a, int c) {
+  a->calcNormals = c;
+  return 0;
+}
+int lsg_advect_set_ignore_voids(lsg_advect *a, int iv) {
+  a->ignoreVoids = iv;
+  return 0;
+}
+int lsg_advect_set_adaptive_time_stepping(lsg_advect *, int, unsigned) { return 0; }
+int lsg_advect_set_update_point_data(lsg_advect *, int) { return 0; }
+int lsg_advect_prepare_ls(lsg_advect *a) {
+  if (a->levelSets.empty()) return fail(-1, "No level sets passed to Advect.");
+  int w = prepareWidth(a->spatial);
+  if (w < 0) return fail(-1, "Advect: Discretization scheme not found.");
+  lsg_domain *d = a->levelSets.back();
+  if (int e = ensureStores(d, d->cur.b.cap)) return e;
+  doExpand(d, w);
+  return afterOp(d);
+}
+int lsg_advect_apply(lsg_advect *a) { return advectApply(a); }
+int lsg_advect_get_advected_time(lsg_advect *a, double *t) {
+  *t = a->advectedTime;
+  return 0;
+}
+int lsg_advect_get_current_time_step(lsg_advect *a, double *dt) {
+  *dt = a->currentTimeStep;
+  return 0;
+}
+int lsg_advect_get_number_of_time_steps(lsg_advect *a, unsigned *n) {
+  *n = a->numSteps;
+  return 0;
+}
+int lsg_advect_get_point_updates(lsg_advect *a, double *n) {
+  *n = a->pointUpdates;
+  return 0;
+}
+int lsg_advect_set_profiling(lsg_advect *a, int on) {
+  if (int e = ctxInit()) return e;
+  double z = 0;
+  LSG_HIP(hipMemcpy(&g_ctx.sc->rateBytes, &z, sizeof(double), hipMemcpyHostToDevice));
+  a->profiling = on != 0;
+  a->rateMs = 0;
+  a->rateLaunches = 0;
+  a->rateBytes = 0;
+  return 0;
+}
+int lsg_advect_get_rate_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes) {
+  *ms = a->rateMs;
+  *launches = a->rateLaunches;
+  *bytes = a->rateBytes;
+  return 0;
+}
+
+int lsg_weno_eval(int order, int64_t n, const double *x, double dx, int plus, double *out) {
+  if (order != 3 && order != 5) return fail(-1, "WENO order must be 3 or 5.");
+  if (int e = ctxInit()) return e;
+  const int w = order == 3 ? 5 : 7;
+  double *dx_ = nullptr, *dout = nullptr;
+  LSG_HIP(hipMalloc(&dx_, (size_t)n * w * sizeof(double) + 8));
+  LSG_HIP(hipMalloc(&dout, (size_t)n * sizeof(double) + 8));
+  LSG_HIP(hipMemcpyAsync(dx_, x, (size_t)n * w * sizeof(double), hipMemcpyHostToDevice, g_ctx.s));
+  launchWeno(g_ctx.s, order, (uint32_t)n, dx_, dx, plus, dout);
+  LSG_HIP(hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  (void)hipFree(dx_);
+  (void)hipFree(dout);
+  return 0;
+}
+
+int lsg_device_synchronize(void) {
+  if (int e = ctxInit()) return e;
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  return 0;
+}
+
+}  // extern "C"

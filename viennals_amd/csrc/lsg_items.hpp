@@ -1,0 +1,660 @@
+// Per-item logic of the advection kernels: the pointwise rules of the band
+// operations, the Hamiltonians, the candidate-bitmap word and the index
+// rebuild of one bitmap word.  The kernels in lsg_kernels.hip are grid-stride
+// loops around these functions.  They are __host__ __device__ so that the
+// same code can be exercised on the host.
+//
+// Floating-point expressions keep the reference's operation order (files
+// cited per function) and the library is built with -ffp-contract=off.
+#pragma once
+
+#include "lsg_device.hpp"
+
+namespace lsg {
+
+LSG_HD inline double dmaxs(double a, double b) { return (a < b) ? b : a; }  // std::max
+LSG_HD inline double dmins(double a, double b) { return (b < a) ? b : a; }  // std::min
+LSG_HD inline double pow2(double v) { return v * v; }
+
+// ------------------------------------------------------------ rules ----
+
+// Expand<T,D>::apply inner rule (lsExpand.hpp:107-157)
+template <int D> LSG_HD inline double ruleExpand(double c, const double *nb, double limit, double totalLimit) {
+  if (fabs(c) <= totalLimit) return c;
+  if (c > -DEPS) {
+    double dist = POS;
+    for (int i = 0; i < 2 * D; i++) {
+      double nv = nb[i] + 1.0;
+      if (dist > nv) dist = nv;
+    }
+    return (dist <= limit) ? dist : POS;
+  }
+  double dist = NEG;
+  for (int i = 0; i < 2 * D; i++) {
+    double nv = nb[i] - 1.0;
+    if (dist < nv) dist = nv;
+  }
+  return (dist >= -limit) ? dist : NEG;
+}
+
+// Advect::rebuildLS inner rule (lsAdvect.hpp:304-414)
+template <int D> LSG_HD inline double ruleRebuild(double c, const double *nb, double cutoff) {
+  if (fabs(c) <= 1.0) {
+    int k = 0;
+    for (; k < 2 * D; k++)
+      if (signbit(nb[k] - 1e-7) != signbit(c + 1e-7)) break;
+    if (k != 2 * D) {
+      if (c > 0.5) {
+        int j = 0;
+        for (; j < 2 * D; j++)
+          if (fabs(nb[j]) <= 1.0 && nb[j] < -0.5) break;
+        return (j == 2 * D) ? c : 0.5;
+      } else if (c < -0.5) {
+        int j = 0;
+        for (; j < 2 * D; j++)
+          if (fabs(nb[j]) <= 1.0 && nb[j] > 0.5) break;
+        return (j == 2 * D) ? c : -0.5;
+      }
+      return c;
+    }
+    return (c < 0) ? NEG : POS;
+  }
+  if (c >= 0) {
+    double dist = POS;
+    for (int i = 0; i < 2 * D; i++) {
+      double v = nb[i];
+      if (fabs(v) <= 1.0 && (v < 0.)) {
+        if (dist > v + 1.0) dist = v + 1.0;
+      }
+    }
+    return (dist <= cutoff) ? dist : POS;
+  }
+  double dist = NEG;
+  for (int i = 0; i < 2 * D; i++) {
+    double v = nb[i];
+    if (fabs(v) <= 1.0 && (v > 0)) {
+      if (dist < v - 1.0) dist = v - 1.0;
+    }
+  }
+  return (dist >= -cutoff) ? dist : NEG;
+}
+
+// Prune<T,D>::apply inner rule (lsPrune.hpp:22-52,121-179)
+LSG_HD inline bool pIsNeg(double a) { return a <= -DEPS; }
+LSG_HD inline bool pSignDiff(double a, double b) { return pIsNeg(a) ^ pIsNeg(b); }
+LSG_HD inline bool pSignDiffOrZero(double a, double b) {
+  if (a == 0. || b == 0.) return true;
+  return pSignDiff(a, b);
+}
+LSG_HD inline bool pMonotone(double a, double c) { return a == 0. || c == 0. || pSignDiff(a, c); }
+
+template <int D> LSG_HD inline double rulePrune(double c, const double *nb, bool rsz) {
+  const bool centerSign = pIsNeg(c);
+  if (!isUndef(c)) {
+    bool keep = true;
+    if (fabs(c) != 0.) {
+      keep = false;
+      for (int i = 0; i < 2 * D; ++i) {
+        if (rsz ? pSignDiffOrZero(c, nb[i]) : pSignDiff(c, nb[i])) {
+          keep = true;
+          break;
+        }
+      }
+    }
+    if (rsz && fabs(c) == 0.) {
+      for (int i = 0; i < D; ++i) {
+        double negVal = nb[i];
+        double posVal = nb[D + i];
+        if (!pMonotone(negVal, posVal)) return fabs(negVal) < fabs(posVal) ? negVal : posVal;
+      }
+    }
+    if (keep) return c;
+  }
+  return centerSign ? NEG : POS;
+}
+
+// BooleanOperation comparators (lsBooleanOperation.hpp:250-269) and the
+// CUSTOM comparator of Advect::combineLevelSets (lsAdvect.hpp:218-230);
+// undefined results keep their sign (lsBooleanOperation.hpp:115-118).
+LSG_HD inline double ruleBoolean(int kind, double a, double b, double wT, double wS) {
+  double r;
+  if (kind == B_INTERSECT) r = (a > b) ? a : b;
+  else if (kind == B_UNION) r = (a < b) ? a : b;
+  else if (kind == B_RELCOMP) r = (a > -b) ? a : -b;
+  else {
+    if (!isUndef(a) && !isUndef(b)) r = wS * a + wT * b;
+    else if (isUndef(a)) r = a;
+    else r = b;
+  }
+  if (isUndef(r)) r = (r < 0) ? NEG : POS;
+  return r;
+}
+
+// Reduce rule (lsReduce.hpp:106-118) and updateLevelSet for one level set
+// (lsAdvect.hpp:700,725-762).
+LSG_HD inline double ruleReduce(double c, double limit) {
+  return (!isUndef(c) && fabs(c) <= limit) ? c : ((c < 0) ? NEG : POS);
+}
+LSG_HD inline double ruleUpdate(double c, double dt, double velocity) {
+  if (!isUndef(c) && fabs(c) <= 0.5) {
+    double rate = dt * velocity;
+    return c - rate;
+  }
+  return (c < 0) ? NEG : POS;
+}
+
+// ------------------------------------------------------- neighbourhood ----
+
+template <int D> LSG_HD inline void gatherStar(const BandView &b, const int32_t c[3], double &center, double *nb) {
+  center = b.value(c[0], c[1], c[2]);
+  for (int d = 0; d < D; ++d) {
+    nb[d] = b.at(c, d, 1);
+    nb[D + d] = b.at(c, d, -1);
+  }
+}
+
+// Coordinates of candidate / entry i at frame position p (entry 0 is the
+// grid minimum point when it lies outside the frame).
+LSG_HD inline void candCoords(const BandView &b, uint32_t i, uint32_t p, int32_t c[3]) {
+  if (i == 0 && b.f.base) {
+    c[0] = b.g.lo[0];
+    c[1] = b.g.lo[1];
+    c[2] = b.g.lo[2];
+  } else {
+    b.coords(p, c);
+  }
+}
+
+// Star-dilation candidate word (visit set of HRLE's
+// ConstSparseStarIterator<.,1>): grid points whose boundary-mapped star
+// neighbourhood holds a run start, their lexicographic successors, and the
+// first point of every row when x is bounded (successor of the previous
+// row's last point).
+template <int D> LSG_HD inline uint64_t candStarWord(const uint64_t *bits, const Frame &f, const GridInfo &g, int64_t w) {
+  const int wpr = f.wpr();
+  const int wshift = f.lx - 6;
+  const int lastBit = f.ext[0] - 1;
+  const int lastWord = lastBit >> 6;
+  const int kw = (int)(w & (wpr - 1));
+  const int64_t row = w >> wshift;
+  const int32_t yr = (int32_t)(row & ((1 << f.ly) - 1));
+  const int32_t zr = (int32_t)(row >> f.ly);
+  if (!(yr < f.ext[1] && zr < f.ext[2] && kw <= lastWord)) return 0;
+  const uint64_t B = bits[w];
+  const uint64_t BL = kw > 0 ? bits[w - 1] : 0;
+  const uint64_t BH = kw + 1 < wpr ? bits[w + 1] : 0;
+  uint64_t dil = B | (B << 1) | (BL >> 63) | (B >> 1) | (BH << 63);
+  uint64_t prevTop = 0;
+  if (kw > 0) {
+    const uint64_t BLL = kw > 1 ? bits[w - 2] : 0;
+    const uint64_t dl = BL | (BL << 1) | (BLL >> 63) | (BL >> 1) | (B << 63);
+    prevTop = dl >> 63;
+  }
+  if (g.bc[0] == PERIODIC && kw == lastWord) {
+    const uint64_t first = bits[w - kw];  // x = xmin - 1 wraps to xmax
+    if (first & 1ull) dil |= 1ull << (lastBit & 63);
+  }
+  const int32_t ca[3] = {0, yr + f.o[1], zr + f.o[2]};
+  for (int d = 1; d < D; ++d) {
+    for (int s = -1; s <= 1; s += 2) {
+      int32_t m = g.map(d, (int64_t)ca[d] + s);
+      int32_t rr = m - f.o[d];
+      if (rr < 0 || rr >= f.ext[d]) continue;
+      int32_t ny = (d == 1) ? rr : yr;
+      int32_t nz = (d == 2) ? rr : zr;
+      int64_t nrow = ((int64_t)nz << f.ly) + ny;
+      int64_t nwi = (nrow << wshift) + kw;
+      dil |= bits[nwi];
+      if (kw > 0) prevTop |= bits[nwi - 1] >> 63;
+    }
+  }
+  uint64_t C = dil | (dil << 1) | prevTop;
+  if (g.bounded(0) && kw == 0) C |= 1ull;
+  if (kw == lastWord) {
+    const int nb = (lastBit & 63) + 1;
+    if (nb < 64) C &= (~0ull >> (64 - nb));
+  }
+  return C;
+}
+
+// Keep flag of candidate i: first entry, defined, or start of a run whose
+// value differs from the previous candidate's.
+LSG_HD inline uint32_t keepFlag(const double *out, uint32_t i) {
+  double v = out[i];
+  if (i == 0) return 1;
+  if (!isUndef(v)) return 1;
+  return v != out[i - 1] ? 1u : 0u;
+}
+
+// Bitmap word and prefix count of the compacted output for word w.
+LSG_HD inline void buildIndexWord(const uint32_t *cpre, const uint32_t *ks, const uint32_t *cpos, int64_t w, int64_t nw,
+                                  int base, uint32_t ncand, uint64_t &word, uint32_t &pre) {
+  uint32_t c0 = (uint32_t)base + cpre[w];
+  if (c0 > ncand) c0 = ncand;
+  uint32_t k0 = ks[c0];
+  pre = k0 - (uint32_t)base;
+  word = 0;
+  if (w == nw) return;
+  uint32_t c1 = (uint32_t)base + cpre[w + 1];
+  if (c1 > ncand) c1 = ncand;
+  uint32_t kprev = k0;
+  for (uint32_t c = c0; c < c1; ++c) {
+    uint32_t kn = ks[c + 1];
+    if (kn != kprev) word |= 1ull << (cpos[c] & 63);
+    kprev = kn;
+  }
+}
+
+// ------------------------------------------------------------- schemes ----
+
+struct Vel {
+  int kind, dir, k;
+  double b, aPos, aNeg;
+  double vec[3];
+  // built-in VelocityField::getScalarVelocity (lsVelocityField.hpp:17-21)
+  LSG_HD double scalar(const double n[3]) const {
+    if (kind == 0) return b;
+    double c = n[dir];
+    double a = (c > 0) ? aPos : aNeg;
+    double ac = fabs(c);
+    double m = 1.0;
+    if (k >= 1) {
+      m = ac;
+      for (int j = 1; j < k; ++j) m = m * ac;
+    }
+    return b + a * m;
+  }
+};
+
+LSG_HD inline Vel velOf(const RateParams &rp) {
+  Vel v;
+  v.kind = rp.velKind;
+  v.dir = rp.velDir;
+  v.k = rp.velK;
+  v.b = rp.velB;
+  v.aPos = rp.velAPos;
+  v.aNeg = rp.velANeg;
+  v.vec[0] = rp.vec[0];
+  v.vec[1] = rp.vec[1];
+  v.vec[2] = rp.vec[2];
+  return v;
+}
+
+// One-sided differences with the second-order ENO correction
+// (lsEngquistOsher.hpp:59-111; identical in lsLaxFriedrichs.hpp:66-119,
+// lsLocalLaxFriedrichs.hpp:85-143, lsLocalLocalLaxFriedrichs.hpp:62-115).
+LSG_HD inline void upwindDiffs(const BandView &b, const int32_t c[3], int i, int order, double phi0, double &diffPos,
+                               double &diffNeg) {
+  const double gridDelta = b.g.dx;
+  const double deltaPos = gridDelta;
+  const double deltaNeg = -gridDelta;
+  const double phiPos = b.at(c, i, 1);
+  const double phiNeg = b.at(c, i, -1);
+  diffPos = (phiPos - phi0) / deltaPos;
+  diffNeg = (phiNeg - phi0) / deltaNeg;
+  if (order == 2) {
+    const double deltaPosPos = 2 * gridDelta;
+    const double deltaNegNeg = -2 * gridDelta;
+    const double phiPosPos = b.at(c, i, 2);
+    const double phiNegNeg = b.at(c, i, -2);
+    const double diff00 = (((deltaNeg * phiPos - deltaPos * phiNeg) / (deltaPos - deltaNeg) + phi0)) / (deltaPos * deltaNeg);
+    const double diffNegNeg =
+        (((deltaNeg * phiNegNeg - deltaNegNeg * phiNeg) / (deltaNegNeg - deltaNeg) + phi0)) / (deltaNegNeg * deltaNeg);
+    const double diffPosPos =
+        (((deltaPos * phiPosPos - deltaPosPos * phiPos) / (deltaPosPos - deltaPos) + phi0)) / (deltaPosPos * deltaPos);
+    if (signbit(diff00) == signbit(diffPosPos)) {
+      if (fabs(diffPosPos * deltaPos) < fabs(diff00 * deltaNeg)) diffPos -= deltaPos * diffPosPos;
+      else diffPos += deltaNeg * diff00;
+    }
+    if (signbit(diff00) == signbit(diffNegNeg)) {
+      if (fabs(diffNegNeg * deltaNeg) < fabs(diff00 * deltaPos)) diffNeg -= deltaNeg * diffNegNeg;
+      else diffNeg += deltaPos * diff00;
+    }
+  }
+}
+
+// FiniteDifferences::weno3 (lsFiniteDifferences.hpp:58-83)
+LSG_HD inline double weno3(const double *x, double delta, bool plus) {
+  const double eps = 1e-6;
+  double d0, d1, d2;
+  if (plus) {
+    d0 = x[2] - x[1];
+    d1 = x[3] - x[2];
+    d2 = x[4] - x[3];
+    double N = eps + pow2(d2 - d1);
+    double Dn = eps + pow2(d1 - d0);
+    double D2 = Dn * Dn;
+    double wp = D2 / (D2 + 2 * N * N);
+    return (d0 + d1 - wp * (d2 - 2 * d1 + d0)) / (2 * delta);
+  }
+  d0 = x[1] - x[0];
+  d1 = x[2] - x[1];
+  d2 = x[3] - x[2];
+  double N = eps + pow2(d1 - d0);
+  double Dn = eps + pow2(d2 - d1);
+  double D2 = Dn * Dn;
+  double wp = D2 / (D2 + 2 * N * N);
+  return (d1 + d2 - wp * (d0 - 2 * d1 + d2)) / (2 * delta);
+}
+
+// FiniteDifferences::weno5 (lsFiniteDifferences.hpp:88-123)
+LSG_HD inline double weno5(const double *x, double dx, bool plus) {
+  const double eps = 1e-6;
+  double d[5];
+  if (!plus) {
+    for (int i = 0; i < 5; ++i) d[i] = x[i + 1] - x[i];
+  } else {
+    for (int i = 0; i < 5; ++i) d[i] = x[6 - i] - x[5 - i];
+  }
+  double s1 = (13.0 / 12.0) * pow2(d[0] - 2 * d[1] + d[2]) + (1.0 / 4.0) * pow2(d[0] - 4 * d[1] + 3 * d[2]);
+  double s2 = (13.0 / 12.0) * pow2(d[1] - 2 * d[2] + d[3]) + (1.0 / 4.0) * pow2(d[1] - d[3]);
+  double s3 = (13.0 / 12.0) * pow2(d[2] - 2 * d[3] + d[4]) + (1.0 / 4.0) * pow2(3 * d[2] - 4 * d[3] + d[4]);
+  double eps_scaled = eps * dx * dx;
+  double al1 = 0.1 / (eps_scaled + s1);
+  double al2 = 0.6 / (eps_scaled + s2);
+  double al3 = 0.3 / (eps_scaled + s3);
+  double alsum = al1 + al2 + al3;
+  double p1 = (1.0 / 6.0) * (2 * d[0] - 7 * d[1] + 11 * d[2]);
+  double p2 = (1.0 / 6.0) * (-d[1] + 5 * d[2] + 2 * d[3]);
+  double p3 = (1.0 / 6.0) * (2 * d[2] + 5 * d[3] - d[4]);
+  return (al1 * p1 + al2 * p2 + al3 * p3) / (alsum * dx);
+}
+
+struct RateOut {
+  double H, Dn;
+};
+
+// EngquistOsher::operator() (lsEngquistOsher.hpp:42-162)
+template <int D>
+LSG_HD inline RateOut schemeEO(const BandView &b, const int32_t c[3], double phi0, int order, bool calcNormals,
+                               const Vel &vel) {
+  double gradPos[D], gradNeg[D];
+  double gradPosTotal = 0, gradNegTotal = 0;
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    gradPosTotal += pow2(dmaxs(diffNeg, 0.0)) + pow2(dmins(diffPos, 0.0));
+    gradNegTotal += pow2(dmins(diffNeg, 0.0)) + pow2(dmaxs(diffPos, 0.0));
+  }
+  double vel_grad = 0.;
+  double n[3] = {0, 0, 0};
+  if (calcNormals) {  // no zero guard (lsEngquistOsher.hpp:124-138)
+    double denominator = 0;
+    for (int i = 0; i < D; i++) {
+      double pos = b.at(c, i, 1) - phi0;
+      double neg = phi0 - b.at(c, i, -1);
+      n[i] = (pos + neg) * 0.5;
+      denominator += n[i] * n[i];
+    }
+    denominator = 1. / sqrt(denominator);
+    for (int i = 0; i < D; ++i) n[i] *= denominator;
+  }
+  double sv = vel.scalar(n);
+  if (sv > 0) vel_grad += sqrt(gradPosTotal) * sv;
+  else vel_grad += sqrt(gradNegTotal) * sv;
+  for (int w = 0; w < D; w++) {
+    if (vel.vec[w] > 0.) vel_grad += vel.vec[w] * gradPos[w];
+    else vel_grad += vel.vec[w] * gradNeg[w];
+  }
+  return {vel_grad, 0.};
+}
+
+// WENO::operator() (lsWENO.hpp:53-161)
+template <int D, int ORDER>
+LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double phi0, bool calcNormals, const Vel &vel) {
+  constexpr int r = (ORDER + 1) / 2;
+  double gradPosTotal = 0, gradNegTotal = 0;
+  double gm[D], gp[D];
+  double st[2 * r + 1];
+  for (int i = 0; i < D; i++) {
+    st[r] = phi0;
+    for (int k = 1; k <= r; ++k) {
+      st[r + k] = b.at(c, i, k);
+      st[r - k] = b.at(c, i, -k);
+    }
+    if (ORDER == 3) {
+      gm[i] = weno3(st, b.g.dx, false);
+      gp[i] = weno3(st, b.g.dx, true);
+    } else {
+      gm[i] = weno5(st, b.g.dx, false);
+      gp[i] = weno5(st, b.g.dx, true);
+    }
+    gradPosTotal += pow2(dmaxs(gm[i], 0.0)) + pow2(dmins(gp[i], 0.0));
+    gradNegTotal += pow2(dmins(gm[i], 0.0)) + pow2(dmaxs(gp[i], 0.0));
+  }
+  double vel_grad = 0.;
+  double n[3] = {0, 0, 0};
+  if (calcNormals) {  // central difference with zero guard (lsWENO.hpp:113-129)
+    double denominator = 0;
+    for (int i = 0; i < D; i++) {
+      n[i] = b.at(c, i, 1) - b.at(c, i, -1);
+      denominator += n[i] * n[i];
+    }
+    if (denominator > 0) {
+      denominator = 1. / sqrt(denominator);
+      for (int i = 0; i < D; ++i) n[i] *= denominator;
+    }
+  }
+  double sv = vel.scalar(n);
+  if (sv > 0) vel_grad += sqrt(gradPosTotal) * sv;
+  else vel_grad += sqrt(gradNegTotal) * sv;
+  for (int w = 0; w < D; w++) {
+    if (vel.vec[w] > 0.) vel_grad += vel.vec[w] * gm[w];
+    else vel_grad += vel.vec[w] * gp[w];
+  }
+  return {vel_grad, 0.};
+}
+
+// LaxFriedrichs::operator() (lsLaxFriedrichs.hpp:46-158)
+template <int D>
+LSG_HD inline RateOut schemeLF(const BandView &b, const int32_t c[3], double phi0, int order, bool calcNormals,
+                               const Vel &vel, double alphaFactor, const double *alphas) {
+  double gradPos[D], gradNeg[D];
+  double grad = 0., dissipation = 0.;
+  double n[3] = {0, 0, 0};
+  double normalModulus = 0;
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    if (calcNormals) {
+      n[i] = (diffNeg + diffPos) * 0.5;
+      normalModulus += n[i] * n[i];
+    }
+    grad += pow2((diffNeg + diffPos) * 0.5);
+    dissipation += alphaFactor * alphas[i] * (diffPos - diffNeg) * 0.5;
+  }
+  if (calcNormals) {
+    normalModulus = 1. / sqrt(normalModulus);
+    for (int i = 0; i < D; ++i) n[i] *= normalModulus;
+  }
+  double sv = vel.scalar(n);
+  double totalGrad = 0.;
+  if (sv != 0.) totalGrad = sv * sqrt(grad);
+  for (int w = 0; w < D; w++) {
+    if (vel.vec[w] > 0.) totalGrad += vel.vec[w] * gradPos[w];
+    else totalGrad += vel.vec[w] * gradNeg[w];
+  }
+  return {totalGrad, (totalGrad != 0.) ? dissipation : 0};
+}
+
+// LocalLocalLaxFriedrichs::operator() (lsLocalLocalLaxFriedrichs.hpp:42-165)
+template <int D>
+LSG_HD inline RateOut schemeLLLF(const BandView &b, const int32_t c[3], double phi0, int order, const Vel &vel,
+                                 double alphaFactor, double *alphaAcc) {
+  double gradPos[D], gradNeg[D];
+  double grad = 0., dissipation = 0.;
+  double n[3] = {0, 0, 0};
+  double normalModulus = 0;
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    n[i] = (diffNeg + diffPos) * 0.5;
+    normalModulus += n[i] * n[i];
+    grad += pow2((diffNeg + diffPos) * 0.5);
+  }
+  normalModulus = 1. / sqrt(normalModulus);
+  for (int i = 0; i < D; ++i) n[i] *= normalModulus;
+  double sv = vel.scalar(n);
+  double totalGrad = 0.;
+  if (sv != 0.) totalGrad = sv * sqrt(grad);
+  for (int w = 0; w < D; w++) {
+    if (vel.vec[w] > 0.) totalGrad += vel.vec[w] * gradPos[w];
+    else totalGrad += vel.vec[w] * gradNeg[w];
+  }
+  if (totalGrad == 0.) return {0., 0.};
+  for (int i = 0; i < D; ++i) {
+    double alpha = fabs((sv + vel.vec[i]) * n[i]);
+    alphaAcc[i] = dmaxs(alphaAcc[i], alpha);
+    dissipation += alphaFactor * alpha * (gradNeg[i] - gradPos[i]) * 0.5;
+  }
+  return {totalGrad, dissipation};
+}
+
+// LocalLaxFriedrichs::operator() (lsLocalLaxFriedrichs.hpp:65-241)
+template <int D>
+LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double phi0, int order, const Vel &vel,
+                                double alphaFactor, double *alphaAcc) {
+  double gradPos[D], gradNeg[D];
+  double grad = 0., dissipation = 0.;
+  double n[3] = {0, 0, 0};
+  double normalModulus = 0;
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    n[i] = (diffNeg + diffPos) * 0.5;
+    normalModulus += n[i] * n[i];
+    grad += pow2((diffNeg + diffPos) * 0.5);
+  }
+  normalModulus = 1. / sqrt(normalModulus);
+  for (int i = 0; i < D; ++i) n[i] *= normalModulus;
+  double sv = vel.scalar(n);
+  double totalGrad = 0.;
+  if (sv != 0.) totalGrad = sv * sqrt(grad);
+  for (int w = 0; w < D; w++) {
+    if (vel.vec[w] > 0.) totalGrad += vel.vec[w] * gradPos[w];
+    else totalGrad += vel.vec[w] * gradNeg[w];
+  }
+  if (totalGrad == 0.) return {0., 0.};
+  double alpha[D];
+  for (int i = 0; i < D; ++i) alpha[i] = 0;
+  int ni[3] = {-1, -1, -1};
+  constexpr int numNeighbors = (D == 3) ? 27 : 9;
+  for (int q = 0; q < numNeighbors; ++q) {  // box loop (lsLocalLaxFriedrichs.hpp:193-231)
+    int32_t nc[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) nc[d] = b.g.map(d, (int64_t)c[d] + ni[d]);
+    double normal[3] = {0, 0, 0};
+    double nm = 0.;
+    const double center = b.value(nc[0], nc[1], nc[2]);
+    for (int dir = 0; dir < D; ++dir) {
+      int32_t pp[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) {
+        pp[d] = b.g.map(d, (int64_t)c[d] + ni[d] + (d == dir ? 1 : 0));
+        pn[d] = b.g.map(d, (int64_t)c[d] + ni[d] - (d == dir ? 1 : 0));
+      }
+      const double neg = b.value(pn[0], pn[1], pn[2]);
+      const double pos = b.value(pp[0], pp[1], pp[2]);
+      const double dP = (pos - center) / b.g.dx;
+      const double dN = (center - neg) / b.g.dx;
+      normal[dir] = (dP + dN) * 0.5;
+      nm += normal[dir] * normal[dir];
+    }
+    nm = 1. / sqrt(nm);
+    for (int dir = 0; dir < D; ++dir) normal[dir] *= nm;
+    const double scaVel = vel.scalar(normal);
+    for (int dir = 0; dir < D; ++dir) {
+      double tempAlpha = fabs((scaVel + vel.vec[dir]) * normal[dir]);
+      alpha[dir] = dmaxs(alpha[dir], tempAlpha);
+      alphaAcc[dir] = dmaxs(alphaAcc[dir], tempAlpha);
+    }
+    int dim = 0;
+    for (; dim < D - 1; ++dim) {
+      if (ni[dim] < 1) break;
+      ni[dim] = -1;
+    }
+    ++ni[dim];
+  }
+  for (int i = 0; i < D; ++i) dissipation += alphaFactor * alpha[i] * (gradNeg[i] - gradPos[i]) * 0.5;
+  return {totalGrad, dissipation};
+}
+
+// Scheme ids of the rate kernel.
+enum SchemeId : int { S_EO = 0, S_WENO5 = 1, S_LF = 2, S_LLLF = 3, S_LLF = 4, S_WENO3 = 5 };
+
+LSG_HD inline int schemeIdOf(int spatial) {
+  switch (spatial) {
+  case 0: case 1: return S_EO;
+  case 2: case 3: return S_LF;
+  case 5: case 6: return S_LLLF;
+  case 7: case 8: return S_LLF;
+  case 10: return S_WENO3;
+  default: return S_WENO5;
+  }
+}
+
+// Hamiltonian and dissipation of one active point; returns the stage
+// velocity after updateLevelSet's checkDissipation rule (lsAdvect.hpp:738-744,
+// precedence kept) and the raw H - D through `velocity`.
+template <int D, int SCHEME>
+LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi0, const RateParams &rp, const Vel &vel,
+                               const double *globalAlphas, double *alphaAcc, double &velocity) {
+  RateOut r;
+  if (SCHEME == S_EO) r = schemeEO<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel);
+  else if (SCHEME == S_WENO5) r = schemeWENO<D, 5>(b, c, phi0, rp.calcNormals != 0, vel);
+  else if (SCHEME == S_WENO3) r = schemeWENO<D, 3>(b, c, phi0, rp.calcNormals != 0, vel);
+  else if (SCHEME == S_LF) r = schemeLF<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel, rp.alphaFactor, globalAlphas);
+  else if (SCHEME == S_LLLF) r = schemeLLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
+  else r = schemeLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
+  velocity = r.H - r.Dn;
+  double v = velocity;
+  if ((rp.checkDissipation && (r.H < 0 && v > 0)) || (r.H > 0 && v < 0)) v = 0;
+  return v;
+}
+
+// Per-point alphas of Advect::findGlobalAlphas (lsAdvect.hpp:150-178).
+template <int D> LSG_HD inline void globalAlphaPoint(const BandView &b, const int32_t c[3], const Vel &vel, double *la) {
+  double normal[3] = {0, 0, 0};
+  double nm = 0.;
+  for (int k = 0; k < D; ++k) {
+    normal[k] = b.at(c, k, 1) - b.at(c, k, -1);
+    nm += normal[k] * normal[k];
+  }
+  nm = 1. / sqrt(nm);
+  for (int k = 0; k < D; ++k) normal[k] *= nm;
+  const double sv = vel.scalar(normal);
+  for (int k = 0; k < D; ++k) {
+    double t = fabs((sv + vel.vec[k]) * normal[k]);
+    la[k] = dmaxs(la[k], t);
+  }
+}
+
+// Time step of a rate pass from max |H - D| (min over points of cfl/|v|
+// equals cfl / max|v| since correctly rounded division is monotone) and the
+// reduceTimeStepHamiltonJacobi bound of the LF family (lsAdvect.hpp:530-606,
+// lsLaxFriedrichs.hpp:160-172).
+LSG_HD inline double stageTimeStep(double maxTimeStep, double cfl, double maxAbs, int D, const double *alphas,
+                                   bool useAlphas, double dx) {
+  double tempMax = maxTimeStep;
+  if (maxAbs > 0.) {
+    double cand = cfl / maxAbs;
+    if (cand < tempMax) tempMax = cand;
+  }
+  if (useAlphas) {
+    double timeStep = 0;
+    for (int i = 0; i < D; ++i) timeStep += alphas[i] / dx;
+    timeStep = 1.0 / timeStep;
+    tempMax = (tempMax < timeStep) ? tempMax : timeStep;
+  }
+  double res = maxTimeStep;
+  if (tempMax < res) res = tempMax;
+  return res;
+}
+
+}  // namespace lsg

@@ -1,0 +1,79 @@
+// Host-visible launch interface of the HIP kernels (lsg_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lsg_device.hpp"
+
+namespace lsg {
+
+constexpr int SCAN_BLOCKS = 512;
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 4;
+
+// Device scalars shared by the kernels of one advection handle.
+struct DevScalars {
+  unsigned long long maxAbsBits;  // max |H - D| over active points (bits of a non-negative double)
+  unsigned long long alphaBits[3];  // LLF/LLLF per-direction alpha maxima; LF global alphas
+  unsigned long long activeCount;   // active points of the last rate pass
+  unsigned long long totalActive;   // accumulated active points (point updates)
+  uint32_t overflow;
+  uint32_t pad;
+  double dt[4];       // stage time steps (dt1, dt2, dt3) and a scratch slot
+  double stepTime;    // time advanced by the last step
+  double maxTimeStep; // bound passed to the first stage
+  double rateBytes;   // accumulated algorithmic bytes of the rate kernel
+};
+
+// Mutable device buffers of a band.
+struct BandBufs {
+  uint64_t *bits;
+  uint32_t *pre;
+  double *val;
+  uint32_t *pos;
+  uint32_t *n;  // device entry count
+  size_t cap;
+};
+
+struct Workspace {
+  uint64_t *cbits;     // nwords + 1
+  uint32_t *cnt;       // nwords + 1
+  uint32_t *cpre;      // nwords + 1
+  uint32_t *cpos;      // ccap
+  double *out;         // ccap
+  uint32_t *ks;        // ccap + 1
+  uint32_t *partials;  // SCAN_BLOCKS + 1
+  uint32_t *ncand;     // device candidate count
+  size_t ccap;
+};
+
+BandView viewOf(const BandBufs &b, const Frame &f, const GridInfo &g);
+
+// Band operations (all asynchronous on `s`; sizes live in device memory).
+void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
+                  Workspace &ws, const OpParams &op, DevScalars *sc);
+void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
+                  Workspace &ws, const OpParams &op, DevScalars *sc);
+void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
+                   const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc);
+// Rate pass: Hamiltonian + dissipation per active entry, CFL reduction.
+void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
+                 double *rates, DevScalars *sc, const double *maxTimeStep, double *dtOut);
+// LF global alphas (Advect::findGlobalAlphas).
+void launchGlobalAlphas(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g,
+                        const RateParams &rp, DevScalars *sc);
+// Small device-scalar kernels.
+void launchStepTime(hipStream_t s, DevScalars *sc, int temporal);
+void launchCopyBand(hipStream_t s, const BandBufs &src, BandBufs &dst, const Frame &f);
+// Count defined entries (to *outCount) and the bounding box of in-frame
+// entries (frame-relative, to bbox[6]: min x,y,z then max x,y,z).
+void launchBandStats(hipStream_t s, const BandBufs &b, const Frame &f, uint32_t *outCount, int32_t *bbox);
+void launchFill32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
+
+}  // namespace lsg
+
+namespace lsg {
+void launchWeno(hipStream_t s, int order, uint32_t n, const double *x, double dx, int plus, double *out);
+}
