@@ -203,7 +203,7 @@ def advect(dom, params):
 
 
 def compute_rates(dom, params, max_dt=np.finfo(np.float64).max):
-    cap = lib().lso_domain_size(dom.h) * 4 + 64
+    cap = lib().lso_domain_size(dom.h) * 32 + 4096
     buf = np.zeros(cap, dtype=np.float64)
     dt = lib().lso_compute_rates(dom.h, ctypes.byref(params), float(max_dt),
                                  buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), cap)

@@ -1,0 +1,197 @@
+"""GPU parity: the HIP path against the CPU oracle on identical inputs.
+
+Integer/index work (band membership, run structure) must be bit-exact; the
+FP64 level-set values are also expected to be bit-exact (same operation
+order, no FMA contraction).  The tolerance the path promises for phi is
+1e-10 grid units; every comparison here asserts bitwise equality, which is
+stronger.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TOL_PHI = 1e-10  # grid units; bitwise equality is asserted
+
+
+def _vls():
+    import viennals_amd as vls
+    return vls
+
+
+def to_gpu(o):
+    vls = _vls()
+    g = vls.Domain.from_index_bounds(o.D, o.gmin, o.gmax, o.bc, o.delta)
+    idx, val = o.get()
+    g.set_runs(idx, val, o.width)
+    return g
+
+
+def assert_same(o, g, tag):
+    io, vo = o.get()
+    ig, vg = g.get_runs()
+    if not cases.bit_equal(io, vo, ig, vg):
+        i, a, b, c, d = cases.first_diff(io, vo, ig, vg)
+        # report the phi deviation on the common defined points for diagnosis
+        pytest.fail(f"{tag}: run lists differ (oracle {len(vo)} vs gpu {len(vg)}), first diff at {i}: "
+                    f"oracle {a} {b} gpu {c} {d}")
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def test_roundtrip_sphere():
+    say("roundtrip sphere r=10 dx=0.25")
+    o = cases.sphere()
+    g = to_gpu(o)
+    assert_same(o, g, "roundtrip")
+
+
+@pytest.mark.parametrize("op", ["expand", "reduce", "rebuild", "prune_rsz", "prune"])
+def test_band_ops_sphere(op):
+    say(f"band op {op} on sphere")
+    o = cases.sphere(r=6.0, dx=0.25)
+    g = to_gpu(o)
+    vls = _vls()
+    for w in (3, 5, 7):
+        o.expand(w)
+        vls.Expand(g, w).apply()
+        assert_same(o, g, f"expand {w}")
+    if op == "reduce":
+        o.reduce(1)
+        vls.Reduce(g, 1).apply()
+    elif op == "rebuild":
+        o.reduce(1)
+        vls.Reduce(g, 1).apply()
+        o.rebuild()
+        vls._lib.check(vls._lib.lib().lsg_rebuild(g._h))
+    elif op.startswith("prune"):
+        rsz = op == "prune_rsz"
+        o.prune(rsz)
+        p = vls.Prune(g)
+        p.setRemoveStrayZeros(rsz)
+        p.apply()
+    assert_same(o, g, op)
+
+
+def test_boolean_trench():
+    say("boolean relative complement: plane minus box (reflective x/y)")
+    vls = _vls()
+    L = 32
+    gmin, gmax, bc = [-L, -L, -L], [L, L, L], [0, 0, 1]
+    op_ = po.Domain(3, gmin, gmax, bc, 1.0)
+    ip, vp = cases.plane_points(3, gmin, gmax, bc, 1.0, 0.0)
+    op_.from_points(ip, vp, 2)
+    ob = po.Domain(3, gmin, gmax, bc, 1.0)
+    ib, vb = cases.box_points(3, [-8, -8, -16], [8, 8, 1], 1.0, gmin, gmax)
+    ob.from_points(ib, vb, 2)
+    gp, gb = to_gpu(op_), to_gpu(ob)
+    op_.boolean(ob, 2)
+    vls.BooleanOperation(gp, gb, vls.BooleanOperationEnum.RELATIVE_COMPLEMENT).apply()
+    assert_same(op_, gp, "relcomp")
+
+
+def _advect_gpu(g, p):
+    vls = _vls()
+    a = vls.Advect(g, vls.VelocityField(kind=p.velKind, dir=p.velDir, k=p.velK, b=p.velB, a_pos=p.velAPos,
+                                        a_neg=p.velANeg, vec=tuple(p.vec)))
+    a.setSpatialScheme(p.spatial)
+    a.setTemporalScheme(p.temporal)
+    a.setTimeStepRatio(p.cfl)
+    a.setAdvectionTime(p.advectionTime)
+    a.setSingleStep(bool(p.singleStep))
+    a.apply()
+    return a
+
+
+ADVECT_CASES = [
+    # (geometry, spatial, temporal, cfl, time, velocity)
+    ("sphere", 0, 0, 0.4, 0.3, cases.VEL_EGG),
+    ("sphere", 1, 1, 0.4, 0.3, cases.VEL_EGG),
+    ("sphere", 2, 0, 0.4, 0.3, cases.VEL_EGG),
+    ("sphere", 3, 2, 0.4, 0.3, cases.VEL_EGG),
+    ("sphere", 5, 0, 0.4, 0.3, cases.VEL_EGG),
+    ("sphere", 6, 1, 0.4, 0.3, cases.VEL_EGG),
+    ("sphere", 7, 0, 0.4, 0.3, cases.VEL_EGG),
+    ("sphere", 8, 2, 0.4, 0.3, cases.VEL_EGG),
+    ("sphere", 10, 2, 0.4, 0.3, cases.VEL_TRANSLATE_X),
+    ("sphere", 11, 2, 0.4, 0.3, cases.VEL_TRANSLATE_X),
+    ("trench", 11, 2, 0.4, 3.0, cases.VEL_ETCH_DIRECTIONAL),
+    ("trench", 0, 0, 0.4, 3.0, cases.VEL_ETCH_DIRECTIONAL),
+    ("periodic", 8, 2, 0.45, 2.0, cases.VEL_NORMAL_SQ),
+    ("periodic", 11, 2, 0.45, 2.0, cases.VEL_NORMAL_SQ),
+    ("sphere2d", 11, 2, 0.4, 1.0, dict(kind=0, b=1.0, vec=(0.5, 0.0, 0.0))),
+    ("sphere2d", 8, 1, 0.4, 1.0, cases.VEL_GROW),
+]
+
+
+def _geom(name):
+    if name == "sphere":
+        return cases.sphere(r=6.0, dx=0.25, origin=[0.3, 0.1, -0.2])
+    if name == "trench":
+        return cases.trench()
+    if name == "periodic":
+        return cases.wavy_periodic()
+    return cases.sphere2d()
+
+
+@pytest.mark.parametrize("geom,spatial,temporal,cfl,time,vel", ADVECT_CASES)
+def test_advect_parity(geom, spatial, temporal, cfl, time, vel):
+    say(f"advect {geom} spatial={spatial} temporal={temporal} cfl={cfl} T={time}")
+    o = _geom(geom)
+    g = to_gpu(o)
+    p = po.make_params(spatial=spatial, temporal=temporal, cfl=cfl, time=time, vel=vel)
+    st, r = po.advect(o, p)
+    assert st == 0
+    a = _advect_gpu(g, p)
+    assert a.getNumberOfTimeSteps() == r["steps"]
+    assert a.getAdvectedTime() == r["advected_time"]
+    assert a.getPointUpdates() == r["point_updates"]
+    assert_same(o, g, f"advect {geom} s{spatial} t{temporal}")
+    assert o.check()[0] == 0
+
+
+def test_config1_sphere_eo1_fe():
+    """Sphere r=10, dx=0.25, infinite, EO1 + FE, CFL 0.4, T=2.0, v=+1."""
+    say("config 1: sphere r=10 dx=0.25 EO1/FE CFL0.4 T=2")
+    o = cases.sphere(r=10.0, dx=0.25)
+    g = to_gpu(o)
+    p = po.make_params(spatial=0, temporal=0, cfl=0.4, time=2.0, vel=cases.VEL_GROW)
+    st, r = po.advect(o, p)
+    a = _advect_gpu(g, p)
+    assert a.getNumberOfTimeSteps() == r["steps"]
+    assert_same(o, g, "config1")
+
+
+def test_weno_kernels_match_oracle():
+    say("weno3/weno5 device evaluation")
+    vls = _vls()
+    rng = np.random.default_rng(5)
+    for order, w in ((3, 5), (5, 7)):
+        x = rng.normal(size=(4096, w)) * rng.choice([1e-3, 1, 10], size=(4096, 1))
+        for plus in (False, True):
+            for dx in (0.25, 1.0):
+                got = vls.weno(order, x, dx, plus)
+                ref = np.array([(po.weno5 if order == 5 else po.weno3)(row, dx, plus) for row in x])
+                assert np.array_equal(got.view(np.int64), ref.view(np.int64))
+
+
+def test_errors_are_reported():
+    say("error paths")
+    vls = _vls()
+    o = cases.sphere(r=3.0, dx=0.5)
+    g = to_gpu(o)
+    a = vls.Advect(g)
+    with pytest.raises(vls.LsgError, match="No velocity field"):
+        a.apply()
+    a.setVelocityField(vls.VelocityField.constant(1.0))
+    a.setSpatialScheme(9)
+    with pytest.raises(vls.LsgError, match="not supported"):
+        a.apply()

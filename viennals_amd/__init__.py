@@ -1,0 +1,394 @@
+"""MI355X-native level-set advection with the ViennaLS Advect API.
+
+Python host layer over the C ABI of libviennals_gpu.so, mirroring the
+reference's Python bindings (viennals.d2 / viennals.d3: Domain, MakeGeometry,
+Sphere, Plane, Box, Expand, Reduce, Prune, BooleanOperation, Advect;
+python/pyWrap.hpp:324-401).  All level-set data lives in GPU memory.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import LsgError, check, lib
+
+__all__ = [
+    "BoundaryConditionEnum", "SpatialSchemeEnum", "TemporalSchemeEnum", "BooleanOperationEnum",
+    "Domain", "Sphere", "Plane", "Box", "MakeGeometry", "Expand", "Reduce", "Prune", "BooleanOperation",
+    "VelocityField", "Advect", "LsgError", "weno",
+]
+
+POS_VALUE = float(np.finfo(np.float64).max)
+NEG_VALUE = -POS_VALUE
+
+
+class BoundaryConditionEnum:
+    """viennahrle::BoundaryType (python/viennals/_core.pyi:26-30)."""
+    REFLECTIVE_BOUNDARY = 0
+    INFINITE_BOUNDARY = 1
+    PERIODIC_BOUNDARY = 2
+    POS_INFINITE_BOUNDARY = 3
+    NEG_INFINITE_BOUNDARY = 4
+
+
+class SpatialSchemeEnum:
+    """lsAdvectIntegrationSchemes.hpp:10-23."""
+    ENGQUIST_OSHER_1ST_ORDER = 0
+    ENGQUIST_OSHER_2ND_ORDER = 1
+    LAX_FRIEDRICHS_1ST_ORDER = 2
+    LAX_FRIEDRICHS_2ND_ORDER = 3
+    LOCAL_LAX_FRIEDRICHS_ANALYTICAL_1ST_ORDER = 4
+    LOCAL_LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 5
+    LOCAL_LOCAL_LAX_FRIEDRICHS_2ND_ORDER = 6
+    LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 7
+    LOCAL_LAX_FRIEDRICHS_2ND_ORDER = 8
+    STENCIL_LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 9
+    WENO_3RD_ORDER = 10
+    WENO_5TH_ORDER = 11
+
+
+class TemporalSchemeEnum:
+    """lsAdvectIntegrationSchemes.hpp:31-35."""
+    FORWARD_EULER = 0
+    RUNGE_KUTTA_2ND_ORDER = 1
+    RUNGE_KUTTA_3RD_ORDER = 2
+
+
+class BooleanOperationEnum:
+    """lsBooleanOperation.hpp:27-33 (INVERT and CUSTOM are internal here)."""
+    INTERSECT = 0
+    UNION = 1
+    RELATIVE_COMPLEMENT = 2
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Domain:
+    """Device-resident level set (Domain<T,D>, lsDomain.hpp:27-269).
+
+    Domain(bounds, boundaryConditions, gridDelta) with bounds
+    [x0, x1, y0, y1(, z0, z1)] in physical units (lsDomain.hpp:68-78), or
+    Domain(gridDelta, dim=3) for an all-infinite domain (lsDomain.hpp:55-66).
+    """
+
+    def __init__(self, bounds=None, boundaryConditions=None, gridDelta=1.0, dim=None):
+        if bounds is None:
+            D = dim or 3
+            gmin = [0] * D
+            gmax = [0] * D
+            bc = [BoundaryConditionEnum.INFINITE_BOUNDARY] * D
+        else:
+            D = len(bounds) // 2
+            gmin = [int(np.floor(bounds[2 * i] / gridDelta)) for i in range(D)]
+            gmax = [int(np.ceil(bounds[2 * i + 1] / gridDelta)) for i in range(D)]
+            bc = list(boundaryConditions) if boundaryConditions is not None else \
+                [BoundaryConditionEnum.INFINITE_BOUNDARY] * D
+        self.D = D
+        self.gridDelta = float(gridDelta)
+        self.gmin, self.gmax, self.bc = gmin, gmax, [int(b) for b in bc]
+        a, pa = _i32(gmin)
+        b, pb = _i32(gmax)
+        c = (ctypes.c_int * D)(*self.bc)
+        h = ctypes.c_void_p()
+        check(lib().lsg_domain_create(D, pa, pb, c, self.gridDelta, ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def from_index_bounds(cls, D, gmin, gmax, bc, gridDelta):
+        """Domain from index bounds (floor(b0/dx), ceil(b1/dx)) directly."""
+        obj = cls.__new__(cls)
+        obj.D = D
+        obj.gridDelta = float(gridDelta)
+        obj.gmin, obj.gmax, obj.bc = list(gmin), list(gmax), [int(x) for x in bc]
+        a, pa = _i32(gmin)
+        b, pb = _i32(gmax)
+        c = (ctypes.c_int * D)(*obj.bc)
+        h = ctypes.c_void_p()
+        check(lib().lsg_domain_create(D, pa, pb, c, obj.gridDelta, ctypes.byref(h)))
+        obj._h = h
+        return obj
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().lsg_domain_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # Domain API
+    def deepCopy(self, other):
+        check(lib().lsg_domain_deep_copy(self._h, other._h))
+
+    def insertPoints(self, indices, values, width=2):
+        """Domain::insertPoints: defined points (index, value) (lsDomain.hpp:135-138)."""
+        a, pa = _i32(indices)
+        b, pb = _f64(values)
+        check(lib().lsg_domain_insert_points(self._h, len(b), pa, pb, int(width)))
+
+    def getNumberOfPoints(self):
+        n = ctypes.c_int64()
+        check(lib().lsg_domain_num_points(self._h, ctypes.byref(n)))
+        return n.value
+
+    def getLevelSetWidth(self):
+        w = ctypes.c_int()
+        check(lib().lsg_domain_width(self._h, ctypes.byref(w)))
+        return w.value
+
+    # run-start list exchange
+    def set_runs(self, indices, values, width):
+        a, pa = _i32(indices)
+        b, pb = _f64(values)
+        check(lib().lsg_domain_set_runs(self._h, len(b), pa, pb, int(width)))
+
+    def get_runs(self):
+        n = ctypes.c_int64()
+        check(lib().lsg_domain_num_runs(self._h, ctypes.byref(n)))
+        idx = np.zeros((n.value, self.D), dtype=np.int32)
+        val = np.zeros(n.value, dtype=np.float64)
+        check(lib().lsg_domain_get_runs(self._h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                        val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n.value))
+        return idx, val
+
+    def defined_points(self):
+        idx, val = self.get_runs()
+        m = np.abs(val) != POS_VALUE
+        return idx[m], val[m]
+
+
+class Sphere:
+    def __init__(self, origin, radius):
+        self.origin, self.radius = list(origin), float(radius)
+
+
+class Plane:
+    def __init__(self, origin, normal):
+        self.origin, self.normal = list(origin), list(normal)
+
+
+class Box:
+    def __init__(self, minPoint, maxPoint):
+        self.minPoint, self.maxPoint = list(minPoint), list(maxPoint)
+
+
+class MakeGeometry:
+    """MakeGeometry(domain, geometry).apply() (lsMakeGeometry.hpp)."""
+
+    def __init__(self, domain, geometry):
+        self.domain, self.geometry = domain, geometry
+
+    def apply(self):
+        g, d = self.geometry, self.domain
+        pad = lambda v: list(v) + [0.0] * (3 - len(v))
+        if isinstance(g, Sphere):
+            a, pa = _f64(pad(g.origin))
+            check(lib().lsg_make_sphere(d._h, pa, g.radius))
+        elif isinstance(g, Plane):
+            a, pa = _f64(pad(g.origin))
+            b, pb = _f64(pad(g.normal))
+            check(lib().lsg_make_plane(d._h, pa, pb))
+        elif isinstance(g, Box):
+            a, pa = _f64(pad(g.minPoint))
+            b, pb = _f64(pad(g.maxPoint))
+            check(lib().lsg_make_box(d._h, pa, pb))
+        else:
+            raise LsgError("Invalid geometry type was specified for MakeGeometry.")
+
+
+class Expand:
+    def __init__(self, domain, width=0):
+        self.domain, self.width = domain, width
+
+    def setWidth(self, w):
+        self.width = w
+
+    def apply(self):
+        check(lib().lsg_expand(self.domain._h, int(self.width)))
+
+
+class Reduce:
+    def __init__(self, domain, width=0, noNewSegment=False):
+        self.domain, self.width = domain, width
+
+    def setWidth(self, w):
+        self.width = w
+
+    def apply(self):
+        check(lib().lsg_reduce(self.domain._h, int(self.width)))
+
+
+class Prune:
+    def __init__(self, domain):
+        self.domain, self.rsz = domain, False
+
+    def setRemoveStrayZeros(self, v):
+        self.rsz = bool(v)
+
+    def apply(self):
+        check(lib().lsg_prune(self.domain._h, int(self.rsz)))
+
+
+class BooleanOperation:
+    def __init__(self, a, b, op=BooleanOperationEnum.INTERSECT):
+        self.a, self.b, self.op = a, b, op
+
+    def apply(self):
+        check(lib().lsg_boolean(self.a._h, self.b._h, int(self.op)))
+
+
+class VelocityField:
+    """Built-in GPU velocity functors replacing VelocityField<T>
+    (lsVelocityField.hpp:11-41):
+
+    scalar = b                                            (constant)
+           = b + (n[dir] > 0 ? a_pos : a_neg) * |n[dir]|^k  (normal power)
+    vector = vec (constant)
+    """
+
+    def __init__(self, kind=0, dir=2, k=1, b=0.0, a_pos=0.0, a_neg=0.0, vec=(0.0, 0.0, 0.0)):
+        self.kind, self.dir, self.k = int(kind), int(dir), int(k)
+        self.b, self.a_pos, self.a_neg = float(b), float(a_pos), float(a_neg)
+        self.vec = tuple(float(v) for v in vec) + (0.0,) * (3 - len(vec))
+
+    @classmethod
+    def constant(cls, scalar=0.0, vector=(0.0, 0.0, 0.0)):
+        return cls(kind=0, b=scalar, vec=vector)
+
+    @classmethod
+    def normal_power(cls, dir, k, b, a_pos, a_neg=0.0, vector=(0.0, 0.0, 0.0)):
+        return cls(kind=1, dir=dir, k=k, b=b, a_pos=a_pos, a_neg=a_neg, vec=vector)
+
+    def as_dict(self):
+        return dict(kind=self.kind, dir=self.dir, k=self.k, b=self.b, a_pos=self.a_pos, a_neg=self.a_neg,
+                    vec=self.vec)
+
+
+class Advect:
+    """Advect<T,D> (lsAdvect.hpp:54-1044) running on the GPU."""
+
+    def __init__(self, domain=None, velocities=None):
+        h = ctypes.c_void_p()
+        check(lib().lsg_advect_create(ctypes.byref(h)))
+        self._h = h
+        self._domains = []
+        if domain is not None:
+            self.insertNextLevelSet(domain)
+        if velocities is not None:
+            self.setVelocityField(velocities)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().lsg_advect_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def insertNextLevelSet(self, d):
+        self._domains.append(d)
+        check(lib().lsg_advect_insert_next_level_set(self._h, d._h))
+
+    def clearLevelSets(self):
+        self._domains = []
+        check(lib().lsg_advect_clear_level_sets(self._h))
+
+    def setVelocityField(self, v):
+        vec = (ctypes.c_double * 3)(*v.vec)
+        check(lib().lsg_advect_set_velocity_builtin(self._h, v.kind, v.dir, v.k, v.b, v.a_pos, v.a_neg, vec))
+
+    def setSpatialScheme(self, s):
+        check(lib().lsg_advect_set_spatial_scheme(self._h, int(s)))
+
+    def setIntegrationScheme(self, s):
+        self.setSpatialScheme(s)
+
+    def setTemporalScheme(self, s):
+        check(lib().lsg_advect_set_temporal_scheme(self._h, int(s)))
+
+    def setAdvectionTime(self, t):
+        check(lib().lsg_advect_set_advection_time(self._h, float(t)))
+
+    def setTimeStepRatio(self, c):
+        check(lib().lsg_advect_set_time_step_ratio(self._h, float(c)))
+
+    def setSingleStep(self, s):
+        check(lib().lsg_advect_set_single_step(self._h, int(bool(s))))
+
+    def setDissipationAlpha(self, a):
+        check(lib().lsg_advect_set_dissipation_alpha(self._h, float(a)))
+
+    def setCheckDissipation(self, c):
+        check(lib().lsg_advect_set_check_dissipation(self._h, int(bool(c))))
+
+    def setCalculateNormalVectors(self, c):
+        check(lib().lsg_advect_set_calculate_normal_vectors(self._h, int(bool(c))))
+
+    def setIgnoreVoids(self, v):
+        check(lib().lsg_advect_set_ignore_voids(self._h, int(bool(v))))
+
+    def setAdaptiveTimeStepping(self, on=True, subdivisions=20):
+        check(lib().lsg_advect_set_adaptive_time_stepping(self._h, int(bool(on)), int(subdivisions)))
+
+    def setUpdatePointData(self, u):
+        check(lib().lsg_advect_set_update_point_data(self._h, int(bool(u))))
+
+    def prepareLS(self):
+        check(lib().lsg_advect_prepare_ls(self._h))
+
+    def apply(self):
+        check(lib().lsg_advect_apply(self._h))
+
+    def getAdvectedTime(self):
+        t = ctypes.c_double()
+        check(lib().lsg_advect_get_advected_time(self._h, ctypes.byref(t)))
+        return t.value
+
+    def getCurrentTimeStep(self):
+        t = ctypes.c_double()
+        check(lib().lsg_advect_get_current_time_step(self._h, ctypes.byref(t)))
+        return t.value
+
+    def getNumberOfTimeSteps(self):
+        n = ctypes.c_uint()
+        check(lib().lsg_advect_get_number_of_time_steps(self._h, ctypes.byref(n)))
+        return n.value
+
+    # no counterpart in the reference
+    def getPointUpdates(self):
+        n = ctypes.c_double()
+        check(lib().lsg_advect_get_point_updates(self._h, ctypes.byref(n)))
+        return n.value
+
+    def setProfiling(self, on):
+        check(lib().lsg_advect_set_profiling(self._h, int(bool(on))))
+
+    def getRateKernelStats(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        b = ctypes.c_double()
+        check(lib().lsg_advect_get_rate_kernel_stats(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)))
+        return ms.value, n.value, b.value
+
+
+def weno(order, stencils, dx, plus):
+    """FiniteDifferences::weno3/weno5 evaluated on the GPU for rows of stencils."""
+    x = np.ascontiguousarray(stencils, dtype=np.float64)
+    n = x.shape[0]
+    out = np.zeros(n, dtype=np.float64)
+    check(lib().lsg_weno_eval(int(order), n, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(dx),
+                              int(bool(plus)), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def synchronize():
+    check(lib().lsg_device_synchronize())

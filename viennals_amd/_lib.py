@@ -1,0 +1,111 @@
+"""ctypes binding of libviennals_gpu.so (the C ABI in include/viennals_gpu.h).
+
+The library is built in-tree by viennals_amd/build.py.  Loading fails loudly
+when it is missing: there is no CPU fallback for the advection path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libviennals_gpu.so")
+_lib = None
+
+
+class LsgError(RuntimeError):
+    pass
+
+
+# every symbol declared in include/viennals_gpu.h
+EXPORTED = [
+    "lsg_last_error", "lsg_domain_create", "lsg_domain_destroy", "lsg_domain_deep_copy",
+    "lsg_domain_insert_points", "lsg_domain_set_runs", "lsg_domain_num_runs", "lsg_domain_get_runs",
+    "lsg_domain_num_points", "lsg_domain_width", "lsg_make_sphere", "lsg_make_plane", "lsg_make_box",
+    "lsg_expand", "lsg_reduce", "lsg_prune", "lsg_boolean", "lsg_rebuild", "lsg_advect_create",
+    "lsg_advect_destroy", "lsg_advect_insert_next_level_set", "lsg_advect_clear_level_sets",
+    "lsg_advect_set_velocity_builtin", "lsg_advect_set_spatial_scheme", "lsg_advect_set_temporal_scheme",
+    "lsg_advect_set_advection_time", "lsg_advect_set_time_step_ratio", "lsg_advect_set_single_step",
+    "lsg_advect_set_dissipation_alpha", "lsg_advect_set_check_dissipation",
+    "lsg_advect_set_calculate_normal_vectors", "lsg_advect_set_ignore_voids",
+    "lsg_advect_set_adaptive_time_stepping", "lsg_advect_set_update_point_data", "lsg_advect_prepare_ls",
+    "lsg_advect_apply", "lsg_advect_get_advected_time", "lsg_advect_get_current_time_step",
+    "lsg_advect_get_number_of_time_steps", "lsg_advect_get_point_updates", "lsg_advect_set_profiling",
+    "lsg_advect_get_rate_kernel_stats", "lsg_weno_eval", "lsg_device_synchronize",
+]
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LsgError(f"{LIB_PATH} is missing: build it with `python -m viennals_amd.build` "
+                       "(the advection path has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    c_int, c_double = ctypes.c_int, ctypes.c_double
+    sig = {
+        "lsg_last_error": ([], ctypes.c_char_p),
+        "lsg_domain_create": ([c_int, i32p, i32p, ip, c_double, PP], c_int),
+        "lsg_domain_destroy": ([P], c_int),
+        "lsg_domain_deep_copy": ([P, P], c_int),
+        "lsg_domain_insert_points": ([P, ctypes.c_int64, i32p, dp, c_int], c_int),
+        "lsg_domain_set_runs": ([P, ctypes.c_int64, i32p, dp, c_int], c_int),
+        "lsg_domain_num_runs": ([P, i64p], c_int),
+        "lsg_domain_get_runs": ([P, i32p, dp, ctypes.c_int64], c_int),
+        "lsg_domain_num_points": ([P, i64p], c_int),
+        "lsg_domain_width": ([P, ip], c_int),
+        "lsg_make_sphere": ([P, dp, c_double], c_int),
+        "lsg_make_plane": ([P, dp, dp], c_int),
+        "lsg_make_box": ([P, dp, dp], c_int),
+        "lsg_expand": ([P, c_int], c_int),
+        "lsg_reduce": ([P, c_int], c_int),
+        "lsg_prune": ([P, c_int], c_int),
+        "lsg_boolean": ([P, P, c_int], c_int),
+        "lsg_rebuild": ([P], c_int),
+        "lsg_advect_create": ([PP], c_int),
+        "lsg_advect_destroy": ([P], c_int),
+        "lsg_advect_insert_next_level_set": ([P, P], c_int),
+        "lsg_advect_clear_level_sets": ([P], c_int),
+        "lsg_advect_set_velocity_builtin": ([P, c_int, c_int, c_int, c_double, c_double, c_double, dp], c_int),
+        "lsg_advect_set_spatial_scheme": ([P, c_int], c_int),
+        "lsg_advect_set_temporal_scheme": ([P, c_int], c_int),
+        "lsg_advect_set_advection_time": ([P, c_double], c_int),
+        "lsg_advect_set_time_step_ratio": ([P, c_double], c_int),
+        "lsg_advect_set_single_step": ([P, c_int], c_int),
+        "lsg_advect_set_dissipation_alpha": ([P, c_double], c_int),
+        "lsg_advect_set_check_dissipation": ([P, c_int], c_int),
+        "lsg_advect_set_calculate_normal_vectors": ([P, c_int], c_int),
+        "lsg_advect_set_ignore_voids": ([P, c_int], c_int),
+        "lsg_advect_set_adaptive_time_stepping": ([P, c_int, ctypes.c_uint], c_int),
+        "lsg_advect_set_update_point_data": ([P, c_int], c_int),
+        "lsg_advect_prepare_ls": ([P], c_int),
+        "lsg_advect_apply": ([P], c_int),
+        "lsg_advect_get_advected_time": ([P, dp], c_int),
+        "lsg_advect_get_current_time_step": ([P, dp], c_int),
+        "lsg_advect_get_number_of_time_steps": ([P, ctypes.POINTER(ctypes.c_uint)], c_int),
+        "lsg_advect_get_point_updates": ([P, dp], c_int),
+        "lsg_advect_set_profiling": ([P, c_int], c_int),
+        "lsg_advect_get_rate_kernel_stats": ([P, dp, i64p, dp], c_int),
+        "lsg_weno_eval": ([c_int, ctypes.c_int64, dp, c_double, c_int, dp], c_int),
+        "lsg_device_synchronize": ([], c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return _lib
+
+
+def check(code):
+    """Raise LsgError with lsg_last_error() when a C-ABI call failed."""
+    if code != 0:
+        msg = lib().lsg_last_error()
+        raise LsgError(f"viennals_gpu error {code}: {msg.decode() if msg else ''}")
+    return code
