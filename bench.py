@@ -1,0 +1,165 @@
+"""Benchmark of the level-set advection step on one MI355X.
+
+A "step" is one advection time step (Advect::apply with advectionTime 0: one
+CFL-limited step; three stages for RK3) over the whole band.  Default
+workload: the 3D square-trench etch (256x256 lateral cells, gridDelta 1,
+reflective x/y, infinite z, plane z=0 minus a centred 64x64x128 trench,
+directional etch velocity -(max(0,n_z) + 0.1)), WENO5 + RK3, CFL 0.4, FP64.
+
+value = point-updates/s: active points (|phi| <= 0.5) summed over the RK
+stages of the K timed steps, divided by the wall time of those steps with
+the band resident in HBM.  The rate (Hamiltonian) kernel is timed live with
+HIP events on the library's stream; its algorithmic bytes are 8 B per band
+entry read plus 8 B per active point written.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def build_workload(vls, name):
+    BC = vls.BoundaryConditionEnum
+    if name == "trench256":
+        L = 128
+        d = vls.Domain([-L, L, -L, L, -L, L], [BC.REFLECTIVE_BOUNDARY, BC.REFLECTIVE_BOUNDARY,
+                                               BC.INFINITE_BOUNDARY], 1.0)
+        vls.MakeGeometry(d, vls.Plane([0.0, 0.0, 0.0], [0.0, 0.0, 1.0])).apply()
+        box = vls.Domain([-L, L, -L, L, -L, L], [BC.REFLECTIVE_BOUNDARY, BC.REFLECTIVE_BOUNDARY,
+                                                 BC.INFINITE_BOUNDARY], 1.0)
+        vls.MakeGeometry(box, vls.Box([-32.0, -32.0, -128.0], [32.0, 32.0, 1.0])).apply()
+        vls.BooleanOperation(d, box, vls.BooleanOperationEnum.RELATIVE_COMPLEMENT).apply()
+        vel = vls.VelocityField.normal_power(dir=2, k=1, b=-0.1, a_pos=-1.0, a_neg=0.0)
+        cfg = dict(workload="trench256_weno5_rk3",
+                   domain="256x256 lateral cells, reflective x/y, infinite z, gridDelta 1.0",
+                   geometry="plane z=0 minus centred 64x64x128 box trench",
+                   velocity="-(1.0*max(0,n_z) + 0.1)", spatial="WENO_5TH_ORDER",
+                   temporal="RUNGE_KUTTA_3RD_ORDER", cfl=0.4)
+        return d, vel, vls.SpatialSchemeEnum.WENO_5TH_ORDER, vls.TemporalSchemeEnum.RUNGE_KUTTA_3RD_ORDER, 0.4, cfg
+    if name == "sphere_eo1":
+        d = vls.Domain([-20, 20] * 3, [BC.INFINITE_BOUNDARY] * 3, 0.25)
+        vls.MakeGeometry(d, vls.Sphere([0.0, 0.0, 0.0], 10.0)).apply()
+        vel = vls.VelocityField.constant(1.0)
+        cfg = dict(workload="sphere_r10_eo1_fe", domain="infinite, gridDelta 0.25", geometry="sphere r=10",
+                   velocity="+1", spatial="ENGQUIST_OSHER_1ST_ORDER", temporal="FORWARD_EULER", cfl=0.4)
+        return d, vel, 0, 0, 0.4, cfg
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(idx, val, width, grid, params, steps, threads):
+    """The oracle (CPU restatement, OpenMP) on the same input band."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    po.set_threads(threads)
+    D, gmin, gmax, bc, dx = grid
+    o = po.Domain(D, gmin, gmax, bc, dx)
+    o.set(idx, val, width)
+    pu = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st, r = po.advect(o, params)
+        pu += r["point_updates"]
+    dt = time.perf_counter() - t0
+    return pu / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="trench256")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU; "
+              "multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import viennals_amd as vls
+
+    d, vel, spatial, temporal, cfl, cfg = build_workload(vls, args.workload)
+    idx0, val0 = d.get_runs()
+    width0 = d.getLevelSetWidth()
+
+    adv = vls.Advect(d, vel)
+    adv.setSpatialScheme(spatial)
+    adv.setTemporalScheme(temporal)
+    adv.setTimeStepRatio(cfl)
+    adv.setAdvectionTime(0.0)
+    for _ in range(args.warmup):
+        adv.apply()
+    vls.synchronize()
+
+    adv.setProfiling(True)
+    pu = 0.0
+    t_adv = 0.0
+    vls.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        adv.apply()
+        pu += adv.getPointUpdates()
+        t_adv += adv.getAdvectedTime()
+    vls.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    ms, launches, rbytes = adv.getRateKernelStats()
+    value = pu / wall
+    achieved = (rbytes / (ms * 1e-3)) / 1e9 if ms > 0 else 0.0
+
+    rec = {
+        "metric": "point-updates/s + %HBM peak (WENO5-RK3 3D)",
+        "value": value,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": dict(cfg, active_point_updates_per_step=pu / args.steps,
+                       advected_time_per_step=t_adv / args.steps, initial_runs=int(len(val0))),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "kRates (WENO5 Hamiltonian + CFL reduction)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "launches": launches,
+            "avg_launch_us": (ms * 1e3 / launches) if launches else None,
+            "algorithmic_bytes_per_launch": (rbytes / launches) if launches else None,
+        },
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        threads = os.cpu_count() or 1
+        params = po.make_params(spatial=spatial, temporal=temporal, cfl=cfl, time=0.0, vel=vel.as_dict())
+        grid = (d.D, d.gmin, d.gmax, d.bc, d.gridDelta)
+        cpu_value, cpu_wall = cpu_baseline(idx0, val0, width0, grid, params, args.cpu_steps, threads)
+        rec["cpu_baseline"] = {
+            "value": cpu_value,
+            "unit": "point-updates/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {args.cpu_steps} steps of the same workload from the same initial band "
+                      f"({cpu_wall:.1f} s, OpenMP {threads} threads)",
+        }
+    print(json.dumps(rec))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

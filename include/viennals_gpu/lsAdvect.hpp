@@ -1,0 +1,435 @@
+// viennals::Advect<T,D> and friends on the MI355X path: a header-only C++
+// facade over the C ABI of libviennals_gpu.so (include/viennals_gpu.h) with
+// the names, setters and error behaviour of the reference
+// (include/viennals/lsAdvect.hpp:54-1044, lsDomain.hpp, lsMakeGeometry.hpp,
+// lsExpand.hpp, lsReduce.hpp, lsPrune.hpp, lsBooleanOperation.hpp).
+//
+// Differences a caller sees:
+//  * T must be double (the GPU path computes in FP64); D is 2 or 3.
+//  * Velocities are GPU functors (BuiltinVelocityField); a user subclass of
+//    VelocityField<T> that is not a BuiltinVelocityField is rejected with an
+//    error, like the reference rejects a missing field.
+//  * Errors are logged to stderr and the call returns, as the reference does
+//    with its Logger (lsAdvect.hpp:1019-1027).
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <utility>
+#include <vector>
+
+#include "../viennals_gpu.h"
+
+namespace viennals {
+
+template <class T> class SmartPointer : public std::shared_ptr<T> {
+public:
+  SmartPointer() = default;
+  SmartPointer(std::nullptr_t) {}
+  SmartPointer(const std::shared_ptr<T> &p) : std::shared_ptr<T>(p) {}
+  template <class U> SmartPointer(const SmartPointer<U> &p) : std::shared_ptr<T>(p) {}
+  template <class... Args> static SmartPointer New(Args &&...args) {
+    return SmartPointer(std::make_shared<T>(std::forward<Args>(args)...));
+  }
+};
+
+template <class T> using Vec3D = std::array<T, 3>;
+
+namespace detail {
+inline bool ok(int code, const char *what) {
+  if (code != 0) {
+    std::fprintf(stderr, "[viennals_gpu] %s: %s\n", what, lsg_last_error());
+    return false;
+  }
+  return true;
+}
+}  // namespace detail
+
+enum class BoundaryConditionEnum : unsigned {
+  REFLECTIVE_BOUNDARY = 0,
+  INFINITE_BOUNDARY = 1,
+  PERIODIC_BOUNDARY = 2,
+  POS_INFINITE_BOUNDARY = 3,
+  NEG_INFINITE_BOUNDARY = 4
+};
+
+enum class SpatialSchemeEnum : unsigned {
+  ENGQUIST_OSHER_1ST_ORDER = 0,
+  ENGQUIST_OSHER_2ND_ORDER = 1,
+  LAX_FRIEDRICHS_1ST_ORDER = 2,
+  LAX_FRIEDRICHS_2ND_ORDER = 3,
+  LOCAL_LAX_FRIEDRICHS_ANALYTICAL_1ST_ORDER = 4,
+  LOCAL_LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 5,
+  LOCAL_LOCAL_LAX_FRIEDRICHS_2ND_ORDER = 6,
+  LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 7,
+  LOCAL_LAX_FRIEDRICHS_2ND_ORDER = 8,
+  STENCIL_LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 9,
+  WENO_3RD_ORDER = 10,
+  WENO_5TH_ORDER = 11
+};
+using IntegrationSchemeEnum = SpatialSchemeEnum;
+
+enum class TemporalSchemeEnum : unsigned { FORWARD_EULER = 0, RUNGE_KUTTA_2ND_ORDER = 1, RUNGE_KUTTA_3RD_ORDER = 2 };
+
+enum struct BooleanOperationEnum : unsigned { INTERSECT = 0, UNION = 1, RELATIVE_COMPLEMENT = 2 };
+
+// ------------------------------------------------------------- Domain ---
+
+template <class T, int D> class Domain {
+  static_assert(std::is_same<T, double>::value, "the GPU path computes in double precision");
+  static_assert(D == 2 || D == 3, "D must be 2 or 3");
+  lsg_domain *h_ = nullptr;
+  double gridDelta_ = 1.0;
+
+public:
+  using BoundaryType = BoundaryConditionEnum;
+  using PointValueVectorType = std::vector<std::pair<std::array<int32_t, D>, T>>;
+  static constexpr T POS_VALUE = 1.7976931348623157e308;
+  static constexpr T NEG_VALUE = -1.7976931348623157e308;
+
+  // Domain(gridDelta): all boundaries infinite (lsDomain.hpp:55-66)
+  explicit Domain(double gridDelta = 1.0) : gridDelta_(gridDelta) {
+    int32_t lo[D] = {}, hi[D] = {};
+    int bc[D];
+    for (int i = 0; i < D; ++i) bc[i] = LSG_INFINITE;
+    detail::ok(lsg_domain_create(D, lo, hi, bc, gridDelta, &h_), "Domain");
+  }
+  // Domain(bounds, boundaryConditions, gridDelta) (lsDomain.hpp:68-78)
+  Domain(const double *bounds, const BoundaryType *bcs, double gridDelta = 1.0) : gridDelta_(gridDelta) {
+    int32_t lo[D], hi[D];
+    int bc[D];
+    for (int i = 0; i < D; ++i) {
+      lo[i] = (int32_t)std::floor(bounds[2 * i] / gridDelta);
+      hi[i] = (int32_t)std::ceil(bounds[2 * i + 1] / gridDelta);
+      bc[i] = (int)bcs[i];
+    }
+    detail::ok(lsg_domain_create(D, lo, hi, bc, gridDelta, &h_), "Domain");
+  }
+  ~Domain() {
+    if (h_) lsg_domain_destroy(h_);
+  }
+  Domain(const Domain &) = delete;
+  Domain &operator=(const Domain &) = delete;
+
+  template <class... Args> static auto New(Args &&...args) {
+    return SmartPointer<Domain>::New(std::forward<Args>(args)...);
+  }
+
+  lsg_domain *handle() const { return h_; }
+  double getGridDelta() const { return gridDelta_; }
+
+  // Domain::deepCopy (lsDomain.hpp:124-129)
+  void deepCopy(const SmartPointer<Domain> &other) { detail::ok(lsg_domain_deep_copy(h_, other->h_), "deepCopy"); }
+  // Domain::insertPoints (lsDomain.hpp:135-138)
+  void insertPoints(const PointValueVectorType &pts, bool = true) {
+    std::vector<int32_t> idx;
+    std::vector<double> val;
+    idx.reserve(pts.size() * D);
+    for (auto &p : pts) {
+      for (int i = 0; i < D; ++i) idx.push_back(p.first[i]);
+      val.push_back(p.second);
+    }
+    detail::ok(lsg_domain_insert_points(h_, (int64_t)val.size(), idx.data(), val.data(), 2), "insertPoints");
+  }
+  unsigned getNumberOfPoints() const {
+    int64_t n = 0;
+    detail::ok(lsg_domain_num_points(h_, &n), "getNumberOfPoints");
+    return (unsigned)n;
+  }
+  int getLevelSetWidth() const {
+    int w = 0;
+    detail::ok(lsg_domain_width(h_, &w), "getLevelSetWidth");
+    return w;
+  }
+  // Run-start list (HRLE iteration order); undefined runs carry +-POS_VALUE.
+  void getRuns(std::vector<std::array<int32_t, D>> &idx, std::vector<T> &val) const {
+    int64_t n = 0;
+    if (!detail::ok(lsg_domain_num_runs(h_, &n), "getRuns")) return;
+    std::vector<int32_t> flat((size_t)n * D);
+    val.resize((size_t)n);
+    detail::ok(lsg_domain_get_runs(h_, flat.data(), val.data(), n), "getRuns");
+    idx.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < D; ++k) idx[i][k] = flat[i * D + k];
+  }
+};
+
+// ---------------------------------------------------------- geometries ---
+
+template <class T, int D> struct Sphere {
+  std::array<T, 3> origin{};
+  T radius;
+  Sphere(const T *o, T r) : radius(r) {
+    for (int i = 0; i < D; ++i) origin[i] = o[i];
+  }
+  template <class... A> static auto New(A &&...a) { return SmartPointer<Sphere>::New(std::forward<A>(a)...); }
+};
+template <class T, int D> struct Plane {
+  std::array<T, 3> origin{}, normal{};
+  Plane(const T *o, const T *n) {
+    for (int i = 0; i < D; ++i) origin[i] = o[i], normal[i] = n[i];
+  }
+  template <class... A> static auto New(A &&...a) { return SmartPointer<Plane>::New(std::forward<A>(a)...); }
+};
+template <class T, int D> struct Box {
+  std::array<T, 3> minCorner{}, maxCorner{};
+  Box(const T *mn, const T *mx) {
+    for (int i = 0; i < D; ++i) minCorner[i] = mn[i], maxCorner[i] = mx[i];
+  }
+  template <class... A> static auto New(A &&...a) { return SmartPointer<Box>::New(std::forward<A>(a)...); }
+};
+
+template <class T, int D> class MakeGeometry {
+  SmartPointer<Domain<T, D>> ls_;
+  int kind_ = -1;
+  std::array<T, 3> a_{}, b_{};
+  T r_ = 0;
+
+public:
+  MakeGeometry(SmartPointer<Domain<T, D>> ls, SmartPointer<Sphere<T, D>> s) : ls_(ls), kind_(0), a_(s->origin), r_(s->radius) {}
+  MakeGeometry(SmartPointer<Domain<T, D>> ls, SmartPointer<Plane<T, D>> p) : ls_(ls), kind_(1), a_(p->origin), b_(p->normal) {}
+  MakeGeometry(SmartPointer<Domain<T, D>> ls, SmartPointer<Box<T, D>> b) : ls_(ls), kind_(2), a_(b->minCorner), b_(b->maxCorner) {}
+  void apply() {
+    if (kind_ == 0) detail::ok(lsg_make_sphere(ls_->handle(), a_.data(), r_), "MakeGeometry");
+    else if (kind_ == 1) detail::ok(lsg_make_plane(ls_->handle(), a_.data(), b_.data()), "MakeGeometry");
+    else if (kind_ == 2) detail::ok(lsg_make_box(ls_->handle(), a_.data(), b_.data()), "MakeGeometry");
+  }
+};
+
+// ------------------------------------------------------ band maintenance ---
+
+template <class T, int D> class Expand {
+  SmartPointer<Domain<T, D>> ls_;
+  int width_ = 0;
+
+public:
+  Expand() = default;
+  Expand(SmartPointer<Domain<T, D>> ls, int width = 0) : ls_(ls), width_(width) {}
+  void setLevelSet(SmartPointer<Domain<T, D>> ls) { ls_ = ls; }
+  void setWidth(int w) { width_ = w; }
+  void apply() {
+    if (!ls_) {
+      std::fprintf(stderr, "[viennals_gpu] No level set passed to Expand. Not expanding.\n");
+      return;
+    }
+    detail::ok(lsg_expand(ls_->handle(), width_), "Expand");
+  }
+};
+
+template <class T, int D> class Reduce {
+  SmartPointer<Domain<T, D>> ls_;
+  int width_ = 0;
+
+public:
+  Reduce() = default;
+  Reduce(SmartPointer<Domain<T, D>> ls, int width = 0, bool = false) : ls_(ls), width_(width) {}
+  void setLevelSet(SmartPointer<Domain<T, D>> ls) { ls_ = ls; }
+  void setWidth(int w) { width_ = w; }
+  void apply() {
+    if (!ls_) {
+      std::fprintf(stderr, "[viennals_gpu] No level set was passed to Reduce.\n");
+      return;
+    }
+    detail::ok(lsg_reduce(ls_->handle(), width_), "Reduce");
+  }
+};
+
+template <class T, int D> class Prune {
+  SmartPointer<Domain<T, D>> ls_;
+  bool rsz_ = false;
+
+public:
+  Prune() = default;
+  explicit Prune(SmartPointer<Domain<T, D>> ls) : ls_(ls) {}
+  void setLevelSet(SmartPointer<Domain<T, D>> ls) { ls_ = ls; }
+  void setRemoveStrayZeros(bool v) { rsz_ = v; }
+  void apply() {
+    if (!ls_) {
+      std::fprintf(stderr, "[viennals_gpu] No level set was passed to Prune.\n");
+      return;
+    }
+    detail::ok(lsg_prune(ls_->handle(), rsz_ ? 1 : 0), "Prune");
+  }
+};
+
+template <class T, int D> class BooleanOperation {
+  SmartPointer<Domain<T, D>> a_, b_;
+  BooleanOperationEnum op_ = BooleanOperationEnum::INTERSECT;
+
+public:
+  BooleanOperation() = default;
+  BooleanOperation(SmartPointer<Domain<T, D>> a, SmartPointer<Domain<T, D>> b,
+                   BooleanOperationEnum op = BooleanOperationEnum::INTERSECT)
+      : a_(a), b_(b), op_(op) {}
+  void setLevelSet(SmartPointer<Domain<T, D>> a) { a_ = a; }
+  void setSecondLevelSet(SmartPointer<Domain<T, D>> b) { b_ = b; }
+  void setBooleanOperation(BooleanOperationEnum op) { op_ = op; }
+  void apply() {
+    if (!a_ || !b_) {
+      std::fprintf(stderr, "[viennals_gpu] BooleanOperation needs two level sets.\n");
+      return;
+    }
+    detail::ok(lsg_boolean(a_->handle(), b_->handle(), (int)op_), "BooleanOperation");
+  }
+};
+
+// ------------------------------------------------------ velocity fields ---
+
+// Abstract velocity field (lsVelocityField.hpp:11-41).  The GPU path evaluates
+// velocities on the device; only BuiltinVelocityField is accepted by Advect.
+template <class T> class VelocityField {
+public:
+  virtual T getScalarVelocity(const Vec3D<T> &, int, const Vec3D<T> &, unsigned long) { return 0; }
+  virtual Vec3D<T> getVectorVelocity(const Vec3D<T> &, int, const Vec3D<T> &, unsigned long) { return {0, 0, 0}; }
+  virtual T getDissipationAlpha(int, int, const Vec3D<T> &) { return 0; }
+  virtual ~VelocityField() = default;
+};
+
+// scalar = b                                          (kind LSG_VEL_CONSTANT)
+//        = b + (n[dir] > 0 ? aPos : aNeg) * |n[dir]|^k (kind LSG_VEL_NORMAL_POWER)
+// vector = vec
+template <class T> class BuiltinVelocityField : public VelocityField<T> {
+public:
+  int kind = LSG_VEL_CONSTANT, dir = 2, k = 1;
+  T b = 0, aPos = 0, aNeg = 0;
+  Vec3D<T> vec{0, 0, 0};
+  static SmartPointer<BuiltinVelocityField> constant(T scalar, Vec3D<T> vector = {0, 0, 0}) {
+    auto v = SmartPointer<BuiltinVelocityField>::New();
+    v->b = scalar;
+    v->vec = vector;
+    return v;
+  }
+  static SmartPointer<BuiltinVelocityField> normalPower(int dir, int k, T b, T aPos, T aNeg = 0,
+                                                       Vec3D<T> vector = {0, 0, 0}) {
+    auto v = SmartPointer<BuiltinVelocityField>::New();
+    v->kind = LSG_VEL_NORMAL_POWER;
+    v->dir = dir;
+    v->k = k;
+    v->b = b;
+    v->aPos = aPos;
+    v->aNeg = aNeg;
+    v->vec = vector;
+    return v;
+  }
+  T getScalarVelocity(const Vec3D<T> &, int, const Vec3D<T> &n, unsigned long) override {
+    if (kind == LSG_VEL_CONSTANT) return b;
+    T c = n[dir], m = 1;
+    if (k >= 1) {
+      m = std::fabs(c);
+      for (int j = 1; j < k; ++j) m = m * std::fabs(c);
+    }
+    return b + ((c > 0) ? aPos : aNeg) * m;
+  }
+  Vec3D<T> getVectorVelocity(const Vec3D<T> &, int, const Vec3D<T> &, unsigned long) override { return vec; }
+};
+
+// --------------------------------------------------------------- Advect ---
+
+template <class T, int D> class Advect {
+  lsg_advect *h_ = nullptr;
+  std::vector<SmartPointer<Domain<T, D>>> levelSets_;
+  SmartPointer<VelocityField<T>> velocities_;
+
+  bool pushVelocity() {
+    auto *b = dynamic_cast<BuiltinVelocityField<T> *>(velocities_.get());
+    if (!b) {
+      std::fprintf(stderr, "[viennals_gpu] Advect: the GPU path needs a BuiltinVelocityField. Not advecting.\n");
+      return false;
+    }
+    double vec[3] = {b->vec[0], b->vec[1], b->vec[2]};
+    return detail::ok(lsg_advect_set_velocity_builtin(h_, b->kind, b->dir, b->k, b->b, b->aPos, b->aNeg, vec),
+                      "setVelocityField");
+  }
+
+public:
+  static constexpr char velocityLabel[] = "AdvectionVelocities";
+  static constexpr char dissipationLabel[] = "Dissipation";
+
+  Advect() { detail::ok(lsg_advect_create(&h_), "Advect"); }
+  explicit Advect(SmartPointer<Domain<T, D>> ls) : Advect() { insertNextLevelSet(ls); }
+  Advect(SmartPointer<Domain<T, D>> ls, SmartPointer<VelocityField<T>> v) : Advect() {
+    insertNextLevelSet(ls);
+    setVelocityField(v);
+  }
+  Advect(std::vector<SmartPointer<Domain<T, D>>> lss, SmartPointer<VelocityField<T>> v) : Advect() {
+    for (auto &l : lss) insertNextLevelSet(l);
+    setVelocityField(v);
+  }
+  ~Advect() {
+    if (h_) lsg_advect_destroy(h_);
+  }
+  Advect(const Advect &) = delete;
+  Advect &operator=(const Advect &) = delete;
+
+  void insertNextLevelSet(SmartPointer<Domain<T, D>> ls) {
+    levelSets_.push_back(ls);
+    detail::ok(lsg_advect_insert_next_level_set(h_, ls->handle()), "insertNextLevelSet");
+  }
+  void clearLevelSets() {
+    levelSets_.clear();
+    detail::ok(lsg_advect_clear_level_sets(h_), "clearLevelSets");
+  }
+  void setVelocityField(SmartPointer<VelocityField<T>> v) { velocities_ = v; }
+  void setAdvectionTime(double t) { detail::ok(lsg_advect_set_advection_time(h_, t), "setAdvectionTime"); }
+  void setSingleStep(bool s) { detail::ok(lsg_advect_set_single_step(h_, s ? 1 : 0), "setSingleStep"); }
+  void setTimeStepRatio(const double &cfl) { detail::ok(lsg_advect_set_time_step_ratio(h_, cfl), "setTimeStepRatio"); }
+  void setCalculateNormalVectors(bool c) {
+    detail::ok(lsg_advect_set_calculate_normal_vectors(h_, c ? 1 : 0), "setCalculateNormalVectors");
+  }
+  void setIgnoreVoids(bool v) { detail::ok(lsg_advect_set_ignore_voids(h_, v ? 1 : 0), "setIgnoreVoids"); }
+  void setAdaptiveTimeStepping(bool on = true, unsigned sub = 20) {
+    detail::ok(lsg_advect_set_adaptive_time_stepping(h_, on ? 1 : 0, sub), "setAdaptiveTimeStepping");
+  }
+  void setSaveAdvectionVelocities(bool) {}
+  void setSpatialScheme(SpatialSchemeEnum s) { detail::ok(lsg_advect_set_spatial_scheme(h_, (int)s), "setSpatialScheme"); }
+  [[deprecated("Use setSpatialScheme instead")]] void setIntegrationScheme(IntegrationSchemeEnum s) { setSpatialScheme(s); }
+  void setTemporalScheme(TemporalSchemeEnum s) {
+    detail::ok(lsg_advect_set_temporal_scheme(h_, (int)s), "setTemporalScheme");
+  }
+  void setDissipationAlpha(const double &a) { detail::ok(lsg_advect_set_dissipation_alpha(h_, a), "setDissipationAlpha"); }
+  void setCheckDissipation(bool c) { detail::ok(lsg_advect_set_check_dissipation(h_, c ? 1 : 0), "setCheckDissipation"); }
+  void setUpdatePointData(bool u) { detail::ok(lsg_advect_set_update_point_data(h_, u ? 1 : 0), "setUpdatePointData"); }
+
+  double getAdvectedTime() const {
+    double t = 0;
+    lsg_advect_get_advected_time(h_, &t);
+    return t;
+  }
+  double getCurrentTimeStep() const {
+    double t = 0;
+    lsg_advect_get_current_time_step(h_, &t);
+    return t;
+  }
+  unsigned getNumberOfTimeSteps() const {
+    unsigned n = 0;
+    lsg_advect_get_number_of_time_steps(h_, &n);
+    return n;
+  }
+
+  void prepareLS() {
+    if (levelSets_.empty()) {
+      std::fprintf(stderr, "[viennals_gpu] No level sets passed to Advect.\n");
+      return;
+    }
+    detail::ok(lsg_advect_prepare_ls(h_), "prepareLS");
+  }
+
+  // Advect::apply (lsAdvect.hpp:1017-1043)
+  void apply() {
+    if (levelSets_.empty()) {
+      std::fprintf(stderr, "[viennals_gpu] No level sets passed to Advect. Not advecting.\n");
+      return;
+    }
+    if (velocities_ == nullptr) {
+      std::fprintf(stderr, "[viennals_gpu] No velocity field passed to Advect. Not advecting.\n");
+      return;
+    }
+    if (!pushVelocity()) return;
+    detail::ok(lsg_advect_apply(h_), "Advect::apply");
+  }
+};
+
+}  // namespace viennals

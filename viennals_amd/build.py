@@ -17,6 +17,19 @@ FLAGS = [
 ]
 
 
+def build_cpp_example(verbose=False):
+    """C++ program using the include/viennals_gpu/lsAdvect.hpp facade (tests/cpp)."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "advect_trench.cpp")
+    out = os.path.join(root, "tests", "cpp", "advect_trench")
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(root, "include"), src, "-L" + HERE, "-lviennals_gpu",
+           "-Wl,-rpath,$ORIGIN/../../viennals_amd", "-o", out]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.check_call(cmd)
+    return out
+
+
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
@@ -30,4 +43,5 @@ def build(verbose=False):
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_example(verbose=True)
     print("built", LIB)
