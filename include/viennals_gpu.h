@@ -44,7 +44,11 @@ enum {
   LSG_LOCAL_LAX_FRIEDRICHS_2ND_ORDER = 8,
   LSG_STENCIL_LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 9,
   LSG_WENO_3RD_ORDER = 10,
-  LSG_WENO_5TH_ORDER = 11
+  LSG_WENO_5TH_ORDER = 11,
+  /* No counterpart in the reference: WENO_5TH_ORDER whose nonlinear weights
+   * come from a trained network (lsg_advect_set_learned_weights) wherever the
+   * smoothness indicators agree within a factor 10; plain weno5 elsewhere. */
+  LSG_WENO_5TH_ORDER_LEARNED = 12
 };
 
 /* Temporal schemes: TemporalSchemeEnum (lsAdvectIntegrationSchemes.hpp:31-35). */
@@ -142,6 +146,30 @@ int lsg_advect_get_rate_kernel_stats(lsg_advect *a, double *ms, int64_t *launche
  * (lsFiniteDifferences.hpp:58-123) evaluated on the GPU for n stencils:
  * x[n*(2r+1)] -> out[n]. */
 int lsg_weno_eval(int order, int64_t n, const double *x, double dx, int plus, double *out);
+
+/* ---- Learned WENO5 weights (no counterpart in the reference) -------------
+ * A network 5 -> 32 -> 32 -> 3 (GELU, softmax) replaces the nonlinear
+ * weights alpha_k / sum(alpha) of FiniteDifferences::weno5
+ * (lsFiniteDifferences.hpp:108-122).  Parameters: 1347 floats laid out
+ * W1[32][5], b1[32], W2[32][32], b2[32], W3[3][32], b3[3]; the GPU uses
+ * them as bf16 with fp32 accumulation. */
+typedef struct lsg_mlp lsg_mlp;
+/* Network parameters of spatial scheme LSG_WENO_5TH_ORDER_LEARNED. */
+int lsg_advect_set_learned_weights(lsg_advect *a, const float *params, int n);
+/* Learned weno5 on n stencils x[n*7] (same arguments as lsg_weno_eval). */
+int lsg_weno5_learned_eval(int64_t n, const double *x, double dx, int plus, const float *params, double *out);
+/* Trainer: fp32 master parameters, Adam (beta 0.9/0.999, eps 1e-8).  A sample
+ * is 8 floats (the 5 inputs d/max|d| and the 3 candidate polynomials / max|d|)
+ * and a target (derivative * dx / max|d|); loss = mean squared error of
+ * sum_k w_k p_k. */
+int lsg_mlp_create(lsg_mlp **out, const float *params);
+int lsg_mlp_destroy(lsg_mlp *m);
+int lsg_mlp_set_samples(lsg_mlp *m, int64_t n, const float *x8, const float *target);
+/* Loss and gradient over samples [offset, offset + count). */
+int lsg_mlp_loss_grad(lsg_mlp *m, int64_t offset, int64_t count, double *loss, float *grad);
+/* iters Adam steps over consecutive minibatches of `batch` samples. */
+int lsg_mlp_train(lsg_mlp *m, int iters, int64_t batch, double lr, double *last_loss);
+int lsg_mlp_get_params(lsg_mlp *m, float *out);
 
 /* Device synchronisation (for timing by callers). */
 int lsg_device_synchronize(void);

@@ -529,7 +529,7 @@ double weno3(const double *x, double delta, bool plus, double eps = 1e-6) {
 }
 
 // FiniteDifferences::weno5 (lsFiniteDifferences.hpp:88-123)
-double weno5(const double *x, double dx, bool plus, double eps = 1e-6) {
+double weno5(const double *x, double dx, bool plus, double eps) {
   double d[5];
   if (!plus) {
     for (int i = 0; i < 5; ++i) d[i] = x[i + 1] - x[i];
@@ -560,6 +560,7 @@ template <int D> struct SchemeCtx {
   double alphaFactor;
   double globalAlphas[3];  // LF only
   const Velocity *vel;
+  const float *mlp = nullptr;  // learned weno5 weights (WENO5_LEARNED)
   double f(const Idx<D> &c, int dim, int off) const {
     Idx<D> p = c;
     p[dim] = ls->g.map(dim, (int64_t)c[dim] + off);
@@ -655,6 +656,9 @@ template <int D> static void schemeWENO(const SchemeCtx<D> &s, const Idx<D> &c, 
     if (order == 3) {
       gm[i] = weno3(st, s.dx, false);
       gp[i] = weno3(st, s.dx, true);
+    } else if (s.mlp) {
+      gm[i] = weno5L(st, s.dx, false, s.mlp);
+      gp[i] = weno5L(st, s.dx, true, s.mlp);
     } else {
       gm[i] = weno5(st, s.dx, false);
       gp[i] = weno5(st, s.dx, true);
@@ -850,7 +854,7 @@ static int prepareWidth(int spatial) {
   case LLF1: return 7;  // lsLocalLaxFriedrichs.hpp:55 (2*(order+2)+1)
   case LLF2: return 9;
   case WENO3: return 5; // lsWENO.hpp:44 (2*r+1)
-  case WENO5: return 7;
+  case WENO5: case WENO5_LEARNED: return 7;
   default: return -1;
   }
 }
@@ -858,7 +862,7 @@ static int schemeOrder(int spatial) {
   switch (spatial) {
   case EO2: case LF2: case LLLF2: case LLF2: return 2;
   case WENO3: return 3;
-  case WENO5: return 5;
+  case WENO5: case WENO5_LEARNED: return 5;
   default: return 1;
   }
 }
@@ -924,7 +928,9 @@ template <int D> struct Advector {
     s.globalAlphas[0] = s.globalAlphas[1] = s.globalAlphas[2] = 0;
     if (P.spatial == LF1 || P.spatial == LF2) globalAlphas(ls, s.globalAlphas);
     // WENO receives dissipationAlpha as its calcNormal flag (lsAdvect.hpp:670-676)
-    if (P.spatial == WENO3 || P.spatial == WENO5) s.calcNormals = (P.dissipationAlpha != 0.0);
+    if (P.spatial == WENO3 || P.spatial == WENO5 || P.spatial == WENO5_LEARNED)
+      s.calcNormals = (P.dissipationAlpha != 0.0);
+    if (P.spatial == WENO5_LEARNED) s.mlp = P.mlp;
     const int order = schemeOrder(P.spatial);
     const size_t n = ls.idx.size();
     rates.assign(n, 0.0);
@@ -948,7 +954,7 @@ template <int D> struct Advector {
         case LF1: case LF2: schemeLF<D>(s, c, order, H, Dn); break;
         case LLLF1: case LLLF2: schemeLLLF<D>(s, c, order, H, Dn, locAlpha); break;
         case LLF1: case LLF2: schemeLLF<D>(s, c, order, H, Dn, locAlpha); break;
-        case WENO3: case WENO5: schemeWENO<D>(s, c, order, H, Dn); break;
+        case WENO3: case WENO5: case WENO5_LEARNED: schemeWENO<D>(s, c, order, H, Dn); break;
         }
         // :530-588, single level set (no material below)
         double velocity = H - Dn;
@@ -1230,7 +1236,7 @@ int lso_check(const lso_domain *d, char *msg, int msglen) {
   return f;
 }
 
-double lso_weno5(const double *x, double dx, int plus) { return weno5(x, dx, plus != 0); }
+double lso_weno5(const double *x, double dx, int plus) { return weno5(x, dx, plus != 0, 1e-6); }
 double lso_weno3(const double *x, double dx, int plus) { return weno3(x, dx, plus != 0); }
 
 // Flat parameter block shared with the test harness (ctypes).
@@ -1243,6 +1249,7 @@ struct lso_params {
   int velKind, velDir, velK;
   double velB, velAPos, velANeg;
   double vec[3];
+  const float *mlp;
 };
 
 static AdvectParams toParams(const lso_params *p) {
@@ -1262,6 +1269,7 @@ static AdvectParams toParams(const lso_params *p) {
   P.vel.aPos = p->velAPos;
   P.vel.aNeg = p->velANeg;
   for (int i = 0; i < 3; ++i) P.vel.vec[i] = p->vec[i];
+  P.mlp = p->mlp;
   return P;
 }
 

@@ -52,7 +52,8 @@ enum BC : int { REFLECTIVE = 0, INFINITE = 1, PERIODIC = 2, POS_INFINITE = 3, NE
 // lsAdvectIntegrationSchemes.hpp:10-23
 enum Spatial : int {
   EO1 = 0, EO2 = 1, LF1 = 2, LF2 = 3, LLFA1 = 4, LLLF1 = 5, LLLF2 = 6,
-  LLF1 = 7, LLF2 = 8, SLLF1 = 9, WENO3 = 10, WENO5 = 11
+  LLF1 = 7, LLF2 = 8, SLLF1 = 9, WENO3 = 10, WENO5 = 11,
+  WENO5_LEARNED = 12  // extension: weno5 with learned weights (mlp_oracle.cpp)
 };
 // lsAdvectIntegrationSchemes.hpp:31-35
 enum Temporal : int { FE = 0, RK2 = 1, RK3 = 2 };
@@ -169,6 +170,7 @@ struct AdvectParams {
   int calcNormals = 1;           // lsAdvect.hpp:68
   int checkDissipation = 1;      // lsAdvect.hpp:76
   Velocity vel;
+  const float *mlp = nullptr;    // WENO5_LEARNED network parameters
 };
 
 struct AdvectResult {
@@ -178,5 +180,19 @@ struct AdvectResult {
   int status = 0;
   double pointUpdates = 0;  // sum over stages of active points
 };
+
+// Learned WENO5 weights (mlp_oracle.cpp).  Parameter layout (floats):
+// W1[32][5], b1[32], W2[32][32], b2[32], W3[3][32], b3[3].
+constexpr int MLP_IN = 5, MLP_H = 32, MLP_OUT = 3;
+constexpr int MLP_W1 = 0, MLP_B1 = 160, MLP_W2 = 192, MLP_B2 = 1216, MLP_W3 = 1248, MLP_B3 = 1344;
+constexpr int MLP_NPARAM = 1347;
+constexpr double MLP_KINK_RATIO = 10.0;
+float bf16Round(float x);
+void mlpForward(const float *P, const float in[MLP_IN], float w[MLP_OUT]);
+void mlpInputs(const double *d, float in[MLP_IN], double &scale);
+bool weno5Kink(const double *s, double dx, double ratio);
+double weno5(const double *x, double dx, bool plus, double eps = 1e-6);
+double weno5L(const double *x, double dx, bool plus, const float *P);
+double mlpLossGrad(const float *P, int64_t n, const float *X, const float *T, double *G, int emulate, int useDouble);
 
 }  // namespace lso

@@ -30,6 +30,7 @@ class Params(ctypes.Structure):
         ("velKind", ctypes.c_int), ("velDir", ctypes.c_int), ("velK", ctypes.c_int),
         ("velB", ctypes.c_double), ("velAPos", ctypes.c_double), ("velANeg", ctypes.c_double),
         ("vec", ctypes.c_double * 3),
+        ("mlp", ctypes.c_void_p),
     ]
 
 
@@ -73,6 +74,14 @@ def lib():
         L.lso_advect.restype = ctypes.c_int
         L.lso_compute_rates.argtypes = [P, ctypes.POINTER(Params), ctypes.c_double, dp, ctypes.c_int64]
         L.lso_compute_rates.restype = ctypes.c_double
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.lso_mlp_forward.argtypes = [fp, fp, fp]
+        L.lso_weno5_learned.argtypes = [dp, ctypes.c_double, ctypes.c_int, fp]
+        L.lso_weno5_learned.restype = ctypes.c_double
+        L.lso_mlp_loss_grad.argtypes = [fp, ctypes.c_int64, fp, fp, dp, ctypes.c_int, ctypes.c_int]
+        L.lso_mlp_loss_grad.restype = ctypes.c_double
+        L.lso_bf16_round.argtypes = [ctypes.c_float]
+        L.lso_bf16_round.restype = ctypes.c_float
         L.lso_num_threads.restype = ctypes.c_int
         L.lso_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -172,9 +181,13 @@ class Domain:
 
 def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
                 dissipation_alpha=1.0, calc_normals=True, check_dissipation=True,
-                vel=None):
-    """vel: dict(kind, dir, k, b, a_pos, a_neg, vec) of the built-in velocity."""
+                vel=None, mlp=None):
+    """vel: dict(kind, dir, k, b, a_pos, a_neg, vec) of the built-in velocity.
+    mlp: float32 array of the learned-weno5 network (spatial scheme 12)."""
     p = Params()
+    if mlp is not None:
+        p._mlp_keep = np.ascontiguousarray(mlp, dtype=np.float32)
+        p.mlp = p._mlp_keep.ctypes.data
     p.spatial = spatial
     p.temporal = temporal
     p.cfl = cfl
@@ -214,6 +227,45 @@ def compute_rates(dom, params, max_dt=np.finfo(np.float64).max):
 def weno5(x, dx, plus):
     a, pa = _f64(x)
     return lib().lso_weno5(pa, float(dx), int(plus))
+
+
+MLP_NPARAM = 1347
+MLP_OFFSETS = dict(W1=(0, (32, 5)), b1=(160, (32,)), W2=(192, (32, 32)), b2=(1216, (32,)),
+                   W3=(1248, (3, 32)), b3=(1344, (3,)))
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def mlp_forward(params, x):
+    P, pp = _f32(params)
+    X, px = _f32(x)
+    w = np.zeros(3, np.float32)
+    lib().lso_mlp_forward(pp, px, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return w
+
+
+def weno5_learned(x, dx, plus, params):
+    a, pa = _f64(x)
+    P, pp = _f32(params)
+    return lib().lso_weno5_learned(pa, float(dx), int(plus), pp)
+
+
+def mlp_loss_grad(params, X, T, emulate=False, use_double=False):
+    """Loss and gradient (float64 array of 1347) over samples X (n x 8), T (n)."""
+    P, pp = _f32(params)
+    Xa, px = _f32(X)
+    Ta, pt = _f32(T)
+    g = np.zeros(MLP_NPARAM, np.float64)
+    loss = lib().lso_mlp_loss_grad(pp, len(Ta), px, pt, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   int(emulate), int(use_double))
+    return loss, g
+
+
+def bf16_round(x):
+    return np.array([lib().lso_bf16_round(float(v)) for v in np.ravel(x)], np.float32).reshape(np.shape(x))
 
 
 def weno3(x, dx, plus):

@@ -44,6 +44,7 @@ class SpatialSchemeEnum:
     STENCIL_LOCAL_LAX_FRIEDRICHS_1ST_ORDER = 9
     WENO_3RD_ORDER = 10
     WENO_5TH_ORDER = 11
+    WENO_5TH_ORDER_LEARNED = 12  # extension: learned weno5 weights (viennals_amd.learned)
 
 
 class TemporalSchemeEnum:
@@ -369,6 +370,12 @@ class Advect:
         check(lib().lsg_advect_get_point_updates(self._h, ctypes.byref(n)))
         return n.value
 
+    def setLearnedWeights(self, params):
+        """Network parameters (1347 floats) of WENO_5TH_ORDER_LEARNED."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        check(lib().lsg_advect_set_learned_weights(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                   int(p.size)))
+
     def setProfiling(self, on):
         check(lib().lsg_advect_set_profiling(self._h, int(bool(on))))
 
@@ -388,6 +395,22 @@ def weno(order, stencils, dx, plus):
     check(lib().lsg_weno_eval(int(order), n, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(dx),
                               int(bool(plus)), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
     return out
+
+
+def weno5_learned(stencils, dx, plus, params):
+    """Learned weno5 (network weights, plain weno5 at kinks) on the GPU."""
+    x = np.ascontiguousarray(stencils, dtype=np.float64)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    out = np.zeros(x.shape[0], dtype=np.float64)
+    check(lib().lsg_weno5_learned_eval(x.shape[0], x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(dx),
+                                       int(bool(plus)), p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def rebuild(domain):
+    """Advect::rebuildLS (lsAdvect.hpp:238-429) applied to a level set."""
+    check(lib().lsg_rebuild(domain._h))
 
 
 def synchronize():

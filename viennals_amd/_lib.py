@@ -30,6 +30,8 @@ EXPORTED = [
     "lsg_advect_apply", "lsg_advect_get_advected_time", "lsg_advect_get_current_time_step",
     "lsg_advect_get_number_of_time_steps", "lsg_advect_get_point_updates", "lsg_advect_set_profiling",
     "lsg_advect_get_rate_kernel_stats", "lsg_weno_eval", "lsg_device_synchronize",
+    "lsg_advect_set_learned_weights", "lsg_weno5_learned_eval", "lsg_mlp_create", "lsg_mlp_destroy",
+    "lsg_mlp_set_samples", "lsg_mlp_loss_grad", "lsg_mlp_train", "lsg_mlp_get_params",
 ]
 
 
@@ -48,6 +50,7 @@ def lib():
     i64p = ctypes.POINTER(ctypes.c_int64)
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
+    fp = ctypes.POINTER(ctypes.c_float)
     c_int, c_double = ctypes.c_int, ctypes.c_double
     sig = {
         "lsg_last_error": ([], ctypes.c_char_p),
@@ -93,6 +96,14 @@ def lib():
         "lsg_advect_set_profiling": ([P, c_int], c_int),
         "lsg_advect_get_rate_kernel_stats": ([P, dp, i64p, dp], c_int),
         "lsg_weno_eval": ([c_int, ctypes.c_int64, dp, c_double, c_int, dp], c_int),
+        "lsg_advect_set_learned_weights": ([P, fp, c_int], c_int),
+        "lsg_weno5_learned_eval": ([ctypes.c_int64, dp, c_double, c_int, fp, dp], c_int),
+        "lsg_mlp_create": ([ctypes.POINTER(P), fp], c_int),
+        "lsg_mlp_destroy": ([P], c_int),
+        "lsg_mlp_set_samples": ([P, ctypes.c_int64, fp, fp], c_int),
+        "lsg_mlp_loss_grad": ([P, ctypes.c_int64, ctypes.c_int64, dp, fp], c_int),
+        "lsg_mlp_train": ([P, c_int, ctypes.c_int64, c_double, dp], c_int),
+        "lsg_mlp_get_params": ([P, fp], c_int),
         "lsg_device_synchronize": ([], c_int),
     }
     for name, (args, res) in sig.items():

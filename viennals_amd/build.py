@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libviennals_gpu.so")
-SOURCES = ["lsg_kernels.hip", "lsg_host.hip", "lsg_geometry.cpp"]
+SOURCES = ["lsg_kernels.hip", "lsg_learned.hip", "lsg_host.hip", "lsg_geometry.cpp"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
     # bitwise parity with the CPU restatement: no FMA contraction

@@ -21,6 +21,15 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+
+// Full unrolling of the short per-dimension loops keeps the small local
+// arrays (stencils, neighbour values, coordinates) in registers.
+#if defined(__clang__)
+#define LSG_UNROLL _Pragma("unroll")
+#else
+#define LSG_UNROLL
+#endif
 
 namespace lsg {
 
@@ -157,7 +166,31 @@ struct RateParams {
   int velKind, velDir, velK;
   double velB, velAPos, velANeg;
   double vec[3];
+  // learned weno5 weights: network parameters as bf16 (WENO5 learned only)
+  const uint16_t *mlp;
 };
+
+// Learned weno5 network: 5 -> 32 -> 32 -> 3 (GELU, softmax); parameter
+// layout W1[32][5], b1[32], W2[32][32], b2[32], W3[3][32], b3[3].
+constexpr int MLP_IN = 5, MLP_H = 32, MLP_OUT = 3;
+constexpr int MLP_W1 = 0, MLP_B1 = 160, MLP_W2 = 192, MLP_B2 = 1216, MLP_W3 = 1248, MLP_B3 = 1344;
+constexpr int MLP_NPARAM = 1347;
+constexpr double MLP_KINK_RATIO = 10.0;
+
+// fp32 -> bf16 bits, round to nearest even (finite inputs; nan stays nan).
+LSG_HD inline uint16_t bf16Bits(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0x007fffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+LSG_HD inline float bf16Float(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
 
 // Pointwise rules of the band operations.
 enum OpKind : int {

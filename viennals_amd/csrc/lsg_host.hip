@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <set>
@@ -20,6 +21,7 @@
 #include "lsg_device.hpp"
 #include "lsg_geometry.hpp"
 #include "lsg_kernels.hpp"
+#include "lsg_learned.hpp"
 
 namespace lsg {
 
@@ -104,6 +106,7 @@ struct WsStore {
     (void)hipFree(w.cpos);
     (void)hipFree(w.out);
     (void)hipFree(w.ks);
+    (void)hipFree(w.keepBits);
     (void)hipFree(w.partials);
     (void)hipFree(w.ncand);
     w = Workspace{};
@@ -117,6 +120,7 @@ struct WsStore {
     LSG_HIP(hipMalloc(&w.cpos, ccap * sizeof(uint32_t)));
     LSG_HIP(hipMalloc(&w.out, ccap * sizeof(double)));
     LSG_HIP(hipMalloc(&w.ks, (ccap + 1) * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&w.keepBits, (ccap / 64 + 2) * sizeof(uint64_t)));
     LSG_HIP(hipMalloc(&w.partials, (SCAN_BLOCKS + 1) * sizeof(uint32_t)));
     LSG_HIP(hipMalloc(&w.ncand, sizeof(uint32_t)));
     w.ccap = ccap;
@@ -185,7 +189,15 @@ static int makeFrame(const lsg_domain *d, const int32_t *bbmin, const int32_t *b
   return 0;
 }
 
-static size_t capFor(size_t n) { return std::max<size_t>((size_t)1 << 20, n * 8 + 4096); }
+// Initial entry capacity of a band.  LSG_TEST_CAPACITY (tests only) sets a
+// small capacity to exercise the overflow-retry path.
+static size_t capFor(size_t n) {
+  if (const char *t = getenv("LSG_TEST_CAPACITY")) {
+    size_t c = (size_t)strtoull(t, nullptr, 10);
+    if (c > 0) return std::max<size_t>(c, n + 64);
+  }
+  return std::max<size_t>((size_t)1 << 20, n * 8 + 4096);
+}
 
 static int ensureStores(lsg_domain *d, size_t cap) {
   const int64_t nw = d->f.nwords;
@@ -524,15 +536,43 @@ struct lsg_advect {
   double *rates = nullptr;
   size_t ratesCap = 0;
   bool profiling = false;
-  std::vector<hipEvent_t> evs;
   double rateMs = 0, rateBytes = 0;
   int64_t rateLaunches = 0;
-  std::vector<uint32_t> rateN;  // band entries per profiled launch
+  // rate-kernel timing events (pairs), reused by every step
+  std::vector<hipEvent_t> evPool;
+  int evUsed = 0;
+  bool capturing = false;
+  // one captured graph per buffer parity of the step
+  struct StepGraph {
+    uint64_t key = 0;
+    hipGraphExec_t exec = nullptr;
+    lsg::Store dCur, dTmp, iCur, iTmp;
+    int dWidth = 0, iWidth = 0, evUsed = 0;
+  };
+  StepGraph graphs[2];
+  int nextGraph = 0;
+  bool useGraphs = true;
+  double *hostMaxDt = nullptr;  // pinned, read by the step's first kernel
+  double *devMaxDt = nullptr;
+  lsg::DevScalars *hostSc = nullptr;  // pinned
+  uint32_t *hostStat = nullptr;       // pinned
+  uint16_t *mlpDev = nullptr;         // learned weno5 parameters (bf16)
+  void dropGraphs() {
+    for (auto &g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = StepGraph{};
+    }
+  }
   ~lsg_advect() {
+    dropGraphs();
     if (init) delete init;
     if (save) delete save;
     if (rates) (void)hipFree(rates);
-    for (auto e : evs) (void)hipEventDestroy(e);
+    for (auto e : evPool) (void)hipEventDestroy(e);
+    if (hostMaxDt) (void)hipHostFree(hostMaxDt);
+    if (hostSc) (void)hipHostFree(hostSc);
+    if (hostStat) (void)hipHostFree(hostStat);
+    if (mlpDev) (void)hipFree(mlpDev);
   }
 };
 
@@ -550,6 +590,7 @@ static int prepareWidth(int spatial) {
   case 8: return 9;   // LLF2
   case 10: return 5;  // WENO3 (lsWENO.hpp:44)
   case 11: return 7;  // WENO5
+  case 12: return 7;  // WENO5 with learned weights (same stencil)
   default: return -1;
   }
 }
@@ -557,7 +598,7 @@ static int schemeOrder(int spatial) {
   switch (spatial) {
   case 1: case 3: case 6: case 8: return 2;
   case 10: return 3;
-  case 11: return 5;
+  case 11: case 12: return 5;
   default: return 1;
   }
 }
@@ -576,15 +617,14 @@ static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *d
   doExpand(d, prepareWidth(a->spatial));  // Advect::prepareLS (lsAdvect.hpp:973-1015)
   RateParams rp = a->rp;
   if (a->spatial == 2 || a->spatial == 3) launchGlobalAlphas(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, g_ctx.sc);
-  if (a->profiling) {
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    (void)hipEventRecord(e0, g_ctx.s);
+  if (a->profiling && a->evUsed + 2 <= (int)a->evPool.size()) {
+    hipEvent_t e0 = a->evPool[a->evUsed], e1 = a->evPool[a->evUsed + 1];
+    a->evUsed += 2;
+    if (a->capturing) (void)hipEventRecordWithFlags(e0, g_ctx.s, hipEventRecordExternal);
+    else (void)hipEventRecord(e0, g_ctx.s);
     launchRates(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, a->rates, g_ctx.sc, maxDt, dtOut);
-    (void)hipEventRecord(e1, g_ctx.s);
-    a->evs.push_back(e0);
-    a->evs.push_back(e1);
+    if (a->capturing) (void)hipEventRecordWithFlags(e1, g_ctx.s, hipEventRecordExternal);
+    else (void)hipEventRecord(e1, g_ctx.s);
   } else {
     launchRates(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, a->rates, g_ctx.sc, maxDt, dtOut);
   }
@@ -610,6 +650,17 @@ static void combine(lsg_advect *a, lsg_domain *d, lsg_domain *init, double wTarg
   doRebuild(d);
 }
 
+// Copy without allocation (dst storage must already fit): usable in capture.
+static void copyDomainNoAlloc(lsg_domain *dst, const lsg_domain *src) {
+  dst->D = src->D;
+  dst->g = src->g;
+  dst->f = src->f;
+  dst->framed = src->framed;
+  dst->width = src->width;
+  launchCopyBand(g_ctx.s, src->cur.b, dst->cur.b, src->f);
+  dst->hostN = src->hostN;
+}
+
 static int copyDomain(lsg_domain *dst, const lsg_domain *src) {
   dst->D = src->D;
   dst->g = src->g;
@@ -628,12 +679,12 @@ static void enqueueStep(lsg_advect *a, lsg_domain *d) {
   if (a->temporal == 0) {
     stageFE(a, d, &sc->maxTimeStep, &sc->dt[0]);
   } else if (a->temporal == 1) {  // evolveRungeKutta2 (lsAdvectIntegrationSchemes.hpp:61-108)
-    copyDomain(a->init, d);
+    copyDomainNoAlloc(a->init, d);
     stageFE(a, d, &sc->maxTimeStep, &sc->dt[0]);
     stageFE(a, d, &sc->dt[0], &sc->dt[1]);
     combine(a, d, a->init, 0.5, 0.5);
   } else {  // evolveRungeKutta3 (lsAdvectIntegrationSchemes.hpp:110-168)
-    copyDomain(a->init, d);
+    copyDomainNoAlloc(a->init, d);
     stageFE(a, d, &sc->maxTimeStep, &sc->dt[0]);
     stageFE(a, d, &sc->dt[0], &sc->dt[1]);
     combine(a, d, a->init, 0.75, 0.25);
@@ -648,30 +699,149 @@ struct StepOut {
   unsigned long long active;
 };
 
+static uint64_t fnv(uint64_t h, const void *p, size_t n) {
+  const unsigned char *c = (const unsigned char *)p;
+  for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  return h;
+}
+
+// Everything a captured step bakes into its kernel arguments.
+static uint64_t stepKey(lsg_advect *a, lsg_domain *d) {
+  uint64_t h = 1469598103934665603ull;
+  auto add = [&](const void *p, size_t n) { h = fnv(h, p, n); };
+  add(&d->cur.b, sizeof(BandBufs));
+  add(&d->tmp.b, sizeof(BandBufs));
+  add(&d->f, sizeof(Frame));
+  add(&d->width, sizeof(int));
+  if (a->init) {
+    add(&a->init->cur.b, sizeof(BandBufs));
+    add(&a->init->tmp.b, sizeof(BandBufs));
+  }
+  add(&a->save->cur.b, sizeof(BandBufs));
+  add(&g_ws->w, sizeof(Workspace));
+  add(&a->rp, sizeof(RateParams));
+  add(&a->temporal, sizeof(int));
+  add(&a->cfl, sizeof(double));
+  add(&a->rates, sizeof(double *));
+  add(&a->profiling, sizeof(bool));
+  return h;
+}
+
+// The device work of one step: prologue (reset flags, take maxTimeStep,
+// keep a copy for retries), the stages, the returned time, band statistics.
+static void enqueueStepBody(lsg_advect *a, lsg_domain *d) {
+  launchStepBegin(g_ctx.s, g_ctx.sc, a->devMaxDt);
+  copyDomainNoAlloc(a->save, d);
+  a->evUsed = 0;
+  enqueueStep(a, d);
+  launchBandStats(g_ctx.s, d->cur.b, d->f, g_ctx.stat, (int32_t *)(g_ctx.stat + 1));
+}
+
 static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &out) {
   for (int attempt = 0; attempt < 6; ++attempt) {
-    // keep a copy of the band to retry after a capacity overflow
+    // allocation happens here, never inside a capture
     if (!a->save) a->save = new lsg_domain();
-    lsg_domain *save = a->save;
-    if (int e = copyDomain(save, d)) return e;
+    if (int e = ensureCur(a->save, d->cur.nwAlloc, d->cur.b.cap)) return e;
+    if (a->init) {
+      if (int e = ensureCur(a->init, d->cur.nwAlloc, d->cur.b.cap)) return e;
+      a->init->f = d->f;
+      if (int e = ensureStores(a->init, d->cur.b.cap)) return e;
+    }
+    if (int e = ensureStores(d, d->cur.b.cap)) return e;
     int widthBefore = d->width;
-    LSG_HIP(hipMemcpyAsync(&g_ctx.sc->maxTimeStep, &maxTimeStep, sizeof(double), hipMemcpyHostToDevice, g_ctx.s));
-    LSG_HIP(hipMemsetAsync(&g_ctx.sc->overflow, 0, sizeof(uint32_t), g_ctx.s));
-    unsigned long long act0 = 0;
-    LSG_HIP(hipMemcpyAsync(&act0, &g_ctx.sc->totalActive, sizeof(act0), hipMemcpyDeviceToHost, g_ctx.s));
-    enqueueStep(a, d);
-    DevScalars h;
-    LSG_HIP(hipMemcpyAsync(&h, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
+    *a->hostMaxDt = maxTimeStep;
+    bool launched = false;
+    if (a->useGraphs) {
+      uint64_t key = stepKey(a, d);
+      lsg_advect::StepGraph *g = nullptr;
+      for (auto &c : a->graphs)
+        if (c.exec && c.key == key) g = &c;
+      if (!g) {
+        g = &a->graphs[a->nextGraph];
+        a->nextGraph ^= 1;
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        *g = lsg_advect::StepGraph{};
+        hipGraph_t graph = nullptr;
+        bool ok = hipStreamBeginCapture(g_ctx.s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+          a->capturing = true;
+          enqueueStepBody(a, d);
+          a->capturing = false;
+          ok = hipStreamEndCapture(g_ctx.s, &graph) == hipSuccess && graph;
+        }
+        if (ok) ok = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!ok) {
+          // capture unsupported here: fall back to direct launches for good
+          (void)hipGetLastError();
+          a->useGraphs = false;
+          a->dropGraphs();
+          return fail(-31, "HIP graph capture failed; retry the call (direct launches are used from now on)");
+        }
+        g->key = key;
+        g->dCur = d->cur;
+        g->dTmp = d->tmp;
+        if (a->init) {
+          g->iCur = a->init->cur;
+          g->iTmp = a->init->tmp;
+          g->iWidth = a->init->width;
+        }
+        g->dWidth = d->width;
+        g->evUsed = a->evUsed;
+      } else {
+        // replay: the host bookkeeping takes the state the captured step leaves
+        d->cur = g->dCur;
+        d->tmp = g->dTmp;
+        d->width = g->dWidth;
+        if (a->init) {
+          a->init->cur = g->iCur;
+          a->init->tmp = g->iTmp;
+          a->init->width = g->iWidth;
+          a->init->f = d->f;
+          a->init->framed = true;
+        }
+        a->evUsed = g->evUsed;
+      }
+      LSG_HIP(hipGraphLaunch(g->exec, g_ctx.s));
+      launched = true;
+    }
+    if (!launched) enqueueStepBody(a, d);
+    LSG_HIP(hipMemcpyAsync(a->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
+    LSG_HIP(hipMemcpyAsync(a->hostStat, g_ctx.stat, 7 * sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
     LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    const DevScalars &h = *a->hostSc;
     if (!h.overflow) {
       out.stepTime = h.stepTime;
       out.dt1 = h.dt[0];
-      out.active = h.totalActive - act0;
-      return maybeReframe(d, false);
+      out.active = h.totalActive - h.stepActive0;
+      if (a->profiling) {
+        for (int k = 0; k + 1 < a->evUsed; k += 2) {
+          float ms = 0;
+          if (hipEventElapsedTime(&ms, a->evPool[k], a->evPool[k + 1]) == hipSuccess) {
+            a->rateMs += ms;
+            a->rateLaunches += 1;
+          }
+        }
+      }
+      // re-frame when the band approaches the frame edge (statistics of this step)
+      const int32_t *bb = (const int32_t *)(a->hostStat + 1);
+      bool need = false;
+      if (bb[0] <= bb[3]) {
+        for (int k = 0; k < d->D; ++k) {
+          if (d->g.bounded(k)) continue;
+          if (bb[k] < REFRAME_SLACK || (d->f.ext[k] - 1) - bb[3 + k] < REFRAME_SLACK) need = true;
+        }
+      }
+      if (need) {
+        a->dropGraphs();
+        return maybeReframe(d, true);
+      }
+      return 0;
     }
     // grow and retry from the saved band
+    a->dropGraphs();
     size_t cap = d->cur.b.cap * 2;
-    if (int e = copyDomain(d, save)) return e;
+    if (int e = copyDomain(d, a->save)) return e;
     d->width = widthBefore;
     if (int e = ensureStores(d, cap)) return e;
     if (a->init && a->init->framed && (a->init->cur.b.cap < cap))
@@ -700,7 +870,10 @@ static int advectApply(lsg_advect *a) {
   a->rp.alphaFactor = a->dissipationAlpha;
   a->rp.checkDissipation = a->checkDissipation;
   // WENO receives dissipationAlpha as its calcNormal flag (lsAdvect.hpp:670-676)
-  a->rp.calcNormals = (a->spatial == 10 || a->spatial == 11) ? (a->dissipationAlpha != 0.0) : a->calcNormals;
+  a->rp.calcNormals = (a->spatial >= 10) ? (a->dissipationAlpha != 0.0) : a->calcNormals;
+  if (a->spatial == 12 && !a->mlpDev)
+    return fail(-8, "Advect: learned WENO5 selected but no network parameters were set");
+  a->rp.mlp = a->mlpDev;
   if (int e = ensureStores(d, d->cur.b.cap)) return e;
   if (int e = ensureRates(a, d->cur.b.cap)) return e;
   if (a->temporal != 0) {
@@ -708,8 +881,17 @@ static int advectApply(lsg_advect *a) {
     if (int e = copyDomain(a->init, d)) return e;
     if (int e = ensureStores(a->init, d->cur.b.cap)) return e;
   }
-  for (auto e : a->evs) (void)hipEventDestroy(e);
-  a->evs.clear();
+  if (!a->hostMaxDt) {
+    LSG_HIP(hipHostMalloc((void **)&a->hostMaxDt, sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    LSG_HIP(hipHostGetDevicePointer((void **)&a->devMaxDt, a->hostMaxDt, 0));
+    LSG_HIP(hipHostMalloc((void **)&a->hostSc, sizeof(DevScalars), hipHostMallocDefault));
+    LSG_HIP(hipHostMalloc((void **)&a->hostStat, 8 * sizeof(uint32_t), hipHostMallocDefault));
+  }
+  if (a->profiling && a->evPool.empty()) {
+    a->evPool.resize(8);
+    for (auto &e : a->evPool) LSG_HIP(hipEventCreate(&e));
+  }
+  if (const char *ng = getenv("LSG_NO_GRAPH")) a->useGraphs = (ng[0] == '0');
   a->pointUpdates = 0;
   StepOut so{};
   if (a->advectionTime == 0.) {
@@ -732,20 +914,7 @@ static int advectApply(lsg_advect *a) {
   }
   a->currentTimeStep = so.dt1;
   if (a->temporal == 0) a->currentTimeStep = so.stepTime;
-  if (a->profiling) {
-    LSG_HIP(hipStreamSynchronize(g_ctx.s));
-    double rb = 0;
-    LSG_HIP(hipMemcpy(&rb, &g_ctx.sc->rateBytes, sizeof(double), hipMemcpyDeviceToHost));
-    a->rateBytes = rb;
-    for (size_t i = 0; i + 1 < a->evs.size(); i += 2) {
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, a->evs[i], a->evs[i + 1]);
-      a->rateMs += ms;
-      a->rateLaunches += 1;
-    }
-    for (auto e : a->evs) (void)hipEventDestroy(e);
-    a->evs.clear();
-  }
+  if (a->profiling) a->rateBytes = a->hostSc->rateBytes;
   return 0;
 }
 
@@ -952,8 +1121,17 @@ int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, dou
   return 0;
 }
 int lsg_advect_set_spatial_scheme(lsg_advect *a, int s) {
-  if (s < 0 || s > 11) return fail(-1, "Advect: Discretization scheme not found.");
+  if (s < 0 || s > 12) return fail(-1, "Advect: Discretization scheme not found.");
   a->spatial = s;
+  return 0;
+}
+int lsg_advect_set_learned_weights(lsg_advect *a, const float *params, int n) {
+  if (n != lsg::MLP_NPARAM) return fail(-1, "learned weno5: expected 1347 network parameters");
+  if (int e = lsg::ctxInit()) return e;
+  std::vector<uint16_t> b(n);
+  for (int i = 0; i < n; ++i) b[i] = lsg::bf16Bits(params[i]);
+  if (!a->mlpDev) LSG_HIP(hipMalloc(&a->mlpDev, n * sizeof(uint16_t)));
+  LSG_HIP(hipMemcpy(a->mlpDev, b.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice));
   return 0;
 }
 int lsg_advect_set_temporal_scheme(lsg_advect *a, int s) {
@@ -1017,6 +1195,7 @@ int lsg_advect_get_point_updates(lsg_advect *a, double *n) {
   return 0;
 }
 int lsg_advect_set_profiling(lsg_advect *a, int on) {
+  a->dropGraphs();
   if (int e = ctxInit()) return e;
   double z = 0;
   LSG_HIP(hipMemcpy(&g_ctx.sc->rateBytes, &z, sizeof(double), hipMemcpyHostToDevice));
@@ -1046,6 +1225,129 @@ int lsg_weno_eval(int order, int64_t n, const double *x, double dx, int plus, do
   LSG_HIP(hipStreamSynchronize(g_ctx.s));
   (void)hipFree(dx_);
   (void)hipFree(dout);
+  return 0;
+}
+
+// ---- learned weno5: network training ----------------------------------------
+
+struct lsg_mlp {
+  float *P = nullptr, *M = nullptr, *V = nullptr, *G = nullptr, *partials = nullptr;
+  float *X = nullptr, *T = nullptr;
+  int64_t n = 0, cap = 0;
+  int rows = 0;
+  int step = 0;
+  ~lsg_mlp() {
+    for (float *p : {P, M, V, G, partials, X, T})
+      if (p) (void)hipFree(p);
+  }
+};
+
+int lsg_mlp_create(lsg_mlp **out, const float *params) {
+  using namespace lsg;
+  if (!out || !params) return fail(-1, "lsg_mlp_create: null argument");
+  if (int e = ctxInit()) return e;
+  auto *m = new lsg_mlp();
+  m->rows = 1024;
+  const size_t pb = MLP_PSTRIDE * sizeof(float);
+  if (hipMalloc(&m->P, pb) != hipSuccess || hipMalloc(&m->M, pb) != hipSuccess || hipMalloc(&m->V, pb) != hipSuccess ||
+      hipMalloc(&m->G, pb) != hipSuccess || hipMalloc(&m->partials, pb * m->rows) != hipSuccess) {
+    delete m;
+    return fail(-20, "lsg_mlp_create: out of device memory");
+  }
+  LSG_HIP(hipMemcpy(m->P, params, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  LSG_HIP(hipMemset(m->M, 0, pb));
+  LSG_HIP(hipMemset(m->V, 0, pb));
+  *out = m;
+  return 0;
+}
+
+int lsg_mlp_destroy(lsg_mlp *m) {
+  delete m;
+  return 0;
+}
+
+int lsg_mlp_set_samples(lsg_mlp *m, int64_t n, const float *X, const float *T) {
+  if (n <= 0 || n > (int64_t)UINT32_MAX / 8) return fail(-1, "lsg_mlp_set_samples: sample count out of range");
+  if (n > m->cap) {
+    if (m->X) (void)hipFree(m->X);
+    if (m->T) (void)hipFree(m->T);
+    m->X = m->T = nullptr;
+    m->cap = 0;
+    LSG_HIP(hipMalloc(&m->X, (size_t)n * 8 * sizeof(float)));
+    LSG_HIP(hipMalloc(&m->T, (size_t)n * sizeof(float)));
+    m->cap = n;
+  }
+  LSG_HIP(hipMemcpy(m->X, X, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice));
+  LSG_HIP(hipMemcpy(m->T, T, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  m->n = n;
+  return 0;
+}
+
+static int mlpRows(int64_t count, int maxRows) {
+  int64_t r = (count + 63) / 64;
+  if (r > maxRows) r = maxRows;
+  return r < 1 ? 1 : (int)r;
+}
+
+int lsg_mlp_loss_grad(lsg_mlp *m, int64_t offset, int64_t count, double *loss, float *grad) {
+  using namespace lsg;
+  if (offset < 0 || count <= 0 || offset + count > m->n) return fail(-1, "lsg_mlp_loss_grad: sample range out of bounds");
+  launchMlpGrad(g_ctx.s, m->X + offset * 8, m->T + offset, (uint32_t)count, m->P, m->partials,
+                mlpRows(count, m->rows), m->G);
+  float h[MLP_PSTRIDE];
+  LSG_HIP(hipMemcpyAsync(h, m->G, sizeof(h), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  if (grad) memcpy(grad, h, MLP_NPARAM * sizeof(float));
+  if (loss) *loss = (double)h[MLP_NPARAM] / (double)count;
+  return 0;
+}
+
+int lsg_mlp_train(lsg_mlp *m, int iters, int64_t batch, double lr, double *lastLoss) {
+  using namespace lsg;
+  if (m->n <= 0) return fail(-1, "lsg_mlp_train: no samples");
+  if (batch <= 0 || batch > m->n) batch = m->n;
+  int64_t off = 0;
+  for (int it = 0; it < iters; ++it) {
+    if (off + batch > m->n) off = 0;
+    launchMlpGrad(g_ctx.s, m->X + off * 8, m->T + off, (uint32_t)batch, m->P, m->partials, mlpRows(batch, m->rows),
+                  m->G);
+    launchMlpAdam(g_ctx.s, m->P, m->M, m->V, m->G, ++m->step, (float)lr);
+    off += batch;
+  }
+  if (lastLoss) {
+    float l;
+    LSG_HIP(hipMemcpyAsync(&l, m->G + MLP_NPARAM, sizeof(float), hipMemcpyDeviceToHost, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    *lastLoss = (double)l / (double)batch;
+  }
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  return 0;
+}
+
+int lsg_mlp_get_params(lsg_mlp *m, float *out) {
+  LSG_HIP(hipMemcpy(out, m->P, lsg::MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int lsg_weno5_learned_eval(int64_t n, const double *x, double dx, int plus, const float *params, double *out) {
+  using namespace lsg;
+  if (n <= 0 || n > (int64_t)UINT32_MAX) return fail(-1, "lsg_weno5_learned_eval: n out of range");
+  if (int e = ctxInit()) return e;
+  std::vector<uint16_t> b(MLP_NPARAM);
+  for (int i = 0; i < MLP_NPARAM; ++i) b[i] = bf16Bits(params[i]);
+  double *dx_ = nullptr, *dout = nullptr;
+  uint16_t *dp = nullptr;
+  LSG_HIP(hipMalloc(&dx_, (size_t)n * 7 * sizeof(double)));
+  LSG_HIP(hipMalloc(&dout, (size_t)n * sizeof(double)));
+  LSG_HIP(hipMalloc(&dp, MLP_NPARAM * sizeof(uint16_t)));
+  LSG_HIP(hipMemcpyAsync(dx_, x, (size_t)n * 7 * sizeof(double), hipMemcpyHostToDevice, g_ctx.s));
+  LSG_HIP(hipMemcpyAsync(dp, b.data(), MLP_NPARAM * sizeof(uint16_t), hipMemcpyHostToDevice, g_ctx.s));
+  launchWenoLearned(g_ctx.s, (uint32_t)n, dx_, dx, plus, dp, dout);
+  LSG_HIP(hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  (void)hipFree(dx_);
+  (void)hipFree(dout);
+  (void)hipFree(dp);
   return 0;
 }
 

@@ -23,6 +23,7 @@ template <int D> LSG_HD inline double ruleExpand(double c, const double *nb, dou
   if (fabs(c) <= totalLimit) return c;
   if (c > -DEPS) {
     double dist = POS;
+    LSG_UNROLL
     for (int i = 0; i < 2 * D; i++) {
       double nv = nb[i] + 1.0;
       if (dist > nv) dist = nv;
@@ -30,6 +31,7 @@ template <int D> LSG_HD inline double ruleExpand(double c, const double *nb, dou
     return (dist <= limit) ? dist : POS;
   }
   double dist = NEG;
+  LSG_UNROLL
   for (int i = 0; i < 2 * D; i++) {
     double nv = nb[i] - 1.0;
     if (dist < nv) dist = nv;
@@ -41,16 +43,19 @@ template <int D> LSG_HD inline double ruleExpand(double c, const double *nb, dou
 template <int D> LSG_HD inline double ruleRebuild(double c, const double *nb, double cutoff) {
   if (fabs(c) <= 1.0) {
     int k = 0;
+    LSG_UNROLL
     for (; k < 2 * D; k++)
       if (signbit(nb[k] - 1e-7) != signbit(c + 1e-7)) break;
     if (k != 2 * D) {
       if (c > 0.5) {
         int j = 0;
+        LSG_UNROLL
         for (; j < 2 * D; j++)
           if (fabs(nb[j]) <= 1.0 && nb[j] < -0.5) break;
         return (j == 2 * D) ? c : 0.5;
       } else if (c < -0.5) {
         int j = 0;
+        LSG_UNROLL
         for (; j < 2 * D; j++)
           if (fabs(nb[j]) <= 1.0 && nb[j] > 0.5) break;
         return (j == 2 * D) ? c : -0.5;
@@ -61,6 +66,7 @@ template <int D> LSG_HD inline double ruleRebuild(double c, const double *nb, do
   }
   if (c >= 0) {
     double dist = POS;
+    LSG_UNROLL
     for (int i = 0; i < 2 * D; i++) {
       double v = nb[i];
       if (fabs(v) <= 1.0 && (v < 0.)) {
@@ -70,6 +76,7 @@ template <int D> LSG_HD inline double ruleRebuild(double c, const double *nb, do
     return (dist <= cutoff) ? dist : POS;
   }
   double dist = NEG;
+  LSG_UNROLL
   for (int i = 0; i < 2 * D; i++) {
     double v = nb[i];
     if (fabs(v) <= 1.0 && (v > 0)) {
@@ -94,6 +101,7 @@ template <int D> LSG_HD inline double rulePrune(double c, const double *nb, bool
     bool keep = true;
     if (fabs(c) != 0.) {
       keep = false;
+      LSG_UNROLL
       for (int i = 0; i < 2 * D; ++i) {
         if (rsz ? pSignDiffOrZero(c, nb[i]) : pSignDiff(c, nb[i])) {
           keep = true;
@@ -102,6 +110,7 @@ template <int D> LSG_HD inline double rulePrune(double c, const double *nb, bool
       }
     }
     if (rsz && fabs(c) == 0.) {
+      LSG_UNROLL
       for (int i = 0; i < D; ++i) {
         double negVal = nb[i];
         double posVal = nb[D + i];
@@ -147,6 +156,7 @@ LSG_HD inline double ruleUpdate(double c, double dt, double velocity) {
 
 template <int D> LSG_HD inline void gatherStar(const BandView &b, const int32_t c[3], double &center, double *nb) {
   center = b.value(c[0], c[1], c[2]);
+  LSG_UNROLL
   for (int d = 0; d < D; ++d) {
     nb[d] = b.at(c, d, 1);
     nb[D + d] = b.at(c, d, -1);
@@ -195,7 +205,9 @@ template <int D> LSG_HD inline uint64_t candStarWord(const uint64_t *bits, const
     if (first & 1ull) dil |= 1ull << (lastBit & 63);
   }
   const int32_t ca[3] = {0, yr + f.o[1], zr + f.o[2]};
+  LSG_UNROLL
   for (int d = 1; d < D; ++d) {
+    LSG_UNROLL
     for (int s = -1; s <= 1; s += 2) {
       int32_t m = g.map(d, (int64_t)ca[d] + s);
       int32_t rr = m - f.o[d];
@@ -260,6 +272,7 @@ struct Vel {
     double m = 1.0;
     if (k >= 1) {
       m = ac;
+      LSG_UNROLL
       for (int j = 1; j < k; ++j) m = m * ac;
     }
     return b + a * m;
@@ -342,8 +355,10 @@ LSG_HD inline double weno5(const double *x, double dx, bool plus) {
   const double eps = 1e-6;
   double d[5];
   if (!plus) {
+    LSG_UNROLL
     for (int i = 0; i < 5; ++i) d[i] = x[i + 1] - x[i];
   } else {
+    LSG_UNROLL
     for (int i = 0; i < 5; ++i) d[i] = x[6 - i] - x[5 - i];
   }
   double s1 = (13.0 / 12.0) * pow2(d[0] - 2 * d[1] + d[2]) + (1.0 / 4.0) * pow2(d[0] - 4 * d[1] + 3 * d[2]);
@@ -360,6 +375,78 @@ LSG_HD inline double weno5(const double *x, double dx, bool plus) {
   return (al1 * p1 + al2 * p2 + al3 * p3) / (alsum * dx);
 }
 
+// ------------------------------------------------- learned weno5 weights ----
+// Extension of weno5: where the regularised smoothness indicators
+// (eps dx^2 + beta_k) agree within a factor MLP_KINK_RATIO, the weights of
+// the three candidate polynomials come from a small network fed with the
+// differences d / max|d|; elsewhere (kinks) plain weno5 is used.  P holds
+// the parameters as floats (bf16 values); fp32 accumulation, bias first.
+LSG_HD inline float mlpGelu(float x) {
+  const float c = 0.7978845608028654f, a = 0.044715f;
+  const float u = c * (x + a * (x * x) * x);
+  return 0.5f * x * (1.0f + tanhf(u));
+}
+
+LSG_HD inline void mlpForward(const float *P, const float in[MLP_IN], float w[MLP_OUT]) {
+  float h1[MLP_H];
+  LSG_UNROLL
+  for (int j = 0; j < MLP_H; ++j) {
+    float z = P[MLP_B1 + j];
+    LSG_UNROLL
+    for (int i = 0; i < MLP_IN; ++i) z += P[MLP_W1 + j * MLP_IN + i] * in[i];
+    h1[j] = mlpGelu(z);
+  }
+  float z3[MLP_OUT] = {P[MLP_B3], P[MLP_B3 + 1], P[MLP_B3 + 2]};
+  for (int j = 0; j < MLP_H; ++j) {
+    float z = P[MLP_B2 + j];
+    LSG_UNROLL
+    for (int i = 0; i < MLP_H; ++i) z += P[MLP_W2 + j * MLP_H + i] * h1[i];
+    const float h2 = mlpGelu(z);
+    z3[0] += P[MLP_W3 + j] * h2;
+    z3[1] += P[MLP_W3 + MLP_H + j] * h2;
+    z3[2] += P[MLP_W3 + 2 * MLP_H + j] * h2;
+  }
+  const float m = fmaxf(z3[0], fmaxf(z3[1], z3[2]));
+  const float e0 = expf(z3[0] - m), e1 = expf(z3[1] - m), e2 = expf(z3[2] - m);
+  const float sum = (e0 + e1) + e2;
+  w[0] = e0 / sum;
+  w[1] = e1 / sum;
+  w[2] = e2 / sum;
+}
+
+// Out of line: six calls per point would otherwise inline six copies of the
+// network and exhaust the register file.
+LSG_HD inline __attribute__((noinline)) double weno5Learned(const double *x, double dx, bool plus, const float *P) {
+  double d[5];
+  if (!plus) {
+    LSG_UNROLL
+    for (int i = 0; i < 5; ++i) d[i] = x[i + 1] - x[i];
+  } else {
+    LSG_UNROLL
+    for (int i = 0; i < 5; ++i) d[i] = x[6 - i] - x[5 - i];
+  }
+  const double s1 = (13.0 / 12.0) * pow2(d[0] - 2 * d[1] + d[2]) + (1.0 / 4.0) * pow2(d[0] - 4 * d[1] + 3 * d[2]);
+  const double s2 = (13.0 / 12.0) * pow2(d[1] - 2 * d[2] + d[3]) + (1.0 / 4.0) * pow2(d[1] - d[3]);
+  const double s3 = (13.0 / 12.0) * pow2(d[2] - 2 * d[3] + d[4]) + (1.0 / 4.0) * pow2(3 * d[2] - 4 * d[3] + d[4]);
+  const double e = 1e-6 * dx * dx;
+  const double a1 = e + s1, a2 = e + s2, a3 = e + s3;
+  const double mx = dmaxs(a1, dmaxs(a2, a3));
+  const double mn = dmins(a1, dmins(a2, a3));
+  double m = 0;
+  LSG_UNROLL
+  for (int i = 0; i < 5; ++i) m = dmaxs(m, fabs(d[i]));
+  if (m == 0. || mx > MLP_KINK_RATIO * mn) return weno5(x, dx, plus);
+  float in[MLP_IN];
+  LSG_UNROLL
+  for (int i = 0; i < 5; ++i) in[i] = (float)(d[i] / m);
+  float w[MLP_OUT];
+  mlpForward(P, in, w);
+  const double p1 = (1.0 / 6.0) * (2 * d[0] - 7 * d[1] + 11 * d[2]);
+  const double p2 = (1.0 / 6.0) * (-d[1] + 5 * d[2] + 2 * d[3]);
+  const double p3 = (1.0 / 6.0) * (2 * d[2] + 5 * d[3] - d[4]);
+  return ((double)w[0] * p1 + (double)w[1] * p2 + (double)w[2] * p3) / dx;
+}
+
 struct RateOut {
   double H, Dn;
 };
@@ -370,6 +457,7 @@ LSG_HD inline RateOut schemeEO(const BandView &b, const int32_t c[3], double phi
                                const Vel &vel) {
   double gradPos[D], gradNeg[D];
   double gradPosTotal = 0, gradNegTotal = 0;
+  LSG_UNROLL
   for (int i = 0; i < D; i++) {
     double diffPos, diffNeg;
     upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
@@ -382,6 +470,7 @@ LSG_HD inline RateOut schemeEO(const BandView &b, const int32_t c[3], double phi
   double n[3] = {0, 0, 0};
   if (calcNormals) {  // no zero guard (lsEngquistOsher.hpp:124-138)
     double denominator = 0;
+    LSG_UNROLL
     for (int i = 0; i < D; i++) {
       double pos = b.at(c, i, 1) - phi0;
       double neg = phi0 - b.at(c, i, -1);
@@ -389,11 +478,13 @@ LSG_HD inline RateOut schemeEO(const BandView &b, const int32_t c[3], double phi
       denominator += n[i] * n[i];
     }
     denominator = 1. / sqrt(denominator);
+    LSG_UNROLL
     for (int i = 0; i < D; ++i) n[i] *= denominator;
   }
   double sv = vel.scalar(n);
   if (sv > 0) vel_grad += sqrt(gradPosTotal) * sv;
   else vel_grad += sqrt(gradNegTotal) * sv;
+  LSG_UNROLL
   for (int w = 0; w < D; w++) {
     if (vel.vec[w] > 0.) vel_grad += vel.vec[w] * gradPos[w];
     else vel_grad += vel.vec[w] * gradNeg[w];
@@ -402,14 +493,17 @@ LSG_HD inline RateOut schemeEO(const BandView &b, const int32_t c[3], double phi
 }
 
 // WENO::operator() (lsWENO.hpp:53-161)
-template <int D, int ORDER>
-LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double phi0, bool calcNormals, const Vel &vel) {
+template <int D, int ORDER, bool LEARNED = false>
+LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double phi0, bool calcNormals, const Vel &vel,
+                                 const float *mlp = nullptr) {
   constexpr int r = (ORDER + 1) / 2;
   double gradPosTotal = 0, gradNegTotal = 0;
   double gm[D], gp[D];
   double st[2 * r + 1];
+  LSG_UNROLL
   for (int i = 0; i < D; i++) {
     st[r] = phi0;
+    LSG_UNROLL
     for (int k = 1; k <= r; ++k) {
       st[r + k] = b.at(c, i, k);
       st[r - k] = b.at(c, i, -k);
@@ -417,6 +511,9 @@ LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double p
     if (ORDER == 3) {
       gm[i] = weno3(st, b.g.dx, false);
       gp[i] = weno3(st, b.g.dx, true);
+    } else if (LEARNED) {
+      gm[i] = weno5Learned(st, b.g.dx, false, mlp);
+      gp[i] = weno5Learned(st, b.g.dx, true, mlp);
     } else {
       gm[i] = weno5(st, b.g.dx, false);
       gp[i] = weno5(st, b.g.dx, true);
@@ -428,18 +525,21 @@ LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double p
   double n[3] = {0, 0, 0};
   if (calcNormals) {  // central difference with zero guard (lsWENO.hpp:113-129)
     double denominator = 0;
+    LSG_UNROLL
     for (int i = 0; i < D; i++) {
       n[i] = b.at(c, i, 1) - b.at(c, i, -1);
       denominator += n[i] * n[i];
     }
     if (denominator > 0) {
       denominator = 1. / sqrt(denominator);
+      LSG_UNROLL
       for (int i = 0; i < D; ++i) n[i] *= denominator;
     }
   }
   double sv = vel.scalar(n);
   if (sv > 0) vel_grad += sqrt(gradPosTotal) * sv;
   else vel_grad += sqrt(gradNegTotal) * sv;
+  LSG_UNROLL
   for (int w = 0; w < D; w++) {
     if (vel.vec[w] > 0.) vel_grad += vel.vec[w] * gm[w];
     else vel_grad += vel.vec[w] * gp[w];
@@ -455,6 +555,7 @@ LSG_HD inline RateOut schemeLF(const BandView &b, const int32_t c[3], double phi
   double grad = 0., dissipation = 0.;
   double n[3] = {0, 0, 0};
   double normalModulus = 0;
+  LSG_UNROLL
   for (int i = 0; i < D; i++) {
     double diffPos, diffNeg;
     upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
@@ -469,11 +570,13 @@ LSG_HD inline RateOut schemeLF(const BandView &b, const int32_t c[3], double phi
   }
   if (calcNormals) {
     normalModulus = 1. / sqrt(normalModulus);
+    LSG_UNROLL
     for (int i = 0; i < D; ++i) n[i] *= normalModulus;
   }
   double sv = vel.scalar(n);
   double totalGrad = 0.;
   if (sv != 0.) totalGrad = sv * sqrt(grad);
+  LSG_UNROLL
   for (int w = 0; w < D; w++) {
     if (vel.vec[w] > 0.) totalGrad += vel.vec[w] * gradPos[w];
     else totalGrad += vel.vec[w] * gradNeg[w];
@@ -489,6 +592,7 @@ LSG_HD inline RateOut schemeLLLF(const BandView &b, const int32_t c[3], double p
   double grad = 0., dissipation = 0.;
   double n[3] = {0, 0, 0};
   double normalModulus = 0;
+  LSG_UNROLL
   for (int i = 0; i < D; i++) {
     double diffPos, diffNeg;
     upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
@@ -499,15 +603,18 @@ LSG_HD inline RateOut schemeLLLF(const BandView &b, const int32_t c[3], double p
     grad += pow2((diffNeg + diffPos) * 0.5);
   }
   normalModulus = 1. / sqrt(normalModulus);
+  LSG_UNROLL
   for (int i = 0; i < D; ++i) n[i] *= normalModulus;
   double sv = vel.scalar(n);
   double totalGrad = 0.;
   if (sv != 0.) totalGrad = sv * sqrt(grad);
+  LSG_UNROLL
   for (int w = 0; w < D; w++) {
     if (vel.vec[w] > 0.) totalGrad += vel.vec[w] * gradPos[w];
     else totalGrad += vel.vec[w] * gradNeg[w];
   }
   if (totalGrad == 0.) return {0., 0.};
+  LSG_UNROLL
   for (int i = 0; i < D; ++i) {
     double alpha = fabs((sv + vel.vec[i]) * n[i]);
     alphaAcc[i] = dmaxs(alphaAcc[i], alpha);
@@ -524,6 +631,7 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
   double grad = 0., dissipation = 0.;
   double n[3] = {0, 0, 0};
   double normalModulus = 0;
+  LSG_UNROLL
   for (int i = 0; i < D; i++) {
     double diffPos, diffNeg;
     upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
@@ -534,27 +642,34 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
     grad += pow2((diffNeg + diffPos) * 0.5);
   }
   normalModulus = 1. / sqrt(normalModulus);
+  LSG_UNROLL
   for (int i = 0; i < D; ++i) n[i] *= normalModulus;
   double sv = vel.scalar(n);
   double totalGrad = 0.;
   if (sv != 0.) totalGrad = sv * sqrt(grad);
+  LSG_UNROLL
   for (int w = 0; w < D; w++) {
     if (vel.vec[w] > 0.) totalGrad += vel.vec[w] * gradPos[w];
     else totalGrad += vel.vec[w] * gradNeg[w];
   }
   if (totalGrad == 0.) return {0., 0.};
   double alpha[D];
+  LSG_UNROLL
   for (int i = 0; i < D; ++i) alpha[i] = 0;
-  int ni[3] = {-1, -1, -1};
   constexpr int numNeighbors = (D == 3) ? 27 : 9;
-  for (int q = 0; q < numNeighbors; ++q) {  // box loop (lsLocalLaxFriedrichs.hpp:193-231)
+  LSG_UNROLL
+  for (int q = 0; q < numNeighbors; ++q) {  // box loop (lsLocalLaxFriedrichs.hpp:193-231), x fastest
+    const int ni[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
     int32_t nc[3] = {0, 0, 0};
+    LSG_UNROLL
     for (int d = 0; d < D; ++d) nc[d] = b.g.map(d, (int64_t)c[d] + ni[d]);
     double normal[3] = {0, 0, 0};
     double nm = 0.;
     const double center = b.value(nc[0], nc[1], nc[2]);
+    LSG_UNROLL
     for (int dir = 0; dir < D; ++dir) {
       int32_t pp[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+      LSG_UNROLL
       for (int d = 0; d < D; ++d) {
         pp[d] = b.g.map(d, (int64_t)c[d] + ni[d] + (d == dir ? 1 : 0));
         pn[d] = b.g.map(d, (int64_t)c[d] + ni[d] - (d == dir ? 1 : 0));
@@ -567,26 +682,23 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
       nm += normal[dir] * normal[dir];
     }
     nm = 1. / sqrt(nm);
+    LSG_UNROLL
     for (int dir = 0; dir < D; ++dir) normal[dir] *= nm;
     const double scaVel = vel.scalar(normal);
+    LSG_UNROLL
     for (int dir = 0; dir < D; ++dir) {
       double tempAlpha = fabs((scaVel + vel.vec[dir]) * normal[dir]);
       alpha[dir] = dmaxs(alpha[dir], tempAlpha);
       alphaAcc[dir] = dmaxs(alphaAcc[dir], tempAlpha);
     }
-    int dim = 0;
-    for (; dim < D - 1; ++dim) {
-      if (ni[dim] < 1) break;
-      ni[dim] = -1;
-    }
-    ++ni[dim];
   }
+  LSG_UNROLL
   for (int i = 0; i < D; ++i) dissipation += alphaFactor * alpha[i] * (gradNeg[i] - gradPos[i]) * 0.5;
   return {totalGrad, dissipation};
 }
 
 // Scheme ids of the rate kernel.
-enum SchemeId : int { S_EO = 0, S_WENO5 = 1, S_LF = 2, S_LLLF = 3, S_LLF = 4, S_WENO3 = 5 };
+enum SchemeId : int { S_EO = 0, S_WENO5 = 1, S_LF = 2, S_LLLF = 3, S_LLF = 4, S_WENO3 = 5, S_WENO5L = 6 };
 
 LSG_HD inline int schemeIdOf(int spatial) {
   switch (spatial) {
@@ -595,6 +707,7 @@ LSG_HD inline int schemeIdOf(int spatial) {
   case 5: case 6: return S_LLLF;
   case 7: case 8: return S_LLF;
   case 10: return S_WENO3;
+  case 12: return S_WENO5L;
   default: return S_WENO5;
   }
 }
@@ -604,11 +717,13 @@ LSG_HD inline int schemeIdOf(int spatial) {
 // precedence kept) and the raw H - D through `velocity`.
 template <int D, int SCHEME>
 LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi0, const RateParams &rp, const Vel &vel,
-                               const double *globalAlphas, double *alphaAcc, double &velocity) {
+                               const double *globalAlphas, double *alphaAcc, double &velocity,
+                               const float *mlp = nullptr) {
   RateOut r;
   if (SCHEME == S_EO) r = schemeEO<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO5) r = schemeWENO<D, 5>(b, c, phi0, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO3) r = schemeWENO<D, 3>(b, c, phi0, rp.calcNormals != 0, vel);
+  else if (SCHEME == S_WENO5L) r = schemeWENO<D, 5, true>(b, c, phi0, rp.calcNormals != 0, vel, mlp);
   else if (SCHEME == S_LF) r = schemeLF<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel, rp.alphaFactor, globalAlphas);
   else if (SCHEME == S_LLLF) r = schemeLLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
   else r = schemeLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
@@ -622,13 +737,16 @@ LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi
 template <int D> LSG_HD inline void globalAlphaPoint(const BandView &b, const int32_t c[3], const Vel &vel, double *la) {
   double normal[3] = {0, 0, 0};
   double nm = 0.;
+  LSG_UNROLL
   for (int k = 0; k < D; ++k) {
     normal[k] = b.at(c, k, 1) - b.at(c, k, -1);
     nm += normal[k] * normal[k];
   }
   nm = 1. / sqrt(nm);
+  LSG_UNROLL
   for (int k = 0; k < D; ++k) normal[k] *= nm;
   const double sv = vel.scalar(normal);
+  LSG_UNROLL
   for (int k = 0; k < D; ++k) {
     double t = fabs((sv + vel.vec[k]) * normal[k]);
     la[k] = dmaxs(la[k], t);
@@ -648,6 +766,7 @@ LSG_HD inline double stageTimeStep(double maxTimeStep, double cfl, double maxAbs
   }
   if (useAlphas) {
     double timeStep = 0;
+    LSG_UNROLL
     for (int i = 0; i < D; ++i) timeStep += alphas[i] / dx;
     timeStep = 1.0 / timeStep;
     tempMax = (tempMax < timeStep) ? tempMax : timeStep;

@@ -39,6 +39,12 @@ BandView viewOf(const BandBufs &b, const Frame &f, const GridInfo &g) {
   if ((sc)->overflow) return
 
 // ------------------------------------------------------------- scans ----
+//
+// Two-kernel scans with device-side sizes.  Phase 1: every block reduces its
+// contiguous chunk (and may produce the values, e.g. evaluate a rule) into
+// partials[b].  Phase 2: every block sums the partials before it (all
+// threads cooperate), rescans its chunk from that offset and scatters.  No
+// block ever waits on another block.
 
 // Block-wide exclusive scan of one value per thread (256 threads, 4 waves).
 __device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t &total) {
@@ -65,6 +71,12 @@ __device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t &total) {
   return x - v + woff;
 }
 
+__device__ inline uint32_t blockSum(uint32_t v) {
+  uint32_t t;
+  blockExclusiveScan(v, t);
+  return t;
+}
+
 // Contiguous chunk [lo, hi) of block blockIdx.x for a device-side size n.
 __device__ inline void blockChunk(uint32_t n, uint32_t &lo, uint32_t &hi) {
   const uint32_t tile = TPB * SCAN_ITEMS;
@@ -76,301 +88,324 @@ __device__ inline void blockChunk(uint32_t n, uint32_t &lo, uint32_t &hi) {
   if (hi > n) hi = n;
 }
 
-// Phase 1: per-block sums of Src::load(i).
-template <class Src> __global__ void kScanReduce(Src src, const uint32_t *nptr, uint32_t *partials) {
-  LSG_ABORT_IF_OVERFLOW(src.sc);
-  const uint32_t n = src.size(nptr);
+// Sum of partials[0 .. blockIdx.x) computed by the whole block.
+__device__ inline uint32_t blockOffset(const uint32_t *partials) {
+  uint32_t s = 0;
+  for (uint32_t k = threadIdx.x; k < blockIdx.x; k += TPB) s += partials[k];
+  return blockSum(s);
+}
+
+// Value of the previous item in thread order within a tile: lane-1 of the
+// wave, the last lane of the previous wave, or `carry` for thread 0.
+__device__ inline double prevInTile(double last, double carry) {
+  __shared__ double wl[TPB / 64];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  double up = __shfl_up(last, 1, 64);
+  if (lane == 63) wl[wid] = last;
+  __syncthreads();
+  double p = (lane > 0) ? up : (wid > 0 ? wl[wid - 1] : carry);
+  __syncthreads();
+  return p;
+}
+
+// ------------------------------------------------------ candidate bits ----
+
+// Phase 1 of candidate generation: candidate bitmap word per word (star
+// dilation or union) and per-block candidate counts.
+template <int D, int MODE>  // MODE 0: star dilation of A; 1: union of A and B
+__global__ __launch_bounds__(256) void kCandCount(const uint64_t *A, const uint64_t *B, Frame f, GridInfo g, uint64_t *cbits,
+                           uint32_t *partials, DevScalars *sc) {
+  if (sc->overflow) return;
+  const uint32_t nw = (uint32_t)f.nwords;
   uint32_t lo, hi;
-  blockChunk(n, lo, hi);
+  blockChunk(nw, lo, hi);
   uint32_t acc = 0;
-  for (uint32_t t = lo; t < hi; t += TPB * SCAN_ITEMS) {
-    uint32_t i0 = t + threadIdx.x * SCAN_ITEMS;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-      uint32_t i = i0 + k;
-      if (i < hi) acc += src.load(i);
-    }
+  for (uint32_t w = lo + threadIdx.x; w < hi; w += TPB) {
+    uint64_t C = (MODE == 0) ? candStarWord<D>(A, f, g, (int64_t)w) : (A[w] | B[w]);
+    cbits[w] = C;
+    acc += (uint32_t)__builtin_popcountll(C);
   }
-  uint32_t tot;
-  blockExclusiveScan(acc, tot);
-  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+  uint32_t t = blockSum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// Phase 2: exclusive scan of the partials (SCAN_BLOCKS <= 2*TPB) in one block.
-template <class Src> __global__ void kScanPartials(Src src, const uint32_t *nptr, uint32_t *partials, int nb) {
-  LSG_ABORT_IF_OVERFLOW(src.sc);
-  uint32_t v = (threadIdx.x < (unsigned)nb) ? partials[threadIdx.x] : 0;
-  uint32_t v2 = (threadIdx.x + TPB < (unsigned)nb) ? partials[threadIdx.x + TPB] : 0;
-  uint32_t t1, t2;
-  uint32_t e1 = blockExclusiveScan(v, t1);
-  uint32_t e2 = blockExclusiveScan(v2, t2);
-  if (threadIdx.x < (unsigned)nb) partials[threadIdx.x] = e1;
-  if (threadIdx.x + TPB < (unsigned)nb) partials[threadIdx.x + TPB] = e2 + t1;
-  if (threadIdx.x == 0) src.total(nptr, t1 + t2);
-}
-
-// Phase 3: rescan each chunk with its offset; Src::store(i, prefix, value).
-template <class Src> __global__ void kScanDown(Src src, const uint32_t *nptr, const uint32_t *partials) {
-  LSG_ABORT_IF_OVERFLOW(src.sc);
-  const uint32_t n = src.size(nptr);
+// Phase 2: word prefix counts and the candidate position list (index
+// base + cpre[w] + j); the last block stores the totals.
+__global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, int64_t nw64, int base, const uint32_t *partials, uint32_t *cpre,
+                          uint32_t *cpos, size_t ccap, uint32_t *ncand, DevScalars *sc) {
+  if (sc->overflow) return;
+  const uint32_t nw = (uint32_t)nw64;
   uint32_t lo, hi;
-  blockChunk(n, lo, hi);
-  uint32_t off = partials[blockIdx.x];
-  for (uint32_t t = lo; t < hi; t += TPB * SCAN_ITEMS) {
-    uint32_t i0 = t + threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-      uint32_t i = i0 + k;
-      v[k] = (i < hi) ? src.load(i) : 0;
-      s += v[k];
-    }
+  blockChunk(nw, lo, hi);
+  uint32_t off = blockOffset(partials);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && base) cpos[0] = 0xFFFFFFFFu;
+  for (uint32_t t = lo; t < hi; t += TPB) {
+    const uint32_t w = t + threadIdx.x;
+    uint64_t c = (w < hi) ? cbits[w] : 0ull;
+    const uint32_t pc = (uint32_t)__builtin_popcountll(c);
     uint32_t tot;
-    uint32_t ex = blockExclusiveScan(s, tot);
-    uint32_t p = off + ex;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-      uint32_t i = i0 + k;
-      if (i < hi) src.store(i, p, v[k]);
-      p += v[k];
+    const uint32_t p = off + blockExclusiveScan(pc, tot);
+    if (w < hi) {
+      cpre[w] = p;
+      size_t q = (size_t)base + p;
+      if (q + pc <= ccap) {
+        while (c) {
+          cpos[q++] = (uint32_t)((w << 6) | (uint32_t)__builtin_ctzll(c));
+          c &= c - 1;
+        }
+      }
     }
     off += tot;
   }
-}
-
-template <class Src> static void runScan(hipStream_t s, Src src, const uint32_t *nptr, uint32_t *partials) {
-  static_assert(SCAN_BLOCKS <= 2 * TPB, "partials scan handles at most 2*TPB blocks");
-  kScanReduce<Src><<<SCAN_BLOCKS, TPB, 0, s>>>(src, nptr, partials);
-  kScanPartials<Src><<<1, TPB, 0, s>>>(src, nptr, partials, SCAN_BLOCKS);
-  kScanDown<Src><<<SCAN_BLOCKS, TPB, 0, s>>>(src, nptr, partials);
-}
-
-// Candidate-count scan over bitmap words: cpre = exclusive prefix of cnt,
-// ncand = base + total (clamped to the workspace capacity, with the
-// overflow flag raised when it does not fit).
-struct WordCountScan {
-  const uint32_t *cnt;
-  uint32_t *cpre;
-  uint32_t *ncand;
-  uint32_t nwords;
-  int base;
-  size_t ccap;
-  DevScalars *sc;
-  __device__ uint32_t size(const uint32_t *) const { return nwords; }
-  __device__ uint32_t load(uint32_t i) const { return cnt[i]; }
-  __device__ void store(uint32_t i, uint32_t p, uint32_t) const { cpre[i] = p; }
-  __device__ void total(const uint32_t *, uint32_t t) const {
-    cpre[nwords] = t;
-    uint64_t nc = (uint64_t)t + (uint64_t)base;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    cpre[nw] = off;
+    uint64_t nc = (uint64_t)off + (uint64_t)base;
     if (nc > ccap) {
       sc->overflow = 1;
       nc = ccap;
     }
     *ncand = (uint32_t)nc;
   }
-};
+}
 
-// Keep-scan over candidates with scatter into the output band.
-struct KeepScan {
-  const double *out;
+// ------------------------------------------------- evaluators of rules ----
+
+template <int D> struct EvalStar {
+  BandView in;
   const uint32_t *cpos;
-  uint32_t *ks;
-  double *val;
-  uint32_t *pos;
-  uint32_t *nOut;
-  size_t cap;
-  DevScalars *sc;
-  __device__ uint32_t size(const uint32_t *nptr) const { return *nptr; }
-  __device__ uint32_t load(uint32_t i) const { return keepFlag(out, i); }
-  __device__ void store(uint32_t i, uint32_t p, uint32_t k) const {
-    ks[i] = p;
-    if (k && p < cap) {
-      val[p] = out[i];
-      pos[p] = cpos[i];
-    }
-  }
-  __device__ void total(const uint32_t *nptr, uint32_t t) const {
-    ks[*nptr] = t;
-    if (t > cap) {
-      sc->overflow = 1;
-      t = (uint32_t)cap;
-    }
-    *nOut = t;
-  }
-};
-
-// ------------------------------------------------------ candidate bits ----
-
-template <int D>
-__global__ void kCandStar(const uint64_t *bits, Frame f, GridInfo g, uint64_t *cbits, uint32_t *cnt, DevScalars *sc) {
-  LSG_ABORT_IF_OVERFLOW(sc);
-  const int64_t nw = f.nwords;
-  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < nw; w += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t C = candStarWord<D>(bits, f, g, w);
-    cbits[w] = C;
-    cnt[w] = (uint32_t)__builtin_popcountll(C);
-  }
-}
-
-// Union of two bitmaps (BooleanOperation visit set).
-__global__ void kCandUnion(const uint64_t *a, const uint64_t *b, int64_t nw, uint64_t *cbits, uint32_t *cnt,
-                           DevScalars *sc) {
-  LSG_ABORT_IF_OVERFLOW(sc);
-  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < nw; w += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t C = a[w] | b[w];
-    cbits[w] = C;
-    cnt[w] = (uint32_t)__builtin_popcountll(C);
-  }
-}
-
-// Expand candidate bits into the position list (index base + cpre[w] + j).
-__global__ void kCandList(const uint64_t *cbits, const uint32_t *cpre, int64_t nw, int base, uint32_t *cpos,
-                          size_t ccap, DevScalars *sc) {
-  LSG_ABORT_IF_OVERFLOW(sc);
-  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < nw; w += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t C = cbits[w];
-    if (w == 0 && base) cpos[0] = 0xFFFFFFFFu;
-    if (!C) continue;
-    size_t t = (size_t)base + cpre[w];
-    if (t + (size_t)__builtin_popcountll(C) > ccap) {
-      sc->overflow = 1;
-      continue;
-    }
-    while (C) {
-      int b = __builtin_ctzll(C);
-      cpos[t++] = (uint32_t)((w << 6) | b);
-      C &= C - 1;
-    }
-  }
-}
-
-// Bitmap words and prefix counts of the compacted output.
-__global__ void kBuildIndex(const uint32_t *cpre, const uint32_t *ks, const uint32_t *cpos, int64_t nw, int base,
-                            const uint32_t *ncand, uint64_t *bitsOut, uint32_t *preOut, DevScalars *sc) {
-  LSG_ABORT_IF_OVERFLOW(sc);
-  const uint32_t nc = *ncand;
-  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w <= nw; w += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t word;
-    uint32_t pre;
-    buildIndexWord(cpre, ks, cpos, w, nw, base, nc, word, pre);
-    bitsOut[w] = word;
-    preOut[w] = pre;
-  }
-}
-
-// ------------------------------------------------------------- rules ----
-
-template <int D>
-__global__ void kEvalStar(BandView in, const uint32_t *cpos, const uint32_t *ncand, OpParams op, double *out,
-                          DevScalars *sc) {
-  LSG_ABORT_IF_OVERFLOW(sc);
-  const uint32_t n = *ncand;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  OpParams op;
+  __device__ double operator()(uint32_t i) const {
     int32_t c[3];
     candCoords(in, i, cpos[i], c);
     double center, nb[2 * D];
     gatherStar<D>(in, c, center, nb);
-    out[i] = (op.kind == OP_EXPAND) ? ruleExpand<D>(center, nb, op.a, op.b) : ruleRebuild<D>(center, nb, op.a);
+    return (op.kind == OP_EXPAND) ? ruleExpand<D>(center, nb, op.a, op.b) : ruleRebuild<D>(center, nb, op.a);
   }
-}
+};
 
-template <int D> __global__ void kEvalSelf(BandView in, const uint32_t *nptr, OpParams op, double *out, DevScalars *sc) {
-  LSG_ABORT_IF_OVERFLOW(sc);
-  const uint32_t n = *nptr;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+template <int D> struct EvalSelf {
+  BandView in;
+  OpParams op;
+  __device__ double operator()(uint32_t i) const {
     double c = in.val[i];
-    double r;
-    if (op.kind == OP_REDUCE) {
-      r = ruleReduce(c, op.a);
-    } else if (op.kind == OP_UPDATE) {
-      r = ruleUpdate(c, *op.dt, op.rates[i]);
-    } else {
-      int32_t cc[3];
-      candCoords(in, i, in.pos[i], cc);
-      double center, nb[2 * D];
-      gatherStar<D>(in, cc, center, nb);
-      r = rulePrune<D>(c, nb, op.kind == OP_PRUNE_RSZ);
-    }
-    out[i] = r;
+    if (op.kind == OP_REDUCE) return ruleReduce(c, op.a);
+    if (op.kind == OP_UPDATE) return ruleUpdate(c, *op.dt, op.rates[i]);
+    int32_t cc[3];
+    candCoords(in, i, in.pos[i], cc);
+    double center, nb[2 * D];
+    gatherStar<D>(in, cc, center, nb);
+    return rulePrune<D>(c, nb, op.kind == OP_PRUNE_RSZ);
   }
-}
+};
 
-__global__ void kEvalBoolean(BandView A, BandView B, const uint32_t *cpos, const uint32_t *ncand, int kind, double wT,
-                             double wS, double *out, DevScalars *sc) {
-  LSG_ABORT_IF_OVERFLOW(sc);
-  const uint32_t n = *ncand;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+struct EvalBool {
+  BandView A, B;
+  const uint32_t *cpos;
+  int kind;
+  double wT, wS;
+  __device__ double operator()(uint32_t i) const {
     int32_t c[3];
     candCoords(A, i, cpos[i], c);
-    double a = A.value(c[0], c[1], c[2]);
-    double b = B.value(c[0], c[1], c[2]);
-    out[i] = ruleBoolean(kind, a, b, wT, wS);
+    return ruleBoolean(kind, A.value(c[0], c[1], c[2]), B.value(c[0], c[1], c[2]), wT, wS);
+  }
+};
+
+// Block-wide exclusive scan of one 0/1 flag per thread via wave ballots.
+__device__ inline uint32_t blockBallotScan(bool k, uint32_t &total, uint64_t &waveMask) {
+  __shared__ uint32_t wc[TPB / 64];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const uint64_t m = (uint64_t)__ballot(k ? 1 : 0);
+  const uint32_t lp = (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wc[wid] = (uint32_t)__builtin_popcountll(m);
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < TPB / 64; ++q) {
+    uint32_t c = wc[q];
+    if (q < wid) woff += c;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  waveMask = m;
+  return woff + lp;
+}
+
+// Keep flag of a candidate from its value and the previous candidate's.
+__device__ inline uint32_t keepOf(uint32_t i, double v, double prev) {
+  if (i == 0) return 1;
+  if (!isUndef(v)) return 1;
+  return v != prev ? 1u : 0u;
+}
+
+// Phase 1 of compaction: evaluate the rule at every candidate (out[i]) and
+// count the kept candidates per block.
+template <class Ev> __global__ __launch_bounds__(256) void kEvalKeep(Ev ev, const uint32_t *nptr, double *out, uint32_t *partials, DevScalars *sc) {
+  if (sc->overflow) return;
+  const uint32_t n = *nptr;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi);
+  __shared__ double carry;
+  if (threadIdx.x == 0) carry = (lo > 0 && lo < hi) ? ev(lo - 1) : 0.0;
+  __syncthreads();
+  uint32_t acc = 0;
+  for (uint32_t t = lo; t < hi; t += TPB) {
+    const uint32_t i = t + threadIdx.x;
+    const bool valid = i < hi;
+    double v = 0.0;
+    if (valid) {
+      v = ev(i);
+      out[i] = v;
+    }
+    const double prev = prevInTile(v, carry);
+    if (valid) acc += keepOf(i, v, prev);
+    __syncthreads();
+    if (threadIdx.x == TPB - 1) carry = v;
+    __syncthreads();
+  }
+  const uint32_t tot = blockSum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// Phase 2: rescan the keep flags from the block offset, scatter the kept
+// candidates into the output band, store the running index ks[] and the
+// keep flags packed one bit per candidate; the last block stores totals.
+__global__ __launch_bounds__(256) void kKeepScatter(const double *out, const uint32_t *cpos, const uint32_t *nptr, const uint32_t *partials,
+                             uint32_t *ks, uint64_t *keepBits, double *val, uint32_t *pos, uint32_t *nOut, size_t cap,
+                             DevScalars *sc) {
+  if (sc->overflow) return;
+  const uint32_t n = *nptr;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi);
+  uint32_t off = blockOffset(partials);
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  for (uint32_t t = lo; t < hi; t += TPB) {
+    const uint32_t i = t + threadIdx.x;
+    const bool valid = i < hi;
+    double v = 0.0;
+    uint32_t k = 0;
+    if (valid) {
+      v = out[i];
+      const double prev = (i > 0) ? out[i - 1] : 0.0;
+      k = keepOf(i, v, prev);
+    }
+    uint32_t tot;
+    uint64_t wm;
+    const uint32_t p = off + blockBallotScan(k != 0, tot, wm);
+    if (valid) {
+      ks[i] = p;
+      if (k && p < cap) {
+        val[p] = v;
+        pos[p] = cpos[i];
+      }
+    }
+    const uint32_t g = t + (uint32_t)wid * 64u;  // 64-aligned: chunks and tiles are multiples of 256
+    if (lane == 0 && g < hi) keepBits[g >> 6] = wm;
+    off += tot;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    ks[n] = off;
+    uint32_t tt = off;
+    if (tt > cap) {
+      sc->overflow = 1;
+      tt = (uint32_t)cap;
+    }
+    *nOut = tt;
+  }
+}
+
+// Bitmap words and prefix counts of the compacted output: the kept flags of
+// the candidates of word w (a contiguous run of the packed keep bits) are
+// deposited onto the set bits of the candidate word.
+__global__ void kBuildIndex(const uint64_t *cwords, const uint32_t *cpre, const uint32_t *ks, const uint64_t *keepBits,
+                            int64_t nw, int base, const uint32_t *ncand, uint64_t *bitsOut, uint32_t *preOut,
+                            DevScalars *sc) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const uint32_t nc = *ncand;
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w <= nw; w += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t c0 = (uint32_t)base + cpre[w];
+    if (c0 > nc) c0 = nc;
+    preOut[w] = ks[c0] - (uint32_t)base;
+    if (w == nw) {
+      bitsOut[w] = 0;
+      continue;
+    }
+    uint64_t src = cwords[w];
+    uint64_t word = 0;
+    if (src) {
+      uint32_t c1 = (uint32_t)base + cpre[w + 1];
+      if (c1 > nc) c1 = nc;
+      const uint32_t len = c1 - c0;
+      const uint32_t q = c0 >> 6, s = c0 & 63;
+      uint64_t m = keepBits[q] >> s;
+      if (s != 0 && s + len > 64) m |= keepBits[q + 1] << (64 - s);
+      if (len < 64) m &= (1ull << len) - 1ull;
+      while (src) {  // deposit m onto the set bits of src, lowest first
+        const uint64_t low = src & (~src + 1ull);
+        if (m & 1ull) word |= low;
+        m >>= 1;
+        src ^= low;
+      }
+    }
+    bitsOut[w] = word;
   }
 }
 
 // ------------------------------------------------------- host wrappers ----
 
-static void compactInto(hipStream_t s, const Workspace &ws, const uint32_t *ncandPtr, const uint32_t *cpos,
-                        const uint32_t *cpre, const Frame &f, BandBufs &out, DevScalars *sc) {
-  KeepScan k;
-  k.out = ws.out;
-  k.cpos = cpos;
-  k.ks = ws.ks;
-  k.val = out.val;
-  k.pos = out.pos;
-  k.nOut = out.n;
-  k.cap = out.cap;
-  k.sc = sc;
-  runScan(s, k, ncandPtr, ws.partials);
-  kBuildIndex<<<gridFor((size_t)f.nwords + 1), TPB, 0, s>>>(cpre, ws.ks, cpos, f.nwords, f.base, ncandPtr, out.bits,
-                                                            out.pre, sc);
+template <class Ev>
+static void compactInto(hipStream_t s, const Ev &ev, const Workspace &ws, const uint32_t *ncandPtr,
+                        const uint32_t *cpos, const uint32_t *cpre, const uint64_t *cwords, const Frame &f,
+                        BandBufs &out, DevScalars *sc) {
+  kEvalKeep<Ev><<<SCAN_BLOCKS, TPB, 0, s>>>(ev, ncandPtr, ws.out, ws.partials, sc);
+  kKeepScatter<<<SCAN_BLOCKS, TPB, 0, s>>>(ws.out, cpos, ncandPtr, ws.partials, ws.ks, ws.keepBits, out.val, out.pos,
+                                           out.n, out.cap, sc);
+  kBuildIndex<<<gridFor((size_t)f.nwords + 1), TPB, 0, s>>>(cwords, cpre, ws.ks, ws.keepBits, f.nwords, f.base,
+                                                            ncandPtr, out.bits, out.pre, sc);
 }
 
-static void candidatesFromBits(hipStream_t s, Workspace &ws, const Frame &f, DevScalars *sc) {
-  WordCountScan wc;
-  wc.cnt = ws.cnt;
-  wc.cpre = ws.cpre;
-  wc.ncand = ws.ncand;
-  wc.nwords = (uint32_t)f.nwords;
-  wc.base = f.base;
-  wc.ccap = ws.ccap;
-  wc.sc = sc;
-  runScan(s, wc, nullptr, ws.partials);
-  kCandList<<<gridFor((size_t)f.nwords), TPB, 0, s>>>(ws.cbits, ws.cpre, f.nwords, f.base, ws.cpos, ws.ccap, sc);
+template <int D, int MODE>
+static void candidates(hipStream_t s, const uint64_t *A, const uint64_t *B, const Frame &f, const GridInfo &g,
+                       Workspace &ws, DevScalars *sc) {
+  kCandCount<D, MODE><<<SCAN_BLOCKS, TPB, 0, s>>>(A, B, f, g, ws.cbits, ws.partials, sc);
+  kCandList<<<SCAN_BLOCKS, TPB, 0, s>>>(ws.cbits, f.nwords, f.base, ws.partials, ws.cpre, ws.cpos, ws.ccap,
+                                        ws.ncand, sc);
+}
+
+template <int D>
+static void starOp(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g, Workspace &ws,
+                   const OpParams &op, DevScalars *sc) {
+  candidates<D, 0>(s, in.bits, nullptr, f, g, ws, sc);
+  EvalStar<D> ev{viewOf(in, f, g), ws.cpos, op};
+  compactInto(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, f, out, sc);
 }
 
 void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc) {
-  const int gw = gridFor((size_t)f.nwords);
-  if (D == 2) kCandStar<2><<<gw, TPB, 0, s>>>(in.bits, f, g, ws.cbits, ws.cnt, sc);
-  else kCandStar<3><<<gw, TPB, 0, s>>>(in.bits, f, g, ws.cbits, ws.cnt, sc);
-  candidatesFromBits(s, ws, f, sc);
-  BandView v = viewOf(in, f, g);
-  const int ge = gridFor(ws.ccap, 8192);
-  if (D == 2) kEvalStar<2><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, op, ws.out, sc);
-  else kEvalStar<3><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, op, ws.out, sc);
-  compactInto(s, ws, ws.ncand, ws.cpos, ws.cpre, f, out, sc);
+  if (D == 2) starOp<2>(s, in, out, f, g, ws, op, sc);
+  else starOp<3>(s, in, out, f, g, ws, op, sc);
 }
 
 void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc) {
-  BandView v = viewOf(in, f, g);
-  const int ge = gridFor(in.cap, 8192);
-  if (D == 2) kEvalSelf<2><<<ge, TPB, 0, s>>>(v, in.n, op, ws.out, sc);
-  else kEvalSelf<3><<<ge, TPB, 0, s>>>(v, in.n, op, ws.out, sc);
-  compactInto(s, ws, in.n, in.pos, in.pre, f, out, sc);
+  if (D == 2) {
+    EvalSelf<2> ev{viewOf(in, f, g), op};
+    compactInto(s, ev, ws, in.n, in.pos, in.pre, in.bits, f, out, sc);
+  } else {
+    EvalSelf<3> ev{viewOf(in, f, g), op};
+    compactInto(s, ev, ws, in.n, in.pos, in.pre, in.bits, f, out, sc);
+  }
 }
 
 void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
                    const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc) {
-  (void)D;
-  kCandUnion<<<gridFor((size_t)f.nwords), TPB, 0, s>>>(A.bits, B.bits, f.nwords, ws.cbits, ws.cnt, sc);
-  candidatesFromBits(s, ws, f, sc);
-  BandView va = viewOf(A, f, g), vb = viewOf(B, f, g);
-  const int ge = gridFor(ws.ccap, 8192);
-  kEvalBoolean<<<ge, TPB, 0, s>>>(va, vb, ws.cpos, ws.ncand, kind, wTarget, wSource, ws.out, sc);
-  compactInto(s, ws, ws.ncand, ws.cpos, ws.cpre, f, out, sc);
+  if (D == 2) candidates<2, 1>(s, A.bits, B.bits, f, g, ws, sc);
+  else candidates<3, 1>(s, A.bits, B.bits, f, g, ws, sc);
+  EvalBool ev{viewOf(A, f, g), viewOf(B, f, g), ws.cpos, kind, wTarget, wSource};
+  compactInto(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, f, out, sc);
 }
 
 // ------------------------------------------------------------ rates ----
@@ -409,37 +444,96 @@ __device__ inline unsigned long long blockSumU64(unsigned long long v) {
   return t;
 }
 
+// Wave-level compaction of the active entries (|phi| <= 0.5, defined) of a
+// band: each wave scans 64 entries at a time, queues the active ones in LDS
+// and runs `fn(i)` on full waves, so the expensive per-point work never runs
+// with idle lanes.  The loop trip count is wave-uniform.
+template <class Fn> __device__ inline void forEachActive(const double *val, uint32_t n, Fn &fn) {
+  __shared__ uint32_t queue[TPB / 64][128];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  uint32_t *q = queue[wid];
+  uint32_t cnt = 0;
+  const uint32_t stride = gridDim.x * (TPB / 64) * 64u;
+  uint32_t base = (blockIdx.x * (TPB / 64) + wid) * 64u;
+  // One call site of fn, so that it is inlined once: the loop alternates
+  // between queueing one 64-entry slice and draining a full wave of queued
+  // entries; after the last slice the remainder (< 64) is drained.
+  for (;;) {
+    if (base < n) {
+      const uint32_t i = base + lane;
+      bool act = false;
+      if (i < n) {
+        const double v = val[i];
+        act = !isUndef(v) && fabs(v) <= 0.5;
+      }
+      const uint64_t m = (uint64_t)__ballot(act ? 1 : 0);
+      if (act) q[cnt + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull))] = i;
+      cnt += (uint32_t)__builtin_popcountll(m);
+      base += stride;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    const bool last = base >= n;
+    if (cnt >= 64 || (last && cnt > 0)) {
+      const uint32_t take = cnt < 64 ? cnt : 64u;
+      if ((uint32_t)lane < take) fn(q[lane]);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const uint32_t rem = cnt - take;
+      const uint32_t moved = ((uint32_t)lane < rem) ? q[64 + lane] : 0u;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if ((uint32_t)lane < rem) q[lane] = moved;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      cnt = rem;
+    }
+    if (last && cnt == 0) break;
+  }
+}
+
 // Hamiltonian + dissipation per active entry of the expanded band
 // (Advect::integrateTime, lsAdvect.hpp:495-593, one level set): writes the
 // stage velocity and reduces max |H - D| (and the LLF/LLLF alphas).
-template <int D, int SCHEME>
-__global__ void kRates(BandView b, const uint32_t *nptr, RateParams rp, double *rates, DevScalars *sc) {
-  LSG_ABORT_IF_OVERFLOW(sc);
-  const uint32_t n = *nptr;
-  const Vel vel = velOf(rp);
-  double lmax = 0.;
-  double la[3] = {0, 0, 0};
-  double ga[3] = {0, 0, 0};
-  unsigned long long act = 0;
-  if (SCHEME == S_LF)
-    for (int i = 0; i < 3; ++i) ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double phi0 = b.val[i];
-    if (isUndef(phi0) || fabs(phi0) > 0.5) continue;
+template <int D, int SCHEME> struct RatePointFn {
+  BandView b;
+  RateParams rp;
+  Vel vel;
+  double *rates;
+  const float *mlp;  // learned weno5 parameters staged in LDS
+  double ga[3];
+  double la[3];
+  double lmax;
+  unsigned long long act;
+  __device__ __attribute__((always_inline)) void operator()(uint32_t i) {
     ++act;
     int32_t c[3];
     candCoords(b, i, b.pos[i], c);
     double velocity;
-    rates[i] = ratePoint<D, SCHEME>(b, c, phi0, rp, vel, ga, la, velocity);
+    rates[i] = ratePoint<D, SCHEME>(b, c, b.val[i], rp, vel, ga, la, velocity, mlp);
     if (velocity != 0.) {
-      double a = fabs(velocity);
+      const double a = fabs(velocity);
       if (lmax < a) lmax = a;
     }
   }
-  blockMaxAtomic((unsigned long long)__double_as_longlong(lmax), &sc->maxAbsBits);
+};
+
+template <int D, int SCHEME>
+__global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *nptr, RateParams rp, double *rates, DevScalars *sc) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const uint32_t n = *nptr;
+  const Vel vel = velOf(rp);
+  // learned weno5: the bf16 parameters are widened to fp32 once per block
+  __shared__ float mlpW[SCHEME == S_WENO5L ? MLP_NPARAM : 1];
+  if (SCHEME == S_WENO5L) {
+    for (int i = threadIdx.x; i < MLP_NPARAM; i += TPB) mlpW[i] = bf16Float(rp.mlp[i]);
+    __syncthreads();
+  }
+  RatePointFn<D, SCHEME> fn{b, rp, vel, rates, mlpW, {0, 0, 0}, {0, 0, 0}, 0., 0ull};
+  if (SCHEME == S_LF)
+    for (int i = 0; i < 3; ++i) fn.ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
+  forEachActive(b.val, n, fn);
+  blockMaxAtomic((unsigned long long)__double_as_longlong(fn.lmax), &sc->maxAbsBits);
   if (SCHEME == S_LLLF || SCHEME == S_LLF)
-    for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(la[d]), &sc->alphaBits[d]);
-  unsigned long long t = blockSumU64(act);
+    for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(fn.la[d]), &sc->alphaBits[d]);
+  unsigned long long t = blockSumU64(fn.act);
   if (threadIdx.x == 0 && t) atomicAdd(&sc->activeCount, t);
 }
 
@@ -473,6 +567,7 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
     switch (scheme) {
     case S_EO: LSG_RATES(2, S_EO); break;
     case S_WENO5: LSG_RATES(2, S_WENO5); break;
+    case S_WENO5L: LSG_RATES(2, S_WENO5L); break;
     case S_LF: LSG_RATES(2, S_LF); break;
     case S_LLLF: LSG_RATES(2, S_LLLF); break;
     case S_LLF: LSG_RATES(2, S_LLF); break;
@@ -482,6 +577,7 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
     switch (scheme) {
     case S_EO: LSG_RATES(3, S_EO); break;
     case S_WENO5: LSG_RATES(3, S_WENO5); break;
+    case S_WENO5L: LSG_RATES(3, S_WENO5L); break;
     case S_LF: LSG_RATES(3, S_LF); break;
     case S_LLLF: LSG_RATES(3, S_LLLF); break;
     case S_LLF: LSG_RATES(3, S_LLF); break;
@@ -494,19 +590,24 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
 }
 
 // Advect::findGlobalAlphas (lsAdvect.hpp:90-195), one level set.
-template <int D> __global__ void kGlobalAlphas(BandView b, const uint32_t *nptr, RateParams rp, DevScalars *sc) {
-  LSG_ABORT_IF_OVERFLOW(sc);
-  const uint32_t n = *nptr;
-  const Vel vel = velOf(rp);
-  double la[3] = {0, 0, 0};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double v0 = b.val[i];
-    if (isUndef(v0) || fabs(v0) > 0.5) continue;
+template <int D> struct AlphaPointFn {
+  BandView b;
+  Vel vel;
+  double la[3];
+  __device__ __attribute__((always_inline)) void operator()(uint32_t i) {
     int32_t c[3];
     candCoords(b, i, b.pos[i], c);
     globalAlphaPoint<D>(b, c, vel, la);
   }
-  for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(la[d]), &sc->alphaBits[d]);
+};
+
+template <int D> __global__ __launch_bounds__(256) void kGlobalAlphas(BandView b, const uint32_t *nptr, RateParams rp, DevScalars *sc) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const uint32_t n = *nptr;
+  const Vel vel = velOf(rp);
+  AlphaPointFn<D> fn{b, vel, {0, 0, 0}};
+  forEachActive(b.val, n, fn);
+  for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(fn.la[d]), &sc->alphaBits[d]);
 }
 
 __global__ void kResetAlphas(DevScalars *sc) { sc->alphaBits[0] = sc->alphaBits[1] = sc->alphaBits[2] = 0; }
@@ -527,6 +628,13 @@ __global__ void kStepTime(DevScalars *sc, int temporal) {
   else sc->stepTime = sc->dt[0];
 }
 void launchStepTime(hipStream_t s, DevScalars *sc, int temporal) { kStepTime<<<1, 1, 0, s>>>(sc, temporal); }
+
+__global__ void kStepBegin(DevScalars *sc, const double *maxDt) {
+  sc->overflow = 0;
+  sc->stepActive0 = sc->totalActive;
+  sc->maxTimeStep = *maxDt;
+}
+void launchStepBegin(hipStream_t s, DevScalars *sc, const double *maxDt) { kStepBegin<<<1, 1, 0, s>>>(sc, maxDt); }
 
 // ---------------------------------------------------------------- misc ----
 
@@ -553,6 +661,21 @@ void launchCopyBand(hipStream_t s, const BandBufs &src, BandBufs &dst, const Fra
                                                 src.pre, dst.bits, dst.pre, f.nwords);
 }
 
+__device__ inline int32_t blockMinI(int32_t v) {
+  __shared__ int32_t smn[TPB / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    int32_t y = __shfl_down(v, o, 64);
+    v = y < v ? y : v;
+  }
+  if ((threadIdx.x & 63) == 0) smn[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int32_t r = smn[0];
+  for (int k = 1; k < TPB / 64; ++k) r = smn[k] < r ? smn[k] : r;
+  __syncthreads();
+  return r;
+}
+
 __global__ void kBandStats(const double *val, const uint32_t *pos, const uint32_t *nptr, Frame f, uint32_t *count,
                            int32_t *bbox) {
   const uint32_t n = *nptr;
@@ -569,10 +692,17 @@ __global__ void kBandStats(const double *val, const uint32_t *pos, const uint32_
     }
   }
   unsigned long long t = blockSumU64(c);
-  if (threadIdx.x == 0 && t) atomicAdd(count, (uint32_t)t);
+  int32_t bmn[3], bmx[3];
   for (int d = 0; d < 3; ++d) {
-    if (mn[d] != INT32_MAX) atomicMin(&bbox[d], mn[d]);
-    if (mx[d] != INT32_MIN) atomicMax(&bbox[3 + d], mx[d]);
+    bmn[d] = blockMinI(mn[d]);
+    bmx[d] = -blockMinI(mx[d] == INT32_MIN ? INT32_MAX : -mx[d]);
+  }
+  if (threadIdx.x == 0) {
+    if (t) atomicAdd(count, (uint32_t)t);
+    for (int d = 0; d < 3; ++d) {
+      if (bmn[d] != INT32_MAX) atomicMin(&bbox[d], bmn[d]);
+      if (bmx[d] > INT32_MIN + 1) atomicMax(&bbox[3 + d], bmx[d]);
+    }
   }
 }
 __global__ void kStatsInit(uint32_t *count, int32_t *bbox) {
@@ -584,7 +714,7 @@ __global__ void kStatsInit(uint32_t *count, int32_t *bbox) {
 }
 void launchBandStats(hipStream_t s, const BandBufs &b, const Frame &f, uint32_t *outCount, int32_t *bbox) {
   kStatsInit<<<1, 1, 0, s>>>(outCount, bbox);
-  kBandStats<<<gridFor(b.cap, 2048), TPB, 0, s>>>(b.val, b.pos, b.n, f, outCount, bbox);
+  kBandStats<<<gridFor(b.cap, 512), TPB, 0, s>>>(b.val, b.pos, b.n, f, outCount, bbox);
 }
 
 __global__ void kFill32(uint32_t *p, uint32_t v, size_t n) {

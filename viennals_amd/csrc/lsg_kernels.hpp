@@ -9,9 +9,9 @@
 
 namespace lsg {
 
-constexpr int SCAN_BLOCKS = 512;
+constexpr int SCAN_BLOCKS = 1024;
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_ITEMS = 1;
 
 // Device scalars shared by the kernels of one advection handle.
 struct DevScalars {
@@ -25,6 +25,7 @@ struct DevScalars {
   double stepTime;    // time advanced by the last step
   double maxTimeStep; // bound passed to the first stage
   double rateBytes;   // accumulated algorithmic bytes of the rate kernel
+  unsigned long long stepActive0;  // totalActive at the start of the step
 };
 
 // Mutable device buffers of a band.
@@ -44,6 +45,7 @@ struct Workspace {
   uint32_t *cpos;      // ccap
   double *out;         // ccap
   uint32_t *ks;        // ccap + 1
+  uint64_t *keepBits;  // ccap / 64 + 2 (keep flag per candidate)
   uint32_t *partials;  // SCAN_BLOCKS + 1
   uint32_t *ncand;     // device candidate count
   size_t ccap;
@@ -66,6 +68,9 @@ void launchGlobalAlphas(hipStream_t s, int D, const BandBufs &b, const Frame &f,
                         const RateParams &rp, DevScalars *sc);
 // Small device-scalar kernels.
 void launchStepTime(hipStream_t s, DevScalars *sc, int temporal);
+// Step prologue: clear the overflow flag, mark the active-point counter and
+// read maxTimeStep from host-mapped memory (so a captured step stays valid).
+void launchStepBegin(hipStream_t s, DevScalars *sc, const double *maxDt);
 void launchCopyBand(hipStream_t s, const BandBufs &src, BandBufs &dst, const Frame &f);
 // Count defined entries (to *outCount) and the bounding box of in-frame
 // entries (frame-relative, to bbox[6]: min x,y,z then max x,y,z).

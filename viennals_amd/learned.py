@@ -1,0 +1,477 @@
+"""Learned WENO5 weights for coarse-grid advection.
+
+A network 5 -> 32 -> 32 -> 3 (GELU, softmax) replaces the nonlinear
+weights of FiniteDifferences::weno5 (lsFiniteDifferences.hpp:88-123) on a
+grid twice as coarse as the run it imitates; plain weno5 stays in charge
+wherever the smoothness indicators disagree by more than a factor 10
+(kinks).  Everything numeric runs on the GPU through the library:
+
+* training data: plain WENO5-RK3 runs of random star-shaped surfaces at
+  gridDelta 0.5.  At every snapshot the fine level set is sampled on the
+  coarse grid (gridDelta 1.0), rebuilt and expanded exactly as the coarse
+  solver prepares its band, and each active coarse point contributes its
+  coarse weno5 stencils (inputs) and the fine-grid weno5 derivative at the
+  same point (target);
+* training: the library's forward/backward kernels (matrix cores for the
+  hidden layer) with Adam;
+* judging: Chamfer distance and volume error of the coarse runs (plain
+  and learned) against the fine plain run.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import (Advect, BoundaryConditionEnum, Domain, Expand, SpatialSchemeEnum, TemporalSchemeEnum, VelocityField,
+               POS_VALUE, check, rebuild, weno)
+from ._lib import lib
+
+NPARAM = 1347
+OFFSETS = dict(W1=(0, (32, 5)), b1=(160, (32,)), W2=(192, (32, 32)), b2=(1216, (32,)), W3=(1248, (3, 32)),
+               b3=(1344, (3,)))
+IDEAL = np.array([0.1, 0.6, 0.3])  # linear weights of weno5
+KINK_RATIO = 10.0
+
+
+def init_params(seed=0):
+    """Glorot-uniform hidden layers; the output layer starts at the linear
+    weno5 weights (zero W3, b3 = log of 0.1/0.6/0.3)."""
+    rng = np.random.default_rng(seed)
+    P = np.zeros(NPARAM, np.float32)
+
+    def glorot(shape):
+        lim = np.sqrt(6.0 / (shape[0] + shape[1]))
+        return rng.uniform(-lim, lim, shape)
+
+    P[0:160] = glorot((32, 5)).ravel()
+    P[192:1216] = glorot((32, 32)).ravel()
+    P[1248:1344] = 0.0
+    P[1344:1347] = np.log(IDEAL)
+    return P
+
+
+# ------------------------------------------------------------ geometry ----
+
+def _real_sph_harm(l, m, theta, phi):
+    from scipy.special import sph_harm_y
+    if m == 0:
+        return np.real(sph_harm_y(l, 0, theta, phi))
+    y = sph_harm_y(l, abs(m), theta, phi)
+    return np.sqrt(2.0) * (np.real(y) if m > 0 else np.imag(y))
+
+
+class StarShape:
+    """r(direction) = R (1 + sum_{l<=lmax} a_lm Y_lm), |perturbation| <= amp."""
+
+    def __init__(self, R, seed, lmax=6, amp=0.3):
+        rng = np.random.default_rng(seed)
+        self.R = float(R)
+        self.terms = []
+        for l in range(1, lmax + 1):
+            for m in range(-l, l + 1):
+                self.terms.append((l, m, rng.standard_normal() / (l + 1.0)))
+        # scale so that max |perturbation| over many directions is amp * u, u in [0.5, 1]
+        u = rng.uniform(0.5, 1.0)
+        th, ph = self._fibonacci(20000)
+        raw = self._pert(th, ph)
+        self.scale = amp * u / np.max(np.abs(raw))
+
+    @staticmethod
+    def _fibonacci(n):
+        i = np.arange(n) + 0.5
+        th = np.arccos(1 - 2 * i / n)
+        ph = np.pi * (1 + 5 ** 0.5) * i
+        return th, np.mod(ph, 2 * np.pi)
+
+    def _pert(self, th, ph):
+        s = np.zeros_like(th)
+        for l, m, a in self.terms:
+            s += a * _real_sph_harm(l, m, th, ph)
+        return s
+
+    def implicit(self, x):
+        """g(x) = |x| - r(x/|x|) for points x (n, 3)."""
+        rr = np.linalg.norm(x, axis=1)
+        th = np.arccos(np.clip(x[:, 2] / np.maximum(rr, 1e-300), -1, 1))
+        ph = np.mod(np.arctan2(x[:, 1], x[:, 0]), 2 * np.pi)
+        return rr - self.R * (1.0 + self.scale * self._pert(th, ph))
+
+    def band(self, dx, halfwidth=1.0):
+        """Defined points (index, phi in grid units) with |phi| <= halfwidth;
+        phi = g / |grad g| (first-order signed distance), gradient by central
+        differences."""
+        rmax = self.R * (1.0 + self.scale * 1.01 * np.max(np.abs(self._pert(*self._fibonacci(20000))))) + 3 * dx
+        n = int(np.ceil(rmax / dx)) + 2
+        ax = np.arange(-n, n + 1)
+        pts, vals = [], []
+        h = 1e-4 * self.R
+        for iz in ax:  # slab by slab keeps memory small
+            X, Y = np.meshgrid(ax, ax, indexing="ij")
+            P = np.stack([X.ravel(), Y.ravel(), np.full(X.size, iz)], 1).astype(np.float64) * dx
+            g = self.implicit(P)
+            keep = np.abs(g) <= (halfwidth + 2.0) * dx * 2.0
+            if not keep.any():
+                continue
+            P = P[keep]
+            g = g[keep]
+            grad = np.zeros_like(P)
+            for a in range(3):
+                e = np.zeros(3)
+                e[a] = h
+                grad[:, a] = (self.implicit(P + e) - self.implicit(P - e)) / (2 * h)
+            phi = g / np.linalg.norm(grad, axis=1) / dx
+            m = np.abs(phi) <= halfwidth
+            idx = np.rint(P[m] / dx).astype(np.int32)
+            pts.append(idx)
+            vals.append(phi[m])
+        idx = np.concatenate(pts)
+        val = np.concatenate(vals)
+        order = np.lexsort((idx[:, 0], idx[:, 1], idx[:, 2]))
+        return idx[order], val[order]
+
+
+def star_domain(shape, dx):
+    d = Domain(gridDelta=dx, dim=3)
+    idx, val = shape.band(dx)
+    d.insertPoints(idx, val, 2)
+    return d
+
+
+def random_velocity(rng):
+    """Constant isotropic speed or a directional (normal-dependent) etch-like speed."""
+    rate = rng.uniform(0.2, 1.0)
+    if rng.uniform() < 0.5:
+        return VelocityField.constant(rate)
+    return VelocityField.normal_power(dir=2, k=1, b=0.1 * rate, a_pos=rate, a_neg=0.2 * rate)
+
+
+# ---------------------------------------------------------------- runs ----
+
+def advect(domain, vel, T, spatial=SpatialSchemeEnum.WENO_5TH_ORDER, params=None, cfl=0.4, snapshot=None):
+    """RK3 advection for physical time T.  Without `snapshot`: one
+    Advect::apply with advectionTime T (the last step is clipped to T,
+    lsAdvect.hpp:1028-1040), timed as a whole.  With `snapshot(domain, t)`:
+    single steps (setSingleStep) with the remaining time as the bound, the
+    callback outside the timed region.  Returns (steps, wall seconds, t)."""
+    a = Advect(domain, vel)
+    a.setSpatialScheme(spatial)
+    a.setTemporalScheme(TemporalSchemeEnum.RUNGE_KUTTA_3RD_ORDER)
+    a.setTimeStepRatio(cfl)
+    if params is not None:
+        a.setLearnedWeights(params)
+    if snapshot is None:
+        a.setAdvectionTime(T)
+        lib().lsg_device_synchronize()
+        t0 = time.perf_counter()
+        a.apply()
+        lib().lsg_device_synchronize()
+        return a.getNumberOfTimeSteps(), time.perf_counter() - t0, a.getAdvectedTime()
+    a.setSingleStep(True)
+    t, steps, wall = 0.0, 0, 0.0
+    while T - t > 1e-12:
+        a.setAdvectionTime(T - t)
+        t0 = time.perf_counter()
+        a.apply()
+        lib().lsg_device_synchronize()
+        wall += time.perf_counter() - t0
+        t += a.getAdvectedTime()
+        steps += 1
+        snapshot(domain, t)
+    return steps, wall, t
+
+
+# ------------------------------------------------------- training data ----
+
+def _dense(idx, val, pad=8):
+    lo = idx.min(axis=0) - pad
+    hi = idx.max(axis=0) + pad
+    A = np.full(tuple(hi - lo + 1), np.nan)
+    A[tuple((idx - lo).T)] = val
+    return A, lo
+
+
+def _weno5_parts(st):
+    """d, beta, candidate polynomials of weno5 minus-stencils st (n, 7)."""
+    d = np.diff(st, axis=1)  # d_i = x_{i+1} - x_i
+    b1 = 13.0 / 12.0 * (d[:, 0] - 2 * d[:, 1] + d[:, 2]) ** 2 + 0.25 * (d[:, 0] - 4 * d[:, 1] + 3 * d[:, 2]) ** 2
+    b2 = 13.0 / 12.0 * (d[:, 1] - 2 * d[:, 2] + d[:, 3]) ** 2 + 0.25 * (d[:, 1] - d[:, 3]) ** 2
+    b3 = 13.0 / 12.0 * (d[:, 2] - 2 * d[:, 3] + d[:, 4]) ** 2 + 0.25 * (3 * d[:, 2] - 4 * d[:, 3] + d[:, 4]) ** 2
+    p1 = (2 * d[:, 0] - 7 * d[:, 1] + 11 * d[:, 2]) / 6.0
+    p2 = (-d[:, 1] + 5 * d[:, 2] + 2 * d[:, 3]) / 6.0
+    p3 = (2 * d[:, 2] + 5 * d[:, 3] - d[:, 4]) / 6.0
+    return d, np.stack([b1, b2, b3], 1), np.stack([p1, p2, p3], 1)
+
+
+def samples_from_fine(fine, dxc=1.0):
+    """Training samples of one fine snapshot (gridDelta dxc/2).
+
+    Returns X (n, 8) float32 and T (n,) float32."""
+    dxf = fine.gridDelta
+    assert abs(dxc - 2 * dxf) < 1e-12
+    f = Domain(gridDelta=dxf, dim=3)
+    f.deepCopy(fine)
+    Expand(f, 9).apply()
+    fi, fv = f.defined_points()
+    F, flo = _dense(fi, fv)
+    # coarse band sampled from the fine one, prepared like the coarse solver
+    even = np.all(fi % 2 == 0, axis=1) & (np.abs(fv) <= 2.0)
+    ci = (fi[even] // 2).astype(np.int32)
+    cv = fv[even] * (dxf / dxc)
+    m = np.abs(cv) <= 1.0
+    c = Domain(gridDelta=dxc, dim=3)
+    c.insertPoints(ci[m], cv[m], 2)
+    rebuild(c)
+    Expand(c, 7).apply()
+    ki, kv = c.defined_points()
+    C, clo = _dense(ki, kv)
+    act = ki[np.abs(kv) <= 0.5]
+    Xs, Ts = [], []
+    ks = np.arange(-3, 4)
+    for a in range(3):
+        e = np.zeros(3, np.int64)
+        e[a] = 1
+        cpos = act[:, None, :] + ks[None, :, None] * e[None, None, :] - clo   # (n, 7, 3)
+        fpos = 2 * act[:, None, :] + ks[None, :, None] * e[None, None, :] - flo
+        okc = np.all((cpos >= 0) & (cpos < np.array(C.shape)), axis=(1, 2))
+        okf = np.all((fpos >= 0) & (fpos < np.array(F.shape)), axis=(1, 2))
+        ok = okc & okf
+        cpos, fpos = cpos[ok], fpos[ok]
+        cs = C[cpos[..., 0], cpos[..., 1], cpos[..., 2]]
+        fs = F[fpos[..., 0], fpos[..., 1], fpos[..., 2]]
+        good = ~(np.isnan(cs).any(1) | np.isnan(fs).any(1))
+        cs, fs = cs[good], fs[good]
+        for plus in (False, True):
+            tgt = weno(5, fs, dxf, plus) * (dxf / dxc)  # fine derivative in coarse grid units
+            st = cs[:, ::-1] if plus else cs
+            d, beta, p = _weno5_parts(st)
+            if plus:
+                d, p = -d, -p  # mirrored stencil: differences change sign
+            eps = 1e-6 * dxc * dxc
+            ab = beta + eps
+            smooth = ab.max(1) <= KINK_RATIO * ab.min(1)
+            mx = np.abs(d).max(1)
+            sel = smooth & (mx > 0)
+            mx = mx[sel]
+            X = np.concatenate([d[sel] / mx[:, None], p[sel] / mx[:, None]], 1)
+            Xs.append(X.astype(np.float32))
+            Ts.append((tgt[sel] * dxc / mx).astype(np.float32))
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def generate_samples(n_shapes=4, R=32.0, T=8.0, every=2, seed=7, log=None):
+    """Fine plain WENO5-RK3 runs (gridDelta 0.5) of random star shapes;
+    samples from every `every`-th step."""
+    rng = np.random.default_rng(seed)
+    Xs, Ts = [], []
+    for s in range(n_shapes):
+        shape = StarShape(R, seed=int(rng.integers(1 << 30)))
+        vel = random_velocity(rng)
+        fine = star_domain(shape, 0.5)
+        count = [0]
+
+        def snap(dom, t):
+            count[0] += 1
+            if count[0] % every == 0:
+                X, Tg = samples_from_fine(dom)
+                Xs.append(X)
+                Ts.append(Tg)
+
+        advect(fine, vel, T, snapshot=snap)
+        if log:
+            log(f"shape {s}: {sum(len(t) for t in Ts)} samples so far")
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+# ------------------------------------------------------------- trainer ----
+
+class Trainer:
+    """GPU training of the network (lsg_mlp_*): fp32 master weights, Adam."""
+
+    def __init__(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        h = ctypes.c_void_p()
+        check(lib().lsg_mlp_create(ctypes.byref(h), p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().lsg_mlp_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def set_samples(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        self.n = len(T)
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(lib().lsg_mlp_set_samples(self._h, self.n, X.ctypes.data_as(fp), T.ctypes.data_as(fp)))
+
+    def loss_grad(self, offset=0, count=None):
+        count = self.n - offset if count is None else count
+        loss = ctypes.c_double()
+        g = np.zeros(NPARAM, np.float32)
+        check(lib().lsg_mlp_loss_grad(self._h, int(offset), int(count), ctypes.byref(loss),
+                                      g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return loss.value, g
+
+    def train(self, iters, batch, lr):
+        last = ctypes.c_double()
+        check(lib().lsg_mlp_train(self._h, int(iters), int(batch), float(lr), ctypes.byref(last)))
+        return last.value
+
+    def params(self):
+        p = np.zeros(NPARAM, np.float32)
+        check(lib().lsg_mlp_get_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return p
+
+
+def train(X, T, iters=3000, batch=16384, lr=2e-3, seed=0, val_frac=0.1, log=None):
+    """Shuffle, hold out a validation split, train on the GPU; returns
+    (params, report)."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(T))
+    X, T = X[perm], T[perm]
+    nv = max(1, int(len(T) * val_frac))
+    Xv, Tv, Xt, Tt = X[:nv], T[:nv], X[nv:], T[nv:]
+    tr = Trainer(init_params(seed))
+    tr.set_samples(Xv, Tv)
+    loss0, _ = tr.loss_grad()
+    tr.set_samples(Xt, Tt)
+    t0 = time.perf_counter()
+    chunks = 10
+    last = float("nan")
+    for c in range(chunks):
+        last = tr.train(iters // chunks, min(batch, len(Tt)), lr)
+        if log:
+            log(f"iter {(c + 1) * (iters // chunks)}: batch loss {last:.3e}")
+    wall = time.perf_counter() - t0
+    tr.set_samples(Xv, Tv)
+    loss1, _ = tr.loss_grad()
+    return tr.params(), dict(samples=int(len(T)), val_loss_init=loss0, val_loss=loss1, last_batch_loss=last,
+                             train_seconds=wall, iters=iters, batch=batch, lr=lr)
+
+
+# -------------------------------------------------------------- judges ----
+
+def surface_points(domain):
+    """Zero crossings of phi along grid edges between defined neighbours
+    (physical coordinates)."""
+    idx, val = domain.defined_points()
+    A, lo = _dense(idx, val, pad=1)
+    dx = domain.gridDelta
+    out = []
+    for a in range(domain.D):
+        sl0 = [slice(None)] * domain.D
+        sl1 = [slice(None)] * domain.D
+        sl0[a] = slice(0, -1)
+        sl1[a] = slice(1, None)
+        v, w = A[tuple(sl0)], A[tuple(sl1)]
+        m = ~np.isnan(v) & ~np.isnan(w) & ((v < 0) != (w < 0)) & (v != w)
+        p = np.argwhere(m).astype(np.float64)
+        t = v[m] / (v[m] - w[m])
+        p[:, a] += t
+        out.append((p + lo) * dx)
+    return np.concatenate(out)
+
+
+def chamfer(a, b):
+    """Mean of the nearest-neighbour distances in both directions."""
+    from scipy.spatial import cKDTree
+    da, _ = cKDTree(b).query(a)
+    db, _ = cKDTree(a).query(b)
+    return (da.sum() + db.sum()) / (len(a) + len(b))
+
+
+def enclosed_volume(domain):
+    """Volume of phi < 0 for a closed surface: band points count with the
+    smeared Heaviside clamp(0.5 - phi, 0, 1), undefined cells enclosed by
+    the band count fully; times dx^3."""
+    from scipy import ndimage
+    idx, val = domain.defined_points()
+    A, lo = _dense(idx, val, pad=2)
+    band = ~np.isnan(A)
+    frac = np.clip(0.5 - A[band], 0.0, 1.0).sum()
+    interior = ndimage.binary_fill_holes(band) & ~band
+    return (frac + interior.sum()) * domain.gridDelta ** 3
+
+
+def compare(shape, vel, T, params, log=None):
+    """Fine plain (dx 0.5) vs coarse plain and coarse learned (dx 1.0)."""
+    res = {}
+    fine = star_domain(shape, 0.5)
+    sf, wf, tf = advect(fine, vel, T)
+    res["fine"] = dict(steps=sf, wall=wf, time=tf)
+    pf = surface_points(fine)
+    vf = enclosed_volume(fine)
+    for name, spatial, prm in (("coarse_plain", SpatialSchemeEnum.WENO_5TH_ORDER, None),
+                               ("coarse_learned", SpatialSchemeEnum.WENO_5TH_ORDER_LEARNED, params)):
+        c = star_domain(shape, 1.0)
+        s, w, t = advect(c, vel, T, spatial=spatial, params=prm)
+        pc = surface_points(c)
+        vc = enclosed_volume(c)
+        res[name] = dict(steps=s, wall=w, time=t, chamfer=chamfer(pc, pf), volume_error=abs(vc - vf) / vf)
+        if log:
+            log(f"{name}: {res[name]}")
+    return res
+
+
+def main(argv=None):
+    """python -m viennals_amd.learned {train,compare} ..."""
+    import argparse
+    import json
+    import os
+    ap = argparse.ArgumentParser(prog="python -m viennals_amd.learned")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train", help="generate fine-run samples and train on the GPU")
+    t.add_argument("--shapes", type=int, default=4)
+    t.add_argument("--time", type=float, default=8.0)
+    t.add_argument("--iters", type=int, default=3000)
+    t.add_argument("--batch", type=int, default=16384)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--seed", type=int, default=7)
+    t.add_argument("--out", required=True)
+    t.add_argument("--report")
+    c = sub.add_parser("compare", help="fine plain vs coarse plain vs coarse learned")
+    c.add_argument("--params", required=True)
+    c.add_argument("--shapes", type=int, default=2)
+    c.add_argument("--time", type=float, default=8.0)
+    c.add_argument("--seed", type=int, default=1234)
+    c.add_argument("--report")
+    args = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    if args.cmd == "train":
+        t0 = time.perf_counter()
+        X, T = generate_samples(n_shapes=args.shapes, T=args.time, seed=args.seed, log=log)
+        gen = time.perf_counter() - t0
+        log(f"{len(T)} samples in {gen:.1f} s")
+        params, rep = train(X, T, iters=args.iters, batch=args.batch, lr=args.lr, log=log)
+        rep["sample_seconds"] = gen
+        os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+        np.save(args.out, params)
+        log(json.dumps(rep))
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump(rep, f, indent=1)
+        return 0
+    params = np.load(args.params)
+    rng = np.random.default_rng(args.seed)
+    out = []
+    for s in range(args.shapes):
+        shape = StarShape(32.0, seed=int(rng.integers(1 << 30)))
+        vel = random_velocity(rng)
+        r = compare(shape, vel, args.time, params, log=log)
+        out.append(r)
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(out, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
