@@ -21,11 +21,6 @@ namespace {
 constexpr int MARGIN = 24;
 constexpr int REFRAME_SLACK = 12;
 
-int ceilLog2(int64_t v) {
-  int l = 0;
-  while (((int64_t)1 << l) < v) ++l;
-  return l;
-}
 
 struct EBand {
   int D = 3;
@@ -70,11 +65,9 @@ Frame makeFrame(const EBand &d, const int32_t *bbmin, const int32_t *bbmax) {
       f.ext[k] = (int32_t)(hi - lo + 1);
     }
   }
-  f.lx = std::max(6, ceilLog2(f.ext[0]));
-  f.ly = ceilLog2(f.ext[1]);
-  int64_t bitsTotal = ((int64_t)f.ext[2] << f.ly) << f.lx;
+  const int64_t bitsTotal = (int64_t)((f.ext[0] + 63) >> 6) * 64 * f.ext[1] * f.ext[2];
   check(bitsTotal < ((int64_t)1 << 32) - 64, "frame too large");
-  f.nwords = bitsTotal >> 6;
+  f.finish();
   bool inFrame = true;
   for (int k = 0; k < d.D; ++k) inFrame = inFrame && f.o[k] == d.g.lo[k];
   f.base = inFrame ? 0 : 1;
@@ -576,6 +569,21 @@ int emu_advect(emu_domain *d, const emu_params *p, double *out) {
   out[2] = steps;
   out[3] = a.pointUpdates;
   return 0;
+}
+
+// Bit helpers of the index kernels on the host.
+void emu_deposit(const uint64_t *x, const uint64_t *m, uint64_t *out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = lsg::depositBits(x[i], m[i]);
+}
+void emu_select(const uint64_t *w, const int32_t *k, int32_t *out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = lsg::selectBit(w[i], k[i]);
+}
+
+// Frame division helper on the host: q[i] = n[i] / d.
+void emu_fastdiv(uint32_t d, const uint32_t *n, uint32_t *q, int64_t count) {
+  lsg::FastDiv f;
+  f.init(d);
+  for (int64_t i = 0; i < count; ++i) q[i] = f.div(n[i]);
 }
 
 }  // extern "C"

@@ -36,6 +36,11 @@ def lib():
         L.emu_rebuild.argtypes = [P]
         L.emu_boolean.argtypes = [P, P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.emu_advect.argtypes = [P, ctypes.c_void_p, dp]
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.emu_fastdiv.argtypes = [ctypes.c_uint32, u32p, u32p, ctypes.c_int64]
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.emu_deposit.argtypes = [u64p, u64p, u64p, ctypes.c_int64]
+        L.emu_select.argtypes = [u64p, i32p, i32p, ctypes.c_int64]
         _lib = L
     return _lib
 
@@ -94,3 +99,31 @@ def advect(dom, params):
     out = (ctypes.c_double * 4)()
     st = lib().emu_advect(dom.h, ctypes.byref(params), out)
     return st, dict(advected_time=out[0], steps=int(out[2]), point_updates=out[3])
+
+
+def fastdiv(d, n):
+    """The frame's run-time-constant division (lsg_device.hpp FastDiv)."""
+    n = np.ascontiguousarray(n, dtype=np.uint32)
+    q = np.zeros_like(n)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    lib().emu_fastdiv(int(d), n.ctypes.data_as(u32p), q.ctypes.data_as(u32p), len(n))
+    return q
+
+
+def deposit(x, m):
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    m = np.ascontiguousarray(m, dtype=np.uint64)
+    out = np.zeros_like(x)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    lib().emu_deposit(x.ctypes.data_as(u64p), m.ctypes.data_as(u64p), out.ctypes.data_as(u64p), len(x))
+    return out
+
+
+def select_bit(w, k):
+    w = np.ascontiguousarray(w, dtype=np.uint64)
+    k = np.ascontiguousarray(k, dtype=np.int32)
+    out = np.zeros_like(k)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    lib().emu_select(w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), k.ctypes.data_as(i32p),
+                     out.ctypes.data_as(i32p), len(w))
+    return out

@@ -88,3 +88,52 @@ def test_advect(geom, spatial, temporal, vel):
     assert ro["steps"] == re["steps"] and ro["advected_time"] == re["advected_time"]
     assert ro["point_updates"] == re["point_updates"]
     assert_same(o, e, f"advect {geom} {spatial} {temporal}")
+
+
+def test_frame_fastdiv_exact():
+    """Division by a run-time constant used to decode frame positions:
+    exact against integer division on edge and random dividends."""
+    rng = np.random.default_rng(9)
+    divisors = [1, 2, 3, 5, 7, 16, 17, 255, 257, 641, 1025, 4097, 65537, 1 << 20, (1 << 20) + 3]
+    divisors += list(rng.integers(1, 1 << 22, 20))
+    for d in divisors:
+        d = int(d)
+        k = np.arange(0, (1 << 32) // d, max(1, ((1 << 32) // d) // 5000), dtype=np.uint64)
+        n = np.concatenate([k * d, k * d + d - 1, k * d + 1, rng.integers(0, 1 << 32, 20000, dtype=np.uint64),
+                            np.array([0, 1, (1 << 32) - 1, (1 << 32) - 2], np.uint64)])
+        n = n[n < (1 << 32)].astype(np.uint32)
+        assert np.array_equal(pe.fastdiv(d, n), n // np.uint32(d)), d
+
+
+def _naive_deposit(x, m):
+    out, j = 0, 0
+    for b in range(64):
+        if (m >> b) & 1:
+            if (x >> j) & 1:
+                out |= 1 << b
+            j += 1
+    return out
+
+
+def test_bit_deposit_and_select():
+    """Bit deposit / select of the index kernels against bit-by-bit loops."""
+    rng = np.random.default_rng(10)
+    full = (1 << 64) - 1
+    ms = [0, full, 1, 1 << 63, 0x5555555555555555, 0xF0F0F0F0F0F0F0F0] + \
+        [int(v) for v in rng.integers(0, 1 << 63, 300, dtype=np.uint64)] + \
+        [int(v) & int(u) for v, u in zip(rng.integers(0, 1 << 63, 100, dtype=np.uint64),
+                                         rng.integers(0, 1 << 63, 100, dtype=np.uint64))]
+    xs = [int(v) | (int(u) << 32) for v, u in zip(rng.integers(0, 1 << 32, len(ms), dtype=np.uint64),
+                                                   rng.integers(0, 1 << 32, len(ms), dtype=np.uint64))]
+    xs[1] = full
+    got = pe.deposit(np.array(xs, np.uint64), np.array(ms, np.uint64))
+    for x, m, g in zip(xs, ms, got):
+        assert int(g) == _naive_deposit(x, m), (hex(x), hex(m))
+    ws, ks, want = [], [], []
+    for m in ms:
+        bits = [b for b in range(64) if (m >> b) & 1]
+        for k in range(0, len(bits), 7):
+            ws.append(m)
+            ks.append(k)
+            want.append(bits[k])
+    assert np.array_equal(pe.select_bit(np.array(ws, np.uint64), np.array(ks, np.int32)), np.array(want))

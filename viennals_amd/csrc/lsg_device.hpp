@@ -78,22 +78,61 @@ struct GridInfo {
   }
 };
 
+// Unsigned 32-bit division by a run-time constant d >= 1 as a multiply-high
+// and a shift (Granlund & Montgomery 1994, round-up variant): exact for
+// every 32-bit dividend.
+struct FastDiv {
+  uint32_t d, m;
+  int s;
+  LSG_HD void init(uint32_t dd) {
+    d = dd;
+    s = 0;
+    while ((1ull << s) < (uint64_t)dd) ++s;
+    m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << s) - dd)) / dd + 1);
+  }
+  LSG_HD uint32_t div(uint32_t n) const {
+    const uint64_t t = ((uint64_t)n * m) >> 32;
+    return (uint32_t)((t + n) >> s);
+  }
+};
+
+// Dense bitmap frame of a band: rows of ext[0] bits padded to whole 64-bit
+// words (wpr words per row), rows ordered y fastest then z, no other
+// padding.  Frame position p = (z * ext[1] + y) * 64 wpr + x.
 struct Frame {
   int32_t o[3];    // absolute index of frame-relative 0 per dim
   int32_t ext[3];  // valid extent per dim (ext[2] == 1 in 2D)
-  int lx;          // log2(bits per row), >= 6
-  int ly;          // log2(rows per z-slab)
+  int wpr;         // words per row
   int base;        // 1 if entry 0 (grid minimum) lies outside the frame
   int64_t nwords;  // words in the bitmap (one zero pad word follows)
-  LSG_HD int wpr() const { return 1 << (lx - 6); }
+  FastDiv divW;    // division by wpr (word -> row)
+  FastDiv divY;    // division by ext[1] (row -> z)
+  // Derived fields from o/ext/base; returns the number of bits.
+  LSG_HD int64_t finish() {
+    wpr = (ext[0] + 63) >> 6;
+    nwords = (int64_t)wpr * ext[1] * ext[2];
+    divW.init((uint32_t)wpr);
+    divY.init((uint32_t)ext[1]);
+    return nwords * 64;
+  }
+  LSG_HD uint32_t rowBits() const { return (uint32_t)wpr << 6; }
   LSG_HD uint32_t encode(int32_t xr, int32_t yr, int32_t zr) const {
-    return (uint32_t)(((((uint64_t)zr << ly) + (uint64_t)yr) << lx) + (uint64_t)xr);
+    return ((uint32_t)zr * (uint32_t)ext[1] + (uint32_t)yr) * rowBits() + (uint32_t)xr;
   }
   LSG_HD void decode(uint32_t p, int32_t &xr, int32_t &yr, int32_t &zr) const {
-    xr = (int32_t)(p & ((1u << lx) - 1));
-    uint32_t row = p >> lx;
-    yr = (int32_t)(row & ((1u << ly) - 1));
-    zr = (int32_t)(row >> ly);
+    const uint32_t row = divW.div(p >> 6);
+    xr = (int32_t)(p - row * rowBits());
+    const uint32_t z = divY.div(row);
+    zr = (int32_t)z;
+    yr = (int32_t)(row - z * (uint32_t)ext[1]);
+  }
+  // Row (y, z) and word-in-row kw of bitmap word w.
+  LSG_HD void wordRow(uint32_t w, int32_t &kw, int32_t &yr, int32_t &zr) const {
+    const uint32_t row = divW.div(w);
+    kw = (int32_t)(w - row * (uint32_t)wpr);
+    const uint32_t z = divY.div(row);
+    zr = (int32_t)z;
+    yr = (int32_t)(row - z * (uint32_t)ext[1]);
   }
 };
 
@@ -115,36 +154,64 @@ struct BandView {
     c[2] = zr + f.o[2];
   }
 
-  // Value of the level set at an absolute, in-grid (already mapped) point.
-  LSG_HD double value(int32_t x, int32_t y, int32_t z) const {
-    int64_t xr = (int64_t)x - f.o[0];
-    int64_t yr = (int64_t)y - f.o[1];
-    int64_t zr = (int64_t)z - f.o[2];
-    uint32_t rank;
+  // A lookup is split in two so that many of them can be issued together:
+  // locate() finds the bitmap word/bit of an absolute, in-grid (already
+  // mapped) point and whether the point lies inside the frame; valueAt()
+  // turns the loaded word and prefix into the value.  Outside the frame the
+  // value is that of the run holding the nearest frame position before it.
+  struct Loc {
+    uint32_t w, bt;
+    bool inside;
+  };
+  LSG_HD Loc locate(int32_t x, int32_t y, int32_t z) const {
+    const int64_t xr = (int64_t)x - f.o[0];
+    const int64_t yr = (int64_t)y - f.o[1];
+    const int64_t zr = (int64_t)z - f.o[2];
+    uint64_t lin;
     bool inside = false;
     if (zr < 0) {
-      return val[0];
+      lin = 0;
     } else if (zr >= f.ext[2]) {
-      rank = (uint32_t)f.base + pre[f.nwords];
-      return val[rank - 1];
+      lin = (uint64_t)f.nwords << 6;
     } else {
       uint64_t yc, xc;
       if (yr < 0) { yc = 0; xc = 0; }
-      else if (yr >= f.ext[1]) { yc = (uint64_t)1 << f.ly; xc = 0; }
+      else if (yr >= f.ext[1]) { yc = (uint64_t)f.ext[1]; xc = 0; }
       else {
         yc = (uint64_t)yr;
         if (xr < 0) xc = 0;
-        else if (xr >= f.ext[0]) xc = (uint64_t)1 << f.lx;
+        else if (xr >= f.ext[0]) xc = f.rowBits();
         else { xc = (uint64_t)xr; inside = true; }
       }
-      uint64_t lin = ((((uint64_t)zr << f.ly) + yc) << f.lx) + xc;
-      uint64_t w = lin >> 6;
-      uint32_t bt = (uint32_t)(lin & 63);
-      uint64_t word = bits[w];
-      rank = (uint32_t)f.base + pre[w] + (uint32_t)__builtin_popcountll(word & ((bt == 0) ? 0ull : (~0ull >> (64 - bt))));
-      if (inside && ((word >> bt) & 1ull)) return val[rank];
-      return val[rank - 1];
+      lin = ((uint64_t)zr * (uint64_t)f.ext[1] + yc) * f.rowBits() + xc;
     }
+    Loc l;
+    l.w = (uint32_t)(lin >> 6);
+    l.bt = (uint32_t)(lin & 63);
+    l.inside = inside;
+    return l;
+  }
+  LSG_HD uint32_t entryOf(const Loc &l, uint64_t word, uint32_t prefix) const {
+    const uint32_t rank =
+        (uint32_t)f.base + prefix + (uint32_t)__builtin_popcountll(word & ((l.bt == 0) ? 0ull : (~0ull >> (64 - l.bt))));
+    if (l.inside && ((word >> l.bt) & 1ull)) return rank;
+    return rank > 0 ? rank - 1 : 0;
+  }
+  // Values at N located points: all word/prefix loads, then all value loads.
+  template <int N> LSG_HD void values(const Loc *l, double *out) const {
+    uint64_t wd[N];
+    uint32_t pr[N];
+    LSG_UNROLL
+    for (int k = 0; k < N; ++k) {
+      wd[k] = bits[l[k].w];
+      pr[k] = pre[l[k].w];
+    }
+    LSG_UNROLL
+    for (int k = 0; k < N; ++k) out[k] = val[entryOf(l[k], wd[k], pr[k])];
+  }
+  LSG_HD double value(int32_t x, int32_t y, int32_t z) const {
+    const Loc l = locate(x, y, z);
+    return val[entryOf(l, bits[l.w], pre[l.w])];
   }
   // Value at centre c displaced by off along dimension d (boundary mapped).
   LSG_HD double at(const int32_t c[3], int d, int off) const {

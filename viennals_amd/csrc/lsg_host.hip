@@ -64,11 +64,6 @@ static int ctxInit() {
   return 0;
 }
 
-static int ceilLog2(int64_t v) {
-  int l = 0;
-  while (((int64_t)1 << l) < v) ++l;
-  return l;
-}
 
 // Device band storage sized for a frame (words) and an entry capacity.
 struct Store {
@@ -179,12 +174,10 @@ static int makeFrame(const lsg_domain *d, const int32_t *bbmin, const int32_t *b
       f.ext[k] = (int32_t)(hi - lo + 1);
     }
   }
-  f.lx = std::max(6, ceilLog2(f.ext[0]));
-  f.ly = ceilLog2(f.ext[1]);
-  int64_t bitsTotal = ((int64_t)f.ext[2] << f.ly) << f.lx;
+  const int64_t bitsTotal = (int64_t)((f.ext[0] + 63) >> 6) * 64 * f.ext[1] * f.ext[2];
   if (bitsTotal >= ((int64_t)1 << 32) - 64)
     return fail(-10, "level set frame exceeds 2^32 grid points; reduce the domain or band extent");
-  f.nwords = bitsTotal >> 6;
+  f.finish();
   f.base = gridMinInFrame(d, f) ? 0 : 1;
   return 0;
 }

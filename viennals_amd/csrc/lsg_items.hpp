@@ -154,12 +154,53 @@ LSG_HD inline double ruleUpdate(double c, double dt, double velocity) {
 
 // ------------------------------------------------------- neighbourhood ----
 
+// Centre value and the 2D star neighbours (boundary mapped) of point c:
+// nb[d] = +e_d, nb[D + d] = -e_d.  All 2D + 1 lookups are issued together.
 template <int D> LSG_HD inline void gatherStar(const BandView &b, const int32_t c[3], double &center, double *nb) {
-  center = b.value(c[0], c[1], c[2]);
+  BandView::Loc l[2 * D + 1];
+  l[2 * D] = b.locate(c[0], c[1], c[2]);
   LSG_UNROLL
   for (int d = 0; d < D; ++d) {
-    nb[d] = b.at(c, d, 1);
-    nb[D + d] = b.at(c, d, -1);
+    int32_t p[3] = {c[0], c[1], c[2]};
+    p[d] = b.g.map(d, (int64_t)c[d] + 1);
+    l[d] = b.locate(p[0], p[1], p[2]);
+    int32_t q[3] = {c[0], c[1], c[2]};
+    q[d] = b.g.map(d, (int64_t)c[d] - 1);
+    l[D + d] = b.locate(q[0], q[1], q[2]);
+  }
+  double v[2 * D + 1];
+  b.values<2 * D + 1>(l, v);
+  LSG_UNROLL
+  for (int k = 0; k < 2 * D; ++k) nb[k] = v[k];
+  center = v[2 * D];
+}
+
+// Values along every axis of point c up to distance R (boundary mapped):
+// st[d][R + k] = phi(c + k e_d) for 0 < |k| <= R; st[d][R] is left to the
+// caller (the centre value).  All 2 D R lookups are issued together.
+template <int D, int R> LSG_HD inline void gatherAxes(const BandView &b, const int32_t c[3], double st[][2 * R + 1]) {
+  BandView::Loc l[2 * D * R];
+  LSG_UNROLL
+  for (int d = 0; d < D; ++d) {
+    LSG_UNROLL
+    for (int k = 1; k <= R; ++k) {
+      int32_t p[3] = {c[0], c[1], c[2]};
+      p[d] = b.g.map(d, (int64_t)c[d] + k);
+      l[(d * R + k - 1) * 2] = b.locate(p[0], p[1], p[2]);
+      int32_t q[3] = {c[0], c[1], c[2]};
+      q[d] = b.g.map(d, (int64_t)c[d] - k);
+      l[(d * R + k - 1) * 2 + 1] = b.locate(q[0], q[1], q[2]);
+    }
+  }
+  double v[2 * D * R];
+  b.values<2 * D * R>(l, v);
+  LSG_UNROLL
+  for (int d = 0; d < D; ++d) {
+    LSG_UNROLL
+    for (int k = 1; k <= R; ++k) {
+      st[d][R + k] = v[(d * R + k - 1) * 2];
+      st[d][R - k] = v[(d * R + k - 1) * 2 + 1];
+    }
   }
 }
 
@@ -181,15 +222,11 @@ LSG_HD inline void candCoords(const BandView &b, uint32_t i, uint32_t p, int32_t
 // first point of every row when x is bounded (successor of the previous
 // row's last point).
 template <int D> LSG_HD inline uint64_t candStarWord(const uint64_t *bits, const Frame &f, const GridInfo &g, int64_t w) {
-  const int wpr = f.wpr();
-  const int wshift = f.lx - 6;
+  const int wpr = f.wpr;
   const int lastBit = f.ext[0] - 1;
   const int lastWord = lastBit >> 6;
-  const int kw = (int)(w & (wpr - 1));
-  const int64_t row = w >> wshift;
-  const int32_t yr = (int32_t)(row & ((1 << f.ly) - 1));
-  const int32_t zr = (int32_t)(row >> f.ly);
-  if (!(yr < f.ext[1] && zr < f.ext[2] && kw <= lastWord)) return 0;
+  int32_t kw, yr, zr;
+  f.wordRow((uint32_t)w, kw, yr, zr);
   const uint64_t B = bits[w];
   const uint64_t BL = kw > 0 ? bits[w - 1] : 0;
   const uint64_t BH = kw + 1 < wpr ? bits[w + 1] : 0;
@@ -214,8 +251,8 @@ template <int D> LSG_HD inline uint64_t candStarWord(const uint64_t *bits, const
       if (rr < 0 || rr >= f.ext[d]) continue;
       int32_t ny = (d == 1) ? rr : yr;
       int32_t nz = (d == 2) ? rr : zr;
-      int64_t nrow = ((int64_t)nz << f.ly) + ny;
-      int64_t nwi = (nrow << wshift) + kw;
+      int64_t nrow = (int64_t)nz * f.ext[1] + ny;
+      int64_t nwi = nrow * wpr + kw;
       dil |= bits[nwi];
       if (kw > 0) prevTop |= bits[nwi - 1] >> 63;
     }
@@ -227,6 +264,52 @@ template <int D> LSG_HD inline uint64_t candStarWord(const uint64_t *bits, const
     if (nb < 64) C &= (~0ull >> (64 - nb));
   }
   return C;
+}
+
+// Parallel bit deposit: the low popcount(m) bits of x placed, in order, at
+// the set bits of m (Hacker's Delight, 2nd ed., 7-5 "expand", 64-bit).
+LSG_HD inline uint64_t depositBits(uint64_t x, uint64_t m) {
+  const uint64_t m0 = m;
+  uint64_t mk = ~m << 1;
+  uint64_t mvs[6];
+  LSG_UNROLL
+  for (int i = 0; i < 6; ++i) {
+    uint64_t mp = mk ^ (mk << 1);
+    mp ^= mp << 2;
+    mp ^= mp << 4;
+    mp ^= mp << 8;
+    mp ^= mp << 16;
+    mp ^= mp << 32;
+    const uint64_t mv = mp & m;
+    mvs[i] = mv;
+    m = (m ^ mv) | (mv >> (1 << i));
+    mk &= ~mp;
+  }
+  LSG_UNROLL
+  for (int i = 5; i >= 0; --i) {
+    const uint64_t mv = mvs[i];
+    const uint64_t t = x << (1 << i);
+    x = (x & ~mv) | (t & mv);
+  }
+  return x & m0;
+}
+
+// Index of the k-th (0-based) set bit of w (k < popcount(w)).
+LSG_HD inline int selectBit(uint64_t w, int k) {
+  int pos = 0;
+  LSG_UNROLL
+  for (int sh = 32; sh >= 1; sh >>= 1) {
+    const uint64_t low = w & ((sh == 64) ? ~0ull : ((1ull << sh) - 1ull));
+    const int c = __builtin_popcountll(low);
+    if (k >= c) {
+      k -= c;
+      w >>= sh;
+      pos += sh;
+    } else {
+      w = low;
+    }
+  }
+  return pos;
 }
 
 // Keep flag of candidate i: first entry, defined, or start of a run whose
@@ -296,20 +379,20 @@ LSG_HD inline Vel velOf(const RateParams &rp) {
 // One-sided differences with the second-order ENO correction
 // (lsEngquistOsher.hpp:59-111; identical in lsLaxFriedrichs.hpp:66-119,
 // lsLocalLaxFriedrichs.hpp:85-143, lsLocalLocalLaxFriedrichs.hpp:62-115).
-LSG_HD inline void upwindDiffs(const BandView &b, const int32_t c[3], int i, int order, double phi0, double &diffPos,
+LSG_HD inline void upwindDiffs(const double *st, double gridDelta, int order, double phi0, double &diffPos,
                                double &diffNeg) {
-  const double gridDelta = b.g.dx;
+  // st: values at offsets -2..2 along the axis (st[2] unused)
   const double deltaPos = gridDelta;
   const double deltaNeg = -gridDelta;
-  const double phiPos = b.at(c, i, 1);
-  const double phiNeg = b.at(c, i, -1);
+  const double phiPos = st[3];
+  const double phiNeg = st[1];
   diffPos = (phiPos - phi0) / deltaPos;
   diffNeg = (phiNeg - phi0) / deltaNeg;
   if (order == 2) {
     const double deltaPosPos = 2 * gridDelta;
     const double deltaNegNeg = -2 * gridDelta;
-    const double phiPosPos = b.at(c, i, 2);
-    const double phiNegNeg = b.at(c, i, -2);
+    const double phiPosPos = st[4];
+    const double phiNegNeg = st[0];
     const double diff00 = (((deltaNeg * phiPos - deltaPos * phiNeg) / (deltaPos - deltaNeg) + phi0)) / (deltaPos * deltaNeg);
     const double diffNegNeg =
         (((deltaNeg * phiNegNeg - deltaNegNeg * phiNeg) / (deltaNegNeg - deltaNeg) + phi0)) / (deltaNegNeg * deltaNeg);
@@ -322,6 +405,23 @@ LSG_HD inline void upwindDiffs(const BandView &b, const int32_t c[3], int i, int
     if (signbit(diff00) == signbit(diffNegNeg)) {
       if (fabs(diffNegNeg * deltaNeg) < fabs(diff00 * deltaPos)) diffNeg -= deltaNeg * diffNegNeg;
       else diffNeg += deltaPos * diff00;
+    }
+  }
+}
+
+// Axis values at offsets -2..2 for the first/second-order schemes (the
+// second-order layer only when order == 2).
+template <int D> LSG_HD inline void gatherUpwind(const BandView &b, const int32_t c[3], int order, double st[][5]) {
+  if (order == 2) {
+    gatherAxes<D, 2>(b, c, st);
+  } else {
+    double s1[D][3];
+    gatherAxes<D, 1>(b, c, s1);
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d) {
+      st[d][1] = s1[d][0];
+      st[d][3] = s1[d][2];
+      st[d][0] = st[d][4] = 0.;
     }
   }
 }
@@ -457,10 +557,12 @@ LSG_HD inline RateOut schemeEO(const BandView &b, const int32_t c[3], double phi
                                const Vel &vel) {
   double gradPos[D], gradNeg[D];
   double gradPosTotal = 0, gradNegTotal = 0;
+  double st[D][5];
+  gatherUpwind<D>(b, c, order, st);
   LSG_UNROLL
   for (int i = 0; i < D; i++) {
     double diffPos, diffNeg;
-    upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
+    upwindDiffs(st[i], b.g.dx, order, phi0, diffPos, diffNeg);
     gradPos[i] = diffNeg;
     gradNeg[i] = diffPos;
     gradPosTotal += pow2(dmaxs(diffNeg, 0.0)) + pow2(dmins(diffPos, 0.0));
@@ -472,8 +574,8 @@ LSG_HD inline RateOut schemeEO(const BandView &b, const int32_t c[3], double phi
     double denominator = 0;
     LSG_UNROLL
     for (int i = 0; i < D; i++) {
-      double pos = b.at(c, i, 1) - phi0;
-      double neg = phi0 - b.at(c, i, -1);
+      double pos = st[i][3] - phi0;
+      double neg = phi0 - st[i][1];
       n[i] = (pos + neg) * 0.5;
       denominator += n[i] * n[i];
     }
@@ -499,15 +601,12 @@ LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double p
   constexpr int r = (ORDER + 1) / 2;
   double gradPosTotal = 0, gradNegTotal = 0;
   double gm[D], gp[D];
-  double st[2 * r + 1];
+  double sts[D][2 * r + 1];
+  gatherAxes<D, r>(b, c, sts);
   LSG_UNROLL
   for (int i = 0; i < D; i++) {
+    double *st = sts[i];
     st[r] = phi0;
-    LSG_UNROLL
-    for (int k = 1; k <= r; ++k) {
-      st[r + k] = b.at(c, i, k);
-      st[r - k] = b.at(c, i, -k);
-    }
     if (ORDER == 3) {
       gm[i] = weno3(st, b.g.dx, false);
       gp[i] = weno3(st, b.g.dx, true);
@@ -527,7 +626,7 @@ LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double p
     double denominator = 0;
     LSG_UNROLL
     for (int i = 0; i < D; i++) {
-      n[i] = b.at(c, i, 1) - b.at(c, i, -1);
+      n[i] = sts[i][r + 1] - sts[i][r - 1];
       denominator += n[i] * n[i];
     }
     if (denominator > 0) {
@@ -555,10 +654,12 @@ LSG_HD inline RateOut schemeLF(const BandView &b, const int32_t c[3], double phi
   double grad = 0., dissipation = 0.;
   double n[3] = {0, 0, 0};
   double normalModulus = 0;
+  double st[D][5];
+  gatherUpwind<D>(b, c, order, st);
   LSG_UNROLL
   for (int i = 0; i < D; i++) {
     double diffPos, diffNeg;
-    upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
+    upwindDiffs(st[i], b.g.dx, order, phi0, diffPos, diffNeg);
     gradPos[i] = diffNeg;
     gradNeg[i] = diffPos;
     if (calcNormals) {
@@ -592,10 +693,12 @@ LSG_HD inline RateOut schemeLLLF(const BandView &b, const int32_t c[3], double p
   double grad = 0., dissipation = 0.;
   double n[3] = {0, 0, 0};
   double normalModulus = 0;
+  double st[D][5];
+  gatherUpwind<D>(b, c, order, st);
   LSG_UNROLL
   for (int i = 0; i < D; i++) {
     double diffPos, diffNeg;
-    upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
+    upwindDiffs(st[i], b.g.dx, order, phi0, diffPos, diffNeg);
     gradPos[i] = diffNeg;
     gradNeg[i] = diffPos;
     n[i] = (diffNeg + diffPos) * 0.5;
@@ -631,10 +734,12 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
   double grad = 0., dissipation = 0.;
   double n[3] = {0, 0, 0};
   double normalModulus = 0;
+  double st[D][5];
+  gatherUpwind<D>(b, c, order, st);
   LSG_UNROLL
   for (int i = 0; i < D; i++) {
     double diffPos, diffNeg;
-    upwindDiffs(b, c, i, order, phi0, diffPos, diffNeg);
+    upwindDiffs(st[i], b.g.dx, order, phi0, diffPos, diffNeg);
     gradPos[i] = diffNeg;
     gradNeg[i] = diffPos;
     n[i] = (diffNeg + diffPos) * 0.5;
@@ -665,7 +770,8 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
     for (int d = 0; d < D; ++d) nc[d] = b.g.map(d, (int64_t)c[d] + ni[d]);
     double normal[3] = {0, 0, 0};
     double nm = 0.;
-    const double center = b.value(nc[0], nc[1], nc[2]);
+    BandView::Loc l[2 * D + 1];
+    l[2 * D] = b.locate(nc[0], nc[1], nc[2]);
     LSG_UNROLL
     for (int dir = 0; dir < D; ++dir) {
       int32_t pp[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
@@ -674,8 +780,16 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
         pp[d] = b.g.map(d, (int64_t)c[d] + ni[d] + (d == dir ? 1 : 0));
         pn[d] = b.g.map(d, (int64_t)c[d] + ni[d] - (d == dir ? 1 : 0));
       }
-      const double neg = b.value(pn[0], pn[1], pn[2]);
-      const double pos = b.value(pp[0], pp[1], pp[2]);
+      l[dir] = b.locate(pp[0], pp[1], pp[2]);
+      l[D + dir] = b.locate(pn[0], pn[1], pn[2]);
+    }
+    double bv[2 * D + 1];
+    b.values<2 * D + 1>(l, bv);
+    const double center = bv[2 * D];
+    LSG_UNROLL
+    for (int dir = 0; dir < D; ++dir) {
+      const double neg = bv[D + dir];
+      const double pos = bv[dir];
       const double dP = (pos - center) / b.g.dx;
       const double dN = (center - neg) / b.g.dx;
       normal[dir] = (dP + dN) * 0.5;
@@ -737,9 +851,11 @@ LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi
 template <int D> LSG_HD inline void globalAlphaPoint(const BandView &b, const int32_t c[3], const Vel &vel, double *la) {
   double normal[3] = {0, 0, 0};
   double nm = 0.;
+  double st[D][3];
+  gatherAxes<D, 1>(b, c, st);
   LSG_UNROLL
   for (int k = 0; k < D; ++k) {
-    normal[k] = b.at(c, k, 1) - b.at(c, k, -1);
+    normal[k] = st[k][2] - st[k][0];
     nm += normal[k] * normal[k];
   }
   nm = 1. / sqrt(nm);

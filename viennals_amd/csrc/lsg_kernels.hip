@@ -131,10 +131,15 @@ __global__ __launch_bounds__(256) void kCandCount(const uint64_t *A, const uint6
 }
 
 // Phase 2: word prefix counts and the candidate position list (index
-// base + cpre[w] + j); the last block stores the totals.
+// base + cpre[w] + j); the last block stores the totals.  The positions of a
+// tile are written by consecutive threads (coalesced): thread s finds the
+// word holding the tile's s-th candidate by binary search over the tile's
+// prefix counts in LDS and selects the bit.
 __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, int64_t nw64, int base, const uint32_t *partials, uint32_t *cpre,
                           uint32_t *cpos, size_t ccap, uint32_t *ncand, DevScalars *sc) {
   if (sc->overflow) return;
+  __shared__ uint64_t sWord[TPB];
+  __shared__ uint32_t sPre[TPB];
   const uint32_t nw = (uint32_t)nw64;
   uint32_t lo, hi;
   blockChunk(nw, lo, hi);
@@ -142,20 +147,27 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, int64_t 
   if (blockIdx.x == 0 && threadIdx.x == 0 && base) cpos[0] = 0xFFFFFFFFu;
   for (uint32_t t = lo; t < hi; t += TPB) {
     const uint32_t w = t + threadIdx.x;
-    uint64_t c = (w < hi) ? cbits[w] : 0ull;
+    const uint64_t c = (w < hi) ? cbits[w] : 0ull;
     const uint32_t pc = (uint32_t)__builtin_popcountll(c);
     uint32_t tot;
-    const uint32_t p = off + blockExclusiveScan(pc, tot);
-    if (w < hi) {
-      cpre[w] = p;
-      size_t q = (size_t)base + p;
-      if (q + pc <= ccap) {
-        while (c) {
-          cpos[q++] = (uint32_t)((w << 6) | (uint32_t)__builtin_ctzll(c));
-          c &= c - 1;
-        }
+    const uint32_t pl = blockExclusiveScan(pc, tot);
+    if (w < hi) cpre[w] = off + pl;
+    sWord[threadIdx.x] = c;
+    sPre[threadIdx.x] = pl;
+    __syncthreads();
+    const size_t q0 = (size_t)base + off;
+    for (uint32_t s = threadIdx.x; s < tot; s += TPB) {
+      if (q0 + s >= ccap) break;
+      int a = 0, b = TPB - 1;  // last j with sPre[j] <= s
+      while (a < b) {
+        const int m = (a + b + 1) >> 1;
+        if (sPre[m] <= s) a = m;
+        else b = m - 1;
       }
+      const int bit = selectBit(sWord[a], (int)(s - sPre[a]));
+      cpos[q0 + s] = ((t + (uint32_t)a) << 6) | (uint32_t)bit;
     }
+    __syncthreads();
     off += tot;
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
@@ -343,12 +355,7 @@ __global__ void kBuildIndex(const uint64_t *cwords, const uint32_t *cpre, const 
       uint64_t m = keepBits[q] >> s;
       if (s != 0 && s + len > 64) m |= keepBits[q + 1] << (64 - s);
       if (len < 64) m &= (1ull << len) - 1ull;
-      while (src) {  // deposit m onto the set bits of src, lowest first
-        const uint64_t low = src & (~src + 1ull);
-        if (m & 1ull) word |= low;
-        m >>= 1;
-        src ^= low;
-      }
+      word = (src == ~0ull) ? m : depositBits(m, src);  // keep flags onto the candidate bits
     }
     bitsOut[w] = word;
   }
