@@ -77,21 +77,34 @@ __device__ inline uint32_t blockSum(uint32_t v) {
   return t;
 }
 
-// Contiguous chunk [lo, hi) of block blockIdx.x for a device-side size n.
+// Logical block index: workgroups are dispatched round-robin over the 8 XCDs
+// (each with its own L2), so consecutive logical blocks are mapped to the
+// same XCD; a contiguous 1/8 of the data then lives in one L2 and stencil
+// neighbours (rows above and below) are found there.
+static constexpr uint32_t NUM_XCD = 8;
+__device__ inline uint32_t logicalBlock() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t xcd = b % NUM_XCD, k = b / NUM_XCD;
+  const uint32_t per = nb / NUM_XCD, rem = nb % NUM_XCD;
+  return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+// Contiguous chunk [lo, hi) of the (logical) block for a device-side size n.
 __device__ inline void blockChunk(uint32_t n, uint32_t &lo, uint32_t &hi) {
   const uint32_t tile = TPB * SCAN_ITEMS;
   uint32_t per = (n + gridDim.x - 1) / gridDim.x;
   per = (per + tile - 1) / tile * tile;
-  lo = blockIdx.x * per;
+  lo = logicalBlock() * per;
   hi = lo + per;
   if (lo > n) lo = n;
   if (hi > n) hi = n;
 }
 
-// Sum of partials[0 .. blockIdx.x) computed by the whole block.
+// Sum of partials[0 .. logical block) computed by the whole block.
 __device__ inline uint32_t blockOffset(const uint32_t *partials) {
   uint32_t s = 0;
-  for (uint32_t k = threadIdx.x; k < blockIdx.x; k += TPB) s += partials[k];
+  const uint32_t me = logicalBlock();
+  for (uint32_t k = threadIdx.x; k < me; k += TPB) s += partials[k];
   return blockSum(s);
 }
 
@@ -127,7 +140,7 @@ __global__ __launch_bounds__(256) void kCandCount(const uint64_t *A, const uint6
     acc += (uint32_t)__builtin_popcountll(C);
   }
   uint32_t t = blockSum(acc);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  if (threadIdx.x == 0) partials[logicalBlock()] = t;
 }
 
 // Phase 2: word prefix counts and the candidate position list (index
@@ -144,7 +157,7 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, int64_t 
   uint32_t lo, hi;
   blockChunk(nw, lo, hi);
   uint32_t off = blockOffset(partials);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && base) cpos[0] = 0xFFFFFFFFu;
+  if (logicalBlock() == 0 && threadIdx.x == 0 && base) cpos[0] = 0xFFFFFFFFu;
   for (uint32_t t = lo; t < hi; t += TPB) {
     const uint32_t w = t + threadIdx.x;
     const uint64_t c = (w < hi) ? cbits[w] : 0ull;
@@ -170,7 +183,7 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, int64_t 
     __syncthreads();
     off += tot;
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+  if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) {
     cpre[nw] = off;
     uint64_t nc = (uint64_t)off + (uint64_t)base;
     if (nc > ccap) {
@@ -278,7 +291,7 @@ template <class Ev> __global__ __launch_bounds__(256) void kEvalKeep(Ev ev, cons
     __syncthreads();
   }
   const uint32_t tot = blockSum(acc);
-  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+  if (threadIdx.x == 0) partials[logicalBlock()] = tot;
 }
 
 // Phase 2: rescan the keep flags from the block offset, scatter the kept
@@ -318,7 +331,7 @@ __global__ __launch_bounds__(256) void kKeepScatter(const double *out, const uin
     if (lane == 0 && g < hi) keepBits[g >> 6] = wm;
     off += tot;
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+  if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) {
     ks[n] = off;
     uint32_t tt = off;
     if (tt > cap) {
@@ -337,7 +350,9 @@ __global__ void kBuildIndex(const uint64_t *cwords, const uint32_t *cpre, const 
                             DevScalars *sc) {
   LSG_ABORT_IF_OVERFLOW(sc);
   const uint32_t nc = *ncand;
-  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w <= nw; w += (int64_t)gridDim.x * blockDim.x) {
+  uint32_t lo, hi;
+  blockChunk((uint32_t)nw + 1, lo, hi);
+  for (int64_t w = lo + threadIdx.x; w < (int64_t)hi; w += TPB) {
     uint32_t c0 = (uint32_t)base + cpre[w];
     if (c0 > nc) c0 = nc;
     preOut[w] = ks[c0] - (uint32_t)base;
@@ -461,16 +476,19 @@ template <class Fn> __device__ inline void forEachActive(const double *val, uint
   const int wid = threadIdx.x >> 6;
   uint32_t *q = queue[wid];
   uint32_t cnt = 0;
-  const uint32_t stride = gridDim.x * (TPB / 64) * 64u;
-  uint32_t base = (blockIdx.x * (TPB / 64) + wid) * 64u;
+  // the block's contiguous chunk of entries, 64-entry slices dealt to its waves
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi);
+  const uint32_t stride = (TPB / 64) * 64u;
+  uint32_t base = lo + (uint32_t)wid * 64u;
   // One call site of fn, so that it is inlined once: the loop alternates
   // between queueing one 64-entry slice and draining a full wave of queued
   // entries; after the last slice the remainder (< 64) is drained.
   for (;;) {
-    if (base < n) {
+    if (base < hi) {
       const uint32_t i = base + lane;
       bool act = false;
-      if (i < n) {
+      if (i < hi) {
         const double v = val[i];
         act = !isUndef(v) && fabs(v) <= 0.5;
       }
@@ -480,7 +498,7 @@ template <class Fn> __device__ inline void forEachActive(const double *val, uint
       base += stride;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    const bool last = base >= n;
+    const bool last = base >= hi;
     if (cnt >= 64 || (last && cnt > 0)) {
       const uint32_t take = cnt < 64 ? cnt : 64u;
       if ((uint32_t)lane < take) fn(q[lane]);

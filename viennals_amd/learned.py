@@ -192,7 +192,7 @@ def _dense(idx, val, pad=8):
 
 def _weno5_parts(st):
     """d, beta, candidate polynomials of weno5 minus-stencils st (n, 7)."""
-    d = np.diff(st, axis=1)  # d_i = x_{i+1} - x_i
+    d = np.diff(st, axis=1)[:, :5]  # d_i = x_{i+1} - x_i, i = 0..4 (x_6 is not used)
     b1 = 13.0 / 12.0 * (d[:, 0] - 2 * d[:, 1] + d[:, 2]) ** 2 + 0.25 * (d[:, 0] - 4 * d[:, 1] + 3 * d[:, 2]) ** 2
     b2 = 13.0 / 12.0 * (d[:, 1] - 2 * d[:, 2] + d[:, 3]) ** 2 + 0.25 * (d[:, 1] - d[:, 3]) ** 2
     b3 = 13.0 / 12.0 * (d[:, 2] - 2 * d[:, 3] + d[:, 4]) ** 2 + 0.25 * (3 * d[:, 2] - 4 * d[:, 3] + d[:, 4]) ** 2
@@ -305,6 +305,8 @@ class Trainer:
     def set_samples(self, X, T):
         X = np.ascontiguousarray(X, dtype=np.float32)
         T = np.ascontiguousarray(T, dtype=np.float32)
+        if X.ndim != 2 or X.shape[1] != 8 or X.shape[0] != len(T):
+            raise ValueError(f"samples must be (n, 8) with n targets, got {X.shape} and {T.shape}")
         self.n = len(T)
         fp = ctypes.POINTER(ctypes.c_float)
         check(lib().lsg_mlp_set_samples(self._h, self.n, X.ctypes.data_as(fp), T.ctypes.data_as(fp)))
