@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -88,6 +90,101 @@ def build_workload(vls, name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def build_batch(vls, n=512, seed=42):
+    """512 independent level sets of uneven size: half spheres (radius
+    U[8,64] cells, constant speed), half square trenches (width U[16,96],
+    depth U[32,192], lateral domain U[128,512] cells, reflective x/y,
+    directional etch); speeds U[0.2,1.0]; gridDelta 1."""
+    BC = vls.BoundaryConditionEnum
+    rng = np.random.default_rng(seed)
+    sets, vels = [], []
+    for k in range(n):
+        rate = float(rng.uniform(0.2, 1.0))
+        if k % 2 == 0:
+            r = float(rng.uniform(8, 64))
+            d = vls.Domain(gridDelta=1.0, dim=3)
+            vls.MakeGeometry(d, vls.Sphere([0.0, 0.0, 0.0], r)).apply()
+            vels.append(vls.VelocityField.constant(rate))
+        else:
+            w = float(rng.uniform(16, 96))
+            depth = float(rng.uniform(32, 192))
+            half = int(rng.uniform(128, 512)) // 2
+            w = min(w, half)
+            bounds = [-half, half, -half, half, -half, half]
+            bcs = [BC.REFLECTIVE_BOUNDARY, BC.REFLECTIVE_BOUNDARY, BC.INFINITE_BOUNDARY]
+            d = vls.Domain(bounds, bcs, 1.0)
+            vls.MakeGeometry(d, vls.Plane([0.0, 0.0, 0.0], [0.0, 0.0, 1.0])).apply()
+            box = vls.Domain(bounds, bcs, 1.0)
+            vls.MakeGeometry(box, vls.Box([-w / 2, -w / 2, -depth], [w / 2, w / 2, 1.0])).apply()
+            vls.BooleanOperation(d, box, vls.BooleanOperationEnum.RELATIVE_COMPLEMENT).apply()
+            vels.append(vls.VelocityField.normal_power(dir=2, k=1, b=-0.1 * rate, a_pos=-rate, a_neg=0.0))
+        sets.append(d)
+    return sets, vels
+
+
+def run_batch(args):
+    import viennals_amd as vls
+    t0 = time.perf_counter()
+    sets, vels = build_batch(vls, args.batch_size)
+    setup = time.perf_counter() - t0
+    W = vls.SpatialSchemeEnum.WENO_5TH_ORDER
+    R3 = vls.TemporalSchemeEnum.RUNGE_KUTTA_3RD_ORDER
+    # sequential per-case launches (same code path, one Advect per level set)
+    seq_sets = []
+    for d in sets:
+        c = vls.Domain.from_index_bounds(d.D, d.gmin, d.gmax, d.bc, d.gridDelta)
+        c.deepCopy(d)
+        seq_sets.append(c)
+    bat = vls.BatchAdvect(sets, vels, spatial=W, temporal=R3, cfl=0.4)
+    blocks, tune_ms = bat.autotune()
+    for _ in range(args.warmup):
+        bat.apply()
+    vls.synchronize()
+    pu0, _ = bat.pointUpdates()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bat.apply()
+    vls.synchronize()
+    wall = time.perf_counter() - t0
+    pu = bat.pointUpdates()[0] - pu0
+    del bat
+    advs = []
+    for d, v in zip(seq_sets, vels):
+        a = vls.Advect(d, v)
+        a.setSpatialScheme(W)
+        a.setTemporalScheme(R3)
+        a.setTimeStepRatio(0.4)
+        advs.append(a)
+    for a in advs:  # warm-up step (captures each level set's graphs)
+        a.apply()
+    vls.synchronize()
+    seq_pu = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.seq_steps):
+        for a in advs:
+            a.apply()
+            seq_pu += a.getPointUpdates()
+    vls.synchronize()
+    seq_wall = time.perf_counter() - t0
+    rec = {
+        "metric": "aggregate point-updates/s (batched 512 level sets, WENO5-RK3 3D)",
+        "value": pu / wall,
+        "unit": "point-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batch{args.batch_size}_weno5_rk3",
+                   "level_sets": "half spheres r~U[8,64], half trenches w~U[16,96] d~U[32,192] L~U[128,512]",
+                   "seed": 42, "active_point_updates_per_step": pu / args.steps, "setup_seconds": setup,
+                   "autotuned_scan_blocks": blocks, "autotune_best_step_ms": tune_ms},
+        "sequential": {"value": seq_pu / seq_wall, "unit": "point-updates/s", "steps": args.seq_steps,
+                       "ms_per_step": seq_wall * 1e3 / args.seq_steps,
+                       "what": "one Advect (graph-captured steps) per level set, launched one after another"},
+        "speedup_vs_sequential": (pu / wall) / (seq_pu / seq_wall),
+    }
+    print(json.dumps(rec))
+    return 0
+
+
 def cpu_baseline(idx, val, width, grid, params, steps, threads):
     """The oracle (CPU restatement, OpenMP) on the same input band."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -113,11 +210,16 @@ def main():
     ap.add_argument("--workload", default="trench256")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-size", type=int, default=512)
+    ap.add_argument("--seq-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU; "
               "multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
         return 2
+
+    if args.workload == "batch512":
+        return run_batch(args)
 
     import viennals_amd as vls
 

@@ -171,6 +171,31 @@ int lsg_mlp_loss_grad(lsg_mlp *m, int64_t offset, int64_t count, double *loss, f
 int lsg_mlp_train(lsg_mlp *m, int iters, int64_t batch, double lr, double *last_loss);
 int lsg_mlp_get_params(lsg_mlp *m, float *out);
 
+/* ---- Batches of independent level sets (no counterpart in the reference) ---
+ * Many Advect<T,D> instances (lsAdvect.hpp:54-1044), one level set each,
+ * stepped together: the bands are concatenated into one segmented band on
+ * the device, every kernel of a step is one launch for all level sets, the
+ * step is captured into a HIP graph, and each level set keeps its own grid,
+ * velocity and CFL time step.  lsg_batch_apply is Advect::apply with
+ * advectionTime 0 (one step) for every level set.  Schemes: Engquist-Osher
+ * and WENO.  Results are written back into the domains by
+ * lsg_batch_write_back. */
+typedef struct lsg_batch lsg_batch;
+int lsg_batch_create(lsg_batch **out, lsg_domain *const *domains, int n);
+int lsg_batch_destroy(lsg_batch *b);
+int lsg_batch_set_velocity_builtin(lsg_batch *b, int k, int kind, int dir, int kk, double b0, double a_pos,
+                                   double a_neg, const double *vec3);
+int lsg_batch_set_schemes(lsg_batch *b, int spatial, int temporal, double cfl);
+int lsg_batch_apply(lsg_batch *b);
+/* Accumulated advected time of every level set (n values). */
+int lsg_batch_get_advected_times(lsg_batch *b, double *t);
+int lsg_batch_get_point_updates(lsg_batch *b, double *n, unsigned *steps);
+int lsg_batch_write_back(lsg_batch *b);
+/* Grid of the scan kernels (blocks), or chosen by timing one step per
+ * candidate grid from the same band (the band is restored). */
+int lsg_batch_set_scan_blocks(lsg_batch *b, int blocks);
+int lsg_batch_autotune(lsg_batch *b, int *chosen, double *ms);
+
 /* Device synchronisation (for timing by callers). */
 int lsg_device_synchronize(void);
 

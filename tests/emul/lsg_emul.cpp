@@ -39,6 +39,7 @@ struct EBand {
     v.pos = pos.data();
     v.f = f;
     v.g = g;
+    v.rb = (uint32_t)f.base;
     return v;
   }
 };

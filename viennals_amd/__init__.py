@@ -14,7 +14,7 @@ from ._lib import LsgError, check, lib
 __all__ = [
     "BoundaryConditionEnum", "SpatialSchemeEnum", "TemporalSchemeEnum", "BooleanOperationEnum",
     "Domain", "Sphere", "Plane", "Box", "MakeGeometry", "Expand", "Reduce", "Prune", "BooleanOperation",
-    "VelocityField", "Advect", "LsgError", "weno",
+    "VelocityField", "Advect", "BatchAdvect", "LsgError", "weno",
 ]
 
 POS_VALUE = float(np.finfo(np.float64).max)
@@ -385,6 +385,62 @@ class Advect:
         b = ctypes.c_double()
         check(lib().lsg_advect_get_rate_kernel_stats(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)))
         return ms.value, n.value, b.value
+
+
+class BatchAdvect:
+    """Many independent Advect instances (one level set each) stepped
+    together on the GPU: one launch per kernel for all level sets, the step
+    captured into a HIP graph, a time step per level set.  apply() is
+    Advect::apply with advectionTime 0 for every level set; writeBack()
+    stores the results in the domains."""
+
+    def __init__(self, domains, velocities, spatial=SpatialSchemeEnum.WENO_5TH_ORDER,
+                 temporal=TemporalSchemeEnum.RUNGE_KUTTA_3RD_ORDER, cfl=0.4):
+        self._domains = list(domains)
+        arr = (ctypes.c_void_p * len(self._domains))(*[d._h.value for d in self._domains])
+        h = ctypes.c_void_p()
+        check(lib().lsg_batch_create(ctypes.byref(h), arr, len(self._domains)))
+        self._h = h
+        for k, v in enumerate(velocities):
+            vec = (ctypes.c_double * 3)(*v.vec)
+            check(lib().lsg_batch_set_velocity_builtin(h, k, v.kind, v.dir, v.k, v.b, v.a_pos, v.a_neg, vec))
+        check(lib().lsg_batch_set_schemes(h, int(spatial), int(temporal), float(cfl)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().lsg_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def apply(self):
+        check(lib().lsg_batch_apply(self._h))
+
+    def advectedTimes(self):
+        t = np.zeros(len(self._domains), np.float64)
+        check(lib().lsg_batch_get_advected_times(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return t
+
+    def pointUpdates(self):
+        n = ctypes.c_double()
+        s = ctypes.c_uint()
+        check(lib().lsg_batch_get_point_updates(self._h, ctypes.byref(n), ctypes.byref(s)))
+        return n.value, s.value
+
+    def writeBack(self):
+        check(lib().lsg_batch_write_back(self._h))
+
+    def setScanBlocks(self, blocks):
+        check(lib().lsg_batch_set_scan_blocks(self._h, int(blocks)))
+
+    def autotune(self):
+        """Times one step per candidate scan grid (band restored); returns (blocks, ms)."""
+        b = ctypes.c_int()
+        ms = ctypes.c_double()
+        check(lib().lsg_batch_autotune(self._h, ctypes.byref(b), ctypes.byref(ms)))
+        return b.value, ms.value
 
 
 def weno(order, stencils, dx, plus):

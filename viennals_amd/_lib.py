@@ -32,6 +32,9 @@ EXPORTED = [
     "lsg_advect_get_rate_kernel_stats", "lsg_weno_eval", "lsg_device_synchronize",
     "lsg_advect_set_learned_weights", "lsg_weno5_learned_eval", "lsg_mlp_create", "lsg_mlp_destroy",
     "lsg_mlp_set_samples", "lsg_mlp_loss_grad", "lsg_mlp_train", "lsg_mlp_get_params",
+    "lsg_batch_create", "lsg_batch_destroy", "lsg_batch_set_velocity_builtin", "lsg_batch_set_schemes",
+    "lsg_batch_apply", "lsg_batch_get_advected_times", "lsg_batch_get_point_updates", "lsg_batch_write_back",
+    "lsg_batch_set_scan_blocks", "lsg_batch_autotune",
 ]
 
 
@@ -104,6 +107,16 @@ def lib():
         "lsg_mlp_loss_grad": ([P, ctypes.c_int64, ctypes.c_int64, dp, fp], c_int),
         "lsg_mlp_train": ([P, c_int, ctypes.c_int64, c_double, dp], c_int),
         "lsg_mlp_get_params": ([P, fp], c_int),
+        "lsg_batch_create": ([ctypes.POINTER(P), ctypes.POINTER(P), c_int], c_int),
+        "lsg_batch_destroy": ([P], c_int),
+        "lsg_batch_set_velocity_builtin": ([P, c_int, c_int, c_int, c_int, c_double, c_double, c_double, dp], c_int),
+        "lsg_batch_set_schemes": ([P, c_int, c_int, c_double], c_int),
+        "lsg_batch_apply": ([P], c_int),
+        "lsg_batch_get_advected_times": ([P, dp], c_int),
+        "lsg_batch_get_point_updates": ([P, dp, ctypes.POINTER(ctypes.c_uint)], c_int),
+        "lsg_batch_write_back": ([P], c_int),
+        "lsg_batch_set_scan_blocks": ([P, c_int], c_int),
+        "lsg_batch_autotune": ([P, ip, dp], c_int),
         "lsg_device_synchronize": ([], c_int),
     }
     for name, (args, res) in sig.items():

@@ -137,13 +137,18 @@ struct Frame {
 };
 
 // Read-only view of a device band.
+// Frame position of an entry that is not in the frame: the grid minimum
+// point of a band whose frame does not contain it (entry 0 of the band).
+constexpr uint32_t POS_NONE = 0xFFFFFFFFu;
+
 struct BandView {
   const uint64_t *bits;  // nwords + 1
   const uint32_t *pre;   // nwords + 1 (exclusive prefix of popcounts, in-frame entries)
   const double *val;     // entries
-  const uint32_t *pos;   // entries (frame bit index; pos[0] unused when base == 1)
+  const uint32_t *pos;   // entries (frame bit index; POS_NONE for the grid-minimum entry)
   Frame f;
   GridInfo g;
+  uint32_t rb;           // entries before the band's first in-frame entry (its grid-minimum entry, if any)
 
   // Absolute coordinates of entry/candidate position p.
   LSG_HD void coords(uint32_t p, int32_t c[3]) const {
@@ -192,8 +197,7 @@ struct BandView {
     return l;
   }
   LSG_HD uint32_t entryOf(const Loc &l, uint64_t word, uint32_t prefix) const {
-    const uint32_t rank =
-        (uint32_t)f.base + prefix + (uint32_t)__builtin_popcountll(word & ((l.bt == 0) ? 0ull : (~0ull >> (64 - l.bt))));
+    const uint32_t rank = rb + prefix + (uint32_t)__builtin_popcountll(word & ((l.bt == 0) ? 0ull : (~0ull >> (64 - l.bt))));
     if (l.inside && ((word >> l.bt) & 1ull)) return rank;
     return rank > 0 ? rank - 1 : 0;
   }

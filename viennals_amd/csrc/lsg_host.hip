@@ -104,6 +104,7 @@ struct WsStore {
     (void)hipFree(w.keepBits);
     (void)hipFree(w.partials);
     (void)hipFree(w.ncand);
+    (void)hipFree(w.cseg);
     w = Workspace{};
     nwAlloc = 0;
   }
@@ -116,8 +117,9 @@ struct WsStore {
     LSG_HIP(hipMalloc(&w.out, ccap * sizeof(double)));
     LSG_HIP(hipMalloc(&w.ks, (ccap + 1) * sizeof(uint32_t)));
     LSG_HIP(hipMalloc(&w.keepBits, (ccap / 64 + 2) * sizeof(uint64_t)));
-    LSG_HIP(hipMalloc(&w.partials, (SCAN_BLOCKS + 1) * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&w.partials, (SCAN_BLOCKS_MAX + 1) * sizeof(uint32_t)));
     LSG_HIP(hipMalloc(&w.ncand, sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&w.cseg, ccap * sizeof(uint16_t)));
     w.ccap = ccap;
     nwAlloc = nw;
     return 0;
@@ -611,15 +613,13 @@ static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *d
   RateParams rp = a->rp;
   if (a->spatial == 2 || a->spatial == 3) launchGlobalAlphas(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, g_ctx.sc);
   if (a->profiling && a->evUsed + 2 <= (int)a->evPool.size()) {
+    // events bracket the rate kernel itself
     hipEvent_t e0 = a->evPool[a->evUsed], e1 = a->evPool[a->evUsed + 1];
     a->evUsed += 2;
-    if (a->capturing) (void)hipEventRecordWithFlags(e0, g_ctx.s, hipEventRecordExternal);
-    else (void)hipEventRecord(e0, g_ctx.s);
-    launchRates(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, a->rates, g_ctx.sc, maxDt, dtOut);
-    if (a->capturing) (void)hipEventRecordWithFlags(e1, g_ctx.s, hipEventRecordExternal);
-    else (void)hipEventRecord(e1, g_ctx.s);
+    launchRates(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, a->rates, g_ctx.sc, maxDt, dtOut, g_ws->w, e0, e1,
+                a->capturing);
   } else {
-    launchRates(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, a->rates, g_ctx.sc, maxDt, dtOut);
+    launchRates(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, a->rates, g_ctx.sc, maxDt, dtOut, g_ws->w);
   }
   // updateLevelSet: Reduce(1) + update (lsAdvect.hpp:686-800)
   OpParams op{};
@@ -908,6 +908,469 @@ static int advectApply(lsg_advect *a) {
   a->currentTimeStep = so.dt1;
   if (a->temporal == 0) a->currentTimeStep = so.stepTime;
   if (a->profiling) a->rateBytes = a->hostSc->rateBytes;
+  return 0;
+}
+
+
+// ================================================================ batch ===
+//
+// A batch advects many independent level sets with one launch per kernel:
+// their bands are concatenated into one segmented band (bitmap words, entries
+// and candidates tagged with their level set), every level set keeps its own
+// frame, grid, velocity and CFL time step (SegInfo table).  A step of the
+// batch is one Advect::apply with advectionTime 0 for every level set, and
+// is captured into a HIP graph like the single-level-set step.
+
+struct BStore {
+  BandBufs b{};
+  int64_t nwAlloc = 0;
+  void release() {
+    Store s;
+    s.b = b;
+    s.nwAlloc = nwAlloc;
+    s.release();
+    if (b.seg) (void)hipFree(b.seg);
+    b = BandBufs{};
+    nwAlloc = 0;
+  }
+  int alloc(int64_t nw, size_t cap) {
+    release();
+    Store s;
+    if (int e = s.alloc(nw, cap)) return e;
+    b = s.b;
+    s.b = BandBufs{};
+    nwAlloc = nw;
+    LSG_HIP(hipMalloc(&b.seg, cap * sizeof(uint16_t)));
+    return 0;
+  }
+};
+
+__global__ void kAddCopy32(const uint32_t *src, uint32_t *dst, uint32_t n, uint32_t add) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i] + add;
+}
+__global__ void kFill16(uint16_t *p, uint16_t v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void kSetScalar32(uint32_t *p, uint32_t v) { *p = v; }
+__global__ void kZero64(uint64_t *p) { *p = 0; }
+__global__ void kGather32(const uint32_t *src, const uint32_t *idx, uint32_t *dst, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+static int gridOf(size_t n) { return (int)std::min<size_t>(4096, std::max<size_t>(1, (n + 255) / 256)); }
+
+}  // namespace lsg
+
+struct lsg_batch {
+  int D = 3;
+  std::vector<lsg_domain *> doms;
+  std::vector<lsg::SegInfo> segsH;
+  std::vector<int> velSet;
+  lsg::SegInfo *segsD = nullptr;
+  uint16_t *wsegD = nullptr;
+  int64_t W = 0;
+  uint32_t totalBase = 0;
+  bool assembled = false;
+  lsg::BStore cur, tmp, icur, itmp, save;
+  int width = 2, iwidth = 2;
+  int spatial = 11, temporal = 2;
+  double cfl = 0.4;
+  lsg::RateParams rp{};
+  double *rates = nullptr;
+  size_t ratesCap = 0;
+  unsigned long long *segMax = nullptr;
+  double *segMaxDt = nullptr, *segDt = nullptr, *segStepTime = nullptr, *segAdvTime = nullptr;
+  uint32_t *bounds = nullptr, *boundsIdx = nullptr;  // pre[] at the level-set word boundaries (write-back)
+  double pointUpdates = 0;
+  unsigned steps = 0;
+  int scanBlocks = lsg::SCAN_BLOCKS;
+  bool useGraphs = true;
+  struct Graph {
+    uint64_t key = 0;
+    hipGraphExec_t exec = nullptr;
+    lsg::BStore cur, tmp, icur, itmp;
+    int width = 0, iwidth = 0;
+  };
+  Graph graphs[2];
+  int nextGraph = 0;
+  lsg::DevScalars *hostSc = nullptr;
+  void dropGraphs() {
+    for (auto &g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = Graph{};
+    }
+  }
+  void freeDevice() {
+    dropGraphs();
+    cur.release();
+    tmp.release();
+    icur.release();
+    itmp.release();
+    save.release();
+    for (void *p : {(void *)segsD, (void *)wsegD, (void *)rates, (void *)segMax, (void *)segMaxDt, (void *)segDt,
+                    (void *)segStepTime, (void *)bounds, (void *)boundsIdx})
+      if (p) (void)hipFree(p);
+    segsD = nullptr;
+    wsegD = nullptr;
+    rates = nullptr;
+    ratesCap = 0;
+    segMax = nullptr;
+    segMaxDt = segDt = segStepTime = nullptr;
+    bounds = boundsIdx = nullptr;
+  }
+  ~lsg_batch() {
+    freeDevice();
+    if (segAdvTime) (void)hipFree(segAdvTime);
+    if (hostSc) (void)hipHostFree(hostSc);
+  }
+};
+
+namespace lsg {
+
+static SegCtx segCtx(const lsg_batch *b) {
+  return SegCtx{b->segsD, (uint32_t)b->doms.size(), b->totalBase, b->wsegD, (uint32_t)b->W};
+}
+
+static int ensureBatchWs(size_t cap, int64_t nw) {
+  if (!g_ws) g_ws = new WsStore();
+  if (g_ws->w.ccap < cap || g_ws->nwAlloc < nw || !g_ws->w.cseg) {
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    if (int e = g_ws->alloc(std::max(g_ws->nwAlloc, nw), std::max(g_ws->w.ccap, cap))) return e;
+  }
+  return 0;
+}
+
+// Concatenate the level sets into one segmented band.
+static int batchAssemble(lsg_batch *b) {
+  const size_t S = b->doms.size();
+  b->freeDevice();
+  std::vector<uint32_t> nEnt(S);
+  b->segsH.resize(S);
+  int64_t W = 0;
+  uint32_t baseAcc = 0;
+  uint64_t entAcc = 0;
+  int width = 1 << 30;
+  for (size_t k = 0; k < S; ++k) {
+    lsg_domain *d = b->doms[k];
+    LSG_HIP(hipMemcpyAsync(&nEnt[k], d->cur.b.n, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+    SegInfo &s = b->segsH[k];
+    s.f = d->f;
+    s.g = d->g;
+    s.wordOff = (uint32_t)W;
+    baseAcc += (uint32_t)d->f.base;
+    s.baseIncl = baseAcc;
+    W += d->f.nwords;
+    if (k > 0 && d->width != width) return fail(-41, "batch: the level sets must have the same band width");
+    width = d->width;
+  }
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  for (size_t k = 0; k < S; ++k) entAcc += nEnt[k];
+  if (W >= (int64_t)UINT32_MAX - 1 || entAcc >= (uint64_t)UINT32_MAX / 2)
+    return fail(-40, "batch too large: more than 2^32 bitmap words or 2^31 entries");
+  b->W = W;
+  b->totalBase = baseAcc;
+  b->width = width;
+  const size_t cap = std::max<size_t>((size_t)1 << 20, (size_t)(entAcc * 3 / 2 + 4096));
+  if (int e = b->cur.alloc(W, cap)) return e;
+  if (int e = b->tmp.alloc(W, cap)) return e;
+  if (int e = b->save.alloc(W, cap)) return e;
+  if (b->temporal != 0) {
+    if (int e = b->icur.alloc(W, cap)) return e;
+    if (int e = b->itmp.alloc(W, cap)) return e;
+  }
+  LSG_HIP(hipMalloc(&b->segsD, S * sizeof(SegInfo)));
+  LSG_HIP(hipMalloc(&b->wsegD, (W + 1) * sizeof(uint16_t)));
+  LSG_HIP(hipMalloc(&b->rates, cap * sizeof(double)));
+  b->ratesCap = cap;
+  LSG_HIP(hipMalloc(&b->segMax, S * sizeof(unsigned long long)));
+  LSG_HIP(hipMalloc(&b->segMaxDt, S * sizeof(double)));
+  LSG_HIP(hipMalloc(&b->segDt, 3 * S * sizeof(double)));
+  LSG_HIP(hipMalloc(&b->segStepTime, S * sizeof(double)));
+  LSG_HIP(hipMalloc(&b->bounds, (S + 1) * sizeof(uint32_t)));
+  LSG_HIP(hipMalloc(&b->boundsIdx, (S + 1) * sizeof(uint32_t)));
+  if (!b->segAdvTime) {
+    LSG_HIP(hipMalloc(&b->segAdvTime, S * sizeof(double)));
+    LSG_HIP(hipMemset(b->segAdvTime, 0, S * sizeof(double)));
+  }
+  LSG_HIP(hipMemcpyAsync(b->segsD, b->segsH.data(), S * sizeof(SegInfo), hipMemcpyHostToDevice, g_ctx.s));
+  // copy every level set's arrays into place
+  uint32_t inFrameBefore = 0;
+  uint64_t e0 = 0;
+  BandBufs &o = b->cur.b;
+  for (size_t k = 0; k < S; ++k) {
+    lsg_domain *d = b->doms[k];
+    const SegInfo &s = b->segsH[k];
+    const int64_t nw = d->f.nwords;
+    LSG_HIP(hipMemcpyAsync(o.bits + s.wordOff, d->cur.b.bits, nw * sizeof(uint64_t), hipMemcpyDeviceToDevice, g_ctx.s));
+    kAddCopy32<<<gridOf(nw), 256, 0, g_ctx.s>>>(d->cur.b.pre, o.pre + s.wordOff, (uint32_t)nw, inFrameBefore);
+    LSG_HIP(hipMemcpyAsync(o.val + e0, d->cur.b.val, nEnt[k] * sizeof(double), hipMemcpyDeviceToDevice, g_ctx.s));
+    LSG_HIP(hipMemcpyAsync(o.pos + e0, d->cur.b.pos, nEnt[k] * sizeof(uint32_t), hipMemcpyDeviceToDevice, g_ctx.s));
+    kFill16<<<gridOf(nEnt[k]), 256, 0, g_ctx.s>>>(o.seg + e0, (uint16_t)k, nEnt[k]);
+    kFill16<<<gridOf(nw), 256, 0, g_ctx.s>>>(b->wsegD + s.wordOff, (uint16_t)k, (size_t)nw);
+    inFrameBefore += nEnt[k] - (uint32_t)d->f.base;
+    e0 += nEnt[k];
+  }
+  kZero64<<<1, 1, 0, g_ctx.s>>>(o.bits + W);
+  kSetScalar32<<<1, 1, 0, g_ctx.s>>>(o.pre + W, inFrameBefore);
+  kSetScalar32<<<1, 1, 0, g_ctx.s>>>(o.n, (uint32_t)e0);
+  kFill16<<<1, 1, 0, g_ctx.s>>>(b->wsegD + W, (uint16_t)(S - 1), 1);
+  std::vector<uint32_t> bi(S + 1);
+  for (size_t k = 0; k < S; ++k) bi[k] = b->segsH[k].wordOff;
+  bi[S] = (uint32_t)W;
+  LSG_HIP(hipMemcpyAsync(b->boundsIdx, bi.data(), (S + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  if (int e = ensureBatchWs(cap, W)) return e;
+  b->assembled = true;
+  return 0;
+}
+
+// Copy the level sets back into their domains.
+static int batchWriteBack(lsg_batch *b) {
+  if (!b->assembled) return 0;
+  const size_t S = b->doms.size();
+  BandBufs &o = b->cur.b;
+  kGather32<<<(int)((S + 1 + 255) / 256), 256, 0, g_ctx.s>>>(o.pre, b->boundsIdx, b->bounds, (uint32_t)(S + 1));
+  std::vector<uint32_t> bd(S + 1);
+  LSG_HIP(hipMemcpyAsync(bd.data(), b->bounds, (S + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  for (size_t k = 0; k < S; ++k) {
+    lsg_domain *d = b->doms[k];
+    const SegInfo &s = b->segsH[k];
+    const uint32_t base = (uint32_t)s.f.base;
+    const uint32_t n = bd[k + 1] - bd[k] + base;
+    const uint64_t e0 = (uint64_t)bd[k] + s.baseIncl - base;
+    d->f = s.f;
+    d->framed = true;
+    if (int e = ensureStores(d, std::max(capFor(n), d->cur.b.cap))) return e;
+    const int64_t nw = s.f.nwords;
+    LSG_HIP(hipMemcpyAsync(d->cur.b.bits, o.bits + s.wordOff, nw * sizeof(uint64_t), hipMemcpyDeviceToDevice, g_ctx.s));
+    kZero64<<<1, 1, 0, g_ctx.s>>>(d->cur.b.bits + nw);
+    kAddCopy32<<<gridOf(nw + 1), 256, 0, g_ctx.s>>>(o.pre + s.wordOff, d->cur.b.pre, (uint32_t)(nw + 1),
+                                                     (uint32_t)(0u - bd[k]));
+    LSG_HIP(hipMemcpyAsync(d->cur.b.val, o.val + e0, n * sizeof(double), hipMemcpyDeviceToDevice, g_ctx.s));
+    LSG_HIP(hipMemcpyAsync(d->cur.b.pos, o.pos + e0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, g_ctx.s));
+    kSetScalar32<<<1, 1, 0, g_ctx.s>>>(d->cur.b.n, n);
+    d->hostN = n;
+    d->width = b->width;
+  }
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  return 0;
+}
+
+static void bSwap(lsg_batch *b) { std::swap(b->cur, b->tmp); }
+static void bSwapInit(lsg_batch *b) { std::swap(b->icur, b->itmp); }
+
+static void bStar(lsg_batch *b, bool init, const OpParams &op) {
+  BStore &in = init ? b->icur : b->cur;
+  BStore &out = init ? b->itmp : b->tmp;
+  launchStarOpSeg(g_ctx.s, b->D, in.b, out.b, segCtx(b), g_ws->w, op, g_ctx.sc);
+  if (init) bSwapInit(b);
+  else bSwap(b);
+}
+static void bSelf(lsg_batch *b, const OpParams &op) {
+  launchSelfOpSeg(g_ctx.s, b->D, b->cur.b, b->tmp.b, segCtx(b), g_ws->w, op, g_ctx.sc);
+  bSwap(b);
+}
+static void bExpand(lsg_batch *b, bool init, int width) {
+  int &w = init ? b->iwidth : b->width;
+  if (width <= w) return;
+  const double totalLimit = width * 0.5;
+  const int startWidth = w;
+  for (int cyc = 0; cyc < width - startWidth; ++cyc) {
+    OpParams op{};
+    op.kind = OP_EXPAND;
+    op.a = (startWidth + cyc + 1) * 0.5;
+    op.b = totalLimit;
+    bStar(b, init, op);
+  }
+  w = width;
+}
+static void bRebuild(lsg_batch *b) {
+  OpParams op{};
+  op.kind = OP_REBUILD;
+  op.a = 1.0;
+  bStar(b, false, op);
+  b->width = 2;
+}
+static void bStageFE(lsg_batch *b, const double *maxDt, double *dtOut) {
+  bExpand(b, false, prepareWidth(b->spatial));
+  launchRatesSeg(g_ctx.s, b->D, b->cur.b, segCtx(b), b->rp, b->rates, g_ctx.sc, b->segMax, maxDt, dtOut, g_ws->w);
+  OpParams op{};
+  op.kind = OP_UPDATE;
+  op.rates = b->rates;
+  op.dt = dtOut;
+  bSelf(b, op);
+  b->width = 1;
+  bRebuild(b);
+}
+static void bCombine(lsg_batch *b, double wT, double wS) {
+  int steps = (b->temporal == 1) ? 2 : (b->temporal == 2 ? 3 : 1);
+  int ew = (int)std::ceil(2.0 * steps * b->cfl + 1);
+  bExpand(b, false, ew);
+  bExpand(b, true, ew);
+  launchBooleanSeg(g_ctx.s, b->D, b->cur.b, b->icur.b, b->tmp.b, segCtx(b), g_ws->w, B_COMBINE, wT, wS, g_ctx.sc);
+  bSwap(b);
+  OpParams p{};
+  p.kind = OP_PRUNE_RSZ;
+  bSelf(b, p);
+  p.kind = OP_PRUNE;
+  bSelf(b, p);
+  b->width = 2;
+  bRebuild(b);
+}
+static void bEnqueueStep(lsg_batch *b) {
+  const uint32_t S = (uint32_t)b->doms.size();
+  launchStepBeginSeg(g_ctx.s, g_ctx.sc, b->segMaxDt, S);
+  launchCopyBandWords(g_ctx.s, b->cur.b, b->save.b, b->W);
+  double *dt0 = b->segDt, *dt1 = b->segDt + S, *dt2 = b->segDt + 2 * S;
+  if (b->temporal == 0) {
+    bStageFE(b, b->segMaxDt, dt0);
+  } else {
+    launchCopyBandWords(g_ctx.s, b->cur.b, b->icur.b, b->W);
+    b->iwidth = b->width;
+    bStageFE(b, b->segMaxDt, dt0);
+    bStageFE(b, dt0, dt1);
+    if (b->temporal == 1) {
+      bCombine(b, 0.5, 0.5);
+    } else {
+      bCombine(b, 0.75, 0.25);
+      bStageFE(b, dt0, dt2);
+      bCombine(b, 1.0 / 3.0, 2.0 / 3.0);
+    }
+  }
+  launchStepTimeSeg(g_ctx.s, g_ctx.sc, b->segDt, S, b->temporal, b->segStepTime, b->segAdvTime);
+  launchFrameCheckSeg(g_ctx.s, b->cur.b, segCtx(b), b->D, REFRAME_SLACK, g_ctx.sc);
+}
+
+static uint64_t batchKey(lsg_batch *b) {
+  uint64_t h = 1469598103934665603ull;
+  auto add = [&](const void *p, size_t n) { h = fnv(h, p, n); };
+  for (BStore *s : {&b->cur, &b->tmp, &b->icur, &b->itmp, &b->save}) add(&s->b, sizeof(BandBufs));
+  add(&b->width, sizeof(int));
+  add(&g_ws->w, sizeof(Workspace));
+  add(&b->rp, sizeof(RateParams));
+  add(&b->temporal, sizeof(int));
+  add(&b->segsD, sizeof(void *));
+  add(&b->W, sizeof(int64_t));
+  add(&b->scanBlocks, sizeof(int));
+  return h;
+}
+
+static int batchStepOnce(lsg_batch *b);
+static int batchStep(lsg_batch *b) {
+  const int e = batchStepOnce(b);
+  setScanBlocks(SCAN_BLOCKS);
+  return e;
+}
+static int batchStepOnce(lsg_batch *b) {
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    setScanBlocks(b->scanBlocks);
+    const int widthBefore = b->width;
+    bool launched = false;
+    if (b->useGraphs) {
+      const uint64_t key = batchKey(b);
+      lsg_batch::Graph *g = nullptr;
+      for (auto &c : b->graphs)
+        if (c.exec && c.key == key) g = &c;
+      if (!g) {
+        g = &b->graphs[b->nextGraph];
+        b->nextGraph ^= 1;
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        *g = lsg_batch::Graph{};
+        hipGraph_t graph = nullptr;
+        bool ok = hipStreamBeginCapture(g_ctx.s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+          bEnqueueStep(b);
+          ok = hipStreamEndCapture(g_ctx.s, &graph) == hipSuccess && graph;
+        }
+        if (ok) ok = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!ok) {
+          (void)hipGetLastError();
+          b->useGraphs = false;
+          b->dropGraphs();
+          return fail(-31, "HIP graph capture failed; retry the call (direct launches are used from now on)");
+        }
+        g->key = key;
+        g->cur = b->cur;
+        g->tmp = b->tmp;
+        g->icur = b->icur;
+        g->itmp = b->itmp;
+        g->width = b->width;
+        g->iwidth = b->iwidth;
+      } else {
+        b->cur = g->cur;
+        b->tmp = g->tmp;
+        b->icur = g->icur;
+        b->itmp = g->itmp;
+        b->width = g->width;
+        b->iwidth = g->iwidth;
+      }
+      LSG_HIP(hipGraphLaunch(g->exec, g_ctx.s));
+      launched = true;
+    }
+    if (!launched) bEnqueueStep(b);
+    LSG_HIP(hipMemcpyAsync(b->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    const DevScalars &hs = *b->hostSc;
+    if (!hs.overflow) {
+      b->pointUpdates += (double)(hs.totalActive - hs.stepActive0);
+      ++b->steps;
+      if (hs.reframe) {
+        // move the frames of the level sets that came close to an edge
+        b->dropGraphs();
+        if (int e = batchWriteBack(b)) return e;
+        for (lsg_domain *d : b->doms)
+          if (int e = maybeReframe(d, false)) return e;
+        return batchAssemble(b);
+      }
+      return 0;
+    }
+    // grow and retry from the saved band
+    b->dropGraphs();
+    const size_t cap = b->cur.b.cap * 2;
+    BStore n;
+    if (int e = n.alloc(b->W, cap)) return e;
+    launchCopyBandWords(g_ctx.s, b->save.b, n.b, b->W);
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    b->cur.release();
+    b->cur = n;
+    n.b = BandBufs{};
+    b->width = widthBefore;
+    for (BStore *s : {&b->tmp, &b->save}) if (int e = s->alloc(b->W, cap)) return e;
+    if (b->temporal != 0)
+      for (BStore *s : {&b->icur, &b->itmp}) if (int e = s->alloc(b->W, cap)) return e;
+    if (b->rates) (void)hipFree(b->rates);
+    LSG_HIP(hipMalloc(&b->rates, cap * sizeof(double)));
+    b->ratesCap = cap;
+    if (int e = ensureBatchWs(cap, b->W)) return e;
+    // a failed step still accumulated per-level-set times: undo from the saved copy is not
+    // needed since stepTime is only added by the completed step's last kernel
+  }
+  return fail(-30, "band capacity overflow persists after retries");
+}
+
+static int batchPrepare(lsg_batch *b) {
+  if (b->doms.empty()) return fail(-1, "empty batch");
+  for (size_t k = 0; k < b->doms.size(); ++k)
+    if (!b->velSet[k]) return fail(-2, "batch: a level set has no velocity");
+  const int sid = schemeIdOf(b->spatial);
+  if (prepareWidth(b->spatial) < 0 || !(sid == S_EO || sid == S_WENO3 || sid == S_WENO5))
+    return fail(-4, "batch: only the Engquist-Osher and WENO schemes are supported");
+  if (!(b->cfl > 0)) return fail(-6, "batch: time step ratio must be positive");
+  b->rp = RateParams{};
+  b->rp.spatial = b->spatial;
+  b->rp.order = schemeOrder(b->spatial);
+  b->rp.cfl = b->cfl;
+  b->rp.alphaFactor = 1.0;
+  b->rp.checkDissipation = 1;
+  b->rp.calcNormals = 1;  // WENO: dissipationAlpha (1) as calcNormal (lsAdvect.hpp:670-676)
+  if (!b->hostSc) LSG_HIP(hipHostMalloc((void **)&b->hostSc, sizeof(DevScalars), hipHostMallocDefault));
+  if (!b->assembled) {
+    for (lsg_domain *d : b->doms)
+      if (!d->framed) return fail(-7, "batch: level set has no data");
+    if (int e = batchAssemble(b)) return e;
+  }
   return 0;
 }
 
@@ -1341,6 +1804,159 @@ int lsg_weno5_learned_eval(int64_t n, const double *x, double dx, int plus, cons
   (void)hipFree(dx_);
   (void)hipFree(dout);
   (void)hipFree(dp);
+  return 0;
+}
+
+// ---- batch -------------------------------------------------------------------
+
+int lsg_batch_create(lsg_batch **out, lsg_domain *const *domains, int n) {
+  if (!out || !domains || n <= 0) return fail(-1, "lsg_batch_create: no level sets");
+  if (n > 65535) return fail(-1, "lsg_batch_create: at most 65535 level sets");
+  if (int e = ctxInit()) return e;
+  auto *b = new lsg_batch();
+  b->D = domains[0]->D;
+  for (int k = 0; k < n; ++k) {
+    if (!domains[k] || domains[k]->D != b->D) {
+      delete b;
+      return fail(-1, "lsg_batch_create: level sets must exist and have the same dimension");
+    }
+    b->doms.push_back(domains[k]);
+  }
+  b->segsH.assign(n, SegInfo{});
+  b->velSet.assign(n, 0);
+  *out = b;
+  return 0;
+}
+
+int lsg_batch_destroy(lsg_batch *b) {
+  delete b;
+  return 0;
+}
+
+int lsg_batch_set_velocity_builtin(lsg_batch *b, int k, int kind, int dir, int kk, double b0, double aPos, double aNeg,
+                                   const double *vec3) {
+  if (k < 0 || k >= (int)b->doms.size()) return fail(-1, "batch: level set index out of range");
+  if (kind != 0 && kind != 1) return fail(-1, "unknown velocity kind");
+  if (dir < 0 || dir > 2) return fail(-1, "velocity direction must be 0, 1 or 2");
+  Vel &v = b->segsH[k].vel;
+  v.kind = kind;
+  v.dir = dir;
+  v.k = kk;
+  v.b = b0;
+  v.aPos = aPos;
+  v.aNeg = aNeg;
+  for (int i = 0; i < 3; ++i) v.vec[i] = vec3 ? vec3[i] : 0.0;
+  b->velSet[k] = 1;
+  if (b->assembled) {
+    b->dropGraphs();
+    LSG_HIP(hipMemcpy(&b->segsD[k], &b->segsH[k], sizeof(SegInfo), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int lsg_batch_set_schemes(lsg_batch *b, int spatial, int temporal, double cfl) {
+  if (spatial < 0 || spatial > 12) return fail(-1, "Advect: Discretization scheme not found.");
+  const bool reassemble = b->assembled && ((temporal != 0) != (b->temporal != 0));
+  b->spatial = spatial;
+  b->temporal = (temporal == 1 || temporal == 2) ? temporal : 0;
+  b->cfl = cfl;
+  b->dropGraphs();
+  if (reassemble) {
+    if (int e = batchWriteBack(b)) return e;
+    b->assembled = false;
+  }
+  return 0;
+}
+
+int lsg_batch_apply(lsg_batch *b) {
+  if (int e = batchPrepare(b)) return e;
+  return batchStep(b);
+}
+
+int lsg_batch_get_advected_times(lsg_batch *b, double *t) {
+  const size_t S = b->doms.size();
+  if (!b->segAdvTime) {
+    for (size_t k = 0; k < S; ++k) t[k] = 0;
+    return 0;
+  }
+  LSG_HIP(hipMemcpy(t, b->segAdvTime, S * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int lsg_batch_get_point_updates(lsg_batch *b, double *n, unsigned *steps) {
+  if (n) *n = b->pointUpdates;
+  if (steps) *steps = b->steps;
+  return 0;
+}
+
+int lsg_batch_write_back(lsg_batch *b) { return batchWriteBack(b); }
+
+int lsg_batch_set_scan_blocks(lsg_batch *b, int blocks) {
+  if (blocks < 8 || blocks > SCAN_BLOCKS_MAX) return fail(-1, "batch: scan grid out of range");
+  b->scanBlocks = blocks;
+  b->dropGraphs();
+  return 0;
+}
+
+int lsg_batch_autotune(lsg_batch *b, int *chosen, double *ms) {
+  // time one step per candidate scan grid, each from the same state (a copy
+  // of every level set, restored before each trial and at the end)
+  if (int e = batchPrepare(b)) return e;
+  if (int e = batchWriteBack(b)) return e;
+  const size_t S = b->doms.size();
+  std::vector<double> adv(S);
+  LSG_HIP(hipMemcpy(adv.data(), b->segAdvTime, S * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<lsg_domain *> snap(S);
+  auto cleanup = [&]() {
+    for (lsg_domain *d : snap) delete d;
+  };
+  for (size_t k = 0; k < S; ++k) {
+    snap[k] = new lsg_domain();
+    if (int e = copyDomain(snap[k], b->doms[k])) {
+      cleanup();
+      return e;
+    }
+  }
+  auto restore = [&]() -> int {
+    for (size_t k = 0; k < S; ++k)
+      if (int e = copyDomain(b->doms[k], snap[k])) return e;
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    b->assembled = false;
+    return batchPrepare(b);
+  };
+  const double pu0 = b->pointUpdates;
+  const unsigned st0 = b->steps;
+  const bool graphs = b->useGraphs;
+  b->useGraphs = false;
+  const int cand[] = {256, 512, 1024, 2048, 4096};
+  int best = b->scanBlocks;
+  double bestMs = 1e300;
+  int err = 0;
+  for (int c : cand) {
+    if ((err = restore())) break;
+    b->scanBlocks = c;
+    LSG_HIP(hipEventRecord(g_ctx.ev0, g_ctx.s));
+    if ((err = batchStep(b))) break;
+    LSG_HIP(hipEventRecord(g_ctx.ev1, g_ctx.s));
+    LSG_HIP(hipEventSynchronize(g_ctx.ev1));
+    float m = 0;
+    LSG_HIP(hipEventElapsedTime(&m, g_ctx.ev0, g_ctx.ev1));
+    if (m < bestMs) {
+      bestMs = m;
+      best = c;
+    }
+  }
+  if (!err) err = restore();
+  cleanup();
+  b->useGraphs = graphs;
+  if (err) return err;
+  LSG_HIP(hipMemcpy(b->segAdvTime, adv.data(), S * sizeof(double), hipMemcpyHostToDevice));
+  b->pointUpdates = pu0;
+  b->steps = st0;
+  b->scanBlocks = best;
+  b->dropGraphs();
+  if (chosen) *chosen = best;
+  if (ms) *ms = bestMs;
   return 0;
 }
 

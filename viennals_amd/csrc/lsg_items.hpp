@@ -207,7 +207,8 @@ template <int D, int R> LSG_HD inline void gatherAxes(const BandView &b, const i
 // Coordinates of candidate / entry i at frame position p (entry 0 is the
 // grid minimum point when it lies outside the frame).
 LSG_HD inline void candCoords(const BandView &b, uint32_t i, uint32_t p, int32_t c[3]) {
-  if (i == 0 && b.f.base) {
+  (void)i;
+  if (p == POS_NONE) {
     c[0] = b.g.lo[0];
     c[1] = b.g.lo[1];
     c[2] = b.g.lo[2];
@@ -373,6 +374,39 @@ LSG_HD inline Vel velOf(const RateParams &rp) {
   v.vec[0] = rp.vec[0];
   v.vec[1] = rp.vec[1];
   v.vec[2] = rp.vec[2];
+  return v;
+}
+
+// One level set of a batch: its frame and grid, where its bitmap words
+// start in the batch bitmap, the grid-minimum entries of the level sets up
+// to and including it (they precede each level set's in-frame entries), and
+// its velocity.
+struct SegInfo {
+  Frame f;
+  GridInfo g;
+  uint32_t wordOff;
+  uint32_t baseIncl;
+  Vel vel;
+};
+
+// Batch context of a kernel (unused when the kernel runs on one level set).
+struct SegArgs {
+  const SegInfo *segs;
+  uint32_t nseg;
+  uint32_t totalBase;     // grid-minimum entries of all level sets
+  const uint16_t *wseg;   // level set of each bitmap word
+  const uint16_t *iseg;   // level set of each input item (candidate or entry)
+  uint16_t *oseg;         // level set of each output entry
+};
+
+// View of level set k of a batch band.
+LSG_HD inline BandView segView(const BandView &all, const SegInfo &S) {
+  BandView v = all;
+  v.bits = all.bits + S.wordOff;
+  v.pre = all.pre + S.wordOff;
+  v.f = S.f;
+  v.g = S.g;
+  v.rb = S.baseIncl;
   return v;
 }
 

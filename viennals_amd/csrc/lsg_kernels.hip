@@ -17,6 +17,14 @@
 namespace lsg {
 
 static constexpr int TPB = 256;
+// Grid of the scan kernels (blocks; at most SCAN_BLOCKS_MAX), chosen at run
+// time (batch autotuning).
+static int g_scanBlocks = SCAN_BLOCKS;
+// Grid of the rate kernel: 4 workgroups per CU, each takes a contiguous
+// 1/1024 of the active list.
+static constexpr int RATE_BLOCKS = 1024;
+void setScanBlocks(int b) { g_scanBlocks = b < 8 ? 8 : (b > SCAN_BLOCKS_MAX ? SCAN_BLOCKS_MAX : b); }
+int scanBlocks() { return g_scanBlocks; }
 static int gridFor(size_t n, int cap = 4096) {
   size_t g = (n + TPB - 1) / TPB;
   if (g < 1) g = 1;
@@ -32,6 +40,7 @@ BandView viewOf(const BandBufs &b, const Frame &f, const GridInfo &g) {
   v.pos = b.pos;
   v.f = f;
   v.g = g;
+  v.rb = (uint32_t)f.base;
   return v;
 }
 
@@ -89,11 +98,12 @@ __device__ inline uint32_t logicalBlock() {
   return xcd * per + (xcd < rem ? xcd : rem) + k;
 }
 
-// Contiguous chunk [lo, hi) of the (logical) block for a device-side size n.
-__device__ inline void blockChunk(uint32_t n, uint32_t &lo, uint32_t &hi) {
-  const uint32_t tile = TPB * SCAN_ITEMS;
+// Contiguous chunk [lo, hi) of the (logical) block for a device-side size n;
+// chunks are multiples of `align` items (the packed keep flags of the
+// compaction are written one 64-bit word per wave).
+__device__ inline void blockChunk(uint32_t n, uint32_t &lo, uint32_t &hi, uint32_t align = 64) {
   uint32_t per = (n + gridDim.x - 1) / gridDim.x;
-  per = (per + tile - 1) / tile * tile;
+  per = (per + align - 1) / align * align;
   lo = logicalBlock() * per;
   hi = lo + per;
   if (lo > n) lo = n;
@@ -126,16 +136,23 @@ __device__ inline double prevInTile(double last, double carry) {
 
 // Phase 1 of candidate generation: candidate bitmap word per word (star
 // dilation or union) and per-block candidate counts.
-template <int D, int MODE>  // MODE 0: star dilation of A; 1: union of A and B
-__global__ __launch_bounds__(256) void kCandCount(const uint64_t *A, const uint64_t *B, Frame f, GridInfo g, uint64_t *cbits,
-                           uint32_t *partials, DevScalars *sc) {
+template <int D, int MODE, bool SEG>  // MODE 0: star dilation of A; 1: union of A and B
+__global__ __launch_bounds__(256) void kCandCount(const uint64_t *A, const uint64_t *B, Frame f, GridInfo g, uint32_t nw,
+                                                  uint64_t *cbits, uint32_t *partials, DevScalars *sc, SegArgs sa) {
   if (sc->overflow) return;
-  const uint32_t nw = (uint32_t)f.nwords;
   uint32_t lo, hi;
   blockChunk(nw, lo, hi);
   uint32_t acc = 0;
   for (uint32_t w = lo + threadIdx.x; w < hi; w += TPB) {
-    uint64_t C = (MODE == 0) ? candStarWord<D>(A, f, g, (int64_t)w) : (A[w] | B[w]);
+    uint64_t C;
+    if (MODE == 1) {
+      C = A[w] | B[w];
+    } else if (!SEG) {
+      C = candStarWord<D>(A, f, g, (int64_t)w);
+    } else {
+      const SegInfo &S = sa.segs[sa.wseg[w]];
+      C = candStarWord<D>(A + S.wordOff, S.f, S.g, (int64_t)(w - S.wordOff));
+    }
     cbits[w] = C;
     acc += (uint32_t)__builtin_popcountll(C);
   }
@@ -147,17 +164,25 @@ __global__ __launch_bounds__(256) void kCandCount(const uint64_t *A, const uint6
 // base + cpre[w] + j); the last block stores the totals.  The positions of a
 // tile are written by consecutive threads (coalesced): thread s finds the
 // word holding the tile's s-th candidate by binary search over the tile's
-// prefix counts in LDS and selects the bit.
-__global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, int64_t nw64, int base, const uint32_t *partials, uint32_t *cpre,
-                          uint32_t *cpos, size_t ccap, uint32_t *ncand, DevScalars *sc) {
+// prefix counts in LDS and selects the bit.  In a batch every level set's
+// candidates are preceded by its grid-minimum candidate (when its frame
+// does not hold the grid minimum), so the candidate index of a word is
+// shifted by the grid-minimum entries up to its level set, and each
+// candidate records its level set in cseg.
+template <bool SEG>
+__global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t nw, int base, const uint32_t *partials,
+                                                 uint32_t *cpre, uint32_t *cpos, size_t ccap, uint32_t *ncand,
+                                                 DevScalars *sc, SegArgs sa, uint16_t *cseg) {
   if (sc->overflow) return;
   __shared__ uint64_t sWord[TPB];
   __shared__ uint32_t sPre[TPB];
-  const uint32_t nw = (uint32_t)nw64;
+  __shared__ uint32_t sShift[SEG ? TPB : 1];
+  __shared__ uint32_t sLocal[SEG ? TPB : 1];  // word index within its level set's bitmap
+  __shared__ uint16_t sSeg[SEG ? TPB : 1];
   uint32_t lo, hi;
   blockChunk(nw, lo, hi);
   uint32_t off = blockOffset(partials);
-  if (logicalBlock() == 0 && threadIdx.x == 0 && base) cpos[0] = 0xFFFFFFFFu;
+  if (!SEG && logicalBlock() == 0 && threadIdx.x == 0 && base) cpos[0] = POS_NONE;
   for (uint32_t t = lo; t < hi; t += TPB) {
     const uint32_t w = t + threadIdx.x;
     const uint64_t c = (w < hi) ? cbits[w] : 0ull;
@@ -167,25 +192,48 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, int64_t 
     if (w < hi) cpre[w] = off + pl;
     sWord[threadIdx.x] = c;
     sPre[threadIdx.x] = pl;
+    if (SEG) {
+      uint32_t sh = 0, lw = 0;
+      uint16_t k = 0;
+      if (w < hi) {
+        k = sa.wseg[w];
+        const SegInfo &S = sa.segs[k];
+        sh = S.baseIncl;
+        lw = w - S.wordOff;
+        if (w == S.wordOff && S.f.base) {  // the level set's grid-minimum candidate precedes its first word
+          const size_t q = (size_t)off + pl + sh - 1;
+          if (q < ccap) {
+            cpos[q] = POS_NONE;
+            cseg[q] = k;
+          }
+        }
+      }
+      sShift[threadIdx.x] = sh;
+      sLocal[threadIdx.x] = lw;
+      sSeg[threadIdx.x] = k;
+    }
     __syncthreads();
-    const size_t q0 = (size_t)base + off;
+    const size_t q0 = SEG ? (size_t)off : (size_t)base + off;
     for (uint32_t s = threadIdx.x; s < tot; s += TPB) {
-      if (q0 + s >= ccap) break;
       int a = 0, b = TPB - 1;  // last j with sPre[j] <= s
       while (a < b) {
         const int m = (a + b + 1) >> 1;
         if (sPre[m] <= s) a = m;
         else b = m - 1;
       }
+      const size_t q = q0 + s + (SEG ? sShift[a] : 0u);
+      if (q >= ccap) continue;
       const int bit = selectBit(sWord[a], (int)(s - sPre[a]));
-      cpos[q0 + s] = ((t + (uint32_t)a) << 6) | (uint32_t)bit;
+      const uint32_t word = SEG ? sLocal[a] : t + (uint32_t)a;  // positions are level-set local
+      cpos[q] = (word << 6) | (uint32_t)bit;
+      if (SEG) cseg[q] = sSeg[a];
     }
     __syncthreads();
     off += tot;
   }
   if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) {
     cpre[nw] = off;
-    uint64_t nc = (uint64_t)off + (uint64_t)base;
+    uint64_t nc = (uint64_t)off + (SEG ? (uint64_t)sa.totalBase : (uint64_t)base);
     if (nc > ccap) {
       sc->overflow = 1;
       nc = ccap;
@@ -196,43 +244,60 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, int64_t 
 
 // ------------------------------------------------- evaluators of rules ----
 
-template <int D> struct EvalStar {
+template <int D, bool SEG> struct EvalStar {
   BandView in;
+  SegArgs sa;
   const uint32_t *cpos;
   OpParams op;
   __device__ double operator()(uint32_t i) const {
+    const BandView v = SEG ? segView(in, sa.segs[sa.iseg[i]]) : in;
     int32_t c[3];
-    candCoords(in, i, cpos[i], c);
+    candCoords(v, i, cpos[i], c);
     double center, nb[2 * D];
-    gatherStar<D>(in, c, center, nb);
+    gatherStar<D>(v, c, center, nb);
     return (op.kind == OP_EXPAND) ? ruleExpand<D>(center, nb, op.a, op.b) : ruleRebuild<D>(center, nb, op.a);
   }
 };
 
-template <int D> struct EvalSelf {
+template <int D, bool SEG> struct EvalSelf {
   BandView in;
+  SegArgs sa;
   OpParams op;
   __device__ double operator()(uint32_t i) const {
     double c = in.val[i];
+    const uint32_t k = SEG ? sa.iseg[i] : 0u;
     if (op.kind == OP_REDUCE) return ruleReduce(c, op.a);
-    if (op.kind == OP_UPDATE) return ruleUpdate(c, *op.dt, op.rates[i]);
+    if (op.kind == OP_UPDATE) return ruleUpdate(c, op.dt[k], op.rates[i]);
+    const BandView v = SEG ? segView(in, sa.segs[k]) : in;
     int32_t cc[3];
-    candCoords(in, i, in.pos[i], cc);
+    candCoords(v, i, v.pos[i], cc);
     double center, nb[2 * D];
-    gatherStar<D>(in, cc, center, nb);
+    gatherStar<D>(v, cc, center, nb);
     return rulePrune<D>(c, nb, op.kind == OP_PRUNE_RSZ);
   }
 };
 
-struct EvalBool {
+template <bool SEG> struct EvalBool {
   BandView A, B;
+  SegArgs sa;
   const uint32_t *cpos;
   int kind;
   double wT, wS;
   __device__ double operator()(uint32_t i) const {
+    BandView va = A, vb = B;
+    if (SEG) {
+      const SegInfo &S = sa.segs[sa.iseg[i]];
+      va = segView(A, S);
+      vb = segView(B, S);
+    }
     int32_t c[3];
-    candCoords(A, i, cpos[i], c);
-    return ruleBoolean(kind, A.value(c[0], c[1], c[2]), B.value(c[0], c[1], c[2]), wT, wS);
+    candCoords(va, i, cpos[i], c);
+    BandView::Loc l[2] = {va.locate(c[0], c[1], c[2]), vb.locate(c[0], c[1], c[2])};
+    const uint64_t wa = va.bits[l[0].w], wb = vb.bits[l[1].w];
+    const uint32_t pa = va.pre[l[0].w], pb = vb.pre[l[1].w];
+    const double a = va.val[va.entryOf(l[0], wa, pa)];
+    const double b = vb.val[vb.entryOf(l[1], wb, pb)];
+    return ruleBoolean(kind, a, b, wT, wS);
   }
 };
 
@@ -258,16 +323,24 @@ __device__ inline uint32_t blockBallotScan(bool k, uint32_t &total, uint64_t &wa
   return woff + lp;
 }
 
-// Keep flag of a candidate from its value and the previous candidate's.
-__device__ inline uint32_t keepOf(uint32_t i, double v, double prev) {
-  if (i == 0) return 1;
+// Keep flag of a candidate from its value and the previous candidate's: the
+// first candidate of a level set, a defined value, or the start of a run
+// whose value differs from the previous candidate's.
+__device__ inline uint32_t keepOf(bool first, double v, double prev) {
+  if (first) return 1;
   if (!isUndef(v)) return 1;
   return v != prev ? 1u : 0u;
+}
+template <bool SEG> __device__ inline bool firstOfSet(uint32_t i, const uint16_t *iseg) {
+  if (i == 0) return true;
+  return SEG ? (iseg[i] != iseg[i - 1]) : false;
 }
 
 // Phase 1 of compaction: evaluate the rule at every candidate (out[i]) and
 // count the kept candidates per block.
-template <class Ev> __global__ __launch_bounds__(256) void kEvalKeep(Ev ev, const uint32_t *nptr, double *out, uint32_t *partials, DevScalars *sc) {
+template <class Ev, bool SEG>
+__global__ __launch_bounds__(256) void kEvalKeep(Ev ev, const uint32_t *nptr, double *out, uint32_t *partials,
+                                                 DevScalars *sc, const uint16_t *iseg) {
   if (sc->overflow) return;
   const uint32_t n = *nptr;
   uint32_t lo, hi;
@@ -285,7 +358,7 @@ template <class Ev> __global__ __launch_bounds__(256) void kEvalKeep(Ev ev, cons
       out[i] = v;
     }
     const double prev = prevInTile(v, carry);
-    if (valid) acc += keepOf(i, v, prev);
+    if (valid) acc += keepOf(firstOfSet<SEG>(i, iseg), v, prev);
     __syncthreads();
     if (threadIdx.x == TPB - 1) carry = v;
     __syncthreads();
@@ -297,9 +370,11 @@ template <class Ev> __global__ __launch_bounds__(256) void kEvalKeep(Ev ev, cons
 // Phase 2: rescan the keep flags from the block offset, scatter the kept
 // candidates into the output band, store the running index ks[] and the
 // keep flags packed one bit per candidate; the last block stores totals.
-__global__ __launch_bounds__(256) void kKeepScatter(const double *out, const uint32_t *cpos, const uint32_t *nptr, const uint32_t *partials,
-                             uint32_t *ks, uint64_t *keepBits, double *val, uint32_t *pos, uint32_t *nOut, size_t cap,
-                             DevScalars *sc) {
+template <bool SEG>
+__global__ __launch_bounds__(256) void kKeepScatter(const double *out, const uint32_t *cpos, const uint32_t *nptr,
+                                                    const uint32_t *partials, uint32_t *ks, uint64_t *keepBits,
+                                                    double *val, uint32_t *pos, uint32_t *nOut, size_t cap,
+                                                    DevScalars *sc, const uint16_t *iseg, uint16_t *oseg) {
   if (sc->overflow) return;
   const uint32_t n = *nptr;
   uint32_t lo, hi;
@@ -315,7 +390,7 @@ __global__ __launch_bounds__(256) void kKeepScatter(const double *out, const uin
     if (valid) {
       v = out[i];
       const double prev = (i > 0) ? out[i - 1] : 0.0;
-      k = keepOf(i, v, prev);
+      k = keepOf(firstOfSet<SEG>(i, iseg), v, prev);
     }
     uint32_t tot;
     uint64_t wm;
@@ -325,6 +400,7 @@ __global__ __launch_bounds__(256) void kKeepScatter(const double *out, const uin
       if (k && p < cap) {
         val[p] = v;
         pos[p] = cpos[i];
+        if (SEG) oseg[p] = iseg[i];
       }
     }
     const uint32_t g = t + (uint32_t)wid * 64u;  // 64-aligned: chunks and tiles are multiples of 256
@@ -344,28 +420,31 @@ __global__ __launch_bounds__(256) void kKeepScatter(const double *out, const uin
 
 // Bitmap words and prefix counts of the compacted output: the kept flags of
 // the candidates of word w (a contiguous run of the packed keep bits) are
-// deposited onto the set bits of the candidate word.
+// deposited onto the set bits of the candidate word.  The candidate index of
+// word w is cpre[w] plus the grid-minimum candidates before it (base, or in
+// a batch those of the level sets up to w's).
+template <bool SEG>
 __global__ void kBuildIndex(const uint64_t *cwords, const uint32_t *cpre, const uint32_t *ks, const uint64_t *keepBits,
-                            int64_t nw, int base, const uint32_t *ncand, uint64_t *bitsOut, uint32_t *preOut,
-                            DevScalars *sc) {
+                            uint32_t nw, int base, const uint32_t *ncand, uint64_t *bitsOut, uint32_t *preOut,
+                            DevScalars *sc, SegArgs sa) {
   LSG_ABORT_IF_OVERFLOW(sc);
   const uint32_t nc = *ncand;
   uint32_t lo, hi;
-  blockChunk((uint32_t)nw + 1, lo, hi);
-  for (int64_t w = lo + threadIdx.x; w < (int64_t)hi; w += TPB) {
-    uint32_t c0 = (uint32_t)base + cpre[w];
+  blockChunk(nw + 1, lo, hi);
+  for (uint32_t w = lo + threadIdx.x; w < hi; w += TPB) {
+    uint32_t sh = (uint32_t)base;
+    if (SEG) sh = (w < nw) ? sa.segs[sa.wseg[w]].baseIncl : sa.totalBase;
+    uint32_t c0 = sh + cpre[w];
     if (c0 > nc) c0 = nc;
-    preOut[w] = ks[c0] - (uint32_t)base;
+    preOut[w] = ks[c0] - sh;
     if (w == nw) {
       bitsOut[w] = 0;
       continue;
     }
-    uint64_t src = cwords[w];
+    const uint64_t src = cwords[w];
     uint64_t word = 0;
     if (src) {
-      uint32_t c1 = (uint32_t)base + cpre[w + 1];
-      if (c1 > nc) c1 = nc;
-      const uint32_t len = c1 - c0;
+      const uint32_t len = (uint32_t)__builtin_popcountll(src);
       const uint32_t q = c0 >> 6, s = c0 & 63;
       uint64_t m = keepBits[q] >> s;
       if (s != 0 && s + len > 64) m |= keepBits[q + 1] << (64 - s);
@@ -378,56 +457,109 @@ __global__ void kBuildIndex(const uint64_t *cwords, const uint32_t *cpre, const 
 
 // ------------------------------------------------------- host wrappers ----
 
-template <class Ev>
+static SegArgs noSeg() { return SegArgs{nullptr, 0, 0, nullptr, nullptr, nullptr}; }
+static SegArgs segArgs(const SegCtx &c, const uint16_t *iseg, uint16_t *oseg) {
+  return SegArgs{c.segs, c.nseg, c.totalBase, c.wseg, iseg, oseg};
+}
+
+template <class Ev, bool SEG>
 static void compactInto(hipStream_t s, const Ev &ev, const Workspace &ws, const uint32_t *ncandPtr,
-                        const uint32_t *cpos, const uint32_t *cpre, const uint64_t *cwords, const Frame &f,
-                        BandBufs &out, DevScalars *sc) {
-  kEvalKeep<Ev><<<SCAN_BLOCKS, TPB, 0, s>>>(ev, ncandPtr, ws.out, ws.partials, sc);
-  kKeepScatter<<<SCAN_BLOCKS, TPB, 0, s>>>(ws.out, cpos, ncandPtr, ws.partials, ws.ks, ws.keepBits, out.val, out.pos,
-                                           out.n, out.cap, sc);
-  kBuildIndex<<<gridFor((size_t)f.nwords + 1), TPB, 0, s>>>(cwords, cpre, ws.ks, ws.keepBits, f.nwords, f.base,
-                                                            ncandPtr, out.bits, out.pre, sc);
+                        const uint32_t *cpos, const uint32_t *cpre, const uint64_t *cwords, uint32_t nw, int base,
+                        BandBufs &out, DevScalars *sc, const SegArgs &sa) {
+  kEvalKeep<Ev, SEG><<<g_scanBlocks, TPB, 0, s>>>(ev, ncandPtr, ws.out, ws.partials, sc, sa.iseg);
+  kKeepScatter<SEG><<<g_scanBlocks, TPB, 0, s>>>(ws.out, cpos, ncandPtr, ws.partials, ws.ks, ws.keepBits, out.val,
+                                                 out.pos, out.n, out.cap, sc, sa.iseg, out.seg);
+  kBuildIndex<SEG><<<gridFor((size_t)nw + 1), TPB, 0, s>>>(cwords, cpre, ws.ks, ws.keepBits, nw, base, ncandPtr,
+                                                           out.bits, out.pre, sc, sa);
 }
 
-template <int D, int MODE>
+template <int D, int MODE, bool SEG>
 static void candidates(hipStream_t s, const uint64_t *A, const uint64_t *B, const Frame &f, const GridInfo &g,
-                       Workspace &ws, DevScalars *sc) {
-  kCandCount<D, MODE><<<SCAN_BLOCKS, TPB, 0, s>>>(A, B, f, g, ws.cbits, ws.partials, sc);
-  kCandList<<<SCAN_BLOCKS, TPB, 0, s>>>(ws.cbits, f.nwords, f.base, ws.partials, ws.cpre, ws.cpos, ws.ccap,
-                                        ws.ncand, sc);
+                       uint32_t nw, Workspace &ws, DevScalars *sc, const SegArgs &sa) {
+  kCandCount<D, MODE, SEG><<<g_scanBlocks, TPB, 0, s>>>(A, B, f, g, nw, ws.cbits, ws.partials, sc, sa);
+  kCandList<SEG><<<g_scanBlocks, TPB, 0, s>>>(ws.cbits, nw, f.base, ws.partials, ws.cpre, ws.cpos, ws.ccap, ws.ncand,
+                                              sc, sa, ws.cseg);
 }
 
-template <int D>
-static void starOp(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g, Workspace &ws,
-                   const OpParams &op, DevScalars *sc) {
-  candidates<D, 0>(s, in.bits, nullptr, f, g, ws, sc);
-  EvalStar<D> ev{viewOf(in, f, g), ws.cpos, op};
-  compactInto(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, f, out, sc);
+template <int D, bool SEG>
+static void starOp(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g, uint32_t nw,
+                   Workspace &ws, const OpParams &op, DevScalars *sc, const SegArgs &sa0) {
+  candidates<D, 0, SEG>(s, in.bits, nullptr, f, g, nw, ws, sc, sa0);
+  SegArgs sa = sa0;
+  sa.iseg = ws.cseg;
+  sa.oseg = out.seg;
+  EvalStar<D, SEG> ev{viewOf(in, f, g), sa, ws.cpos, op};
+  compactInto<EvalStar<D, SEG>, SEG>(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, nw, f.base, out, sc, sa);
+}
+
+template <int D, bool SEG>
+static void selfOp(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g, uint32_t nw,
+                   Workspace &ws, const OpParams &op, DevScalars *sc, const SegArgs &sa0) {
+  SegArgs sa = sa0;
+  sa.iseg = in.seg;
+  sa.oseg = out.seg;
+  EvalSelf<D, SEG> ev{viewOf(in, f, g), sa, op};
+  compactInto<EvalSelf<D, SEG>, SEG>(s, ev, ws, in.n, in.pos, in.pre, in.bits, nw, f.base, out, sc, sa);
+}
+
+template <int D, bool SEG>
+static void boolOp(hipStream_t s, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
+                   const GridInfo &g, uint32_t nw, Workspace &ws, int kind, double wT, double wS, DevScalars *sc,
+                   const SegArgs &sa0) {
+  candidates<D, 1, SEG>(s, A.bits, B.bits, f, g, nw, ws, sc, sa0);
+  SegArgs sa = sa0;
+  sa.iseg = ws.cseg;
+  sa.oseg = out.seg;
+  EvalBool<SEG> ev{viewOf(A, f, g), viewOf(B, f, g), sa, ws.cpos, kind, wT, wS};
+  compactInto<EvalBool<SEG>, SEG>(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, nw, f.base, out, sc, sa);
 }
 
 void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc) {
-  if (D == 2) starOp<2>(s, in, out, f, g, ws, op, sc);
-  else starOp<3>(s, in, out, f, g, ws, op, sc);
+  const uint32_t nw = (uint32_t)f.nwords;
+  if (D == 2) starOp<2, false>(s, in, out, f, g, nw, ws, op, sc, noSeg());
+  else starOp<3, false>(s, in, out, f, g, nw, ws, op, sc, noSeg());
 }
 
 void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc) {
-  if (D == 2) {
-    EvalSelf<2> ev{viewOf(in, f, g), op};
-    compactInto(s, ev, ws, in.n, in.pos, in.pre, in.bits, f, out, sc);
-  } else {
-    EvalSelf<3> ev{viewOf(in, f, g), op};
-    compactInto(s, ev, ws, in.n, in.pos, in.pre, in.bits, f, out, sc);
-  }
+  const uint32_t nw = (uint32_t)f.nwords;
+  if (D == 2) selfOp<2, false>(s, in, out, f, g, nw, ws, op, sc, noSeg());
+  else selfOp<3, false>(s, in, out, f, g, nw, ws, op, sc, noSeg());
 }
 
 void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
                    const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc) {
-  if (D == 2) candidates<2, 1>(s, A.bits, B.bits, f, g, ws, sc);
-  else candidates<3, 1>(s, A.bits, B.bits, f, g, ws, sc);
-  EvalBool ev{viewOf(A, f, g), viewOf(B, f, g), ws.cpos, kind, wTarget, wSource};
-  compactInto(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, f, out, sc);
+  const uint32_t nw = (uint32_t)f.nwords;
+  if (D == 2) boolOp<2, false>(s, A, B, out, f, g, nw, ws, kind, wTarget, wSource, sc, noSeg());
+  else boolOp<3, false>(s, A, B, out, f, g, nw, ws, kind, wTarget, wSource, sc, noSeg());
+}
+
+// Batch variants: one launch per kernel for every level set of the batch.
+// The frame and grid arguments of the kernels are unused (each item takes
+// its level set's from the table).
+void launchStarOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const SegCtx &c, Workspace &ws,
+                     const OpParams &op, DevScalars *sc) {
+  Frame f{};
+  GridInfo g{};
+  if (D == 2) starOp<2, true>(s, in, out, f, g, c.nw, ws, op, sc, segArgs(c, nullptr, nullptr));
+  else starOp<3, true>(s, in, out, f, g, c.nw, ws, op, sc, segArgs(c, nullptr, nullptr));
+}
+
+void launchSelfOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const SegCtx &c, Workspace &ws,
+                     const OpParams &op, DevScalars *sc) {
+  Frame f{};
+  GridInfo g{};
+  if (D == 2) selfOp<2, true>(s, in, out, f, g, c.nw, ws, op, sc, segArgs(c, nullptr, nullptr));
+  else selfOp<3, true>(s, in, out, f, g, c.nw, ws, op, sc, segArgs(c, nullptr, nullptr));
+}
+
+void launchBooleanSeg(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const SegCtx &c,
+                      Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc) {
+  Frame f{};
+  GridInfo g{};
+  if (D == 2) boolOp<2, true>(s, A, B, out, f, g, c.nw, ws, kind, wTarget, wSource, sc, segArgs(c, nullptr, nullptr));
+  else boolOp<3, true>(s, A, B, out, f, g, c.nw, ws, kind, wTarget, wSource, sc, segArgs(c, nullptr, nullptr));
 }
 
 // ------------------------------------------------------------ rates ----
@@ -514,10 +646,46 @@ template <class Fn> __device__ inline void forEachActive(const double *val, uint
   }
 }
 
+// Active entries (|phi| <= 0.5, defined; lsAdvect.hpp:498-499) in entry
+// order: per-block counts, then the list (two-kernel scan).  The rate kernel
+// then gives every lane one active point.
+__device__ inline bool isActive(double v) { return !isUndef(v) && fabs(v) <= 0.5; }
+
+__global__ __launch_bounds__(256) void kActiveCount(const double *val, const uint32_t *nptr, uint32_t *partials,
+                                                    DevScalars *sc) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const uint32_t n = *nptr;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
+  uint32_t acc = 0;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += TPB) acc += isActive(val[i]) ? 1u : 0u;
+  const uint32_t t = blockSum(acc);
+  if (threadIdx.x == 0) partials[logicalBlock()] = t;
+}
+
+__global__ __launch_bounds__(256) void kActiveList(const double *val, const uint32_t *nptr, const uint32_t *partials,
+                                                   uint32_t *list, uint32_t *nact, DevScalars *sc) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const uint32_t n = *nptr;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
+  uint32_t off = blockOffset(partials);
+  for (uint32_t t = lo; t < hi; t += TPB) {
+    const uint32_t i = t + threadIdx.x;
+    const bool a = (i < hi) && isActive(val[i]);
+    uint32_t tot;
+    uint64_t wm;
+    const uint32_t p = off + blockBallotScan(a, tot, wm);
+    if (a) list[p] = i;
+    off += tot;
+  }
+  if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) *nact = off;
+}
+
 // Hamiltonian + dissipation per active entry of the expanded band
 // (Advect::integrateTime, lsAdvect.hpp:495-593, one level set): writes the
 // stage velocity and reduces max |H - D| (and the LLF/LLLF alphas).
-template <int D, int SCHEME> struct RatePointFn {
+template <int D, int SCHEME, bool SEG> struct RatePointFn {
   BandView b;
   RateParams rp;
   Vel vel;
@@ -527,12 +695,30 @@ template <int D, int SCHEME> struct RatePointFn {
   double la[3];
   double lmax;
   unsigned long long act;
+  // batch: level set of each entry, per-level-set maxima; the running
+  // maximum of a lane belongs to curSeg and is flushed when that changes
+  SegArgs sa;
+  unsigned long long *segMax;
+  uint32_t curSeg;
   __device__ __attribute__((always_inline)) void operator()(uint32_t i) {
     ++act;
+    BandView v = b;
+    Vel ve = vel;
+    if (SEG) {
+      const uint32_t k = sa.iseg[i];
+      const SegInfo &S = sa.segs[k];
+      v = segView(b, S);
+      ve = S.vel;
+      if (k != curSeg) {
+        if (lmax > 0.) atomicMax(&segMax[curSeg], (unsigned long long)__double_as_longlong(lmax));
+        lmax = 0.;
+        curSeg = k;
+      }
+    }
     int32_t c[3];
-    candCoords(b, i, b.pos[i], c);
+    candCoords(v, i, v.pos[i], c);
     double velocity;
-    rates[i] = ratePoint<D, SCHEME>(b, c, b.val[i], rp, vel, ga, la, velocity, mlp);
+    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp);
     if (velocity != 0.) {
       const double a = fabs(velocity);
       if (lmax < a) lmax = a;
@@ -540,10 +726,13 @@ template <int D, int SCHEME> struct RatePointFn {
   }
 };
 
-template <int D, int SCHEME>
-__global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *nptr, RateParams rp, double *rates, DevScalars *sc) {
+template <int D, int SCHEME, bool SEG>
+__global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, const uint32_t *nact, RateParams rp,
+                                              double *rates, DevScalars *sc, SegArgs sa, unsigned long long *segMax) {
   LSG_ABORT_IF_OVERFLOW(sc);
-  const uint32_t n = *nptr;
+  const uint32_t n = *nact;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
   const Vel vel = velOf(rp);
   // learned weno5: the bf16 parameters are widened to fp32 once per block
   __shared__ float mlpW[SCHEME == S_WENO5L ? MLP_NPARAM : 1];
@@ -551,13 +740,31 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *nptr, 
     for (int i = threadIdx.x; i < MLP_NPARAM; i += TPB) mlpW[i] = bf16Float(rp.mlp[i]);
     __syncthreads();
   }
-  RatePointFn<D, SCHEME> fn{b, rp, vel, rates, mlpW, {0, 0, 0}, {0, 0, 0}, 0., 0ull};
+  const uint32_t segFirst = (SEG && lo < hi) ? sa.iseg[list[lo]] : 0u;
+  RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst};
   if (SCHEME == S_LF)
     for (int i = 0; i < 3; ++i) fn.ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
-  forEachActive(b.val, n, fn);
-  blockMaxAtomic((unsigned long long)__double_as_longlong(fn.lmax), &sc->maxAbsBits);
-  if (SCHEME == S_LLLF || SCHEME == S_LLF)
-    for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(fn.la[d]), &sc->alphaBits[d]);
+  for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) fn(list[j]);
+  if (!SEG) {
+    blockMaxAtomic((unsigned long long)__double_as_longlong(fn.lmax), &sc->maxAbsBits);
+    if (SCHEME == S_LLLF || SCHEME == S_LLF)
+      for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(fn.la[d]), &sc->alphaBits[d]);
+  } else {
+    // the level sets of a block's chunk are few and consecutive: reduce the
+    // lanes' maxima in LDS slots, then one atomic per level set
+    constexpr int NSLOT = 8;
+    __shared__ unsigned long long slot[NSLOT];
+    if (threadIdx.x < NSLOT) slot[threadIdx.x] = 0ull;
+    __syncthreads();
+    if (fn.lmax > 0.) {
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(fn.lmax);
+      const uint32_t d = fn.curSeg - segFirst;
+      if (d < NSLOT) atomicMax(&slot[d], bits);
+      else atomicMax(&segMax[fn.curSeg], bits);
+    }
+    __syncthreads();
+    if (threadIdx.x < NSLOT && slot[threadIdx.x]) atomicMax(&segMax[segFirst + threadIdx.x], slot[threadIdx.x]);
+  }
   unsigned long long t = blockSumU64(fn.act);
   if (threadIdx.x == 0 && t) atomicAdd(&sc->activeCount, t);
 }
@@ -581,13 +788,29 @@ __global__ void kResetRates(DevScalars *sc, int keepAlphas) {
   sc->activeCount = 0;
 }
 
+// Active list of band b into ws.cpos / ws.ncand (both free between band operations).
+static void activeList(hipStream_t s, const BandBufs &b, Workspace &ws, DevScalars *sc) {
+  kActiveCount<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, sc);
+  kActiveList<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, ws.cpos, ws.ncand, sc);
+}
+
+static void recordEvent(hipStream_t s, hipEvent_t e, bool external) {
+  if (!e) return;
+  if (external) (void)hipEventRecordWithFlags(e, s, hipEventRecordExternal);
+  else (void)hipEventRecord(e, s);
+}
+
 void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
-                 double *rates, DevScalars *sc, const double *maxTimeStep, double *dtOut) {
+                 double *rates, DevScalars *sc, const double *maxTimeStep, double *dtOut, Workspace &ws,
+                 hipEvent_t e0, hipEvent_t e1, bool externalEvents) {
   const int scheme = schemeIdOf(rp.spatial);
   kResetRates<<<1, 1, 0, s>>>(sc, scheme == S_LF ? 1 : 0);
+  activeList(s, b, ws, sc);
   BandView v = viewOf(b, f, g);
-  const int ge = gridFor(b.cap, 8192);
-#define LSG_RATES(DD, SS) kRates<DD, SS><<<ge, TPB, 0, s>>>(v, b.n, rp, rates, sc)
+  const int ge = RATE_BLOCKS;
+  recordEvent(s, e0, externalEvents);
+#define LSG_RATES(DD, SS) \
+  kRates<DD, SS, false><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, rp, rates, sc, noSeg(), nullptr)
   if (D == 2) {
     switch (scheme) {
     case S_EO: LSG_RATES(2, S_EO); break;
@@ -610,8 +833,114 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
     }
   }
 #undef LSG_RATES
+  recordEvent(s, e1, externalEvents);
   const int useAlphas = (scheme == S_LF || scheme == S_LLLF || scheme == S_LLF) ? 1 : 0;
   kTimeStep<<<1, 1, 0, s>>>(sc, rp, D, g.dx, maxTimeStep, useAlphas, dtOut, b.n);
+}
+
+// Batch rate pass: every level set's stage velocities and max |H - D| per
+// level set; then the per-level-set time steps (EO and WENO schemes).
+__global__ void kTimeStepSeg(DevScalars *sc, const SegInfo *segs, uint32_t nseg, double cfl, int D,
+                             const unsigned long long *segMax, const double *maxDt, double *dtOut, const uint32_t *nb) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nseg) {
+    const double m = __longlong_as_double((long long)segMax[k]);
+    dtOut[k] = stageTimeStep(maxDt[k], cfl, m, D, nullptr, false, segs[k].g.dx);
+  }
+  if (k == 0) {
+    sc->totalActive += sc->activeCount;
+    sc->rateBytes += 8.0 * (double)(*nb) + 8.0 * (double)sc->activeCount;
+  }
+}
+__global__ void kResetRatesSeg(DevScalars *sc, unsigned long long *segMax, uint32_t nseg) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nseg) segMax[k] = 0ull;
+  if (k == 0) sc->activeCount = 0;
+}
+
+void launchRatesSeg(hipStream_t s, int D, const BandBufs &b, const SegCtx &c, const RateParams &rp, double *rates,
+                    DevScalars *sc, unsigned long long *segMax, const double *maxDt, double *dtOut, Workspace &ws) {
+  const int scheme = schemeIdOf(rp.spatial);
+  const int gs = (int)((c.nseg + TPB - 1) / TPB);
+  kResetRatesSeg<<<gs, TPB, 0, s>>>(sc, segMax, c.nseg);
+  activeList(s, b, ws, sc);
+  Frame f{};
+  GridInfo g{};
+  BandView v = viewOf(b, f, g);
+  SegArgs sa = segArgs(c, b.seg, nullptr);
+  const int ge = RATE_BLOCKS;
+#define LSG_RATES_SEG(DD, SS) kRates<DD, SS, true><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, rp, rates, sc, sa, segMax)
+  if (D == 2) {
+    switch (scheme) {
+    case S_EO: LSG_RATES_SEG(2, S_EO); break;
+    case S_WENO3: LSG_RATES_SEG(2, S_WENO3); break;
+    case S_WENO5L: LSG_RATES_SEG(2, S_WENO5L); break;
+    default: LSG_RATES_SEG(2, S_WENO5); break;
+    }
+  } else {
+    switch (scheme) {
+    case S_EO: LSG_RATES_SEG(3, S_EO); break;
+    case S_WENO3: LSG_RATES_SEG(3, S_WENO3); break;
+    case S_WENO5L: LSG_RATES_SEG(3, S_WENO5L); break;
+    default: LSG_RATES_SEG(3, S_WENO5); break;
+    }
+  }
+#undef LSG_RATES_SEG
+  kTimeStepSeg<<<gs, TPB, 0, s>>>(sc, c.segs, c.nseg, rp.cfl, D, segMax, maxDt, dtOut, b.n);
+}
+
+// Per-level-set step bookkeeping of a batch: stage-1 bound, returned time of
+// the step (lsAdvectIntegrationSchemes.hpp:58,107,167) and advected time.
+__global__ void kStepBeginSeg(DevScalars *sc, double *maxDt, uint32_t nseg) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nseg) maxDt[k] = 1.7976931348623157e308;
+  if (k == 0) {
+    sc->overflow = 0;
+    sc->stepActive0 = sc->totalActive;
+    sc->reframe = 0;
+  }
+}
+__global__ void kStepTimeSeg(const DevScalars *sc, const double *dt, uint32_t nseg, int temporal, double *stepTime,
+                             double *advTime) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nseg) return;
+  const double d1 = dt[k], d2 = dt[nseg + k], d3 = dt[2 * nseg + k];
+  double t;
+  if (temporal == 1) t = 0.5 * d1 + 0.5 * d2;
+  else if (temporal == 2) t = (d1 + d2 + 4.0 * d3) / 6.0;
+  else t = d1;
+  stepTime[k] = t;
+  advTime[k] += t;
+}
+void launchStepBeginSeg(hipStream_t s, DevScalars *sc, double *maxDt, uint32_t nseg) {
+  kStepBeginSeg<<<(nseg + TPB - 1) / TPB, TPB, 0, s>>>(sc, maxDt, nseg);
+}
+void launchStepTimeSeg(hipStream_t s, const DevScalars *sc, const double *dt, uint32_t nseg, int temporal,
+                       double *stepTime, double *advTime) {
+  kStepTimeSeg<<<(nseg + TPB - 1) / TPB, TPB, 0, s>>>(sc, dt, nseg, temporal, stepTime, advTime);
+}
+
+// Flags (sc->reframe = 1) an entry of a batch band closer than `slack` to its
+// frame edge in an infinite dimension: the host then moves the frames.
+__global__ void kFrameCheckSeg(const uint32_t *pos, const uint16_t *seg, const uint32_t *nptr, const SegInfo *segs,
+                               int D, int slack, DevScalars *sc) {
+  const uint32_t n = *nptr;
+  bool near = false;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t p = pos[i];
+    if (p == POS_NONE) continue;
+    const SegInfo &S = segs[seg[i]];
+    int32_t r[3];
+    S.f.decode(p, r[0], r[1], r[2]);
+    for (int d = 0; d < D; ++d)
+      if (!S.g.bounded(d) && (r[d] < slack || S.f.ext[d] - 1 - r[d] < slack)) near = true;
+  }
+  if (near) sc->reframe = 1;
+}
+void launchFrameCheckSeg(hipStream_t s, const BandBufs &b, const SegCtx &c, int D, int slack, DevScalars *sc) {
+  kFrameCheckSeg<<<gridFor(b.cap, 2048), TPB, 0, s>>>(b.pos, b.seg, b.n, c.segs, D, slack, sc);
 }
 
 // Advect::findGlobalAlphas (lsAdvect.hpp:90-195), one level set.
@@ -665,7 +994,7 @@ void launchStepBegin(hipStream_t s, DevScalars *sc, const double *maxDt) { kStep
 
 __global__ void kCopyBand(const double *sv, const uint32_t *sp, const uint32_t *sn, double *dv, uint32_t *dp,
                           uint32_t *dn, size_t cap, const uint64_t *sb, const uint32_t *spre, uint64_t *db,
-                          uint32_t *dpre, int64_t nw) {
+                          uint32_t *dpre, int64_t nw, const uint16_t *ss, uint16_t *ds) {
   uint32_t n = *sn;
   if (n > cap) n = (uint32_t)cap;
   size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -673,6 +1002,7 @@ __global__ void kCopyBand(const double *sv, const uint32_t *sp, const uint32_t *
   for (size_t i = tid; i < n; i += stride) {
     dv[i] = sv[i];
     dp[i] = sp[i];
+    if (ds) ds[i] = ss[i];
   }
   for (size_t w = tid; w <= (size_t)nw; w += stride) {
     db[w] = sb[w];
@@ -680,10 +1010,12 @@ __global__ void kCopyBand(const double *sv, const uint32_t *sp, const uint32_t *
   }
   if (tid == 0) *dn = n;
 }
-void launchCopyBand(hipStream_t s, const BandBufs &src, BandBufs &dst, const Frame &f) {
-  size_t work = src.cap > (size_t)f.nwords ? src.cap : (size_t)f.nwords;
+void launchCopyBand(hipStream_t s, const BandBufs &src, BandBufs &dst, const Frame &f) { launchCopyBandWords(s, src, dst, f.nwords); }
+void launchCopyBandWords(hipStream_t s, const BandBufs &src, BandBufs &dst, int64_t nw) {
+  size_t work = src.cap > (size_t)nw ? src.cap : (size_t)nw;
   kCopyBand<<<gridFor(work, 8192), TPB, 0, s>>>(src.val, src.pos, src.n, dst.val, dst.pos, dst.n, dst.cap, src.bits,
-                                                src.pre, dst.bits, dst.pre, f.nwords);
+                                                src.pre, dst.bits, dst.pre, nw, (src.seg && dst.seg) ? src.seg : nullptr,
+                                                (src.seg && dst.seg) ? dst.seg : nullptr);
 }
 
 __device__ inline int32_t blockMinI(int32_t v) {
@@ -708,7 +1040,7 @@ __global__ void kBandStats(const double *val, const uint32_t *pos, const uint32_
   unsigned long long c = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (!isUndef(val[i])) ++c;
-    if (i == 0 && f.base) continue;
+    if (pos[i] == POS_NONE) continue;
     int32_t r[3];
     f.decode(pos[i], r[0], r[1], r[2]);
     for (int d = 0; d < 3; ++d) {

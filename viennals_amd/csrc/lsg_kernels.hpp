@@ -5,11 +5,12 @@
 
 #include <cstdint>
 
-#include "lsg_device.hpp"
+#include "lsg_items.hpp"
 
 namespace lsg {
 
 constexpr int SCAN_BLOCKS = 1024;
+constexpr int SCAN_BLOCKS_MAX = 4096;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 1;
 
@@ -20,7 +21,7 @@ struct DevScalars {
   unsigned long long activeCount;   // active points of the last rate pass
   unsigned long long totalActive;   // accumulated active points (point updates)
   uint32_t overflow;
-  uint32_t pad;
+  uint32_t reframe;  // batch: an entry came close to its frame edge
   double dt[4];       // stage time steps (dt1, dt2, dt3) and a scratch slot
   double stepTime;    // time advanced by the last step
   double maxTimeStep; // bound passed to the first stage
@@ -36,6 +37,7 @@ struct BandBufs {
   uint32_t *pos;
   uint32_t *n;  // device entry count
   size_t cap;
+  uint16_t *seg;  // batch bands only: level set of each entry
 };
 
 struct Workspace {
@@ -49,9 +51,35 @@ struct Workspace {
   uint32_t *partials;  // SCAN_BLOCKS + 1
   uint32_t *ncand;     // device candidate count
   size_t ccap;
+  uint16_t *cseg;      // ccap (batch: level set of each candidate)
+};
+
+// A batch launch: level-set table and per-word level set (device), words.
+struct SegCtx {
+  const SegInfo *segs;
+  uint32_t nseg;
+  uint32_t totalBase;
+  const uint16_t *wseg;
+  uint32_t nw;
 };
 
 BandView viewOf(const BandBufs &b, const Frame &f, const GridInfo &g);
+
+void setScanBlocks(int blocks);
+int scanBlocks();
+void launchStarOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const SegCtx &c, Workspace &ws,
+                     const OpParams &op, DevScalars *sc);
+void launchSelfOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const SegCtx &c, Workspace &ws,
+                     const OpParams &op, DevScalars *sc);
+void launchRatesSeg(hipStream_t s, int D, const BandBufs &b, const SegCtx &c, const RateParams &rp, double *rates,
+                    DevScalars *sc, unsigned long long *segMax, const double *maxDt, double *dtOut, Workspace &ws);
+void launchStepBeginSeg(hipStream_t s, DevScalars *sc, double *maxDt, uint32_t nseg);
+void launchStepTimeSeg(hipStream_t s, const DevScalars *sc, const double *dt, uint32_t nseg, int temporal,
+                       double *stepTime, double *advTime);
+void launchFrameCheckSeg(hipStream_t s, const BandBufs &b, const SegCtx &c, int D, int slack, DevScalars *sc);
+void launchCopyBandWords(hipStream_t s, const BandBufs &src, BandBufs &dst, int64_t nw);
+void launchBooleanSeg(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const SegCtx &c,
+                      Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc);
 
 // Band operations (all asynchronous on `s`; sizes live in device memory).
 void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
@@ -62,7 +90,8 @@ void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, B
                    const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc);
 // Rate pass: Hamiltonian + dissipation per active entry, CFL reduction.
 void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
-                 double *rates, DevScalars *sc, const double *maxTimeStep, double *dtOut);
+                 double *rates, DevScalars *sc, const double *maxTimeStep, double *dtOut, Workspace &ws,
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool externalEvents = false);
 // LF global alphas (Advect::findGlobalAlphas).
 void launchGlobalAlphas(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g,
                         const RateParams &rp, DevScalars *sc);
