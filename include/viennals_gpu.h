@@ -116,6 +116,20 @@ int lsg_advect_clear_level_sets(lsg_advect *a);
 /* setVelocityField (lsAdvect.hpp:864) with a built-in functor. */
 int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, double b, double a_pos, double a_neg,
                                     const double *vec3);
+/* setVelocityField (lsAdvect.hpp:864) with a VelocityField<T> subclass
+ * evaluated on the host (lsVelocityField.hpp:17-38): for every active point
+ * of every stage the library calls scalar(coordinate, material, normal,
+ * pointId, user) and vector(..., out[3]) with coordinate = index * gridDelta
+ * (3 components), material 0, the normal the scheme computes and pointId =
+ * rank among the defined points of the expanded band, in point order, on the
+ * calling thread.  Either callback may be null (zero).  Supported with the
+ * Engquist-Osher, local-local Lax-Friedrichs and WENO schemes. */
+typedef double (*lsg_scalar_velocity_fn)(const double *coordinate, int material, const double *normal,
+                                         uint64_t point_id, void *user);
+typedef void (*lsg_vector_velocity_fn)(const double *coordinate, int material, const double *normal,
+                                       uint64_t point_id, void *user, double *out3);
+int lsg_advect_set_velocity_host(lsg_advect *a, lsg_scalar_velocity_fn scalar, lsg_vector_velocity_fn vector,
+                                 void *user);
 int lsg_advect_set_spatial_scheme(lsg_advect *a, int scheme);     /* :936 */
 int lsg_advect_set_temporal_scheme(lsg_advect *a, int scheme);    /* :949 */
 int lsg_advect_set_advection_time(lsg_advect *a, double time);    /* :873 */

@@ -14,7 +14,7 @@ from ._lib import LsgError, check, lib
 __all__ = [
     "BoundaryConditionEnum", "SpatialSchemeEnum", "TemporalSchemeEnum", "BooleanOperationEnum",
     "Domain", "Sphere", "Plane", "Box", "MakeGeometry", "Expand", "Reduce", "Prune", "BooleanOperation",
-    "VelocityField", "Advect", "BatchAdvect", "LsgError", "weno",
+    "VelocityField", "HostVelocityField", "Advect", "BatchAdvect", "LsgError", "weno",
 ]
 
 POS_VALUE = float(np.finfo(np.float64).max)
@@ -273,6 +273,19 @@ class VelocityField:
                     vec=self.vec)
 
 
+class HostVelocityField:
+    """VelocityField<T> (lsVelocityField.hpp:11-41) implemented in Python:
+    override getScalarVelocity / getVectorVelocity.  Evaluated on the host
+    for every active point of every stage (the slow path; the built-in
+    VelocityField functors run on the GPU)."""
+
+    def getScalarVelocity(self, coordinate, material, normal, pointId):
+        return 0.0
+
+    def getVectorVelocity(self, coordinate, material, normal, pointId):
+        return (0.0, 0.0, 0.0)
+
+
 class Advect:
     """Advect<T,D> (lsAdvect.hpp:54-1044) running on the GPU."""
 
@@ -304,6 +317,21 @@ class Advect:
         check(lib().lsg_advect_clear_level_sets(self._h))
 
     def setVelocityField(self, v):
+        if isinstance(v, HostVelocityField):
+            from ._lib import SCALAR_VELOCITY_FN, VECTOR_VELOCITY_FN
+
+            def scalar(c, m, n, pid, user):
+                return float(v.getScalarVelocity((c[0], c[1], c[2]), m, (n[0], n[1], n[2]), pid))
+
+            def vector(c, m, n, pid, user, out):
+                r = v.getVectorVelocity((c[0], c[1], c[2]), m, (n[0], n[1], n[2]), pid)
+                out[0], out[1], out[2] = float(r[0]), float(r[1]), float(r[2])
+
+            self._callbacks = (SCALAR_VELOCITY_FN(scalar), VECTOR_VELOCITY_FN(vector))
+            check(lib().lsg_advect_set_velocity_host(self._h, ctypes.cast(self._callbacks[0], ctypes.c_void_p),
+                                                     ctypes.cast(self._callbacks[1], ctypes.c_void_p), None))
+            return
+        self._callbacks = None
         vec = (ctypes.c_double * 3)(*v.vec)
         check(lib().lsg_advect_set_velocity_builtin(self._h, v.kind, v.dir, v.k, v.b, v.a_pos, v.a_neg, vec))
 

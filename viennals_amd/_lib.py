@@ -34,7 +34,7 @@ EXPORTED = [
     "lsg_mlp_set_samples", "lsg_mlp_loss_grad", "lsg_mlp_train", "lsg_mlp_get_params",
     "lsg_batch_create", "lsg_batch_destroy", "lsg_batch_set_velocity_builtin", "lsg_batch_set_schemes",
     "lsg_batch_apply", "lsg_batch_get_advected_times", "lsg_batch_get_point_updates", "lsg_batch_write_back",
-    "lsg_batch_set_scan_blocks", "lsg_batch_autotune",
+    "lsg_batch_set_scan_blocks", "lsg_batch_autotune", "lsg_advect_set_velocity_host",
 ]
 
 
@@ -117,6 +117,7 @@ def lib():
         "lsg_batch_write_back": ([P], c_int),
         "lsg_batch_set_scan_blocks": ([P, c_int], c_int),
         "lsg_batch_autotune": ([P, ip, dp], c_int),
+        "lsg_advect_set_velocity_host": ([P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], c_int),
         "lsg_device_synchronize": ([], c_int),
     }
     for name, (args, res) in sig.items():
@@ -133,3 +134,11 @@ def check(code):
         msg = lib().lsg_last_error()
         raise LsgError(f"viennals_gpu error {code}: {msg.decode() if msg else ''}")
     return code
+
+
+# Host velocity callbacks (lsg_scalar_velocity_fn / lsg_vector_velocity_fn).
+SCALAR_VELOCITY_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_uint64, ctypes.c_void_p)
+VECTOR_VELOCITY_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_uint64, ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_double))

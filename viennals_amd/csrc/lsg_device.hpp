@@ -239,6 +239,9 @@ struct RateParams {
   double vec[3];
   // learned weno5 weights: network parameters as bf16 (WENO5 learned only)
   const uint16_t *mlp;
+  // host-evaluated velocities: scalar and vector per active point, in
+  // active-list order (4 doubles each); null for the built-in functor
+  const double *hostVel;
 };
 
 // Learned weno5 network: 5 -> 32 -> 32 -> 3 (GELU, softmax); parameter

@@ -552,6 +552,15 @@ struct lsg_advect {
   lsg::DevScalars *hostSc = nullptr;  // pinned
   uint32_t *hostStat = nullptr;       // pinned
   uint16_t *mlpDev = nullptr;         // learned weno5 parameters (bf16)
+  // host-evaluated velocity (VelocityField subclass on the host)
+  lsg_scalar_velocity_fn hostScalar = nullptr;
+  lsg_vector_velocity_fn hostVector = nullptr;
+  void *hostUser = nullptr;
+  double *velIn = nullptr, *velOut = nullptr;  // device: 7 / 4 doubles per active point
+  uint32_t *dlist = nullptr, *ndef = nullptr;
+  size_t velCap = 0;
+  std::vector<double> velInH, velOutH;
+  int hostErr = 0;
   void dropGraphs() {
     for (auto &g : graphs) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -568,6 +577,8 @@ struct lsg_advect {
     if (hostSc) (void)hipHostFree(hostSc);
     if (hostStat) (void)hipHostFree(hostStat);
     if (mlpDev) (void)hipFree(mlpDev);
+    for (void *p : {(void *)velIn, (void *)velOut, (void *)dlist, (void *)ndef})
+      if (p) (void)hipFree(p);
   }
 };
 
@@ -608,9 +619,53 @@ static int ensureRates(lsg_advect *a, size_t cap) {
 
 // One forward-Euler stage (lsAdvectIntegrationSchemes.hpp:46-59):
 // computeRates (prepareLS + integrateTime), updateLevelSet, rebuildLS.
+// Host-evaluated velocities of the active points of the expanded band: the
+// inputs of getScalarVelocity / getVectorVelocity are gathered on the device,
+// the callbacks run on the host in point order, the results go back as the
+// rate kernel's per-point velocities.
+static int hostVelocities(lsg_advect *a, lsg_domain *d) {
+  const size_t cap = d->cur.b.cap;
+  if (a->velCap < cap) {
+    for (void *p : {(void *)a->velIn, (void *)a->velOut, (void *)a->dlist}) if (p) (void)hipFree(p);
+    a->velIn = a->velOut = nullptr;
+    a->dlist = nullptr;
+    a->velCap = 0;
+    LSG_HIP(hipMalloc(&a->velIn, cap * 7 * sizeof(double)));
+    LSG_HIP(hipMalloc(&a->velOut, cap * 4 * sizeof(double)));
+    LSG_HIP(hipMalloc(&a->dlist, cap * sizeof(uint32_t)));
+    a->velCap = cap;
+  }
+  if (!a->ndef) LSG_HIP(hipMalloc(&a->ndef, sizeof(uint32_t)));
+  launchVelInputs(g_ctx.s, d->D, d->cur.b, d->f, d->g, a->rp, g_ws->w, a->dlist, a->ndef, a->velIn, g_ctx.sc);
+  uint32_t nact = 0;
+  LSG_HIP(hipMemcpyAsync(&nact, g_ws->w.ncand, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  a->velInH.resize((size_t)nact * 7);
+  a->velOutH.resize((size_t)nact * 4);
+  if (nact) {
+    LSG_HIP(hipMemcpy(a->velInH.data(), a->velIn, (size_t)nact * 7 * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  for (uint32_t j = 0; j < nact; ++j) {
+    const double *in = &a->velInH[(size_t)j * 7];
+    double *o = &a->velOutH[(size_t)j * 4];
+    const uint64_t pid = (uint64_t)in[6];
+    o[0] = a->hostScalar ? a->hostScalar(in, 0, in + 3, pid, a->hostUser) : 0.0;
+    o[1] = o[2] = o[3] = 0.0;
+    if (a->hostVector) a->hostVector(in, 0, in + 3, pid, a->hostUser, o + 1);
+  }
+  if (nact)
+    LSG_HIP(hipMemcpy(a->velOut, a->velOutH.data(), (size_t)nact * 4 * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
 static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *dtOut) {
   doExpand(d, prepareWidth(a->spatial));  // Advect::prepareLS (lsAdvect.hpp:973-1015)
   RateParams rp = a->rp;
+  rp.hostVel = nullptr;
+  if (a->hostScalar || a->hostVector) {
+    if (!a->hostErr) a->hostErr = hostVelocities(a, d);
+    rp.hostVel = a->velOut;
+  }
   if (a->spatial == 2 || a->spatial == 3) launchGlobalAlphas(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, g_ctx.sc);
   if (a->profiling && a->evUsed + 2 <= (int)a->evPool.size()) {
     // events bracket the rate kernel itself
@@ -727,7 +782,7 @@ static void enqueueStepBody(lsg_advect *a, lsg_domain *d) {
   copyDomainNoAlloc(a->save, d);
   a->evUsed = 0;
   enqueueStep(a, d);
-  launchBandStats(g_ctx.s, d->cur.b, d->f, g_ctx.stat, (int32_t *)(g_ctx.stat + 1));
+  launchFrameCheck(g_ctx.s, d->cur.b, d->f, d->g, d->D, REFRAME_SLACK, g_ctx.sc);
 }
 
 static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &out) {
@@ -744,7 +799,9 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
     int widthBefore = d->width;
     *a->hostMaxDt = maxTimeStep;
     bool launched = false;
-    if (a->useGraphs) {
+    const bool hostVel = a->hostScalar || a->hostVector;
+    a->hostErr = 0;
+    if (a->useGraphs && !hostVel) {
       uint64_t key = stepKey(a, d);
       lsg_advect::StepGraph *g = nullptr;
       for (auto &c : a->graphs)
@@ -799,8 +856,8 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
       launched = true;
     }
     if (!launched) enqueueStepBody(a, d);
+    if (a->hostErr) return a->hostErr;
     LSG_HIP(hipMemcpyAsync(a->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
-    LSG_HIP(hipMemcpyAsync(a->hostStat, g_ctx.stat, 7 * sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
     LSG_HIP(hipStreamSynchronize(g_ctx.s));
     const DevScalars &h = *a->hostSc;
     if (!h.overflow) {
@@ -816,16 +873,8 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
           }
         }
       }
-      // re-frame when the band approaches the frame edge (statistics of this step)
-      const int32_t *bb = (const int32_t *)(a->hostStat + 1);
-      bool need = false;
-      if (bb[0] <= bb[3]) {
-        for (int k = 0; k < d->D; ++k) {
-          if (d->g.bounded(k)) continue;
-          if (bb[k] < REFRAME_SLACK || (d->f.ext[k] - 1) - bb[3 + k] < REFRAME_SLACK) need = true;
-        }
-      }
-      if (need) {
+      // re-frame when the band approached the frame edge in this step
+      if (h.reframe) {
         a->dropGraphs();
         return maybeReframe(d, true);
       }
@@ -864,6 +913,10 @@ static int advectApply(lsg_advect *a) {
   a->rp.checkDissipation = a->checkDissipation;
   // WENO receives dissipationAlpha as its calcNormal flag (lsAdvect.hpp:670-676)
   a->rp.calcNormals = (a->spatial >= 10) ? (a->dissipationAlpha != 0.0) : a->calcNormals;
+  if ((a->hostScalar || a->hostVector) &&
+      !(a->spatial == 0 || a->spatial == 1 || a->spatial == 5 || a->spatial == 6 || a->spatial >= 10))
+    return fail(-9, "Advect: host-evaluated velocities are supported with the Engquist-Osher, local-local "
+                    "Lax-Friedrichs and WENO schemes");
   if (a->spatial == 12 && !a->mlpDev)
     return fail(-8, "Advect: learned WENO5 selected but no network parameters were set");
   a->rp.mlp = a->mlpDev;
@@ -1562,8 +1615,21 @@ int lsg_advect_clear_level_sets(lsg_advect *a) {
   a->levelSets.clear();
   return 0;
 }
+int lsg_advect_set_velocity_host(lsg_advect *a, lsg_scalar_velocity_fn scalar, lsg_vector_velocity_fn vector,
+                                 void *user) {
+  if (!scalar && !vector) return fail(-1, "lsg_advect_set_velocity_host: no callback");
+  a->hostScalar = scalar;
+  a->hostVector = vector;
+  a->hostUser = user;
+  a->dropGraphs();
+  a->velSet = true;
+  return 0;
+}
 int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, double b, double aPos, double aNeg,
                                     const double *vec3) {
+  a->hostScalar = nullptr;
+  a->hostVector = nullptr;
+  a->hostUser = nullptr;
   if (kind != 0 && kind != 1) return fail(-1, "unknown velocity kind");
   if (dir < 0 || dir > 2) return fail(-1, "velocity direction must be 0, 1 or 2");
   a->rp.velKind = kind;

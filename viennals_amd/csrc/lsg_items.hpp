@@ -845,6 +845,59 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
   return {totalGrad, dissipation};
 }
 
+// Normal vector a scheme passes to VelocityField::getScalarVelocity at an
+// active point (host-evaluated velocities): EO without a zero guard
+// (lsEngquistOsher.hpp:124-138), WENO with one (lsWENO.hpp:113-129), both
+// only with calcNormals; LLLF always, from the order-corrected one-sided
+// differences (lsLocalLocalLaxFriedrichs.hpp:95-120).
+template <int D> LSG_HD inline void schemeNormal(const BandView &b, const int32_t c[3], double phi0, int scheme,
+                                                 int order, bool calcNormals, double n[3]) {
+  n[0] = n[1] = n[2] = 0.;
+  if (scheme == 3 /* S_LLLF */) {
+    double st[D][5];
+    gatherUpwind<D>(b, c, order, st);
+    double nm = 0;
+    LSG_UNROLL
+    for (int i = 0; i < D; i++) {
+      double diffPos, diffNeg;
+      upwindDiffs(st[i], b.g.dx, order, phi0, diffPos, diffNeg);
+      n[i] = (diffNeg + diffPos) * 0.5;
+      nm += n[i] * n[i];
+    }
+    nm = 1. / sqrt(nm);
+    LSG_UNROLL
+    for (int i = 0; i < D; ++i) n[i] *= nm;
+    return;
+  }
+  if (!calcNormals) return;
+  double st[D][3];
+  gatherAxes<D, 1>(b, c, st);
+  double den = 0;
+  if (scheme == 0 /* S_EO */) {
+    LSG_UNROLL
+    for (int i = 0; i < D; i++) {
+      const double pos = st[i][2] - phi0;
+      const double neg = phi0 - st[i][0];
+      n[i] = (pos + neg) * 0.5;
+      den += n[i] * n[i];
+    }
+    den = 1. / sqrt(den);
+    LSG_UNROLL
+    for (int i = 0; i < D; ++i) n[i] *= den;
+  } else {  // WENO
+    LSG_UNROLL
+    for (int i = 0; i < D; i++) {
+      n[i] = st[i][2] - st[i][0];
+      den += n[i] * n[i];
+    }
+    if (den > 0) {
+      den = 1. / sqrt(den);
+      LSG_UNROLL
+      for (int i = 0; i < D; ++i) n[i] *= den;
+    }
+  }
+}
+
 // Scheme ids of the rate kernel.
 enum SchemeId : int { S_EO = 0, S_WENO5 = 1, S_LF = 2, S_LLLF = 3, S_LLF = 4, S_WENO3 = 5, S_WENO5L = 6 };
 

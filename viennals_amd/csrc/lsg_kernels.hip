@@ -162,9 +162,8 @@ __global__ __launch_bounds__(256) void kCandCount(const uint64_t *A, const uint6
 
 // Phase 2: word prefix counts and the candidate position list (index
 // base + cpre[w] + j); the last block stores the totals.  The positions of a
-// tile are written by consecutive threads (coalesced): thread s finds the
-// word holding the tile's s-th candidate by binary search over the tile's
-// prefix counts in LDS and selects the bit.  In a batch every level set's
+// word are written by the 64 lanes of a wave at once (lane = bit), so the
+// stores of a full word are one contiguous 256-byte segment.  In a batch every level set's
 // candidates are preceded by its grid-minimum candidate (when its frame
 // does not hold the grid minimum), so the candidate index of a word is
 // shifted by the grid-minimum entries up to its level set, and each
@@ -213,20 +212,23 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t
       sSeg[threadIdx.x] = k;
     }
     __syncthreads();
+    // each wave writes the positions of its 64 words, one word per
+    // iteration with lane = bit (consecutive lanes write consecutive slots)
     const size_t q0 = SEG ? (size_t)off : (size_t)base + off;
-    for (uint32_t s = threadIdx.x; s < tot; s += TPB) {
-      int a = 0, b = TPB - 1;  // last j with sPre[j] <= s
-      while (a < b) {
-        const int m = (a + b + 1) >> 1;
-        if (sPre[m] <= s) a = m;
-        else b = m - 1;
+    const int lane = threadIdx.x & 63;
+    const int wbase = threadIdx.x & ~63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int a = wbase; a < wbase + 64; ++a) {
+      const uint64_t cw = sWord[a];
+      if (cw == 0ull) continue;  // wave-uniform
+      if ((cw >> lane) & 1ull) {
+        const size_t q = q0 + sPre[a] + (uint32_t)__builtin_popcountll(cw & below) + (SEG ? sShift[a] : 0u);
+        if (q < ccap) {
+          const uint32_t word = SEG ? sLocal[a] : t + (uint32_t)a;  // positions are level-set local
+          cpos[q] = (word << 6) | (uint32_t)lane;
+          if (SEG) cseg[q] = sSeg[a];
+        }
       }
-      const size_t q = q0 + s + (SEG ? sShift[a] : 0u);
-      if (q >= ccap) continue;
-      const int bit = selectBit(sWord[a], (int)(s - sPre[a]));
-      const uint32_t word = SEG ? sLocal[a] : t + (uint32_t)a;  // positions are level-set local
-      cpos[q] = (word << 6) | (uint32_t)bit;
-      if (SEG) cseg[q] = sSeg[a];
     }
     __syncthreads();
     off += tot;
@@ -652,8 +654,13 @@ template <class Fn> __device__ inline void forEachActive(const double *val, uint
 __device__ inline bool isActive(double v) { return !isUndef(v) && fabs(v) <= 0.5; }
 
 __global__ __launch_bounds__(256) void kActiveCount(const double *val, const uint32_t *nptr, uint32_t *partials,
-                                                    DevScalars *sc) {
+                                                    DevScalars *sc, int resetRates, int keepAlphas) {
   LSG_ABORT_IF_OVERFLOW(sc);
+  if (resetRates && logicalBlock() == 0 && threadIdx.x == 0) {  // the rate pass starts from zero maxima
+    sc->maxAbsBits = 0;
+    if (!keepAlphas) sc->alphaBits[0] = sc->alphaBits[1] = sc->alphaBits[2] = 0;
+    sc->activeCount = 0;
+  }
   const uint32_t n = *nptr;
   uint32_t lo, hi;
   blockChunk(n, lo, hi, 1);
@@ -682,6 +689,62 @@ __global__ __launch_bounds__(256) void kActiveList(const double *val, const uint
   if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) *nact = off;
 }
 
+// Defined entries in entry order (point ids of the host-velocity path).
+__global__ __launch_bounds__(256) void kDefinedCount(const double *val, const uint32_t *nptr, uint32_t *partials) {
+  const uint32_t n = *nptr;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
+  uint32_t acc = 0;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += TPB) acc += isUndef(val[i]) ? 0u : 1u;
+  const uint32_t t = blockSum(acc);
+  if (threadIdx.x == 0) partials[logicalBlock()] = t;
+}
+__global__ __launch_bounds__(256) void kDefinedList(const double *val, const uint32_t *nptr, const uint32_t *partials,
+                                                    uint32_t *list, uint32_t *ndef) {
+  const uint32_t n = *nptr;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
+  uint32_t off = blockOffset(partials);
+  for (uint32_t t = lo; t < hi; t += TPB) {
+    const uint32_t i = t + threadIdx.x;
+    const bool a = (i < hi) && !isUndef(val[i]);
+    uint32_t tot;
+    uint64_t wm;
+    const uint32_t p = off + blockBallotScan(a, tot, wm);
+    if (a) list[p] = i;
+    off += tot;
+  }
+  if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) *ndef = off;
+}
+
+// Inputs of VelocityField::getScalarVelocity / getVectorVelocity for every
+// active point (lsEngquistOsher.hpp:140-145, lsWENO.hpp:132-137): coordinate
+// index * gridDelta (3 components), the scheme's normal, and the point id
+// (rank among the defined points of the expanded band); 7 doubles each.
+template <int D>
+__global__ void kVelInputs(BandView b, const uint32_t *alist, const uint32_t *nact, const uint32_t *dlist,
+                           const uint32_t *ndef, RateParams rp, double *out) {
+  const uint32_t n = *nact, nd = *ndef;
+  const int scheme = schemeIdOf(rp.spatial);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t i = alist[j];
+    int32_t c[3];
+    candCoords(b, i, b.pos[i], c);
+    double nrm[3];
+    schemeNormal<D>(b, c, b.val[i], scheme, rp.order, rp.calcNormals != 0, nrm);
+    uint32_t lo = 0, hi = nd;  // first defined entry >= i
+    while (lo < hi) {
+      const uint32_t m = (lo + hi) >> 1;
+      if (dlist[m] < i) lo = m + 1;
+      else hi = m;
+    }
+    double *o = out + 7 * (size_t)j;
+    for (int d = 0; d < 3; ++d) o[d] = (d < D) ? (double)c[d] * b.g.dx : 0.0;
+    for (int d = 0; d < 3; ++d) o[3 + d] = nrm[d];
+    o[6] = (double)lo;
+  }
+}
+
 // Hamiltonian + dissipation per active entry of the expanded band
 // (Advect::integrateTime, lsAdvect.hpp:495-593, one level set): writes the
 // stage velocity and reduces max |H - D| (and the LLF/LLLF alphas).
@@ -700,10 +763,17 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
   SegArgs sa;
   unsigned long long *segMax;
   uint32_t curSeg;
-  __device__ __attribute__((always_inline)) void operator()(uint32_t i) {
+  __device__ __attribute__((always_inline)) void operator()(uint32_t i, uint32_t j) {
     ++act;
     BandView v = b;
     Vel ve = vel;
+    if (!SEG && rp.hostVel) {  // velocity evaluated on the host for active point j
+      ve.kind = 0;
+      ve.b = rp.hostVel[4 * (size_t)j];
+      ve.vec[0] = rp.hostVel[4 * (size_t)j + 1];
+      ve.vec[1] = rp.hostVel[4 * (size_t)j + 2];
+      ve.vec[2] = rp.hostVel[4 * (size_t)j + 3];
+    }
     if (SEG) {
       const uint32_t k = sa.iseg[i];
       const SegInfo &S = sa.segs[k];
@@ -744,7 +814,7 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
   RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst};
   if (SCHEME == S_LF)
     for (int i = 0; i < 3; ++i) fn.ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
-  for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) fn(list[j]);
+  for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) fn(list[j], j);
   if (!SEG) {
     blockMaxAtomic((unsigned long long)__double_as_longlong(fn.lmax), &sc->maxAbsBits);
     if (SCHEME == S_LLLF || SCHEME == S_LLF)
@@ -789,8 +859,9 @@ __global__ void kResetRates(DevScalars *sc, int keepAlphas) {
 }
 
 // Active list of band b into ws.cpos / ws.ncand (both free between band operations).
-static void activeList(hipStream_t s, const BandBufs &b, Workspace &ws, DevScalars *sc) {
-  kActiveCount<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, sc);
+static void activeList(hipStream_t s, const BandBufs &b, Workspace &ws, DevScalars *sc, int resetRates = 0,
+                       int keepAlphas = 0) {
+  kActiveCount<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, sc, resetRates, keepAlphas);
   kActiveList<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, ws.cpos, ws.ncand, sc);
 }
 
@@ -804,8 +875,8 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
                  double *rates, DevScalars *sc, const double *maxTimeStep, double *dtOut, Workspace &ws,
                  hipEvent_t e0, hipEvent_t e1, bool externalEvents) {
   const int scheme = schemeIdOf(rp.spatial);
-  kResetRates<<<1, 1, 0, s>>>(sc, scheme == S_LF ? 1 : 0);
-  activeList(s, b, ws, sc);
+  if (!rp.hostVel) activeList(s, b, ws, sc, 1, scheme == S_LF ? 1 : 0);  // resets the rate maxima too
+  else kResetRates<<<1, 1, 0, s>>>(sc, scheme == S_LF ? 1 : 0);  // the list of launchVelInputs is still in ws
   BandView v = viewOf(b, f, g);
   const int ge = RATE_BLOCKS;
   recordEvent(s, e0, externalEvents);
@@ -836,6 +907,17 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
   recordEvent(s, e1, externalEvents);
   const int useAlphas = (scheme == S_LF || scheme == S_LLLF || scheme == S_LLF) ? 1 : 0;
   kTimeStep<<<1, 1, 0, s>>>(sc, rp, D, g.dx, maxTimeStep, useAlphas, dtOut, b.n);
+}
+
+// Active list, defined list and the velocity inputs of the active points.
+void launchVelInputs(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
+                     Workspace &ws, uint32_t *dlist, uint32_t *ndef, double *out, DevScalars *sc) {
+  activeList(s, b, ws, sc);
+  kDefinedCount<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials);
+  kDefinedList<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, dlist, ndef);
+  BandView v = viewOf(b, f, g);
+  if (D == 2) kVelInputs<2><<<RATE_BLOCKS, TPB, 0, s>>>(v, ws.cpos, ws.ncand, dlist, ndef, rp, out);
+  else kVelInputs<3><<<RATE_BLOCKS, TPB, 0, s>>>(v, ws.cpos, ws.ncand, dlist, ndef, rp, out);
 }
 
 // Batch rate pass: every level set's stage velocities and max |H - D| per
@@ -922,6 +1004,27 @@ void launchStepTimeSeg(hipStream_t s, const DevScalars *sc, const double *dt, ui
   kStepTimeSeg<<<(nseg + TPB - 1) / TPB, TPB, 0, s>>>(sc, dt, nseg, temporal, stepTime, advTime);
 }
 
+// Flags (sc->reframe = 1) an entry closer than `slack` to the frame edge in
+// an infinite dimension (the host then moves the frame between steps).
+__global__ void kFrameCheck(const uint32_t *pos, const uint32_t *nptr, Frame f, GridInfo g, int D, int slack,
+                            DevScalars *sc) {
+  const uint32_t n = *nptr;
+  bool near = false;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t p = pos[i];
+    if (p == POS_NONE) continue;
+    int32_t r[3];
+    f.decode(p, r[0], r[1], r[2]);
+    for (int d = 0; d < D; ++d)
+      if (!g.bounded(d) && (r[d] < slack || f.ext[d] - 1 - r[d] < slack)) near = true;
+  }
+  if (near) sc->reframe = 1;
+}
+void launchFrameCheck(hipStream_t s, const BandBufs &b, const Frame &f, const GridInfo &g, int D, int slack,
+                      DevScalars *sc) {
+  kFrameCheck<<<gridFor(b.cap, 2048), TPB, 0, s>>>(b.pos, b.n, f, g, D, slack, sc);
+}
+
 // Flags (sc->reframe = 1) an entry of a batch band closer than `slack` to its
 // frame edge in an infinite dimension: the host then moves the frames.
 __global__ void kFrameCheckSeg(const uint32_t *pos, const uint16_t *seg, const uint32_t *nptr, const SegInfo *segs,
@@ -985,6 +1088,7 @@ void launchStepTime(hipStream_t s, DevScalars *sc, int temporal) { kStepTime<<<1
 
 __global__ void kStepBegin(DevScalars *sc, const double *maxDt) {
   sc->overflow = 0;
+  sc->reframe = 0;
   sc->stepActive0 = sc->totalActive;
   sc->maxTimeStep = *maxDt;
 }

@@ -71,11 +71,15 @@ void launchStarOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, co
                      const OpParams &op, DevScalars *sc);
 void launchSelfOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const SegCtx &c, Workspace &ws,
                      const OpParams &op, DevScalars *sc);
+void launchVelInputs(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
+                     Workspace &ws, uint32_t *dlist, uint32_t *ndef, double *out, DevScalars *sc);
 void launchRatesSeg(hipStream_t s, int D, const BandBufs &b, const SegCtx &c, const RateParams &rp, double *rates,
                     DevScalars *sc, unsigned long long *segMax, const double *maxDt, double *dtOut, Workspace &ws);
 void launchStepBeginSeg(hipStream_t s, DevScalars *sc, double *maxDt, uint32_t nseg);
 void launchStepTimeSeg(hipStream_t s, const DevScalars *sc, const double *dt, uint32_t nseg, int temporal,
                        double *stepTime, double *advTime);
+void launchFrameCheck(hipStream_t s, const BandBufs &b, const Frame &f, const GridInfo &g, int D, int slack,
+                      DevScalars *sc);
 void launchFrameCheckSeg(hipStream_t s, const BandBufs &b, const SegCtx &c, int D, int slack, DevScalars *sc);
 void launchCopyBandWords(hipStream_t s, const BandBufs &src, BandBufs &dst, int64_t nw);
 void launchBooleanSeg(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const SegCtx &c,

@@ -400,12 +400,36 @@ def enclosed_volume(domain):
     return (frac + interior.sum()) * domain.gridDelta ** 3
 
 
+def surface_distance(a, b):
+    """Symmetric mean distance between the zero level sets of a and b: the
+    surface points of one evaluated in the other's level set (trilinear
+    interpolation of phi times its gridDelta), averaged over both."""
+    from scipy.ndimage import map_coordinates
+
+    def one_way(src, dst):
+        pts = surface_points(src)
+        idx, val = dst.defined_points()
+        A, lo = _dense(idx, val, pad=2)
+        coords = (pts / dst.gridDelta - lo).T
+        phi = map_coordinates(A, coords, order=1, mode="constant", cval=np.nan)
+        ok = ~np.isnan(phi)
+        return np.abs(phi[ok]) * dst.gridDelta, int((~ok).sum())
+
+    da, ma = one_way(a, b)
+    db, mb = one_way(b, a)
+    return (da.sum() + db.sum()) / (len(da) + len(db)), ma + mb
+
+
 def compare(shape, vel, T, params, log=None):
-    """Fine plain (dx 0.5) vs coarse plain and coarse learned (dx 1.0)."""
+    """Fine plain (dx 0.5) vs coarse plain and coarse learned (dx 1.0):
+    Chamfer distance of the surface point sets, surface distance, relative
+    volume error, wall time of Advect::apply(T) on the GPU."""
     res = {}
     fine = star_domain(shape, 0.5)
     sf, wf, tf = advect(fine, vel, T)
     res["fine"] = dict(steps=sf, wall=wf, time=tf)
+    if log:
+        log(f"fine: {res['fine']}")
     pf = surface_points(fine)
     vf = enclosed_volume(fine)
     for name, spatial, prm in (("coarse_plain", SpatialSchemeEnum.WENO_5TH_ORDER, None),
@@ -414,7 +438,9 @@ def compare(shape, vel, T, params, log=None):
         s, w, t = advect(c, vel, T, spatial=spatial, params=prm)
         pc = surface_points(c)
         vc = enclosed_volume(c)
-        res[name] = dict(steps=s, wall=w, time=t, chamfer=chamfer(pc, pf), volume_error=abs(vc - vf) / vf)
+        sd, miss = surface_distance(c, fine)
+        res[name] = dict(steps=s, wall=w, time=t, chamfer=float(chamfer(pc, pf)), surface_distance=float(sd),
+                         unmatched=miss, volume_error=float(abs(vc - vf) / vf), speedup_vs_fine=wf / w)
         if log:
             log(f"{name}: {res[name]}")
     return res
