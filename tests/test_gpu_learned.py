@@ -126,3 +126,23 @@ def test_learned_scheme_requires_weights():
     a.setSpatialScheme(vls.SpatialSchemeEnum.WENO_5TH_ORDER_LEARNED)
     with pytest.raises(vls.LsgError, match="network parameters"):
         a.apply()
+
+
+def test_trained_network_gain_over_plain_coarse():
+    """The committed network (trained on fine WENO5-RK3 runs by the GPU
+    trainer) against plain WENO5 on the same coarse grid, both judged against
+    the fine plain run of the same star shape: it must not lose accuracy,
+    and on this shape it reduces the surface distance."""
+    import os
+    import viennals_amd as vls
+    from viennals_amd import learned as L
+    P = np.load(os.path.join(os.path.dirname(vls.__file__), "data", "weno5_learned.npy"))
+    rng = np.random.default_rng(1234)
+    shape = L.StarShape(32.0, seed=int(rng.integers(1 << 30)))
+    vel = L.random_velocity(rng)
+    r = L.compare(shape, vel, 6.0, P)
+    plain, learned = r["coarse_plain"], r["coarse_learned"]
+    print("surface distance plain %.5f learned %.5f; volume error plain %.2e learned %.2e" %
+          (plain["surface_distance"], learned["surface_distance"], plain["volume_error"], learned["volume_error"]))
+    assert learned["surface_distance"] < plain["surface_distance"]
+    assert learned["chamfer"] <= plain["chamfer"] * 1.01

@@ -548,9 +548,7 @@ LSG_HD inline void mlpForward(const float *P, const float in[MLP_IN], float w[ML
   w[2] = e2 / sum;
 }
 
-// Out of line: six calls per point would otherwise inline six copies of the
-// network and exhaust the register file.
-LSG_HD inline __attribute__((noinline)) double weno5Learned(const double *x, double dx, bool plus, const float *P) {
+LSG_HD inline double weno5Learned(const double *x, double dx, bool plus, const float *P) {
   double d[5];
   if (!plus) {
     LSG_UNROLL
@@ -629,14 +627,83 @@ LSG_HD inline RateOut schemeEO(const BandView &b, const int32_t c[3], double phi
 }
 
 // WENO::operator() (lsWENO.hpp:53-161)
+// The 2D one-sided weno5 derivatives of a point with learned weights
+// (weno5Learned for each): the stencil work is unrolled, the network runs in
+// one loop over the evaluations that need it (one copy of its code), its
+// inputs and outputs staged in the caller's scratch `io` (2D*5 floats).
+template <int D>
+LSG_HD inline void weno5LearnedAxes(double sts[][7], double dx, const float *P, float *io, double *gm, double *gp) {
+  double res[2 * D];
+  double pe[2 * D][3];
+  uint32_t net = 0;
+  LSG_UNROLL
+  for (int e = 0; e < 2 * D; ++e) {
+    const double *x = sts[e >> 1];
+    const bool plus = (e & 1) != 0;
+    double d[5];
+    if (!plus) {
+      LSG_UNROLL
+      for (int i = 0; i < 5; ++i) d[i] = x[i + 1] - x[i];
+    } else {
+      LSG_UNROLL
+      for (int i = 0; i < 5; ++i) d[i] = x[6 - i] - x[5 - i];
+    }
+    const double s1 = (13.0 / 12.0) * pow2(d[0] - 2 * d[1] + d[2]) + (1.0 / 4.0) * pow2(d[0] - 4 * d[1] + 3 * d[2]);
+    const double s2 = (13.0 / 12.0) * pow2(d[1] - 2 * d[2] + d[3]) + (1.0 / 4.0) * pow2(d[1] - d[3]);
+    const double s3 = (13.0 / 12.0) * pow2(d[2] - 2 * d[3] + d[4]) + (1.0 / 4.0) * pow2(3 * d[2] - 4 * d[3] + d[4]);
+    const double ee = 1e-6 * dx * dx;
+    const double a1 = ee + s1, a2 = ee + s2, a3 = ee + s3;
+    const double mx = dmaxs(a1, dmaxs(a2, a3));
+    const double mn = dmins(a1, dmins(a2, a3));
+    double m = 0;
+    LSG_UNROLL
+    for (int i = 0; i < 5; ++i) m = dmaxs(m, fabs(d[i]));
+    res[e] = 0.;
+    if (m == 0. || mx > MLP_KINK_RATIO * mn) {
+      res[e] = weno5(x, dx, plus);
+    } else {
+      net |= 1u << e;
+      LSG_UNROLL
+      for (int i = 0; i < 5; ++i) io[e * 5 + i] = (float)(d[i] / m);
+    }
+    pe[e][0] = (1.0 / 6.0) * (2 * d[0] - 7 * d[1] + 11 * d[2]);
+    pe[e][1] = (1.0 / 6.0) * (-d[1] + 5 * d[2] + 2 * d[3]);
+    pe[e][2] = (1.0 / 6.0) * (2 * d[2] + 5 * d[3] - d[4]);
+  }
+#if defined(__clang__)
+#pragma unroll 1
+#endif
+  for (int e = 0; e < 2 * D; ++e) {
+    if (!((net >> e) & 1u)) continue;
+    float in[MLP_IN], w[MLP_OUT];
+    LSG_UNROLL
+    for (int i = 0; i < MLP_IN; ++i) in[i] = io[e * 5 + i];
+    mlpForward(P, in, w);
+    LSG_UNROLL
+    for (int k = 0; k < MLP_OUT; ++k) io[e * 5 + k] = w[k];
+  }
+  LSG_UNROLL
+  for (int e = 0; e < 2 * D; ++e) {
+    if ((net >> e) & 1u)
+      res[e] = ((double)io[e * 5] * pe[e][0] + (double)io[e * 5 + 1] * pe[e][1] + (double)io[e * 5 + 2] * pe[e][2]) / dx;
+    if (e & 1) gp[e >> 1] = res[e];
+    else gm[e >> 1] = res[e];
+  }
+}
+
 template <int D, int ORDER, bool LEARNED = false>
 LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double phi0, bool calcNormals, const Vel &vel,
-                                 const float *mlp = nullptr) {
+                                 const float *mlp = nullptr, float *io = nullptr) {
   constexpr int r = (ORDER + 1) / 2;
   double gradPosTotal = 0, gradNegTotal = 0;
   double gm[D], gp[D];
   double sts[D][2 * r + 1];
   gatherAxes<D, r>(b, c, sts);
+  if constexpr (LEARNED) {
+    LSG_UNROLL
+    for (int i = 0; i < D; i++) sts[i][r] = phi0;
+    weno5LearnedAxes<D>(sts, b.g.dx, mlp, io, gm, gp);
+  }
   LSG_UNROLL
   for (int i = 0; i < D; i++) {
     double *st = sts[i];
@@ -644,10 +711,7 @@ LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double p
     if (ORDER == 3) {
       gm[i] = weno3(st, b.g.dx, false);
       gp[i] = weno3(st, b.g.dx, true);
-    } else if (LEARNED) {
-      gm[i] = weno5Learned(st, b.g.dx, false, mlp);
-      gp[i] = weno5Learned(st, b.g.dx, true, mlp);
-    } else {
+    } else if (!LEARNED) {
       gm[i] = weno5(st, b.g.dx, false);
       gp[i] = weno5(st, b.g.dx, true);
     }
@@ -919,12 +983,12 @@ LSG_HD inline int schemeIdOf(int spatial) {
 template <int D, int SCHEME>
 LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi0, const RateParams &rp, const Vel &vel,
                                const double *globalAlphas, double *alphaAcc, double &velocity,
-                               const float *mlp = nullptr) {
+                               const float *mlp = nullptr, float *io = nullptr) {
   RateOut r;
   if (SCHEME == S_EO) r = schemeEO<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO5) r = schemeWENO<D, 5>(b, c, phi0, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO3) r = schemeWENO<D, 3>(b, c, phi0, rp.calcNormals != 0, vel);
-  else if (SCHEME == S_WENO5L) r = schemeWENO<D, 5, true>(b, c, phi0, rp.calcNormals != 0, vel, mlp);
+  else if (SCHEME == S_WENO5L) r = schemeWENO<D, 5, true>(b, c, phi0, rp.calcNormals != 0, vel, mlp, io);
   else if (SCHEME == S_LF) r = schemeLF<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel, rp.alphaFactor, globalAlphas);
   else if (SCHEME == S_LLLF) r = schemeLLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
   else r = schemeLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);

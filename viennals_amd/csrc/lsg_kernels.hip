@@ -754,6 +754,7 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
   Vel vel;
   double *rates;
   const float *mlp;  // learned weno5 parameters staged in LDS
+  float *io;         // learned weno5: this lane's network inputs/outputs (LDS)
   double ga[3];
   double la[3];
   double lmax;
@@ -788,7 +789,7 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
     int32_t c[3];
     candCoords(v, i, v.pos[i], c);
     double velocity;
-    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp);
+    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp, io);
     if (velocity != 0.) {
       const double a = fabs(velocity);
       if (lmax < a) lmax = a;
@@ -811,7 +812,9 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
     __syncthreads();
   }
   const uint32_t segFirst = (SEG && lo < hi) ? sa.iseg[list[lo]] : 0u;
-  RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst};
+  __shared__ float mlpIO[SCHEME == S_WENO5L ? TPB * 2 * D * MLP_IN : 1];
+  float *io = (SCHEME == S_WENO5L) ? &mlpIO[threadIdx.x * 2 * D * MLP_IN] : nullptr;
+  RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, io, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst};
   if (SCHEME == S_LF)
     for (int i = 0; i < 3; ++i) fn.ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
   for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) fn(list[j], j);

@@ -406,14 +406,21 @@ def surface_distance(a, b):
     interpolation of phi times its gridDelta), averaged over both."""
     from scipy.ndimage import map_coordinates
 
+    from scipy.ndimage import distance_transform_edt
+
     def one_way(src, dst):
         pts = surface_points(src)
         idx, val = dst.defined_points()
         A, lo = _dense(idx, val, pad=2)
+        # cells outside the narrow band take their nearest defined value, so
+        # that every interpolation cell around the surface is complete
+        miss = np.isnan(A)
+        if miss.any():
+            near = distance_transform_edt(miss, return_distances=False, return_indices=True)
+            A = A[tuple(near)]
         coords = (pts / dst.gridDelta - lo).T
-        phi = map_coordinates(A, coords, order=1, mode="constant", cval=np.nan)
-        ok = ~np.isnan(phi)
-        return np.abs(phi[ok]) * dst.gridDelta, int((~ok).sum())
+        phi = map_coordinates(A, coords, order=1, mode="nearest")
+        return np.abs(phi) * dst.gridDelta, 0
 
     da, ma = one_way(a, b)
     db, mb = one_way(b, a)
