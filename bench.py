@@ -131,7 +131,7 @@ def run_batch(args):
     R3 = vls.TemporalSchemeEnum.RUNGE_KUTTA_3RD_ORDER
     # sequential per-case launches (same code path, one Advect per level set)
     seq_sets = []
-    for d in sets:
+    for d in (sets if args.seq_steps > 0 else []):
         c = vls.Domain.from_index_bounds(d.D, d.gmin, d.gmax, d.bc, d.gridDelta)
         c.deepCopy(d)
         seq_sets.append(c)
@@ -148,6 +148,28 @@ def run_batch(args):
     wall = time.perf_counter() - t0
     pu = bat.pointUpdates()[0] - pu0
     del bat
+    seq = None
+    if args.seq_steps > 0:
+        seq = sequential(vls, seq_sets, vels, W, R3, args.seq_steps)
+    rec = {
+        "metric": "aggregate point-updates/s (batched 512 level sets, WENO5-RK3 3D)",
+        "value": pu / wall,
+        "unit": "point-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batch{args.batch_size}_weno5_rk3",
+                   "level_sets": "half spheres r~U[8,64], half trenches w~U[16,96] d~U[32,192] L~U[128,512]",
+                   "seed": 42, "active_point_updates_per_step": pu / args.steps, "setup_seconds": setup,
+                   "autotuned_scan_blocks": blocks, "autotune_best_step_ms": tune_ms},
+        "sequential": seq,
+        "speedup_vs_sequential": (pu / wall) / seq["value"] if seq else None,
+    }
+    print(json.dumps(rec))
+    return 0
+
+
+def sequential(vls, seq_sets, vels, W, R3, steps):
+    """One Advect per level set, launched one after another (graph-captured steps)."""
     advs = []
     for d, v in zip(seq_sets, vels):
         a = vls.Advect(d, v)
@@ -160,29 +182,29 @@ def run_batch(args):
     vls.synchronize()
     seq_pu = 0.0
     t0 = time.perf_counter()
-    for _ in range(args.seq_steps):
+    for _ in range(steps):
         for a in advs:
             a.apply()
             seq_pu += a.getPointUpdates()
     vls.synchronize()
     seq_wall = time.perf_counter() - t0
-    rec = {
-        "metric": "aggregate point-updates/s (batched 512 level sets, WENO5-RK3 3D)",
-        "value": pu / wall,
-        "unit": "point-updates/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"batch{args.batch_size}_weno5_rk3",
-                   "level_sets": "half spheres r~U[8,64], half trenches w~U[16,96] d~U[32,192] L~U[128,512]",
-                   "seed": 42, "active_point_updates_per_step": pu / args.steps, "setup_seconds": setup,
-                   "autotuned_scan_blocks": blocks, "autotune_best_step_ms": tune_ms},
-        "sequential": {"value": seq_pu / seq_wall, "unit": "point-updates/s", "steps": args.seq_steps,
-                       "ms_per_step": seq_wall * 1e3 / args.seq_steps,
-                       "what": "one Advect (graph-captured steps) per level set, launched one after another"},
-        "speedup_vs_sequential": (pu / wall) / (seq_pu / seq_wall),
-    }
-    print(json.dumps(rec))
-    return 0
+    return {"value": seq_pu / seq_wall, "unit": "point-updates/s", "steps": steps,
+            "ms_per_step": seq_wall * 1e3 / steps,
+            "what": "one Advect (graph-captured steps) per level set, launched one after another"}
+
+
+def measured_traffic(workload):
+    """HBM bytes per rate-kernel launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (scaled by the copy-kernel
+    calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"kRates_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(idx, val, width, grid, params, steps, threads):
@@ -274,7 +296,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(cfg["workload"]),
             "launches": launches,
             "avg_launch_us": (ms * 1e3 / launches) if launches else None,
             "algorithmic_bytes_per_launch": (rbytes / launches) if launches else None,

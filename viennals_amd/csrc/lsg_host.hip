@@ -1036,7 +1036,7 @@ struct lsg_batch {
   uint32_t *bounds = nullptr, *boundsIdx = nullptr;  // pre[] at the level-set word boundaries (write-back)
   double pointUpdates = 0;
   unsigned steps = 0;
-  int scanBlocks = lsg::SCAN_BLOCKS;
+  int scanBlocks = lsg::defaultScanBlocks();
   bool useGraphs = true;
   struct Graph {
     uint64_t key = 0;
@@ -1312,7 +1312,7 @@ static uint64_t batchKey(lsg_batch *b) {
 static int batchStepOnce(lsg_batch *b);
 static int batchStep(lsg_batch *b) {
   const int e = batchStepOnce(b);
-  setScanBlocks(SCAN_BLOCKS);
+  setScanBlocks(defaultScanBlocks());
   return e;
 }
 static int batchStepOnce(lsg_batch *b) {

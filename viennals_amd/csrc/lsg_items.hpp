@@ -699,6 +699,9 @@ LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double p
   double gm[D], gp[D];
   double sts[D][2 * r + 1];
   gatherAxes<D, r>(b, c, sts);
+  double cd[D];  // central differences of the normal (lsWENO.hpp:113-129)
+  LSG_UNROLL
+  for (int i = 0; i < D; i++) cd[i] = sts[i][r + 1] - sts[i][r - 1];
   if constexpr (LEARNED) {
     LSG_UNROLL
     for (int i = 0; i < D; i++) sts[i][r] = phi0;
@@ -724,7 +727,7 @@ LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double p
     double denominator = 0;
     LSG_UNROLL
     for (int i = 0; i < D; i++) {
-      n[i] = sts[i][r + 1] - sts[i][r - 1];
+      n[i] = cd[i];
       denominator += n[i] * n[i];
     }
     if (denominator > 0) {

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 
 #include "lsg_device.hpp"
 #include "lsg_items.hpp"
@@ -18,12 +19,21 @@ namespace lsg {
 
 static constexpr int TPB = 256;
 // Grid of the scan kernels (blocks; at most SCAN_BLOCKS_MAX), chosen at run
-// time (batch autotuning).
-static int g_scanBlocks = SCAN_BLOCKS;
+// time (batch autotuning).  Default: SCAN_BLOCKS, or LSG_SCAN_BLOCKS from the
+// environment (grid sweeps).
+static int clampBlocks(int b) { return b < 8 ? 8 : (b > SCAN_BLOCKS_MAX ? SCAN_BLOCKS_MAX : b); }
+int defaultScanBlocks() {
+  static const int v = [] {
+    const char *e = getenv("LSG_SCAN_BLOCKS");
+    return e ? clampBlocks(atoi(e)) : SCAN_BLOCKS;
+  }();
+  return v;
+}
+static int g_scanBlocks = defaultScanBlocks();
 // Grid of the rate kernel: 4 workgroups per CU, each takes a contiguous
 // 1/1024 of the active list.
 static constexpr int RATE_BLOCKS = 1024;
-void setScanBlocks(int b) { g_scanBlocks = b < 8 ? 8 : (b > SCAN_BLOCKS_MAX ? SCAN_BLOCKS_MAX : b); }
+void setScanBlocks(int b) { g_scanBlocks = clampBlocks(b); }
 int scanBlocks() { return g_scanBlocks; }
 static int gridFor(size_t n, int cap = 4096) {
   size_t g = (n + TPB - 1) / TPB;
@@ -817,7 +827,19 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
   RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, io, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst};
   if (SCHEME == S_LF)
     for (int i = 0; i < 3; ++i) fn.ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
-  for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) fn(list[j], j);
+  if (SEG) {
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) fn(list[j], j);
+  } else {
+    // 64-point chunks dealt to the waves in launch order (round-robin over
+    // the XCDs and CUs), so that few active points still fill whole waves
+    // spread over the whole chip
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (TPB / 64);
+    for (uint32_t base = (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * 64u; base < n; base += waves * 64u) {
+      const uint32_t j = base + lane;
+      if (j < n) fn(list[j], j);
+    }
+  }
   if (!SEG) {
     blockMaxAtomic((unsigned long long)__double_as_longlong(fn.lmax), &sc->maxAbsBits);
     if (SCHEME == S_LLLF || SCHEME == S_LLF)

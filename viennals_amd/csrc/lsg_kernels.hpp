@@ -66,6 +66,7 @@ struct SegCtx {
 BandView viewOf(const BandBufs &b, const Frame &f, const GridInfo &g);
 
 void setScanBlocks(int blocks);
+int defaultScanBlocks();
 int scanBlocks();
 void launchStarOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const SegCtx &c, Workspace &ws,
                      const OpParams &op, DevScalars *sc);
