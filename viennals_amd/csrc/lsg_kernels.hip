@@ -228,9 +228,9 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t
     const int lane = threadIdx.x & 63;
     const int wbase = threadIdx.x & ~63;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll 4
     for (int a = wbase; a < wbase + 64; ++a) {
       const uint64_t cw = sWord[a];
-      if (cw == 0ull) continue;  // wave-uniform
       if ((cw >> lane) & 1ull) {
         const size_t q = q0 + sPre[a] + (uint32_t)__builtin_popcountll(cw & below) + (SEG ? sShift[a] : 0u);
         if (q < ccap) {
