@@ -333,12 +333,31 @@ template <class T, int D> class Advect {
   std::vector<SmartPointer<Domain<T, D>>> levelSets_;
   SmartPointer<VelocityField<T>> velocities_;
 
+  // Any other VelocityField subclass is evaluated on the host through its
+  // virtual methods, called back for every active point of every stage.
+  static double hostScalar(const double *c, int material, const double *n, uint64_t id, void *user) {
+    auto *v = static_cast<VelocityField<T> *>(user);
+    return (double)v->getScalarVelocity(Vec3D<T>{(T)c[0], (T)c[1], (T)c[2]}, material,
+                                        Vec3D<T>{(T)n[0], (T)n[1], (T)n[2]}, (unsigned long)id);
+  }
+  static void hostVector(const double *c, int material, const double *n, uint64_t id, void *user, double *out) {
+    auto *v = static_cast<VelocityField<T> *>(user);
+    const Vec3D<T> r = v->getVectorVelocity(Vec3D<T>{(T)c[0], (T)c[1], (T)c[2]}, material,
+                                            Vec3D<T>{(T)n[0], (T)n[1], (T)n[2]}, (unsigned long)id);
+    out[0] = (double)r[0];
+    out[1] = (double)r[1];
+    out[2] = (double)r[2];
+  }
+
   bool pushVelocity() {
-    auto *b = dynamic_cast<BuiltinVelocityField<T> *>(velocities_.get());
-    if (!b) {
-      std::fprintf(stderr, "[viennals_gpu] Advect: the GPU path needs a BuiltinVelocityField. Not advecting.\n");
+    if (!velocities_) {
+      std::fprintf(stderr, "[viennals_gpu] No velocity field passed to Advect. Not advecting.\n");
       return false;
     }
+    auto *b = dynamic_cast<BuiltinVelocityField<T> *>(velocities_.get());
+    if (!b)
+      return detail::ok(lsg_advect_set_velocity_host(h_, &Advect::hostScalar, &Advect::hostVector, velocities_.get()),
+                        "setVelocityField");
     double vec[3] = {b->vec[0], b->vec[1], b->vec[2]};
     return detail::ok(lsg_advect_set_velocity_builtin(h_, b->kind, b->dir, b->k, b->b, b->aPos, b->aNeg, vec),
                       "setVelocityField");

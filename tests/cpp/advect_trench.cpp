@@ -2,9 +2,13 @@
 // (compare examples/SquareEtch and tests/Advection): builds a trench with
 // MakeGeometry + BooleanOperation, advects it with WENO5 + RK3, and writes
 // the run lists before and after to binary files for the parity test.
-//   usage: advect_trench <out_prefix> <time>
+// With "host" as third argument the velocity is a user-defined
+// VelocityField subclass (evaluated on the host through its virtual
+// methods) computing the same function as the built-in functor.
+//   usage: advect_trench <out_prefix> <time> [builtin|host]
 #include <viennals_gpu/lsAdvect.hpp>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -25,6 +29,17 @@ static void dump(const ls::SmartPointer<ls::Domain<T, D>> &d, const std::string 
   std::fclose(f);
 }
 
+// The directional etch velocity -(max(0, n_z) + 0.1) written as reference
+// code would write it (same operations as the built-in normal-power functor).
+class EtchVelocity : public ls::VelocityField<T> {
+public:
+  T getScalarVelocity(const ls::Vec3D<T> &, int, const ls::Vec3D<T> &n, unsigned long) override {
+    const T c = n[2];
+    const T m = std::fabs(c);
+    return -0.1 + ((c > 0) ? -1.0 : 0.0) * m;
+  }
+};
+
 int main(int argc, char **argv) {
   std::string prefix = argc > 1 ? argv[1] : "trench";
   double time = argc > 2 ? std::atof(argv[2]) : 2.0;
@@ -41,7 +56,10 @@ int main(int argc, char **argv) {
   ls::BooleanOperation<T, D>(substrate, trench, ls::BooleanOperationEnum::RELATIVE_COMPLEMENT).apply();
   dump(substrate, prefix + "_initial.bin");
 
-  auto velocity = ls::BuiltinVelocityField<T>::normalPower(2, 1, -0.1, -1.0, 0.0);
+  const bool host = argc > 3 && std::string(argv[3]) == "host";
+  ls::SmartPointer<ls::VelocityField<T>> velocity;
+  if (host) velocity = ls::SmartPointer<EtchVelocity>::New();
+  else velocity = ls::BuiltinVelocityField<T>::normalPower(2, 1, -0.1, -1.0, 0.0);
   ls::Advect<T, D> advect;
   advect.insertNextLevelSet(substrate);
   advect.setVelocityField(velocity);

@@ -40,3 +40,20 @@ def test_cpp_facade_trench(tmp_path):
     io, vo = o.get()
     assert cases.bit_equal(io, vo, i1, v1), cases.first_diff(io, vo, i1, v1)
     assert f"steps {r['steps']}" in out.stdout
+
+
+def test_cpp_facade_user_velocity_field(tmp_path):
+    """A VelocityField subclass written as reference code writes it runs
+    through the host-callback path and gives the built-in functor's result."""
+    print("[gpu-case] C++ facade: trench with a user VelocityField subclass", flush=True)
+    exe = os.path.join(ROOT, "tests", "cpp", "advect_trench")
+    outs = {}
+    for mode in ("builtin", "host"):
+        prefix = str(tmp_path / mode)
+        out = subprocess.run([exe, prefix, "1.0", mode], capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        outs[mode] = (read_runs(prefix + "_final.bin"), out.stdout)
+    (ib, vb), sb = outs["builtin"]
+    (ih, vh), sh = outs["host"]
+    assert sb == sh
+    assert np.array_equal(ib, ih) and np.array_equal(vb, vh)
