@@ -156,8 +156,46 @@ LSG_HD inline double ruleUpdate(double c, double dt, double velocity) {
 
 // Centre value and the 2D star neighbours (boundary mapped) of point c:
 // nb[d] = +e_d, nb[D + d] = -e_d.  All 2D + 1 lookups are issued together.
-template <int D> LSG_HD inline void gatherStar(const BandView &b, const int32_t c[3], double &center, double *nb) {
-  BandView::Loc l[2 * D + 1];
+// Frame bit index of point c when c and every point up to R steps from it
+// along each axis lie inside the grid bounds (so boundary mapping is the
+// identity) and inside the frame; -1 otherwise.  Such neighbours are at
+// fixed bit offsets (1, rowBits, ext[1] rowBits) from c and their locate()
+// is exactly that index with inside = true.
+template <int D> LSG_HD inline int64_t interiorLin(const BandView &b, const int32_t c[3], int R) {
+  bool ok = true;
+  LSG_UNROLL
+  for (int d = 0; d < D; ++d) ok = ok && c[d] - R >= b.g.lo[d] && c[d] + R <= b.g.hi[d];
+  const int64_t xr = (int64_t)c[0] - b.f.o[0];
+  const int64_t yr = (int64_t)c[1] - b.f.o[1];
+  const int64_t zr = (int64_t)c[2] - b.f.o[2];
+  ok = ok && xr >= R && xr + R < b.f.ext[0] && yr >= R && yr + R < b.f.ext[1];
+  if (D == 3) ok = ok && zr >= R && zr + R < b.f.ext[2];
+  else ok = ok && zr >= 0 && zr < b.f.ext[2];
+  return ok ? ((zr * b.f.ext[1] + yr) * (int64_t)b.f.rowBits() + xr) : -1;
+}
+LSG_HD inline BandView::Loc locAt(int64_t lin) {
+  BandView::Loc l;
+  l.w = (uint32_t)(lin >> 6);
+  l.bt = (uint32_t)(lin & 63);
+  l.inside = true;
+  return l;
+}
+template <int D> LSG_HD inline int64_t axisStride(const BandView &b, int d) {
+  return d == 0 ? 1 : (d == 1 ? (int64_t)b.f.rowBits() : (int64_t)b.f.ext[1] * b.f.rowBits());
+}
+
+template <int D> LSG_HD inline void starLocs(const BandView &b, const int32_t c[3], BandView::Loc *l) {
+  const int64_t lin = interiorLin<D>(b, c, 1);
+  if (lin >= 0) {
+    l[2 * D] = locAt(lin);
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d) {
+      const int64_t st = axisStride<D>(b, d);
+      l[d] = locAt(lin + st);
+      l[D + d] = locAt(lin - st);
+    }
+    return;
+  }
   l[2 * D] = b.locate(c[0], c[1], c[2]);
   LSG_UNROLL
   for (int d = 0; d < D; ++d) {
@@ -168,18 +206,33 @@ template <int D> LSG_HD inline void gatherStar(const BandView &b, const int32_t 
     q[d] = b.g.map(d, (int64_t)c[d] - 1);
     l[D + d] = b.locate(q[0], q[1], q[2]);
   }
+}
+template <int D> LSG_HD inline void gatherStar(const BandView &b, const int32_t c[3], double &center, double *nb) {
+  BandView::Loc l[2 * D + 1];
+  starLocs<D>(b, c, l);
   double v[2 * D + 1];
   b.values<2 * D + 1>(l, v);
   LSG_UNROLL
   for (int k = 0; k < 2 * D; ++k) nb[k] = v[k];
   center = v[2 * D];
 }
-
 // Values along every axis of point c up to distance R (boundary mapped):
 // st[d][R + k] = phi(c + k e_d) for 0 < |k| <= R; st[d][R] is left to the
 // caller (the centre value).  All 2 D R lookups are issued together.
 template <int D, int R> LSG_HD inline void gatherAxes(const BandView &b, const int32_t c[3], double st[][2 * R + 1]) {
   BandView::Loc l[2 * D * R];
+  const int64_t lin = interiorLin<D>(b, c, R);
+  if (lin >= 0) {
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d) {
+      const int64_t sd = axisStride<D>(b, d);
+      LSG_UNROLL
+      for (int k = 1; k <= R; ++k) {
+        l[(d * R + k - 1) * 2] = locAt(lin + k * sd);
+        l[(d * R + k - 1) * 2 + 1] = locAt(lin - k * sd);
+      }
+    }
+  } else {
   LSG_UNROLL
   for (int d = 0; d < D; ++d) {
     LSG_UNROLL
@@ -191,6 +244,7 @@ template <int D, int R> LSG_HD inline void gatherAxes(const BandView &b, const i
       q[d] = b.g.map(d, (int64_t)c[d] - k);
       l[(d * R + k - 1) * 2 + 1] = b.locate(q[0], q[1], q[2]);
     }
+  }
   }
   double v[2 * D * R];
   b.values<2 * D * R>(l, v);

@@ -30,9 +30,15 @@ int defaultScanBlocks() {
   return v;
 }
 static int g_scanBlocks = defaultScanBlocks();
-// Grid of the rate kernel: 4 workgroups per CU, each takes a contiguous
-// 1/1024 of the active list.
-static constexpr int RATE_BLOCKS = 1024;
+// Grid of the rate kernel (4 workgroups per CU); LSG_RATE_BLOCKS from the
+// environment overrides it (grid sweeps).
+static int rateBlocks() {
+  static const int v = [] {
+    const char *e = getenv("LSG_RATE_BLOCKS");
+    return e ? clampBlocks(atoi(e)) : 1024;
+  }();
+  return v;
+}
 void setScanBlocks(int b) { g_scanBlocks = clampBlocks(b); }
 int scanBlocks() { return g_scanBlocks; }
 static int gridFor(size_t n, int cap = 4096) {
@@ -111,9 +117,12 @@ __device__ inline uint32_t logicalBlock() {
 // Contiguous chunk [lo, hi) of the (logical) block for a device-side size n;
 // chunks are multiples of `align` items (the packed keep flags of the
 // compaction are written one 64-bit word per wave).
+__device__ inline uint32_t chunkSize(uint32_t n, uint32_t align) {
+  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+  return (per + align - 1) / align * align;
+}
 __device__ inline void blockChunk(uint32_t n, uint32_t &lo, uint32_t &hi, uint32_t align = 64) {
-  uint32_t per = (n + gridDim.x - 1) / gridDim.x;
-  per = (per + align - 1) / align * align;
+  const uint32_t per = chunkSize(n, align);
   lo = logicalBlock() * per;
   hi = lo + per;
   if (lo > n) lo = n;
@@ -129,8 +138,9 @@ __device__ inline uint32_t blockOffset(const uint32_t *partials) {
 }
 
 // Value of the previous item in thread order within a tile: lane-1 of the
-// wave, the last lane of the previous wave, or `carry` for thread 0.
-__device__ inline double prevInTile(double last, double carry) {
+// wave, the last lane of the previous wave, or `carry` for thread 0.  With
+// tileLast, every thread also receives the tile's last item.
+__device__ inline double prevInTile(double last, double carry, double *tileLast = nullptr) {
   __shared__ double wl[TPB / 64];
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
@@ -138,6 +148,7 @@ __device__ inline double prevInTile(double last, double carry) {
   if (lane == 63) wl[wid] = last;
   __syncthreads();
   double p = (lane > 0) ? up : (wid > 0 ? wl[wid - 1] : carry);
+  if (tileLast) *tileLast = wl[TPB / 64 - 1];
   __syncthreads();
   return p;
 }
@@ -182,65 +193,60 @@ template <bool SEG>
 __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t nw, int base, const uint32_t *partials,
                                                  uint32_t *cpre, uint32_t *cpos, size_t ccap, uint32_t *ncand,
                                                  DevScalars *sc, SegArgs sa, uint16_t *cseg) {
-  if (sc->overflow) return;
-  __shared__ uint64_t sWord[TPB];
-  __shared__ uint32_t sPre[TPB];
-  __shared__ uint32_t sShift[SEG ? TPB : 1];
-  __shared__ uint32_t sLocal[SEG ? TPB : 1];  // word index within its level set's bitmap
-  __shared__ uint16_t sSeg[SEG ? TPB : 1];
   uint32_t lo, hi;
   blockChunk(nw, lo, hi);
+  // a tile's words are loaded one tile ahead; the first tile's loads are
+  // issued before the block offset is summed
+  uint64_t cNext = (lo + threadIdx.x < hi) ? cbits[lo + threadIdx.x] : 0ull;
   uint32_t off = blockOffset(partials);
+  if (sc->overflow) return;  // uniform: after the block-wide sum
   if (!SEG && logicalBlock() == 0 && threadIdx.x == 0 && base) cpos[0] = POS_NONE;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (uint32_t t = lo; t < hi; t += TPB) {
     const uint32_t w = t + threadIdx.x;
-    const uint64_t c = (w < hi) ? cbits[w] : 0ull;
+    const uint64_t c = cNext;
+    if (t + TPB < hi) cNext = (w + TPB < hi) ? cbits[w + TPB] : 0ull;
     const uint32_t pc = (uint32_t)__builtin_popcountll(c);
     uint32_t tot;
     const uint32_t pl = blockExclusiveScan(pc, tot);
     if (w < hi) cpre[w] = off + pl;
-    sWord[threadIdx.x] = c;
-    sPre[threadIdx.x] = pl;
-    if (SEG) {
-      uint32_t sh = 0, lw = 0;
-      uint16_t k = 0;
-      if (w < hi) {
-        k = sa.wseg[w];
-        const SegInfo &S = sa.segs[k];
-        sh = S.baseIncl;
-        lw = w - S.wordOff;
-        if (w == S.wordOff && S.f.base) {  // the level set's grid-minimum candidate precedes its first word
-          const size_t q = (size_t)off + pl + sh - 1;
-          if (q < ccap) {
-            cpos[q] = POS_NONE;
-            cseg[q] = k;
-          }
-        }
-      }
-      sShift[threadIdx.x] = sh;
-      sLocal[threadIdx.x] = lw;
-      sSeg[threadIdx.x] = k;
-    }
-    __syncthreads();
-    // each wave writes the positions of its 64 words, one word per
-    // iteration with lane = bit (consecutive lanes write consecutive slots)
-    const size_t q0 = SEG ? (size_t)off : (size_t)base + off;
-    const int lane = threadIdx.x & 63;
-    const int wbase = threadIdx.x & ~63;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll 4
-    for (int a = wbase; a < wbase + 64; ++a) {
-      const uint64_t cw = sWord[a];
-      if ((cw >> lane) & 1ull) {
-        const size_t q = q0 + sPre[a] + (uint32_t)__builtin_popcountll(cw & below) + (SEG ? sShift[a] : 0u);
+    uint32_t sh = 0, lw = 0, k = 0;
+    if (SEG && w < hi) {
+      k = sa.wseg[w];
+      const SegInfo &S = sa.segs[k];
+      sh = S.baseIncl;
+      lw = w - S.wordOff;  // positions are level-set local
+      if (w == S.wordOff && S.f.base) {  // the level set's grid-minimum candidate precedes its first word
+        const size_t q = (size_t)off + pl + sh - 1;
         if (q < ccap) {
-          const uint32_t word = SEG ? sLocal[a] : t + (uint32_t)a;  // positions are level-set local
-          cpos[q] = (word << 6) | (uint32_t)lane;
-          if (SEG) cseg[q] = sSeg[a];
+          cpos[q] = POS_NONE;
+          cseg[q] = (uint16_t)k;
         }
       }
     }
-    __syncthreads();
+    // each wave writes the positions of its own non-empty words (read from
+    // the owning lanes), one word per iteration with lane = bit, so
+    // consecutive lanes write consecutive slots
+    const size_t q0 = SEG ? (size_t)off : (size_t)base + off;
+    uint64_t todo = (uint64_t)__ballot(c != 0ull);
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1ull;
+      const uint64_t cw = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j) << 32) |
+                          (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
+      const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)pl, j);
+      const uint32_t word = SEG ? (uint32_t)__builtin_amdgcn_readlane((int)lw, j)
+                                : t + (uint32_t)(threadIdx.x & ~63) + (uint32_t)j;
+      const uint32_t shj = SEG ? (uint32_t)__builtin_amdgcn_readlane((int)sh, j) : 0u;
+      if ((cw >> lane) & 1ull) {
+        const size_t q = q0 + pj + (uint32_t)__builtin_popcountll(cw & below) + shj;
+        if (q < ccap) {
+          cpos[q] = (word << 6) | (uint32_t)lane;
+          if (SEG) cseg[q] = (uint16_t)__builtin_amdgcn_readlane((int)k, j);
+        }
+      }
+    }
     off += tot;
   }
   if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) {
@@ -349,31 +355,27 @@ template <bool SEG> __device__ inline bool firstOfSet(uint32_t i, const uint16_t
 }
 
 // Phase 1 of compaction: evaluate the rule at every candidate (out[i]) and
-// count the kept candidates per block.
+// count the kept candidates per block.  A block's first keep flag needs the
+// value of the candidate before its chunk: the block evaluates its chunk
+// from that candidate on (evaluated again, not stored), so no thread waits
+// on a separate evaluation of it.
 template <class Ev, bool SEG>
 __global__ __launch_bounds__(256) void kEvalKeep(Ev ev, const uint32_t *nptr, double *out, uint32_t *partials,
                                                  DevScalars *sc, const uint16_t *iseg) {
-  if (sc->overflow) return;
   const uint32_t n = *nptr;
+  if (sc->overflow) return;
   uint32_t lo, hi;
   blockChunk(n, lo, hi);
-  __shared__ double carry;
-  if (threadIdx.x == 0) carry = (lo > 0 && lo < hi) ? ev(lo - 1) : 0.0;
-  __syncthreads();
+  const uint32_t from = (lo > 0 && lo < hi) ? lo - 1 : lo;
+  double carry = 0.0;  // last value of the previous tile
   uint32_t acc = 0;
-  for (uint32_t t = lo; t < hi; t += TPB) {
+  for (uint32_t t = from; t < hi; t += TPB) {
     const uint32_t i = t + threadIdx.x;
-    const bool valid = i < hi;
-    double v = 0.0;
-    if (valid) {
-      v = ev(i);
-      out[i] = v;
-    }
-    const double prev = prevInTile(v, carry);
-    if (valid) acc += keepOf(firstOfSet<SEG>(i, iseg), v, prev);
-    __syncthreads();
-    if (threadIdx.x == TPB - 1) carry = v;
-    __syncthreads();
+    const bool ok = i >= lo && i < hi;
+    const double v = ev(i < hi ? i : hi - 1);  // past the chunk: evaluated, not stored
+    if (ok) out[i] = v;
+    const double prev = prevInTile(v, carry, &carry);
+    if (ok) acc += keepOf(firstOfSet<SEG>(i, iseg), v, prev);
   }
   const uint32_t tot = blockSum(acc);
   if (threadIdx.x == 0) partials[logicalBlock()] = tot;
@@ -387,23 +389,36 @@ __global__ __launch_bounds__(256) void kKeepScatter(const double *out, const uin
                                                     const uint32_t *partials, uint32_t *ks, uint64_t *keepBits,
                                                     double *val, uint32_t *pos, uint32_t *nOut, size_t cap,
                                                     DevScalars *sc, const uint16_t *iseg, uint16_t *oseg) {
-  if (sc->overflow) return;
   const uint32_t n = *nptr;
+  if (sc->overflow) return;
   uint32_t lo, hi;
   blockChunk(n, lo, hi);
-  uint32_t off = blockOffset(partials);
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
+  // a tile's values are loaded one tile ahead; the first tile's loads are
+  // issued before the block offset is summed
+  double vNext = 0.0, pNext = 0.0;
+  {
+    const uint32_t i = lo + threadIdx.x;
+    if (i < hi) {
+      vNext = out[i];
+      pNext = (i > 0) ? out[i - 1] : 0.0;
+    }
+  }
+  uint32_t off = blockOffset(partials);
   for (uint32_t t = lo; t < hi; t += TPB) {
     const uint32_t i = t + threadIdx.x;
     const bool valid = i < hi;
-    double v = 0.0;
-    uint32_t k = 0;
-    if (valid) {
-      v = out[i];
-      const double prev = (i > 0) ? out[i - 1] : 0.0;
-      k = keepOf(firstOfSet<SEG>(i, iseg), v, prev);
+    const double v = vNext, prev = pNext;
+    if (t + TPB < hi) {
+      const uint32_t j = i + TPB;
+      if (j < hi) {
+        vNext = out[j];
+        pNext = out[j - 1];
+      }
     }
+    uint32_t k = 0;
+    if (valid) k = keepOf(firstOfSet<SEG>(i, iseg), v, prev);
     uint32_t tot;
     uint64_t wm;
     const uint32_t p = off + blockBallotScan(k != 0, tot, wm);
@@ -810,8 +825,8 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
 template <int D, int SCHEME, bool SEG>
 __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, const uint32_t *nact, RateParams rp,
                                               double *rates, DevScalars *sc, SegArgs sa, unsigned long long *segMax) {
-  LSG_ABORT_IF_OVERFLOW(sc);
   const uint32_t n = *nact;
+  LSG_ABORT_IF_OVERFLOW(sc);
   uint32_t lo, hi;
   blockChunk(n, lo, hi, 1);
   const Vel vel = velOf(rp);
@@ -834,8 +849,8 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
     // the XCDs and CUs), so that few active points still fill whole waves
     // spread over the whole chip
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t waves = gridDim.x * (TPB / 64);
-    for (uint32_t base = (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * 64u; base < n; base += waves * 64u) {
+    const uint32_t wave = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6), waves = gridDim.x * (TPB / 64);
+    for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
       const uint32_t j = base + lane;
       if (j < n) fn(list[j], j);
     }
@@ -903,7 +918,7 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
   if (!rp.hostVel) activeList(s, b, ws, sc, 1, scheme == S_LF ? 1 : 0);  // resets the rate maxima too
   else kResetRates<<<1, 1, 0, s>>>(sc, scheme == S_LF ? 1 : 0);  // the list of launchVelInputs is still in ws
   BandView v = viewOf(b, f, g);
-  const int ge = RATE_BLOCKS;
+  const int ge = rateBlocks();
   recordEvent(s, e0, externalEvents);
 #define LSG_RATES(DD, SS) \
   kRates<DD, SS, false><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, rp, rates, sc, noSeg(), nullptr)
@@ -941,8 +956,8 @@ void launchVelInputs(hipStream_t s, int D, const BandBufs &b, const Frame &f, co
   kDefinedCount<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials);
   kDefinedList<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, dlist, ndef);
   BandView v = viewOf(b, f, g);
-  if (D == 2) kVelInputs<2><<<RATE_BLOCKS, TPB, 0, s>>>(v, ws.cpos, ws.ncand, dlist, ndef, rp, out);
-  else kVelInputs<3><<<RATE_BLOCKS, TPB, 0, s>>>(v, ws.cpos, ws.ncand, dlist, ndef, rp, out);
+  if (D == 2) kVelInputs<2><<<rateBlocks(), TPB, 0, s>>>(v, ws.cpos, ws.ncand, dlist, ndef, rp, out);
+  else kVelInputs<3><<<rateBlocks(), TPB, 0, s>>>(v, ws.cpos, ws.ncand, dlist, ndef, rp, out);
 }
 
 // Batch rate pass: every level set's stage velocities and max |H - D| per
@@ -976,7 +991,7 @@ void launchRatesSeg(hipStream_t s, int D, const BandBufs &b, const SegCtx &c, co
   GridInfo g{};
   BandView v = viewOf(b, f, g);
   SegArgs sa = segArgs(c, b.seg, nullptr);
-  const int ge = RATE_BLOCKS;
+  const int ge = rateBlocks();
 #define LSG_RATES_SEG(DD, SS) kRates<DD, SS, true><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, rp, rates, sc, sa, segMax)
   if (D == 2) {
     switch (scheme) {
