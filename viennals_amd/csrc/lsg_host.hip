@@ -110,6 +110,8 @@ struct WsStore {
   }
   int alloc(int64_t nw, size_t ccap) {
     release();
+    // the candidate list kernel addresses its slots with 32-bit byte offsets
+    if (ccap >= ((size_t)1 << 30)) return fail(-12, "band capacity exceeds 2^30 entries");
     LSG_HIP(hipMalloc(&w.cbits, (nw + 1) * sizeof(uint64_t)));
     LSG_HIP(hipMalloc(&w.cnt, (nw + 1) * sizeof(uint32_t)));
     LSG_HIP(hipMalloc(&w.cpre, (nw + 1) * sizeof(uint32_t)));

@@ -227,23 +227,30 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t
     }
     // each wave writes the positions of its own non-empty words (read from
     // the owning lanes), one word per iteration with lane = bit, so
-    // consecutive lanes write consecutive slots
-    const size_t q0 = SEG ? (size_t)off : (size_t)base + off;
+    // consecutive lanes write consecutive slots.  Slot arithmetic is 32-bit:
+    // candidates are distinct frame bits (< 2^32) and the candidate capacity
+    // is below 2^30, so q < ccap also bounds the byte offset.
+    const uint32_t q0 = (SEG ? 0u : (uint32_t)base) + off;
+    const uint32_t cap32 = (uint32_t)ccap;
+    const uint32_t lsh = (uint32_t)lane & 31u;
+    const uint32_t belowLo = lane < 32 ? ((1u << lsh) - 1u) : 0xFFFFFFFFu;
+    const uint32_t belowHi = lane < 32 ? 0u : ((1u << lsh) - 1u);
     uint64_t todo = (uint64_t)__ballot(c != 0ull);
     while (todo) {
       const int j = __builtin_ctzll(todo);
       todo &= todo - 1ull;
-      const uint64_t cw = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j) << 32) |
-                          (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
+      const uint32_t cwLo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
+      const uint32_t cwHi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j);
       const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)pl, j);
       const uint32_t word = SEG ? (uint32_t)__builtin_amdgcn_readlane((int)lw, j)
                                 : t + (uint32_t)(threadIdx.x & ~63) + (uint32_t)j;
       const uint32_t shj = SEG ? (uint32_t)__builtin_amdgcn_readlane((int)sh, j) : 0u;
-      if ((cw >> lane) & 1ull) {
-        const size_t q = q0 + pj + (uint32_t)__builtin_popcountll(cw & below) + shj;
-        if (q < ccap) {
-          cpos[q] = (word << 6) | (uint32_t)lane;
-          if (SEG) cseg[q] = (uint16_t)__builtin_amdgcn_readlane((int)k, j);
+      if ((((lane < 32) ? cwLo : cwHi) >> lsh) & 1u) {
+        const uint32_t q = q0 + pj + (uint32_t)__builtin_popcount(cwLo & belowLo) +
+                           (uint32_t)__builtin_popcount(cwHi & belowHi) + shj;
+        if (q < cap32) {
+          *(uint32_t *)((char *)cpos + (q << 2)) = (word << 6) | (uint32_t)lane;
+          if (SEG) *(uint16_t *)((char *)cseg + (q << 1)) = (uint16_t)__builtin_amdgcn_readlane((int)k, j);
         }
       }
     }
@@ -845,14 +852,20 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
   if (SEG) {
     for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) fn(list[j], j);
   } else {
-    // 64-point chunks dealt to the waves in launch order (round-robin over
-    // the XCDs and CUs), so that few active points still fill whole waves
-    // spread over the whole chip
+    // Each XCD takes a contiguous slab of the active list in proportion to
+    // its blocks (the stencil reads of neighbouring points then meet in that
+    // XCD's L2), dealt in 64-point chunks to the XCD's waves in launch order
+    // so that few active points still fill whole waves on every CU.
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6), waves = gridDim.x * (TPB / 64);
-    for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
+    const uint32_t xcd = blockIdx.x % NUM_XCD, k = blockIdx.x / NUM_XCD;
+    const uint32_t per = gridDim.x / NUM_XCD, rem = gridDim.x % NUM_XCD;
+    const uint32_t nbx = per + (xcd < rem ? 1u : 0u), b0 = xcd * per + (xcd < rem ? xcd : rem);
+    const uint32_t first = (uint32_t)((uint64_t)n * b0 / gridDim.x);
+    const uint32_t last = (uint32_t)((uint64_t)n * (b0 + nbx) / gridDim.x);
+    const uint32_t wave = k * (TPB / 64) + (threadIdx.x >> 6), waves = nbx * (TPB / 64);
+    for (uint32_t base = first + wave * 64u; base < last; base += waves * 64u) {
       const uint32_t j = base + lane;
-      if (j < n) fn(list[j], j);
+      if (j < last) fn(list[j], j);
     }
   }
   if (!SEG) {
