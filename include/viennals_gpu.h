@@ -155,6 +155,11 @@ int lsg_advect_get_number_of_time_steps(lsg_advect *a, unsigned *n); /* :926 */
 int lsg_advect_get_point_updates(lsg_advect *a, double *n);
 int lsg_advect_set_profiling(lsg_advect *a, int on);
 int lsg_advect_get_rate_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes);
+/* No counterpart in the reference: the same figures for the evaluation
+ * kernel of the star operations (Expand cycles and rebuilds), the kernel
+ * that takes most of a step; algorithmic bytes per launch are 8 per input
+ * band entry plus 8 per candidate. */
+int lsg_advect_get_star_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes);
 
 /* Finite-difference kernels FiniteDifferences::weno3/weno5
  * (lsFiniteDifferences.hpp:58-123) evaluated on the GPU for n stencils:
@@ -205,8 +210,9 @@ int lsg_batch_apply(lsg_batch *b);
 int lsg_batch_get_advected_times(lsg_batch *b, double *t);
 int lsg_batch_get_point_updates(lsg_batch *b, double *n, unsigned *steps);
 int lsg_batch_write_back(lsg_batch *b);
-/* Grid of the scan kernels (blocks), or chosen by timing one step per
- * candidate grid from the same band (the band is restored). */
+/* Grid of the scan kernels (blocks), or chosen by timing a step per
+ * candidate grid (the second of two from the same band; the band is
+ * restored afterwards). */
 int lsg_batch_set_scan_blocks(lsg_batch *b, int blocks);
 int lsg_batch_autotune(lsg_batch *b, int *chosen, double *ms);
 

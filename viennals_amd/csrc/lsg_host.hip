@@ -387,8 +387,27 @@ static void runsFromPoints(const lsg_domain *d, std::vector<int32_t> pidx, std::
 
 static void swapCur(lsg_domain *d) { std::swap(d->cur, d->tmp); }
 
+// Event pairs around the evaluation kernel of star operations, taken from a
+// pool while an Advect with profiling on enqueues its step.
+struct StarTiming {
+  hipEvent_t *ev = nullptr;
+  int cap = 0, used = 0;
+  bool external = false;
+};
+static StarTiming *g_starTiming = nullptr;
+
 static void opStar(lsg_domain *d, const OpParams &op) {
-  launchStarOp(g_ctx.s, d->D, d->cur.b, d->tmp.b, d->f, d->g, g_ws->w, op, g_ctx.sc);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool external = false;
+  if (StarTiming *t = g_starTiming) {
+    if (t->used + 2 <= t->cap) {
+      e0 = t->ev[t->used];
+      e1 = t->ev[t->used + 1];
+      t->used += 2;
+      external = t->external;
+    }
+  }
+  launchStarOp(g_ctx.s, d->D, d->cur.b, d->tmp.b, d->f, d->g, g_ws->w, op, g_ctx.sc, e0, e1, external);
   swapCur(d);
 }
 static void opSelf(lsg_domain *d, const OpParams &op) {
@@ -539,12 +558,17 @@ struct lsg_advect {
   std::vector<hipEvent_t> evPool;
   int evUsed = 0;
   bool capturing = false;
+  // star-operation evaluation kernel (profiling)
+  std::vector<hipEvent_t> starPool;
+  lsg::StarTiming starT;
+  double starMs = 0, starBytes = 0;
+  int64_t starLaunches = 0;
   // one captured graph per buffer parity of the step
   struct StepGraph {
     uint64_t key = 0;
     hipGraphExec_t exec = nullptr;
     lsg::Store dCur, dTmp, iCur, iTmp;
-    int dWidth = 0, iWidth = 0, evUsed = 0;
+    int dWidth = 0, iWidth = 0, evUsed = 0, starUsed = 0;
   };
   StepGraph graphs[2];
   int nextGraph = 0;
@@ -575,6 +599,7 @@ struct lsg_advect {
     if (save) delete save;
     if (rates) (void)hipFree(rates);
     for (auto e : evPool) (void)hipEventDestroy(e);
+    for (auto e : starPool) (void)hipEventDestroy(e);
     if (hostMaxDt) (void)hipHostFree(hostMaxDt);
     if (hostSc) (void)hipHostFree(hostSc);
     if (hostStat) (void)hipHostFree(hostStat);
@@ -783,7 +808,10 @@ static void enqueueStepBody(lsg_advect *a, lsg_domain *d) {
   launchStepBegin(g_ctx.s, g_ctx.sc, a->devMaxDt);
   copyDomainNoAlloc(a->save, d);
   a->evUsed = 0;
+  a->starT = StarTiming{a->starPool.data(), (int)a->starPool.size(), 0, a->capturing};
+  if (a->profiling) g_starTiming = &a->starT;
   enqueueStep(a, d);
+  g_starTiming = nullptr;
   launchFrameCheck(g_ctx.s, d->cur.b, d->f, d->g, d->D, REFRAME_SLACK, g_ctx.sc);
 }
 
@@ -840,6 +868,7 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
         }
         g->dWidth = d->width;
         g->evUsed = a->evUsed;
+        g->starUsed = a->starT.used;
       } else {
         // replay: the host bookkeeping takes the state the captured step leaves
         d->cur = g->dCur;
@@ -853,6 +882,7 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
           a->init->framed = true;
         }
         a->evUsed = g->evUsed;
+        a->starT.used = g->starUsed;
       }
       LSG_HIP(hipGraphLaunch(g->exec, g_ctx.s));
       launched = true;
@@ -872,6 +902,13 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
           if (hipEventElapsedTime(&ms, a->evPool[k], a->evPool[k + 1]) == hipSuccess) {
             a->rateMs += ms;
             a->rateLaunches += 1;
+          }
+        }
+        for (int k = 0; k + 1 < a->starT.used; k += 2) {
+          float ms = 0;
+          if (hipEventElapsedTime(&ms, a->starPool[k], a->starPool[k + 1]) == hipSuccess) {
+            a->starMs += ms;
+            a->starLaunches += 1;
           }
         }
       }
@@ -938,6 +975,8 @@ static int advectApply(lsg_advect *a) {
   if (a->profiling && a->evPool.empty()) {
     a->evPool.resize(8);
     for (auto &e : a->evPool) LSG_HIP(hipEventCreate(&e));
+    a->starPool.resize(96);
+    for (auto &e : a->starPool) LSG_HIP(hipEventCreate(&e));
   }
   if (const char *ng = getenv("LSG_NO_GRAPH")) a->useGraphs = (ng[0] == '0');
   a->pointUpdates = 0;
@@ -962,7 +1001,10 @@ static int advectApply(lsg_advect *a) {
   }
   a->currentTimeStep = so.dt1;
   if (a->temporal == 0) a->currentTimeStep = so.stepTime;
-  if (a->profiling) a->rateBytes = a->hostSc->rateBytes;
+  if (a->profiling) {
+    a->rateBytes = a->hostSc->rateBytes;
+    a->starBytes = a->hostSc->starBytes;
+  }
   return 0;
 }
 
@@ -1723,16 +1765,26 @@ int lsg_advect_set_profiling(lsg_advect *a, int on) {
   if (int e = ctxInit()) return e;
   double z = 0;
   LSG_HIP(hipMemcpy(&g_ctx.sc->rateBytes, &z, sizeof(double), hipMemcpyHostToDevice));
+  LSG_HIP(hipMemcpy(&g_ctx.sc->starBytes, &z, sizeof(double), hipMemcpyHostToDevice));
   a->profiling = on != 0;
   a->rateMs = 0;
   a->rateLaunches = 0;
   a->rateBytes = 0;
+  a->starMs = 0;
+  a->starLaunches = 0;
+  a->starBytes = 0;
   return 0;
 }
 int lsg_advect_get_rate_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes) {
   *ms = a->rateMs;
   *launches = a->rateLaunches;
   *bytes = a->rateBytes;
+  return 0;
+}
+int lsg_advect_get_star_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes) {
+  *ms = a->starMs;
+  *launches = a->starLaunches;
+  *bytes = a->starBytes;
   return 0;
 }
 
@@ -2003,6 +2055,9 @@ int lsg_batch_autotune(lsg_batch *b, int *chosen, double *ms) {
   for (int c : cand) {
     if ((err = restore())) break;
     b->scanBlocks = c;
+    // two steps from the restored state; the second one is timed (the
+    // first also touches freshly restored memory and grows buffers)
+    if ((err = batchStep(b))) break;
     LSG_HIP(hipEventRecord(g_ctx.ev0, g_ctx.s));
     if ((err = batchStep(b))) break;
     LSG_HIP(hipEventRecord(g_ctx.ev1, g_ctx.s));

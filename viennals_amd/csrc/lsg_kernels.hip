@@ -270,10 +270,13 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t
 // ------------------------------------------------- evaluators of rules ----
 
 template <int D, bool SEG> struct EvalStar {
+  static constexpr bool kStar = !SEG;  // single band: algorithmic bytes are counted
   BandView in;
   SegArgs sa;
   const uint32_t *cpos;
   OpParams op;
+  // entries of the input band (in-frame ones plus the grid-minimum entry)
+  __device__ uint32_t inputEntries() const { return in.pre[in.f.nwords] + in.rb; }
   __device__ double operator()(uint32_t i) const {
     const BandView v = SEG ? segView(in, sa.segs[sa.iseg[i]]) : in;
     int32_t c[3];
@@ -285,6 +288,8 @@ template <int D, bool SEG> struct EvalStar {
 };
 
 template <int D, bool SEG> struct EvalSelf {
+  static constexpr bool kStar = false;
+  __device__ uint32_t inputEntries() const { return 0; }
   BandView in;
   SegArgs sa;
   OpParams op;
@@ -303,6 +308,8 @@ template <int D, bool SEG> struct EvalSelf {
 };
 
 template <bool SEG> struct EvalBool {
+  static constexpr bool kStar = false;
+  __device__ uint32_t inputEntries() const { return 0; }
   BandView A, B;
   SegArgs sa;
   const uint32_t *cpos;
@@ -386,6 +393,11 @@ __global__ __launch_bounds__(256) void kEvalKeep(Ev ev, const uint32_t *nptr, do
   }
   const uint32_t tot = blockSum(acc);
   if (threadIdx.x == 0) partials[logicalBlock()] = tot;
+  // algorithmic bytes of a star operation's evaluation: every input entry
+  // read once (the star supports lie in the band), one value written per
+  // candidate
+  if constexpr (Ev::kStar)
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->starBytes += 8.0 * ((double)ev.inputEntries() + (double)n);
 }
 
 // Phase 2: rescan the keep flags from the block offset, scatter the kept
@@ -496,11 +508,20 @@ static SegArgs segArgs(const SegCtx &c, const uint16_t *iseg, uint16_t *oseg) {
   return SegArgs{c.segs, c.nseg, c.totalBase, c.wseg, iseg, oseg};
 }
 
+static void recordEvent(hipStream_t s, hipEvent_t e, bool external) {
+  if (!e) return;
+  if (external) (void)hipEventRecordWithFlags(e, s, hipEventRecordExternal);
+  else (void)hipEventRecord(e, s);
+}
+
 template <class Ev, bool SEG>
 static void compactInto(hipStream_t s, const Ev &ev, const Workspace &ws, const uint32_t *ncandPtr,
                         const uint32_t *cpos, const uint32_t *cpre, const uint64_t *cwords, uint32_t nw, int base,
-                        BandBufs &out, DevScalars *sc, const SegArgs &sa) {
+                        BandBufs &out, DevScalars *sc, const SegArgs &sa, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr, bool externalEvents = false) {
+  recordEvent(s, e0, externalEvents);
   kEvalKeep<Ev, SEG><<<g_scanBlocks, TPB, 0, s>>>(ev, ncandPtr, ws.out, ws.partials, sc, sa.iseg);
+  recordEvent(s, e1, externalEvents);
   kKeepScatter<SEG><<<g_scanBlocks, TPB, 0, s>>>(ws.out, cpos, ncandPtr, ws.partials, ws.ks, ws.keepBits, out.val,
                                                  out.pos, out.n, out.cap, sc, sa.iseg, out.seg);
   kBuildIndex<SEG><<<gridFor((size_t)nw + 1), TPB, 0, s>>>(cwords, cpre, ws.ks, ws.keepBits, nw, base, ncandPtr,
@@ -517,13 +538,15 @@ static void candidates(hipStream_t s, const uint64_t *A, const uint64_t *B, cons
 
 template <int D, bool SEG>
 static void starOp(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g, uint32_t nw,
-                   Workspace &ws, const OpParams &op, DevScalars *sc, const SegArgs &sa0) {
+                   Workspace &ws, const OpParams &op, DevScalars *sc, const SegArgs &sa0, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr, bool externalEvents = false) {
   candidates<D, 0, SEG>(s, in.bits, nullptr, f, g, nw, ws, sc, sa0);
   SegArgs sa = sa0;
   sa.iseg = ws.cseg;
   sa.oseg = out.seg;
   EvalStar<D, SEG> ev{viewOf(in, f, g), sa, ws.cpos, op};
-  compactInto<EvalStar<D, SEG>, SEG>(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, nw, f.base, out, sc, sa);
+  compactInto<EvalStar<D, SEG>, SEG>(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, nw, f.base, out, sc, sa, e0, e1,
+                                     externalEvents);
 }
 
 template <int D, bool SEG>
@@ -549,10 +572,11 @@ static void boolOp(hipStream_t s, const BandBufs &A, const BandBufs &B, BandBufs
 }
 
 void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
-                  Workspace &ws, const OpParams &op, DevScalars *sc) {
+                  Workspace &ws, const OpParams &op, DevScalars *sc, hipEvent_t e0, hipEvent_t e1,
+                  bool externalEvents) {
   const uint32_t nw = (uint32_t)f.nwords;
-  if (D == 2) starOp<2, false>(s, in, out, f, g, nw, ws, op, sc, noSeg());
-  else starOp<3, false>(s, in, out, f, g, nw, ws, op, sc, noSeg());
+  if (D == 2) starOp<2, false>(s, in, out, f, g, nw, ws, op, sc, noSeg(), e0, e1, externalEvents);
+  else starOp<3, false>(s, in, out, f, g, nw, ws, op, sc, noSeg(), e0, e1, externalEvents);
 }
 
 void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
@@ -916,12 +940,6 @@ static void activeList(hipStream_t s, const BandBufs &b, Workspace &ws, DevScala
                        int keepAlphas = 0) {
   kActiveCount<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, sc, resetRates, keepAlphas);
   kActiveList<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, ws.cpos, ws.ncand, sc);
-}
-
-static void recordEvent(hipStream_t s, hipEvent_t e, bool external) {
-  if (!e) return;
-  if (external) (void)hipEventRecordWithFlags(e, s, hipEventRecordExternal);
-  else (void)hipEventRecord(e, s);
 }
 
 void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,

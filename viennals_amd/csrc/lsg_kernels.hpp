@@ -26,6 +26,7 @@ struct DevScalars {
   double stepTime;    // time advanced by the last step
   double maxTimeStep; // bound passed to the first stage
   double rateBytes;   // accumulated algorithmic bytes of the rate kernel
+  double starBytes;   // accumulated algorithmic bytes of the star-operation evaluation (single band)
   unsigned long long stepActive0;  // totalActive at the start of the step
 };
 
@@ -87,8 +88,11 @@ void launchBooleanSeg(hipStream_t s, int D, const BandBufs &A, const BandBufs &B
                       Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc);
 
 // Band operations (all asynchronous on `s`; sizes live in device memory).
+// Star operations: e0/e1 (optional) are recorded around the evaluation
+// kernel (kEvalKeep<EvalStar>), external when the stream is being captured.
 void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
-                  Workspace &ws, const OpParams &op, DevScalars *sc);
+                  Workspace &ws, const OpParams &op, DevScalars *sc, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr, bool externalEvents = false);
 void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc);
 void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
