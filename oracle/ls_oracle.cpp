@@ -912,6 +912,196 @@ template <int D> struct Advector {
     for (int i = 0; i < D; ++i) alphas[i] = acc[i];
   }
 
+  // ---- multi-material advection (lsAdvect.hpp:435-612, 686-813) ----
+  // lower: the level sets below the top, bottom first (levelSets[0 .. L-2]);
+  // empty for one level set.  Material m of a point is the first level set
+  // (bottom up, the top included) whose value there is <= the point's
+  // value + 1e-4 (:513-527); its velocity functor is P.velFor(m).
+  static constexpr double WRAP_EPS = 1e-4;  // wrappingLayerEpsilon (:80)
+  std::vector<Band<D> *> lower;
+  std::vector<double> mmV, mmS;  // per expanded-band entry, L slots: velocity (after checkDissipation), stop value
+  int nMat() const { return (int)lower.size() + 1; }
+
+  void scheme(SchemeCtx<D> s, int material, const Idx<D> &c, int order, double &H, double &Dn, double *alphaAcc) {
+    s.vel = &P.velFor(material);
+    switch (P.spatial) {
+    case EO1: case EO2: schemeEO<D>(s, c, order, H, Dn); break;
+    case LF1: case LF2: schemeLF<D>(s, c, order, H, Dn); break;
+    case LLLF1: case LLLF2: schemeLLLF<D>(s, c, order, H, Dn, alphaAcc); break;
+    case LLF1: case LLF2: schemeLLF<D>(s, c, order, H, Dn, alphaAcc); break;
+    case WENO3: case WENO5: case WENO5_LEARNED: schemeWENO<D>(s, c, order, H, Dn); break;
+    }
+  }
+
+  // values of every level set at point c of the top (the top's is phiTop)
+  void levelValues(const Idx<D> &c, double phiTop, double *lv) const {
+    const int L = nMat();
+    for (int m = 0; m < L - 1; ++m) lv[m] = lower[m]->value(c);
+    lv[L - 1] = phiTop;
+  }
+
+  // Advect::findGlobalAlphas with the material of each point (lsAdvect.hpp:90-195)
+  void globalAlphasMM(const Band<D> &ls, double *alphas) {
+    const int L = nMat();
+    double acc[3] = {0, 0, 0};
+    for (size_t e = 0; e < ls.idx.size(); ++e) {
+      double v = ls.val[e];
+      if (isUndef(v) || std::fabs(v) > 0.5) continue;
+      const Idx<D> &c = ls.idx[e];
+      double lv[8];
+      levelValues(c, v, lv);
+      int m = 0;
+      for (; m < L; ++m)
+        if (lv[m] <= v + WRAP_EPS) break;
+      if (m == L) continue;
+      const Velocity &vel = P.velFor(m);
+      double normal[3] = {0, 0, 0};
+      double nm = 0.;
+      for (int i = 0; i < D; ++i) {
+        Idx<D> pp = c, pn = c;
+        pp[i] = ls.g.map(i, (int64_t)c[i] + 1);
+        pn[i] = ls.g.map(i, (int64_t)c[i] - 1);
+        normal[i] = ls.value(pp) - ls.value(pn);
+        nm += normal[i] * normal[i];
+      }
+      nm = 1. / std::sqrt(nm);
+      for (int i = 0; i < D; ++i) normal[i] *= nm;
+      double sv = vel.scalar(normal);
+      for (int i = 0; i < D; ++i) acc[i] = smax(acc[i], std::fabs((sv + vel.vec[i]) * normal[i]));
+    }
+    for (int i = 0; i < 3; ++i) alphas[i] = acc[i];
+  }
+
+  // integrateTime with level sets below the top (lsAdvect.hpp:496-593):
+  // per active point a list of (velocity, stop value) entries, one per
+  // material the point reaches within the step; dt = min over points of
+  // the summed step limits (run serially: the reference's LF-family dt
+  // depends on its thread count, see lsAdvect.hpp:595-606).
+  double computeRatesMM(Band<D> &ls, double maxTimeStep) {
+    int w = prepareWidth(P.spatial);
+    if (w < 0) { err = "Advect: Discretization scheme not found."; return -1.; }
+    expand(ls, w);
+    SchemeCtx<D> s;
+    s.ls = &ls;
+    s.spatial = P.spatial;
+    s.dx = ls.g.delta;
+    s.calcNormals = P.calcNormals != 0;
+    s.alphaFactor = P.dissipationAlpha;
+    s.vel = &P.vel;
+    s.globalAlphas[0] = s.globalAlphas[1] = s.globalAlphas[2] = 0;
+    if (P.spatial == LF1 || P.spatial == LF2) globalAlphasMM(ls, s.globalAlphas);
+    if (P.spatial == WENO3 || P.spatial == WENO5 || P.spatial == WENO5_LEARNED)
+      s.calcNormals = (P.dissipationAlpha != 0.0);
+    if (P.spatial == WENO5_LEARNED) s.mlp = P.mlp;
+    const int order = schemeOrder(P.spatial);
+    const int L = nMat();
+    const size_t n = ls.idx.size();
+    mmV.assign(n * L, 0.0);
+    mmS.assign(n * L, 0.0);
+    double tempMax = maxTimeStep;
+    double alphaAcc[3] = {0, 0, 0};
+    double active = 0;
+    const double MAXV = std::numeric_limits<double>::max();
+    for (size_t e = 0; e < n; ++e) {
+      double value = ls.val[e];
+      if (isUndef(value) || std::fabs(value) > 0.5) continue;  // :498-499
+      active += 1;
+      const Idx<D> &c = ls.idx[e];
+      double lv[8];
+      levelValues(c, value, lv);
+      double maxStepTime = 0;
+      double cfl = P.cfl;
+      int k = 0;
+      for (int cur = L - 1; cur >= 0; --cur) {  // :507-589
+        double H = 0, Dn = 0;
+        for (int m = 0; m < L; ++m)
+          if (lv[m] <= value + WRAP_EPS) {
+            scheme(s, m, c, order, H, Dn, alphaAcc);
+            break;
+          }
+        const double velocity = H - Dn;
+        double stop;
+        bool done = true;
+        if (velocity > 0.) {
+          maxStepTime += cfl / velocity;
+          stop = -MAXV;
+        } else if (velocity == 0.) {
+          maxStepTime = MAXV;
+          stop = MAXV;
+        } else {
+          const double valueBelow = (cur > 0) ? lv[cur - 1] : MAXV;
+          const double difference = std::fabs(valueBelow - value);
+          if (difference >= cfl) {
+            maxStepTime -= cfl / velocity;
+            stop = MAXV;
+          } else {  // terminal step onto the material below (adaptive sub-stepping off)
+            cfl -= difference;
+            value = valueBelow;
+            maxStepTime -= difference / velocity;
+            stop = valueBelow;
+            done = false;
+          }
+        }
+        double v = velocity;  // updateLevelSet's rule (:738-744, :754-757)
+        if ((P.checkDissipation && (H < 0 && v > 0)) || (H > 0 && v < 0)) v = 0;
+        mmV[e * L + k] = v;
+        mmS[e * L + k] = stop;
+        ++k;
+        if (done) break;
+      }
+      if (maxStepTime < tempMax) tempMax = maxStepTime;
+    }
+    pointUpdates += active;
+    const double *fa = nullptr;
+    if (P.spatial == LF1 || P.spatial == LF2) fa = s.globalAlphas;
+    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2) fa = alphaAcc;
+    if (fa) {
+      double timeStep = 0;
+      for (int i = 0; i < D; ++i) timeStep += fa[i] / ls.g.delta;
+      timeStep = 1.0 / timeStep;
+      tempMax = smin(timeStep, tempMax);
+    }
+    if (tempMax < maxTimeStep) maxTimeStep = tempMax;
+    return maxTimeStep;
+  }
+
+  // updateLevelSet with material changes within the step (lsAdvect.hpp:725-771)
+  void updateLevelSetMM(Band<D> &ls, double dt) {
+    const int L = nMat();
+    Band<D> out;
+    std::vector<double> vals(ls.idx.size());
+    for (size_t e = 0; e < ls.idx.size(); ++e) {
+      double value = ls.val[e];
+      if (isUndef(value) || std::fabs(value) > 0.5) {
+        vals[e] = (value < 0) ? NEG : POS;
+        continue;
+      }
+      const double *V = &mmV[e * L], *S = &mmS[e * L];
+      double time = dt;
+      int k = 0;
+      double velocity = V[0];
+      double rate = time * velocity;
+      while (std::fabs(S[k] - value) < std::fabs(rate)) {
+        time -= std::fabs((S[k] - value) / velocity);
+        value = S[k];
+        ++k;
+        velocity = V[k];
+        rate = time * velocity;
+      }
+      vals[e] = value - rate;
+    }
+    out.g = ls.g;
+    compressInto<D>(out, ls.idx, vals);
+    out.width = 1;
+    ls = std::move(out);
+  }
+
+  // Advect::adjustLowerLayers (lsAdvect.hpp:802-813): every lower level set
+  // intersected with the top (BooleanOperation INTERSECT = max + Prune)
+  void adjustLowerLayers(const Band<D> &top) {
+    for (Band<D> *b : lower) booleanOp(*b, top, [](double x, double y) { return maxComp(x, y); });
+  }
+
   // Advect::computeRates + integrateTime (lsAdvect.hpp:435-682), single LS.
   // Returns the time step; fills rates for every entry (0 for inactive).
   double computeRates(Band<D> &ls, double maxTimeStep) {
@@ -1015,49 +1205,79 @@ template <int D> struct Advector {
   }
 
   // AdvectTimeIntegration::evolveForwardEuler (lsAdvectIntegrationSchemes.hpp:46-59)
-  double evolveFE(Band<D> &ls, double maxTimeStep) {
-    currentTimeStep = computeRates(ls, maxTimeStep);
-    if (currentTimeStep < 0) return currentTimeStep;
-    updateLevelSet(ls, currentTimeStep);
+  double evolveFE(Band<D> &ls, double maxTimeStep, bool updateLowerLayers = true) {
+    if (lower.empty()) {
+      currentTimeStep = computeRates(ls, maxTimeStep);
+      if (currentTimeStep < 0) return currentTimeStep;
+      updateLevelSet(ls, currentTimeStep);
+    } else {
+      currentTimeStep = computeRatesMM(ls, maxTimeStep);
+      if (currentTimeStep < 0) return currentTimeStep;
+      updateLevelSetMM(ls, currentTimeStep);
+    }
     rebuild(ls);
+    if (updateLowerLayers) adjustLowerLayers(ls);
     return currentTimeStep;
   }
 
-  // Advect::combineLevelSets (lsAdvect.hpp:199-236)
-  void combine(Band<D> &ls, Band<D> &init, double wTarget, double wSource) {
+  // Advect::combineLevelSets (lsAdvect.hpp:199-236); returns movedDown
+  bool combine(Band<D> &ls, Band<D> &init, double wTarget, double wSource) {
     int steps = 1;
     if (P.temporal == RK2) steps = 2;
     else if (P.temporal == RK3) steps = 3;
     int ew = (int)std::ceil(2.0 * steps * P.cfl + 1);
     expand(ls, ew);
     expand(init, ew);
+    bool movedDown = false;  // written by the serial comparator loop below
+    Band<D> B = init;
+    {
+      const Grid<D> &g = ls.g;
+      std::vector<Idx<D>> cand;
+      std::merge(ls.idx.begin(), ls.idx.end(), B.idx.begin(), B.idx.end(), std::back_inserter(cand), idxLess<D>);
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      for (const Idx<D> &c : cand) {
+        double a = ls.value(c), b = B.value(c);
+        if (!isUndef(a) && !isUndef(b) && wSource * a + wTarget * b > b) movedDown = true;  // :221-223
+      }
+      (void)g;
+    }
     booleanOp<D>(ls, init, [wTarget, wSource](double a, double b) -> double {
       if (!isUndef(a) && !isUndef(b)) return wSource * a + wTarget * b;
       if (isUndef(a)) return a;
       return b;
     });
     rebuild(ls);
+    return movedDown;
   }
 
-  // lsAdvectIntegrationSchemes.hpp:61-108
+  // lsAdvectIntegrationSchemes.hpp:61-108 (no velocity-update callback)
   double evolveRK2(Band<D> &ls, double maxTimeStep) {
     Band<D> init = ls;
-    double dt1 = evolveFE(ls, maxTimeStep);
+    double dt1 = evolveFE(ls, maxTimeStep, false);
     if (dt1 <= 0.) return (dt1 < 0) ? dt1 : 0.;
-    double dt2 = evolveFE(ls, dt1);
+    double dt2 = evolveFE(ls, dt1, false);
     combine(ls, init, 0.5, 0.5);
+    adjustLowerLayers(ls);
     return 0.5 * dt1 + 0.5 * dt2;
   }
 
-  // lsAdvectIntegrationSchemes.hpp:110-168
+  // lsAdvectIntegrationSchemes.hpp:110-168 (no velocity-update callback):
+  // the lower level sets are restored from the start of the step when the
+  // last combine moved the surface down anywhere
   double evolveRK3(Band<D> &ls, double maxTimeStep) {
     Band<D> init = ls;
-    double dt1 = evolveFE(ls, maxTimeStep);
+    std::vector<Band<D>> lowerInit;
+    for (Band<D> *b : lower) lowerInit.push_back(*b);
+    double dt1 = evolveFE(ls, maxTimeStep, false);
     if (dt1 <= 0.) return (dt1 < 0) ? dt1 : 0.;
-    double dt2 = evolveFE(ls, dt1);
+    double dt2 = evolveFE(ls, dt1, false);
     combine(ls, init, 0.75, 0.25);
-    double dt3 = evolveFE(ls, dt1);
-    combine(ls, init, 1.0 / 3.0, 2.0 / 3.0);
+    adjustLowerLayers(ls);
+    double dt3 = evolveFE(ls, dt1, false);
+    const bool etched2 = combine(ls, init, 1.0 / 3.0, 2.0 / 3.0);
+    if (etched2)
+      for (size_t i = 0; i < lower.size(); ++i) *lower[i] = lowerInit[i];
+    adjustLowerLayers(ls);
     return (dt1 + dt2 + 4.0 * dt3) / 6.0;
   }
 
@@ -1240,6 +1460,11 @@ double lso_weno5(const double *x, double dx, int plus) { return weno5(x, dx, plu
 double lso_weno3(const double *x, double dx, int plus) { return weno3(x, dx, plus != 0); }
 
 // Flat parameter block shared with the test harness (ctypes).
+struct lso_vel {
+  int kind, dir, k;
+  double b, aPos, aNeg;
+  double vec[3];
+};
 struct lso_params {
   int spatial, temporal;
   double cfl, advectionTime;
@@ -1250,6 +1475,8 @@ struct lso_params {
   double velB, velAPos, velANeg;
   double vec[3];
   const float *mlp;
+  int nMatVel;  // functors of materials 0 .. nMatVel-1 (multi-material)
+  lso_vel matVel[8];
 };
 
 static AdvectParams toParams(const lso_params *p) {
@@ -1270,6 +1497,17 @@ static AdvectParams toParams(const lso_params *p) {
   P.vel.aNeg = p->velANeg;
   for (int i = 0; i < 3; ++i) P.vel.vec[i] = p->vec[i];
   P.mlp = p->mlp;
+  for (int m = 0; m < p->nMatVel && m < 8; ++m) {
+    Velocity v;
+    v.kind = p->matVel[m].kind;
+    v.dir = p->matVel[m].dir;
+    v.k = p->matVel[m].k;
+    v.b = p->matVel[m].b;
+    v.aPos = p->matVel[m].aPos;
+    v.aNeg = p->matVel[m].aNeg;
+    for (int i = 0; i < 3; ++i) v.vec[i] = p->matVel[m].vec[i];
+    P.matVel.push_back(v);
+  }
   return P;
 }
 
@@ -1285,6 +1523,35 @@ int lso_advect(lso_domain *d, const lso_params *p, double *out) {
     Advector<3> a;
     a.P = toParams(p);
     r = a.apply(d->b3);
+  }
+  if (out) {
+    out[0] = r.advectedTime;
+    out[1] = r.currentTimeStep;
+    out[2] = r.numSteps;
+    out[3] = r.pointUpdates;
+  }
+  return r.status;
+}
+
+// Advect::apply with several level sets (levels[0] bottom ... levels[n-1]
+// the advected top, all on the same grid); out as lso_advect.
+int lso_advect_multi(lso_domain **levels, int n, const lso_params *p, double *out) {
+  if (n < 1 || n > 8) return -3;
+  const int D = levels[n - 1]->D;
+  for (int i = 0; i < n; ++i)
+    if (levels[i]->D != D) return -3;
+  AdvectResult r;
+  auto run = [&](auto &adv, auto member) {
+    adv.P = toParams(p);
+    for (int i = 0; i + 1 < n; ++i) adv.lower.push_back(&(levels[i]->*member));
+    r = adv.apply(levels[n - 1]->*member);
+  };
+  if (D == 2) {
+    Advector<2> a;
+    run(a, &lso_domain::b2);
+  } else {
+    Advector<3> a;
+    run(a, &lso_domain::b3);
   }
   if (out) {
     out[0] = r.advectedTime;

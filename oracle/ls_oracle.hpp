@@ -171,6 +171,10 @@ struct AdvectParams {
   int checkDissipation = 1;      // lsAdvect.hpp:76
   Velocity vel;
   const float *mlp = nullptr;    // WENO5_LEARNED network parameters
+  // Multi-material: functor of material m (the index of the level set below
+  // or at the point, lsAdvect.hpp:513-527); materials without one use vel.
+  std::vector<Velocity> matVel;
+  const Velocity &velFor(int m) const { return (m >= 0 && m < (int)matVel.size()) ? matVel[m] : vel; }
 };
 
 struct AdvectResult {

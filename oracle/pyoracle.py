@@ -21,6 +21,14 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+class Vel(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int), ("dir", ctypes.c_int), ("k", ctypes.c_int),
+        ("b", ctypes.c_double), ("aPos", ctypes.c_double), ("aNeg", ctypes.c_double),
+        ("vec", ctypes.c_double * 3),
+    ]
+
+
 class Params(ctypes.Structure):
     _fields_ = [
         ("spatial", ctypes.c_int), ("temporal", ctypes.c_int),
@@ -31,6 +39,8 @@ class Params(ctypes.Structure):
         ("velB", ctypes.c_double), ("velAPos", ctypes.c_double), ("velANeg", ctypes.c_double),
         ("vec", ctypes.c_double * 3),
         ("mlp", ctypes.c_void_p),
+        ("nMatVel", ctypes.c_int),
+        ("matVel", Vel * 8),
     ]
 
 
@@ -72,6 +82,8 @@ def lib():
         L.lso_weno3.restype = ctypes.c_double
         L.lso_advect.argtypes = [P, ctypes.POINTER(Params), dp]
         L.lso_advect.restype = ctypes.c_int
+        L.lso_advect_multi.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(Params), dp]
+        L.lso_advect_multi.restype = ctypes.c_int
         L.lso_compute_rates.argtypes = [P, ctypes.POINTER(Params), ctypes.c_double, dp, ctypes.c_int64]
         L.lso_compute_rates.restype = ctypes.c_double
         fp = ctypes.POINTER(ctypes.c_float)
@@ -181,9 +193,11 @@ class Domain:
 
 def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
                 dissipation_alpha=1.0, calc_normals=True, check_dissipation=True,
-                vel=None, mlp=None):
+                vel=None, mlp=None, mat_vels=None):
     """vel: dict(kind, dir, k, b, a_pos, a_neg, vec) of the built-in velocity.
-    mlp: float32 array of the learned-weno5 network (spatial scheme 12)."""
+    mlp: float32 array of the learned-weno5 network (spatial scheme 12).
+    mat_vels: list of such dicts, the velocity of materials 0, 1, ...
+    (multi-material advection; other materials use vel)."""
     p = Params()
     if mlp is not None:
         p._mlp_keep = np.ascontiguousarray(mlp, dtype=np.float32)
@@ -206,12 +220,32 @@ def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
     v = vel.get("vec", (0.0, 0.0, 0.0))
     for i in range(3):
         p.vec[i] = v[i]
+    mat_vels = mat_vels or []
+    if len(mat_vels) > 8:
+        raise ValueError("at most 8 material velocities")
+    p.nMatVel = len(mat_vels)
+    for m, mv in enumerate(mat_vels):
+        t = p.matVel[m]
+        t.kind, t.dir, t.k = mv.get("kind", 0), mv.get("dir", 2), mv.get("k", 1)
+        t.b, t.aPos, t.aNeg = mv.get("b", 0.0), mv.get("a_pos", 0.0), mv.get("a_neg", 0.0)
+        vv = mv.get("vec", (0.0, 0.0, 0.0))
+        for i in range(3):
+            t.vec[i] = vv[i]
     return p
 
 
 def advect(dom, params):
     out = (ctypes.c_double * 4)()
     st = lib().lso_advect(dom.h, ctypes.byref(params), out)
+    return st, dict(advected_time=out[0], time_step=out[1], steps=int(out[2]), point_updates=out[3])
+
+
+def advect_multi(doms, params):
+    """Advect::apply with several level sets (doms[0] bottom, doms[-1] the
+    advected top)."""
+    arr = (ctypes.c_void_p * len(doms))(*[d.h for d in doms])
+    out = (ctypes.c_double * 4)()
+    st = lib().lso_advect_multi(arr, len(doms), ctypes.byref(params), out)
     return st, dict(advected_time=out[0], time_step=out[1], steps=int(out[2]), point_updates=out[3])
 
 
