@@ -116,6 +116,14 @@ int lsg_advect_clear_level_sets(lsg_advect *a);
 /* setVelocityField (lsAdvect.hpp:864) with a built-in functor. */
 int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, double b, double a_pos, double a_neg,
                                     const double *vec3);
+/* Several level sets (Advect::insertNextLevelSet called more than once,
+ * lsAdvect.hpp:856): the reference's VelocityField receives the material
+ * index, the number of the level set below or at the point
+ * (lsVelocityField.hpp:17-38, lsAdvect.hpp:513-527).  This sets the
+ * built-in functor of one material (0..7; materials without one use the
+ * functor of lsg_advect_set_velocity_builtin, which clears these). */
+int lsg_advect_set_velocity_builtin_material(lsg_advect *a, int material, int kind, int dir, int k, double b,
+                                             double a_pos, double a_neg, const double *vec3);
 /* setVelocityField (lsAdvect.hpp:864) with a VelocityField<T> subclass
  * evaluated on the host (lsVelocityField.hpp:17-38): for every active point
  * of every stage the library calls scalar(coordinate, material, normal,

@@ -273,6 +273,25 @@ class VelocityField:
                     vec=self.vec)
 
 
+class MaterialVelocityField:
+    """A VelocityField whose built-in functor depends on the material
+    (several level sets: the material of a point is the index of the level
+    set below or at it, lsAdvect.hpp:513-527).  `per_material[m]` is the
+    VelocityField of material m; materials beyond the list use `default`
+    (or the last entry)."""
+
+    def __init__(self, per_material, default=None):
+        self.per_material = list(per_material)
+        if not self.per_material:
+            raise ValueError("MaterialVelocityField needs at least one functor")
+        if len(self.per_material) > 8:
+            raise ValueError("at most 8 materials")
+        self.default = default if default is not None else self.per_material[-1]
+
+    def as_dicts(self):
+        return [v.as_dict() for v in self.per_material]
+
+
 class HostVelocityField:
     """VelocityField<T> (lsVelocityField.hpp:11-41) implemented in Python:
     override getScalarVelocity / getVectorVelocity.  Evaluated on the host
@@ -332,6 +351,15 @@ class Advect:
                                                      ctypes.cast(self._callbacks[1], ctypes.c_void_p), None))
             return
         self._callbacks = None
+        if isinstance(v, MaterialVelocityField):
+            d = v.default
+            vec = (ctypes.c_double * 3)(*d.vec)
+            check(lib().lsg_advect_set_velocity_builtin(self._h, d.kind, d.dir, d.k, d.b, d.a_pos, d.a_neg, vec))
+            for m, f in enumerate(v.per_material):
+                vec = (ctypes.c_double * 3)(*f.vec)
+                check(lib().lsg_advect_set_velocity_builtin_material(self._h, m, f.kind, f.dir, f.k, f.b, f.a_pos,
+                                                                     f.a_neg, vec))
+            return
         vec = (ctypes.c_double * 3)(*v.vec)
         check(lib().lsg_advect_set_velocity_builtin(self._h, v.kind, v.dir, v.k, v.b, v.a_pos, v.a_neg, vec))
 

@@ -242,6 +242,11 @@ struct RateParams {
   // host-evaluated velocities: scalar and vector per active point, in
   // active-list order (4 doubles each); null for the built-in functor
   const double *hostVel;
+  // several level sets: built-in functor of material m where matSet[m]
+  // (the others use the functor above); kind, dir, k, b, aPos, aNeg, vec
+  int matSet[8];
+  int matKind[8], matDir[8], matK[8];
+  double matB[8], matAPos[8], matANeg[8], matVec[8][3];
 };
 
 // Learned weno5 network: 5 -> 32 -> 32 -> 3 (GELU, softmax); parameter
@@ -274,7 +279,8 @@ enum OpKind : int {
   OP_PRUNE = 3,     // Prune
   OP_PRUNE_RSZ = 4, // Prune with removeStrayZeros
   OP_UPDATE = 5,    // Reduce(1) + value -= dt * rate
-  OP_CHECK = 6
+  OP_CHECK = 6,
+  OP_UPDATE_MM = 7  // Reduce(1) + update through the rate list of a point (several level sets)
 };
 enum BoolKind : int { B_INTERSECT = 0, B_UNION = 1, B_RELCOMP = 2, B_COMBINE = 3 };
 
@@ -283,6 +289,7 @@ struct OpParams {
   double a, b;  // expand: limit, totalLimit; reduce: valueLimit; rebuild: cutoff; combine: wTarget, wSource
   const double *rates;
   const double *dt;  // device pointer (update)
+  int nmat;          // OP_UPDATE_MM: rate-list slots per entry (level sets)
 };
 
 }  // namespace lsg

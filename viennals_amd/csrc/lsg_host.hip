@@ -452,8 +452,9 @@ static void doRebuild(lsg_domain *d) {  // lsAdvect.hpp:238-429
   d->width = 2;
 }
 // booleanOpInternal + pruning (lsBooleanOperation.hpp:59-213)
-static void doBoolean(lsg_domain *a, const lsg_domain *b, int kind, double wT, double wS) {
-  launchBoolean(g_ctx.s, a->D, a->cur.b, b->cur.b, a->tmp.b, a->f, a->g, g_ws->w, kind, wT, wS, g_ctx.sc);
+static void doBoolean(lsg_domain *a, const lsg_domain *b, int kind, double wT, double wS,
+                      uint32_t *movedDown = nullptr) {
+  launchBoolean(g_ctx.s, a->D, a->cur.b, b->cur.b, a->tmp.b, a->f, a->g, g_ws->w, kind, wT, wS, g_ctx.sc, movedDown);
   swapCur(a);
   doPrune(a, true);
   doPrune(a, false);
@@ -504,28 +505,38 @@ static int maybeReframe(lsg_domain *d, bool force) {
   return uploadRuns(d, idx, val, w);
 }
 
-// Bring two level sets onto a common frame (BooleanOperation inputs).
-static int commonFrame(lsg_domain *a, lsg_domain *b) {
-  if (a->D != b->D) return fail(-20, "level sets of different dimension");
-  for (int k = 0; k < 3; ++k)
-    if (a->g.lo[k] != b->g.lo[k] || a->g.hi[k] != b->g.hi[k] || a->g.bc[k] != b->g.bc[k])
-      return fail(-21, "level sets on different grids");
-  if (std::memcmp(&a->f, &b->f, sizeof(Frame)) == 0 && a->cur.nwAlloc == b->cur.nwAlloc) return 0;
-  uint32_t nd;
-  int32_t amin[3], amax[3], bmin[3], bmax[3];
-  bool aa, ba;
-  if (int e = bandStats(a, nd, amin, amax, aa)) return e;
-  if (int e = bandStats(b, nd, bmin, bmax, ba)) return e;
-  int32_t mn[3], mx[3];
-  for (int k = 0; k < 3; ++k) {
-    mn[k] = aa ? (ba ? std::min(amin[k], bmin[k]) : amin[k]) : bmin[k];
-    mx[k] = aa ? (ba ? std::max(amax[k], bmax[k]) : amax[k]) : bmax[k];
+// Bring level sets onto one frame covering all their bands (BooleanOperation
+// inputs; the level sets of a multi-material Advect).
+static int commonFrameAll(const std::vector<lsg_domain *> &ds) {
+  lsg_domain *a = ds[0];
+  bool same = true;
+  for (lsg_domain *d : ds) {
+    if (a->D != d->D) return fail(-20, "level sets of different dimension");
+    for (int k = 0; k < 3; ++k)
+      if (a->g.lo[k] != d->g.lo[k] || a->g.hi[k] != d->g.hi[k] || a->g.bc[k] != d->g.bc[k])
+        return fail(-21, "level sets on different grids");
+    if (a->g.dx != d->g.dx) return fail(-21, "level sets on different grids");
+    if (!d->framed) return fail(-7, "level set has no data");
+    if (std::memcmp(&a->f, &d->f, sizeof(Frame)) != 0) same = false;
   }
-  if (!aa && !ba)
-    for (int k = 0; k < 3; ++k) mn[k] = mx[k] = 0;
+  if (same) return 0;
+  int32_t mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+  bool any = false;
+  for (lsg_domain *d : ds) {
+    uint32_t nd;
+    int32_t dmin[3], dmax[3];
+    bool da;
+    if (int e = bandStats(d, nd, dmin, dmax, da)) return e;
+    if (!da) continue;
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = any ? std::min(mn[k], dmin[k]) : dmin[k];
+      mx[k] = any ? std::max(mx[k], dmax[k]) : dmax[k];
+    }
+    any = true;
+  }
   Frame f;
   if (int e = makeFrame(a, mn, mx, f)) return e;
-  for (lsg_domain *d : {a, b}) {
+  for (lsg_domain *d : ds) {
     std::vector<int32_t> idx;
     std::vector<double> val;
     if (int e = downloadRuns(d, idx, val)) return e;
@@ -533,6 +544,7 @@ static int commonFrame(lsg_domain *a, lsg_domain *b) {
   }
   return 0;
 }
+static int commonFrame(lsg_domain *a, lsg_domain *b) { return commonFrameAll({a, b}); }
 
 }  // namespace lsg
 
@@ -549,6 +561,7 @@ struct lsg_advect {
   unsigned numSteps = 0;
   lsg_domain *init = nullptr;  // RK initial level set
   lsg_domain *save = nullptr;  // band at the start of a step (capacity-overflow retry)
+  std::vector<lsg_domain *> lowerSave;  // several level sets: the lower ones at the start of a step
   double *rates = nullptr;
   size_t ratesCap = 0;
   bool profiling = false;
@@ -597,6 +610,7 @@ struct lsg_advect {
     dropGraphs();
     if (init) delete init;
     if (save) delete save;
+    for (lsg_domain *d : lowerSave) delete d;
     if (rates) (void)hipFree(rates);
     for (auto e : evPool) (void)hipEventDestroy(e);
     for (auto e : starPool) (void)hipEventDestroy(e);
@@ -933,10 +947,146 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
   return fail(-30, "band capacity overflow persists after retries");
 }
 
+// ---------------------------------------------------- several level sets ---
+// Advect with level sets below the advected top (lsAdvect.hpp:435-612,
+// 686-813; lsAdvectIntegrationSchemes.hpp:46-168 without a velocity-update
+// callback).  All level sets share the top's frame during a step.  Steps
+// are launched directly (no graph): RK3 reads the last combine's movedDown
+// flag on the host to decide whether the lower level sets are restored.
+
+static void adjustLowerLayers(lsg_advect *a) {  // lsAdvect.hpp:802-813
+  lsg_domain *top = a->levelSets.back();
+  for (size_t i = 0; i + 1 < a->levelSets.size(); ++i) doBoolean(a->levelSets[i], top, B_INTERSECT, 0, 0);
+}
+
+static void stageMM(lsg_advect *a, const double *maxDt, double *dtOut) {
+  lsg_domain *top = a->levelSets.back();
+  const int L = (int)a->levelSets.size();
+  doExpand(top, prepareWidth(a->spatial));
+  LowerViews lw{};
+  lw.n = L - 1;
+  for (int m = 0; m < L - 1; ++m) {
+    lw.bits[m] = a->levelSets[m]->cur.b.bits;
+    lw.pre[m] = a->levelSets[m]->cur.b.pre;
+    lw.val[m] = a->levelSets[m]->cur.b.val;
+  }
+  launchRatesMM(g_ctx.s, top->D, top->cur.b, top->f, top->g, lw, a->rp, a->rates, g_ctx.sc, maxDt, dtOut,
+                g_ws->w);
+  OpParams op{};
+  op.kind = OP_UPDATE_MM;
+  op.rates = a->rates;
+  op.dt = dtOut;
+  op.nmat = L;
+  opSelf(top, op);
+  top->width = 1;
+  doRebuild(top);
+}
+
+static void combineMM(lsg_advect *a, double wTarget, double wSource, uint32_t *movedDown) {
+  lsg_domain *top = a->levelSets.back();
+  const int ew = (int)std::ceil(2.0 * (a->temporal == 1 ? 2 : 3) * a->cfl + 1);
+  doExpand(top, ew);
+  doExpand(a->init, ew);
+  doBoolean(top, a->init, B_COMBINE, wTarget, wSource, movedDown);
+  doRebuild(top);
+}
+
+static int enqueueStepMM(lsg_advect *a) {
+  DevScalars *sc = g_ctx.sc;
+  lsg_domain *top = a->levelSets.back();
+  const size_t nl = a->levelSets.size() - 1;
+  launchStepBegin(g_ctx.s, sc, a->devMaxDt);
+  copyDomainNoAlloc(a->save, top);
+  for (size_t i = 0; i < nl; ++i) copyDomainNoAlloc(a->lowerSave[i], a->levelSets[i]);
+  if (a->temporal == 0) {
+    stageMM(a, &sc->maxTimeStep, &sc->dt[0]);
+    adjustLowerLayers(a);
+  } else if (a->temporal == 1) {
+    copyDomainNoAlloc(a->init, top);
+    stageMM(a, &sc->maxTimeStep, &sc->dt[0]);
+    stageMM(a, &sc->dt[0], &sc->dt[1]);
+    combineMM(a, 0.5, 0.5, nullptr);
+    adjustLowerLayers(a);
+  } else {
+    copyDomainNoAlloc(a->init, top);
+    stageMM(a, &sc->maxTimeStep, &sc->dt[0]);
+    stageMM(a, &sc->dt[0], &sc->dt[1]);
+    combineMM(a, 0.75, 0.25, nullptr);
+    adjustLowerLayers(a);
+    stageMM(a, &sc->dt[0], &sc->dt[2]);
+    launchSetFlag(g_ctx.s, &sc->movedDown, 0);
+    combineMM(a, 1.0 / 3.0, 2.0 / 3.0, &sc->movedDown);
+    // the lower level sets go back to the start of the step when the last
+    // combine moved the surface down anywhere (lsAdvectIntegrationSchemes.hpp:157-163)
+    LSG_HIP(hipMemcpyAsync(a->hostSc, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    if (a->hostSc->movedDown && !a->hostSc->overflow)
+      for (size_t i = 0; i < nl; ++i) copyDomainNoAlloc(a->levelSets[i], a->lowerSave[i]);
+    adjustLowerLayers(a);
+  }
+  launchStepTime(g_ctx.s, sc, a->temporal);
+  launchFrameCheck(g_ctx.s, top->cur.b, top->f, top->g, top->D, REFRAME_SLACK, sc);
+  return 0;
+}
+
+static int runStepMM(lsg_advect *a, double maxTimeStep, StepOut &out) {
+  lsg_domain *top = a->levelSets.back();
+  const size_t L = a->levelSets.size();
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    if (int e = commonFrameAll(a->levelSets)) return e;
+    size_t cap = top->cur.b.cap;
+    for (lsg_domain *d : a->levelSets) cap = std::max(cap, d->cur.b.cap);
+    for (lsg_domain *d : a->levelSets)
+      if (int e = ensureStores(d, cap)) return e;
+    if (!a->save) a->save = new lsg_domain();
+    if (int e = ensureCur(a->save, top->cur.nwAlloc, cap)) return e;
+    while (a->lowerSave.size() < L - 1) a->lowerSave.push_back(new lsg_domain());
+    for (size_t i = 0; i + 1 < L; ++i)
+      if (int e = ensureCur(a->lowerSave[i], top->cur.nwAlloc, cap)) return e;
+    if (a->init) {
+      if (int e = ensureCur(a->init, top->cur.nwAlloc, cap)) return e;
+      a->init->f = top->f;
+      if (int e = ensureStores(a->init, cap)) return e;
+    }
+    if (int e = ensureRates(a, cap * 2 * L)) return e;
+    std::vector<int> widths;
+    for (lsg_domain *d : a->levelSets) widths.push_back(d->width);
+    *a->hostMaxDt = maxTimeStep;
+    if (int e = enqueueStepMM(a)) return e;
+    LSG_HIP(hipMemcpyAsync(a->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    const DevScalars &h = *a->hostSc;
+    if (!h.overflow) {
+      out.stepTime = h.stepTime;
+      out.dt1 = h.dt[0];
+      out.active = h.totalActive - h.stepActive0;
+      if (h.reframe) return maybeReframe(top, true);  // the next step re-frames the others
+      return 0;
+    }
+    // grow every band and retry from the start of the step
+    if (int e = copyDomain(top, a->save)) return e;
+    for (size_t i = 0; i + 1 < L; ++i)
+      if (int e = copyDomain(a->levelSets[i], a->lowerSave[i])) return e;
+    for (size_t i = 0; i < L; ++i) a->levelSets[i]->width = widths[i];
+    for (lsg_domain *d : a->levelSets)
+      if (int e = ensureStores(d, cap * 2)) return e;
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  }
+  return fail(-30, "band capacity overflow persists after retries");
+}
+
 static int advectApply(lsg_advect *a) {
   if (a->levelSets.empty()) return fail(-1, "No level sets passed to Advect. Not advecting.");
   if (!a->velSet) return fail(-2, "No velocity field passed to Advect. Not advecting.");
-  if (a->levelSets.size() > 1) return fail(-3, "Advect: more than one level set (multi-material) is not supported");
+  const bool multi = a->levelSets.size() > 1;
+  if (multi) {
+    if (a->levelSets.size() > (size_t)MAX_LEVEL_SETS) return fail(-3, "Advect: at most 8 level sets are supported");
+    if (a->spatial == 2 || a->spatial == 3 || a->spatial == 12)
+      return fail(-3, "Advect: with several level sets the Lax-Friedrichs (global alpha) and learned WENO5 schemes "
+                      "are not supported");
+    if (a->hostScalar || a->hostVector)
+      return fail(-3, "Advect: with several level sets only the built-in velocity functors are supported");
+  }
   if (prepareWidth(a->spatial) < 0)
     return fail(-4, "Advect: Discretization scheme not supported on the GPU (schemes 4 and 9 are not implemented)");
   if (a->ignoreVoids) return fail(-5, "Advect: setIgnoreVoids(true) is not supported");
@@ -981,8 +1131,9 @@ static int advectApply(lsg_advect *a) {
   if (const char *ng = getenv("LSG_NO_GRAPH")) a->useGraphs = (ng[0] == '0');
   a->pointUpdates = 0;
   StepOut so{};
+  auto step = [&](double maxT) { return multi ? runStepMM(a, maxT, so) : runStep(a, d, maxT, so); };
   if (a->advectionTime == 0.) {
-    if (int e = runStep(a, d, std::numeric_limits<double>::max(), so)) return e;
+    if (int e = step(std::numeric_limits<double>::max())) return e;
     a->advectedTime = so.stepTime;
     a->numSteps = 1;
     a->pointUpdates += (double)so.active;
@@ -990,7 +1141,7 @@ static int advectApply(lsg_advect *a) {
     double t = 0.0;
     a->numSteps = 0;
     while (t < a->advectionTime) {
-      if (int e = runStep(a, d, a->advectionTime - t, so)) return e;
+      if (int e = step(a->advectionTime - t)) return e;
       a->pointUpdates += (double)so.active;
       if (!(so.stepTime > 0.)) return fail(-8, "Advect: time step is not positive; advection stalled");
       t += so.stepTime;
@@ -1659,6 +1810,23 @@ int lsg_advect_clear_level_sets(lsg_advect *a) {
   a->levelSets.clear();
   return 0;
 }
+int lsg_advect_set_velocity_builtin_material(lsg_advect *a, int material, int kind, int dir, int k, double b,
+                                             double aPos, double aNeg, const double *vec3) {
+  if (material < 0 || material >= MAX_LEVEL_SETS) return fail(-1, "material index out of range (0..7)");
+  if (kind != 0 && kind != 1) return fail(-1, "unknown velocity kind");
+  if (dir < 0 || dir > 2) return fail(-1, "velocity direction must be 0, 1 or 2");
+  RateParams &rp = a->rp;
+  rp.matSet[material] = 1;
+  rp.matKind[material] = kind;
+  rp.matDir[material] = dir;
+  rp.matK[material] = k;
+  rp.matB[material] = b;
+  rp.matAPos[material] = aPos;
+  rp.matANeg[material] = aNeg;
+  for (int i = 0; i < 3; ++i) rp.matVec[material][i] = vec3 ? vec3[i] : 0.0;
+  a->dropGraphs();
+  return 0;
+}
 int lsg_advect_set_velocity_host(lsg_advect *a, lsg_scalar_velocity_fn scalar, lsg_vector_velocity_fn vector,
                                  void *user) {
   if (!scalar && !vector) return fail(-1, "lsg_advect_set_velocity_host: no callback");
@@ -1683,6 +1851,7 @@ int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, dou
   a->rp.velAPos = aPos;
   a->rp.velANeg = aNeg;
   for (int i = 0; i < 3; ++i) a->rp.vec[i] = vec3 ? vec3[i] : 0.0;
+  for (int m = 0; m < MAX_LEVEL_SETS; ++m) a->rp.matSet[m] = 0;  // a new field replaces per-material functors
   a->velSet = true;
   return 0;
 }

@@ -151,6 +151,27 @@ LSG_HD inline double ruleUpdate(double c, double dt, double velocity) {
   }
   return (c < 0) ? NEG : POS;
 }
+// updateLevelSet with several level sets (lsAdvect.hpp:725-771): the point's
+// rate list holds (velocity after the dissipation rule, stop value) pairs;
+// while the step would carry the value past a stop value, the value moves
+// to it and the remaining time continues with the next entry.
+LSG_HD inline double ruleUpdateMM(double c, double dt, const double *list) {
+  if (!isUndef(c) && fabs(c) <= 0.5) {
+    double value = c, time = dt;
+    int k = 0;
+    double velocity = list[0];
+    double rate = time * velocity;
+    while (fabs(list[2 * k + 1] - value) < fabs(rate)) {
+      time -= fabs((list[2 * k + 1] - value) / velocity);
+      value = list[2 * k + 1];
+      ++k;
+      velocity = list[2 * k];
+      rate = time * velocity;
+    }
+    return value - rate;
+  }
+  return (c < 0) ? NEG : POS;
+}
 
 // ------------------------------------------------------- neighbourhood ----
 
@@ -416,6 +437,22 @@ struct Vel {
     return b + a * m;
   }
 };
+
+// Functor of material m (several level sets): its own if set, else `base`.
+LSG_HD inline Vel velOfMaterial(const RateParams &rp, const Vel &base, int m) {
+  if (m < 0 || m >= 8 || !rp.matSet[m]) return base;
+  Vel v;
+  v.kind = rp.matKind[m];
+  v.dir = rp.matDir[m];
+  v.k = rp.matK[m];
+  v.b = rp.matB[m];
+  v.aPos = rp.matAPos[m];
+  v.aNeg = rp.matANeg[m];
+  v.vec[0] = rp.matVec[m][0];
+  v.vec[1] = rp.matVec[m][1];
+  v.vec[2] = rp.matVec[m][2];
+  return v;
+}
 
 LSG_HD inline Vel velOf(const RateParams &rp) {
   Vel v;
@@ -1081,6 +1118,25 @@ template <int D> LSG_HD inline void globalAlphaPoint(const BandView &b, const in
 // equals cfl / max|v| since correctly rounded division is monotone) and the
 // reduceTimeStepHamiltonJacobi bound of the LF family (lsAdvect.hpp:530-606,
 // lsLaxFriedrichs.hpp:160-172).
+// Time step of a rate pass with several level sets: the minimum over the
+// active points of their summed step limits (lsAdvect.hpp:591-592), then the
+// LF-family bound as in stageTimeStep.
+LSG_HD inline double stageTimeStepMM(double maxTimeStep, double minStep, int D, const double *alphas, bool useAlphas,
+                                     double dx) {
+  double tempMax = maxTimeStep;
+  if (minStep < tempMax) tempMax = minStep;
+  if (useAlphas) {
+    double timeStep = 0;
+    LSG_UNROLL
+    for (int i = 0; i < D; ++i) timeStep += alphas[i] / dx;
+    timeStep = 1.0 / timeStep;
+    tempMax = (tempMax < timeStep) ? tempMax : timeStep;
+  }
+  double res = maxTimeStep;
+  if (tempMax < res) res = tempMax;
+  return res;
+}
+
 LSG_HD inline double stageTimeStep(double maxTimeStep, double cfl, double maxAbs, int D, const double *alphas,
                                    bool useAlphas, double dx) {
   double tempMax = maxTimeStep;

@@ -298,6 +298,7 @@ template <int D, bool SEG> struct EvalSelf {
     const uint32_t k = SEG ? sa.iseg[i] : 0u;
     if (op.kind == OP_REDUCE) return ruleReduce(c, op.a);
     if (op.kind == OP_UPDATE) return ruleUpdate(c, op.dt[k], op.rates[i]);
+    if (op.kind == OP_UPDATE_MM) return ruleUpdateMM(c, op.dt[k], op.rates + (size_t)i * 2 * op.nmat);
     const BandView v = SEG ? segView(in, sa.segs[k]) : in;
     int32_t cc[3];
     candCoords(v, i, v.pos[i], cc);
@@ -315,6 +316,7 @@ template <bool SEG> struct EvalBool {
   const uint32_t *cpos;
   int kind;
   double wT, wS;
+  uint32_t *movedDown;  // combine only: set where the result exceeds b (both defined)
   __device__ double operator()(uint32_t i) const {
     BandView va = A, vb = B;
     if (SEG) {
@@ -329,7 +331,9 @@ template <bool SEG> struct EvalBool {
     const uint32_t pa = va.pre[l[0].w], pb = vb.pre[l[1].w];
     const double a = va.val[va.entryOf(l[0], wa, pa)];
     const double b = vb.val[vb.entryOf(l[1], wb, pb)];
-    return ruleBoolean(kind, a, b, wT, wS);
+    const double r = ruleBoolean(kind, a, b, wT, wS);
+    if (movedDown && kind == B_COMBINE && !isUndef(a) && !isUndef(b) && r > b) *movedDown = 1u;
+    return r;
   }
 };
 
@@ -562,12 +566,12 @@ static void selfOp(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame
 template <int D, bool SEG>
 static void boolOp(hipStream_t s, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
                    const GridInfo &g, uint32_t nw, Workspace &ws, int kind, double wT, double wS, DevScalars *sc,
-                   const SegArgs &sa0) {
+                   const SegArgs &sa0, uint32_t *movedDown = nullptr) {
   candidates<D, 1, SEG>(s, A.bits, B.bits, f, g, nw, ws, sc, sa0);
   SegArgs sa = sa0;
   sa.iseg = ws.cseg;
   sa.oseg = out.seg;
-  EvalBool<SEG> ev{viewOf(A, f, g), viewOf(B, f, g), sa, ws.cpos, kind, wT, wS};
+  EvalBool<SEG> ev{viewOf(A, f, g), viewOf(B, f, g), sa, ws.cpos, kind, wT, wS, movedDown};
   compactInto<EvalBool<SEG>, SEG>(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, nw, f.base, out, sc, sa);
 }
 
@@ -587,10 +591,11 @@ void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const
 }
 
 void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
-                   const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc) {
+                   const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc,
+                   uint32_t *movedDown) {
   const uint32_t nw = (uint32_t)f.nwords;
-  if (D == 2) boolOp<2, false>(s, A, B, out, f, g, nw, ws, kind, wTarget, wSource, sc, noSeg());
-  else boolOp<3, false>(s, A, B, out, f, g, nw, ws, kind, wTarget, wSource, sc, noSeg());
+  if (D == 2) boolOp<2, false>(s, A, B, out, f, g, nw, ws, kind, wTarget, wSource, sc, noSeg(), movedDown);
+  else boolOp<3, false>(s, A, B, out, f, g, nw, ws, kind, wTarget, wSource, sc, noSeg(), movedDown);
 }
 
 // Batch variants: one launch per kernel for every level set of the batch.
@@ -979,6 +984,161 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
   const int useAlphas = (scheme == S_LF || scheme == S_LLLF || scheme == S_LLF) ? 1 : 0;
   kTimeStep<<<1, 1, 0, s>>>(sc, rp, D, g.dx, maxTimeStep, useAlphas, dtOut, b.n);
 }
+
+// ------------------------------------------------ several level sets ----
+
+// integrateTime with level sets below the top (lsAdvect.hpp:496-593): for
+// each active point of the top, the material is the first level set
+// (bottom up, the top included) whose value there is <= value + 1e-4; an
+// etching point closer to the level set below than the remaining CFL
+// budget stops on it and continues with the material there.  Writes the
+// point's (velocity after the dissipation rule, stop value) list and
+// reduces the minimum step limit.
+template <int D, int SCHEME>
+__global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const uint32_t *list, const uint32_t *nact,
+                                                RateParams rp, double *lists, DevScalars *sc) {
+  const uint32_t n = *nact;
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const Vel base = velOf(rp);
+  const int L = lw.n + 1;
+  const double MAXV = 1.7976931348623157e308;
+  double ga[3] = {0, 0, 0}, la[3] = {0, 0, 0};
+  double minStep = MAXV;
+  unsigned long long act = 0;
+  for (uint32_t j = blockIdx.x * TPB + threadIdx.x; j < n; j += gridDim.x * TPB) {
+    const uint32_t i = list[j];
+    int32_t c[3];
+    candCoords(b, i, b.pos[i], c);
+    const double phi0 = b.val[i];
+    ++act;
+    // values of every level set at c (the lower ones share the top's frame)
+    double lv[MAX_LEVEL_SETS];
+    const BandView::Loc l = b.locate(c[0], c[1], c[2]);
+    LSG_UNROLL
+    for (int m = 0; m < MAX_LEVEL_SETS - 1; ++m) {
+      if (m < lw.n) {
+        BandView v = b;
+        v.bits = lw.bits[m];
+        v.pre = lw.pre[m];
+        v.val = lw.val[m];
+        lv[m] = v.val[v.entryOf(l, v.bits[l.w], v.pre[l.w])];
+      }
+    }
+    lv[L - 1] = phi0;
+    double value = phi0, cfl = rp.cfl, maxStepTime = 0.;
+    double *out = lists + (size_t)i * 2 * L;
+    int k = 0;
+    for (int cur = L - 1; cur >= 0; --cur) {  // :507-589
+      int mat = -1;
+      for (int m = 0; m < L; ++m)
+        if (lv[m] <= value + 1e-4) {  // wrappingLayerEpsilon (:80)
+          mat = m;
+          break;
+        }
+      double velocity = 0., v = 0.;
+      if (mat >= 0) {
+        const Vel vm = velOfMaterial(rp, base, mat);
+        v = ratePoint<D, SCHEME>(b, c, phi0, rp, vm, ga, la, velocity);
+      }
+      double stop;
+      bool done = true;
+      if (velocity > 0.) {
+        maxStepTime += cfl / velocity;
+        stop = -MAXV;
+      } else if (velocity == 0.) {
+        maxStepTime = MAXV;
+        stop = MAXV;
+      } else {
+        const double valueBelow = (cur > 0) ? lv[cur - 1] : MAXV;
+        const double difference = fabs(valueBelow - value);
+        if (difference >= cfl) {
+          maxStepTime -= cfl / velocity;
+          stop = MAXV;
+        } else {  // terminal step onto the material below
+          cfl -= difference;
+          value = valueBelow;
+          maxStepTime -= difference / velocity;
+          stop = valueBelow;
+          done = false;
+        }
+      }
+      out[2 * k] = v;
+      out[2 * k + 1] = stop;
+      ++k;
+      if (done) break;
+    }
+    if (maxStepTime < minStep) minStep = maxStepTime;
+  }
+  // step limits are >= 0, so their bit patterns order like the values
+  {
+    unsigned long long mb = (unsigned long long)__double_as_longlong(minStep);
+    __shared__ unsigned long long sm[TPB / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      unsigned long long y = __shfl_down(mb, o, 64);
+      mb = (y < mb) ? y : mb;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sm[wid] = mb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long m = sm[0];
+      for (int q = 1; q < TPB / 64; ++q) m = (sm[q] < m) ? sm[q] : m;
+      atomicMin(&sc->minStepBits, m);
+    }
+    __syncthreads();
+  }
+  if (SCHEME == S_LLLF || SCHEME == S_LLF)
+    for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(la[d]), &sc->alphaBits[d]);
+  unsigned long long t = blockSumU64(act);
+  if (threadIdx.x == 0 && t) atomicAdd(&sc->activeCount, t);
+}
+
+__global__ void kResetMM(DevScalars *sc) { sc->minStepBits = (unsigned long long)__double_as_longlong(1.7976931348623157e308); }
+
+__global__ void kTimeStepMM(DevScalars *sc, int D, double dx, const double *maxTimeStepPtr, int useAlphas,
+                            double *dtOut) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  double al[3];
+  for (int i = 0; i < 3; ++i) al[i] = __longlong_as_double((long long)sc->alphaBits[i]);
+  const double m = __longlong_as_double((long long)sc->minStepBits);
+  *dtOut = stageTimeStepMM(*maxTimeStepPtr, m, D, al, useAlphas != 0, dx);
+  sc->totalActive += sc->activeCount;
+}
+
+void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const LowerViews &lw,
+                   const RateParams &rp, double *lists, DevScalars *sc, const double *maxTimeStep, double *dtOut,
+                   Workspace &ws) {
+  const int scheme = schemeIdOf(rp.spatial);
+  activeList(s, b, ws, sc, 1, 0);  // resets the rate maxima too
+  kResetMM<<<1, 1, 0, s>>>(sc);
+  BandView v = viewOf(b, f, g);
+  const int ge = rateBlocks();
+#define LSG_RATES_MM(DD, SS) kRatesMM<DD, SS><<<ge, TPB, 0, s>>>(v, lw, ws.cpos, ws.ncand, rp, lists, sc)
+  if (D == 2) {
+    switch (scheme) {
+    case S_EO: LSG_RATES_MM(2, S_EO); break;
+    case S_WENO5: LSG_RATES_MM(2, S_WENO5); break;
+    case S_LLLF: LSG_RATES_MM(2, S_LLLF); break;
+    case S_LLF: LSG_RATES_MM(2, S_LLF); break;
+    default: LSG_RATES_MM(2, S_WENO3); break;
+    }
+  } else {
+    switch (scheme) {
+    case S_EO: LSG_RATES_MM(3, S_EO); break;
+    case S_WENO5: LSG_RATES_MM(3, S_WENO5); break;
+    case S_LLLF: LSG_RATES_MM(3, S_LLLF); break;
+    case S_LLF: LSG_RATES_MM(3, S_LLF); break;
+    default: LSG_RATES_MM(3, S_WENO3); break;
+    }
+  }
+#undef LSG_RATES_MM
+  const int useAlphas = (scheme == S_LLLF || scheme == S_LLF) ? 1 : 0;
+  kTimeStepMM<<<1, 1, 0, s>>>(sc, D, g.dx, maxTimeStep, useAlphas, dtOut);
+}
+
+__global__ void kSetFlag(uint32_t *flag, uint32_t v) { *flag = v; }
+void launchSetFlag(hipStream_t s, uint32_t *flag, uint32_t v) { kSetFlag<<<1, 1, 0, s>>>(flag, v); }
 
 // Active list, defined list and the velocity inputs of the active points.
 void launchVelInputs(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,

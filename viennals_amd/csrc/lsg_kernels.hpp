@@ -27,6 +27,8 @@ struct DevScalars {
   double maxTimeStep; // bound passed to the first stage
   double rateBytes;   // accumulated algorithmic bytes of the rate kernel
   double starBytes;   // accumulated algorithmic bytes of the star-operation evaluation (single band)
+  unsigned long long minStepBits;  // several level sets: min over points of the step limit (bits of a double >= 0)
+  uint32_t movedDown;              // combine: the weighted value exceeded the start value somewhere
   unsigned long long stepActive0;  // totalActive at the start of the step
 };
 
@@ -87,6 +89,23 @@ void launchCopyBandWords(hipStream_t s, const BandBufs &src, BandBufs &dst, int6
 void launchBooleanSeg(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const SegCtx &c,
                       Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc);
 
+// Level sets below the advected one (several level sets), in the top's frame.
+constexpr int MAX_LEVEL_SETS = 8;
+struct LowerViews {
+  int n;
+  const uint64_t *bits[MAX_LEVEL_SETS - 1];
+  const uint32_t *pre[MAX_LEVEL_SETS - 1];
+  const double *val[MAX_LEVEL_SETS - 1];
+};
+// Rate pass with several level sets: per active point of the top a list of
+// (velocity, stop value) pairs, (lw.n + 1) pairs per entry of `lists`; dt
+// from the per-point step limits.  LF1/LF2 and the learned scheme are not
+// supported here.
+void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const LowerViews &lw,
+                   const RateParams &rp, double *lists, DevScalars *sc, const double *maxTimeStep, double *dtOut,
+                   Workspace &ws);
+void launchSetFlag(hipStream_t s, uint32_t *flag, uint32_t v);
+
 // Band operations (all asynchronous on `s`; sizes live in device memory).
 // Star operations: e0/e1 (optional) are recorded around the evaluation
 // kernel (kEvalKeep<EvalStar>), external when the stream is being captured.
@@ -95,8 +114,11 @@ void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const
                   hipEvent_t e1 = nullptr, bool externalEvents = false);
 void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc);
+// movedDown (optional): set to 1 where a combine's weighted value exceeds
+// the second band's value at a point defined in both (lsAdvect.hpp:218-224).
 void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
-                   const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc);
+                   const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc,
+                   uint32_t *movedDown = nullptr);
 // Rate pass: Hamiltonian + dissipation per active entry, CFL reduction.
 void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
                  double *rates, DevScalars *sc, const double *maxTimeStep, double *dtOut, Workspace &ws,
