@@ -326,6 +326,28 @@ public:
   Vec3D<T> getVectorVelocity(const Vec3D<T> &, int, const Vec3D<T> &, unsigned long) override { return vec; }
 };
 
+// Built-in functors per material (several level sets: the material of a
+// point is the index of the level set below or at it, lsAdvect.hpp:513-527).
+// Material m uses perMaterial[m]; later materials use the last entry.
+template <class T> class MaterialVelocityField : public VelocityField<T> {
+public:
+  std::vector<SmartPointer<BuiltinVelocityField<T>>> perMaterial;
+  explicit MaterialVelocityField(std::vector<SmartPointer<BuiltinVelocityField<T>>> v = {})
+      : perMaterial(std::move(v)) {}
+  BuiltinVelocityField<T> *of(int m) const {
+    if (perMaterial.empty()) return nullptr;
+    return perMaterial[(m >= 0 && m < (int)perMaterial.size()) ? m : (int)perMaterial.size() - 1].get();
+  }
+  T getScalarVelocity(const Vec3D<T> &c, int m, const Vec3D<T> &n, unsigned long id) override {
+    auto *f = of(m);
+    return f ? f->getScalarVelocity(c, m, n, id) : T(0);
+  }
+  Vec3D<T> getVectorVelocity(const Vec3D<T> &c, int m, const Vec3D<T> &n, unsigned long id) override {
+    auto *f = of(m);
+    return f ? f->getVectorVelocity(c, m, n, id) : Vec3D<T>{0, 0, 0};
+  }
+};
+
 // --------------------------------------------------------------- Advect ---
 
 template <class T, int D> class Advect {
@@ -353,6 +375,26 @@ template <class T, int D> class Advect {
     if (!velocities_) {
       std::fprintf(stderr, "[viennals_gpu] No velocity field passed to Advect. Not advecting.\n");
       return false;
+    }
+    if (auto *mv = dynamic_cast<MaterialVelocityField<T> *>(velocities_.get())) {
+      if (mv->perMaterial.empty() || mv->perMaterial.size() > 8) {
+        std::fprintf(stderr, "[viennals_gpu] MaterialVelocityField needs 1 to 8 functors.\n");
+        return false;
+      }
+      auto *d = mv->perMaterial.back().get();
+      double vd[3] = {d->vec[0], d->vec[1], d->vec[2]};
+      if (!detail::ok(lsg_advect_set_velocity_builtin(h_, d->kind, d->dir, d->k, d->b, d->aPos, d->aNeg, vd),
+                      "setVelocityField"))
+        return false;
+      for (size_t m = 0; m < mv->perMaterial.size(); ++m) {
+        auto *f = mv->perMaterial[m].get();
+        double vf[3] = {f->vec[0], f->vec[1], f->vec[2]};
+        if (!detail::ok(lsg_advect_set_velocity_builtin_material(h_, (int)m, f->kind, f->dir, f->k, f->b, f->aPos,
+                                                                 f->aNeg, vf),
+                        "setVelocityField"))
+          return false;
+      }
+      return true;
     }
     auto *b = dynamic_cast<BuiltinVelocityField<T> *>(velocities_.get());
     if (!b)
