@@ -85,6 +85,7 @@ int lsg_domain_get_runs(lsg_domain *d, int32_t *idx, double *val, int64_t cap);
 /* Domain::getNumberOfPoints (lsDomain.hpp:156), getLevelSetWidth (:158). */
 int lsg_domain_num_points(lsg_domain *d, int64_t *n);
 int lsg_domain_width(lsg_domain *d, int *width);
+int lsg_domain_set_width(lsg_domain *d, int width); /* Domain::setLevelSetWidth (lsDomain.hpp:160) */
 
 /* ---- MakeGeometry (lsMakeGeometry.hpp) --------------------------------- */
 /* makeSphere (lsMakeGeometry.hpp:177-241). */
@@ -147,7 +148,7 @@ int lsg_advect_set_dissipation_alpha(lsg_advect *a, double alpha);/* :955 */
 int lsg_advect_set_check_dissipation(lsg_advect *a, int check);   /* :958 */
 int lsg_advect_set_calculate_normal_vectors(lsg_advect *a, int c);/* :891 */
 int lsg_advect_set_ignore_voids(lsg_advect *a, int iv);           /* :899; only 0 supported */
-int lsg_advect_set_adaptive_time_stepping(lsg_advect *a, int on, unsigned subdivisions); /* :904; no effect on one level set */
+int lsg_advect_set_adaptive_time_stepping(lsg_advect *a, int on, unsigned subdivisions); /* :904; acts with several level sets */
 int lsg_advect_set_update_point_data(lsg_advect *a, int update);  /* :962; point data is not carried */
 /* Advect::prepareLS (lsAdvect.hpp:973-1015). */
 int lsg_advect_prepare_ls(lsg_advect *a);

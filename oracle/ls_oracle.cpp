@@ -1031,10 +1031,14 @@ template <int D> struct Advector {
         } else {
           const double valueBelow = (cur > 0) ? lv[cur - 1] : MAXV;
           const double difference = std::fabs(valueBelow - value);
+          const double adaptiveFactor = 1.0 / P.subdivisions;  // :456
           if (difference >= cfl) {
             maxStepTime -= cfl / velocity;
             stop = MAXV;
-          } else {  // terminal step onto the material below (adaptive sub-stepping off)
+          } else if (P.adaptive && difference > adaptiveFactor * cfl) {  // :568-578
+            maxStepTime -= adaptiveFactor * cfl / velocity;
+            stop = MAXV;
+          } else {  // terminal step onto the material below
             cfl -= difference;
             value = valueBelow;
             maxStepTime -= difference / velocity;
@@ -1477,6 +1481,8 @@ struct lso_params {
   const float *mlp;
   int nMatVel;  // functors of materials 0 .. nMatVel-1 (multi-material)
   lso_vel matVel[8];
+  int adaptive;
+  int subdivisions;
 };
 
 static AdvectParams toParams(const lso_params *p) {
@@ -1508,6 +1514,8 @@ static AdvectParams toParams(const lso_params *p) {
     for (int i = 0; i < 3; ++i) v.vec[i] = p->matVel[m].vec[i];
     P.matVel.push_back(v);
   }
+  P.adaptive = p->adaptive;
+  P.subdivisions = p->subdivisions < 1 ? 1u : (unsigned)p->subdivisions;
   return P;
 }
 

@@ -174,6 +174,8 @@ struct AdvectParams {
   // Multi-material: functor of material m (the index of the level set below
   // or at the point, lsAdvect.hpp:513-527); materials without one use vel.
   std::vector<Velocity> matVel;
+  int adaptive = 0;             // setAdaptiveTimeStepping (lsAdvect.hpp:904-912)
+  unsigned subdivisions = 20;
   const Velocity &velFor(int m) const { return (m >= 0 && m < (int)matVel.size()) ? matVel[m] : vel; }
 };
 

@@ -41,6 +41,8 @@ class Params(ctypes.Structure):
         ("mlp", ctypes.c_void_p),
         ("nMatVel", ctypes.c_int),
         ("matVel", Vel * 8),
+        ("adaptive", ctypes.c_int),
+        ("subdivisions", ctypes.c_int),
     ]
 
 
@@ -193,12 +195,15 @@ class Domain:
 
 def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
                 dissipation_alpha=1.0, calc_normals=True, check_dissipation=True,
-                vel=None, mlp=None, mat_vels=None):
+                vel=None, mlp=None, mat_vels=None, adaptive=False, subdivisions=20):
     """vel: dict(kind, dir, k, b, a_pos, a_neg, vec) of the built-in velocity.
     mlp: float32 array of the learned-weno5 network (spatial scheme 12).
     mat_vels: list of such dicts, the velocity of materials 0, 1, ...
-    (multi-material advection; other materials use vel)."""
+    (multi-material advection; other materials use vel).
+    adaptive, subdivisions: setAdaptiveTimeStepping."""
     p = Params()
+    p.adaptive = int(bool(adaptive))
+    p.subdivisions = int(subdivisions)
     if mlp is not None:
         p._mlp_keep = np.ascontiguousarray(mlp, dtype=np.float32)
         p.mlp = p._mlp_keep.ctypes.data

@@ -23,14 +23,16 @@ def plane2(z0, gmin=(-16, -20), gmax=(16, 20)):
     return d
 
 
-def run_both(levels, spatial, temporal, cfl, time, mat_vels, steps=1):
+def run_both(levels, spatial, temporal, cfl, time, mat_vels, steps=1, adaptive=0):
     """Advect the oracle level sets and GPU copies of them; compare every
     level set and the advected time after each apply()."""
     import viennals_amd as vls
     gpu = [to_gpu(o) for o in levels]
     p = po.make_params(spatial=spatial, temporal=temporal, cfl=cfl, time=time, vel=mat_vels[-1],
-                       mat_vels=mat_vels)
+                       mat_vels=mat_vels, adaptive=adaptive > 0, subdivisions=max(adaptive, 1))
     a = vls.Advect()
+    if adaptive:
+        a.setAdaptiveTimeStepping(True, adaptive)
     for g in gpu:
         a.insertNextLevelSet(g)
     a.setVelocityField(vls.MaterialVelocityField([vls.VelocityField(**v) for v in mat_vels]))
@@ -58,6 +60,12 @@ def test_unreached_lower_layer(spatial, temporal):
 def test_etch_stop_layer(spatial, temporal):
     say(f"two level sets, etch stop at z=-2.3: spatial {spatial} temporal {temporal}")
     run_both([plane2(-2.3), plane2(0.0)], spatial, temporal, 0.4, 5.0, [dict(b=0.0), dict(b=-1.0)])
+
+
+@pytest.mark.parametrize("spatial,temporal", [(0, 0), (11, 2)])
+def test_etch_stop_adaptive_time_stepping(spatial, temporal):
+    say(f"two level sets, etch stop with adaptive time stepping (8): spatial {spatial} temporal {temporal}")
+    run_both([plane2(-2.3), plane2(0.0)], spatial, temporal, 0.4, 5.0, [dict(b=0.0), dict(b=-1.0)], adaptive=8)
 
 
 @pytest.mark.parametrize("spatial,temporal", [(0, 0), (5, 1), (7, 2)])
@@ -106,3 +114,38 @@ def test_too_many_level_sets_and_unsupported_scheme():
     a.setSpatialScheme(3)  # LF2: global alphas per material are not supported
     with pytest.raises(vls.LsgError, match="several level sets"):
         a.apply()
+
+
+def test_reference_style_script_through_d3():
+    """A ViennaLS Python script (sphere on a plane, etched) written against
+    viennals.d3 runs unchanged on viennals_amd.d3 and gives the same band as
+    the core API."""
+    import viennals_amd as core
+    from viennals_amd import d3 as vls
+    say("reference-style d3 script: plane + sphere union, directional etch, 3 steps")
+
+    def script(mod, Domain):
+        bounds = [-12.0, 12.0, -12.0, 12.0, -12.0, 12.0]
+        bcs = [mod.BoundaryConditionEnum.REFLECTIVE_BOUNDARY, mod.BoundaryConditionEnum.REFLECTIVE_BOUNDARY,
+               mod.BoundaryConditionEnum.INFINITE_BOUNDARY]
+        substrate = Domain(bounds, bcs, 1.0)
+        mod.MakeGeometry(substrate, mod.Plane([0.0, 0.0, 0.0], [0.0, 0.0, 1.0])).apply()
+        ball = Domain(bounds, bcs, 1.0)
+        mod.MakeGeometry(ball, mod.Sphere([0.0, 0.0, 1.0], 4.0)).apply()
+        mod.BooleanOperation(substrate, ball, mod.BooleanOperationEnum.UNION).apply()
+        adv = mod.Advect(substrate, mod.VelocityField.normal_power(2, 1, -0.1, -1.0))
+        adv.setSpatialScheme(mod.SpatialSchemeEnum.WENO_5TH_ORDER)
+        adv.setTemporalScheme(mod.TemporalSchemeEnum.RUNGE_KUTTA_3RD_ORDER)
+        adv.setTimeStepRatio(0.4)
+        for _ in range(3):
+            adv.apply()
+        return substrate
+
+    a = script(vls, vls.Domain)
+    b = script(core, lambda bo, bc, dx: core.Domain(bo, bc, dx))
+    ia, va = a.get_runs()
+    ib, vb = b.get_runs()
+    assert cases.bit_equal(ia, va, ib, vb)
+    c = vls.Domain(a)  # copy constructor
+    ic, vc = c.get_runs()
+    assert cases.bit_equal(ia, va, ic, vc) and c.getNumberOfSegments() == 1

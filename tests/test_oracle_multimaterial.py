@@ -96,3 +96,20 @@ def test_mask_keeps_masked_surface():
     for x in range(6, 10):
         assert after[(x, 2)] == before[(x, 2)] == 0.0, x
         assert not any(k[0] == x and k[1] < 1 for k in after), x
+
+
+def test_adaptive_time_stepping_near_the_stop_layer():
+    # setAdaptiveTimeStepping: within one CFL budget of the layer below the
+    # step is limited to cfl/subdivisions (lsAdvect.hpp:568-578) until the
+    # remaining distance is below that; the surface still lands on the layer
+    runs = {}
+    for adaptive in (False, True):
+        top, low = plane(0.0), plane(-2.3)
+        p = po.make_params(spatial=0, temporal=0, cfl=0.4, time=5.0, vel=dict(b=-1.0),
+                           mat_vels=[dict(b=0.0), dict(b=-1.0)], adaptive=adaptive, subdivisions=8)
+        st, r = po.advect_multi([low, top], p)
+        assert st == 0 and r["advected_time"] == 5.0
+        t, l = defined(as_dict(top)), defined(as_dict(low))
+        assert t and all(t[k] == l[k] for k in t)
+        runs[adaptive] = r["steps"]
+    assert runs[True] > runs[False] + 4, runs

@@ -145,6 +145,19 @@ class Domain:
         check(lib().lsg_domain_width(self._h, ctypes.byref(w)))
         return w.value
 
+    def setLevelSetWidth(self, w):
+        check(lib().lsg_domain_set_width(self._h, int(w)))
+
+    def getNumberOfSegments(self):
+        return 1  # one device band (the reference splits by OpenMP thread)
+
+    def clearMetaData(self):
+        pass  # no metadata is kept on the device band
+
+    def print(self):
+        print(f"Domain: D={self.D} gridDelta={self.gridDelta} index bounds {self.gmin}..{self.gmax} "
+              f"bc {self.bc}; {self.getNumberOfPoints()} defined points, width {self.getLevelSetWidth()}")
+
     # run-start list exchange
     def set_runs(self, indices, values, width):
         a, pa = _i32(indices)
@@ -182,13 +195,26 @@ class Box:
 
 
 class MakeGeometry:
-    """MakeGeometry(domain, geometry).apply() (lsMakeGeometry.hpp)."""
+    """MakeGeometry(domain, geometry).apply() (lsMakeGeometry.hpp); also
+    MakeGeometry() / MakeGeometry(domain) with setLevelSet / setGeometry."""
 
-    def __init__(self, domain, geometry):
+    def __init__(self, domain=None, geometry=None):
         self.domain, self.geometry = domain, geometry
+
+    def setLevelSet(self, domain):
+        self.domain = domain
+
+    def setGeometry(self, geometry):
+        self.geometry = geometry
+
+    def setIgnoreBoundaryConditions(self, ignore):
+        if ignore:
+            raise LsgError("MakeGeometry.setIgnoreBoundaryConditions(True) is not supported")
 
     def apply(self):
         g, d = self.geometry, self.domain
+        if d is None:
+            raise LsgError("No level set was passed to MakeGeometry.")
         pad = lambda v: list(v) + [0.0] * (3 - len(v))
         if isinstance(g, Sphere):
             a, pa = _f64(pad(g.origin))
@@ -206,8 +232,13 @@ class MakeGeometry:
 
 
 class Expand:
-    def __init__(self, domain, width=0):
+    """Expand(domain, width) (lsExpand.hpp)."""
+
+    def __init__(self, domain=None, width=0):
         self.domain, self.width = domain, width
+
+    def setLevelSet(self, domain):
+        self.domain = domain
 
     def setWidth(self, w):
         self.width = w
@@ -217,19 +248,33 @@ class Expand:
 
 
 class Reduce:
-    def __init__(self, domain, width=0, noNewSegment=False):
+    """Reduce(domain, width, noNewSegment) (lsReduce.hpp); the GPU band has
+    one segment, so noNewSegment has no effect."""
+
+    def __init__(self, domain=None, width=0, noNewSegment=False):
         self.domain, self.width = domain, width
+
+    def setLevelSet(self, domain):
+        self.domain = domain
 
     def setWidth(self, w):
         self.width = w
+
+    def setNoNewSegment(self, v):
+        pass
 
     def apply(self):
         check(lib().lsg_reduce(self.domain._h, int(self.width)))
 
 
 class Prune:
-    def __init__(self, domain):
+    """Prune(domain) (lsPrune.hpp)."""
+
+    def __init__(self, domain=None):
         self.domain, self.rsz = domain, False
+
+    def setLevelSet(self, domain):
+        self.domain = domain
 
     def setRemoveStrayZeros(self, v):
         self.rsz = bool(v)
@@ -239,10 +284,28 @@ class Prune:
 
 
 class BooleanOperation:
-    def __init__(self, a, b, op=BooleanOperationEnum.INTERSECT):
+    """BooleanOperation(a, b, op) (lsBooleanOperation.hpp); also the
+    reference's (), (a), (a, b), (a, op) forms and setters."""
+
+    def __init__(self, a=None, b=None, op=BooleanOperationEnum.INTERSECT):
+        if isinstance(b, (int, np.integer)) and not isinstance(b, Domain):
+            b, op = None, b
         self.a, self.b, self.op = a, b, op
 
+    def setLevelSet(self, a):
+        self.a = a
+
+    setLevelset = setLevelSet  # the reference's Python spelling
+
+    def setSecondLevelSet(self, b):
+        self.b = b
+
+    def setBooleanOperation(self, op):
+        self.op = op
+
     def apply(self):
+        if self.a is None or self.b is None:
+            raise LsgError("BooleanOperation needs two level sets (single-level-set operations are not supported)")
         check(lib().lsg_boolean(self.a._h, self.b._h, int(self.op)))
 
 
@@ -271,6 +334,31 @@ class VelocityField:
     def as_dict(self):
         return dict(kind=self.kind, dir=self.dir, k=self.k, b=self.b, a_pos=self.a_pos, a_neg=self.a_neg,
                     vec=self.vec)
+
+    # VelocityField<T> interface (lsVelocityField.hpp:17-38).  A subclass
+    # that overrides these is evaluated on the host for every active point
+    # (like HostVelocityField); the functor itself runs on the GPU.
+    def getScalarVelocity(self, coordinate, material, normal, pointId):
+        if self.kind == 0:
+            return self.b
+        c = normal[self.dir]
+        m = 1.0
+        if self.k >= 1:
+            m = abs(c)
+            for _ in range(1, self.k):
+                m = m * abs(c)
+        return self.b + (self.a_pos if c > 0 else self.a_neg) * m
+
+    def getVectorVelocity(self, coordinate, material, normal, pointId):
+        return tuple(self.vec)
+
+    def getDissipationAlpha(self, direction, material, centralDifferences):
+        return 0.0
+
+    def _overridden(self):
+        t = type(self)
+        return (t.getScalarVelocity is not VelocityField.getScalarVelocity or
+                t.getVectorVelocity is not VelocityField.getVectorVelocity)
 
 
 class MaterialVelocityField:
@@ -336,7 +424,7 @@ class Advect:
         check(lib().lsg_advect_clear_level_sets(self._h))
 
     def setVelocityField(self, v):
-        if isinstance(v, HostVelocityField):
+        if isinstance(v, HostVelocityField) or (isinstance(v, VelocityField) and v._overridden()):
             from ._lib import SCALAR_VELOCITY_FN, VECTOR_VELOCITY_FN
 
             def scalar(c, m, n, pid, user):
@@ -376,7 +464,11 @@ class Advect:
         check(lib().lsg_advect_set_advection_time(self._h, float(t)))
 
     def setTimeStepRatio(self, c):
+        self._cfl = float(c)
         check(lib().lsg_advect_set_time_step_ratio(self._h, float(c)))
+
+    def getTimeStepRatio(self):
+        return getattr(self, "_cfl", 0.4999)  # lsAdvect.hpp:66
 
     def setSingleStep(self, s):
         check(lib().lsg_advect_set_single_step(self._h, int(bool(s))))
@@ -388,7 +480,21 @@ class Advect:
         check(lib().lsg_advect_set_check_dissipation(self._h, int(bool(c))))
 
     def setCalculateNormalVectors(self, c):
+        self._calcNormals = bool(c)
         check(lib().lsg_advect_set_calculate_normal_vectors(self._h, int(bool(c))))
+
+    def getCalculateNormalVectors(self):
+        return getattr(self, "_calcNormals", True)  # lsAdvect.hpp:68
+
+    def setSaveAdvectionVelocities(self, on):
+        """Point data is not carried by the GPU band: only False is accepted."""
+        if on:
+            raise LsgError("setSaveAdvectionVelocities(True): point data is not supported on the GPU path")
+
+    def setVelocityUpdateCallback(self, callback):
+        """Called between RK stages by the reference; not supported here."""
+        if callback is not None:
+            raise LsgError("setVelocityUpdateCallback: not supported on the GPU path")
 
     def setIgnoreVoids(self, v):
         check(lib().lsg_advect_set_ignore_voids(self._h, int(bool(v))))

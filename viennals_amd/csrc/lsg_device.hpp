@@ -245,6 +245,8 @@ struct RateParams {
   // several level sets: built-in functor of material m where matSet[m]
   // (the others use the functor above); kind, dir, k, b, aPos, aNeg, vec
   int matSet[8];
+  int adaptive;           // setAdaptiveTimeStepping (several level sets)
+  double adaptiveFactor;  // 1 / subdivisions
   int matKind[8], matDir[8], matK[8];
   double matB[8], matAPos[8], matANeg[8], matVec[8][3];
 };

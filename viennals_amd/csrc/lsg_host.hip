@@ -1727,6 +1727,11 @@ int lsg_domain_width(lsg_domain *d, int *w) {
   *w = d->width;
   return 0;
 }
+int lsg_domain_set_width(lsg_domain *d, int w) {
+  if (w < 1) return fail(-1, "level set width must be at least 1");
+  d->width = w;
+  return 0;
+}
 
 int lsg_make_sphere(lsg_domain *d, const double *origin, double radius) {
   std::vector<int32_t> pi;
@@ -1901,7 +1906,16 @@ int lsg_advect_set_ignore_voids(lsg_advect *a, int iv) {
   a->ignoreVoids = iv;
   return 0;
 }
-int lsg_advect_set_adaptive_time_stepping(lsg_advect *, int, unsigned) { return 0; }
+int lsg_advect_set_adaptive_time_stepping(lsg_advect *a, int on, unsigned subdivisions) {
+  if (subdivisions < 1) {
+    fprintf(stderr, "[viennals_gpu] Advect: Adaptive time stepping subdivisions must be at least 1. Setting to 1.\n");
+    subdivisions = 1;
+  }
+  a->rp.adaptive = on ? 1 : 0;
+  a->rp.adaptiveFactor = 1.0 / subdivisions;
+  a->dropGraphs();
+  return 0;
+}
 int lsg_advect_set_update_point_data(lsg_advect *, int) { return 0; }
 int lsg_advect_prepare_ls(lsg_advect *a) {
   if (a->levelSets.empty()) return fail(-1, "No level sets passed to Advect.");

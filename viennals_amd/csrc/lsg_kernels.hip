@@ -1054,6 +1054,9 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
         if (difference >= cfl) {
           maxStepTime -= cfl / velocity;
           stop = MAXV;
+        } else if (rp.adaptive && difference > rp.adaptiveFactor * cfl) {  // sub-steps near the layer (:568-578)
+          maxStepTime -= rp.adaptiveFactor * cfl / velocity;
+          stop = MAXV;
         } else {  // terminal step onto the material below
           cfl -= difference;
           value = valueBelow;
