@@ -125,6 +125,12 @@ int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, dou
  * functor of lsg_advect_set_velocity_builtin, which clears these). */
 int lsg_advect_set_velocity_builtin_material(lsg_advect *a, int material, int kind, int dir, int k, double b,
                                              double a_pos, double a_neg, const double *vec3);
+/* VelocityField::getDissipationAlpha (lsVelocityField.hpp:32-38) of the
+ * built-in functors, used by LOCAL_LAX_FRIEDRICHS_ANALYTICAL_1ST_ORDER
+ * (lsLocalLaxFriedrichsAnalytical.hpp): a constant per direction, for the
+ * default functor (material -1) or one material's; the velocity setters
+ * above reset it to 0 (the reference's default). */
+int lsg_advect_set_dissipation_alpha_builtin(lsg_advect *a, int material, const double *alpha3);
 /* setVelocityField (lsAdvect.hpp:864) with a VelocityField<T> subclass
  * evaluated on the host (lsVelocityField.hpp:17-38): for every active point
  * of every stage the library calls scalar(coordinate, material, normal,

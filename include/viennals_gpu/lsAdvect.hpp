@@ -296,6 +296,7 @@ public:
   int kind = LSG_VEL_CONSTANT, dir = 2, k = 1;
   T b = 0, aPos = 0, aNeg = 0;
   Vec3D<T> vec{0, 0, 0};
+  Vec3D<T> alpha{0, 0, 0};  // getDissipationAlpha: constant per direction (LLFA)
   static SmartPointer<BuiltinVelocityField> constant(T scalar, Vec3D<T> vector = {0, 0, 0}) {
     auto v = SmartPointer<BuiltinVelocityField>::New();
     v->b = scalar;
@@ -324,6 +325,7 @@ public:
     return b + ((c > 0) ? aPos : aNeg) * m;
   }
   Vec3D<T> getVectorVelocity(const Vec3D<T> &, int, const Vec3D<T> &, unsigned long) override { return vec; }
+  T getDissipationAlpha(int direction, int, const Vec3D<T> &) override { return alpha[direction]; }
 };
 
 // Built-in functors per material (several level sets: the material of a
@@ -383,8 +385,10 @@ template <class T, int D> class Advect {
       }
       auto *d = mv->perMaterial.back().get();
       double vd[3] = {d->vec[0], d->vec[1], d->vec[2]};
+      double ad[3] = {d->alpha[0], d->alpha[1], d->alpha[2]};
       if (!detail::ok(lsg_advect_set_velocity_builtin(h_, d->kind, d->dir, d->k, d->b, d->aPos, d->aNeg, vd),
-                      "setVelocityField"))
+                      "setVelocityField") ||
+          !detail::ok(lsg_advect_set_dissipation_alpha_builtin(h_, -1, ad), "setVelocityField"))
         return false;
       for (size_t m = 0; m < mv->perMaterial.size(); ++m) {
         auto *f = mv->perMaterial[m].get();
@@ -393,6 +397,8 @@ template <class T, int D> class Advect {
                                                                  f->aNeg, vf),
                         "setVelocityField"))
           return false;
+        double af[3] = {f->alpha[0], f->alpha[1], f->alpha[2]};
+        if (!detail::ok(lsg_advect_set_dissipation_alpha_builtin(h_, (int)m, af), "setVelocityField")) return false;
       }
       return true;
     }
@@ -401,8 +407,10 @@ template <class T, int D> class Advect {
       return detail::ok(lsg_advect_set_velocity_host(h_, &Advect::hostScalar, &Advect::hostVector, velocities_.get()),
                         "setVelocityField");
     double vec[3] = {b->vec[0], b->vec[1], b->vec[2]};
+    double al[3] = {b->alpha[0], b->alpha[1], b->alpha[2]};
     return detail::ok(lsg_advect_set_velocity_builtin(h_, b->kind, b->dir, b->k, b->b, b->aPos, b->aNeg, vec),
-                      "setVelocityField");
+                      "setVelocityField") &&
+           detail::ok(lsg_advect_set_dissipation_alpha_builtin(h_, -1, al), "setVelocityField");
   }
 
 public:

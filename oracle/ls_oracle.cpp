@@ -765,6 +765,47 @@ static void schemeLLLF(const SchemeCtx<D> &s, const Idx<D> &c, int order, double
   Dn = dissipation;
 }
 
+// LocalLaxFriedrichsAnalytical<T,D,1>::operator() (lsLocalLaxFriedrichsAnalytical.hpp:66-229):
+// as LLLF, with the alphas of the velocity field's getDissipationAlpha over
+// the 3^D box (constant per direction for the built-in functor, so the box
+// normals do not enter) and no dissipation factor.
+template <int D>
+static void schemeLLFA(const SchemeCtx<D> &s, const Idx<D> &c, double &H, double &Dn, double *alphaAcc) {
+  double gradPos[D], gradNeg[D];
+  double grad = 0., dissipation = 0.;
+  double n[3] = {0, 0, 0};
+  double normalModulus = 0;
+  const double phi0 = s.ls->value(c);
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs<D>(s, c, i, 1, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    n[i] = (diffNeg + diffPos) * 0.5;
+    normalModulus += n[i] * n[i];
+    grad += pow2((diffNeg + diffPos) * 0.5);
+  }
+  normalModulus = 1. / std::sqrt(normalModulus);
+  for (int i = 0; i < D; ++i) n[i] *= normalModulus;
+  double sv = s.vel->scalar(n);
+  const double *vv = s.vel->vec;
+  double totalGrad = 0.;
+  if (sv != 0.) totalGrad = sv * std::sqrt(grad);
+  for (int w = 0; w < D; w++) {
+    if (vv[w] > 0.) totalGrad += vv[w] * gradPos[w];
+    else totalGrad += vv[w] * gradNeg[w];
+  }
+  if (totalGrad == 0.) { H = 0; Dn = 0; return; }
+  for (int i = 0; i < D; ++i) {
+    const double a = s.vel->alpha[i];
+    const double alpha = smax(0.0, a);       // max over the box of a constant
+    alphaAcc[i] = smax(alphaAcc[i], a);      // finalAlphas (:214)
+    dissipation += alpha * (gradNeg[i] - gradPos[i]) * 0.5;  // :224-226
+  }
+  H = totalGrad;
+  Dn = dissipation;
+}
+
 // LocalLaxFriedrichs::operator() (lsLocalLaxFriedrichs.hpp:65-241)
 template <int D>
 static void schemeLLF(const SchemeCtx<D> &s, const Idx<D> &c, int order, double &H, double &Dn, double *alphaAcc) {
@@ -852,6 +893,7 @@ static int prepareWidth(int spatial) {
   case LLLF1: return 3; // lsLocalLocalLaxFriedrichs.hpp:32
   case LLLF2: return 5;
   case LLF1: return 7;  // lsLocalLaxFriedrichs.hpp:55 (2*(order+2)+1)
+  case LLFA1: return 7; // lsLocalLaxFriedrichsAnalytical.hpp:51-55
   case LLF2: return 9;
   case WENO3: return 5; // lsWENO.hpp:44 (2*r+1)
   case WENO5: case WENO5_LEARNED: return 7;
@@ -928,6 +970,7 @@ template <int D> struct Advector {
     case EO1: case EO2: schemeEO<D>(s, c, order, H, Dn); break;
     case LF1: case LF2: schemeLF<D>(s, c, order, H, Dn); break;
     case LLLF1: case LLLF2: schemeLLLF<D>(s, c, order, H, Dn, alphaAcc); break;
+    case LLFA1: schemeLLFA<D>(s, c, H, Dn, alphaAcc); break;
     case LLF1: case LLF2: schemeLLF<D>(s, c, order, H, Dn, alphaAcc); break;
     case WENO3: case WENO5: case WENO5_LEARNED: schemeWENO<D>(s, c, order, H, Dn); break;
     }
@@ -1058,7 +1101,8 @@ template <int D> struct Advector {
     pointUpdates += active;
     const double *fa = nullptr;
     if (P.spatial == LF1 || P.spatial == LF2) fa = s.globalAlphas;
-    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2) fa = alphaAcc;
+    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2 || P.spatial == LLFA1)
+      fa = alphaAcc;
     if (fa) {
       double timeStep = 0;
       for (int i = 0; i < D; ++i) timeStep += fa[i] / ls.g.delta;
@@ -1147,6 +1191,7 @@ template <int D> struct Advector {
         case EO1: case EO2: schemeEO<D>(s, c, order, H, Dn); break;
         case LF1: case LF2: schemeLF<D>(s, c, order, H, Dn); break;
         case LLLF1: case LLLF2: schemeLLLF<D>(s, c, order, H, Dn, locAlpha); break;
+        case LLFA1: schemeLLFA<D>(s, c, H, Dn, locAlpha); break;
         case LLF1: case LLF2: schemeLLF<D>(s, c, order, H, Dn, locAlpha); break;
         case WENO3: case WENO5: case WENO5_LEARNED: schemeWENO<D>(s, c, order, H, Dn); break;
         }
@@ -1175,7 +1220,8 @@ template <int D> struct Advector {
     // lsLocalLaxFriedrichs.hpp:243-254; LLLF: lsLocalLocalLaxFriedrichs.hpp:167-178)
     const double *fa = nullptr;
     if (P.spatial == LF1 || P.spatial == LF2) fa = s.globalAlphas;
-    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2) fa = alphaAcc;
+    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2 || P.spatial == LLFA1)
+      fa = alphaAcc;
     if (fa) {
       double timeStep = 0;
       for (int i = 0; i < D; ++i) timeStep += fa[i] / ls.g.delta;
@@ -1468,6 +1514,7 @@ struct lso_vel {
   int kind, dir, k;
   double b, aPos, aNeg;
   double vec[3];
+  double alpha[3];
 };
 struct lso_params {
   int spatial, temporal;
@@ -1483,6 +1530,7 @@ struct lso_params {
   lso_vel matVel[8];
   int adaptive;
   int subdivisions;
+  double velAlpha[3];
 };
 
 static AdvectParams toParams(const lso_params *p) {
@@ -1512,8 +1560,10 @@ static AdvectParams toParams(const lso_params *p) {
     v.aPos = p->matVel[m].aPos;
     v.aNeg = p->matVel[m].aNeg;
     for (int i = 0; i < 3; ++i) v.vec[i] = p->matVel[m].vec[i];
+    for (int i = 0; i < 3; ++i) v.alpha[i] = p->matVel[m].alpha[i];
     P.matVel.push_back(v);
   }
+  for (int i = 0; i < 3; ++i) P.vel.alpha[i] = p->velAlpha[i];
   P.adaptive = p->adaptive;
   P.subdivisions = p->subdivisions < 1 ? 1u : (unsigned)p->subdivisions;
   return P;

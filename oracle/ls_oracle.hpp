@@ -146,6 +146,9 @@ struct Velocity {
   int k = 1;
   double b = 0, aPos = 0, aNeg = 0;
   double vec[3] = {0, 0, 0};
+  // getDissipationAlpha of the built-in functor (LOCAL_LAX_FRIEDRICHS_ANALYTICAL):
+  // a constant per direction, whatever the central differences
+  double alpha[3] = {0, 0, 0};
   double scalar(const double n[3]) const {
     if (kind == 0) return b;
     double c = n[dir];

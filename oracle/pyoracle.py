@@ -26,6 +26,7 @@ class Vel(ctypes.Structure):
         ("kind", ctypes.c_int), ("dir", ctypes.c_int), ("k", ctypes.c_int),
         ("b", ctypes.c_double), ("aPos", ctypes.c_double), ("aNeg", ctypes.c_double),
         ("vec", ctypes.c_double * 3),
+        ("alpha", ctypes.c_double * 3),
     ]
 
 
@@ -43,6 +44,7 @@ class Params(ctypes.Structure):
         ("matVel", Vel * 8),
         ("adaptive", ctypes.c_int),
         ("subdivisions", ctypes.c_int),
+        ("velAlpha", ctypes.c_double * 3),
     ]
 
 
@@ -225,6 +227,9 @@ def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
     v = vel.get("vec", (0.0, 0.0, 0.0))
     for i in range(3):
         p.vec[i] = v[i]
+    al = vel.get("alpha", (0.0, 0.0, 0.0))
+    for i in range(3):
+        p.velAlpha[i] = al[i]
     mat_vels = mat_vels or []
     if len(mat_vels) > 8:
         raise ValueError("at most 8 material velocities")
@@ -234,8 +239,10 @@ def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
         t.kind, t.dir, t.k = mv.get("kind", 0), mv.get("dir", 2), mv.get("k", 1)
         t.b, t.aPos, t.aNeg = mv.get("b", 0.0), mv.get("a_pos", 0.0), mv.get("a_neg", 0.0)
         vv = mv.get("vec", (0.0, 0.0, 0.0))
+        al = mv.get("alpha", (0.0, 0.0, 0.0))
         for i in range(3):
             t.vec[i] = vv[i]
+            t.alpha[i] = al[i]
     return p
 
 

@@ -318,10 +318,12 @@ class VelocityField:
     vector = vec (constant)
     """
 
-    def __init__(self, kind=0, dir=2, k=1, b=0.0, a_pos=0.0, a_neg=0.0, vec=(0.0, 0.0, 0.0)):
+    def __init__(self, kind=0, dir=2, k=1, b=0.0, a_pos=0.0, a_neg=0.0, vec=(0.0, 0.0, 0.0), alpha=(0.0, 0.0, 0.0)):
         self.kind, self.dir, self.k = int(kind), int(dir), int(k)
         self.b, self.a_pos, self.a_neg = float(b), float(a_pos), float(a_neg)
         self.vec = tuple(float(v) for v in vec) + (0.0,) * (3 - len(vec))
+        # getDissipationAlpha (LOCAL_LAX_FRIEDRICHS_ANALYTICAL): constant per direction
+        self.alpha = tuple(float(v) for v in alpha) + (0.0,) * (3 - len(alpha))
 
     @classmethod
     def constant(cls, scalar=0.0, vector=(0.0, 0.0, 0.0)):
@@ -333,7 +335,7 @@ class VelocityField:
 
     def as_dict(self):
         return dict(kind=self.kind, dir=self.dir, k=self.k, b=self.b, a_pos=self.a_pos, a_neg=self.a_neg,
-                    vec=self.vec)
+                    vec=self.vec, alpha=self.alpha)
 
     # VelocityField<T> interface (lsVelocityField.hpp:17-38).  A subclass
     # that overrides these is evaluated on the host for every active point
@@ -353,7 +355,7 @@ class VelocityField:
         return tuple(self.vec)
 
     def getDissipationAlpha(self, direction, material, centralDifferences):
-        return 0.0
+        return getattr(self, "alpha", (0.0, 0.0, 0.0))[direction]
 
     def _overridden(self):
         t = type(self)
@@ -443,13 +445,16 @@ class Advect:
             d = v.default
             vec = (ctypes.c_double * 3)(*d.vec)
             check(lib().lsg_advect_set_velocity_builtin(self._h, d.kind, d.dir, d.k, d.b, d.a_pos, d.a_neg, vec))
+            check(lib().lsg_advect_set_dissipation_alpha_builtin(self._h, -1, (ctypes.c_double * 3)(*d.alpha)))
             for m, f in enumerate(v.per_material):
                 vec = (ctypes.c_double * 3)(*f.vec)
                 check(lib().lsg_advect_set_velocity_builtin_material(self._h, m, f.kind, f.dir, f.k, f.b, f.a_pos,
                                                                      f.a_neg, vec))
+                check(lib().lsg_advect_set_dissipation_alpha_builtin(self._h, m, (ctypes.c_double * 3)(*f.alpha)))
             return
         vec = (ctypes.c_double * 3)(*v.vec)
         check(lib().lsg_advect_set_velocity_builtin(self._h, v.kind, v.dir, v.k, v.b, v.a_pos, v.a_neg, vec))
+        check(lib().lsg_advect_set_dissipation_alpha_builtin(self._h, -1, (ctypes.c_double * 3)(*v.alpha)))
 
     def setSpatialScheme(self, s):
         check(lib().lsg_advect_set_spatial_scheme(self._h, int(s)))

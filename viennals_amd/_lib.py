@@ -22,7 +22,8 @@ EXPORTED = [
     "lsg_domain_num_points", "lsg_domain_width", "lsg_domain_set_width", "lsg_make_sphere", "lsg_make_plane", "lsg_make_box",
     "lsg_expand", "lsg_reduce", "lsg_prune", "lsg_boolean", "lsg_rebuild", "lsg_advect_create",
     "lsg_advect_destroy", "lsg_advect_insert_next_level_set", "lsg_advect_clear_level_sets",
-    "lsg_advect_set_velocity_builtin", "lsg_advect_set_velocity_builtin_material", "lsg_advect_set_spatial_scheme",
+    "lsg_advect_set_velocity_builtin", "lsg_advect_set_velocity_builtin_material",
+    "lsg_advect_set_dissipation_alpha_builtin", "lsg_advect_set_spatial_scheme",
     "lsg_advect_set_temporal_scheme",
     "lsg_advect_set_advection_time", "lsg_advect_set_time_step_ratio", "lsg_advect_set_single_step",
     "lsg_advect_set_dissipation_alpha", "lsg_advect_set_check_dissipation",
@@ -83,6 +84,7 @@ def lib():
         "lsg_advect_set_velocity_builtin": ([P, c_int, c_int, c_int, c_double, c_double, c_double, dp], c_int),
         "lsg_advect_set_velocity_builtin_material": ([P, c_int, c_int, c_int, c_int, c_double, c_double, c_double, dp],
                                                      c_int),
+        "lsg_advect_set_dissipation_alpha_builtin": ([P, c_int, dp], c_int),
         "lsg_advect_set_spatial_scheme": ([P, c_int], c_int),
         "lsg_advect_set_temporal_scheme": ([P, c_int], c_int),
         "lsg_advect_set_advection_time": ([P, c_double], c_int),

@@ -237,6 +237,7 @@ struct RateParams {
   int velKind, velDir, velK;
   double velB, velAPos, velANeg;
   double vec[3];
+  double velAlpha[3];  // getDissipationAlpha of the built-in functor (LLFA)
   // learned weno5 weights: network parameters as bf16 (WENO5 learned only)
   const uint16_t *mlp;
   // host-evaluated velocities: scalar and vector per active point, in
@@ -248,7 +249,7 @@ struct RateParams {
   int adaptive;           // setAdaptiveTimeStepping (several level sets)
   double adaptiveFactor;  // 1 / subdivisions
   int matKind[8], matDir[8], matK[8];
-  double matB[8], matAPos[8], matANeg[8], matVec[8][3];
+  double matB[8], matAPos[8], matANeg[8], matVec[8][3], matAlpha[8][3];
 };
 
 // Learned weno5 network: 5 -> 32 -> 32 -> 3 (GELU, softmax); parameter

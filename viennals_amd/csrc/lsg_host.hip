@@ -633,6 +633,7 @@ static int prepareWidth(int spatial) {
   case 3: return 5;   // LF2
   case 5: return 3;   // LLLF1 (lsLocalLocalLaxFriedrichs.hpp:32)
   case 6: return 5;   // LLLF2
+  case 4: return 7;   // LLFA1 (lsLocalLaxFriedrichsAnalytical.hpp:51-55)
   case 7: return 7;   // LLF1 (lsLocalLaxFriedrichs.hpp:55)
   case 8: return 9;   // LLF2
   case 10: return 5;  // WENO3 (lsWENO.hpp:44)
@@ -1088,7 +1089,7 @@ static int advectApply(lsg_advect *a) {
       return fail(-3, "Advect: with several level sets only the built-in velocity functors are supported");
   }
   if (prepareWidth(a->spatial) < 0)
-    return fail(-4, "Advect: Discretization scheme not supported on the GPU (schemes 4 and 9 are not implemented)");
+    return fail(-4, "Advect: Discretization scheme not supported on the GPU (scheme 9 is not implemented)");
   if (a->ignoreVoids) return fail(-5, "Advect: setIgnoreVoids(true) is not supported");
   if (!(a->cfl > 0)) return fail(-6, "Advect: time step ratio must be positive");
   if (int e = ctxInit()) return e;
@@ -1815,6 +1816,13 @@ int lsg_advect_clear_level_sets(lsg_advect *a) {
   a->levelSets.clear();
   return 0;
 }
+int lsg_advect_set_dissipation_alpha_builtin(lsg_advect *a, int material, const double *alpha3) {
+  if (material < -1 || material >= MAX_LEVEL_SETS) return fail(-1, "material index out of range (-1..7)");
+  double *dst = (material < 0) ? a->rp.velAlpha : a->rp.matAlpha[material];
+  for (int i = 0; i < 3; ++i) dst[i] = alpha3 ? alpha3[i] : 0.0;
+  a->dropGraphs();
+  return 0;
+}
 int lsg_advect_set_velocity_builtin_material(lsg_advect *a, int material, int kind, int dir, int k, double b,
                                              double aPos, double aNeg, const double *vec3) {
   if (material < 0 || material >= MAX_LEVEL_SETS) return fail(-1, "material index out of range (0..7)");
@@ -1829,6 +1837,7 @@ int lsg_advect_set_velocity_builtin_material(lsg_advect *a, int material, int ki
   rp.matAPos[material] = aPos;
   rp.matANeg[material] = aNeg;
   for (int i = 0; i < 3; ++i) rp.matVec[material][i] = vec3 ? vec3[i] : 0.0;
+  for (int i = 0; i < 3; ++i) rp.matAlpha[material][i] = 0.0;
   a->dropGraphs();
   return 0;
 }
@@ -1857,6 +1866,7 @@ int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, dou
   a->rp.velANeg = aNeg;
   for (int i = 0; i < 3; ++i) a->rp.vec[i] = vec3 ? vec3[i] : 0.0;
   for (int m = 0; m < MAX_LEVEL_SETS; ++m) a->rp.matSet[m] = 0;  // a new field replaces per-material functors
+  for (int i = 0; i < 3; ++i) a->rp.velAlpha[i] = 0.0;
   a->velSet = true;
   return 0;
 }

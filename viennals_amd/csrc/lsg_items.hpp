@@ -422,6 +422,7 @@ struct Vel {
   int kind, dir, k;
   double b, aPos, aNeg;
   double vec[3];
+  double alpha[3];  // getDissipationAlpha (LLFA): constant per direction
   // built-in VelocityField::getScalarVelocity (lsVelocityField.hpp:17-21)
   LSG_HD double scalar(const double n[3]) const {
     if (kind == 0) return b;
@@ -451,6 +452,9 @@ LSG_HD inline Vel velOfMaterial(const RateParams &rp, const Vel &base, int m) {
   v.vec[0] = rp.matVec[m][0];
   v.vec[1] = rp.matVec[m][1];
   v.vec[2] = rp.matVec[m][2];
+  v.alpha[0] = rp.matAlpha[m][0];
+  v.alpha[1] = rp.matAlpha[m][1];
+  v.alpha[2] = rp.matAlpha[m][2];
   return v;
 }
 
@@ -465,6 +469,9 @@ LSG_HD inline Vel velOf(const RateParams &rp) {
   v.vec[0] = rp.vec[0];
   v.vec[1] = rp.vec[1];
   v.vec[2] = rp.vec[2];
+  v.alpha[0] = rp.velAlpha[0];
+  v.alpha[1] = rp.velAlpha[1];
+  v.alpha[2] = rp.velAlpha[2];
   return v;
 }
 
@@ -918,6 +925,51 @@ LSG_HD inline RateOut schemeLLLF(const BandView &b, const int32_t c[3], double p
   return {totalGrad, dissipation};
 }
 
+// LocalLaxFriedrichsAnalytical<T,D,1>::operator()
+// (lsLocalLaxFriedrichsAnalytical.hpp:66-229): LLLF's Hamiltonian with the
+// velocity field's analytical alphas (the built-in functor's are constant
+// per direction, so the 3^D box of normals does not enter), no dissipation
+// factor.
+template <int D>
+LSG_HD inline RateOut schemeLLFA(const BandView &b, const int32_t c[3], double phi0, const Vel &vel, double *alphaAcc) {
+  double gradPos[D], gradNeg[D];
+  double grad = 0., dissipation = 0.;
+  double n[3] = {0, 0, 0};
+  double normalModulus = 0;
+  double st[D][5];
+  gatherUpwind<D>(b, c, 1, st);
+  LSG_UNROLL
+  for (int i = 0; i < D; i++) {
+    double diffPos, diffNeg;
+    upwindDiffs(st[i], b.g.dx, 1, phi0, diffPos, diffNeg);
+    gradPos[i] = diffNeg;
+    gradNeg[i] = diffPos;
+    n[i] = (diffNeg + diffPos) * 0.5;
+    normalModulus += n[i] * n[i];
+    grad += pow2((diffNeg + diffPos) * 0.5);
+  }
+  normalModulus = 1. / sqrt(normalModulus);
+  LSG_UNROLL
+  for (int i = 0; i < D; ++i) n[i] *= normalModulus;
+  double sv = vel.scalar(n);
+  double totalGrad = 0.;
+  if (sv != 0.) totalGrad = sv * sqrt(grad);
+  LSG_UNROLL
+  for (int w = 0; w < D; w++) {
+    if (vel.vec[w] > 0.) totalGrad += vel.vec[w] * gradPos[w];
+    else totalGrad += vel.vec[w] * gradNeg[w];
+  }
+  if (totalGrad == 0.) return {0., 0.};
+  LSG_UNROLL
+  for (int i = 0; i < D; ++i) {
+    const double a = vel.alpha[i];
+    const double alpha = dmaxs(0.0, a);
+    alphaAcc[i] = dmaxs(alphaAcc[i], a);
+    dissipation += alpha * (gradNeg[i] - gradPos[i]) * 0.5;
+  }
+  return {totalGrad, dissipation};
+}
+
 // LocalLaxFriedrichs::operator() (lsLocalLaxFriedrichs.hpp:65-241)
 template <int D>
 LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double phi0, int order, const Vel &vel,
@@ -1057,12 +1109,13 @@ template <int D> LSG_HD inline void schemeNormal(const BandView &b, const int32_
 }
 
 // Scheme ids of the rate kernel.
-enum SchemeId : int { S_EO = 0, S_WENO5 = 1, S_LF = 2, S_LLLF = 3, S_LLF = 4, S_WENO3 = 5, S_WENO5L = 6 };
+enum SchemeId : int { S_EO = 0, S_WENO5 = 1, S_LF = 2, S_LLLF = 3, S_LLF = 4, S_WENO3 = 5, S_WENO5L = 6, S_LLFA = 7 };
 
 LSG_HD inline int schemeIdOf(int spatial) {
   switch (spatial) {
   case 0: case 1: return S_EO;
   case 2: case 3: return S_LF;
+  case 4: return S_LLFA;
   case 5: case 6: return S_LLLF;
   case 7: case 8: return S_LLF;
   case 10: return S_WENO3;
@@ -1085,6 +1138,7 @@ LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi
   else if (SCHEME == S_WENO5L) r = schemeWENO<D, 5, true>(b, c, phi0, rp.calcNormals != 0, vel, mlp, io);
   else if (SCHEME == S_LF) r = schemeLF<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel, rp.alphaFactor, globalAlphas);
   else if (SCHEME == S_LLLF) r = schemeLLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
+  else if (SCHEME == S_LLFA) r = schemeLLFA<D>(b, c, phi0, vel, alphaAcc);
   else r = schemeLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
   velocity = r.H - r.Dn;
   double v = velocity;

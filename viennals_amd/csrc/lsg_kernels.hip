@@ -899,7 +899,7 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
   }
   if (!SEG) {
     blockMaxAtomic((unsigned long long)__double_as_longlong(fn.lmax), &sc->maxAbsBits);
-    if (SCHEME == S_LLLF || SCHEME == S_LLF)
+    if (SCHEME == S_LLLF || SCHEME == S_LLF || SCHEME == S_LLFA)
       for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(fn.la[d]), &sc->alphaBits[d]);
   } else {
     // the level sets of a block's chunk are few and consecutive: reduce the
@@ -965,6 +965,7 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
     case S_WENO5L: LSG_RATES(2, S_WENO5L); break;
     case S_LF: LSG_RATES(2, S_LF); break;
     case S_LLLF: LSG_RATES(2, S_LLLF); break;
+    case S_LLFA: LSG_RATES(2, S_LLFA); break;
     case S_LLF: LSG_RATES(2, S_LLF); break;
     default: LSG_RATES(2, S_WENO3); break;
     }
@@ -975,13 +976,14 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
     case S_WENO5L: LSG_RATES(3, S_WENO5L); break;
     case S_LF: LSG_RATES(3, S_LF); break;
     case S_LLLF: LSG_RATES(3, S_LLLF); break;
+    case S_LLFA: LSG_RATES(3, S_LLFA); break;
     case S_LLF: LSG_RATES(3, S_LLF); break;
     default: LSG_RATES(3, S_WENO3); break;
     }
   }
 #undef LSG_RATES
   recordEvent(s, e1, externalEvents);
-  const int useAlphas = (scheme == S_LF || scheme == S_LLLF || scheme == S_LLF) ? 1 : 0;
+  const int useAlphas = (scheme == S_LF || scheme == S_LLLF || scheme == S_LLF || scheme == S_LLFA) ? 1 : 0;
   kTimeStep<<<1, 1, 0, s>>>(sc, rp, D, g.dx, maxTimeStep, useAlphas, dtOut, b.n);
 }
 
@@ -1091,7 +1093,7 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
     }
     __syncthreads();
   }
-  if (SCHEME == S_LLLF || SCHEME == S_LLF)
+  if (SCHEME == S_LLLF || SCHEME == S_LLF || SCHEME == S_LLFA)
     for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(la[d]), &sc->alphaBits[d]);
   unsigned long long t = blockSumU64(act);
   if (threadIdx.x == 0 && t) atomicAdd(&sc->activeCount, t);
@@ -1123,6 +1125,7 @@ void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, cons
     case S_EO: LSG_RATES_MM(2, S_EO); break;
     case S_WENO5: LSG_RATES_MM(2, S_WENO5); break;
     case S_LLLF: LSG_RATES_MM(2, S_LLLF); break;
+    case S_LLFA: LSG_RATES_MM(2, S_LLFA); break;
     case S_LLF: LSG_RATES_MM(2, S_LLF); break;
     default: LSG_RATES_MM(2, S_WENO3); break;
     }
@@ -1131,12 +1134,13 @@ void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, cons
     case S_EO: LSG_RATES_MM(3, S_EO); break;
     case S_WENO5: LSG_RATES_MM(3, S_WENO5); break;
     case S_LLLF: LSG_RATES_MM(3, S_LLLF); break;
+    case S_LLFA: LSG_RATES_MM(3, S_LLFA); break;
     case S_LLF: LSG_RATES_MM(3, S_LLF); break;
     default: LSG_RATES_MM(3, S_WENO3); break;
     }
   }
 #undef LSG_RATES_MM
-  const int useAlphas = (scheme == S_LLLF || scheme == S_LLF) ? 1 : 0;
+  const int useAlphas = (scheme == S_LLLF || scheme == S_LLF || scheme == S_LLFA) ? 1 : 0;
   kTimeStepMM<<<1, 1, 0, s>>>(sc, D, g.dx, maxTimeStep, useAlphas, dtOut);
 }
 
