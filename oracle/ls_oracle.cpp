@@ -882,6 +882,132 @@ static void schemeLLF(const SchemeCtx<D> &s, const Idx<D> &c, int order, double 
   Dn = dissipation;
 }
 
+// StencilLocalLaxFriedrichsScalar<T,D,1>::operator()
+// (lsStencilLocalLaxFriedrichsScalar.hpp:238-383), first-order differences,
+// simple adaptive epsilon (ADAPTIVE_EPSILON_STRATEGY 1, :62-74).
+// calculateNormal / calculateGradient at a box offset differentiate along
+// axis i over the row through the offset with coordinate i set to -1, 0, +1
+// (:142-186), so only the 3^D box around c is read.  A box point whose
+// normal is below 1e-6 in every component ends the alpha loop: its
+// `continue` (:282-287) skips incrementIndex, so every later slot is that
+// point again and stays zero.  The alphas do not include dissipationAlpha
+// (the constructor stores it, :231-234, operator() does not use it).
+// Returns true when the point set finalAlphas (scalar velocity != 0,
+// :264-266); reduceTimeStepHamiltonJacobi (:385-397) uses the finalAlphas
+// of the last such point.
+static const double SLLF_BASE_EPS = std::cbrt(std::numeric_limits<double>::epsilon());  // :44-45
+static constexpr double SLLF_MAX_DISSIPATION = std::numeric_limits<double>::max();       // :400-404
+
+template <int D>
+static bool schemeSLLF(const SchemeCtx<D> &s, const Idx<D> &c, double &H, double &Dn, double *finalAlphas) {
+  constexpr int NB = (D == 3) ? 27 : 9;
+  double box[NB];  // box[(ox+1) + 3 (oy+1) + 9 (oz+1)], boundary mapped per coordinate
+  for (int q = 0; q < NB; ++q) {
+    const int o[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
+    Idx<D> p;
+    for (int d = 0; d < D; ++d) p[d] = s.ls->g.map(d, (int64_t)c[d] + o[d]);
+    box[q] = s.ls->value(p);
+  }
+  auto at = [&](const int *o) { return box[(o[0] + 1) + 3 * (o[1] + 1) + (D == 3 ? 9 * (o[2] + 1) : 0)]; };
+  auto rowValues = [&](const int *off, int i, double *v) {
+    int idx[3] = {off[0], off[1], off[2]};
+    for (int j = 0; j < 3; ++j) {
+      idx[i] = j - 1;
+      v[j] = at(idx);
+    }
+  };
+  // FiniteDifferences<T, FIRST_ORDER>::calculateGradient (lsFiniteDifferences.hpp:177-181)
+  auto gradient = [&](const int *off, double *g) {
+    for (int i = 0; i < D; ++i) {
+      double v[3];
+      rowValues(off, i, v);
+      g[i] = ((v[2] - v[1]) / s.dx + (v[1] - v[0]) / s.dx) * 0.5;
+    }
+  };
+  auto normalAt = [&](const int *off, double *n) {  // calculateNormal (:142-163)
+    double g[3] = {0, 0, 0};
+    gradient(off, g);
+    double modulus = 0.;
+    for (int i = 0; i < D; ++i) modulus += g[i] * g[i];
+    modulus = 1. / std::sqrt(modulus);
+    n[0] = n[1] = n[2] = 0.;
+    for (int i = 0; i < D; ++i) n[i] = g[i] * modulus;
+  };
+  const Velocity &vel = *s.vel;
+  const int zero[3] = {0, 0, 0};
+  double normal[3];
+  normalAt(zero, normal);
+  double sv = vel.scalar(normal);
+  for (int i = 0; i < D; ++i) sv += vel.vec[i] * normal[i];
+  if (sv == 0.) {
+    H = 0;
+    Dn = 0;
+    return false;
+  }
+  double g0[3] = {0, 0, 0};
+  gradient(zero, g0);
+  double dot = 0;  // Norm = sqrt(DotProduct) (ViennaCore vcVectorType.hpp)
+  for (int i = 0; i < D; ++i) dot += g0[i] * g0[i];
+  const double hamiltonian = std::sqrt(dot) * sv;
+  double maxAlpha[3] = {0, 0, 0};
+  int cur[3] = {-1, -1, (D == 3) ? -1 : 0};
+  const double baseEps = SLLF_BASE_EPS * s.dx;
+  for (int slot = 0; slot < NB; ++slot) {
+    double ln[3];
+    normalAt(cur, ln);
+    if (std::fabs(ln[0]) < 1e-6 && std::fabs(ln[1]) < 1e-6 && std::fabs(ln[2]) < 1e-6) break;
+    double lsv = vel.scalar(ln);
+    for (int d = 0; d < D; ++d) lsv += vel.vec[d] * ln[d];
+    const double absVel = std::fabs(lsv);
+    double DN = baseEps;
+    if (absVel > 1e-10) DN = baseEps * smax(0.1, smin(10.0, absVel));
+    double np[3] = {ln[0], ln[1], ln[2]}, nn[3] = {ln[0], ln[1], ln[2]};
+    double vd[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) {  // :324-338
+      np[k] -= DN;
+      nn[k] += DN;
+      const double vp = vel.scalar(np);
+      const double vn = vel.scalar(nn);
+      vd[k] = (vn - vp) / (2.0 * DN);
+      np[k] += DN;
+      nn[k] -= DN;
+    }
+    double g[3] = {0, 0, 0};
+    gradient(cur, g);
+    double den = 0;
+    for (int i = 0; i < D; ++i) den += g[i] * g[i];
+    for (int k = 0; k < D; ++k) {  // :341-364
+      double monti = 0, toifl = 0;
+      for (int j = 0; j < D - 1; ++j) {
+        const int idx = (k + 1 + j) % D;
+        monti += g[idx] * g[idx];
+        toifl += g[idx] * vd[idx];
+      }
+      monti *= vd[k] / den;
+      toifl *= -g[k] / den;
+      const double osher = lsv * ln[k];
+      maxAlpha[k] = smax(maxAlpha[k], std::fabs(monti + toifl + osher));  // :371-376
+    }
+    int dim = 0;  // incrementIndex (:212-220)
+    for (; dim < D - 1; ++dim) {
+      if (cur[dim] < 1) break;
+      cur[dim] = -1;
+    }
+    ++cur[dim];
+  }
+  double dissipation = 0.;
+  for (int d = 0; d < D; ++d) {  // :369-380
+    double v[3];
+    rowValues(zero, d, v);
+    const double gd = ((v[2] - v[1]) / s.dx - (v[1] - v[0]) / s.dx) * 0.5;  // calculateGradientDiff
+    finalAlphas[d] = smin(maxAlpha[d], SLLF_MAX_DISSIPATION);
+    dissipation += finalAlphas[d] * gd;
+  }
+  H = hamiltonian;
+  Dn = dissipation;
+  return true;
+}
+
 // scheme stencil width for prepareLS (lsAdvect.hpp:973-1015 and the
 // scheme prepareLS functions)
 static int prepareWidth(int spatial) {
@@ -895,6 +1021,7 @@ static int prepareWidth(int spatial) {
   case LLF1: return 7;  // lsLocalLaxFriedrichs.hpp:55 (2*(order+2)+1)
   case LLFA1: return 7; // lsLocalLaxFriedrichsAnalytical.hpp:51-55
   case LLF2: return 9;
+  case SLLF1: return 8; // lsStencilLocalLaxFriedrichsScalar.hpp:226 (2*(order+1)+4)
   case WENO3: return 5; // lsWENO.hpp:44 (2*r+1)
   case WENO5: case WENO5_LEARNED: return 7;
   default: return -1;
@@ -972,6 +1099,12 @@ template <int D> struct Advector {
     case LLLF1: case LLLF2: schemeLLLF<D>(s, c, order, H, Dn, alphaAcc); break;
     case LLFA1: schemeLLFA<D>(s, c, H, Dn, alphaAcc); break;
     case LLF1: case LLF2: schemeLLF<D>(s, c, order, H, Dn, alphaAcc); break;
+    case SLLF1: {  // the alphas of the last call that set them (calls run in point order)
+      double fa[3] = {0, 0, 0};
+      if (schemeSLLF<D>(s, c, H, Dn, fa))
+        for (int i = 0; i < 3; ++i) alphaAcc[i] = fa[i];
+      break;
+    }
     case WENO3: case WENO5: case WENO5_LEARNED: schemeWENO<D>(s, c, order, H, Dn); break;
     }
   }
@@ -1101,7 +1234,8 @@ template <int D> struct Advector {
     pointUpdates += active;
     const double *fa = nullptr;
     if (P.spatial == LF1 || P.spatial == LF2) fa = s.globalAlphas;
-    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2 || P.spatial == LLFA1)
+    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2 || P.spatial == LLFA1 ||
+        P.spatial == SLLF1)
       fa = alphaAcc;
     if (fa) {
       double timeStep = 0;
@@ -1144,9 +1278,17 @@ template <int D> struct Advector {
     ls = std::move(out);
   }
 
+  // Advect::rebuildLS with the scheme's cutoff and final width: 1.5 / 3 for
+  // SLLF, 1.0 / 2 otherwise (lsAdvect.hpp:248-256)
+  void rebuildStage(Band<D> &ls) {
+    if (P.spatial == SLLF1) rebuild(ls, 1.5, 3);
+    else rebuild(ls);
+  }
+
   // Advect::adjustLowerLayers (lsAdvect.hpp:802-813): every lower level set
   // intersected with the top (BooleanOperation INTERSECT = max + Prune)
   void adjustLowerLayers(const Band<D> &top) {
+    if (P.spatial == SLLF1) return;  // SLLF keeps the lower level sets (:804-805)
     for (Band<D> *b : lower) booleanOp(*b, top, [](double x, double y) { return maxComp(x, y); });
   }
 
@@ -1175,11 +1317,16 @@ template <int D> struct Advector {
     double tempMax = maxTimeStep;
     double alphaAcc[3] = {0, 0, 0};
     double active = 0;
+    // SLLF: finalAlphas of the last point in entry order that set them; a
+    // thread's dynamic chunks come in increasing entry order
+    int64_t lastE = -1;
 #pragma omp parallel
     {
       double locMax = maxTimeStep;
       double locAlpha[3] = {0, 0, 0};
       double locActive = 0;
+      int64_t locLastE = -1;
+      double locFA[3] = {0, 0, 0};
 #pragma omp for schedule(dynamic, 256)
       for (size_t e = 0; e < n; ++e) {
         double value = ls.val[e];
@@ -1193,6 +1340,14 @@ template <int D> struct Advector {
         case LLLF1: case LLLF2: schemeLLLF<D>(s, c, order, H, Dn, locAlpha); break;
         case LLFA1: schemeLLFA<D>(s, c, H, Dn, locAlpha); break;
         case LLF1: case LLF2: schemeLLF<D>(s, c, order, H, Dn, locAlpha); break;
+        case SLLF1: {
+          double fa[3] = {0, 0, 0};
+          if (schemeSLLF<D>(s, c, H, Dn, fa)) {
+            locLastE = (int64_t)e;
+            for (int i = 0; i < 3; ++i) locFA[i] = fa[i];
+          }
+          break;
+        }
         case WENO3: case WENO5: case WENO5_LEARNED: schemeWENO<D>(s, c, order, H, Dn); break;
         }
         // :530-588, single level set (no material below)
@@ -1213,6 +1368,10 @@ template <int D> struct Advector {
         if (locMax < tempMax) tempMax = locMax;
         for (int i = 0; i < 3; ++i) alphaAcc[i] = smax(alphaAcc[i], locAlpha[i]);
         active += locActive;
+        if (locLastE > lastE) {
+          lastE = locLastE;
+          for (int i = 0; i < 3; ++i) alphaAcc[i] = locFA[i];
+        }
       }
     }
     pointUpdates += active;
@@ -1220,7 +1379,8 @@ template <int D> struct Advector {
     // lsLocalLaxFriedrichs.hpp:243-254; LLLF: lsLocalLocalLaxFriedrichs.hpp:167-178)
     const double *fa = nullptr;
     if (P.spatial == LF1 || P.spatial == LF2) fa = s.globalAlphas;
-    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2 || P.spatial == LLFA1)
+    if (P.spatial == LLF1 || P.spatial == LLF2 || P.spatial == LLLF1 || P.spatial == LLLF2 || P.spatial == LLFA1 ||
+        P.spatial == SLLF1)
       fa = alphaAcc;
     if (fa) {
       double timeStep = 0;
@@ -1265,7 +1425,7 @@ template <int D> struct Advector {
       if (currentTimeStep < 0) return currentTimeStep;
       updateLevelSetMM(ls, currentTimeStep);
     }
-    rebuild(ls);
+    rebuildStage(ls);
     if (updateLowerLayers) adjustLowerLayers(ls);
     return currentTimeStep;
   }
@@ -1296,7 +1456,7 @@ template <int D> struct Advector {
       if (isUndef(a)) return a;
       return b;
     });
-    rebuild(ls);
+    rebuildStage(ls);
     return movedDown;
   }
 

@@ -25,7 +25,11 @@
 // Parity status (see the function comments for each citation):
 //   pinned by closed forms / invariants / fixture: weno3/weno5, EO/LF/LLF/LLLF
 //   and WENO Hamiltonians on planes, Expand/Reduce/Prune/Boolean/rebuild,
-//   run order and sentinel (BooleanOperationExactZero fixture).
+//   run order and sentinel (BooleanOperationExactZero fixture), the
+//   multi-material rate lists and updates, LLFA, SLLF (plane translation,
+//   last-point alphas on exact circles, Monti/Toifl closed form, the
+//   reference's SLLF sphere accuracy test).  SLLF's corrupted-normal exit is
+//   restated but no input of the tests reaches it.
 //   parity unpinned: reflective/periodic mirroring convention of HRLE's Grid
 //   (HRLE absent), FillDomainWithSignedDistance undefined-run signs.
 #pragma once

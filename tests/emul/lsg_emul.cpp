@@ -269,12 +269,12 @@ template <int D> void doPrune(EBand &d, bool rsz) {
   d = selfOp<D>(d, op, nullptr, 0);
   d.width = 2;
 }
-template <int D> void doRebuild(EBand &d) {
+template <int D> void doRebuild(EBand &d, bool sllf = false) {
   OpParams op{};
   op.kind = OP_REBUILD;
-  op.a = 1.0;
+  op.a = sllf ? 1.5 : 1.0;
   d = starOp<D>(d, op);
-  d.width = 2;
+  d.width = sllf ? 3 : 2;
 }
 template <int D> void doBoolean(EBand &a, const EBand &b, int kind, double wT, double wS) {
   a = booleanOp(a, b, kind, wT, wS);
@@ -334,7 +334,7 @@ void commonFrame(EBand &a, EBand &b) {
 }
 
 int prepareWidth(int spatial) {
-  static const int w[12] = {3, 5, 3, 5, -1, 3, 5, 7, 9, -1, 5, 7};
+  static const int w[12] = {3, 5, 3, 5, 7, 3, 5, 7, 9, 8, 5, 7};
   return (spatial >= 0 && spatial < 12) ? w[spatial] : -1;
 }
 int schemeOrder(int s) {
@@ -365,13 +365,14 @@ struct EAdvect {
       int32_t c[3];
       candCoords(v, i, d.pos[i], c);
       double velocity;
+      // SLLF leaves la at the finalAlphas of the last point that set them
       rates[i] = ratePoint<D, S>(v, c, phi0, rp, vel, ga, la, velocity);
       if (velocity != 0.) {
         double a = fabs(velocity);
         if (lmax < a) lmax = a;
       }
     }
-    const bool useAlphas = (S == S_LF || S == S_LLLF || S == S_LLF);
+    const bool useAlphas = (S == S_LF || S == S_LLLF || S == S_LLF || S == S_LLFA || S == S_SLLF);
     const double *al = (S == S_LF) ? ga : la;
     return stageTimeStep(maxTimeStep, rp.cfl, lmax, D, al, useAlphas, d.g.dx);
   }
@@ -397,6 +398,8 @@ struct EAdvect {
     case S_LF: dt = ratesPass<D, S_LF>(d, maxTimeStep, ga); break;
     case S_LLLF: dt = ratesPass<D, S_LLLF>(d, maxTimeStep, ga); break;
     case S_LLF: dt = ratesPass<D, S_LLF>(d, maxTimeStep, ga); break;
+    case S_LLFA: dt = ratesPass<D, S_LLFA>(d, maxTimeStep, ga); break;
+    case S_SLLF: dt = ratesPass<D, S_SLLF>(d, maxTimeStep, ga); break;
     case S_WENO3: dt = ratesPass<D, S_WENO3>(d, maxTimeStep, ga); break;
     default: dt = ratesPass<D, S_WENO5>(d, maxTimeStep, ga); break;
     }
@@ -404,7 +407,7 @@ struct EAdvect {
     op.kind = OP_UPDATE;
     d = selfOp<D>(d, op, &rates, dt);
     d.width = 1;
-    doRebuild<D>(d);
+    doRebuild<D>(d, rp.spatial == 9);
     return dt;
   }
 
@@ -414,7 +417,7 @@ struct EAdvect {
     doExpand<D>(d, ew);
     doExpand<D>(init, ew);
     doBoolean<D>(d, init, B_COMBINE, wT, wS);
-    doRebuild<D>(d);
+    doRebuild<D>(d, rp.spatial == 9);
   }
 
   template <int D> double step(EBand &d, double maxTimeStep) {
@@ -536,6 +539,7 @@ int emu_advect(emu_domain *d, const emu_params *p, double *out) {
   a.rp.cfl = p->cfl;
   a.rp.alphaFactor = p->dissipationAlpha;
   a.rp.checkDissipation = p->checkDissipation;
+  a.rp.sllfBaseEps = std::cbrt(std::numeric_limits<double>::epsilon());
   a.rp.calcNormals = (p->spatial == 10 || p->spatial == 11) ? (p->dissipationAlpha != 0.0) : p->calcNormals;
   a.rp.velKind = p->velKind;
   a.rp.velDir = p->velDir;

@@ -17,8 +17,7 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB):
-            build()
+        build()  # make is incremental: rebuilds after a change to the device headers
         L = ctypes.CDLL(_LIB)
         P = ctypes.c_void_p
         i32p = ctypes.POINTER(ctypes.c_int32)

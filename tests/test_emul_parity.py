@@ -74,6 +74,10 @@ ADVECT = [
     ("trench", 0, 0, cases.VEL_ETCH_DIRECTIONAL), ("periodic", 8, 2, cases.VEL_NORMAL_SQ),
     ("periodic", 11, 1, cases.VEL_NORMAL_SQ), ("sphere2d", 11, 2, dict(kind=0, b=1.0, vec=(0.5, 0.0, 0.0))),
     ("sphere2d", 8, 1, cases.VEL_GROW),
+    # stencil local Lax-Friedrichs (enum 9)
+    ("sphere", 9, 0, cases.VEL_EGG), ("sphere", 9, 2, cases.VEL_TRANSLATE_X),
+    ("trench", 9, 0, cases.VEL_ETCH_DIRECTIONAL), ("periodic", 9, 1, cases.VEL_NORMAL_SQ),
+    ("sphere2d", 9, 2, cases.VEL_GROW),
 ]
 
 

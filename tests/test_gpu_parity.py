@@ -192,6 +192,5 @@ def test_errors_are_reported():
     with pytest.raises(vls.LsgError, match="No velocity field"):
         a.apply()
     a.setVelocityField(vls.VelocityField.constant(1.0))
-    a.setSpatialScheme(9)
-    with pytest.raises(vls.LsgError, match="not supported"):
-        a.apply()
+    with pytest.raises(vls.LsgError, match="scheme not found"):
+        a.setSpatialScheme(13)

@@ -250,6 +250,9 @@ struct RateParams {
   double adaptiveFactor;  // 1 / subdivisions
   int matKind[8], matDir[8], matK[8];
   double matB[8], matAPos[8], matANeg[8], matVec[8][3], matAlpha[8][3];
+  // stencil local Lax-Friedrichs: cbrt(DBL_EPSILON), computed on the host
+  // (the base of the normal perturbation, lsStencilLocalLaxFriedrichsScalar.hpp:44-45)
+  double sllfBaseEps;
 };
 
 // Learned weno5 network: 5 -> 32 -> 32 -> 3 (GELU, softmax); parameter

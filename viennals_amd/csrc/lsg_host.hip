@@ -444,12 +444,14 @@ static void doPrune(lsg_domain *d, bool rsz) {  // lsPrune.hpp:74-194
   opSelf(d, op);
   d->width = 2;
 }
-static void doRebuild(lsg_domain *d) {  // lsAdvect.hpp:238-429
+// Advect::rebuildLS (lsAdvect.hpp:238-429): cutoff 1.0 and final width 2,
+// 1.5 and 3 for the stencil local Lax-Friedrichs scheme (:248-256)
+static void doRebuild(lsg_domain *d, bool sllf = false) {
   OpParams op{};
   op.kind = OP_REBUILD;
-  op.a = 1.0;
+  op.a = sllf ? 1.5 : 1.0;
   opStar(d, op);
-  d->width = 2;
+  d->width = sllf ? 3 : 2;
 }
 // booleanOpInternal + pruning (lsBooleanOperation.hpp:59-213)
 static void doBoolean(lsg_domain *a, const lsg_domain *b, int kind, double wT, double wS,
@@ -636,6 +638,7 @@ static int prepareWidth(int spatial) {
   case 4: return 7;   // LLFA1 (lsLocalLaxFriedrichsAnalytical.hpp:51-55)
   case 7: return 7;   // LLF1 (lsLocalLaxFriedrichs.hpp:55)
   case 8: return 9;   // LLF2
+  case 9: return 8;   // SLLF1 (lsStencilLocalLaxFriedrichsScalar.hpp:226, 2*(order+1)+4)
   case 10: return 5;  // WENO3 (lsWENO.hpp:44)
   case 11: return 7;  // WENO5
   case 12: return 7;  // WENO5 with learned weights (same stencil)
@@ -725,7 +728,7 @@ static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *d
   op.dt = dtOut;
   opSelf(d, op);
   d->width = 1;
-  doRebuild(d);
+  doRebuild(d, a->spatial == 9);
 }
 
 // Advect::combineLevelSets (lsAdvect.hpp:199-236)
@@ -737,7 +740,7 @@ static void combine(lsg_advect *a, lsg_domain *d, lsg_domain *init, double wTarg
   doExpand(d, ew);
   doExpand(init, ew);
   doBoolean(d, init, B_COMBINE, wTarget, wSource);
-  doRebuild(d);
+  doRebuild(d, a->spatial == 9);
 }
 
 // Copy without allocation (dst storage must already fit): usable in capture.
@@ -956,6 +959,7 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
 // flag on the host to decide whether the lower level sets are restored.
 
 static void adjustLowerLayers(lsg_advect *a) {  // lsAdvect.hpp:802-813
+  if (a->spatial == 9) return;  // the stencil local Lax-Friedrichs scheme keeps the lower level sets
   lsg_domain *top = a->levelSets.back();
   for (size_t i = 0; i + 1 < a->levelSets.size(); ++i) doBoolean(a->levelSets[i], top, B_INTERSECT, 0, 0);
 }
@@ -980,7 +984,7 @@ static void stageMM(lsg_advect *a, const double *maxDt, double *dtOut) {
   op.nmat = L;
   opSelf(top, op);
   top->width = 1;
-  doRebuild(top);
+  doRebuild(top, a->spatial == 9);
 }
 
 static void combineMM(lsg_advect *a, double wTarget, double wSource, uint32_t *movedDown) {
@@ -989,7 +993,7 @@ static void combineMM(lsg_advect *a, double wTarget, double wSource, uint32_t *m
   doExpand(top, ew);
   doExpand(a->init, ew);
   doBoolean(top, a->init, B_COMBINE, wTarget, wSource, movedDown);
-  doRebuild(top);
+  doRebuild(top, a->spatial == 9);
 }
 
 static int enqueueStepMM(lsg_advect *a) {
@@ -1088,8 +1092,7 @@ static int advectApply(lsg_advect *a) {
     if (a->hostScalar || a->hostVector)
       return fail(-3, "Advect: with several level sets only the built-in velocity functors are supported");
   }
-  if (prepareWidth(a->spatial) < 0)
-    return fail(-4, "Advect: Discretization scheme not supported on the GPU (scheme 9 is not implemented)");
+  if (prepareWidth(a->spatial) < 0) return fail(-4, "Advect: Discretization scheme not found.");
   if (a->ignoreVoids) return fail(-5, "Advect: setIgnoreVoids(true) is not supported");
   if (!(a->cfl > 0)) return fail(-6, "Advect: time step ratio must be positive");
   if (int e = ctxInit()) return e;
@@ -1101,6 +1104,7 @@ static int advectApply(lsg_advect *a) {
   a->rp.cfl = a->cfl;
   a->rp.alphaFactor = a->dissipationAlpha;
   a->rp.checkDissipation = a->checkDissipation;
+  a->rp.sllfBaseEps = std::cbrt(std::numeric_limits<double>::epsilon());
   // WENO receives dissipationAlpha as its calcNormal flag (lsAdvect.hpp:670-676)
   a->rp.calcNormals = (a->spatial >= 10) ? (a->dissipationAlpha != 0.0) : a->calcNormals;
   if ((a->hostScalar || a->hostVector) &&

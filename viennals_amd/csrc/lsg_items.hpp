@@ -1055,6 +1055,142 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
   return {totalGrad, dissipation};
 }
 
+// Values of the 3^D box around c, boundary mapped per coordinate of the
+// offset from the centre: box[(ox + 1) + 3 (oy + 1) + 9 (oz + 1)].  All
+// lookups are issued together.
+template <int D> LSG_HD inline void gatherBox(const BandView &b, const int32_t c[3], double *box) {
+  constexpr int NB = (D == 3) ? 27 : 9;
+  BandView::Loc l[NB];
+  const int64_t lin = interiorLin<D>(b, c, 1);
+  if (lin >= 0) {
+    const int64_t sy = axisStride<D>(b, 1), sz = (D == 3) ? axisStride<D>(b, 2) : 0;
+    LSG_UNROLL
+    for (int q = 0; q < NB; ++q) l[q] = locAt(lin + (q % 3 - 1) + ((q / 3) % 3 - 1) * sy + (q / 9 - 1) * sz);
+  } else {
+    LSG_UNROLL
+    for (int q = 0; q < NB; ++q) {
+      const int o[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
+      int32_t p[3] = {0, 0, 0};
+      LSG_UNROLL
+      for (int d = 0; d < D; ++d) p[d] = b.g.map(d, (int64_t)c[d] + o[d]);
+      l[q] = b.locate(p[0], p[1], p[2]);
+    }
+  }
+  b.values<NB>(l, box);
+}
+
+// StencilLocalLaxFriedrichsScalar<T,D,1>::operator()
+// (lsStencilLocalLaxFriedrichsScalar.hpp:238-383), first-order differences
+// and the simple adaptive epsilon (:62-74).  The normal and gradient at a
+// box offset differentiate along axis i over the row through the offset with
+// coordinate i at -1, 0, +1 (:142-186), so the 3^D box is the whole support.
+// A box point whose normal is below 1e-6 in every component ends the alpha
+// loop (its `continue` skips incrementIndex: the later slots stay zero).
+// `used` is set when the point sets finalAlphas (scalar velocity != 0).
+template <int D>
+LSG_HD inline RateOut schemeSLLF(const BandView &b, const int32_t c[3], const Vel &vel, double baseNormalEps,
+                                 double *finalAlphas, bool &used) {
+  constexpr int NB = (D == 3) ? 27 : 9;
+  double box[NB];
+  gatherBox<D>(b, c, box);
+  const double dx = b.g.dx;
+  used = false;
+#define LSG_BOX(x, y, z) box[((x) + 1) + 3 * ((y) + 1) + (D == 3 ? 9 * ((z) + 1) : 0)]
+  // FiniteDifferences<T, FIRST_ORDER>::calculateGradient (lsFiniteDifferences.hpp:177-181)
+  auto gradient = [&](int ox, int oy, int oz, double *g) {
+    g[0] = ((LSG_BOX(1, oy, oz) - LSG_BOX(0, oy, oz)) / dx + (LSG_BOX(0, oy, oz) - LSG_BOX(-1, oy, oz)) / dx) * 0.5;
+    g[1] = ((LSG_BOX(ox, 1, oz) - LSG_BOX(ox, 0, oz)) / dx + (LSG_BOX(ox, 0, oz) - LSG_BOX(ox, -1, oz)) / dx) * 0.5;
+    g[2] = 0.;
+    if (D == 3)
+      g[2] = ((LSG_BOX(ox, oy, 1) - LSG_BOX(ox, oy, 0)) / dx + (LSG_BOX(ox, oy, 0) - LSG_BOX(ox, oy, -1)) / dx) * 0.5;
+  };
+  // calculateNormal (:142-163)
+  auto normalAt = [&](int ox, int oy, int oz, double *n) {
+    double g[3];
+    gradient(ox, oy, oz, g);
+    double modulus = 0.;
+    LSG_UNROLL
+    for (int i = 0; i < D; ++i) modulus += g[i] * g[i];
+    modulus = 1. / sqrt(modulus);
+    n[0] = n[1] = n[2] = 0.;
+    LSG_UNROLL
+    for (int i = 0; i < D; ++i) n[i] = g[i] * modulus;
+  };
+  double normal[3];
+  normalAt(0, 0, 0, normal);
+  double sv = vel.scalar(normal);
+  LSG_UNROLL
+  for (int i = 0; i < D; ++i) sv += vel.vec[i] * normal[i];
+  if (sv == 0.) return {0., 0.};
+  double g0[3];
+  gradient(0, 0, 0, g0);
+  double dot = 0;
+  LSG_UNROLL
+  for (int i = 0; i < D; ++i) dot += g0[i] * g0[i];
+  const double hamiltonian = sqrt(dot) * sv;
+  double maxAlpha[3] = {0., 0., 0.};
+  const double baseEps = baseNormalEps * dx;
+  bool alive = true;  // cleared by the first box point with a corrupted normal
+  LSG_UNROLL
+  for (int q = 0; q < NB; ++q) {  // incrementIndex order (:212-220): x fastest
+    const int ox = q % 3 - 1, oy = (q / 3) % 3 - 1, oz = (D == 3) ? q / 9 - 1 : 0;
+    double ln[3];
+    normalAt(ox, oy, oz, ln);
+    alive = alive && !(fabs(ln[0]) < 1e-6 && fabs(ln[1]) < 1e-6 && fabs(ln[2]) < 1e-6);
+    if (!alive) continue;
+    double lsv = vel.scalar(ln);
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d) lsv += vel.vec[d] * ln[d];
+    const double absVel = fabs(lsv);
+    double DN = baseEps;
+    if (absVel > 1e-10) DN = baseEps * dmaxs(0.1, dmins(10.0, absVel));
+    double np[3] = {ln[0], ln[1], ln[2]}, nn[3] = {ln[0], ln[1], ln[2]};
+    double vd[3] = {0., 0., 0.};
+    LSG_UNROLL
+    for (int k = 0; k < D; ++k) {  // :324-338
+      np[k] -= DN;
+      nn[k] += DN;
+      const double vp = vel.scalar(np);
+      const double vn = vel.scalar(nn);
+      vd[k] = (vn - vp) / (2.0 * DN);
+      np[k] += DN;
+      nn[k] -= DN;
+    }
+    double g[3];
+    gradient(ox, oy, oz, g);
+    double den = 0;
+    LSG_UNROLL
+    for (int i = 0; i < D; ++i) den += g[i] * g[i];
+    LSG_UNROLL
+    for (int k = 0; k < D; ++k) {  // :341-364
+      double monti = 0, toifl = 0;
+      LSG_UNROLL
+      for (int j = 0; j < D - 1; ++j) {
+        const int idx = (k + 1 + j) % D;
+        monti += g[idx] * g[idx];
+        toifl += g[idx] * vd[idx];
+      }
+      monti *= vd[k] / den;
+      toifl *= -g[k] / den;
+      const double osher = lsv * ln[k];
+      maxAlpha[k] = dmaxs(maxAlpha[k], fabs(monti + toifl + osher));  // :371-376
+    }
+  }
+  double dissipation = 0.;
+  LSG_UNROLL
+  for (int d = 0; d < D; ++d) {  // :369-380, calculateGradientDiff at the centre
+    const double v0 = (d == 0) ? LSG_BOX(-1, 0, 0) : (d == 1) ? LSG_BOX(0, -1, 0) : LSG_BOX(0, 0, -1);
+    const double v2 = (d == 0) ? LSG_BOX(1, 0, 0) : (d == 1) ? LSG_BOX(0, 1, 0) : LSG_BOX(0, 0, 1);
+    const double v1 = LSG_BOX(0, 0, 0);
+    const double gd = ((v2 - v1) / dx - (v1 - v0) / dx) * 0.5;
+    finalAlphas[d] = dmins(maxAlpha[d], 1.7976931348623157e308);  // min(maxAlpha, maxDissipation)
+    dissipation += finalAlphas[d] * gd;
+  }
+#undef LSG_BOX
+  used = true;
+  return {hamiltonian, dissipation};
+}
+
 // Normal vector a scheme passes to VelocityField::getScalarVelocity at an
 // active point (host-evaluated velocities): EO without a zero guard
 // (lsEngquistOsher.hpp:124-138), WENO with one (lsWENO.hpp:113-129), both
@@ -1109,7 +1245,9 @@ template <int D> LSG_HD inline void schemeNormal(const BandView &b, const int32_
 }
 
 // Scheme ids of the rate kernel.
-enum SchemeId : int { S_EO = 0, S_WENO5 = 1, S_LF = 2, S_LLLF = 3, S_LLF = 4, S_WENO3 = 5, S_WENO5L = 6, S_LLFA = 7 };
+enum SchemeId : int {
+  S_EO = 0, S_WENO5 = 1, S_LF = 2, S_LLLF = 3, S_LLF = 4, S_WENO3 = 5, S_WENO5L = 6, S_LLFA = 7, S_SLLF = 8
+};
 
 LSG_HD inline int schemeIdOf(int spatial) {
   switch (spatial) {
@@ -1118,6 +1256,7 @@ LSG_HD inline int schemeIdOf(int spatial) {
   case 4: return S_LLFA;
   case 5: case 6: return S_LLLF;
   case 7: case 8: return S_LLF;
+  case 9: return S_SLLF;
   case 10: return S_WENO3;
   case 12: return S_WENO5L;
   default: return S_WENO5;
@@ -1126,13 +1265,18 @@ LSG_HD inline int schemeIdOf(int spatial) {
 
 // Hamiltonian and dissipation of one active point; returns the stage
 // velocity after updateLevelSet's checkDissipation rule (lsAdvect.hpp:738-744,
-// precedence kept) and the raw H - D through `velocity`.
+// precedence kept) and the raw H - D through `velocity`.  SLLF writes the
+// point's finalAlphas to alphaAcc and sets *sllfUsed when it set them.
 template <int D, int SCHEME>
 LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi0, const RateParams &rp, const Vel &vel,
                                const double *globalAlphas, double *alphaAcc, double &velocity,
-                               const float *mlp = nullptr, float *io = nullptr) {
+                               const float *mlp = nullptr, float *io = nullptr, bool *sllfUsed = nullptr) {
   RateOut r;
-  if (SCHEME == S_EO) r = schemeEO<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel);
+  if (SCHEME == S_SLLF) {
+    bool used = false;
+    r = schemeSLLF<D>(b, c, vel, rp.sllfBaseEps, alphaAcc, used);
+    if (sllfUsed) *sllfUsed = used;
+  } else if (SCHEME == S_EO) r = schemeEO<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO5) r = schemeWENO<D, 5>(b, c, phi0, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO3) r = schemeWENO<D, 3>(b, c, phi0, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO5L) r = schemeWENO<D, 5, true>(b, c, phi0, rp.calcNormals != 0, vel, mlp, io);

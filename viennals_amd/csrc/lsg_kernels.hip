@@ -719,6 +719,7 @@ __global__ __launch_bounds__(256) void kActiveCount(const double *val, const uin
   LSG_ABORT_IF_OVERFLOW(sc);
   if (resetRates && logicalBlock() == 0 && threadIdx.x == 0) {  // the rate pass starts from zero maxima
     sc->maxAbsBits = 0;
+    sc->sllfKey = 0;
     if (!keepAlphas) sc->alphaBits[0] = sc->alphaBits[1] = sc->alphaBits[2] = 0;
     sc->activeCount = 0;
   }
@@ -825,6 +826,7 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
   SegArgs sa;
   unsigned long long *segMax;
   uint32_t curSeg;
+  unsigned long long sllfKey;  // SLLF: key of this lane's last active point that set its alphas
   __device__ __attribute__((always_inline)) void operator()(uint32_t i, uint32_t j) {
     ++act;
     BandView v = b;
@@ -850,7 +852,9 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
     int32_t c[3];
     candCoords(v, i, v.pos[i], c);
     double velocity;
-    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp, io);
+    bool used = false;
+    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp, io, &used);
+    if (SCHEME == S_SLLF && used) sllfKey = (unsigned long long)(j + 1) << 4;  // lanes visit j in increasing order
     if (velocity != 0.) {
       const double a = fabs(velocity);
       if (lmax < a) lmax = a;
@@ -875,7 +879,7 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
   const uint32_t segFirst = (SEG && lo < hi) ? sa.iseg[list[lo]] : 0u;
   __shared__ float mlpIO[SCHEME == S_WENO5L ? TPB * 2 * D * MLP_IN : 1];
   float *io = (SCHEME == S_WENO5L) ? &mlpIO[threadIdx.x * 2 * D * MLP_IN] : nullptr;
-  RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, io, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst};
+  RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, io, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst, 0ull};
   if (SCHEME == S_LF)
     for (int i = 0; i < 3; ++i) fn.ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
   if (SEG) {
@@ -901,6 +905,7 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
     blockMaxAtomic((unsigned long long)__double_as_longlong(fn.lmax), &sc->maxAbsBits);
     if (SCHEME == S_LLLF || SCHEME == S_LLF || SCHEME == S_LLFA)
       for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(fn.la[d]), &sc->alphaBits[d]);
+    if (SCHEME == S_SLLF) blockMaxAtomic(fn.sllfKey, &sc->sllfKey);
   } else {
     // the level sets of a block's chunk are few and consecutive: reduce the
     // lanes' maxima in LDS slots, then one atomic per level set
@@ -921,6 +926,29 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
   if (threadIdx.x == 0 && t) atomicAdd(&sc->activeCount, t);
 }
 
+// StencilLocalLaxFriedrichsScalar::reduceTimeStepHamiltonJacobi
+// (lsStencilLocalLaxFriedrichsScalar.hpp:385-397) uses the finalAlphas of the
+// last point (entry order) whose scheme call set them: that call is evaluated
+// again here (same inputs, same result) and its alphas become the stage's.
+template <int D>
+__global__ void kSllfAlphas(BandView b, const uint32_t *list, RateParams rp, DevScalars *sc, int severalLevelSets) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const unsigned long long key = sc->sllfKey;
+  double fa[3] = {0., 0., 0.};
+  if (key) {
+    const uint32_t j = (uint32_t)(key >> 4) - 1u;
+    const int mat = (int)(key & 15u);
+    const uint32_t i = list[j];
+    int32_t c[3];
+    candCoords(b, i, b.pos[i], c);
+    const Vel base = velOf(rp);
+    const Vel vel = severalLevelSets ? velOfMaterial(rp, base, mat) : base;
+    bool used = false;
+    (void)schemeSLLF<D>(b, c, vel, rp.sllfBaseEps, fa, used);
+  }
+  for (int d = 0; d < 3; ++d) sc->alphaBits[d] = (unsigned long long)__double_as_longlong(fa[d]);
+}
+
 __global__ void kTimeStep(DevScalars *sc, RateParams rp, int D, double dx, const double *maxTimeStepPtr, int useAlphas,
                           double *dtOut, const uint32_t *nb) {
   LSG_ABORT_IF_OVERFLOW(sc);
@@ -936,6 +964,7 @@ __global__ void kTimeStep(DevScalars *sc, RateParams rp, int D, double dx, const
 
 __global__ void kResetRates(DevScalars *sc, int keepAlphas) {
   sc->maxAbsBits = 0;
+  sc->sllfKey = 0;
   if (!keepAlphas) sc->alphaBits[0] = sc->alphaBits[1] = sc->alphaBits[2] = 0;
   sc->activeCount = 0;
 }
@@ -967,6 +996,7 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
     case S_LLLF: LSG_RATES(2, S_LLLF); break;
     case S_LLFA: LSG_RATES(2, S_LLFA); break;
     case S_LLF: LSG_RATES(2, S_LLF); break;
+    case S_SLLF: LSG_RATES(2, S_SLLF); break;
     default: LSG_RATES(2, S_WENO3); break;
     }
   } else {
@@ -978,12 +1008,18 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
     case S_LLLF: LSG_RATES(3, S_LLLF); break;
     case S_LLFA: LSG_RATES(3, S_LLFA); break;
     case S_LLF: LSG_RATES(3, S_LLF); break;
+    case S_SLLF: LSG_RATES(3, S_SLLF); break;
     default: LSG_RATES(3, S_WENO3); break;
     }
   }
 #undef LSG_RATES
   recordEvent(s, e1, externalEvents);
-  const int useAlphas = (scheme == S_LF || scheme == S_LLLF || scheme == S_LLF || scheme == S_LLFA) ? 1 : 0;
+  if (scheme == S_SLLF) {
+    if (D == 2) kSllfAlphas<2><<<1, 1, 0, s>>>(v, ws.cpos, rp, sc, 0);
+    else kSllfAlphas<3><<<1, 1, 0, s>>>(v, ws.cpos, rp, sc, 0);
+  }
+  const int useAlphas =
+      (scheme == S_LF || scheme == S_LLLF || scheme == S_LLF || scheme == S_LLFA || scheme == S_SLLF) ? 1 : 0;
   kTimeStep<<<1, 1, 0, s>>>(sc, rp, D, g.dx, maxTimeStep, useAlphas, dtOut, b.n);
 }
 
@@ -1007,6 +1043,7 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
   double ga[3] = {0, 0, 0}, la[3] = {0, 0, 0};
   double minStep = MAXV;
   unsigned long long act = 0;
+  unsigned long long sllfKey = 0;  // SLLF: (j + 1) << 4 | material of this thread's last call that set alphas
   for (uint32_t j = blockIdx.x * TPB + threadIdx.x; j < n; j += gridDim.x * TPB) {
     const uint32_t i = list[j];
     int32_t c[3];
@@ -1040,7 +1077,9 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
       double velocity = 0., v = 0.;
       if (mat >= 0) {
         const Vel vm = velOfMaterial(rp, base, mat);
-        v = ratePoint<D, SCHEME>(b, c, phi0, rp, vm, ga, la, velocity);
+        bool used = false;
+        v = ratePoint<D, SCHEME>(b, c, phi0, rp, vm, ga, la, velocity, nullptr, nullptr, &used);
+        if (SCHEME == S_SLLF && used) sllfKey = ((unsigned long long)(j + 1) << 4) | (unsigned long long)mat;
       }
       double stop;
       bool done = true;
@@ -1095,6 +1134,7 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
   }
   if (SCHEME == S_LLLF || SCHEME == S_LLF || SCHEME == S_LLFA)
     for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(la[d]), &sc->alphaBits[d]);
+  if (SCHEME == S_SLLF) blockMaxAtomic(sllfKey, &sc->sllfKey);
   unsigned long long t = blockSumU64(act);
   if (threadIdx.x == 0 && t) atomicAdd(&sc->activeCount, t);
 }
@@ -1127,6 +1167,7 @@ void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, cons
     case S_LLLF: LSG_RATES_MM(2, S_LLLF); break;
     case S_LLFA: LSG_RATES_MM(2, S_LLFA); break;
     case S_LLF: LSG_RATES_MM(2, S_LLF); break;
+    case S_SLLF: LSG_RATES_MM(2, S_SLLF); break;
     default: LSG_RATES_MM(2, S_WENO3); break;
     }
   } else {
@@ -1136,11 +1177,16 @@ void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, cons
     case S_LLLF: LSG_RATES_MM(3, S_LLLF); break;
     case S_LLFA: LSG_RATES_MM(3, S_LLFA); break;
     case S_LLF: LSG_RATES_MM(3, S_LLF); break;
+    case S_SLLF: LSG_RATES_MM(3, S_SLLF); break;
     default: LSG_RATES_MM(3, S_WENO3); break;
     }
   }
 #undef LSG_RATES_MM
-  const int useAlphas = (scheme == S_LLLF || scheme == S_LLF || scheme == S_LLFA) ? 1 : 0;
+  if (scheme == S_SLLF) {
+    if (D == 2) kSllfAlphas<2><<<1, 1, 0, s>>>(v, ws.cpos, rp, sc, 1);
+    else kSllfAlphas<3><<<1, 1, 0, s>>>(v, ws.cpos, rp, sc, 1);
+  }
+  const int useAlphas = (scheme == S_LLLF || scheme == S_LLF || scheme == S_LLFA || scheme == S_SLLF) ? 1 : 0;
   kTimeStepMM<<<1, 1, 0, s>>>(sc, D, g.dx, maxTimeStep, useAlphas, dtOut);
 }
 

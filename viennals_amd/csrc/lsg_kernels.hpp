@@ -30,6 +30,9 @@ struct DevScalars {
   unsigned long long minStepBits;  // several level sets: min over points of the step limit (bits of a double >= 0)
   uint32_t movedDown;              // combine: the weighted value exceeded the start value somewhere
   unsigned long long stepActive0;  // totalActive at the start of the step
+  // stencil local Lax-Friedrichs: ((j + 1) << 4) | material of the last
+  // active point j (active-list order) whose scheme call set its alphas
+  unsigned long long sllfKey;
 };
 
 // Mutable device buffers of a band.
