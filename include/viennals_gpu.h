@@ -87,6 +87,25 @@ int lsg_domain_num_points(lsg_domain *d, int64_t *n);
 int lsg_domain_width(lsg_domain *d, int *width);
 int lsg_domain_set_width(lsg_domain *d, int width); /* Domain::setLevelSetWidth (lsDomain.hpp:160) */
 
+/* Domain::getPointData (lsDomain.hpp:44, 166-168; PointData of ViennaCore):
+ * named arrays of `components` = 1 (scalar) or 3 (vector) doubles per
+ * defined point, in point-id order, kept on the device and translated by
+ * every band operation (the newDataSourceIds of Expand, Reduce, Prune,
+ * BooleanOperation and Advect::rebuildLS).  insert: append
+ * (insertNextScalarData / insertNextVectorData, replace = 0) or replace the
+ * first array of that label (insertReplaceScalarData, replace = 1); n tuples.
+ * count / label / size / get address the arrays of one kind by index in
+ * insertion order (label returns the label's length, negative on error);
+ * size is the tuple count after the last translation (every array holds
+ * one tuple per recorded source then). */
+int lsg_domain_point_data_insert(lsg_domain *d, const char *label, int components, const double *values, int64_t n,
+                                 int replace);
+int lsg_domain_point_data_count(const lsg_domain *d, int components);
+int lsg_domain_point_data_label(const lsg_domain *d, int components, int i, char *buf, int len);
+int64_t lsg_domain_point_data_size(const lsg_domain *d, int components, int i);
+int lsg_domain_point_data_get(const lsg_domain *d, int components, int i, double *out, int64_t n);
+int lsg_domain_point_data_clear(lsg_domain *d);
+
 /* ---- MakeGeometry (lsMakeGeometry.hpp) --------------------------------- */
 /* makeSphere (lsMakeGeometry.hpp:177-241). */
 int lsg_make_sphere(lsg_domain *d, const double *origin, double radius);
@@ -155,7 +174,10 @@ int lsg_advect_set_check_dissipation(lsg_advect *a, int check);   /* :958 */
 int lsg_advect_set_calculate_normal_vectors(lsg_advect *a, int c);/* :891 */
 int lsg_advect_set_ignore_voids(lsg_advect *a, int iv);           /* :899; only 0 supported */
 int lsg_advect_set_adaptive_time_stepping(lsg_advect *a, int on, unsigned subdivisions); /* :904; acts with several level sets */
-int lsg_advect_set_update_point_data(lsg_advect *a, int update);  /* :962; point data is not carried */
+int lsg_advect_set_update_point_data(lsg_advect *a, int update);  /* :962; rebuildLS translates or drops point data */
+/* Advect::setSaveAdvectionVelocities (lsAdvect.hpp:916): point data
+ * "AdvectionVelocities" and "Dissipation" after every stage (:762-796). */
+int lsg_advect_set_save_advection_velocities(lsg_advect *a, int save);
 /* Advect::prepareLS (lsAdvect.hpp:973-1015). */
 int lsg_advect_prepare_ls(lsg_advect *a);
 /* Advect::apply (lsAdvect.hpp:1017-1043). */

@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -78,14 +79,85 @@ enum struct BooleanOperationEnum : unsigned { INTERSECT = 0, UNION = 1, RELATIVE
 
 // ------------------------------------------------------------- Domain ---
 
+// Point data of a Domain (ViennaCore PointData, as used by lsDomain.hpp:44):
+// named scalar arrays (one value per defined point, point-id order) and
+// vector arrays (three values per point), resident on the device and carried
+// through every band operation and Advect.  The getters return copies.
+template <class T> class PointData {
+  lsg_domain *h_;
+
+public:
+  using ScalarDataType = std::vector<T>;
+  using VectorDataType = std::vector<std::array<T, 3>>;
+  explicit PointData(lsg_domain *h) : h_(h) {}
+
+  void insertNextScalarData(const ScalarDataType &s, const std::string &label = "Scalars") { put(s.data(), s.size(), 1, label, 0); }
+  void insertNextVectorData(const VectorDataType &v, const std::string &label = "Vectors") { put(v.data()->data(), v.size(), 3, label, 0); }
+  void insertReplaceScalarData(const ScalarDataType &s, const std::string &label = "Scalars") { put(s.data(), s.size(), 1, label, 1); }
+  void insertReplaceVectorData(const VectorDataType &v, const std::string &label = "Vectors") { put(v.data()->data(), v.size(), 3, label, 1); }
+  unsigned getScalarDataSize() const { return (unsigned)lsg_domain_point_data_count(h_, 1); }
+  unsigned getVectorDataSize() const { return (unsigned)lsg_domain_point_data_count(h_, 3); }
+  std::string getScalarDataLabel(int i) const { return label(1, i); }
+  std::string getVectorDataLabel(int i) const { return label(3, i); }
+  std::shared_ptr<ScalarDataType> getScalarData(int i) const {
+    auto out = std::make_shared<ScalarDataType>();
+    if (!fetch(1, i, out->data(), 0, out.get())) return nullptr;
+    return out;
+  }
+  std::shared_ptr<ScalarDataType> getScalarData(const std::string &l, bool noWarning = false) const {
+    const int i = find(1, l, noWarning);
+    return i < 0 ? nullptr : getScalarData(i);
+  }
+  std::shared_ptr<VectorDataType> getVectorData(int i) const {
+    auto out = std::make_shared<VectorDataType>();
+    std::vector<T> flat;
+    if (!fetch(3, i, nullptr, 0, &flat)) return nullptr;
+    out->resize(flat.size() / 3);
+    for (size_t k = 0; k < out->size(); ++k) (*out)[k] = {flat[3 * k], flat[3 * k + 1], flat[3 * k + 2]};
+    return out;
+  }
+  std::shared_ptr<VectorDataType> getVectorData(const std::string &l, bool noWarning = false) const {
+    const int i = find(3, l, noWarning);
+    return i < 0 ? nullptr : getVectorData(i);
+  }
+  void clear() { detail::ok(lsg_domain_point_data_clear(h_), "PointData::clear"); }
+  bool empty() const { return getScalarDataSize() == 0 && getVectorDataSize() == 0; }
+
+private:
+  void put(const T *p, size_t n, int comps, const std::string &label, int replace) {
+    detail::ok(lsg_domain_point_data_insert(h_, label.c_str(), comps, p, (int64_t)n, replace), "PointData::insert");
+  }
+  std::string label(int comps, int i) const {
+    char buf[1024] = {0};
+    const int n = lsg_domain_point_data_label(h_, comps, i, buf, sizeof(buf));
+    if (n < 0) detail::ok(n, "PointData::label");
+    return buf;
+  }
+  int find(int comps, const std::string &l, bool noWarning) const {
+    const int n = lsg_domain_point_data_count(h_, comps);
+    for (int i = 0; i < n; ++i)
+      if (label(comps, i) == l) return i;
+    if (!noWarning) std::fprintf(stderr, "[viennals_gpu] PointData: no data with label %s\n", l.c_str());
+    return -1;
+  }
+  bool fetch(int comps, int i, T *, int, std::vector<T> *out) const {
+    const int64_t n = lsg_domain_point_data_size(h_, comps, i);
+    if (n < 0) return false;
+    out->resize((size_t)n * comps);
+    return detail::ok(lsg_domain_point_data_get(h_, comps, i, out->data(), n), "PointData::get");
+  }
+};
+
 template <class T, int D> class Domain {
   static_assert(std::is_same<T, double>::value, "the GPU path computes in double precision");
   static_assert(D == 2 || D == 3, "D must be 2 or 3");
   lsg_domain *h_ = nullptr;
   double gridDelta_ = 1.0;
+  PointData<T> pointData_{nullptr};
 
 public:
   using BoundaryType = BoundaryConditionEnum;
+  using PointDataType = PointData<T>;
   using PointValueVectorType = std::vector<std::pair<std::array<int32_t, D>, T>>;
   static constexpr T POS_VALUE = 1.7976931348623157e308;
   static constexpr T NEG_VALUE = -1.7976931348623157e308;
@@ -96,6 +168,7 @@ public:
     int bc[D];
     for (int i = 0; i < D; ++i) bc[i] = LSG_INFINITE;
     detail::ok(lsg_domain_create(D, lo, hi, bc, gridDelta, &h_), "Domain");
+    pointData_ = PointData<T>(h_);
   }
   // Domain(bounds, boundaryConditions, gridDelta) (lsDomain.hpp:68-78)
   Domain(const double *bounds, const BoundaryType *bcs, double gridDelta = 1.0) : gridDelta_(gridDelta) {
@@ -107,6 +180,7 @@ public:
       bc[i] = (int)bcs[i];
     }
     detail::ok(lsg_domain_create(D, lo, hi, bc, gridDelta, &h_), "Domain");
+    pointData_ = PointData<T>(h_);
   }
   ~Domain() {
     if (h_) lsg_domain_destroy(h_);
@@ -120,6 +194,8 @@ public:
 
   lsg_domain *handle() const { return h_; }
   double getGridDelta() const { return gridDelta_; }
+  // Domain::getPointData (lsDomain.hpp:166-168)
+  PointData<T> &getPointData() { return pointData_; }
 
   // Domain::deepCopy (lsDomain.hpp:124-129)
   void deepCopy(const SmartPointer<Domain> &other) { detail::ok(lsg_domain_deep_copy(h_, other->h_), "deepCopy"); }
@@ -452,7 +528,9 @@ public:
   void setAdaptiveTimeStepping(bool on = true, unsigned sub = 20) {
     detail::ok(lsg_advect_set_adaptive_time_stepping(h_, on ? 1 : 0, sub), "setAdaptiveTimeStepping");
   }
-  void setSaveAdvectionVelocities(bool) {}
+  void setSaveAdvectionVelocities(bool s) {
+    detail::ok(lsg_advect_set_save_advection_velocities(h_, s ? 1 : 0), "setSaveAdvectionVelocities");
+  }
   void setSpatialScheme(SpatialSchemeEnum s) { detail::ok(lsg_advect_set_spatial_scheme(h_, (int)s), "setSpatialScheme"); }
   [[deprecated("Use setSpatialScheme instead")]] void setIntegrationScheme(IntegrationSchemeEnum s) { setSpatialScheme(s); }
   void setTemporalScheme(TemporalSchemeEnum s) {

@@ -216,6 +216,80 @@ static void pointOp(const Band<D> &in, Band<D> &out, OpMode mode, F &&func) {
   compressInto<D>(out, cand, vals);
 }
 
+// ----------------------------------------------------------- point data ---
+
+// Point id of every entry: the number of defined entries before it (HRLE
+// iteration order; Appendix of the band model: pointId = rank of a defined
+// point).
+template <int D> static std::vector<int64_t> definedRanks(const Band<D> &b) {
+  std::vector<int64_t> dr(b.idx.size() + 1, 0);
+  for (size_t e = 0; e < b.idx.size(); ++e) dr[e + 1] = dr[e] + (isUndef(b.val[e]) ? 0 : 1);
+  return dr;
+}
+// Point id of grid point p (already boundary mapped), -1 if p is not defined.
+template <int D> static int64_t pointIdOf(const Band<D> &b, const std::vector<int64_t> &dr, const Idx<D> &p) {
+  auto it = std::lower_bound(b.idx.begin(), b.idx.end(), p, idxLess<D>);
+  if (it == b.idx.end() || *it != p) return -1;
+  const size_t e = (size_t)(it - b.idx.begin());
+  return isUndef(b.val[e]) ? -1 : dr[e];
+}
+
+// PointData::translateFromMultiData (ViennaCore PointData): the new arrays
+// hold, in order, the old value of every listed source id, with the old
+// labels.  An id past the end of an old array gives 0 (the reference reads
+// out of range there; it can only happen after Prune's stray-zero rewrite,
+// which inserts a point without recording a source, lsPrune.hpp:145-157).
+static PointData translate(const PointData &src, const std::vector<int64_t> &ids) {
+  PointData out;
+  out.sLabel = src.sLabel;
+  out.vLabel = src.vLabel;
+  for (const auto &a : src.s) {
+    std::vector<double> n(ids.size(), 0.0);
+    for (size_t k = 0; k < ids.size(); ++k)
+      if (ids[k] >= 0 && (size_t)ids[k] < a.size()) n[k] = a[(size_t)ids[k]];
+    out.s.push_back(std::move(n));
+  }
+  for (const auto &a : src.v) {
+    std::vector<std::array<double, 3>> n(ids.size(), std::array<double, 3>{0, 0, 0});
+    for (size_t k = 0; k < ids.size(); ++k)
+      if (ids[k] >= 0 && (size_t)ids[k] < a.size()) n[k] = a[(size_t)ids[k]];
+    out.v.push_back(std::move(n));
+  }
+  return out;
+}
+
+// newDataSourceIds of a star / self band operation: for every defined entry
+// of `out` in order, sel(c, nb) names the point whose data it takes -- 2D for
+// the centre, k for star neighbour k (nb order of pointOp), -1 for none (no
+// id is recorded) -- evaluated on `in` at that point.
+template <int D, class S>
+static std::vector<int64_t> sourceIds(const Band<D> &in, const Band<D> &out, S &&sel) {
+  const std::vector<int64_t> dr = definedRanks(in);
+  std::vector<int64_t> ids;
+  ids.reserve(out.idx.size());
+  for (size_t e = 0; e < out.idx.size(); ++e) {
+    if (isUndef(out.val[e])) continue;
+    const Idx<D> &p = out.idx[e];
+    double nb[2 * D];
+    for (int k = 0; k < D; ++k) {
+      Idx<D> q = p;
+      q[k] = in.g.map(k, (int64_t)p[k] + 1);
+      nb[k] = in.value(q);
+      q[k] = in.g.map(k, (int64_t)p[k] - 1);
+      nb[D + k] = in.value(q);
+    }
+    const int s = sel(in.value(p), nb);
+    if (s < 0) continue;
+    Idx<D> q = p;
+    if (s < 2 * D) {
+      const int k = s % D;
+      q[k] = in.g.map(k, (int64_t)p[k] + (s < D ? 1 : -1));
+    }
+    ids.push_back(pointIdOf<D>(in, dr, q));
+  }
+  return ids;
+}
+
 // ------------------------------------------------------------ band ops ---
 
 // Expand<T,D>::apply (lsExpand.hpp:44-172)
@@ -246,6 +320,27 @@ template <int D> void expand(Band<D> &ls, int width) {
         return (dist >= -limit) ? dist : NEG;
       }
     });
+    if (!ls.pd.empty())  // newDataSourceIds (:109-150): the centre or the chosen neighbour
+      out.pd = translate(ls.pd, sourceIds<D>(ls, out, [&](double c, const double *nb) -> int {
+        if (std::fabs(c) <= totalLimit) return 2 * D;
+        int k = -1;
+        if (c > -EPS) {
+          double dist = POS;
+          for (int i = 0; i < 2 * D; ++i)
+            if (dist > nb[i] + 1.0) {
+              dist = nb[i] + 1.0;
+              k = i;
+            }
+        } else {
+          double dist = NEG;
+          for (int i = 0; i < 2 * D; ++i)
+            if (dist < nb[i] - 1.0) {
+              dist = nb[i] - 1.0;
+              k = i;
+            }
+        }
+        return k;
+      }));
     out.width = ls.width;
     ls = std::move(out);
   }
@@ -261,6 +356,7 @@ template <int D> void reduce(Band<D> &ls, int width) {
     if (!isUndef(c) && std::fabs(c) <= valueLimit) return c;  // :107-111
     return (c < 0) ? NEG : POS;                                // :113-117
   });
+  if (!ls.pd.empty()) out.pd = translate(ls.pd, sourceIds<D>(ls, out, [](double, const double *) { return 2 * D; }));
   out.width = width;
   ls = std::move(out);
 }
@@ -300,13 +396,23 @@ template <int D> void prune(Band<D> &ls, bool removeStrayZeros) {
     }
     return centerSign ? NEG : POS;  // :168-179
   });
+  if (!ls.pd.empty())  // a kept point records its centre; a stray-zero rewrite records nothing (:145-167)
+    out.pd = translate(ls.pd, sourceIds<D>(ls, out, [&](double c, const double *nb) -> int {
+      if (removeStrayZeros && std::fabs(c) == 0.)
+        for (int i = 0; i < D; ++i)
+          if (!pMonotone(nb[i], nb[D + i])) return -1;
+      return 2 * D;
+    }));
   out.width = 2;  // :193
   ls = std::move(out);
 }
 
 // BooleanOperation::booleanOpInternal (lsBooleanOperation.hpp:59-213) on the
-// union of both run-start lists; comp(a, b) returns the new value.
-template <int D, class C> void booleanOp(Band<D> &A, const Band<D> &B, C &&comp, bool pruneResult = true) {
+// union of both run-start lists; comp(a, b) returns the new value, fromA(a, b)
+// whether it is taken from A (the comparator's second member) -- the point
+// data of a result point comes from that level set (:146-196).
+template <int D, class C, class S>
+void booleanOp(Band<D> &A, const Band<D> &B, C &&comp, S &&fromA, bool pruneResult = true) {
   const Grid<D> &g = A.g;
   std::vector<Idx<D>> cand;
   cand.reserve(A.idx.size() + B.idx.size());
@@ -324,6 +430,47 @@ template <int D, class C> void booleanOp(Band<D> &A, const Band<D> &B, C &&comp,
   Band<D> out;
   out.g = g;
   compressInto<D>(out, cand, vals);
+  if (!A.pd.empty()) {
+    const std::vector<int64_t> drA = definedRanks(A), drB = definedRanks(B);
+    std::vector<int64_t> ids;
+    std::vector<char> side;
+    for (size_t e = 0; e < out.idx.size(); ++e) {
+      if (isUndef(out.val[e])) continue;
+      const Idx<D> &p = out.idx[e];
+      const bool a = fromA(A.value(p), B.value(p));
+      side.push_back(a ? 1 : 0);
+      ids.push_back(a ? pointIdOf<D>(A, drA, p) : pointIdOf<D>(B, drB, p));
+    }
+    auto at = [](const auto &arr, int64_t id, auto zero) { return (id >= 0 && (size_t)id < arr.size()) ? arr[(size_t)id] : zero; };
+    // A's arrays; a B-sourced point takes B's array of the same label, else 0 (:156-196)
+    for (size_t i = 0; i < A.pd.s.size(); ++i) {
+      const std::vector<double> *bArr = nullptr;
+      for (size_t j = 0; j < B.pd.sLabel.size(); ++j)
+        if (B.pd.sLabel[j] == A.pd.sLabel[i]) {
+          bArr = &B.pd.s[j];
+          break;
+        }
+      std::vector<double> n(ids.size(), 0.0);
+      for (size_t k = 0; k < ids.size(); ++k)
+        n[k] = side[k] ? at(A.pd.s[i], ids[k], 0.0) : (bArr ? at(*bArr, ids[k], 0.0) : 0.0);
+      out.pd.sLabel.push_back(A.pd.sLabel[i]);
+      out.pd.s.push_back(std::move(n));
+    }
+    for (size_t i = 0; i < A.pd.v.size(); ++i) {
+      const std::vector<std::array<double, 3>> *bArr = nullptr;
+      for (size_t j = 0; j < B.pd.vLabel.size(); ++j)
+        if (B.pd.vLabel[j] == A.pd.vLabel[i]) {
+          bArr = &B.pd.v[j];
+          break;
+        }
+      const std::array<double, 3> z{0, 0, 0};
+      std::vector<std::array<double, 3>> n(ids.size(), z);
+      for (size_t k = 0; k < ids.size(); ++k)
+        n[k] = side[k] ? at(A.pd.v[i], ids[k], z) : (bArr ? at(*bArr, ids[k], z) : z);
+      out.pd.vLabel.push_back(A.pd.vLabel[i]);
+      out.pd.v.push_back(std::move(n));
+    }
+  }
   out.width = A.width;  // :201
   if (pruneResult) {    // :203-210
     prune(out, true);
@@ -336,8 +483,10 @@ template <int D, class C> void booleanOp(Band<D> &A, const Band<D> &B, C &&comp,
 inline double minComp(double a, double b) { return (a < b) ? a : b; }
 inline double maxComp(double a, double b) { return (a > b) ? a : b; }
 
-// Advect::rebuildLS (lsAdvect.hpp:238-429), cutoff 1.0 / width 2 (non-SLLF)
-template <int D> void rebuild(Band<D> &ls, double cutoff = 1.0, int finalWidth = 2) {
+// Advect::rebuildLS (lsAdvect.hpp:238-429), cutoff 1.0 / width 2 (non-SLLF).
+// Point data follows newDataSourceIds when updateData, else the rebuilt
+// level set has none (:262-267, 418-427).
+template <int D> void rebuild(Band<D> &ls, double cutoff = 1.0, int finalWidth = 2, bool updateData = true) {
   Band<D> out;
   pointOp<D>(ls, out, STAR, [&](const Idx<D> &, double c, const double *nb) -> double {
     if (std::fabs(c) <= 1.0) {  // :306
@@ -379,6 +528,36 @@ template <int D> void rebuild(Band<D> &ls, double cutoff = 1.0, int finalWidth =
     }
     return (dist >= -cutoff) ? dist : NEG;
   });
+  if (updateData && !ls.pd.empty())
+    out.pd = translate(ls.pd, sourceIds<D>(ls, out, [&](double c, const double *nb) -> int {
+      if (std::fabs(c) <= 1.0) {  // a defined result here had a neighbour of opposite sign
+        if (c > 0.5) {            // :316-333: the first active neighbour < -0.5, else the centre
+          for (int j = 0; j < 2 * D; j++)
+            if (std::fabs(nb[j]) <= 1.0 && nb[j] < -0.5) return j;
+        } else if (c < -0.5) {    // :334-352
+          for (int j = 0; j < 2 * D; j++)
+            if (std::fabs(nb[j]) <= 1.0 && nb[j] > 0.5) return j;
+        }
+        return 2 * D;
+      }
+      int used = -1;              // :367-413: the neighbour that set the distance
+      if (c >= 0) {
+        double dist = POS;
+        for (int i = 0; i < 2 * D; i++)
+          if (std::fabs(nb[i]) <= 1.0 && (nb[i] < 0.) && dist > nb[i] + 1.0) {
+            dist = nb[i] + 1.0;
+            used = i;
+          }
+      } else {
+        double dist = NEG;
+        for (int i = 0; i < 2 * D; i++)
+          if (std::fabs(nb[i]) <= 1.0 && (nb[i] > 0) && dist < nb[i] - 1.0) {
+            dist = nb[i] - 1.0;
+            used = i;
+          }
+      }
+      return used;
+    }));
   out.width = finalWidth;
   ls = std::move(out);
 }
@@ -1041,6 +1220,7 @@ static int schemeOrder(int spatial) {
 template <int D> struct Advector {
   AdvectParams P;
   std::vector<double> rates;  // per entry of the expanded band: final velocity (after checkDissipation)
+  std::vector<double> diss;   // per entry: the scheme's dissipation (saveAdvectionVelocities)
   double currentTimeStep = -1;
   double pointUpdates = 0;
   std::string err;
@@ -1089,6 +1269,7 @@ template <int D> struct Advector {
   static constexpr double WRAP_EPS = 1e-4;  // wrappingLayerEpsilon (:80)
   std::vector<Band<D> *> lower;
   std::vector<double> mmV, mmS;  // per expanded-band entry, L slots: velocity (after checkDissipation), stop value
+  std::vector<double> mmD;       // the same slots: the scheme's dissipation
   int nMat() const { return (int)lower.size() + 1; }
 
   void scheme(SchemeCtx<D> s, int material, const Idx<D> &c, int order, double &H, double &Dn, double *alphaAcc) {
@@ -1173,6 +1354,7 @@ template <int D> struct Advector {
     const int L = nMat();
     const size_t n = ls.idx.size();
     mmV.assign(n * L, 0.0);
+    mmD.assign(n * L, 0.0);
     mmS.assign(n * L, 0.0);
     double tempMax = maxTimeStep;
     double alphaAcc[3] = {0, 0, 0};
@@ -1226,6 +1408,7 @@ template <int D> struct Advector {
         if ((P.checkDissipation && (H < 0 && v > 0)) || (H > 0 && v < 0)) v = 0;
         mmV[e * L + k] = v;
         mmS[e * L + k] = stop;
+        mmD[e * L + k] = Dn;
         ++k;
         if (done) break;
       }
@@ -1252,6 +1435,8 @@ template <int D> struct Advector {
     const int L = nMat();
     Band<D> out;
     std::vector<double> vals(ls.idx.size());
+    lastRate.assign(ls.idx.size(), 0.0);
+    lastDiss.assign(ls.idx.size(), 0.0);
     for (size_t e = 0; e < ls.idx.size(); ++e) {
       double value = ls.val[e];
       if (isUndef(value) || std::fabs(value) > 0.5) {
@@ -1271,25 +1456,54 @@ template <int D> struct Advector {
         rate = time * velocity;
       }
       vals[e] = value - rate;
+      lastRate[e] = rate;
+      lastDiss[e] = mmD[e * L + k];
     }
     out.g = ls.g;
     compressInto<D>(out, ls.idx, vals);
     out.width = 1;
+    reducedData(ls, out);
     ls = std::move(out);
+  }
+
+  // Point data after updateLevelSet's Reduce(1) (every kept point records
+  // itself, lsReduce.hpp:105-111) and, with saveAdvectionVelocities, the
+  // arrays "AdvectionVelocities" (the last rate = time * velocity) and
+  // "Dissipation" of every point (lsAdvect.hpp:762-796).
+  std::vector<double> lastRate, lastDiss;
+  void reducedData(const Band<D> &in, Band<D> &out) {
+    std::vector<int64_t> ids;
+    std::vector<double> vel, dis;
+    int64_t id = 0;
+    for (size_t e = 0; e < in.idx.size(); ++e) {
+      if (isUndef(in.val[e])) continue;
+      if (std::fabs(in.val[e]) <= 0.5) {
+        ids.push_back(id);
+        vel.push_back(lastRate[e]);
+        dis.push_back(lastDiss[e]);
+      }
+      ++id;
+    }
+    if (!in.pd.empty()) out.pd = translate(in.pd, ids);
+    if (P.saveVelocities) {
+      out.pd.insertReplaceScalar("AdvectionVelocities", std::move(vel));  // velocityLabel (:833)
+      out.pd.insertReplaceScalar("Dissipation", std::move(dis));          // dissipationLabel (:834)
+    }
   }
 
   // Advect::rebuildLS with the scheme's cutoff and final width: 1.5 / 3 for
   // SLLF, 1.0 / 2 otherwise (lsAdvect.hpp:248-256)
   void rebuildStage(Band<D> &ls) {
-    if (P.spatial == SLLF1) rebuild(ls, 1.5, 3);
-    else rebuild(ls);
+    if (P.spatial == SLLF1) rebuild(ls, 1.5, 3, P.updatePointData != 0);
+    else rebuild(ls, 1.0, 2, P.updatePointData != 0);
   }
 
   // Advect::adjustLowerLayers (lsAdvect.hpp:802-813): every lower level set
   // intersected with the top (BooleanOperation INTERSECT = max + Prune)
   void adjustLowerLayers(const Band<D> &top) {
     if (P.spatial == SLLF1) return;  // SLLF keeps the lower level sets (:804-805)
-    for (Band<D> *b : lower) booleanOp(*b, top, [](double x, double y) { return maxComp(x, y); });
+    for (Band<D> *b : lower)
+      booleanOp(*b, top, [](double x, double y) { return maxComp(x, y); }, [](double x, double y) { return x > y; });
   }
 
   // Advect::computeRates + integrateTime (lsAdvect.hpp:435-682), single LS.
@@ -1314,6 +1528,7 @@ template <int D> struct Advector {
     const int order = schemeOrder(P.spatial);
     const size_t n = ls.idx.size();
     rates.assign(n, 0.0);
+    diss.assign(n, 0.0);
     double tempMax = maxTimeStep;
     double alphaAcc[3] = {0, 0, 0};
     double active = 0;
@@ -1362,6 +1577,7 @@ template <int D> struct Advector {
         double v = velocity;
         if ((P.checkDissipation && (H < 0 && v > 0)) || (H > 0 && v < 0)) v = 0;
         rates[e] = v;
+        diss[e] = Dn;
       }
 #pragma omp critical
       {
@@ -1398,12 +1614,16 @@ template <int D> struct Advector {
     const double cut = 0.5;
     Band<D> out;
     std::vector<double> vals(ls.idx.size());
+    lastRate.assign(ls.idx.size(), 0.0);
+    lastDiss.assign(ls.idx.size(), 0.0);
 #pragma omp parallel for schedule(static)
     for (size_t e = 0; e < ls.idx.size(); ++e) {
       double c = ls.val[e];
       if (!isUndef(c) && std::fabs(c) <= cut) {
         double rate = dt * rates[e];
         vals[e] = c - rate;
+        lastRate[e] = rate;
+        lastDiss[e] = diss[e];
       } else {
         vals[e] = (c < 0) ? NEG : POS;
       }
@@ -1411,6 +1631,7 @@ template <int D> struct Advector {
     out.g = ls.g;
     compressInto<D>(out, ls.idx, vals);
     out.width = 1;
+    reducedData(ls, out);
     ls = std::move(out);
   }
 
@@ -1451,11 +1672,14 @@ template <int D> struct Advector {
       }
       (void)g;
     }
-    booleanOp<D>(ls, init, [wTarget, wSource](double a, double b) -> double {
-      if (!isUndef(a) && !isUndef(b)) return wSource * a + wTarget * b;
-      if (isUndef(a)) return a;
-      return b;
-    });
+    booleanOp<D>(
+        ls, init,
+        [wTarget, wSource](double a, double b) -> double {
+          if (!isUndef(a) && !isUndef(b)) return wSource * a + wTarget * b;
+          if (isUndef(a)) return a;
+          return b;
+        },
+        [](double a, double b) { return !isUndef(a) && !isUndef(b); });
     rebuildStage(ls);
     return movedDown;
   }
@@ -1615,6 +1839,49 @@ void lso_make_sphere(lso_domain *d, const double *origin, double radius) {
   else makeSphere<3>(d->b3, origin, radius);
 }
 
+// Point data of a domain: kind 0 = scalar arrays, 1 = vector arrays
+// (3 doubles per point).
+static PointData &pdOf(lso_domain *d) { return d->D == 2 ? d->b2.pd : d->b3.pd; }
+static const PointData &pdOfC(const lso_domain *d) { return d->D == 2 ? d->b2.pd : d->b3.pd; }
+void lso_pd_insert(lso_domain *d, int kind, const char *label, const double *v, int64_t n) {
+  PointData &pd = pdOf(d);
+  if (kind == 0) {
+    pd.sLabel.push_back(label);
+    pd.s.emplace_back(v, v + n);
+  } else {
+    std::vector<std::array<double, 3>> a((size_t)n);
+    for (int64_t i = 0; i < n; ++i) a[(size_t)i] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+    pd.vLabel.push_back(label);
+    pd.v.push_back(std::move(a));
+  }
+}
+int lso_pd_count(const lso_domain *d, int kind) {
+  const PointData &pd = pdOfC(d);
+  return (int)(kind == 0 ? pd.s.size() : pd.v.size());
+}
+int lso_pd_label(const lso_domain *d, int kind, int i, char *buf, int len) {
+  const PointData &pd = pdOfC(d);
+  const std::string &l = kind == 0 ? pd.sLabel.at((size_t)i) : pd.vLabel.at((size_t)i);
+  std::snprintf(buf, (size_t)len, "%s", l.c_str());
+  return (int)l.size();
+}
+int64_t lso_pd_size(const lso_domain *d, int kind, int i) {
+  const PointData &pd = pdOfC(d);
+  return (int64_t)(kind == 0 ? pd.s.at((size_t)i).size() : pd.v.at((size_t)i).size());
+}
+void lso_pd_get(const lso_domain *d, int kind, int i, double *out) {
+  const PointData &pd = pdOfC(d);
+  if (kind == 0) {
+    const auto &a = pd.s.at((size_t)i);
+    std::copy(a.begin(), a.end(), out);
+  } else {
+    const auto &a = pd.v.at((size_t)i);
+    for (size_t k = 0; k < a.size(); ++k)
+      for (int c = 0; c < 3; ++c) out[3 * k + c] = a[k][c];
+  }
+}
+void lso_pd_clear(lso_domain *d) { pdOf(d).clear(); }
+
 void lso_expand(lso_domain *d, int width) {
   if (d->D == 2) expand<2>(d->b2, width);
   else expand<3>(d->b3, width);
@@ -1635,9 +1902,9 @@ void lso_rebuild(lso_domain *d) {
 int lso_boolean(lso_domain *a, const lso_domain *b, int op) {
   auto run = [&](auto &A, const auto &B) {
     switch (op) {
-    case 0: booleanOp(A, B, [](double x, double y) { return maxComp(x, y); }); break;
-    case 1: booleanOp(A, B, [](double x, double y) { return minComp(x, y); }); break;
-    case 2: booleanOp(A, B, [](double x, double y) { return maxComp(x, -y); }); break;
+    case 0: booleanOp(A, B, [](double x, double y) { return maxComp(x, y); }, [](double x, double y) { return x > y; }); break;
+    case 1: booleanOp(A, B, [](double x, double y) { return minComp(x, y); }, [](double x, double y) { return x < y; }); break;
+    case 2: booleanOp(A, B, [](double x, double y) { return maxComp(x, -y); }, [](double x, double y) { return x > -y; }); break;
     }
   };
   if (op < 0 || op > 2 || a->D != b->D) return -1;
@@ -1652,8 +1919,9 @@ void lso_combine(lso_domain *a, const lso_domain *b, double wTarget, double wSou
     if (isUndef(x)) return x;
     return y;
   };
-  if (a->D == 2) booleanOp(a->b2, b->b2, comp);
-  else booleanOp(a->b3, b->b3, comp);
+  auto fromA = [](double x, double y) { return !isUndef(x) && !isUndef(y); };
+  if (a->D == 2) booleanOp(a->b2, b->b2, comp, fromA);
+  else booleanOp(a->b3, b->b3, comp, fromA);
 }
 
 int lso_check(const lso_domain *d, char *msg, int msglen) {
@@ -1691,6 +1959,8 @@ struct lso_params {
   int adaptive;
   int subdivisions;
   double velAlpha[3];
+  int updatePointData;
+  int saveVelocities;
 };
 
 static AdvectParams toParams(const lso_params *p) {
@@ -1726,6 +1996,8 @@ static AdvectParams toParams(const lso_params *p) {
   for (int i = 0; i < 3; ++i) P.vel.alpha[i] = p->velAlpha[i];
   P.adaptive = p->adaptive;
   P.subdivisions = p->subdivisions < 1 ? 1u : (unsigned)p->subdivisions;
+  P.updatePointData = p->updatePointData;
+  P.saveVelocities = p->saveVelocities;
   return P;
 }
 

@@ -121,11 +121,33 @@ template <int D> inline int64_t rowKey(const Idx<D> &p) {
   else return ((int64_t)p[2] << 32) + ((int64_t)p[1] + 2147483648LL);
 }
 
+// Domain::getPointData (lsDomain.hpp:44, ViennaCore PointData v2.2.1, absent
+// from the reference tree): named scalar arrays (one value per defined
+// point, in point-id order) and vector arrays (three values per point).
+struct PointData {
+  std::vector<std::string> sLabel, vLabel;
+  std::vector<std::vector<double>> s;
+  std::vector<std::vector<std::array<double, 3>>> v;
+  bool empty() const { return s.empty() && v.empty(); }
+  void clear() { *this = PointData(); }
+  // insertReplaceScalarData: replace the first array of that label, else append
+  void insertReplaceScalar(const std::string &label, std::vector<double> data) {
+    for (size_t i = 0; i < sLabel.size(); ++i)
+      if (sLabel[i] == label) {
+        s[i] = std::move(data);
+        return;
+      }
+    sLabel.push_back(label);
+    s.push_back(std::move(data));
+  }
+};
+
 template <int D> struct Band {
   Grid<D> g;
   std::vector<Idx<D>> idx;
   std::vector<double> val;
   int width = 1;  // Domain::levelSetWidth (lsDomain.hpp:42)
+  PointData pd;
   // row index: one entry per distinct row (all dims but x), in order
   std::vector<int64_t> rkeys;
   std::vector<int64_t> rstart;  // size rows+1
@@ -183,6 +205,8 @@ struct AdvectParams {
   std::vector<Velocity> matVel;
   int adaptive = 0;             // setAdaptiveTimeStepping (lsAdvect.hpp:904-912)
   unsigned subdivisions = 20;
+  int updatePointData = 1;      // setUpdatePointData (lsAdvect.hpp:75, 962)
+  int saveVelocities = 0;       // setSaveAdvectionVelocities (lsAdvect.hpp:74, 916)
   const Velocity &velFor(int m) const { return (m >= 0 && m < (int)matVel.size()) ? matVel[m] : vel; }
 };
 

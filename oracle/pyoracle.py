@@ -45,6 +45,8 @@ class Params(ctypes.Structure):
         ("adaptive", ctypes.c_int),
         ("subdivisions", ctypes.c_int),
         ("velAlpha", ctypes.c_double * 3),
+        ("updatePointData", ctypes.c_int),
+        ("saveVelocities", ctypes.c_int),
     ]
 
 
@@ -98,6 +100,15 @@ def lib():
         L.lso_mlp_loss_grad.restype = ctypes.c_double
         L.lso_bf16_round.argtypes = [ctypes.c_float]
         L.lso_bf16_round.restype = ctypes.c_float
+        L.lso_pd_insert.argtypes = [P, ctypes.c_int, ctypes.c_char_p, dp, ctypes.c_int64]
+        L.lso_pd_count.argtypes = [P, ctypes.c_int]
+        L.lso_pd_count.restype = ctypes.c_int
+        L.lso_pd_label.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.lso_pd_label.restype = ctypes.c_int
+        L.lso_pd_size.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.lso_pd_size.restype = ctypes.c_int64
+        L.lso_pd_get.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
+        L.lso_pd_clear.argtypes = [P]
         L.lso_num_threads.restype = ctypes.c_int
         L.lso_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -194,16 +205,46 @@ class Domain:
         m = np.abs(val) != POS
         return idx[m], val[m]
 
+    # point data: kind 0 = scalar arrays, 1 = vector arrays (n x 3)
+    def insert_data(self, label, values, vector=False):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        n = len(a)
+        a = a.reshape(-1)
+        lib().lso_pd_insert(self.h, int(vector), label.encode(), a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+
+    def data_labels(self, vector=False):
+        k = int(vector)
+        out = []
+        for i in range(lib().lso_pd_count(self.h, k)):
+            buf = ctypes.create_string_buffer(256)
+            lib().lso_pd_label(self.h, k, i, buf, 256)
+            out.append(buf.value.decode())
+        return out
+
+    def data(self, label, vector=False):
+        k = int(vector)
+        i = self.data_labels(vector).index(label)
+        n = lib().lso_pd_size(self.h, k, i)
+        a = np.zeros(n * (3 if vector else 1), dtype=np.float64)
+        lib().lso_pd_get(self.h, k, i, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return a.reshape(n, 3) if vector else a
+
+    def clear_data(self):
+        lib().lso_pd_clear(self.h)
+
 
 def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
                 dissipation_alpha=1.0, calc_normals=True, check_dissipation=True,
-                vel=None, mlp=None, mat_vels=None, adaptive=False, subdivisions=20):
+                vel=None, mlp=None, mat_vels=None, adaptive=False, subdivisions=20,
+                update_point_data=True, save_velocities=False):
     """vel: dict(kind, dir, k, b, a_pos, a_neg, vec) of the built-in velocity.
     mlp: float32 array of the learned-weno5 network (spatial scheme 12).
     mat_vels: list of such dicts, the velocity of materials 0, 1, ...
     (multi-material advection; other materials use vel).
     adaptive, subdivisions: setAdaptiveTimeStepping."""
     p = Params()
+    p.updatePointData = int(bool(update_point_data))
+    p.saveVelocities = int(bool(save_velocities))
     p.adaptive = int(bool(adaptive))
     p.subdivisions = int(subdivisions)
     if mlp is not None:

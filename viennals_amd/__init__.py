@@ -71,6 +71,85 @@ def _f64(a):
     return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+class PointData:
+    """Point data of a Domain (ViennaCore PointData as bound in
+    python/viennals/_core.pyi:819-845): named scalar arrays (one value per
+    defined point, point-id order) and vector arrays (three values per
+    point), kept on the device."""
+
+    def __init__(self, domain):
+        self._d = domain
+
+    def _insert(self, values, label, comps, replace):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        if comps == 3:
+            a = a.reshape(-1, 3)
+        n = a.shape[0] if a.ndim else 0
+        check(lib().lsg_domain_point_data_insert(self._d._h, label.encode(), comps, a.ctypes.data, n, int(replace)))
+
+    def insertNextScalarData(self, scalars, label="Scalars"):
+        self._insert(scalars, label, 1, False)
+
+    def insertNextVectorData(self, vectors, label="Vectors"):
+        self._insert(vectors, label, 3, False)
+
+    def insertReplaceScalarData(self, scalars, label="Scalars"):
+        self._insert(scalars, label, 1, True)
+
+    def insertReplaceVectorData(self, vectors, label="Vectors"):
+        self._insert(vectors, label, 3, True)
+
+    def getScalarDataSize(self):
+        return lib().lsg_domain_point_data_count(self._d._h, 1)
+
+    def getVectorDataSize(self):
+        return lib().lsg_domain_point_data_count(self._d._h, 3)
+
+    def _label(self, comps, i):
+        buf = ctypes.create_string_buffer(1024)
+        n = lib().lsg_domain_point_data_label(self._d._h, comps, int(i), buf, 1024)
+        if n < 0:
+            check(n)
+        return buf.value.decode()
+
+    def getScalarDataLabel(self, i):
+        return self._label(1, i)
+
+    def getVectorDataLabel(self, i):
+        return self._label(3, i)
+
+    def _get(self, comps, key, noWarning):
+        count = lib().lsg_domain_point_data_count(self._d._h, comps)
+        if isinstance(key, str):
+            idx = [i for i in range(count) if self._label(comps, i) == key]
+            if not idx:
+                if not noWarning:
+                    import warnings
+                    warnings.warn(f"PointData: no data with label {key!r}")
+                return None
+            i = idx[0]
+        else:
+            i = int(key)
+        n = lib().lsg_domain_point_data_size(self._d._h, comps, i)
+        if n < 0:
+            check(int(n))
+        out = np.zeros(n * comps, dtype=np.float64)
+        check(lib().lsg_domain_point_data_get(self._d._h, comps, i, out.ctypes.data, n))
+        return out.reshape(n, 3) if comps == 3 else out
+
+    def getScalarData(self, key, noWarning=False):
+        return self._get(1, key, noWarning)
+
+    def getVectorData(self, key, noWarning=False):
+        return self._get(3, key, noWarning)
+
+    def clear(self):
+        check(lib().lsg_domain_point_data_clear(self._d._h))
+
+    def empty(self):
+        return self.getScalarDataSize() == 0 and self.getVectorDataSize() == 0
+
+
 class Domain:
     """Device-resident level set (Domain<T,D>, lsDomain.hpp:27-269).
 
@@ -153,6 +232,11 @@ class Domain:
 
     def clearMetaData(self):
         pass  # no metadata is kept on the device band
+
+    def getPointData(self):
+        """Domain::getPointData (lsDomain.hpp:44): the device-resident
+        arrays carried through every band operation and Advect."""
+        return PointData(self)
 
     def print(self):
         print(f"Domain: D={self.D} gridDelta={self.gridDelta} index bounds {self.gmin}..{self.gmax} "
@@ -492,9 +576,9 @@ class Advect:
         return getattr(self, "_calcNormals", True)  # lsAdvect.hpp:68
 
     def setSaveAdvectionVelocities(self, on):
-        """Point data is not carried by the GPU band: only False is accepted."""
-        if on:
-            raise LsgError("setSaveAdvectionVelocities(True): point data is not supported on the GPU path")
+        """Store "AdvectionVelocities" and "Dissipation" as point data of the
+        advected level set (lsAdvect.hpp:762-796, 916)."""
+        check(lib().lsg_advect_set_save_advection_velocities(self._h, int(bool(on))))
 
     def setVelocityUpdateCallback(self, callback):
         """Called between RK stages by the reference; not supported here."""

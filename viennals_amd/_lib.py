@@ -37,6 +37,9 @@ EXPORTED = [
     "lsg_batch_create", "lsg_batch_destroy", "lsg_batch_set_velocity_builtin", "lsg_batch_set_schemes",
     "lsg_batch_apply", "lsg_batch_get_advected_times", "lsg_batch_get_point_updates", "lsg_batch_write_back",
     "lsg_batch_set_scan_blocks", "lsg_batch_autotune", "lsg_advect_set_velocity_host",
+    "lsg_advect_set_save_advection_velocities", "lsg_domain_point_data_insert", "lsg_domain_point_data_count",
+    "lsg_domain_point_data_label", "lsg_domain_point_data_size", "lsg_domain_point_data_get",
+    "lsg_domain_point_data_clear",
 ]
 
 
@@ -96,6 +99,13 @@ def lib():
         "lsg_advect_set_ignore_voids": ([P, c_int], c_int),
         "lsg_advect_set_adaptive_time_stepping": ([P, c_int, ctypes.c_uint], c_int),
         "lsg_advect_set_update_point_data": ([P, c_int], c_int),
+        "lsg_advect_set_save_advection_velocities": ([P, c_int], c_int),
+        "lsg_domain_point_data_insert": ([P, ctypes.c_char_p, c_int, ctypes.c_void_p, ctypes.c_int64, c_int], c_int),
+        "lsg_domain_point_data_count": ([P, c_int], c_int),
+        "lsg_domain_point_data_label": ([P, c_int, c_int, ctypes.c_char_p, c_int], c_int),
+        "lsg_domain_point_data_size": ([P, c_int, c_int], ctypes.c_int64),
+        "lsg_domain_point_data_get": ([P, c_int, c_int, ctypes.c_void_p, ctypes.c_int64], c_int),
+        "lsg_domain_point_data_clear": ([P], c_int),
         "lsg_advect_prepare_ls": ([P], c_int),
         "lsg_advect_apply": ([P], c_int),
         "lsg_advect_get_advected_time": ([P, dp], c_int),

@@ -253,6 +253,16 @@ struct RateParams {
   // stencil local Lax-Friedrichs: cbrt(DBL_EPSILON), computed on the host
   // (the base of the normal perturbation, lsStencilLocalLaxFriedrichsScalar.hpp:44-45)
   double sllfBaseEps;
+  // saveAdvectionVelocities: the scheme's dissipation per point (one level
+  // set: per band entry; several: per entry and list slot), else null
+  double *dissOut;
+};
+
+// Which point a band operation's result takes its point data from.
+struct PdSrcParams {
+  int kind;           // SrcKind
+  double totalLimit;  // Expand's width / 2
+  int boolKind;       // BoolKind of a boolean
 };
 
 // Learned weno5 network: 5 -> 32 -> 32 -> 3 (GELU, softmax); parameter

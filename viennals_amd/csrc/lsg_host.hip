@@ -132,6 +132,16 @@ struct WsStore {
 
 using namespace lsg;
 
+// One array of Domain::getPointData (lsDomain.hpp:44): a scalar (1
+// component) or vector (3 components) value per defined point, in point-id
+// order, resident on the device; `alt` receives the next translation.
+struct PdArray {
+  std::string label;
+  int comps = 1;
+  double *buf = nullptr, *alt = nullptr;
+  size_t cap = 0;  // tuples
+};
+
 struct lsg_domain {
   int D = 3;
   GridInfo g{};
@@ -140,10 +150,21 @@ struct lsg_domain {
   Store cur, tmp;  // current band and the output buffer of the next op
   uint32_t hostN = 0;
   bool framed = false;
+  std::vector<PdArray> pd;    // point data, insertion order
+  uint32_t *pdLen = nullptr;  // device: tuples in every array
 
+  void pdRelease() {
+    for (PdArray &a : pd) {
+      if (a.buf) (void)hipFree(a.buf);
+      if (a.alt) (void)hipFree(a.alt);
+    }
+    pd.clear();
+  }
   ~lsg_domain() {
     cur.release();
     tmp.release();
+    pdRelease();
+    if (pdLen) (void)hipFree(pdLen);
   }
 };
 
@@ -415,6 +436,134 @@ static void opSelf(lsg_domain *d, const OpParams &op) {
   swapCur(d);
 }
 
+// ------------------------------------------------------------ point data ---
+// PointData::translateFromMultiData after every band operation (all of them
+// update point data by default: lsExpand.hpp:27, lsReduce.hpp:19,
+// lsPrune.hpp:19, lsBooleanOperation.hpp:56; Advect::rebuildLS only with
+// updatePointData, lsAdvect.hpp:75,262).  The input band of the operation is
+// the domain's `tmp` store right after it ran.
+struct PdWork {
+  uint32_t *rankIn = nullptr, *rankB = nullptr, *dlist = nullptr, *src = nullptr, *entry = nullptr;
+  uint8_t *side = nullptr;
+  uint32_t *ndef = nullptr, *nPush = nullptr;
+  size_t cap = 0;
+  int ensure(size_t c) {
+    if (c <= cap) return 0;
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    for (void *p : {(void *)rankIn, (void *)rankB, (void *)dlist, (void *)src, (void *)entry, (void *)side})
+      if (p) (void)hipFree(p);
+    LSG_HIP(hipMalloc(&rankIn, c * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&rankB, c * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&dlist, c * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&src, c * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&entry, c * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&side, c));
+    if (!ndef) LSG_HIP(hipMalloc(&ndef, sizeof(uint32_t)));
+    if (!nPush) LSG_HIP(hipMalloc(&nPush, sizeof(uint32_t)));
+    cap = c;
+    return 0;
+  }
+};
+static PdWork *g_pd = nullptr;
+
+static int pdEnsureArray(PdArray &a, size_t cap, const uint32_t *len) {
+  if (a.cap >= cap) return 0;
+  double *nb = nullptr, *na = nullptr;
+  LSG_HIP(hipMalloc(&nb, cap * a.comps * sizeof(double)));
+  LSG_HIP(hipMalloc(&na, cap * a.comps * sizeof(double)));
+  if (a.buf) {
+    (void)len;
+    LSG_HIP(hipMemcpyAsync(nb, a.buf, a.cap * a.comps * sizeof(double), hipMemcpyDeviceToDevice, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    (void)hipFree(a.buf);
+  }
+  if (a.alt) (void)hipFree(a.alt);
+  a.buf = nb;
+  a.alt = na;
+  a.cap = cap;
+  return 0;
+}
+static int pdEnsure(lsg_domain *d, size_t cap) {
+  if (!d->pdLen) {
+    LSG_HIP(hipMalloc(&d->pdLen, sizeof(uint32_t)));
+    LSG_HIP(hipMemsetAsync(d->pdLen, 0, sizeof(uint32_t), g_ctx.s));
+  }
+  for (PdArray &a : d->pd)
+    if (int e = pdEnsureArray(a, cap, d->pdLen)) return e;
+  return 0;
+}
+static PdArray *pdFind(lsg_domain *d, const char *label, int comps) {
+  for (PdArray &a : d->pd)
+    if (a.comps == comps && a.label == label) return &a;
+  return nullptr;
+}
+static const PdArray *pdFind(const lsg_domain *d, const std::string &label, int comps) {
+  for (const PdArray &a : d->pd)
+    if (a.comps == comps && a.label == label) return &a;
+  return nullptr;
+}
+
+// Sources of the result of the operation that just ran on d (into g_pd).
+static int pdSources(lsg_domain *d, const PdSrcParams &p, const lsg_domain *B = nullptr) {
+  if (!g_pd) g_pd = new PdWork();
+  const size_t cap = std::max(std::max(d->cur.b.cap, d->tmp.b.cap), B ? B->cur.b.cap : (size_t)0);
+  if (int e = g_pd->ensure(cap)) return e;
+  launchPointIds(g_ctx.s, d->tmp.b, g_ws->w, g_pd->dlist, g_pd->ndef, g_pd->rankIn);
+  if (B) launchPointIds(g_ctx.s, B->cur.b, g_ws->w, g_pd->dlist, g_pd->ndef, g_pd->rankB);
+  launchPdSources(g_ctx.s, d->D, d->cur.b, d->tmp.b, B ? &B->cur.b : nullptr, d->f, d->g, p, g_ws->w, g_pd->rankIn,
+                  B ? g_pd->rankB : nullptr, g_pd->src, g_pd->side, g_pd->entry, g_pd->nPush);
+  return 0;
+}
+// Gather every array through the sources (a boolean's B-sourced points take
+// B's array of the same label, else 0: lsBooleanOperation.hpp:146-196).
+static int pdApply(lsg_domain *d, const lsg_domain *B = nullptr) {
+  if (int e = pdEnsure(d, d->cur.b.cap)) return e;
+  for (PdArray &a : d->pd) {
+    const PdArray *b = B ? pdFind(B, a.label, a.comps) : nullptr;
+    launchPdGather(g_ctx.s, a.comps, a.buf, d->pdLen, b ? b->buf : nullptr, b ? B->pdLen : nullptr, g_pd->src,
+                   g_pd->side, g_pd->nPush, d->cur.b.cap, a.alt);
+    std::swap(a.buf, a.alt);
+  }
+  launchCopyU32(g_ctx.s, g_pd->nPush, d->pdLen);
+  return 0;
+}
+static void pdTranslate(lsg_domain *d, const PdSrcParams &p, const lsg_domain *B = nullptr) {
+  if (d->pd.empty()) return;
+  if (pdSources(d, p, B) == 0) (void)pdApply(d, B);
+}
+static void pdClear(lsg_domain *d) {
+  if (d->pd.empty()) return;
+  (void)hipStreamSynchronize(g_ctx.s);
+  d->pdRelease();
+}
+// dst's point data becomes a copy of src's (Domain::deepCopy, lsDomain.hpp:143-150).
+static int pdCopy(lsg_domain *dst, const lsg_domain *src) {
+  if (dst->pd.empty() && src->pd.empty()) return 0;
+  if (dst->pd.size() != src->pd.size()) {
+    (void)hipStreamSynchronize(g_ctx.s);
+    dst->pdRelease();
+    dst->pd.resize(src->pd.size());
+  }
+  for (size_t i = 0; i < src->pd.size(); ++i) {
+    dst->pd[i].label = src->pd[i].label;
+    if (dst->pd[i].comps != src->pd[i].comps) {
+      if (dst->pd[i].buf) (void)hipFree(dst->pd[i].buf);
+      if (dst->pd[i].alt) (void)hipFree(dst->pd[i].alt);
+      dst->pd[i].buf = dst->pd[i].alt = nullptr;
+      dst->pd[i].cap = 0;
+      dst->pd[i].comps = src->pd[i].comps;
+    }
+  }
+  size_t cap = 0;
+  for (const PdArray &a : src->pd) cap = std::max(cap, a.cap);
+  if (int e = pdEnsure(dst, cap)) return e;
+  for (size_t i = 0; i < src->pd.size(); ++i)
+    LSG_HIP(hipMemcpyAsync(dst->pd[i].buf, src->pd[i].buf, src->pd[i].cap * src->pd[i].comps * sizeof(double),
+                           hipMemcpyDeviceToDevice, g_ctx.s));
+  if (src->pdLen) launchCopyU32(g_ctx.s, src->pdLen, dst->pdLen);
+  return 0;
+}
+
 // Expand<T,D>::apply (lsExpand.hpp:44-172); the early exit for a band
 // without defined points is not taken (its result is then unchanged).
 static void doExpand(lsg_domain *d, int width) {
@@ -427,6 +576,7 @@ static void doExpand(lsg_domain *d, int width) {
     op.a = (startWidth + cyc + 1) * 0.5;
     op.b = totalLimit;
     opStar(d, op);
+    pdTranslate(d, PdSrcParams{SRC_EXPAND, totalLimit, 0});
   }
   d->width = width;
 }
@@ -436,21 +586,27 @@ static void doReduce(lsg_domain *d, int width) {  // lsReduce.hpp:55-134
   op.kind = OP_REDUCE;
   op.a = width * 0.5;
   opSelf(d, op);
+  pdTranslate(d, PdSrcParams{SRC_CENTRE, 0, 0});
   d->width = width;
 }
 static void doPrune(lsg_domain *d, bool rsz) {  // lsPrune.hpp:74-194
   OpParams op{};
   op.kind = rsz ? OP_PRUNE_RSZ : OP_PRUNE;
   opSelf(d, op);
+  pdTranslate(d, PdSrcParams{rsz ? SRC_PRUNE_RSZ : SRC_PRUNE, 0, 0});
   d->width = 2;
 }
 // Advect::rebuildLS (lsAdvect.hpp:238-429): cutoff 1.0 and final width 2,
-// 1.5 and 3 for the stencil local Lax-Friedrichs scheme (:248-256)
-static void doRebuild(lsg_domain *d, bool sllf = false) {
+// 1.5 and 3 for the stencil local Lax-Friedrichs scheme (:248-256); the
+// point data follows when updateData, else the result has none (:262-267,
+// 418-427).
+static void doRebuild(lsg_domain *d, bool sllf = false, bool updateData = true) {
   OpParams op{};
   op.kind = OP_REBUILD;
   op.a = sllf ? 1.5 : 1.0;
   opStar(d, op);
+  if (updateData) pdTranslate(d, PdSrcParams{SRC_REBUILD, 0, 0});
+  else pdClear(d);
   d->width = sllf ? 3 : 2;
 }
 // booleanOpInternal + pruning (lsBooleanOperation.hpp:59-213)
@@ -458,6 +614,7 @@ static void doBoolean(lsg_domain *a, const lsg_domain *b, int kind, double wT, d
                       uint32_t *movedDown = nullptr) {
   launchBoolean(g_ctx.s, a->D, a->cur.b, b->cur.b, a->tmp.b, a->f, a->g, g_ws->w, kind, wT, wS, g_ctx.sc, movedDown);
   swapCur(a);
+  pdTranslate(a, PdSrcParams{SRC_BOOLEAN, 0, kind}, b);
   doPrune(a, true);
   doPrune(a, false);
 }
@@ -566,6 +723,10 @@ struct lsg_advect {
   std::vector<lsg_domain *> lowerSave;  // several level sets: the lower ones at the start of a step
   double *rates = nullptr;
   size_t ratesCap = 0;
+  // point data (lsAdvect.hpp:74-75): translated by rebuildLS, and the saved
+  // velocities; diss holds the scheme's dissipation per rate slot
+  bool updatePointData = true, saveVelocities = false;
+  double *diss = nullptr;
   bool profiling = false;
   double rateMs = 0, rateBytes = 0;
   int64_t rateLaunches = 0;
@@ -614,6 +775,7 @@ struct lsg_advect {
     if (save) delete save;
     for (lsg_domain *d : lowerSave) delete d;
     if (rates) (void)hipFree(rates);
+    if (diss) (void)hipFree(diss);
     for (auto e : evPool) (void)hipEventDestroy(e);
     for (auto e : starPool) (void)hipEventDestroy(e);
     if (hostMaxDt) (void)hipHostFree(hostMaxDt);
@@ -655,11 +817,34 @@ static int schemeOrder(int spatial) {
 }
 
 static int ensureRates(lsg_advect *a, size_t cap) {
-  if (a->ratesCap >= cap) return 0;
-  if (a->rates) (void)hipFree(a->rates);
-  LSG_HIP(hipMalloc(&a->rates, cap * sizeof(double)));
-  a->ratesCap = cap;
+  if (a->ratesCap < cap) {
+    if (a->rates) (void)hipFree(a->rates);
+    if (a->diss) (void)hipFree(a->diss);
+    a->diss = nullptr;
+    LSG_HIP(hipMalloc(&a->rates, cap * sizeof(double)));
+    a->ratesCap = cap;
+  }
+  if (a->saveVelocities && !a->diss) LSG_HIP(hipMalloc(&a->diss, a->ratesCap * sizeof(double)));
   return 0;
+}
+
+// Point data after updateLevelSet: its Reduce(1) keeps the active points,
+// each recording itself (lsReduce.hpp:105-111); with saveAdvectionVelocities
+// the arrays "AdvectionVelocities" (the last rate = time * velocity) and
+// "Dissipation" are inserted or replaced (lsAdvect.hpp:762-796).
+static void pdUpdate(lsg_advect *a, lsg_domain *d, const double *dt, int nmat) {
+  if (d->pd.empty() && !a->saveVelocities) return;
+  if (pdSources(d, PdSrcParams{SRC_CENTRE, 0, 0}) != 0) return;
+  if (pdApply(d) != 0) return;
+  if (!a->saveVelocities) return;
+  for (const char *l : {"AdvectionVelocities", "Dissipation"})  // velocityLabel, dissipationLabel (:833-834)
+    if (!pdFind(d, l, 1)) {
+      d->pd.emplace_back();
+      d->pd.back().label = l;
+    }
+  if (pdEnsure(d, d->cur.b.cap) != 0) return;
+  launchPdVelocities(g_ctx.s, g_pd->entry, g_pd->nPush, d->tmp.b.val, a->rates, a->diss, dt, nmat, d->cur.b.cap,
+                     pdFind(d, "AdvectionVelocities", 1)->buf, pdFind(d, "Dissipation", 1)->buf);
 }
 
 // One forward-Euler stage (lsAdvectIntegrationSchemes.hpp:46-59):
@@ -707,6 +892,7 @@ static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *d
   doExpand(d, prepareWidth(a->spatial));  // Advect::prepareLS (lsAdvect.hpp:973-1015)
   RateParams rp = a->rp;
   rp.hostVel = nullptr;
+  rp.dissOut = a->saveVelocities ? a->diss : nullptr;
   if (a->hostScalar || a->hostVector) {
     if (!a->hostErr) a->hostErr = hostVelocities(a, d);
     rp.hostVel = a->velOut;
@@ -727,8 +913,9 @@ static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *d
   op.rates = a->rates;
   op.dt = dtOut;
   opSelf(d, op);
+  pdUpdate(a, d, dtOut, 1);
   d->width = 1;
-  doRebuild(d, a->spatial == 9);
+  doRebuild(d, a->spatial == 9, a->updatePointData);
 }
 
 // Advect::combineLevelSets (lsAdvect.hpp:199-236)
@@ -740,10 +927,12 @@ static void combine(lsg_advect *a, lsg_domain *d, lsg_domain *init, double wTarg
   doExpand(d, ew);
   doExpand(init, ew);
   doBoolean(d, init, B_COMBINE, wTarget, wSource);
-  doRebuild(d, a->spatial == 9);
+  doRebuild(d, a->spatial == 9, a->updatePointData);
 }
 
 // Copy without allocation (dst storage must already fit): usable in capture.
+// (point data is copied too; it allocates, which only happens on steps
+// launched directly: point data turns graph capture off)
 static void copyDomainNoAlloc(lsg_domain *dst, const lsg_domain *src) {
   dst->D = src->D;
   dst->g = src->g;
@@ -752,6 +941,7 @@ static void copyDomainNoAlloc(lsg_domain *dst, const lsg_domain *src) {
   dst->width = src->width;
   launchCopyBand(g_ctx.s, src->cur.b, dst->cur.b, src->f);
   dst->hostN = src->hostN;
+  (void)pdCopy(dst, src);
 }
 
 static int copyDomain(lsg_domain *dst, const lsg_domain *src) {
@@ -763,7 +953,7 @@ static int copyDomain(lsg_domain *dst, const lsg_domain *src) {
   if (int e = ensureCur(dst, src->cur.nwAlloc, src->cur.b.cap)) return e;
   launchCopyBand(g_ctx.s, src->cur.b, dst->cur.b, src->f);
   dst->hostN = src->hostN;
-  return 0;
+  return pdCopy(dst, src);
 }
 
 // One time step on the device (Advect::advect, lsAdvect.hpp:817-830).
@@ -849,7 +1039,13 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
     bool launched = false;
     const bool hostVel = a->hostScalar || a->hostVector;
     a->hostErr = 0;
-    if (a->useGraphs && !hostVel) {
+    // point data allocates during the step: such steps are launched directly
+    const bool pointData = !d->pd.empty() || a->saveVelocities;
+    if (!pointData) {
+      pdClear(a->save);
+      if (a->init) pdClear(a->init);
+    }
+    if (a->useGraphs && !hostVel && !pointData) {
       uint64_t key = stepKey(a, d);
       lsg_advect::StepGraph *g = nullptr;
       for (auto &c : a->graphs)
@@ -975,16 +1171,18 @@ static void stageMM(lsg_advect *a, const double *maxDt, double *dtOut) {
     lw.pre[m] = a->levelSets[m]->cur.b.pre;
     lw.val[m] = a->levelSets[m]->cur.b.val;
   }
-  launchRatesMM(g_ctx.s, top->D, top->cur.b, top->f, top->g, lw, a->rp, a->rates, g_ctx.sc, maxDt, dtOut,
-                g_ws->w);
+  RateParams rp = a->rp;
+  rp.dissOut = a->saveVelocities ? a->diss : nullptr;
+  launchRatesMM(g_ctx.s, top->D, top->cur.b, top->f, top->g, lw, rp, a->rates, g_ctx.sc, maxDt, dtOut, g_ws->w);
   OpParams op{};
   op.kind = OP_UPDATE_MM;
   op.rates = a->rates;
   op.dt = dtOut;
   op.nmat = L;
   opSelf(top, op);
+  pdUpdate(a, top, dtOut, L);
   top->width = 1;
-  doRebuild(top, a->spatial == 9);
+  doRebuild(top, a->spatial == 9, a->updatePointData);
 }
 
 static void combineMM(lsg_advect *a, double wTarget, double wSource, uint32_t *movedDown) {
@@ -993,7 +1191,7 @@ static void combineMM(lsg_advect *a, double wTarget, double wSource, uint32_t *m
   doExpand(top, ew);
   doExpand(a->init, ew);
   doBoolean(top, a->init, B_COMBINE, wTarget, wSource, movedDown);
-  doRebuild(top, a->spatial == 9);
+  doRebuild(top, a->spatial == 9, a->updatePointData);
 }
 
 static int enqueueStepMM(lsg_advect *a) {
@@ -1314,6 +1512,7 @@ static int batchAssemble(lsg_batch *b) {
     s.baseIncl = baseAcc;
     W += d->f.nwords;
     if (k > 0 && d->width != width) return fail(-41, "batch: the level sets must have the same band width");
+    if (!d->pd.empty()) return fail(-41, "batch: level sets with point data are not supported");
     width = d->width;
   }
   LSG_HIP(hipStreamSynchronize(g_ctx.s));
@@ -1709,6 +1908,83 @@ int lsg_domain_num_runs(lsg_domain *d, int64_t *n) {
   return 0;
 }
 
+// ------------------------------------------------------------ point data ---
+static int pdIndex(const lsg_domain *d, int comps, int i) {
+  int k = 0;
+  for (size_t j = 0; j < d->pd.size(); ++j)
+    if (d->pd[j].comps == comps && k++ == i) return (int)j;
+  return -1;
+}
+int lsg_domain_point_data_insert(lsg_domain *d, const char *label, int components, const double *values, int64_t n,
+                                 int replace) {
+  if (components != 1 && components != 3) return fail(-1, "point data: components must be 1 or 3");
+  if (!label || n < 0 || (n > 0 && !values)) return fail(-1, "point data: invalid arguments");
+  if (int e = ctxInit()) return e;
+  int j = -1;
+  if (replace)
+    for (size_t q = 0; q < d->pd.size() && j < 0; ++q)
+      if (d->pd[q].comps == components && d->pd[q].label == label) j = (int)q;
+  if (j < 0) {
+    d->pd.emplace_back();
+    j = (int)d->pd.size() - 1;
+    d->pd[j].label = label;
+    d->pd[j].comps = components;
+  }
+  size_t cap = std::max<size_t>((size_t)n, d->cur.b.cap);
+  for (const PdArray &a : d->pd) cap = std::max(cap, a.cap);
+  if (int e = pdEnsure(d, std::max<size_t>(cap, 1))) return e;
+  PdArray &a = d->pd[j];
+  LSG_HIP(hipMemsetAsync(a.buf, 0, a.cap * a.comps * sizeof(double), g_ctx.s));
+  if (n > 0)
+    LSG_HIP(hipMemcpyAsync(a.buf, values, (size_t)n * components * sizeof(double), hipMemcpyHostToDevice, g_ctx.s));
+  uint32_t len = 0;
+  LSG_HIP(hipMemcpyAsync(&len, d->pdLen, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  if ((uint32_t)n > len) {
+    len = (uint32_t)n;
+    LSG_HIP(hipMemcpy(d->pdLen, &len, sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+int lsg_domain_point_data_count(const lsg_domain *d, int components) {
+  int k = 0;
+  for (const PdArray &a : d->pd) k += (a.comps == components) ? 1 : 0;
+  return k;
+}
+int lsg_domain_point_data_label(const lsg_domain *d, int components, int i, char *buf, int len) {
+  const int j = pdIndex(d, components, i);
+  if (j < 0) return fail(-1, "point data: no such array");
+  if (buf && len > 0) snprintf(buf, (size_t)len, "%s", d->pd[j].label.c_str());
+  return (int)d->pd[j].label.size();
+}
+// Tuples per array: every array holds one value (or vector) per point id
+// of the last translation (arrays inserted since hold what was given,
+// padded with zeros up to the longest).
+int64_t lsg_domain_point_data_size(const lsg_domain *d, int components, int i) {
+  if (pdIndex(d, components, i) < 0) return fail(-1, "point data: no such array");
+  if (int e = ctxInit()) return e;
+  uint32_t len = 0;
+  LSG_HIP(hipMemcpyAsync(&len, d->pdLen, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  return (int64_t)len;
+}
+int lsg_domain_point_data_get(const lsg_domain *d, int components, int i, double *out, int64_t n) {
+  const int j = pdIndex(d, components, i);
+  if (j < 0) return fail(-1, "point data: no such array");
+  const int64_t len = lsg_domain_point_data_size(d, components, i);
+  if (len < 0) return (int)len;
+  if (n < len) return fail(-1, "point data: output buffer too small");
+  if (len > 0)
+    LSG_HIP(hipMemcpyAsync(out, d->pd[j].buf, (size_t)len * components * sizeof(double), hipMemcpyDeviceToHost,
+                           g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  return 0;
+}
+int lsg_domain_point_data_clear(lsg_domain *d) {
+  pdClear(d);
+  return 0;
+}
+
 int lsg_domain_get_runs(lsg_domain *d, int32_t *idx, double *val, int64_t cap) {
   std::vector<int32_t> ri;
   std::vector<double> rv;
@@ -1930,7 +2206,15 @@ int lsg_advect_set_adaptive_time_stepping(lsg_advect *a, int on, unsigned subdiv
   a->dropGraphs();
   return 0;
 }
-int lsg_advect_set_update_point_data(lsg_advect *, int) { return 0; }
+int lsg_advect_set_update_point_data(lsg_advect *a, int update) {
+  a->updatePointData = update != 0;
+  return 0;
+}
+int lsg_advect_set_save_advection_velocities(lsg_advect *a, int save) {
+  a->saveVelocities = save != 0;
+  a->dropGraphs();
+  return 0;
+}
 int lsg_advect_prepare_ls(lsg_advect *a) {
   if (a->levelSets.empty()) return fail(-1, "No level sets passed to Advect.");
   int w = prepareWidth(a->spatial);

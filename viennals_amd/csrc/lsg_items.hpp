@@ -173,6 +173,104 @@ LSG_HD inline double ruleUpdateMM(double c, double dt, const double *list) {
   return (c < 0) ? NEG : POS;
 }
 
+// ------------------------------------------------- point-data sources ----
+// newDataSourceIds of the band operations: the point whose data a defined
+// result takes -- 2D for the centre, k for star neighbour k (nb order), -1
+// for none (no id is recorded).
+
+enum SrcKind : int { SRC_CENTRE = 0, SRC_EXPAND = 1, SRC_REBUILD = 2, SRC_PRUNE = 3, SRC_PRUNE_RSZ = 4, SRC_BOOLEAN = 5 };
+
+// Expand (lsExpand.hpp:107-157): the centre, else the neighbour that set the
+// distance (first of equal ones).
+template <int D> LSG_HD inline int srcExpand(double c, const double *nb, double totalLimit) {
+  if (fabs(c) <= totalLimit) return 2 * D;
+  int k = -1;
+  if (c > -DEPS) {
+    double dist = POS;
+    LSG_UNROLL
+    for (int i = 0; i < 2 * D; ++i)
+      if (dist > nb[i] + 1.0) {
+        dist = nb[i] + 1.0;
+        k = i;
+      }
+  } else {
+    double dist = NEG;
+    LSG_UNROLL
+    for (int i = 0; i < 2 * D; ++i)
+      if (dist < nb[i] - 1.0) {
+        dist = nb[i] - 1.0;
+        k = i;
+      }
+  }
+  return k;
+}
+// Advect::rebuildLS (lsAdvect.hpp:304-413)
+template <int D> LSG_HD inline int srcRebuild(double c, const double *nb) {
+  if (fabs(c) <= 1.0) {
+    if (c > 0.5) {
+      for (int j = 0; j < 2 * D; j++)
+        if (fabs(nb[j]) <= 1.0 && nb[j] < -0.5) return j;
+    } else if (c < -0.5) {
+      for (int j = 0; j < 2 * D; j++)
+        if (fabs(nb[j]) <= 1.0 && nb[j] > 0.5) return j;
+    }
+    return 2 * D;
+  }
+  int used = -1;
+  if (c >= 0) {
+    double dist = POS;
+    for (int i = 0; i < 2 * D; i++)
+      if (fabs(nb[i]) <= 1.0 && (nb[i] < 0.) && dist > nb[i] + 1.0) {
+        dist = nb[i] + 1.0;
+        used = i;
+      }
+  } else {
+    double dist = NEG;
+    for (int i = 0; i < 2 * D; i++)
+      if (fabs(nb[i]) <= 1.0 && (nb[i] > 0) && dist < nb[i] - 1.0) {
+        dist = nb[i] - 1.0;
+        used = i;
+      }
+  }
+  return used;
+}
+// Prune (lsPrune.hpp:135-167): a kept point records itself; the stray-zero
+// rewrite records nothing.
+template <int D> LSG_HD inline int srcPrune(double c, const double *nb, bool rsz) {
+  if (rsz && fabs(c) == 0.) {
+    LSG_UNROLL
+    for (int i = 0; i < D; ++i)
+      if (!pMonotone(nb[i], nb[D + i])) return -1;
+  }
+  return 2 * D;
+}
+// The comparator's second member: the result is taken from A
+// (lsBooleanOperation.hpp:250-280, lsAdvect.hpp:218-230).
+LSG_HD inline bool boolFromA(int kind, double a, double b) {
+  if (kind == B_INTERSECT) return a > b;
+  if (kind == B_UNION) return a < b;
+  if (kind == B_RELCOMP) return a > -b;
+  return !isUndef(a) && !isUndef(b);
+}
+
+// The rate (time * velocity) of the last list entry a point's update walks
+// to, and that entry's index (saveAdvectionVelocities, lsAdvect.hpp:733-765).
+LSG_HD inline double updateRateMM(double c, double dt, const double *list, int &last) {
+  double value = c, time = dt;
+  int k = 0;
+  double velocity = list[0];
+  double rate = time * velocity;
+  while (fabs(list[2 * k + 1] - value) < fabs(rate)) {
+    time -= fabs((list[2 * k + 1] - value) / velocity);
+    value = list[2 * k + 1];
+    ++k;
+    velocity = list[2 * k];
+    rate = time * velocity;
+  }
+  last = k;
+  return rate;
+}
+
 // ------------------------------------------------------- neighbourhood ----
 
 // Centre value and the 2D star neighbours (boundary mapped) of point c:
@@ -1270,7 +1368,8 @@ LSG_HD inline int schemeIdOf(int spatial) {
 template <int D, int SCHEME>
 LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi0, const RateParams &rp, const Vel &vel,
                                const double *globalAlphas, double *alphaAcc, double &velocity,
-                               const float *mlp = nullptr, float *io = nullptr, bool *sllfUsed = nullptr) {
+                               const float *mlp = nullptr, float *io = nullptr, bool *sllfUsed = nullptr,
+                               double *dnOut = nullptr) {
   RateOut r;
   if (SCHEME == S_SLLF) {
     bool used = false;
@@ -1285,6 +1384,7 @@ LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi
   else if (SCHEME == S_LLFA) r = schemeLLFA<D>(b, c, phi0, vel, alphaAcc);
   else r = schemeLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
   velocity = r.H - r.Dn;
+  if (dnOut) *dnOut = r.Dn;
   double v = velocity;
   if ((rp.checkDissipation && (r.H < 0 && v > 0)) || (r.H > 0 && v < 0)) v = 0;
   return v;

@@ -853,7 +853,9 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
     candCoords(v, i, v.pos[i], c);
     double velocity;
     bool used = false;
-    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp, io, &used);
+    double dn = 0.;
+    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp, io, &used, &dn);
+    if (!SEG && rp.dissOut) rp.dissOut[i] = dn;
     if (SCHEME == S_SLLF && used) sllfKey = (unsigned long long)(j + 1) << 4;  // lanes visit j in increasing order
     if (velocity != 0.) {
       const double a = fabs(velocity);
@@ -1074,11 +1076,11 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
           mat = m;
           break;
         }
-      double velocity = 0., v = 0.;
+      double velocity = 0., v = 0., dn = 0.;
       if (mat >= 0) {
         const Vel vm = velOfMaterial(rp, base, mat);
         bool used = false;
-        v = ratePoint<D, SCHEME>(b, c, phi0, rp, vm, ga, la, velocity, nullptr, nullptr, &used);
+        v = ratePoint<D, SCHEME>(b, c, phi0, rp, vm, ga, la, velocity, nullptr, nullptr, &used, &dn);
         if (SCHEME == S_SLLF && used) sllfKey = ((unsigned long long)(j + 1) << 4) | (unsigned long long)mat;
       }
       double stop;
@@ -1108,6 +1110,7 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
       }
       out[2 * k] = v;
       out[2 * k + 1] = stop;
+      if (rp.dissOut) rp.dissOut[(size_t)i * L + k] = dn;
       ++k;
       if (done) break;
     }
@@ -1466,6 +1469,173 @@ __global__ void kFill32(uint32_t *p, uint32_t v, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 void launchFill32(hipStream_t s, uint32_t *p, uint32_t v, size_t n) { kFill32<<<gridFor(n), TPB, 0, s>>>(p, v, n); }
+
+// ------------------------------------------------------------ point data ---
+// PointData::translateFromMultiData after a band operation: the point id of
+// every defined entry of the operation's input, then for every defined
+// entry of the output (in order) the id it takes its data from (the
+// newDataSourceIds of lsExpand.hpp:109-150, lsReduce.hpp:105-111,
+// lsPrune.hpp:145-167, lsBooleanOperation.hpp:104-115, lsAdvect.hpp:304-413),
+// then one gather per array.
+
+__global__ void kRankScatter(const uint32_t *dlist, const uint32_t *ndef, uint32_t *rankOf) {
+  const uint32_t n = *ndef;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) rankOf[dlist[k]] = k;
+}
+void launchPointIds(hipStream_t s, const BandBufs &b, Workspace &ws, uint32_t *dlist, uint32_t *ndef,
+                    uint32_t *rankOf) {
+  kDefinedCount<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials);
+  kDefinedList<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, dlist, ndef);
+  kRankScatter<<<gridFor(b.cap), TPB, 0, s>>>(dlist, ndef, rankOf);
+}
+
+struct PdSrcArgs {
+  BandView out, in, inB;  // the result, the operation's input (A), and B of a boolean
+  const uint32_t *outN;   // entries of the result
+  PdSrcParams p;
+  const uint32_t *rankIn, *rankB;
+};
+
+// Source of defined output entry i: false when it records none.
+template <int D>
+__device__ inline bool pdSource(const PdSrcArgs &a, uint32_t i, uint32_t &id, uint8_t &side, uint32_t &entry) {
+  if (isUndef(a.out.val[i])) return false;
+  int32_t c[3];
+  candCoords(a.out, i, a.out.pos[i], c);
+  if (a.p.kind == SRC_BOOLEAN) {
+    const BandView::Loc l = a.in.locate(c[0], c[1], c[2]);
+    const uint32_t ea = a.in.entryOf(l, a.in.bits[l.w], a.in.pre[l.w]);
+    const uint32_t eb = a.inB.entryOf(l, a.inB.bits[l.w], a.inB.pre[l.w]);  // same frame
+    const bool fromA = boolFromA(a.p.boolKind, a.in.val[ea], a.inB.val[eb]);
+    side = fromA ? 1 : 0;
+    entry = fromA ? ea : eb;
+    id = fromA ? a.rankIn[ea] : a.rankB[eb];
+    return true;
+  }
+  int s = 2 * D;
+  if (a.p.kind != SRC_CENTRE) {
+    double cv, nb[2 * D];
+    gatherStar<D>(a.in, c, cv, nb);
+    if (a.p.kind == SRC_EXPAND) s = srcExpand<D>(cv, nb, a.p.totalLimit);
+    else if (a.p.kind == SRC_REBUILD) s = srcRebuild<D>(cv, nb);
+    else s = srcPrune<D>(cv, nb, a.p.kind == SRC_PRUNE_RSZ);
+  }
+  if (s < 0) return false;
+  int32_t q[3] = {c[0], c[1], c[2]};
+  if (s < 2 * D) {
+    const int k = s % D;
+    q[k] = a.in.g.map(k, (int64_t)c[k] + (s < D ? 1 : -1));
+  }
+  const BandView::Loc l = a.in.locate(q[0], q[1], q[2]);
+  entry = a.in.entryOf(l, a.in.bits[l.w], a.in.pre[l.w]);
+  id = a.rankIn[entry];
+  side = 1;
+  return true;
+}
+
+template <int D> __global__ __launch_bounds__(256) void kPdSrcCount(PdSrcArgs a, uint32_t *partials) {
+  const uint32_t n = *a.outN;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
+  uint32_t acc = 0;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += TPB) {
+    uint32_t id, entry;
+    uint8_t side;
+    acc += pdSource<D>(a, i, id, side, entry) ? 1u : 0u;
+  }
+  const uint32_t t = blockSum(acc);
+  if (threadIdx.x == 0) partials[logicalBlock()] = t;
+}
+template <int D>
+__global__ __launch_bounds__(256) void kPdSrcList(PdSrcArgs a, const uint32_t *partials, uint32_t *src, uint8_t *sideOut,
+                                                  uint32_t *entryOut, uint32_t *nPush) {
+  const uint32_t n = *a.outN;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
+  uint32_t off = blockOffset(partials);
+  for (uint32_t t = lo; t < hi; t += TPB) {
+    const uint32_t i = t + threadIdx.x;
+    uint32_t id = 0, entry = 0;
+    uint8_t side = 0;
+    const bool push = (i < hi) && pdSource<D>(a, i, id, side, entry);
+    uint32_t tot;
+    uint64_t wm;
+    const uint32_t q = off + blockBallotScan(push, tot, wm);
+    if (push) {
+      src[q] = id;
+      sideOut[q] = side;
+      entryOut[q] = entry;
+    }
+    off += tot;
+  }
+  if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) *nPush = off;
+}
+
+void launchPdSources(hipStream_t s, int D, const BandBufs &out, const BandBufs &in, const BandBufs *inB, const Frame &f,
+                     const GridInfo &g, const PdSrcParams &p, Workspace &ws, const uint32_t *rankIn,
+                     const uint32_t *rankB, uint32_t *src, uint8_t *side, uint32_t *entry, uint32_t *nPush) {
+  PdSrcArgs a;
+  a.out = viewOf(out, f, g);
+  a.outN = out.n;
+  a.in = viewOf(in, f, g);
+  a.inB = inB ? viewOf(*inB, f, g) : a.in;
+  a.p = p;
+  a.rankIn = rankIn;
+  a.rankB = rankB ? rankB : rankIn;
+  if (D == 2) {
+    kPdSrcCount<2><<<g_scanBlocks, TPB, 0, s>>>(a, ws.partials);
+    kPdSrcList<2><<<g_scanBlocks, TPB, 0, s>>>(a, ws.partials, src, side, entry, nPush);
+  } else {
+    kPdSrcCount<3><<<g_scanBlocks, TPB, 0, s>>>(a, ws.partials);
+    kPdSrcList<3><<<g_scanBlocks, TPB, 0, s>>>(a, ws.partials, src, side, entry, nPush);
+  }
+}
+
+// out[k] = the source's tuple (A where side[k], else B; 0 without B or past
+// the end of the source array)
+__global__ void kPdGather(int comps, const double *A, const uint32_t *lenA, const double *B, const uint32_t *lenB,
+                          const uint32_t *src, const uint8_t *side, const uint32_t *nPush, double *out) {
+  const uint32_t n = *nPush, la = *lenA, lb = lenB ? *lenB : 0u;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t id = src[k];
+    const double *from = side[k] ? A : B;
+    const bool ok = from && id < (side[k] ? la : lb);
+    for (int j = 0; j < comps; ++j) out[(size_t)k * comps + j] = ok ? from[(size_t)id * comps + j] : 0.0;
+  }
+}
+void launchPdGather(hipStream_t s, int comps, const double *A, const uint32_t *lenA, const double *B,
+                    const uint32_t *lenB, const uint32_t *src, const uint8_t *side, const uint32_t *nPush, size_t cap,
+                    double *out) {
+  kPdGather<<<gridFor(cap), TPB, 0, s>>>(comps, A, lenA, B, lenB, src, side, nPush, out);
+}
+
+__global__ void kCopyU32(const uint32_t *from, uint32_t *to) { *to = *from; }
+void launchCopyU32(hipStream_t s, const uint32_t *from, uint32_t *to) { kCopyU32<<<1, 1, 0, s>>>(from, to); }
+
+// saveAdvectionVelocities (lsAdvect.hpp:733-796): per point of the reduced
+// band (slot k, input entry entry[k]) the last rate = time * velocity and the
+// dissipation of that list entry; one level set: rate = dt * rates[e].
+__global__ void kPdVelocities(const uint32_t *entry, const uint32_t *nPush, const double *val, const double *rates,
+                              const double *diss, const double *dtp, int nmat, double *vel, double *dis) {
+  const uint32_t n = *nPush;
+  const double dt = *dtp;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t e = entry[k];
+    if (nmat <= 1) {
+      vel[k] = dt * rates[e];
+      dis[k] = diss[e];
+    } else {
+      int last = 0;
+      vel[k] = updateRateMM(val[e], dt, rates + (size_t)e * 2 * nmat, last);
+      dis[k] = diss[(size_t)e * nmat + last];
+    }
+  }
+}
+void launchPdVelocities(hipStream_t s, const uint32_t *entry, const uint32_t *nPush, const double *val,
+                        const double *rates, const double *diss, const double *dt, int nmat, size_t cap, double *vel,
+                        double *dis) {
+  kPdVelocities<<<gridFor(cap), TPB, 0, s>>>(entry, nPush, val, rates, diss, dt, nmat, vel, dis);
+}
 
 // Direct evaluation of FiniteDifferences::weno3/weno5 on n stencils.
 __global__ void kWeno(int order, uint32_t n, const double *x, double dx, int plus, double *out) {

@@ -140,6 +140,21 @@ void launchCopyBand(hipStream_t s, const BandBufs &src, BandBufs &dst, const Fra
 void launchBandStats(hipStream_t s, const BandBufs &b, const Frame &f, uint32_t *outCount, int32_t *bbox);
 void launchFill32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 
+// Point data: point ids of a band, the sources of an operation's result,
+// one gather per array, the saved advection velocities.
+void launchPointIds(hipStream_t s, const BandBufs &b, Workspace &ws, uint32_t *dlist, uint32_t *ndef,
+                    uint32_t *rankOf);
+void launchPdSources(hipStream_t s, int D, const BandBufs &out, const BandBufs &in, const BandBufs *inB, const Frame &f,
+                     const GridInfo &g, const PdSrcParams &p, Workspace &ws, const uint32_t *rankIn,
+                     const uint32_t *rankB, uint32_t *src, uint8_t *side, uint32_t *entry, uint32_t *nPush);
+void launchPdGather(hipStream_t s, int comps, const double *A, const uint32_t *lenA, const double *B,
+                    const uint32_t *lenB, const uint32_t *src, const uint8_t *side, const uint32_t *nPush, size_t cap,
+                    double *out);
+void launchCopyU32(hipStream_t s, const uint32_t *from, uint32_t *to);
+void launchPdVelocities(hipStream_t s, const uint32_t *entry, const uint32_t *nPush, const double *val,
+                        const double *rates, const double *diss, const double *dt, int nmat, size_t cap, double *vel,
+                        double *dis);
+
 }  // namespace lsg
 
 namespace lsg {
