@@ -105,6 +105,10 @@ int lsg_domain_point_data_label(const lsg_domain *d, int components, int i, char
 int64_t lsg_domain_point_data_size(const lsg_domain *d, int components, int i);
 int lsg_domain_point_data_get(const lsg_domain *d, int components, int i, double *out, int64_t n);
 int lsg_domain_point_data_clear(lsg_domain *d);
+/* MarkVoidPoints<T,D>::apply with its default options, as Advect uses it
+ * (lsMarkVoidPoints.hpp:156-331, lsAdvect.hpp:443-447): point data
+ * "VoidPointMarkers", 1 for a point enclosed away from the top region. */
+int lsg_mark_void_points(lsg_domain *d);
 
 /* ---- MakeGeometry (lsMakeGeometry.hpp) --------------------------------- */
 /* makeSphere (lsMakeGeometry.hpp:177-241). */
@@ -172,7 +176,7 @@ int lsg_advect_set_single_step(lsg_advect *a, int single);        /* :879 */
 int lsg_advect_set_dissipation_alpha(lsg_advect *a, double alpha);/* :955 */
 int lsg_advect_set_check_dissipation(lsg_advect *a, int check);   /* :958 */
 int lsg_advect_set_calculate_normal_vectors(lsg_advect *a, int c);/* :891 */
-int lsg_advect_set_ignore_voids(lsg_advect *a, int iv);           /* :899; only 0 supported */
+int lsg_advect_set_ignore_voids(lsg_advect *a, int iv);           /* :899 */
 int lsg_advect_set_adaptive_time_stepping(lsg_advect *a, int on, unsigned subdivisions); /* :904; acts with several level sets */
 int lsg_advect_set_update_point_data(lsg_advect *a, int update);  /* :962; rebuildLS translates or drops point data */
 /* Advect::setSaveAdvectionVelocities (lsAdvect.hpp:916): point data

@@ -258,6 +258,18 @@ template <class T, int D> struct Box {
   template <class... A> static auto New(A &&...a) { return SmartPointer<Box>::New(std::forward<A>(a)...); }
 };
 
+// MarkVoidPoints with its default options (lsMarkVoidPoints.hpp:156-331).
+template <class T, int D> class MarkVoidPoints {
+  SmartPointer<Domain<T, D>> ls_;
+
+public:
+  static constexpr char voidPointLabel[] = "VoidPointMarkers";
+  MarkVoidPoints() = default;
+  explicit MarkVoidPoints(SmartPointer<Domain<T, D>> ls) : ls_(ls) {}
+  void setLevelSet(SmartPointer<Domain<T, D>> ls) { ls_ = ls; }
+  void apply() { detail::ok(lsg_mark_void_points(ls_->handle()), "MarkVoidPoints"); }
+};
+
 template <class T, int D> class MakeGeometry {
   SmartPointer<Domain<T, D>> ls_;
   int kind_ = -1;

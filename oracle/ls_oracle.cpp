@@ -290,6 +290,111 @@ static std::vector<int64_t> sourceIds(const Band<D> &in, const Band<D> &out, S &
   return ids;
 }
 
+// ------------------------------------------------------ void detection ---
+
+// MarkVoidPoints<T,D>::apply (lsMarkVoidPoints.hpp:156-331) with the options
+// Advect uses (lsAdvect.hpp:443-453): point data "VoidPointMarkers", 1 for a
+// void point.  Components are the face-connected regions of one sign
+// (value >= 0 or < 0, :38-40) of the grid; periodic ends connect, reflective
+// and infinite ends add nothing.  The reference builds them over ViennaHRLE
+// runs (absent from the reference tree) and takes as top the component of
+// the last vertex it created in HRLE order, stepping down to one with
+// non-negative points (:262-280); here the top is the component of the
+// lexicographically last grid point with a value >= 0 -- the same region for
+// a surface with the open side up (the reference's tests).  Infinite
+// dimensions are covered by the defined points' bounding box +-2: beyond it
+// every layer repeats the sign pattern of the first one.  A point >= 0 is
+// void when outside the top; a negative point when none of its star
+// neighbours of the other sign bit lies in the top (:288-312).
+template <int D> static std::vector<double> markVoidPoints(const Band<D> &ls) {
+  const Grid<D> &g = ls.g;
+  std::vector<double> marks;
+  int32_t bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+  bool any = false;
+  for (size_t e = 0; e < ls.idx.size(); ++e) {
+    if (isUndef(ls.val[e])) continue;
+    for (int d = 0; d < D; ++d) {
+      bmin[d] = any ? std::min(bmin[d], ls.idx[e][d]) : ls.idx[e][d];
+      bmax[d] = any ? std::max(bmax[d], ls.idx[e][d]) : ls.idx[e][d];
+    }
+    any = true;
+  }
+  if (!any) return marks;
+  int64_t lo[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
+  for (int d = 0; d < D; ++d) {
+    if (g.bounded(d)) {
+      lo[d] = g.lo[d];
+      ext[d] = (int64_t)g.hi[d] - g.lo[d] + 1;
+    } else {
+      lo[d] = std::max<int64_t>((int64_t)bmin[d] - 2, g.lo[d]);
+      const int64_t hi = std::min<int64_t>((int64_t)bmax[d] + 2, g.hi[d]);
+      ext[d] = hi - lo[d] + 1;
+    }
+  }
+  const int64_t N = ext[0] * ext[1] * ext[2];
+  auto qOf = [&](const Idx<D> &p) {
+    int64_t q = 0;
+    for (int d = D - 1; d >= 0; --d) q = q * ext[d] + ((int64_t)p[d] - lo[d]);
+    return q;
+  };
+  auto pOf = [&](int64_t q) {
+    Idx<D> p;
+    for (int d = 0; d < D; ++d) {
+      p[d] = (int32_t)(lo[d] + q % ext[d]);
+      q /= ext[d];
+    }
+    return p;
+  };
+  std::vector<char> pos((size_t)N);
+  for (int64_t q = 0; q < N; ++q) pos[(size_t)q] = ls.value(pOf(q)) >= 0 ? 1 : 0;
+  std::vector<int64_t> parent((size_t)N);
+  for (int64_t q = 0; q < N; ++q) parent[(size_t)q] = q;
+  auto find = [&](int64_t a) {
+    while (parent[(size_t)a] != a) {
+      parent[(size_t)a] = parent[(size_t)parent[(size_t)a]];
+      a = parent[(size_t)a];
+    }
+    return a;
+  };
+  for (int64_t q = 0; q < N; ++q) {
+    const Idx<D> p = pOf(q);
+    for (int d = 0; d < D; ++d) {
+      Idx<D> n = p;
+      if ((int64_t)p[d] - lo[d] + 1 < ext[d]) n[d] = p[d] + 1;
+      else if (g.bc[d] == PERIODIC) n[d] = g.map(d, (int64_t)p[d] + 1);
+      else continue;
+      const int64_t r = qOf(n);
+      if (pos[(size_t)q] != pos[(size_t)r]) continue;
+      const int64_t a = find(q), b = find(r);
+      if (a != b) parent[(size_t)std::max(a, b)] = std::min(a, b);
+    }
+  }
+  int64_t top = -1;
+  for (int64_t q = N - 1; q >= 0 && top < 0; --q)
+    if (pos[(size_t)q]) top = find(q);
+  for (size_t e = 0; e < ls.idx.size(); ++e) {
+    const double c = ls.val[e];
+    if (isUndef(c)) continue;
+    const Idx<D> &p = ls.idx[e];
+    bool isVoid;
+    if (c >= 0) {
+      isVoid = find(qOf(p)) != top;
+    } else {
+      isVoid = true;
+      for (int k = 0; k < 2 * D && isVoid; ++k) {
+        Idx<D> n = p;
+        const int d = k % D;
+        n[d] = g.map(d, (int64_t)p[d] + (k < D ? 1 : -1));
+        const double v = ls.value(n);
+        if (std::signbit(v) == std::signbit(c)) continue;
+        if (find(qOf(n)) == top) isVoid = false;
+      }
+    }
+    marks.push_back(isVoid ? 1.0 : 0.0);
+  }
+  return marks;
+}
+
 // ------------------------------------------------------------ band ops ---
 
 // Expand<T,D>::apply (lsExpand.hpp:44-172)
@@ -1360,6 +1465,13 @@ template <int D> struct Advector {
     double alphaAcc[3] = {0, 0, 0};
     double active = 0;
     const double MAXV = std::numeric_limits<double>::max();
+    std::vector<double> voidMark;
+    std::vector<int64_t> dr;
+    if (P.ignoreVoids) {
+      voidMark = markVoidPoints(ls);
+      ls.pd.insertReplaceScalar("VoidPointMarkers", voidMark);
+      dr = definedRanks(ls);
+    }
     for (size_t e = 0; e < n; ++e) {
       double value = ls.val[e];
       if (isUndef(value) || std::fabs(value) > 0.5) continue;  // :498-499
@@ -1372,11 +1484,12 @@ template <int D> struct Advector {
       int k = 0;
       for (int cur = L - 1; cur >= 0; --cur) {  // :507-589
         double H = 0, Dn = 0;
-        for (int m = 0; m < L; ++m)
-          if (lv[m] <= value + WRAP_EPS) {
-            scheme(s, m, c, order, H, Dn, alphaAcc);
-            break;
-          }
+        if (voidMark.empty() || voidMark[(size_t)dr[e]] == 0.)  // void points: no scheme call (:510)
+          for (int m = 0; m < L; ++m)
+            if (lv[m] <= value + WRAP_EPS) {
+              scheme(s, m, c, order, H, Dn, alphaAcc);
+              break;
+            }
         const double velocity = H - Dn;
         double stop;
         bool done = true;
@@ -1535,6 +1648,14 @@ template <int D> struct Advector {
     // SLLF: finalAlphas of the last point in entry order that set them; a
     // thread's dynamic chunks come in increasing entry order
     int64_t lastE = -1;
+    // ignoreVoids: void points skip the scheme (lsAdvect.hpp:443-453, 510-528)
+    std::vector<double> voidMark;
+    std::vector<int64_t> dr;
+    if (P.ignoreVoids) {
+      voidMark = markVoidPoints(ls);
+      ls.pd.insertReplaceScalar("VoidPointMarkers", voidMark);
+      dr = definedRanks(ls);
+    }
 #pragma omp parallel
     {
       double locMax = maxTimeStep;
@@ -1549,7 +1670,9 @@ template <int D> struct Advector {
         locActive += 1;
         double H = 0, Dn = 0;
         const Idx<D> &c = ls.idx[e];
-        switch (P.spatial) {
+        if (!voidMark.empty() && voidMark[(size_t)dr[e]] != 0.) {
+          // void point: no scheme call, velocity 0
+        } else switch (P.spatial) {
         case EO1: case EO2: schemeEO<D>(s, c, order, H, Dn); break;
         case LF1: case LF2: schemeLF<D>(s, c, order, H, Dn); break;
         case LLLF1: case LLLF2: schemeLLLF<D>(s, c, order, H, Dn, locAlpha); break;
@@ -1881,6 +2004,11 @@ void lso_pd_get(const lso_domain *d, int kind, int i, double *out) {
   }
 }
 void lso_pd_clear(lso_domain *d) { pdOf(d).clear(); }
+// MarkVoidPoints with Advect's options: inserts / replaces "VoidPointMarkers"
+void lso_mark_void_points(lso_domain *d) {
+  if (d->D == 2) d->b2.pd.insertReplaceScalar("VoidPointMarkers", markVoidPoints<2>(d->b2));
+  else d->b3.pd.insertReplaceScalar("VoidPointMarkers", markVoidPoints<3>(d->b3));
+}
 
 void lso_expand(lso_domain *d, int width) {
   if (d->D == 2) expand<2>(d->b2, width);
@@ -1961,6 +2089,7 @@ struct lso_params {
   double velAlpha[3];
   int updatePointData;
   int saveVelocities;
+  int ignoreVoids;
 };
 
 static AdvectParams toParams(const lso_params *p) {
@@ -1998,6 +2127,7 @@ static AdvectParams toParams(const lso_params *p) {
   P.subdivisions = p->subdivisions < 1 ? 1u : (unsigned)p->subdivisions;
   P.updatePointData = p->updatePointData;
   P.saveVelocities = p->saveVelocities;
+  P.ignoreVoids = p->ignoreVoids;
   return P;
 }
 

@@ -207,6 +207,7 @@ struct AdvectParams {
   unsigned subdivisions = 20;
   int updatePointData = 1;      // setUpdatePointData (lsAdvect.hpp:75, 962)
   int saveVelocities = 0;       // setSaveAdvectionVelocities (lsAdvect.hpp:74, 916)
+  int ignoreVoids = 0;          // setIgnoreVoids (lsAdvect.hpp:899)
   const Velocity &velFor(int m) const { return (m >= 0 && m < (int)matVel.size()) ? matVel[m] : vel; }
 };
 

@@ -47,6 +47,7 @@ class Params(ctypes.Structure):
         ("velAlpha", ctypes.c_double * 3),
         ("updatePointData", ctypes.c_int),
         ("saveVelocities", ctypes.c_int),
+        ("ignoreVoids", ctypes.c_int),
     ]
 
 
@@ -109,6 +110,7 @@ def lib():
         L.lso_pd_size.restype = ctypes.c_int64
         L.lso_pd_get.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
         L.lso_pd_clear.argtypes = [P]
+        L.lso_mark_void_points.argtypes = [P]
         L.lso_num_threads.restype = ctypes.c_int
         L.lso_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -232,11 +234,14 @@ class Domain:
     def clear_data(self):
         lib().lso_pd_clear(self.h)
 
+    def mark_void_points(self):
+        lib().lso_mark_void_points(self.h)
+
 
 def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
                 dissipation_alpha=1.0, calc_normals=True, check_dissipation=True,
                 vel=None, mlp=None, mat_vels=None, adaptive=False, subdivisions=20,
-                update_point_data=True, save_velocities=False):
+                update_point_data=True, save_velocities=False, ignore_voids=False):
     """vel: dict(kind, dir, k, b, a_pos, a_neg, vec) of the built-in velocity.
     mlp: float32 array of the learned-weno5 network (spatial scheme 12).
     mat_vels: list of such dicts, the velocity of materials 0, 1, ...
@@ -245,6 +250,7 @@ def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
     p = Params()
     p.updatePointData = int(bool(update_point_data))
     p.saveVelocities = int(bool(save_velocities))
+    p.ignoreVoids = int(bool(ignore_voids))
     p.adaptive = int(bool(adaptive))
     p.subdivisions = int(subdivisions)
     if mlp is not None:

@@ -14,7 +14,8 @@ from ._lib import LsgError, check, lib
 __all__ = [
     "BoundaryConditionEnum", "SpatialSchemeEnum", "TemporalSchemeEnum", "BooleanOperationEnum",
     "Domain", "Sphere", "Plane", "Box", "MakeGeometry", "Expand", "Reduce", "Prune", "BooleanOperation",
-    "VelocityField", "HostVelocityField", "Advect", "BatchAdvect", "LsgError", "weno",
+    "VelocityField", "HostVelocityField", "Advect", "BatchAdvect", "LsgError", "weno", "PointData",
+    "MarkVoidPoints",
 ]
 
 POS_VALUE = float(np.finfo(np.float64).max)
@@ -365,6 +366,22 @@ class Prune:
 
     def apply(self):
         check(lib().lsg_prune(self.domain._h, int(self.rsz)))
+
+
+class MarkVoidPoints:
+    """MarkVoidPoints(domain).apply() with the default options
+    (lsMarkVoidPoints.hpp:156-331): point data "VoidPointMarkers"."""
+
+    voidPointLabel = "VoidPointMarkers"
+
+    def __init__(self, domain=None):
+        self.domain = domain
+
+    def setLevelSet(self, domain):
+        self.domain = domain
+
+    def apply(self):
+        check(lib().lsg_mark_void_points(self.domain._h))
 
 
 class BooleanOperation:

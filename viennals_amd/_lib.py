@@ -39,7 +39,7 @@ EXPORTED = [
     "lsg_batch_set_scan_blocks", "lsg_batch_autotune", "lsg_advect_set_velocity_host",
     "lsg_advect_set_save_advection_velocities", "lsg_domain_point_data_insert", "lsg_domain_point_data_count",
     "lsg_domain_point_data_label", "lsg_domain_point_data_size", "lsg_domain_point_data_get",
-    "lsg_domain_point_data_clear",
+    "lsg_domain_point_data_clear", "lsg_mark_void_points",
 ]
 
 
@@ -106,6 +106,7 @@ def lib():
         "lsg_domain_point_data_size": ([P, c_int, c_int], ctypes.c_int64),
         "lsg_domain_point_data_get": ([P, c_int, c_int, ctypes.c_void_p, ctypes.c_int64], c_int),
         "lsg_domain_point_data_clear": ([P], c_int),
+        "lsg_mark_void_points": ([P], c_int),
         "lsg_advect_prepare_ls": ([P], c_int),
         "lsg_advect_apply": ([P], c_int),
         "lsg_advect_get_advected_time": ([P, dp], c_int),

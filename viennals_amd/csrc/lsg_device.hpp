@@ -256,6 +256,10 @@ struct RateParams {
   // saveAdvectionVelocities: the scheme's dissipation per point (one level
   // set: per band entry; several: per entry and list slot), else null
   double *dissOut;
+  // ignoreVoids: the void marker per point id and the point id of every
+  // band entry; a void point gets no scheme call (lsAdvect.hpp:510-528)
+  const double *voidMarks;
+  const uint32_t *voidRank;
 };
 
 // Which point a band operation's result takes its point data from.

@@ -531,6 +531,60 @@ static void pdTranslate(lsg_domain *d, const PdSrcParams &p, const lsg_domain *B
   if (d->pd.empty()) return;
   if (pdSources(d, p, B) == 0) (void)pdApply(d, B);
 }
+// MarkVoidPoints on the expanded band before the rate pass (lsAdvect.hpp:443-
+// 453): "VoidPointMarkers" becomes point data of the level set (one value
+// per point of this band) and the rate kernels skip the void points.
+struct VoidWork {
+  uint32_t *dlist = nullptr, *ndef = nullptr, *rank = nullptr, *par = nullptr, *topQ = nullptr;
+  uint8_t *sign = nullptr;
+  size_t cap = 0;
+  uint64_t nbox = 0;
+  int ensure(size_t c, uint64_t N) {
+    if (c > cap) {
+      LSG_HIP(hipStreamSynchronize(g_ctx.s));
+      for (void *p : {(void *)dlist, (void *)rank}) if (p) (void)hipFree(p);
+      LSG_HIP(hipMalloc(&dlist, c * sizeof(uint32_t)));
+      LSG_HIP(hipMalloc(&rank, c * sizeof(uint32_t)));
+      cap = c;
+    }
+    if (N > nbox) {
+      LSG_HIP(hipStreamSynchronize(g_ctx.s));
+      for (void *p : {(void *)par, (void *)sign}) if (p) (void)hipFree(p);
+      LSG_HIP(hipMalloc(&par, N * sizeof(uint32_t)));
+      LSG_HIP(hipMalloc(&sign, N));
+      nbox = N;
+    }
+    if (!ndef) LSG_HIP(hipMalloc(&ndef, sizeof(uint32_t)));
+    if (!topQ) LSG_HIP(hipMalloc(&topQ, sizeof(uint32_t)));
+    return 0;
+  }
+};
+static VoidWork *g_void = nullptr;
+static void pdInsertEmpty(lsg_domain *d, const char *label) {
+  for (PdArray &a : d->pd)
+    if (a.comps == 1 && a.label == label) return;
+  d->pd.emplace_back();
+  d->pd.back().label = label;
+}
+static int markVoids(lsg_domain *d, RateParams &rp) {
+  if (!g_void) g_void = new VoidWork();
+  const uint64_t N = (uint64_t)d->f.ext[0] * d->f.ext[1] * d->f.ext[2];
+  if (N >= 0xFFFFFFFEull) return fail(-5, "Advect: ignoreVoids needs a frame of fewer than 2^32 points");
+  if (int e = g_void->ensure(d->cur.b.cap, N)) return e;
+  launchPointIds(g_ctx.s, d->cur.b, g_ws->w, g_void->dlist, g_void->ndef, g_void->rank);
+  pdInsertEmpty(d, "VoidPointMarkers");  // MarkVoidPoints::voidPointLabel (lsMarkVoidPoints.hpp:86)
+  if (int e = pdEnsure(d, d->cur.b.cap)) return e;
+  double *marks = nullptr;
+  for (PdArray &a : d->pd)
+    if (a.comps == 1 && a.label == "VoidPointMarkers") marks = a.buf;
+  launchVoidMarks(g_ctx.s, d->D, d->cur.b, d->f, d->g, g_void->sign, g_void->par, g_void->topQ, g_void->dlist,
+                  g_void->ndef, marks);
+  launchCopyU32(g_ctx.s, g_void->ndef, d->pdLen);
+  rp.voidMarks = marks;
+  rp.voidRank = g_void->rank;
+  return 0;
+}
+
 static void pdClear(lsg_domain *d) {
   if (d->pd.empty()) return;
   (void)hipStreamSynchronize(g_ctx.s);
@@ -898,6 +952,7 @@ static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *d
     rp.hostVel = a->velOut;
   }
   if (a->spatial == 2 || a->spatial == 3) launchGlobalAlphas(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, g_ctx.sc);
+  if (a->ignoreVoids && !a->hostErr) a->hostErr = markVoids(d, rp);
   if (a->profiling && a->evUsed + 2 <= (int)a->evPool.size()) {
     // events bracket the rate kernel itself
     hipEvent_t e0 = a->evPool[a->evUsed], e1 = a->evPool[a->evUsed + 1];
@@ -1040,7 +1095,7 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
     const bool hostVel = a->hostScalar || a->hostVector;
     a->hostErr = 0;
     // point data allocates during the step: such steps are launched directly
-    const bool pointData = !d->pd.empty() || a->saveVelocities;
+    const bool pointData = !d->pd.empty() || a->saveVelocities || a->ignoreVoids;
     if (!pointData) {
       pdClear(a->save);
       if (a->init) pdClear(a->init);
@@ -1173,6 +1228,7 @@ static void stageMM(lsg_advect *a, const double *maxDt, double *dtOut) {
   }
   RateParams rp = a->rp;
   rp.dissOut = a->saveVelocities ? a->diss : nullptr;
+  if (a->ignoreVoids && !a->hostErr) a->hostErr = markVoids(top, rp);
   launchRatesMM(g_ctx.s, top->D, top->cur.b, top->f, top->g, lw, rp, a->rates, g_ctx.sc, maxDt, dtOut, g_ws->w);
   OpParams op{};
   op.kind = OP_UPDATE_MM;
@@ -1255,7 +1311,9 @@ static int runStepMM(lsg_advect *a, double maxTimeStep, StepOut &out) {
     std::vector<int> widths;
     for (lsg_domain *d : a->levelSets) widths.push_back(d->width);
     *a->hostMaxDt = maxTimeStep;
+    a->hostErr = 0;
     if (int e = enqueueStepMM(a)) return e;
+    if (a->hostErr) return a->hostErr;
     LSG_HIP(hipMemcpyAsync(a->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
     LSG_HIP(hipStreamSynchronize(g_ctx.s));
     const DevScalars &h = *a->hostSc;
@@ -1291,7 +1349,6 @@ static int advectApply(lsg_advect *a) {
       return fail(-3, "Advect: with several level sets only the built-in velocity functors are supported");
   }
   if (prepareWidth(a->spatial) < 0) return fail(-4, "Advect: Discretization scheme not found.");
-  if (a->ignoreVoids) return fail(-5, "Advect: setIgnoreVoids(true) is not supported");
   if (!(a->cfl > 0)) return fail(-6, "Advect: time step ratio must be positive");
   if (int e = ctxInit()) return e;
   lsg_domain *d = a->levelSets.back();
@@ -1982,6 +2039,15 @@ int lsg_domain_point_data_get(const lsg_domain *d, int components, int i, double
 }
 int lsg_domain_point_data_clear(lsg_domain *d) {
   pdClear(d);
+  return 0;
+}
+int lsg_mark_void_points(lsg_domain *d) {
+  if (int e = ctxInit()) return e;
+  if (!d->framed) return fail(-7, "MarkVoidPoints: level set has no data");
+  if (int e = ensureStores(d, d->cur.b.cap)) return e;
+  RateParams rp{};
+  if (int e = markVoids(d, rp)) return e;
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
   return 0;
 }
 

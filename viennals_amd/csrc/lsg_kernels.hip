@@ -849,6 +849,11 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
         curSeg = k;
       }
     }
+    if (!SEG && rp.voidMarks && rp.voidMarks[rp.voidRank[i]] != 0.) {  // void point: velocity 0
+      rates[i] = 0.;
+      if (rp.dissOut) rp.dissOut[i] = 0.;
+      return;
+    }
     int32_t c[3];
     candCoords(v, i, v.pos[i], c);
     double velocity;
@@ -1066,6 +1071,7 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
       }
     }
     lv[L - 1] = phi0;
+    const bool isVoid = rp.voidMarks && rp.voidMarks[rp.voidRank[i]] != 0.;
     double value = phi0, cfl = rp.cfl, maxStepTime = 0.;
     double *out = lists + (size_t)i * 2 * L;
     int k = 0;
@@ -1077,7 +1083,7 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
           break;
         }
       double velocity = 0., v = 0., dn = 0.;
-      if (mat >= 0) {
+      if (mat >= 0 && !isVoid) {
         const Vel vm = velOfMaterial(rp, base, mat);
         bool used = false;
         v = ratePoint<D, SCHEME>(b, c, phi0, rp, vm, ga, la, velocity, nullptr, nullptr, &used, &dn);
@@ -1635,6 +1641,141 @@ void launchPdVelocities(hipStream_t s, const uint32_t *entry, const uint32_t *nP
                         const double *rates, const double *diss, const double *dt, int nmat, size_t cap, double *vel,
                         double *dis) {
   kPdVelocities<<<gridFor(cap), TPB, 0, s>>>(entry, nPush, val, rates, diss, dt, nmat, vel, dis);
+}
+
+// -------------------------------------------------------- void detection ---
+// MarkVoidPoints with Advect's options (lsMarkVoidPoints.hpp:156-331): the
+// face-connected regions of one sign (value >= 0 or < 0) of the frame box,
+// by lock-free union-find: while the links are made, parents are read with
+// device-scope atomic loads and written only by compare-and-swap (a root is
+// only ever linked to a smaller root), so every XCD sees every link; a
+// separate pass then points every entry at its root.  The top is the region
+// of the last frame point with a value >= 0 in lexicographic order.
+// q = (z ext1 + y) ext0 + x over the frame box.
+
+__device__ inline uint32_t ufLoad(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t ufRoot(const uint32_t *par, uint32_t a) {
+  uint32_t p = ufLoad(&par[a]);
+  while (p != a) {
+    a = p;
+    p = ufLoad(&par[a]);
+  }
+  return a;
+}
+__device__ inline void ufUnion(uint32_t *par, uint32_t a, uint32_t b) {
+  a = ufRoot(par, a);
+  b = ufRoot(par, b);
+  while (a != b) {
+    if (a < b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    const uint32_t old = atomicCAS(&par[a], a, b);
+    if (old == a) return;
+    a = ufRoot(par, old);
+    b = ufRoot(par, b);
+  }
+}
+
+__device__ inline void voidCoords(const Frame &f, uint64_t q, int32_t r[3]) {
+  r[0] = (int32_t)(q % (uint64_t)f.ext[0]);
+  const uint64_t t = q / (uint64_t)f.ext[0];
+  r[1] = (int32_t)(t % (uint64_t)f.ext[1]);
+  r[2] = (int32_t)(t / (uint64_t)f.ext[1]);
+}
+
+__global__ void kVoidInit(BandView b, uint64_t N, uint8_t *sign, uint32_t *par, uint32_t *topQ) {
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < N; q += (uint64_t)gridDim.x * blockDim.x) {
+    int32_t r[3];
+    voidCoords(b.f, q, r);
+    sign[q] = b.value(r[0] + b.f.o[0], r[1] + b.f.o[1], r[2] + b.f.o[2]) >= 0 ? 1 : 0;
+    par[q] = (uint32_t)q;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *topQ = 0;
+}
+template <int D> __global__ void kVoidHook(Frame f, GridInfo g, uint64_t N, const uint8_t *sign, uint32_t *par) {
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < N; q += (uint64_t)gridDim.x * blockDim.x) {
+    int32_t r[3];
+    voidCoords(f, q, r);
+    uint64_t stride = 1;
+    for (int d = 0; d < D; ++d) {
+      uint64_t n;
+      bool has = true;
+      if (r[d] + 1 < f.ext[d]) n = q + stride;
+      else if (g.bc[d] == PERIODIC) n = q - (uint64_t)(f.ext[d] - 1) * stride;
+      else has = false;
+      if (has && sign[n] == sign[q]) ufUnion(par, (uint32_t)q, (uint32_t)n);
+      stride *= (uint64_t)f.ext[d];
+    }
+  }
+}
+// (the links are final here: a thread reads either an original parent or a
+// root another thread already stored, both on the path to the same root)
+__global__ void kVoidFlatten(uint64_t N, const uint8_t *sign, uint32_t *par, uint32_t *topQ) {
+  uint32_t best = 0;
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < N; q += (uint64_t)gridDim.x * blockDim.x) {
+    par[q] = ufRoot(par, (uint32_t)q);
+    if (sign[q]) best = (uint32_t)q + 1u;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t y = __shfl_down(best, o, 64);
+    best = (y > best) ? y : best;
+  }
+  if ((threadIdx.x & 63) == 0 && best) atomicMax(topQ, best);
+}
+// Marker of the k-th defined entry (point id k).
+template <int D>
+__global__ void kVoidMarks(BandView b, const uint32_t *dlist, const uint32_t *ndef, const uint32_t *par,
+                           const uint32_t *topQ, double *marks) {
+  const uint32_t n = *ndef;
+  const uint32_t tq = *topQ;
+  const uint32_t top = tq ? par[tq - 1] : 0xFFFFFFFFu;
+  auto label = [&](const int32_t c[3]) -> uint32_t {
+    int64_t r[3], q = 0;
+    for (int d = 0; d < 3; ++d) {
+      r[d] = (int64_t)c[d] - b.f.o[d];
+      if (r[d] < 0 || r[d] >= b.f.ext[d]) return 0xFFFFFFFEu;
+    }
+    q = (r[2] * b.f.ext[1] + r[1]) * b.f.ext[0] + r[0];
+    return par[q];
+  };
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t i = dlist[k];
+    int32_t c[3];
+    candCoords(b, i, b.pos[i], c);
+    const double v = b.val[i];
+    bool isVoid;
+    if (v >= 0) {
+      isVoid = label(c) != top;
+    } else {
+      isVoid = true;
+      for (int s = 0; s < 2 * D && isVoid; ++s) {
+        int32_t p[3] = {c[0], c[1], c[2]};
+        const int d = s % D;
+        p[d] = b.g.map(d, (int64_t)c[d] + (s < D ? 1 : -1));
+        const double w = b.value(p[0], p[1], p[2]);
+        if (signbit(w) == signbit(v)) continue;
+        if (label(p) == top) isVoid = false;
+      }
+    }
+    marks[k] = isVoid ? 1.0 : 0.0;
+  }
+}
+
+void launchVoidMarks(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, uint8_t *sign,
+                     uint32_t *par, uint32_t *topQ, const uint32_t *dlist, const uint32_t *ndef, double *marks) {
+  const uint64_t N = (uint64_t)f.ext[0] * f.ext[1] * f.ext[2];
+  BandView v = viewOf(b, f, g);
+  const int gb = gridFor((size_t)N, 8192);
+  kVoidInit<<<gb, TPB, 0, s>>>(v, N, sign, par, topQ);
+  if (D == 2) kVoidHook<2><<<gb, TPB, 0, s>>>(f, g, N, sign, par);
+  else kVoidHook<3><<<gb, TPB, 0, s>>>(f, g, N, sign, par);
+  kVoidFlatten<<<gb, TPB, 0, s>>>(N, sign, par, topQ);
+  if (D == 2) kVoidMarks<2><<<gridFor(b.cap), TPB, 0, s>>>(v, dlist, ndef, par, topQ, marks);
+  else kVoidMarks<3><<<gridFor(b.cap), TPB, 0, s>>>(v, dlist, ndef, par, topQ, marks);
 }
 
 // Direct evaluation of FiniteDifferences::weno3/weno5 on n stencils.

@@ -151,6 +151,11 @@ void launchPdGather(hipStream_t s, int comps, const double *A, const uint32_t *l
                     const uint32_t *lenB, const uint32_t *src, const uint8_t *side, const uint32_t *nPush, size_t cap,
                     double *out);
 void launchCopyU32(hipStream_t s, const uint32_t *from, uint32_t *to);
+// MarkVoidPoints: the marker (1 void, 0 not) of every defined point of b
+// (dlist / ndef: the defined entries in order); sign / par hold one byte /
+// one word per frame point.
+void launchVoidMarks(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, uint8_t *sign,
+                     uint32_t *par, uint32_t *topQ, const uint32_t *dlist, const uint32_t *ndef, double *marks);
 void launchPdVelocities(hipStream_t s, const uint32_t *entry, const uint32_t *nPush, const double *val,
                         const double *rates, const double *diss, const double *dt, int nmat, size_t cap, double *vel,
                         double *dis);
