@@ -110,6 +110,17 @@ int lsg_domain_point_data_clear(lsg_domain *d);
  * "VoidPointMarkers", 1 for a point enclosed away from the top region. */
 int lsg_mark_void_points(lsg_domain *d);
 
+/* Surface points of a level set: the zero crossing on every grid edge between
+ * face-adjacent defined points of opposite sign, linearly interpolated, in
+ * physical coordinates (D doubles each) -- the node positions of
+ * ToSurfaceMesh (lsToSurfaceMesh.hpp) without its node merging.  *n receives
+ * the count; out may be null to query it. */
+int lsg_surface_points(lsg_domain *d, double *out, int64_t cap, int64_t *n);
+/* CompareChamfer::apply (lsCompareChamfer.hpp:104-245) on those surface
+ * points: out[0..4] = forward (target -> sample), backward, Chamfer, RMS and
+ * maximum nearest-neighbour distance; counts[0..1] = target / sample points. */
+int lsg_compare_chamfer(lsg_domain *target, lsg_domain *sample, double *out, int64_t *counts);
+
 /* ---- MakeGeometry (lsMakeGeometry.hpp) --------------------------------- */
 /* makeSphere (lsMakeGeometry.hpp:177-241). */
 int lsg_make_sphere(lsg_domain *d, const double *origin, double radius);

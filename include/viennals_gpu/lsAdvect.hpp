@@ -270,6 +270,27 @@ public:
   void apply() { detail::ok(lsg_mark_void_points(ls_->handle()), "MarkVoidPoints"); }
 };
 
+// CompareChamfer (lsCompareChamfer.hpp:104-330) on the GPU surface points.
+template <class T, int D> class CompareChamfer {
+  SmartPointer<Domain<T, D>> target_, sample_;
+  double r_[5] = {0, 0, 0, 0, 0};
+  int64_t n_[2] = {0, 0};
+
+public:
+  CompareChamfer() = default;
+  CompareChamfer(SmartPointer<Domain<T, D>> t, SmartPointer<Domain<T, D>> s) : target_(t), sample_(s) {}
+  void setLevelSetTarget(SmartPointer<Domain<T, D>> t) { target_ = t; }
+  void setLevelSetSample(SmartPointer<Domain<T, D>> s) { sample_ = s; }
+  void apply() { detail::ok(lsg_compare_chamfer(target_->handle(), sample_->handle(), r_, n_), "CompareChamfer"); }
+  T getForwardDistance() const { return r_[0]; }
+  T getBackwardDistance() const { return r_[1]; }
+  T getChamferDistance() const { return r_[2]; }
+  T getRMSChamferDistance() const { return r_[3]; }
+  T getMaxDistance() const { return r_[4]; }
+  unsigned getNumTargetPoints() const { return (unsigned)n_[0]; }
+  unsigned getNumSamplePoints() const { return (unsigned)n_[1]; }
+};
+
 template <class T, int D> class MakeGeometry {
   SmartPointer<Domain<T, D>> ls_;
   int kind_ = -1;

@@ -207,3 +207,21 @@ def test_check_detects_invalid_band():
     val[undef[len(undef) // 2]] *= -1.0  # flip one run's sign
     d.set(idx, val, 2)
     assert d.check()[0] > 0
+
+
+# ------------------------------------------------------------ judges ----
+
+def test_judge_plane_closed_forms():
+    """Surface points of an exact plane lie on it; the Chamfer distance of a
+    surface to itself is 0 and to the same plane moved by delta is delta."""
+    gmin, gmax, bc = [-6, -6, -8], [6, 6, 8], [0, 0, 1]
+    def plane(z0):
+        d = po.Domain(3, gmin, gmax, bc, 0.5)
+        i, v = cases.plane_points(3, gmin, gmax, bc, 0.5, z0)
+        d.from_points(i, v, 2)
+        return judges.surface_points(*d.get(), 0.5)
+    a, b = plane(0.3), plane(0.3 + 0.2)
+    np.testing.assert_allclose(a[:, 2], 0.3, atol=1e-12)
+    assert len(a) == 13 * 13
+    assert judges.chamfer(a, a) == 0.0
+    assert judges.chamfer(a, b) == pytest.approx(0.2, abs=1e-12)

@@ -15,7 +15,7 @@ __all__ = [
     "BoundaryConditionEnum", "SpatialSchemeEnum", "TemporalSchemeEnum", "BooleanOperationEnum",
     "Domain", "Sphere", "Plane", "Box", "MakeGeometry", "Expand", "Reduce", "Prune", "BooleanOperation",
     "VelocityField", "HostVelocityField", "Advect", "BatchAdvect", "LsgError", "weno", "PointData",
-    "MarkVoidPoints",
+    "MarkVoidPoints", "CompareChamfer", "surface_points",
 ]
 
 POS_VALUE = float(np.finfo(np.float64).max)
@@ -382,6 +382,61 @@ class MarkVoidPoints:
 
     def apply(self):
         check(lib().lsg_mark_void_points(self.domain._h))
+
+
+def surface_points(domain):
+    """Zero crossings of a level set on grid edges between face-adjacent
+    defined points (physical coordinates, n x D), extracted on the GPU."""
+    n = ctypes.c_int64(0)
+    check(lib().lsg_surface_points(domain._h, None, 0, ctypes.byref(n)))
+    out = np.zeros((n.value, domain.D), dtype=np.float64)
+    if n.value:
+        check(lib().lsg_surface_points(domain._h, out.ctypes.data, n.value, ctypes.byref(n)))
+    return out
+
+
+class CompareChamfer:
+    """CompareChamfer (lsCompareChamfer.hpp:104-330) on the GPU surface points
+    (see surface_points): forward / backward / Chamfer / RMS / max distance."""
+
+    def __init__(self, target=None, sample=None):
+        self.target, self.sample = target, sample
+        self._r = [float("nan")] * 5
+        self._n = [0, 0]
+
+    def setLevelSetTarget(self, d):
+        self.target = d
+
+    def setLevelSetSample(self, d):
+        self.sample = d
+
+    def apply(self):
+        out = np.zeros(5)
+        cnt = np.zeros(2, dtype=np.int64)
+        check(lib().lsg_compare_chamfer(self.target._h, self.sample._h, out.ctypes.data, cnt.ctypes.data))
+        self._r = [float(x) for x in out]
+        self._n = [int(x) for x in cnt]
+
+    def getForwardDistance(self):
+        return self._r[0]
+
+    def getBackwardDistance(self):
+        return self._r[1]
+
+    def getChamferDistance(self):
+        return self._r[2]
+
+    def getRMSChamferDistance(self):
+        return self._r[3]
+
+    def getMaxDistance(self):
+        return self._r[4]
+
+    def getNumTargetPoints(self):
+        return self._n[0]
+
+    def getNumSamplePoints(self):
+        return self._n[1]
 
 
 class BooleanOperation:

@@ -39,7 +39,7 @@ EXPORTED = [
     "lsg_batch_set_scan_blocks", "lsg_batch_autotune", "lsg_advect_set_velocity_host",
     "lsg_advect_set_save_advection_velocities", "lsg_domain_point_data_insert", "lsg_domain_point_data_count",
     "lsg_domain_point_data_label", "lsg_domain_point_data_size", "lsg_domain_point_data_get",
-    "lsg_domain_point_data_clear", "lsg_mark_void_points",
+    "lsg_domain_point_data_clear", "lsg_mark_void_points", "lsg_surface_points", "lsg_compare_chamfer",
 ]
 
 
@@ -107,6 +107,8 @@ def lib():
         "lsg_domain_point_data_get": ([P, c_int, c_int, ctypes.c_void_p, ctypes.c_int64], c_int),
         "lsg_domain_point_data_clear": ([P], c_int),
         "lsg_mark_void_points": ([P], c_int),
+        "lsg_surface_points": ([P, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)], c_int),
+        "lsg_compare_chamfer": ([P, P, ctypes.c_void_p, ctypes.c_void_p], c_int),
         "lsg_advect_prepare_ls": ([P], c_int),
         "lsg_advect_apply": ([P], c_int),
         "lsg_advect_get_advected_time": ([P, dp], c_int),

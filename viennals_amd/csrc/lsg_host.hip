@@ -2041,6 +2041,105 @@ int lsg_domain_point_data_clear(lsg_domain *d) {
   pdClear(d);
   return 0;
 }
+// ------------------------------------------------------ surface & Chamfer ---
+struct SurfBuf {
+  double *pts = nullptr;
+  uint32_t *start = nullptr, *total = nullptr;
+  uint32_t n = 0;  // surface points
+  int release() {
+    for (void *p : {(void *)pts, (void *)start, (void *)total}) if (p) (void)hipFree(p);
+    return 0;
+  }
+};
+static int surfaceOf(lsg_domain *d, SurfBuf &s) {
+  if (int e = ensureStores(d, d->cur.b.cap)) return e;
+  uint32_t ne = 0;
+  LSG_HIP(hipMemcpyAsync(&ne, d->cur.b.n, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  const size_t maxPts = (size_t)ne * d->D + 1;
+  LSG_HIP(hipMalloc(&s.pts, maxPts * d->D * sizeof(double)));
+  LSG_HIP(hipMalloc(&s.start, ((size_t)ne + 1) * sizeof(uint32_t)));
+  LSG_HIP(hipMalloc(&s.total, sizeof(uint32_t)));
+  launchSurfacePoints(g_ctx.s, d->D, d->cur.b, d->f, d->g, g_ws->w, s.pts, s.start, s.total);
+  LSG_HIP(hipMemcpyAsync(&s.n, s.total, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  return 0;
+}
+int lsg_surface_points(lsg_domain *d, double *out, int64_t cap, int64_t *n) {
+  if (int e = ctxInit()) return e;
+  if (!d->framed) return fail(-7, "surface points: level set has no data");
+  SurfBuf s;
+  if (int e = surfaceOf(d, s)) return s.release(), e;
+  *n = s.n;
+  int rc = 0;
+  if (out) {
+    if ((int64_t)s.n > cap) rc = fail(-1, "surface points: output buffer too small");
+    else if (s.n) rc = hipMemcpy(out, s.pts, (size_t)s.n * d->D * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess ? 0 : fail(-2, "copy failed");
+  }
+  s.release();
+  return rc;
+}
+// Nearest distances of the surface points of `from` to the surface of `to`.
+static int nearestAll(lsg_domain *from, const SurfBuf &sf, lsg_domain *to, const SurfBuf &st, std::vector<double> &dist) {
+  dist.assign(sf.n, 0.0);
+  if (sf.n == 0) return 0;
+  int maxShell = 2;
+  for (int k = 0; k < to->D; ++k) {
+    const int64_t lo = std::min<int64_t>(from->f.o[k], to->f.o[k]);
+    const int64_t hi = std::max<int64_t>((int64_t)from->f.o[k] + from->f.ext[k], (int64_t)to->f.o[k] + to->f.ext[k]);
+    maxShell = std::max<int>(maxShell, (int)(hi - lo) + 2);
+  }
+  double *dd = nullptr;
+  LSG_HIP(hipMalloc(&dd, (size_t)sf.n * sizeof(double)));
+  launchNearest(g_ctx.s, to->D, sf.pts, sf.total, sf.n, to->cur.b, to->f, to->g, st.start, st.pts, maxShell, dd);
+  hipError_t e = hipMemcpyAsync(dist.data(), dd, (size_t)sf.n * sizeof(double), hipMemcpyDeviceToHost, g_ctx.s);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_ctx.s);
+  (void)hipFree(dd);
+  if (e != hipSuccess) return fail(-2, hipGetErrorString(e));
+  return 0;
+}
+// CompareChamfer::apply (lsCompareChamfer.hpp:104-245); out = forward,
+// backward, Chamfer, RMS and max distance; counts = target, sample points.
+// Sums run on the host in point order, as the reference's loops do.
+int lsg_compare_chamfer(lsg_domain *target, lsg_domain *sample, double *out, int64_t *counts) {
+  if (int e = ctxInit()) return e;
+  if (target->D != sample->D) return fail(-1, "CompareChamfer: dimensions differ");
+  if (!target->framed || !sample->framed) return fail(-7, "CompareChamfer: level set has no data");
+  if (target->g.dx != sample->g.dx) return fail(-1, "CompareChamfer: grid deltas differ");
+  SurfBuf st, ss;
+  int rc = surfaceOf(target, st);
+  if (!rc) rc = surfaceOf(sample, ss);
+  std::vector<double> fwd, bwd;
+  if (!rc && st.n && ss.n) rc = nearestAll(target, st, sample, ss, fwd);
+  if (!rc && st.n && ss.n) rc = nearestAll(sample, ss, target, st, bwd);
+  counts[0] = st.n;
+  counts[1] = ss.n;
+  st.release();
+  ss.release();
+  if (rc) return rc;
+  if (fwd.empty() || bwd.empty()) {
+    for (int k = 0; k < 5; ++k) out[k] = std::numeric_limits<double>::infinity();
+    return 0;
+  }
+  double sf = 0, sf2 = 0, mf = 0, sb = 0, sb2 = 0, mb = 0;
+  for (double x : fwd) {
+    sf += x;
+    sf2 += x * x;
+    mf = std::max(mf, x);
+  }
+  for (double x : bwd) {
+    sb += x;
+    sb2 += x * x;
+    mb = std::max(mb, x);
+  }
+  out[0] = sf / (double)fwd.size();
+  out[1] = sb / (double)bwd.size();
+  out[2] = (sf + sb) / (double)(fwd.size() + bwd.size());
+  out[3] = std::sqrt((sf2 + sb2) / (double)(fwd.size() + bwd.size()));
+  out[4] = std::max(mf, mb);
+  return 0;
+}
+
 int lsg_mark_void_points(lsg_domain *d) {
   if (int e = ctxInit()) return e;
   if (!d->framed) return fail(-7, "MarkVoidPoints: level set has no data");

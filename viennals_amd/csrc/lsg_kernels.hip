@@ -1778,6 +1778,132 @@ void launchVoidMarks(hipStream_t s, int D, const BandBufs &b, const Frame &f, co
   else kVoidMarks<3><<<gridFor(b.cap), TPB, 0, s>>>(v, dlist, ndef, par, topQ, marks);
 }
 
+// ------------------------------------------------------ surface & Chamfer ---
+// Surface points of a band: the zero crossing on every grid edge between a
+// defined point a and its defined +e_d neighbour b of the other sign
+// ((va > 0) != (vb > 0)), at a + t e_d with t = va / (va - vb) -- the node
+// positions of a marching-cubes surface (lsToSurfaceMesh.hpp) before its node
+// merging.  Items j = entry * D + d in entry order, so the crossings of an
+// entry are contiguous: start[i] is the first of entry i (start[n] = total),
+// and the band's own bitmap serves as the spatial index of the points.
+
+template <int D> __device__ inline bool surfCrossing(const BandView &b, uint32_t i, int d, double *pt) {
+  const double va = b.val[i];
+  if (isUndef(va)) return false;
+  int32_t c[3];
+  candCoords(b, i, b.pos[i], c);
+  if (c[d] >= b.g.hi[d]) return false;
+  int32_t q[3] = {c[0], c[1], c[2]};
+  q[d] += 1;
+  const BandView::Loc l = b.locate(q[0], q[1], q[2]);
+  if (!l.inside || !((b.bits[l.w] >> l.bt) & 1ull)) return false;  // no entry at the neighbour
+  const double vb = b.val[b.entryOf(l, b.bits[l.w], b.pre[l.w])];
+  if (isUndef(vb) || (va > 0) == (vb > 0)) return false;
+  if (pt) {
+    const double t = va / (va - vb);
+    for (int k = 0; k < D; ++k) pt[k] = ((double)c[k] + (k == d ? t : 0.0)) * b.g.dx;
+  }
+  return true;
+}
+
+template <int D> __global__ __launch_bounds__(256) void kSurfCount(BandView b, const uint32_t *nptr, uint32_t *partials) {
+  const uint32_t n = *nptr * D;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
+  uint32_t acc = 0;
+  for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) acc += surfCrossing<D>(b, j / D, (int)(j % D), nullptr) ? 1u : 0u;
+  const uint32_t t = blockSum(acc);
+  if (threadIdx.x == 0) partials[logicalBlock()] = t;
+}
+template <int D>
+__global__ __launch_bounds__(256) void kSurfList(BandView b, const uint32_t *nptr, const uint32_t *partials,
+                                                 double *pts, uint32_t *start, uint32_t *total) {
+  const uint32_t ne = *nptr, n = ne * D;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
+  uint32_t off = blockOffset(partials);
+  for (uint32_t t = lo; t < hi; t += TPB) {
+    const uint32_t j = t + threadIdx.x;
+    double pt[3];
+    const bool hit = (j < hi) && surfCrossing<D>(b, j / D, (int)(j % D), pt);
+    uint32_t tot;
+    uint64_t wm;
+    const uint32_t q = off + blockBallotScan(hit, tot, wm);
+    if (j < hi && j % D == 0) start[j / D] = q;
+    if (hit)
+      for (int k = 0; k < D; ++k) pts[(size_t)q * D + k] = pt[k];
+    off += tot;
+  }
+  if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) {
+    *total = off;
+    start[ne] = off;
+  }
+}
+void launchSurfacePoints(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, Workspace &ws,
+                         double *pts, uint32_t *start, uint32_t *total) {
+  BandView v = viewOf(b, f, g);
+  if (D == 2) {
+    kSurfCount<2><<<g_scanBlocks, TPB, 0, s>>>(v, b.n, ws.partials);
+    kSurfList<2><<<g_scanBlocks, TPB, 0, s>>>(v, b.n, ws.partials, pts, start, total);
+  } else {
+    kSurfCount<3><<<g_scanBlocks, TPB, 0, s>>>(v, b.n, ws.partials);
+    kSurfList<3><<<g_scanBlocks, TPB, 0, s>>>(v, b.n, ws.partials, pts, start, total);
+  }
+}
+
+// Distance from every point of P to the nearest surface point of band Q:
+// grid points of Q are visited in Chebyshev shells around the query's cell;
+// after shell r every unvisited grid point's crossings lie at least r dx away.
+template <int D>
+__global__ void kNearest(const double *P, const uint32_t *nP, BandView q, const uint32_t *qStart, const double *qPts,
+                         int maxShell, double *dist) {
+  const uint32_t n = *nP;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    double p[3] = {0, 0, 0};
+    int32_t c[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+      p[d] = P[(size_t)k * D + d];
+      c[d] = (int32_t)floor(p[d] / q.g.dx);
+    }
+    double best2 = __builtin_inf();
+    for (int r = 0; r <= maxShell; ++r) {
+      const int zr = (D == 3) ? r : 0;
+      for (int oz = -zr; oz <= zr; ++oz)
+        for (int oy = -r; oy <= r; ++oy)
+          for (int ox = -r; ox <= r; ++ox) {
+            const int m = max(abs(ox), max(abs(oy), abs(oz)));
+            if (m != r) continue;  // shell r only
+            const int32_t g[3] = {c[0] + ox, c[1] + oy, c[2] + oz};
+            bool in = true;
+            for (int d = 0; d < D; ++d) in = in && g[d] >= q.g.lo[d] && g[d] <= q.g.hi[d];
+            if (!in) continue;
+            const BandView::Loc l = q.locate(g[0], g[1], g[2]);
+            if (!l.inside || !((q.bits[l.w] >> l.bt) & 1ull)) continue;
+            const uint32_t e = q.entryOf(l, q.bits[l.w], q.pre[l.w]);
+            for (uint32_t s = qStart[e]; s < qStart[e + 1]; ++s) {
+              double d2 = 0;
+              for (int d = 0; d < D; ++d) {
+                const double t = p[d] - qPts[(size_t)s * D + d];
+                d2 += t * t;
+              }
+              if (d2 < best2) best2 = d2;
+            }
+          }
+      // (one shell of margin: p / dx may round up to the next cell)
+      const double rr = (double)(r - 1) * q.g.dx;
+      if (r >= 1 && best2 <= rr * rr) break;
+    }
+    dist[k] = sqrt(best2);
+  }
+}
+void launchNearest(hipStream_t s, int D, const double *P, const uint32_t *nP, size_t cap, const BandBufs &q,
+                   const Frame &f, const GridInfo &g, const uint32_t *qStart, const double *qPts, int maxShell,
+                   double *dist) {
+  BandView v = viewOf(q, f, g);
+  if (D == 2) kNearest<2><<<gridFor(cap), TPB, 0, s>>>(P, nP, v, qStart, qPts, maxShell, dist);
+  else kNearest<3><<<gridFor(cap), TPB, 0, s>>>(P, nP, v, qStart, qPts, maxShell, dist);
+}
+
 // Direct evaluation of FiniteDifferences::weno3/weno5 on n stencils.
 __global__ void kWeno(int order, uint32_t n, const double *x, double dx, int plus, double *out) {
   const int w = (order == 3) ? 5 : 7;

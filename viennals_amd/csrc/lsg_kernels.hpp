@@ -151,6 +151,14 @@ void launchPdGather(hipStream_t s, int comps, const double *A, const uint32_t *l
                     const uint32_t *lenB, const uint32_t *src, const uint8_t *side, const uint32_t *nPush, size_t cap,
                     double *out);
 void launchCopyU32(hipStream_t s, const uint32_t *from, uint32_t *to);
+// Surface points (zero crossings on grid edges) of a band: D doubles each,
+// start[entry] = first point of the entry, *total = count.
+void launchSurfacePoints(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, Workspace &ws,
+                         double *pts, uint32_t *start, uint32_t *total);
+// Nearest surface point of band q for each of the *nP points P.
+void launchNearest(hipStream_t s, int D, const double *P, const uint32_t *nP, size_t cap, const BandBufs &q,
+                   const Frame &f, const GridInfo &g, const uint32_t *qStart, const double *qPts, int maxShell,
+                   double *dist);
 // MarkVoidPoints: the marker (1 void, 0 not) of every defined point of b
 // (dlist / ndef: the defined entries in order); sign / par hold one byte /
 // one word per frame point.
