@@ -718,6 +718,16 @@ template <int D> int check(const Band<D> &ls, std::string *msg) {
 template <int D>
 void fromPoints(Band<D> &ls, std::vector<std::pair<Idx<D>, double>> pts, int width) {
   const Grid<D> &g = ls.g;
+  // points outside the grid are dropped (a sphere crossing a reflective
+  // boundary, examples/VoidEtching/VoidEtching.cpp:62-66; HRLE's
+  // insertPoints, which decides this in the reference, is absent)
+  pts.erase(std::remove_if(pts.begin(), pts.end(),
+                           [&](const auto &q) {
+                             for (int d = 0; d < D; ++d)
+                               if (q.first[d] < g.lo[d] || q.first[d] > g.hi[d]) return true;
+                             return false;
+                           }),
+            pts.end());
   std::sort(pts.begin(), pts.end(), [](const auto &a, const auto &b) { return idxLess<D>(a.first, b.first); });
   std::vector<Idx<D>> c;
   std::vector<double> v;

@@ -353,6 +353,18 @@ static bool idxLessD(int D, const int32_t *a, const int32_t *b) {
 static void runsFromPoints(const lsg_domain *d, std::vector<int32_t> pidx, std::vector<double> pval,
                            std::vector<int32_t> &idx, std::vector<double> &val) {
   const int D = d->D;
+  {  // points outside the grid are dropped (as in the CPU restatement)
+    size_t w = 0;
+    for (size_t i = 0; i < pval.size(); ++i) {
+      bool in = true;
+      for (int k = 0; k < D; ++k) in = in && pidx[i * D + k] >= d->g.lo[k] && pidx[i * D + k] <= d->g.hi[k];
+      if (!in) continue;
+      for (int k = 0; k < D; ++k) pidx[w * D + k] = pidx[i * D + k];
+      pval[w++] = pval[i];
+    }
+    pidx.resize(w * D);
+    pval.resize(w);
+  }
   const size_t n = pval.size();
   std::vector<size_t> order(n);
   for (size_t i = 0; i < n; ++i) order[i] = i;
@@ -1942,9 +1954,6 @@ int lsg_domain_deep_copy(lsg_domain *dst, const lsg_domain *src) {
 int lsg_domain_insert_points(lsg_domain *d, int64_t n, const int32_t *idx, const double *val, int width) {
   std::vector<int32_t> pi(idx, idx + n * d->D);
   std::vector<double> pv(val, val + n);
-  for (int64_t i = 0; i < n; ++i)
-    for (int k = 0; k < d->D; ++k)
-      if (pi[i * d->D + k] < d->g.lo[k] || pi[i * d->D + k] > d->g.hi[k]) return fail(-1, "point outside the grid");
   std::vector<int32_t> ri;
   std::vector<double> rv;
   runsFromPoints(d, pi, pv, ri, rv);
