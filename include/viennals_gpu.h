@@ -137,7 +137,9 @@ int lsg_reduce(lsg_domain *d, int width);
 /* Prune<T,D>(ls) with setRemoveStrayZeros (lsPrune.hpp:74-194). */
 int lsg_prune(lsg_domain *d, int remove_stray_zeros);
 /* BooleanOperation<T,D>(a, b, op).apply() (lsBooleanOperation.hpp:59-213,
- * 329-366); the result is pruned as with the default pruneResult. */
+ * 329-366) for op INTERSECT (0), UNION (1), RELATIVE_COMPLEMENT (2); the
+ * result is pruned as with the default pruneResult.  op INVERT (3) negates a
+ * (lsBooleanOperation.hpp:214-247); b may then be null. */
 int lsg_boolean(lsg_domain *a, const lsg_domain *b, int op);
 /* Advect::rebuildLS (lsAdvect.hpp:238-429) on a single level set. */
 int lsg_rebuild(lsg_domain *d);

@@ -2038,6 +2038,11 @@ void lso_rebuild(lso_domain *d) {
 }
 // op: 0 INTERSECT, 1 UNION, 2 RELATIVE_COMPLEMENT (lsBooleanOperation.hpp:27-33)
 int lso_boolean(lso_domain *a, const lso_domain *b, int op) {
+  if (op == 3) {  // INVERT (lsBooleanOperation.hpp:214-247): every entry negated
+    if (a->D == 2) for (double &v : a->b2.val) v = -v;
+    else for (double &v : a->b3.val) v = -v;
+    return 0;
+  }
   auto run = [&](auto &A, const auto &B) {
     switch (op) {
     case 0: booleanOp(A, B, [](double x, double y) { return maxComp(x, y); }, [](double x, double y) { return x > y; }); break;

@@ -2,6 +2,7 @@
 against the oracle (pinned in tests/test_oracle_sllf.py): the 3^D box
 alphas, the time step from the last point's alphas, the rebuild width 3 and
 the lower level sets left as they are."""
+import numpy as np
 import pytest
 
 import cases
@@ -81,3 +82,72 @@ def test_sllf_rejects_host_velocity():
     a.setSpatialScheme(SLLF)
     with pytest.raises(vls.LsgError, match="host-evaluated"):
         a.apply()
+
+
+def _oracle_prepare(levelSets, isDepo):
+    """PrepareStencilLocalLaxFriedrichs (lsStencilLocalLaxFriedrichsScalar.hpp:
+    423-479) composed from the oracle's band operations."""
+    n = len(levelSets)
+    isDepo = (list(isDepo) + [False] * n)[:n]
+    g = levelSets[0]
+    top = po.Domain(g.D, g.gmin, g.gmax, g.bc, g.delta)
+    top.set(g.get()[0][:1], np.array([np.finfo(float).max]), 1)  # the grid's minimum point only
+    layer, above = n - 1, False
+    for k in range(n):
+        mask = levelSets[n - 1 - k]
+        if isDepo[int(isDepo[n - 1 - k])]:
+            if not above:
+                layer, above = n - 1 - k, True
+        else:
+            if above:
+                la = levelSets[layer].copy()
+                la.boolean(mask, 2)
+                top.boolean(la, 1)
+                top.prune(False)
+            above = False
+    if above:
+        top.boolean(levelSets[layer], 1)
+    top.boolean(top, 3)
+    levelSets[-1].set(*top.get(), top.width)
+
+
+def test_sllf_layer_wrapping_epitaxy_like():
+    """examples/Epitaxy/Epitaxy.py's flow with built-in velocities: mask plane,
+    fin box united with the mask, PrepareStencilLocalLaxFriedrichs, SLLF with
+    two materials, FinalizeStencilLocalLaxFriedrichs -- bit-exact against
+    the oracle at every stage."""
+    import viennals_amd as vls
+    say("SLLF layer wrapping (Epitaxy-like, 3D)")
+    n = 20
+    gmin, gmax, bc = [-n, -n, -n], [n, n, n], [0, 0, 1]
+    dx = 0.05
+    mask = po.Domain(3, gmin, gmax, bc, dx)
+    i, v = cases.plane_points(3, gmin, gmax, bc, dx, 0.0)
+    mask.from_points(i, v, 2)
+    sub = po.Domain(3, gmin, gmax, bc, dx)
+    i, v = cases.box_points(3, [-0.25, -0.25, 0.0], [0.25, 0.25, 0.2], dx, gmin, gmax)
+    sub.from_points(i, v, 2)
+    sub.boolean(mask, 1)
+    gm, gs = to_gpu(mask), to_gpu(sub)
+    _oracle_prepare([mask, sub], [False, True])
+    vls.PrepareStencilLocalLaxFriedrichs([gm, gs], [False, True])
+    assert_same(sub, gs, "prepared top")
+    mats = [dict(b=0.0), dict(kind=1, dir=2, k=1, b=-0.2, a_pos=-0.3, a_neg=-0.3)]
+    p = po.make_params(spatial=SLLF, temporal=0, cfl=0.4, time=0.1, vel=mats[-1], mat_vels=mats)
+    st, r = po.advect_multi([mask, sub], p)
+    assert st == 0
+    a = vls.Advect()
+    a.insertNextLevelSet(gm)
+    a.insertNextLevelSet(gs)
+    a.setVelocityField(vls.MaterialVelocityField([vls.VelocityField(**m) for m in mats]))
+    a.setSpatialScheme(SLLF)
+    a.setTimeStepRatio(0.4)
+    a.setAdvectionTime(0.1)
+    a.apply()
+    assert a.getAdvectedTime() == r["advected_time"]
+    assert_same(sub, gs, "advected top")
+    assert_same(mask, gm, "mask")
+    sub.boolean(sub, 3)
+    sub.boolean(mask, 1)
+    vls.FinalizeStencilLocalLaxFriedrichs([gm, gs])
+    assert_same(sub, gs, "finalized")

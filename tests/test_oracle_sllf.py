@@ -134,3 +134,15 @@ def test_lower_level_sets_are_not_adjusted():
         after = lower.get()
         same = cases.bit_equal(before[0], before[1], after[0], after[1])
         assert same == kept
+
+
+def test_invert_negates_every_entry():
+    # BooleanOperation INVERT (lsBooleanOperation.hpp:214-247): defined values
+    # negated, the undefined signs swapped; twice is the identity
+    d = cases.sphere2d()
+    i0, v0 = d.get()
+    d.boolean(d, 3)
+    i1, v1 = d.get()
+    assert np.array_equal(i0, i1) and np.array_equal(v1, -v0)
+    d.boolean(d, 3)
+    assert cases.bit_equal(i0, v0, *d.get())

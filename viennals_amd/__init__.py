@@ -15,7 +15,8 @@ __all__ = [
     "BoundaryConditionEnum", "SpatialSchemeEnum", "TemporalSchemeEnum", "BooleanOperationEnum",
     "Domain", "Sphere", "Plane", "Box", "MakeGeometry", "Expand", "Reduce", "Prune", "BooleanOperation",
     "VelocityField", "HostVelocityField", "Advect", "BatchAdvect", "LsgError", "weno", "PointData",
-    "MarkVoidPoints", "CompareChamfer", "surface_points",
+    "MarkVoidPoints", "CompareChamfer", "surface_points", "PrepareStencilLocalLaxFriedrichs",
+    "FinalizeStencilLocalLaxFriedrichs",
 ]
 
 POS_VALUE = float(np.finfo(np.float64).max)
@@ -56,10 +57,11 @@ class TemporalSchemeEnum:
 
 
 class BooleanOperationEnum:
-    """lsBooleanOperation.hpp:27-33 (INVERT and CUSTOM are internal here)."""
+    """lsBooleanOperation.hpp:27-33 (CUSTOM is internal here)."""
     INTERSECT = 0
     UNION = 1
     RELATIVE_COMPLEMENT = 2
+    INVERT = 3
 
 
 def _i32(a):
@@ -460,9 +462,68 @@ class BooleanOperation:
         self.op = op
 
     def apply(self):
+        if int(self.op) == BooleanOperationEnum.INVERT:
+            check(lib().lsg_boolean(self.a._h, None, 3))
+            return
         if self.a is None or self.b is None:
-            raise LsgError("BooleanOperation needs two level sets (single-level-set operations are not supported)")
+            raise LsgError("BooleanOperation needs two level sets")
         check(lib().lsg_boolean(self.a._h, self.b._h, int(self.op)))
+
+
+def _empty_like(d):
+    """Domain::New(grid): the same grid, no defined points (width 1)."""
+    e = type(d).from_index_bounds(d.D, d.gmin, d.gmax, d.bc, d.gridDelta)
+    idx, _ = d.get_runs()  # the first entry is the grid's minimum point
+    e.set_runs(idx[:1].astype(np.int32), np.array([POS_VALUE]), 1)
+    return e
+
+
+def _copy_of(d):
+    c = type(d).from_index_bounds(d.D, d.gmin, d.gmax, d.bc, d.gridDelta)
+    c.deepCopy(d)
+    return c
+
+
+def PrepareStencilLocalLaxFriedrichs(levelSets, isDepo):
+    """The layer wrapping for the SLLF scheme
+    (lsStencilLocalLaxFriedrichsScalar.hpp:423-479): the top level set
+    becomes the inverted union of every deposition layer minus the masking
+    layer below it.  The condition reads isDepo[isDepo[k]] for the k-th
+    level set from the top, as the reference's loop does (:446-447)."""
+    n = len(levelSets)
+    isDepo = [bool(x) for x in isDepo]
+    if len(isDepo) < n:
+        import warnings
+        warnings.warn("PrepareStencilLocalLaxFriedrichs: isDepo does not have enough elements. "
+                      "Assuming all higher layers are not depo layers.")
+    isDepo = (isDepo + [False] * n)[:n]
+    layer = n - 1
+    finalTop = _empty_like(levelSets[0])
+    layerAboveIsDepo = False
+    for k in range(n):
+        mask = levelSets[n - 1 - k]
+        if isDepo[int(isDepo[n - 1 - k])]:
+            if not layerAboveIsDepo:
+                layer = n - 1 - k
+                layerAboveIsDepo = True
+        else:
+            if layerAboveIsDepo:
+                layerAbove = _copy_of(levelSets[layer])
+                BooleanOperation(layerAbove, mask, BooleanOperationEnum.RELATIVE_COMPLEMENT).apply()
+                BooleanOperation(finalTop, layerAbove, BooleanOperationEnum.UNION).apply()
+                Prune(finalTop).apply()
+            layerAboveIsDepo = False
+    if layerAboveIsDepo:
+        BooleanOperation(finalTop, levelSets[layer], BooleanOperationEnum.UNION).apply()
+    BooleanOperation(finalTop, None, BooleanOperationEnum.INVERT).apply()
+    levelSets[-1].deepCopy(finalTop)
+
+
+def FinalizeStencilLocalLaxFriedrichs(levelSets):
+    """Restore the usual layer wrapping after SLLF advection
+    (lsStencilLocalLaxFriedrichsScalar.hpp:483-492)."""
+    BooleanOperation(levelSets[-1], None, BooleanOperationEnum.INVERT).apply()
+    BooleanOperation(levelSets[-1], levelSets[-2], BooleanOperationEnum.UNION).apply()
 
 
 class VelocityField:

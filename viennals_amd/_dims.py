@@ -4,8 +4,9 @@ viennals.d3 (python/viennals/d2.pyi, d3.pyi) for the classes of the
 advection path: Domain, Advect, MakeGeometry, Sphere, Plane, Box,
 BooleanOperation, Expand, Reduce, Prune, VelocityField and the enums."""
 from . import (Advect, BooleanOperation, BooleanOperationEnum, BoundaryConditionEnum, Box, CompareChamfer, Expand,
-               HostVelocityField, LsgError, MakeGeometry, MarkVoidPoints, MaterialVelocityField, Plane, PointData,
-               Prune, Reduce, Sphere, SpatialSchemeEnum, TemporalSchemeEnum, VelocityField)
+               FinalizeStencilLocalLaxFriedrichs, HostVelocityField, LsgError, MakeGeometry, MarkVoidPoints,
+               MaterialVelocityField, Plane, PointData, PrepareStencilLocalLaxFriedrichs, Prune, Reduce, Sphere,
+               SpatialSchemeEnum, TemporalSchemeEnum, VelocityField)
 from . import Domain as _Domain
 
 

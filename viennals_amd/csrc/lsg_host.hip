@@ -2247,6 +2247,13 @@ int lsg_rebuild(lsg_domain *d) {
   return afterOp(d);
 }
 int lsg_boolean(lsg_domain *a, const lsg_domain *b, int op) {
+  if (op == 3) {  // INVERT: b is not used (lsBooleanOperation.hpp:214-247)
+    if (int e = ctxInit()) return e;
+    if (!a->framed) return fail(-7, "BooleanOperation: level set has no data");
+    launchNegate(g_ctx.s, a->cur.b);
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    return 0;
+  }
   if (op < 0 || op > 2) return fail(-1, "unsupported boolean operation");
   if (int e = commonFrame(a, const_cast<lsg_domain *>(b))) return e;
   if (int e = ensureStores(a, std::max(a->cur.b.cap, b->cur.b.cap))) return e;

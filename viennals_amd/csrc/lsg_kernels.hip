@@ -1471,6 +1471,15 @@ void launchBandStats(hipStream_t s, const BandBufs &b, const Frame &f, uint32_t 
   kBandStats<<<gridFor(b.cap, 512), TPB, 0, s>>>(b.val, b.pos, b.n, f, outCount, bbox);
 }
 
+// BooleanOperation INVERT (lsBooleanOperation.hpp:214-247): every defined
+// value negated and the two undefined run signs swapped, i.e. every entry of
+// the run list negated; the run structure stays.
+__global__ void kNegate(double *val, const uint32_t *nptr) {
+  const uint32_t n = *nptr;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) val[i] = -val[i];
+}
+void launchNegate(hipStream_t s, const BandBufs &b) { kNegate<<<gridFor(b.cap), TPB, 0, s>>>(b.val, b.n); }
+
 __global__ void kFill32(uint32_t *p, uint32_t v, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }

@@ -139,6 +139,7 @@ void launchCopyBand(hipStream_t s, const BandBufs &src, BandBufs &dst, const Fra
 // entries (frame-relative, to bbox[6]: min x,y,z then max x,y,z).
 void launchBandStats(hipStream_t s, const BandBufs &b, const Frame &f, uint32_t *outCount, int32_t *bbox);
 void launchFill32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
+void launchNegate(hipStream_t s, const BandBufs &b);
 
 // Point data: point ids of a band, the sources of an operation's result,
 // one gather per array, the saved advection velocities.

@@ -2,12 +2,13 @@
 (python/viennals/d3.pyi) for the advection path: the same class names and
 call forms, bound to 3 dimensions.  `from viennals_amd import d3 as vls`."""
 from ._dims import (Advect, BooleanOperation, BooleanOperationEnum, BoundaryConditionEnum, Box, CompareChamfer,
-                    Expand, HostVelocityField, MakeGeometry, MarkVoidPoints, MaterialVelocityField, Plane, PointData,
-                    Prune, Reduce, Sphere, SpatialSchemeEnum, TemporalSchemeEnum, VelocityField, domain_class)
+                    Expand, FinalizeStencilLocalLaxFriedrichs, HostVelocityField, MakeGeometry, MarkVoidPoints,
+                    MaterialVelocityField, Plane, PointData, PrepareStencilLocalLaxFriedrichs, Prune, Reduce, Sphere,
+                    SpatialSchemeEnum, TemporalSchemeEnum, VelocityField, domain_class)
 
 D = 3
 Domain = domain_class(3)
 
 __all__ = ["Advect", "BooleanOperation", "BooleanOperationEnum", "BoundaryConditionEnum", "Box", "CompareChamfer",
-           "Domain", "Expand", "HostVelocityField", "MakeGeometry", "MarkVoidPoints", "MaterialVelocityField", "Plane",
-           "PointData", "Prune", "Reduce", "Sphere", "SpatialSchemeEnum", "TemporalSchemeEnum", "VelocityField"]
+           "Domain", "Expand", "FinalizeStencilLocalLaxFriedrichs", "HostVelocityField", "MakeGeometry", "MarkVoidPoints", "MaterialVelocityField", "Plane",
+           "PointData", "PrepareStencilLocalLaxFriedrichs", "Prune", "Reduce", "Sphere", "SpatialSchemeEnum", "TemporalSchemeEnum", "VelocityField"]
