@@ -62,8 +62,10 @@ enum { LSG_INTERSECT = 0, LSG_UNION = 1, LSG_RELATIVE_COMPLEMENT = 2 };
  *   getScalarVelocity = b                                   (LSG_VEL_CONSTANT)
  *                     = b + (n[dir] > 0 ? aPos : aNeg) * |n[dir]|^k
  *                                                           (LSG_VEL_NORMAL_POWER)
- *   getVectorVelocity = (vec[0], vec[1], vec[2])            (both kinds) */
-enum { LSG_VEL_CONSTANT = 0, LSG_VEL_NORMAL_POWER = 1 };
+ *                     = b + aPos * (|n[0]| + |n[1]| + |n[2]|) (LSG_VEL_NORMAL_SUM,
+ *                       the velocity of examples/AirGapDeposition/AirGapDeposition.cpp:20-27)
+ *   getVectorVelocity = (vec[0], vec[1], vec[2])            (every kind) */
+enum { LSG_VEL_CONSTANT = 0, LSG_VEL_NORMAL_POWER = 1, LSG_VEL_NORMAL_SUM = 2 };
 
 const char *lsg_last_error(void);
 

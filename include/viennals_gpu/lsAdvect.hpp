@@ -399,6 +399,7 @@ public:
 
 // scalar = b                                          (kind LSG_VEL_CONSTANT)
 //        = b + (n[dir] > 0 ? aPos : aNeg) * |n[dir]|^k (kind LSG_VEL_NORMAL_POWER)
+//        = b + aPos * (|n[0]| + |n[1]| + |n[2]|)     (kind LSG_VEL_NORMAL_SUM)
 // vector = vec
 template <class T> class BuiltinVelocityField : public VelocityField<T> {
 public:
@@ -424,8 +425,17 @@ public:
     v->vec = vector;
     return v;
   }
+  static SmartPointer<BuiltinVelocityField> normalSum(T a, T b = 0, Vec3D<T> vector = {0, 0, 0}) {
+    auto v = SmartPointer<BuiltinVelocityField>::New();
+    v->kind = LSG_VEL_NORMAL_SUM;
+    v->b = b;
+    v->aPos = a;
+    v->vec = vector;
+    return v;
+  }
   T getScalarVelocity(const Vec3D<T> &, int, const Vec3D<T> &n, unsigned long) override {
     if (kind == LSG_VEL_CONSTANT) return b;
+    if (kind == LSG_VEL_NORMAL_SUM) return b + aPos * (std::fabs(n[0]) + std::fabs(n[1]) + std::fabs(n[2]));
     T c = n[dir], m = 1;
     if (k >= 1) {
       m = std::fabs(c);

@@ -165,6 +165,7 @@ template <int D> struct Band {
 // Built-in velocity functors (the device-functor set of the HIP library).
 // scalar(n) = b                                           (kind 0)
 //           = b + (n[dir] > 0 ? aPos : aNeg) * |n[dir]|^k (kind 1)
+//           = b + aPos * (|n_0| + |n_1| + |n_2|)          (kind 2)
 // vector    = vec (constant)
 struct Velocity {
   int kind = 0;
@@ -177,6 +178,11 @@ struct Velocity {
   double alpha[3] = {0, 0, 0};
   double scalar(const double n[3]) const {
     if (kind == 0) return b;
+    if (kind == 2) {  // b + aPos * sum |n_i| (examples/AirGapDeposition/AirGapDeposition.cpp:20-27)
+      double s = 0.0;
+      for (int i = 0; i < 3; ++i) s += std::fabs(n[i]);
+      return b + aPos * s;
+    }
     double c = n[dir];
     double a = (c > 0) ? aPos : aNeg;
     double ac = std::fabs(c);

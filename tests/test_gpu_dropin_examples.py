@@ -57,3 +57,71 @@ def test_void_etching_example():
     marks = substrate.getPointData().getScalarData("VoidPointMarkers")
     np.testing.assert_array_equal(marks, o.data("VoidPointMarkers"))
     assert marks.sum() > 0 and passed > 0
+
+
+def _oracle_copy(d):
+    idx, val = d.get_runs()
+    o = po.Domain(d.D, d.gmin, d.gmax, d.bc, d.gridDelta)
+    o.set(idx, val, d.getLevelSetWidth())
+    return o
+
+
+@pytest.mark.parametrize("temporal,host", [(0, True), (1, True), (2, True), (2, False)])
+def test_air_gap_deposition_example(temporal, host):
+    """examples/AirGapDeposition/AirGapDeposition.py without its mesh output
+    and with 6 time units instead of 16.5 (the air gap closes after 4): a trench (plane minus box), a new
+    layer copied from it, a VelocityField subclass returning sum |n_i| for
+    every material (the host callback path with two level sets),
+    setIgnoreVoids(True); both level sets bit-exact against the oracle after
+    every apply, the oracle using the built-in normal-sum functor (which the
+    GPU runs too when host is False)."""
+    import viennals_amd.viennals as vls
+    say(f"AirGapDeposition.py through viennals_amd.viennals (2D, two level sets, host velocity {host}), "
+        f"temporal {temporal}")
+    vls.setDimension(2)
+
+    class velocityField(vls.VelocityField):
+        def getScalarVelocity(self, coord, material, normal, pointId):
+            velocity = 0.0
+            for i in range(len(normal)):
+                velocity += abs(normal[i])
+            return velocity
+
+        def getVectorVelocity(self, coord, material, normal, pointId):
+            return (0.0, 0.0, 0.0)
+
+    extent, gridDelta = 30.0, 0.5
+    bounds = (-extent, extent, -extent, extent)
+    bcs = (vls.BoundaryConditionEnum.REFLECTIVE_BOUNDARY, vls.BoundaryConditionEnum.INFINITE_BOUNDARY)
+    substrate = vls.Domain(bounds, bcs, gridDelta)
+    vls.MakeGeometry(substrate, vls.Plane((0.0, 0.0), (0.0, 1.0))).apply()
+    trench = vls.Domain(bounds, bcs, gridDelta)
+    vls.MakeGeometry(trench, vls.Box((-extent / 6.0, -25.0), (extent / 6.0, 1.0))).apply()
+    vls.BooleanOperation(substrate, trench, vls.BooleanOperationEnum.RELATIVE_COMPLEMENT).apply()
+    sub = vls.Domain(substrate)
+    layer = vls.Domain(sub)
+    osub, olayer = _oracle_copy(sub), _oracle_copy(layer)
+
+    kernel = vls.Advect()
+    kernel.insertNextLevelSet(sub)
+    kernel.insertNextLevelSet(layer)
+    kernel.setVelocityField(velocityField() if host else vls.VelocityField.normal_sum(1.0))
+    kernel.setIgnoreVoids(True)
+    kernel.setTemporalScheme(temporal)
+    vel = dict(kind=2, b=0.0, a_pos=1.0)
+    passed, total, interval = 0.0, 6.0, 1.0
+    while passed < total:
+        dt = min(interval, total - passed)
+        kernel.setAdvectionTime(dt)
+        kernel.apply()
+        p = po.make_params(spatial=0, temporal=temporal, time=dt, vel=vel, mat_vels=[vel, vel], ignore_voids=True)
+        st, r = po.advect_multi([osub, olayer], p)
+        assert st == 0
+        assert kernel.getAdvectedTime() == r["advected_time"]
+        passed += kernel.getAdvectedTime()
+        assert_same(olayer, layer, f"new layer at t={passed}")
+        assert_same(osub, sub, f"substrate at t={passed}")
+    assert passed == pytest.approx(total)
+    marks = layer.getPointData().getScalarData("VoidPointMarkers")
+    np.testing.assert_array_equal(marks, olayer.data("VoidPointMarkers"))
+    assert marks.sum() > 0  # the air gap

@@ -225,3 +225,33 @@ def test_judge_plane_closed_forms():
     assert len(a) == 13 * 13
     assert judges.chamfer(a, a) == 0.0
     assert judges.chamfer(a, b) == pytest.approx(0.2, abs=1e-12)
+
+
+# ------------------------------------------ the normal-sum velocity functor ----
+
+def test_normal_sum_velocity_closed_forms():
+    """Built-in functor kind 2, b + a * (|n_0| + |n_1| + |n_2|)
+    (examples/AirGapDeposition/AirGapDeposition.cpp:20-27): on an axis plane
+    |n| sums to 1, so the rates equal those of the constant b + a bit for bit;
+    on a diagonal 2D plane the sum is sqrt(2) and H = (b + a*sqrt(2)) / dx."""
+    dx = 0.5
+    gmin, gmax, bc = [-8, -8, -8], [8, 8, 8], [0, 0, 1]
+    d = po.Domain(3, gmin, gmax, bc, dx)
+    ip, vp = cases.plane_points(3, gmin, gmax, bc, dx, 0.3)
+    d.from_points(ip, vp, 2)
+    _, r2 = po.compute_rates(d, po.make_params(spatial=0, cfl=0.4, vel=dict(kind=2, b=0.25, a_pos=1.5)))
+    _, r0 = po.compute_rates(d, po.make_params(spatial=0, cfl=0.4, vel=dict(kind=0, b=1.75)))
+    assert np.array_equal(r2, r0)
+    # diagonal plane phi = (i + j + 0.3) / sqrt(2) in grid units, all points within 3
+    g = np.arange(-20, 21)
+    I, J = np.meshgrid(g, g, indexing="ij")
+    phi = (I + J + 0.3) / np.sqrt(2.0)
+    m = np.abs(phi) <= 3.0
+    e = po.Domain(2, [-20, -20], [20, 20], [cases.INF, cases.INF], dx)
+    e.from_points(np.stack([I[m], J[m]], 1).astype(np.int32), phi[m], 3)
+    a, b = 1.5, 0.25
+    _, r = po.compute_rates(e, po.make_params(spatial=0, cfl=0.4, vel=dict(kind=2, b=b, a_pos=a)))
+    idx, val = e.get()
+    inner = (np.abs(val) <= 0.5) & (np.abs(idx[:, 0]) <= 8) & (np.abs(idx[:, 1]) <= 8)
+    assert inner.sum() > 10
+    np.testing.assert_allclose(r[inner], (b + a * np.sqrt(2.0)) / dx, rtol=1e-12)

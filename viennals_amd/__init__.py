@@ -158,10 +158,17 @@ class Domain:
 
     Domain(bounds, boundaryConditions, gridDelta) with bounds
     [x0, x1, y0, y1(, z0, z1)] in physical units (lsDomain.hpp:68-78), or
-    Domain(gridDelta, dim=3) for an all-infinite domain (lsDomain.hpp:55-66).
+    Domain(gridDelta, dim=3) for an all-infinite domain (lsDomain.hpp:55-66),
+    or Domain(other) for a deep copy of another level set (lsDomain.hpp:108).
     """
 
     def __init__(self, bounds=None, boundaryConditions=None, gridDelta=1.0, dim=None):
+        if isinstance(bounds, Domain):
+            src = bounds
+            Domain.__init__(self, None, None, src.gridDelta, dim=src.D)
+            self.gmin, self.gmax, self.bc = list(src.gmin), list(src.gmax), list(src.bc)
+            self.deepCopy(src)
+            return
         if bounds is None:
             D = dim or 3
             gmin = [0] * D
@@ -530,8 +537,9 @@ class VelocityField:
     """Built-in GPU velocity functors replacing VelocityField<T>
     (lsVelocityField.hpp:11-41):
 
-    scalar = b                                            (constant)
-           = b + (n[dir] > 0 ? a_pos : a_neg) * |n[dir]|^k  (normal power)
+    scalar = b                                            (constant, kind 0)
+           = b + (n[dir] > 0 ? a_pos : a_neg) * |n[dir]|^k  (normal power, kind 1)
+           = b + a_pos * (|n_0| + |n_1| + |n_2|)           (normal sum, kind 2)
     vector = vec (constant)
     """
 
@@ -550,6 +558,10 @@ class VelocityField:
     def normal_power(cls, dir, k, b, a_pos, a_neg=0.0, vector=(0.0, 0.0, 0.0)):
         return cls(kind=1, dir=dir, k=k, b=b, a_pos=a_pos, a_neg=a_neg, vec=vector)
 
+    @classmethod
+    def normal_sum(cls, a, b=0.0, vector=(0.0, 0.0, 0.0)):
+        return cls(kind=2, b=b, a_pos=a, vec=vector)
+
     def as_dict(self):
         return dict(kind=self.kind, dir=self.dir, k=self.k, b=self.b, a_pos=self.a_pos, a_neg=self.a_neg,
                     vec=self.vec, alpha=self.alpha)
@@ -560,6 +572,11 @@ class VelocityField:
     def getScalarVelocity(self, coordinate, material, normal, pointId):
         if self.kind == 0:
             return self.b
+        if self.kind == 2:
+            s = 0.0
+            for i in range(3):
+                s += abs(normal[i])
+            return self.b + self.a_pos * s
         c = normal[self.dir]
         m = 1.0
         if self.k >= 1:

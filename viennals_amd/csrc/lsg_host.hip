@@ -824,7 +824,8 @@ struct lsg_advect {
   lsg_scalar_velocity_fn hostScalar = nullptr;
   lsg_vector_velocity_fn hostVector = nullptr;
   void *hostUser = nullptr;
-  double *velIn = nullptr, *velOut = nullptr;  // device: 7 / 4 doubles per active point
+  double *velIn = nullptr, *velOut = nullptr;  // device: 7 / 4 (x materials) doubles per active point
+  int velMat = 1;
   uint32_t *dlist = nullptr, *ndef = nullptr;
   size_t velCap = 0;
   std::vector<double> velInH, velOutH;
@@ -919,17 +920,21 @@ static void pdUpdate(lsg_advect *a, lsg_domain *d, const double *dt, int nmat) {
 // inputs of getScalarVelocity / getVectorVelocity are gathered on the device,
 // the callbacks run on the host in point order, the results go back as the
 // rate kernel's per-point velocities.
-static int hostVelocities(lsg_advect *a, lsg_domain *d) {
+// With several level sets the callbacks run for every material m of every
+// active point (velocities per (point, m), 4 doubles each): the material a
+// point's scheme call uses is only known on the device.
+static int hostVelocities(lsg_advect *a, lsg_domain *d, int nmat = 1) {
   const size_t cap = d->cur.b.cap;
-  if (a->velCap < cap) {
+  if (a->velCap < cap || a->velMat < nmat) {
     for (void *p : {(void *)a->velIn, (void *)a->velOut, (void *)a->dlist}) if (p) (void)hipFree(p);
     a->velIn = a->velOut = nullptr;
     a->dlist = nullptr;
     a->velCap = 0;
     LSG_HIP(hipMalloc(&a->velIn, cap * 7 * sizeof(double)));
-    LSG_HIP(hipMalloc(&a->velOut, cap * 4 * sizeof(double)));
+    LSG_HIP(hipMalloc(&a->velOut, cap * 4 * (size_t)nmat * sizeof(double)));
     LSG_HIP(hipMalloc(&a->dlist, cap * sizeof(uint32_t)));
     a->velCap = cap;
+    a->velMat = nmat;
   }
   if (!a->ndef) LSG_HIP(hipMalloc(&a->ndef, sizeof(uint32_t)));
   launchVelInputs(g_ctx.s, d->D, d->cur.b, d->f, d->g, a->rp, g_ws->w, a->dlist, a->ndef, a->velIn, g_ctx.sc);
@@ -937,20 +942,22 @@ static int hostVelocities(lsg_advect *a, lsg_domain *d) {
   LSG_HIP(hipMemcpyAsync(&nact, g_ws->w.ncand, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
   LSG_HIP(hipStreamSynchronize(g_ctx.s));
   a->velInH.resize((size_t)nact * 7);
-  a->velOutH.resize((size_t)nact * 4);
+  a->velOutH.resize((size_t)nact * 4 * nmat);
   if (nact) {
     LSG_HIP(hipMemcpy(a->velInH.data(), a->velIn, (size_t)nact * 7 * sizeof(double), hipMemcpyDeviceToHost));
   }
   for (uint32_t j = 0; j < nact; ++j) {
     const double *in = &a->velInH[(size_t)j * 7];
-    double *o = &a->velOutH[(size_t)j * 4];
     const uint64_t pid = (uint64_t)in[6];
-    o[0] = a->hostScalar ? a->hostScalar(in, 0, in + 3, pid, a->hostUser) : 0.0;
-    o[1] = o[2] = o[3] = 0.0;
-    if (a->hostVector) a->hostVector(in, 0, in + 3, pid, a->hostUser, o + 1);
+    for (int m = 0; m < nmat; ++m) {
+      double *o = &a->velOutH[((size_t)j * nmat + m) * 4];
+      o[0] = a->hostScalar ? a->hostScalar(in, m, in + 3, pid, a->hostUser) : 0.0;
+      o[1] = o[2] = o[3] = 0.0;
+      if (a->hostVector) a->hostVector(in, m, in + 3, pid, a->hostUser, o + 1);
+    }
   }
   if (nact)
-    LSG_HIP(hipMemcpy(a->velOut, a->velOutH.data(), (size_t)nact * 4 * sizeof(double), hipMemcpyHostToDevice));
+    LSG_HIP(hipMemcpy(a->velOut, a->velOutH.data(), (size_t)nact * 4 * nmat * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -1240,6 +1247,11 @@ static void stageMM(lsg_advect *a, const double *maxDt, double *dtOut) {
   }
   RateParams rp = a->rp;
   rp.dissOut = a->saveVelocities ? a->diss : nullptr;
+  rp.hostVel = nullptr;
+  if (a->hostScalar || a->hostVector) {
+    if (!a->hostErr) a->hostErr = hostVelocities(a, top, L);
+    rp.hostVel = a->velOut;
+  }
   if (a->ignoreVoids && !a->hostErr) a->hostErr = markVoids(top, rp);
   launchRatesMM(g_ctx.s, top->D, top->cur.b, top->f, top->g, lw, rp, a->rates, g_ctx.sc, maxDt, dtOut, g_ws->w);
   OpParams op{};
@@ -1357,8 +1369,6 @@ static int advectApply(lsg_advect *a) {
     if (a->spatial == 2 || a->spatial == 3 || a->spatial == 12)
       return fail(-3, "Advect: with several level sets the Lax-Friedrichs (global alpha) and learned WENO5 schemes "
                       "are not supported");
-    if (a->hostScalar || a->hostVector)
-      return fail(-3, "Advect: with several level sets only the built-in velocity functors are supported");
   }
   if (prepareWidth(a->spatial) < 0) return fail(-4, "Advect: Discretization scheme not found.");
   if (!(a->cfl > 0)) return fail(-6, "Advect: time step ratio must be positive");
@@ -2287,7 +2297,7 @@ int lsg_advect_set_dissipation_alpha_builtin(lsg_advect *a, int material, const 
 int lsg_advect_set_velocity_builtin_material(lsg_advect *a, int material, int kind, int dir, int k, double b,
                                              double aPos, double aNeg, const double *vec3) {
   if (material < 0 || material >= MAX_LEVEL_SETS) return fail(-1, "material index out of range (0..7)");
-  if (kind != 0 && kind != 1) return fail(-1, "unknown velocity kind");
+  if (kind < 0 || kind > 2) return fail(-1, "unknown velocity kind");
   if (dir < 0 || dir > 2) return fail(-1, "velocity direction must be 0, 1 or 2");
   RateParams &rp = a->rp;
   rp.matSet[material] = 1;
@@ -2317,7 +2327,7 @@ int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, dou
   a->hostScalar = nullptr;
   a->hostVector = nullptr;
   a->hostUser = nullptr;
-  if (kind != 0 && kind != 1) return fail(-1, "unknown velocity kind");
+  if (kind < 0 || kind > 2) return fail(-1, "unknown velocity kind");
   if (dir < 0 || dir > 2) return fail(-1, "velocity direction must be 0, 1 or 2");
   a->rp.velKind = kind;
   a->rp.velDir = dir;
@@ -2618,7 +2628,7 @@ int lsg_batch_destroy(lsg_batch *b) {
 int lsg_batch_set_velocity_builtin(lsg_batch *b, int k, int kind, int dir, int kk, double b0, double aPos, double aNeg,
                                    const double *vec3) {
   if (k < 0 || k >= (int)b->doms.size()) return fail(-1, "batch: level set index out of range");
-  if (kind != 0 && kind != 1) return fail(-1, "unknown velocity kind");
+  if (kind < 0 || kind > 2) return fail(-1, "unknown velocity kind");
   if (dir < 0 || dir > 2) return fail(-1, "velocity direction must be 0, 1 or 2");
   Vel &v = b->segsH[k].vel;
   v.kind = kind;

@@ -524,6 +524,11 @@ struct Vel {
   // built-in VelocityField::getScalarVelocity (lsVelocityField.hpp:17-21)
   LSG_HD double scalar(const double n[3]) const {
     if (kind == 0) return b;
+    if (kind == 2) {  // b + aPos * sum |n_i| (examples/AirGapDeposition/AirGapDeposition.cpp:20-27)
+      double s = 0.0;
+      for (int i = 0; i < 3; ++i) s += fabs(n[i]);
+      return b + aPos * s;
+    }
     double c = n[dir];
     double a = (c > 0) ? aPos : aNeg;
     double ac = fabs(c);

@@ -1084,7 +1084,15 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
         }
       double velocity = 0., v = 0., dn = 0.;
       if (mat >= 0 && !isVoid) {
-        const Vel vm = velOfMaterial(rp, base, mat);
+        Vel vm = velOfMaterial(rp, base, mat);
+        if (rp.hostVel) {  // evaluated on the host for (active point j, material mat)
+          const double *h = rp.hostVel + ((size_t)j * L + mat) * 4;
+          vm.kind = 0;
+          vm.b = h[0];
+          vm.vec[0] = h[1];
+          vm.vec[1] = h[2];
+          vm.vec[2] = h[3];
+        }
         bool used = false;
         v = ratePoint<D, SCHEME>(b, c, phi0, rp, vm, ga, la, velocity, nullptr, nullptr, &used, &dn);
         if (SCHEME == S_SLLF && used) sllfKey = ((unsigned long long)(j + 1) << 4) | (unsigned long long)mat;
