@@ -194,6 +194,19 @@ int lsg_advect_set_calculate_normal_vectors(lsg_advect *a, int c);/* :891 */
 int lsg_advect_set_ignore_voids(lsg_advect *a, int iv);           /* :899 */
 int lsg_advect_set_adaptive_time_stepping(lsg_advect *a, int on, unsigned subdivisions); /* :904; acts with several level sets */
 int lsg_advect_set_update_point_data(lsg_advect *a, int update);  /* :962; rebuildLS translates or drops point data */
+/* Advect::setVelocityUpdateCallback (lsAdvect.hpp:82,966-969), the
+ * std::function<bool(SmartPointer<Domain>)>: fn(top, user) runs after the
+ * first stage of RK2 and after the first and second stages of RK3, with the
+ * advected (top) level set (lsAdvectIntegrationSchemes.hpp:87-89,128-130,
+ * 148-150); its return value is ignored, as in the reference.  With several
+ * level sets a callback also makes the first stage adjust the lower level
+ * sets and every combine that etched restore them (:72-77,98-103,139-144).
+ * The callback may read level sets and change this Advect's velocity field;
+ * changing the advected band, or advecting from inside it, returns an error.
+ * A step that outgrows its band buffers restarts, so the callback can run
+ * again for that step.  fn = NULL removes the callback. */
+typedef int (*lsg_velocity_update_fn)(lsg_domain *top, void *user);
+int lsg_advect_set_velocity_update_callback(lsg_advect *a, lsg_velocity_update_fn fn, void *user);
 /* Advect::setSaveAdvectionVelocities (lsAdvect.hpp:916): point data
  * "AdvectionVelocities" and "Dissipation" after every stage (:762-796). */
 int lsg_advect_set_save_advection_velocities(lsg_advect *a, int save);

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <memory>
 #include <string>
 #include <utility>
@@ -475,6 +476,16 @@ template <class T, int D> class Advect {
   lsg_advect *h_ = nullptr;
   std::vector<SmartPointer<Domain<T, D>>> levelSets_;
   SmartPointer<VelocityField<T>> velocities_;
+  std::function<bool(SmartPointer<Domain<T, D>>)> velocityUpdateCallback_;
+
+  // Between RK stages: the user's callback with the advected level set, then
+  // the velocity field again, so built-in functors it changed take effect.
+  static int velocityUpdate(lsg_domain *, void *user) {
+    auto *self = static_cast<Advect *>(user);
+    (void)self->velocityUpdateCallback_(self->levelSets_.back());
+    (void)self->pushVelocity();
+    return 0;
+  }
 
   // Any other VelocityField subclass is evaluated on the host through its
   // virtual methods, called back for every active point of every stage.
@@ -570,6 +581,13 @@ public:
   void setIgnoreVoids(bool v) { detail::ok(lsg_advect_set_ignore_voids(h_, v ? 1 : 0), "setIgnoreVoids"); }
   void setAdaptiveTimeStepping(bool on = true, unsigned sub = 20) {
     detail::ok(lsg_advect_set_adaptive_time_stepping(h_, on ? 1 : 0, sub), "setAdaptiveTimeStepping");
+  }
+  // Advect::setVelocityUpdateCallback (lsAdvect.hpp:966-969)
+  void setVelocityUpdateCallback(std::function<bool(SmartPointer<Domain<T, D>>)> callback) {
+    velocityUpdateCallback_ = std::move(callback);
+    detail::ok(lsg_advect_set_velocity_update_callback(h_, velocityUpdateCallback_ ? &Advect::velocityUpdate : nullptr,
+                                                       this),
+               "setVelocityUpdateCallback");
   }
   void setSaveAdvectionVelocities(bool s) {
     detail::ok(lsg_advect_set_save_advection_velocities(h_, s ? 1 : 0), "setSaveAdvectionVelocities");

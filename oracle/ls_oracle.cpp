@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <sstream>
 
@@ -1383,6 +1384,8 @@ template <int D> struct Advector {
   // value + 1e-4 (:513-527); its velocity functor is P.velFor(m).
   static constexpr double WRAP_EPS = 1e-4;  // wrappingLayerEpsilon (:80)
   std::vector<Band<D> *> lower;
+  // Advect::velocityUpdateCallback (lsAdvect.hpp:82): run between RK stages
+  std::function<void()> velocityUpdate;
   std::vector<double> mmV, mmS;  // per expanded-band entry, L slots: velocity (after checkDissipation), stop value
   std::vector<double> mmD;       // the same slots: the scheme's dissipation
   int nMat() const { return (int)lower.size() + 1; }
@@ -1817,33 +1820,49 @@ template <int D> struct Advector {
     return movedDown;
   }
 
-  // lsAdvectIntegrationSchemes.hpp:61-108 (no velocity-update callback)
+  void restoreLower(const std::vector<Band<D>> &lowerInit) {
+    for (size_t i = 0; i < lower.size(); ++i) *lower[i] = lowerInit[i];
+  }
+
+  // lsAdvectIntegrationSchemes.hpp:61-108.  With a velocity-update callback
+  // the first stage adjusts the lower level sets, the callback runs after
+  // it, and an etching combine restores the lower level sets.
   double evolveRK2(Band<D> &ls, double maxTimeStep) {
+    const bool cb = bool(velocityUpdate);
     Band<D> init = ls;
-    double dt1 = evolveFE(ls, maxTimeStep, false);
+    std::vector<Band<D>> lowerInit;
+    if (cb)
+      for (Band<D> *b : lower) lowerInit.push_back(*b);
+    double dt1 = evolveFE(ls, maxTimeStep, cb);
     if (dt1 <= 0.) return (dt1 < 0) ? dt1 : 0.;
+    if (cb) velocityUpdate();
     double dt2 = evolveFE(ls, dt1, false);
-    combine(ls, init, 0.5, 0.5);
+    const bool etched = combine(ls, init, 0.5, 0.5);
+    if (etched && cb) restoreLower(lowerInit);
     adjustLowerLayers(ls);
     return 0.5 * dt1 + 0.5 * dt2;
   }
 
-  // lsAdvectIntegrationSchemes.hpp:110-168 (no velocity-update callback):
-  // the lower level sets are restored from the start of the step when the
-  // last combine moved the surface down anywhere
+  // lsAdvectIntegrationSchemes.hpp:110-168: the lower level sets are
+  // restored from the start of the step when the last combine moved the
+  // surface down anywhere (with a velocity-update callback also after the
+  // first combine, and the callback runs after the first two stages)
   double evolveRK3(Band<D> &ls, double maxTimeStep) {
+    const bool cb = bool(velocityUpdate);
     Band<D> init = ls;
     std::vector<Band<D>> lowerInit;
     for (Band<D> *b : lower) lowerInit.push_back(*b);
-    double dt1 = evolveFE(ls, maxTimeStep, false);
+    double dt1 = evolveFE(ls, maxTimeStep, cb);
     if (dt1 <= 0.) return (dt1 < 0) ? dt1 : 0.;
+    if (cb) velocityUpdate();
     double dt2 = evolveFE(ls, dt1, false);
-    combine(ls, init, 0.75, 0.25);
+    const bool etched1 = combine(ls, init, 0.75, 0.25);
+    if (etched1 && cb) restoreLower(lowerInit);
     adjustLowerLayers(ls);
+    if (cb) velocityUpdate();
     double dt3 = evolveFE(ls, dt1, false);
     const bool etched2 = combine(ls, init, 1.0 / 3.0, 2.0 / 3.0);
-    if (etched2)
-      for (size_t i = 0; i < lower.size(); ++i) *lower[i] = lowerInit[i];
+    if (etched2) restoreLower(lowerInit);
     adjustLowerLayers(ls);
     return (dt1 + dt2 + 4.0 * dt3) / 6.0;
   }
@@ -1948,6 +1967,7 @@ lso_domain *lso_domain_copy(const lso_domain *s) { return new lso_domain(*s); }
 
 int64_t lso_domain_size(const lso_domain *d) { return d->D == 2 ? (int64_t)d->b2.idx.size() : (int64_t)d->b3.idx.size(); }
 int lso_domain_width(const lso_domain *d) { return d->D == 2 ? d->b2.width : d->b3.width; }
+int lso_domain_dim(const lso_domain *d) { return d->D; }
 
 void lso_domain_get(const lso_domain *d, int32_t *idx, double *val) {
   if (d->D == 2) getBand(d->b2, idx, val);
@@ -2105,6 +2125,11 @@ struct lso_params {
   int updatePointData;
   int saveVelocities;
   int ignoreVoids;
+  // Advect::setVelocityUpdateCallback (lsAdvect.hpp:966-969): called with the
+  // advected level set and a copy of these parameters, whose velocity
+  // functors it may change for the rest of the apply
+  int (*velUpdate)(lso_domain *top, lso_params *p, void *user);
+  void *velUpdateUser;
 };
 
 static AdvectParams toParams(const lso_params *p) {
@@ -2148,15 +2173,33 @@ static AdvectParams toParams(const lso_params *p) {
 
 // Advect::apply; out[0]=advectedTime, out[1]=currentTimeStep,
 // out[2]=numSteps, out[3]=point updates.  Returns status.
+// The velocity-update callback of an Advector advecting `top`: the callback
+// sees a copy of the parameters and its velocity functors take effect.
+extern "C++" {
+template <class A> void hookVelocityUpdate(A &adv, lso_domain *top, const lso_params *p, lso_params &q) {
+  if (!p->velUpdate) return;
+  q = *p;
+  adv.velocityUpdate = [&adv, top, &q]() {
+    (void)q.velUpdate(top, &q, q.velUpdateUser);
+    AdvectParams n = toParams(&q);
+    adv.P.vel = n.vel;
+    adv.P.matVel = n.matVel;
+  };
+}
+}
+
 int lso_advect(lso_domain *d, const lso_params *p, double *out) {
   AdvectResult r;
+  lso_params q;
   if (d->D == 2) {
     Advector<2> a;
     a.P = toParams(p);
+    hookVelocityUpdate(a, d, p, q);
     r = a.apply(d->b2);
   } else {
     Advector<3> a;
     a.P = toParams(p);
+    hookVelocityUpdate(a, d, p, q);
     r = a.apply(d->b3);
   }
   if (out) {
@@ -2176,8 +2219,10 @@ int lso_advect_multi(lso_domain **levels, int n, const lso_params *p, double *ou
   for (int i = 0; i < n; ++i)
     if (levels[i]->D != D) return -3;
   AdvectResult r;
+  lso_params q;
   auto run = [&](auto &adv, auto member) {
     adv.P = toParams(p);
+    hookVelocityUpdate(adv, levels[n - 1], p, q);
     for (int i = 0; i + 1 < n; ++i) adv.lower.push_back(&(levels[i]->*member));
     r = adv.apply(levels[n - 1]->*member);
   };

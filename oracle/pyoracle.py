@@ -48,7 +48,12 @@ class Params(ctypes.Structure):
         ("updatePointData", ctypes.c_int),
         ("saveVelocities", ctypes.c_int),
         ("ignoreVoids", ctypes.c_int),
+        ("velUpdate", ctypes.c_void_p),
+        ("velUpdateUser", ctypes.c_void_p),
     ]
+
+
+VELOCITY_UPDATE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_void_p)
 
 
 def lib():
@@ -70,6 +75,8 @@ def lib():
         L.lso_domain_size.argtypes = [P]
         L.lso_domain_width.restype = ctypes.c_int
         L.lso_domain_width.argtypes = [P]
+        L.lso_domain_dim.restype = ctypes.c_int
+        L.lso_domain_dim.argtypes = [P]
         L.lso_domain_get.argtypes = [P, i32p, dp]
         L.lso_domain_set.argtypes = [P, ctypes.c_int64, i32p, dp, ctypes.c_int]
         L.lso_domain_from_points.argtypes = [P, ctypes.c_int64, i32p, dp, ctypes.c_int]
@@ -238,16 +245,45 @@ class Domain:
         lib().lso_mark_void_points(self.h)
 
 
+class _View:
+    """Read access to a level set the oracle is advecting (not owned)."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def width(self):
+        return lib().lso_domain_width(self.h)
+
+    def get(self):
+        D = lib().lso_domain_dim(self.h)
+        n = lib().lso_domain_size(self.h)
+        idx = np.zeros((n, D), dtype=np.int32)
+        val = np.zeros(n, dtype=np.float64)
+        lib().lso_domain_get(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                             val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return idx, val
+
+
 def make_params(spatial=0, temporal=0, cfl=0.4999, time=0.0, single_step=False,
                 dissipation_alpha=1.0, calc_normals=True, check_dissipation=True,
                 vel=None, mlp=None, mat_vels=None, adaptive=False, subdivisions=20,
-                update_point_data=True, save_velocities=False, ignore_voids=False):
+                update_point_data=True, save_velocities=False, ignore_voids=False, velocity_update=None):
     """vel: dict(kind, dir, k, b, a_pos, a_neg, vec) of the built-in velocity.
     mlp: float32 array of the learned-weno5 network (spatial scheme 12).
     mat_vels: list of such dicts, the velocity of materials 0, 1, ...
     (multi-material advection; other materials use vel).
-    adaptive, subdivisions: setAdaptiveTimeStepping."""
+    adaptive, subdivisions: setAdaptiveTimeStepping.
+    velocity_update: callback(top, params) between RK stages
+    (setVelocityUpdateCallback); top is a view of the advected level set,
+    params the parameters whose velocity fields (velB, matVel, ...) it may
+    change for the rest of the apply."""
     p = Params()
+    if velocity_update is not None:
+        def trampoline(h, params, user):
+            velocity_update(_View(h), params.contents)
+            return 0
+        p._velUpdateKeep = VELOCITY_UPDATE_FN(trampoline)
+        p.velUpdate = ctypes.cast(p._velUpdateKeep, ctypes.c_void_p)
     p.updatePointData = int(bool(update_point_data))
     p.saveVelocities = int(bool(save_velocities))
     p.ignoreVoids = int(bool(ignore_voids))

@@ -731,9 +731,32 @@ class Advect:
         check(lib().lsg_advect_set_save_advection_velocities(self._h, int(bool(on))))
 
     def setVelocityUpdateCallback(self, callback):
-        """Called between RK stages by the reference; not supported here."""
-        if callback is not None:
-            raise LsgError("setVelocityUpdateCallback: not supported on the GPU path")
+        """callback(domain) runs after the first stage of RK2 and after the
+        first and second stages of RK3 with the advected (last inserted)
+        level set (lsAdvect.hpp:966-969, lsAdvectIntegrationSchemes.hpp:87-89,
+        128-130,148-150); its return value is ignored.  It may read level sets
+        and change the velocity field (setVelocityField or a host
+        VelocityField's state); an exception raised in it is raised again by
+        apply().  None removes it."""
+        from ._lib import VELOCITY_UPDATE_FN
+        if callback is None:
+            self._updateCallback = None
+            check(lib().lsg_advect_set_velocity_update_callback(self._h, None, None))
+            return
+        import weakref
+        ref = weakref.ref(self)
+
+        def trampoline(top, user):
+            me = ref()
+            try:
+                callback(me._domains[-1])
+            except BaseException as e:  # surfaced by apply()
+                me._updateError = e
+            return 0
+
+        self._updateCallback = VELOCITY_UPDATE_FN(trampoline)
+        check(lib().lsg_advect_set_velocity_update_callback(
+            self._h, ctypes.cast(self._updateCallback, ctypes.c_void_p), None))
 
     def setIgnoreVoids(self, v):
         check(lib().lsg_advect_set_ignore_voids(self._h, int(bool(v))))
@@ -748,7 +771,12 @@ class Advect:
         check(lib().lsg_advect_prepare_ls(self._h))
 
     def apply(self):
-        check(lib().lsg_advect_apply(self._h))
+        self._updateError = None
+        st = lib().lsg_advect_apply(self._h)
+        err, self._updateError = self._updateError, None
+        if err is not None:
+            raise err
+        check(st)
 
     def getAdvectedTime(self):
         t = ctypes.c_double()

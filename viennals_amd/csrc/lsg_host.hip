@@ -830,6 +830,9 @@ struct lsg_advect {
   size_t velCap = 0;
   std::vector<double> velInH, velOutH;
   int hostErr = 0;
+  // Advect::setVelocityUpdateCallback (lsAdvect.hpp:82,966-969)
+  lsg_velocity_update_fn velUpdate = nullptr;
+  void *velUpdateUser = nullptr;
   void dropGraphs() {
     for (auto &g : graphs) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -1030,25 +1033,60 @@ static int copyDomain(lsg_domain *dst, const lsg_domain *src) {
   return pdCopy(dst, src);
 }
 
+// Set while a velocity-update callback runs: advection calls made from it
+// would overwrite the step's device scalars and are refused.
+static bool g_inVelocityUpdate = false;
+
+// The velocity-update callback between two stages of an RK step
+// (lsAdvectIntegrationSchemes.hpp:87-89,128-130,148-150).  The stages so far
+// are finished first; a step that already overflowed its band buffers
+// returns 1 without calling back (the caller restarts it, so a restarted
+// step can call back again).  The callback may read the level sets and
+// change the velocity field; a change of the advected band is refused.
+static int velocityUpdate(lsg_advect *a, lsg_domain *top) {
+  LSG_HIP(hipMemcpyAsync(a->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  if (a->hostSc->overflow) return 1;
+  if (a->hostErr) return a->hostErr;
+  const BandBufs b = top->cur.b;
+  const Frame f = top->f;
+  const int width = top->width;
+  g_inVelocityUpdate = true;
+  (void)a->velUpdate(top, a->velUpdateUser);  // the reference ignores the returned bool
+  g_inVelocityUpdate = false;
+  if (std::memcmp(&b, &top->cur.b, sizeof(BandBufs)) || std::memcmp(&f, &top->f, sizeof(Frame)) ||
+      width != top->width)
+    return fail(-1, "the velocity update callback must not modify the advected level set");
+  return 0;
+}
+
 // One time step on the device (Advect::advect, lsAdvect.hpp:817-830).
-static void enqueueStep(lsg_advect *a, lsg_domain *d) {
+// Returns 1 when a velocity-update callback found the step overflowed.
+static int enqueueStep(lsg_advect *a, lsg_domain *d) {
   DevScalars *sc = g_ctx.sc;
   if (a->temporal == 0) {
     stageFE(a, d, &sc->maxTimeStep, &sc->dt[0]);
   } else if (a->temporal == 1) {  // evolveRungeKutta2 (lsAdvectIntegrationSchemes.hpp:61-108)
     copyDomainNoAlloc(a->init, d);
     stageFE(a, d, &sc->maxTimeStep, &sc->dt[0]);
+    if (a->velUpdate)
+      if (int r = velocityUpdate(a, d)) return r;
     stageFE(a, d, &sc->dt[0], &sc->dt[1]);
     combine(a, d, a->init, 0.5, 0.5);
   } else {  // evolveRungeKutta3 (lsAdvectIntegrationSchemes.hpp:110-168)
     copyDomainNoAlloc(a->init, d);
     stageFE(a, d, &sc->maxTimeStep, &sc->dt[0]);
+    if (a->velUpdate)
+      if (int r = velocityUpdate(a, d)) return r;
     stageFE(a, d, &sc->dt[0], &sc->dt[1]);
     combine(a, d, a->init, 0.75, 0.25);
+    if (a->velUpdate)
+      if (int r = velocityUpdate(a, d)) return r;
     stageFE(a, d, &sc->dt[0], &sc->dt[2]);
     combine(a, d, a->init, 1.0 / 3.0, 2.0 / 3.0);
   }
   launchStepTime(g_ctx.s, sc, a->temporal);
+  return 0;
 }
 
 struct StepOut {
@@ -1086,15 +1124,17 @@ static uint64_t stepKey(lsg_advect *a, lsg_domain *d) {
 
 // The device work of one step: prologue (reset flags, take maxTimeStep,
 // keep a copy for retries), the stages, the returned time, band statistics.
-static void enqueueStepBody(lsg_advect *a, lsg_domain *d) {
+static int enqueueStepBody(lsg_advect *a, lsg_domain *d) {
   launchStepBegin(g_ctx.s, g_ctx.sc, a->devMaxDt);
   copyDomainNoAlloc(a->save, d);
   a->evUsed = 0;
   a->starT = StarTiming{a->starPool.data(), (int)a->starPool.size(), 0, a->capturing};
   if (a->profiling) g_starTiming = &a->starT;
-  enqueueStep(a, d);
+  const int r = enqueueStep(a, d);
   g_starTiming = nullptr;
+  if (r) return r;
   launchFrameCheck(g_ctx.s, d->cur.b, d->f, d->g, d->D, REFRAME_SLACK, g_ctx.sc);
+  return 0;
 }
 
 static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &out) {
@@ -1119,7 +1159,9 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
       pdClear(a->save);
       if (a->init) pdClear(a->init);
     }
-    if (a->useGraphs && !hostVel && !pointData) {
+    // a velocity-update callback runs between the stages: no graph
+    const bool callback = a->velUpdate && a->temporal != 0;
+    if (a->useGraphs && !hostVel && !pointData && !callback) {
       uint64_t key = stepKey(a, d);
       lsg_advect::StepGraph *g = nullptr;
       for (auto &c : a->graphs)
@@ -1133,7 +1175,7 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
         bool ok = hipStreamBeginCapture(g_ctx.s, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
           a->capturing = true;
-          enqueueStepBody(a, d);
+          (void)enqueueStepBody(a, d);  // no callback while capturing
           a->capturing = false;
           ok = hipStreamEndCapture(g_ctx.s, &graph) == hipSuccess && graph;
         }
@@ -1175,7 +1217,8 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
       LSG_HIP(hipGraphLaunch(g->exec, g_ctx.s));
       launched = true;
     }
-    if (!launched) enqueueStepBody(a, d);
+    if (!launched)
+      if (int e = enqueueStepBody(a, d); e < 0) return e;
     if (a->hostErr) return a->hostErr;
     LSG_HIP(hipMemcpyAsync(a->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
     LSG_HIP(hipStreamSynchronize(g_ctx.s));
@@ -1223,10 +1266,11 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
 
 // ---------------------------------------------------- several level sets ---
 // Advect with level sets below the advected top (lsAdvect.hpp:435-612,
-// 686-813; lsAdvectIntegrationSchemes.hpp:46-168 without a velocity-update
-// callback).  All level sets share the top's frame during a step.  Steps
-// are launched directly (no graph): RK3 reads the last combine's movedDown
-// flag on the host to decide whether the lower level sets are restored.
+// 686-813; lsAdvectIntegrationSchemes.hpp:46-168).  All level sets share
+// the top's frame during a step.  Steps are launched directly (no graph): a
+// combine's movedDown flag is read on the host to decide whether the lower
+// level sets are restored (RK3's last combine, and with a velocity-update
+// callback every combine).
 
 static void adjustLowerLayers(lsg_advect *a) {  // lsAdvect.hpp:802-813
   if (a->spatial == 9) return;  // the stencil local Lax-Friedrichs scheme keeps the lower level sets
@@ -1281,30 +1325,54 @@ static int enqueueStepMM(lsg_advect *a) {
   launchStepBegin(g_ctx.s, sc, a->devMaxDt);
   copyDomainNoAlloc(a->save, top);
   for (size_t i = 0; i < nl; ++i) copyDomainNoAlloc(a->lowerSave[i], a->levelSets[i]);
+  const bool cb = a->velUpdate != nullptr;
+  // a combine that moved the surface down anywhere sends the lower level sets
+  // back to the start of the step (lsAdvectIntegrationSchemes.hpp:98-103,
+  // 139-144,157-163)
+  auto combineRestore = [&](double wTarget, double wSource) -> int {
+    launchSetFlag(g_ctx.s, &sc->movedDown, 0);
+    combineMM(a, wTarget, wSource, &sc->movedDown);
+    LSG_HIP(hipMemcpyAsync(a->hostSc, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    if (a->hostSc->movedDown && !a->hostSc->overflow)
+      for (size_t i = 0; i < nl; ++i) copyDomainNoAlloc(a->levelSets[i], a->lowerSave[i]);
+    return 0;
+  };
   if (a->temporal == 0) {
     stageMM(a, &sc->maxTimeStep, &sc->dt[0]);
     adjustLowerLayers(a);
   } else if (a->temporal == 1) {
     copyDomainNoAlloc(a->init, top);
     stageMM(a, &sc->maxTimeStep, &sc->dt[0]);
+    if (cb) {  // with a callback the first stage updates the lower level sets
+      adjustLowerLayers(a);
+      if (int r = velocityUpdate(a, top)) return r;
+    }
     stageMM(a, &sc->dt[0], &sc->dt[1]);
-    combineMM(a, 0.5, 0.5, nullptr);
+    if (cb) {
+      if (int r = combineRestore(0.5, 0.5)) return r;
+    } else {
+      combineMM(a, 0.5, 0.5, nullptr);
+    }
     adjustLowerLayers(a);
   } else {
     copyDomainNoAlloc(a->init, top);
     stageMM(a, &sc->maxTimeStep, &sc->dt[0]);
+    if (cb) {
+      adjustLowerLayers(a);
+      if (int r = velocityUpdate(a, top)) return r;
+    }
     stageMM(a, &sc->dt[0], &sc->dt[1]);
-    combineMM(a, 0.75, 0.25, nullptr);
+    if (cb) {
+      if (int r = combineRestore(0.75, 0.25)) return r;
+    } else {
+      combineMM(a, 0.75, 0.25, nullptr);
+    }
     adjustLowerLayers(a);
+    if (cb)
+      if (int r = velocityUpdate(a, top)) return r;
     stageMM(a, &sc->dt[0], &sc->dt[2]);
-    launchSetFlag(g_ctx.s, &sc->movedDown, 0);
-    combineMM(a, 1.0 / 3.0, 2.0 / 3.0, &sc->movedDown);
-    // the lower level sets go back to the start of the step when the last
-    // combine moved the surface down anywhere (lsAdvectIntegrationSchemes.hpp:157-163)
-    LSG_HIP(hipMemcpyAsync(a->hostSc, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
-    LSG_HIP(hipStreamSynchronize(g_ctx.s));
-    if (a->hostSc->movedDown && !a->hostSc->overflow)
-      for (size_t i = 0; i < nl; ++i) copyDomainNoAlloc(a->levelSets[i], a->lowerSave[i]);
+    if (int r = combineRestore(1.0 / 3.0, 2.0 / 3.0)) return r;
     adjustLowerLayers(a);
   }
   launchStepTime(g_ctx.s, sc, a->temporal);
@@ -1336,7 +1404,7 @@ static int runStepMM(lsg_advect *a, double maxTimeStep, StepOut &out) {
     for (lsg_domain *d : a->levelSets) widths.push_back(d->width);
     *a->hostMaxDt = maxTimeStep;
     a->hostErr = 0;
-    if (int e = enqueueStepMM(a)) return e;
+    if (int e = enqueueStepMM(a); e < 0) return e;  // 1: overflow seen before a callback
     if (a->hostErr) return a->hostErr;
     LSG_HIP(hipMemcpyAsync(a->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
     LSG_HIP(hipStreamSynchronize(g_ctx.s));
@@ -2415,7 +2483,15 @@ int lsg_advect_prepare_ls(lsg_advect *a) {
   doExpand(d, w);
   return afterOp(d);
 }
-int lsg_advect_apply(lsg_advect *a) { return advectApply(a); }
+int lsg_advect_apply(lsg_advect *a) {
+  if (g_inVelocityUpdate) return fail(-1, "Advect::apply called from a velocity update callback");
+  return advectApply(a);
+}
+int lsg_advect_set_velocity_update_callback(lsg_advect *a, lsg_velocity_update_fn fn, void *user) {
+  a->velUpdate = fn;
+  a->velUpdateUser = fn ? user : nullptr;
+  return 0;
+}
 int lsg_advect_get_advected_time(lsg_advect *a, double *t) {
   *t = a->advectedTime;
   return 0;
@@ -2661,6 +2737,7 @@ int lsg_batch_set_schemes(lsg_batch *b, int spatial, int temporal, double cfl) {
 }
 
 int lsg_batch_apply(lsg_batch *b) {
+  if (g_inVelocityUpdate) return fail(-1, "batch advection called from a velocity update callback");
   if (int e = batchPrepare(b)) return e;
   return batchStep(b);
 }
