@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
 #include <functional>
 #include <memory>
 #include <string>
@@ -477,13 +478,20 @@ template <class T, int D> class Advect {
   std::vector<SmartPointer<Domain<T, D>>> levelSets_;
   SmartPointer<VelocityField<T>> velocities_;
   std::function<bool(SmartPointer<Domain<T, D>>)> velocityUpdateCallback_;
+  std::exception_ptr callbackError_;
 
   // Between RK stages: the user's callback with the advected level set, then
   // the velocity field again, so built-in functors it changed take effect.
+  // An exception is held here (it must not cross the C library) and
+  // rethrown by apply().
   static int velocityUpdate(lsg_domain *, void *user) {
     auto *self = static_cast<Advect *>(user);
-    (void)self->velocityUpdateCallback_(self->levelSets_.back());
-    (void)self->pushVelocity();
+    try {
+      (void)self->velocityUpdateCallback_(self->levelSets_.back());
+      (void)self->pushVelocity();
+    } catch (...) {
+      if (!self->callbackError_) self->callbackError_ = std::current_exception();
+    }
     return 0;
   }
 
@@ -636,7 +644,9 @@ public:
       return;
     }
     if (!pushVelocity()) return;
+    callbackError_ = nullptr;
     detail::ok(lsg_advect_apply(h_), "Advect::apply");
+    if (callbackError_) std::rethrow_exception(std::exchange(callbackError_, nullptr));
   }
 };
 

@@ -4,8 +4,10 @@
 // the run lists before and after to binary files for the parity test.
 // With "host" as third argument the velocity is a user-defined
 // VelocityField subclass (evaluated on the host through its virtual
-// methods) computing the same function as the built-in functor.
-//   usage: advect_trench <out_prefix> <time> [builtin|host]
+// methods) computing the same function as the built-in functor.  With
+// "callback" a velocity-update callback lowers the built-in functor's
+// constant by 0.01 at every call (between the RK stages).
+//   usage: advect_trench <out_prefix> <time> [builtin|host|callback]
 #include <viennals_gpu/lsAdvect.hpp>
 
 #include <cmath>
@@ -67,7 +69,17 @@ int main(int argc, char **argv) {
   advect.setTemporalScheme(ls::TemporalSchemeEnum::RUNGE_KUTTA_3RD_ORDER);
   advect.setTimeStepRatio(0.4);
   advect.setAdvectionTime(time);
+  unsigned calls = 0;
+  if (argc > 3 && std::string(argv[3]) == "callback") {
+    auto builtin = std::dynamic_pointer_cast<ls::BuiltinVelocityField<T>>(velocity);
+    advect.setVelocityUpdateCallback([builtin, &calls](ls::SmartPointer<ls::Domain<T, D>> top) {
+      ++calls;
+      builtin->b -= 0.01;
+      return top != nullptr;
+    });
+  }
   advect.apply();
+  if (calls) std::printf("callbacks %u\n", calls);
   dump(substrate, prefix + "_final.bin");
   std::printf("steps %u advected %.17g points %u width %d\n", advect.getNumberOfTimeSteps(),
               advect.getAdvectedTime(), substrate->getNumberOfPoints(), substrate->getLevelSetWidth());

@@ -57,3 +57,30 @@ def test_cpp_facade_user_velocity_field(tmp_path):
     (ih, vh), sh = outs["host"]
     assert sb == sh
     assert np.array_equal(ib, ih) and np.array_equal(vb, vh)
+
+
+def test_cpp_facade_velocity_update_callback(tmp_path):
+    """Advect::setVelocityUpdateCallback through the C++ facade: a lambda
+    that lowers the built-in functor's constant between the RK stages, equal
+    to the oracle changing its velocity in the same calls."""
+    print("[gpu-case] C++ facade: trench with a velocity update callback", flush=True)
+    exe = os.path.join(ROOT, "tests", "cpp", "advect_trench")
+    prefix = str(tmp_path / "callback")
+    out = subprocess.run([exe, prefix, "1.0", "callback"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    i0, v0 = read_runs(prefix + "_initial.bin")
+    i1, v1 = read_runs(prefix + "_final.bin")
+    o = po.Domain(3, [-32, -32, -32], [32, 32, 32], [0, 0, 1], 1.0)
+    o.set(i0, v0, 2)
+    calls = []
+
+    def cb(view, p):
+        calls.append(1)
+        p.velB -= 0.01
+    p = po.make_params(spatial=11, temporal=2, cfl=0.4, time=1.0, vel=cases.VEL_ETCH_DIRECTIONAL, velocity_update=cb)
+    st, r = po.advect(o, p)
+    assert st == 0
+    io, vo = o.get()
+    assert cases.bit_equal(io, vo, i1, v1), cases.first_diff(io, vo, i1, v1)
+    assert f"steps {r['steps']}" in out.stdout and f"callbacks {len(calls)}" in out.stdout
+    assert len(calls) == 2 * r["steps"]
