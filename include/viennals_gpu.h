@@ -126,6 +126,11 @@ int lsg_compare_chamfer(lsg_domain *target, lsg_domain *sample, double *out, int
 /* ---- MakeGeometry (lsMakeGeometry.hpp) --------------------------------- */
 /* makeSphere (lsMakeGeometry.hpp:177-241). */
 int lsg_make_sphere(lsg_domain *d, const double *origin, double radius);
+/* MakeGeometry::setIgnoreBoundaryConditions (lsMakeGeometry.hpp:125-141)
+ * with a sphere: bit d of ignore_mask leaves the indices of dimension d
+ * unmapped when it is periodic (:227-235); points that then lie outside the
+ * grid are dropped, as on every insertion. */
+int lsg_make_sphere_ignore_bc(lsg_domain *d, const double *origin, double radius, int ignore_mask);
 /* makePlane (lsMakeGeometry.hpp:245-360): signed Euclidean distance band. */
 int lsg_make_plane(lsg_domain *d, const double *origin, const double *normal);
 /* makeBox (lsMakeGeometry.hpp:363-419): signed Euclidean distance band. */

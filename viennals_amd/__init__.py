@@ -302,17 +302,23 @@ class MakeGeometry:
         self.geometry = geometry
 
     def setIgnoreBoundaryConditions(self, ignore):
-        if ignore:
-            raise LsgError("MakeGeometry.setIgnoreBoundaryConditions(True) is not supported")
+        """One flag for every dimension or a list of flags
+        (lsMakeGeometry.hpp:125-141); spheres only: a periodic dimension
+        whose flag is set keeps its indices unmapped (:227-235)."""
+        flags = list(ignore) if hasattr(ignore, "__len__") else [bool(ignore)] * 3
+        self.ignoreMask = sum(1 << i for i, f in enumerate(flags[:3]) if f)
 
     def apply(self):
         g, d = self.geometry, self.domain
         if d is None:
             raise LsgError("No level set was passed to MakeGeometry.")
         pad = lambda v: list(v) + [0.0] * (3 - len(v))
+        mask = getattr(self, "ignoreMask", 0)
+        if mask and not isinstance(g, Sphere):
+            raise LsgError("MakeGeometry.setIgnoreBoundaryConditions is supported for spheres only")
         if isinstance(g, Sphere):
             a, pa = _f64(pad(g.origin))
-            check(lib().lsg_make_sphere(d._h, pa, g.radius))
+            check(lib().lsg_make_sphere_ignore_bc(d._h, pa, g.radius, int(mask)))
         elif isinstance(g, Plane):
             a, pa = _f64(pad(g.origin))
             b, pb = _f64(pad(g.normal))

@@ -20,6 +20,7 @@ EXPORTED = [
     "lsg_last_error", "lsg_domain_create", "lsg_domain_destroy", "lsg_domain_deep_copy",
     "lsg_domain_insert_points", "lsg_domain_set_runs", "lsg_domain_num_runs", "lsg_domain_get_runs",
     "lsg_domain_num_points", "lsg_domain_width", "lsg_domain_set_width", "lsg_make_sphere", "lsg_make_plane", "lsg_make_box",
+    "lsg_make_sphere_ignore_bc",
     "lsg_expand", "lsg_reduce", "lsg_prune", "lsg_boolean", "lsg_rebuild", "lsg_advect_create",
     "lsg_advect_destroy", "lsg_advect_insert_next_level_set", "lsg_advect_clear_level_sets",
     "lsg_advect_set_velocity_builtin", "lsg_advect_set_velocity_builtin_material",
@@ -74,6 +75,7 @@ def lib():
         "lsg_domain_width": ([P, ip], c_int),
         "lsg_domain_set_width": ([P, c_int], c_int),
         "lsg_make_sphere": ([P, dp, c_double], c_int),
+        "lsg_make_sphere_ignore_bc": ([P, dp, c_double, c_int], c_int),
         "lsg_make_plane": ([P, dp, dp], c_int),
         "lsg_make_box": ([P, dp, dp], c_int),
         "lsg_expand": ([P, c_int], c_int),

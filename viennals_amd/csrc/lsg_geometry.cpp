@@ -10,8 +10,9 @@ namespace lsg {
 
 // MakeGeometry::makeSphere (lsMakeGeometry.hpp:177-241): along-axis distance
 // to the sphere, kept where |distance| <= gridDelta + 1e-5, visited x fastest.
+// Bit d of ignoreBcMask: setIgnoreBoundaryConditions for dimension d.
 void sphereBand(int D, const GridInfo &g, const double *origin, double radius, std::vector<int32_t> &idx,
-                std::vector<double> &val) {
+                std::vector<double> &val, int ignoreBcMask) {
   const double gridDelta = g.dx;
   int32_t index[3] = {0, 0, 0}, endIndex[3] = {0, 0, 0};
   for (int i = 0; i < D; ++i) {
@@ -41,7 +42,9 @@ void sphereBand(int D, const GridInfo &g, const double *origin, double radius, s
     if (std::fabs(distance) <= valueLimit) {
       for (int d = 0; d < D; ++d) {
         int32_t c = index[d];
-        if (g.bc[d] == PERIODIC) c = g.map(d, c);
+        // periodic indices are mapped into the grid unless the boundary
+        // condition is ignored (lsMakeGeometry.hpp:227-235)
+        if (g.bc[d] == PERIODIC && !((ignoreBcMask >> d) & 1)) c = g.map(d, c);
         idx.push_back(c);
       }
       val.push_back(distance / gridDelta);

@@ -2267,9 +2267,12 @@ int lsg_domain_set_width(lsg_domain *d, int w) {
 }
 
 int lsg_make_sphere(lsg_domain *d, const double *origin, double radius) {
+  return lsg_make_sphere_ignore_bc(d, origin, radius, 0);
+}
+int lsg_make_sphere_ignore_bc(lsg_domain *d, const double *origin, double radius, int ignore_mask) {
   std::vector<int32_t> pi;
   std::vector<double> pv;
-  sphereBand(d->D, d->g, origin, radius, pi, pv);
+  sphereBand(d->D, d->g, origin, radius, pi, pv, ignore_mask);
   std::vector<int32_t> ri;
   std::vector<double> rv;
   runsFromPoints(d, pi, pv, ri, rv);
