@@ -25,6 +25,22 @@ static inline double smax(double a, double b) { return (a < b) ? b : a; }
 static inline double smin(double a, double b) { return (b < a) ? b : a; }
 static constexpr double EPS = std::numeric_limits<double>::epsilon();
 
+// Planted mistakes for the pin tests (tests/test_oracle_offplane.py builds
+// the oracle with -DLSO_MUTATION=n and checks that its pins fail); 0, the
+// default, is the restatement itself.
+//   1: the LF / LLF / LLLF dissipation sign flipped
+//   2: the second-order ENO correction of the one-sided differences dropped
+//   3: LLF alphas from the centre point instead of the 3^D box
+//   4: Engquist-Osher upwinding of the scalar velocity swapped
+#ifndef LSO_MUTATION
+#define LSO_MUTATION 0
+#endif
+#if LSO_MUTATION == 1
+#define LSO_DISS(a, b) ((b) - (a))
+#else
+#define LSO_DISS(a, b) ((a) - (b))
+#endif
+
 // ----------------------------------------------------------------- Band ---
 
 template <int D> void Band<D>::buildRows() {
@@ -876,7 +892,7 @@ static void upwindDiffs(const SchemeCtx<D> &s, const Idx<D> &c, int i, int order
   const double phiNeg = s.f(c, i, -1);
   diffPos = (phiPos - phi0) / deltaPos;
   diffNeg = (phiNeg - phi0) / deltaNeg;
-  if (order == 2) {
+  if (order == 2 && LSO_MUTATION != 2) {
     const double deltaPosPos = 2 * gridDelta;
     const double deltaNegNeg = -2 * gridDelta;
     const double phiPosPos = s.f(c, i, 2);
@@ -925,7 +941,7 @@ template <int D> static void schemeEO(const SchemeCtx<D> &s, const Idx<D> &c, in
   }
   double sv = s.vel->scalar(n);
   const double *vv = s.vel->vec;
-  if (sv > 0) vel_grad += std::sqrt(gradPosTotal) * sv;
+  if ((sv > 0) != (LSO_MUTATION == 4)) vel_grad += std::sqrt(gradPosTotal) * sv;
   else vel_grad += std::sqrt(gradNegTotal) * sv;
   for (int w = 0; w < D; w++) {
     if (vv[w] > 0.) vel_grad += vv[w] * gradPos[w];
@@ -1005,7 +1021,7 @@ template <int D> static void schemeLF(const SchemeCtx<D> &s, const Idx<D> &c, in
       normalModulus += n[i] * n[i];
     }
     grad += pow2((diffNeg + diffPos) * 0.5);
-    dissipation += s.alphaFactor * s.globalAlphas[i] * (diffPos - diffNeg) * 0.5;
+    dissipation += s.alphaFactor * s.globalAlphas[i] * LSO_DISS(diffPos, diffNeg) * 0.5;
   }
   if (s.calcNormals) {
     normalModulus = 1. / std::sqrt(normalModulus);
@@ -1054,7 +1070,7 @@ static void schemeLLLF(const SchemeCtx<D> &s, const Idx<D> &c, int order, double
   for (int i = 0; i < D; ++i) {
     double alpha = std::fabs((sv + vv[i]) * n[i]);
     alphaAcc[i] = smax(alphaAcc[i], alpha);
-    dissipation += s.alphaFactor * alpha * (gradNeg[i] - gradPos[i]) * 0.5;
+    dissipation += s.alphaFactor * alpha * LSO_DISS(gradNeg[i], gradPos[i]) * 0.5;
   }
   H = totalGrad;
   Dn = dissipation;
@@ -1136,6 +1152,19 @@ static void schemeLLF(const SchemeCtx<D> &s, const Idx<D> &c, int order, double 
   int numNeighbors = 1;
   for (int d = 0; d < D; ++d) numNeighbors *= 3;
   for (int q = 0; q < numNeighbors; ++q) {  // :193-231
+#if LSO_MUTATION == 3
+    bool centre = true;
+    for (int d = 0; d < D; ++d) centre = centre && ni[d] == 0;
+    if (!centre) {
+      int dm = 0;
+      for (; dm < D - 1; ++dm) {
+        if (ni[dm] < 1) break;
+        ni[dm] = -1;
+      }
+      ++ni[dm];
+      continue;
+    }
+#endif
     Idx<D> nc;
     for (int d = 0; d < D; ++d) nc[d] = s.ls->g.map(d, (int64_t)c[d] + ni[d]);
     // neighbour values around the box point (box iterator offsets, mapped
@@ -1172,7 +1201,7 @@ static void schemeLLF(const SchemeCtx<D> &s, const Idx<D> &c, int order, double 
     }
     ++ni[dim];
   }
-  for (int i = 0; i < D; ++i) dissipation += s.alphaFactor * alpha[i] * (gradNeg[i] - gradPos[i]) * 0.5;
+  for (int i = 0; i < D; ++i) dissipation += s.alphaFactor * alpha[i] * LSO_DISS(gradNeg[i], gradPos[i]) * 0.5;
   H = totalGrad;
   Dn = dissipation;
 }
@@ -2256,6 +2285,30 @@ double lso_compute_rates(lso_domain *d, const lso_params *p, double maxTimeStep,
     a.P = toParams(p);
     dt = a.computeRates(d->b3, maxTimeStep);
     for (int64_t i = 0; i < (int64_t)a.rates.size() && i < cap; ++i) rates[i] = a.rates[i];
+  }
+  return dt;
+}
+
+// The same pass, with the scheme's dissipation per entry beside the rates.
+double lso_compute_rates_diss(lso_domain *d, const lso_params *p, double maxTimeStep, double *rates, double *diss,
+                              int64_t cap) {
+  double dt;
+  if (d->D == 2) {
+    Advector<2> a;
+    a.P = toParams(p);
+    dt = a.computeRates(d->b2, maxTimeStep);
+    for (int64_t i = 0; i < (int64_t)a.rates.size() && i < cap; ++i) {
+      rates[i] = a.rates[i];
+      diss[i] = a.diss[i];
+    }
+  } else {
+    Advector<3> a;
+    a.P = toParams(p);
+    dt = a.computeRates(d->b3, maxTimeStep);
+    for (int64_t i = 0; i < (int64_t)a.rates.size() && i < cap; ++i) {
+      rates[i] = a.rates[i];
+      diss[i] = a.diss[i];
+    }
   }
   return dt;
 }

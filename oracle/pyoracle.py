@@ -10,7 +10,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liblsoracle.so")
+# LSO_ORACLE_LIB: another build of the oracle (the pin tests load builds
+# with planted mistakes through it)
+_LIB_PATH = os.environ.get("LSO_ORACLE_LIB") or os.path.join(_HERE, "_build", "liblsoracle.so")
 _lib = None
 
 POS = np.finfo(np.float64).max
@@ -100,6 +102,8 @@ def lib():
         L.lso_advect_multi.restype = ctypes.c_int
         L.lso_compute_rates.argtypes = [P, ctypes.POINTER(Params), ctypes.c_double, dp, ctypes.c_int64]
         L.lso_compute_rates.restype = ctypes.c_double
+        L.lso_compute_rates_diss.argtypes = [P, ctypes.POINTER(Params), ctypes.c_double, dp, dp, ctypes.c_int64]
+        L.lso_compute_rates_diss.restype = ctypes.c_double
         fp = ctypes.POINTER(ctypes.c_float)
         L.lso_mlp_forward.argtypes = [fp, fp, fp]
         L.lso_weno5_learned.argtypes = [dp, ctypes.c_double, ctypes.c_int, fp]
@@ -351,6 +355,18 @@ def compute_rates(dom, params, max_dt=np.finfo(np.float64).max):
                                  buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), cap)
     n = lib().lso_domain_size(dom.h)
     return dt, buf[:n]
+
+
+def compute_rates_diss(dom, params, max_dt=np.finfo(np.float64).max):
+    """(dt, rates, dissipation) of one prepareLS + integrateTime pass."""
+    cap = lib().lso_domain_size(dom.h) * 32 + 4096
+    r = np.zeros(cap, dtype=np.float64)
+    q = np.zeros(cap, dtype=np.float64)
+    dpp = ctypes.POINTER(ctypes.c_double)
+    dt = lib().lso_compute_rates_diss(dom.h, ctypes.byref(params), float(max_dt), r.ctypes.data_as(dpp),
+                                      q.ctypes.data_as(dpp), cap)
+    n = lib().lso_domain_size(dom.h)
+    return dt, r[:n], q[:n]
 
 
 def weno5(x, dx, plus):
