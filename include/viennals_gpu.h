@@ -122,6 +122,12 @@ int lsg_surface_points(lsg_domain *d, double *out, int64_t cap, int64_t *n);
  * points: out[0..4] = forward (target -> sample), backward, Chamfer, RMS and
  * maximum nearest-neighbour distance; counts[0..1] = target / sample points. */
 int lsg_compare_chamfer(lsg_domain *target, lsg_domain *sample, double *out, int64_t *counts);
+/* CompareVolume::apply (lsCompareVolume.hpp:176-289; getCellCount,
+ * getVolumeMismatch): the cells whose corner average is inside (<= 0) in
+ * exactly one of the two level sets, after expanding narrower level sets to
+ * width 3 on copies; both must share grid bounds, boundary conditions and
+ * gridDelta.  cells and volume may be NULL. */
+int lsg_compare_volume(lsg_domain *target, lsg_domain *sample, uint64_t *cells, double *volume);
 
 /* ---- MakeGeometry (lsMakeGeometry.hpp) --------------------------------- */
 /* makeSphere (lsMakeGeometry.hpp:177-241). */
@@ -131,10 +137,20 @@ int lsg_make_sphere(lsg_domain *d, const double *origin, double radius);
  * unmapped when it is periodic (:227-235); points that then lie outside the
  * grid are dropped, as on every insertion. */
 int lsg_make_sphere_ignore_bc(lsg_domain *d, const double *origin, double radius, int ignore_mask);
-/* makePlane (lsMakeGeometry.hpp:245-360): signed Euclidean distance band. */
+/* makePlane (lsMakeGeometry.hpp:245-360): the plane's mesh over the grid
+ * (one line or two triangles reaching one cell beyond the grid bounds along
+ * the axes other than the single infinite one) converted by FromSurfaceMesh
+ * as for makeBox; boundary elements are always dropped. */
 int lsg_make_plane(lsg_domain *d, const double *origin, const double *normal);
-/* makeBox (lsMakeGeometry.hpp:363-419): signed Euclidean distance band. */
+/* makeBox (lsMakeGeometry.hpp:363-419): the box's surface mesh (four lines
+ * or twelve triangles) converted by FromSurfaceMesh
+ * (lsFromSurfaceMesh.hpp:243-480): grid-line crossings, distances along the
+ * lines times 1 - 1e-7, clipped to +-1, the smallest sign distance per point;
+ * mesh elements on the grid boundary are dropped except along periodic axes.
+ * The _ignore_bc variant takes setIgnoreBoundaryConditions per axis (bit d):
+ * then boundary elements are dropped along periodic axis d too. */
 int lsg_make_box(lsg_domain *d, const double *min_corner, const double *max_corner);
+int lsg_make_box_ignore_bc(lsg_domain *d, const double *min_corner, const double *max_corner, int ignore_mask);
 
 /* ---- band maintenance ----------------------------------------------------- */
 /* Expand<T,D>(ls, width).apply() (lsExpand.hpp:44-172). */
@@ -203,7 +219,10 @@ int lsg_advect_set_update_point_data(lsg_advect *a, int update);  /* :962; rebui
  * std::function<bool(SmartPointer<Domain>)>: fn(top, user) runs after the
  * first stage of RK2 and after the first and second stages of RK3, with the
  * advected (top) level set (lsAdvectIntegrationSchemes.hpp:87-89,128-130,
- * 148-150); its return value is ignored, as in the reference.  With several
+ * 148-150); its return value is ignored, as in the reference, unless it is
+ * negative: a callback that failed (an exception in a language binding) ends
+ * the apply at that stage with an error, as the exception ends the
+ * reference's apply (the level set keeps the stages done).  With several
  * level sets a callback also makes the first stage adjust the lower level
  * sets and every combine that etched restore them (:72-77,98-103,139-144).
  * The callback may read level sets and change this Advect's velocity field;

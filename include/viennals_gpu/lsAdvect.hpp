@@ -6,9 +6,10 @@
 //
 // Differences a caller sees:
 //  * T must be double (the GPU path computes in FP64); D is 2 or 3.
-//  * Velocities are GPU functors (BuiltinVelocityField); a user subclass of
-//    VelocityField<T> that is not a BuiltinVelocityField is rejected with an
-//    error, like the reference rejects a missing field.
+//  * Velocities: a BuiltinVelocityField runs as a GPU functor; any other
+//    subclass of VelocityField<T> is evaluated on the host through its
+//    virtual methods for every active point of every stage (host callbacks
+//    of the library).
 //  * Errors are logged to stderr and the call returns, as the reference does
 //    with its Logger (lsAdvect.hpp:1019-1027).
 #pragma once
@@ -293,20 +294,45 @@ public:
   unsigned getNumSamplePoints() const { return (unsigned)n_[1]; }
 };
 
+// CompareVolume (lsCompareVolume.hpp:176-289) on the GPU (CompareArea in 2D).
+template <class T, int D> class CompareVolume {
+  SmartPointer<Domain<T, D>> target_, sample_;
+  uint64_t cells_ = 0;
+  double volume_ = 0;
+
+public:
+  CompareVolume() = default;
+  CompareVolume(SmartPointer<Domain<T, D>> t, SmartPointer<Domain<T, D>> s) : target_(t), sample_(s) {}
+  void setLevelSetTarget(SmartPointer<Domain<T, D>> t) { target_ = t; }
+  void setLevelSetSample(SmartPointer<Domain<T, D>> s) { sample_ = s; }
+  void apply() { detail::ok(lsg_compare_volume(target_->handle(), sample_->handle(), &cells_, &volume_), "CompareVolume"); }
+  double getVolumeMismatch() const { return volume_; }
+  double getAreaMismatch() const { return volume_; }
+  unsigned long getCellCount() const { return (unsigned long)cells_; }
+};
+
 template <class T, int D> class MakeGeometry {
   SmartPointer<Domain<T, D>> ls_;
   int kind_ = -1;
   std::array<T, 3> a_{}, b_{};
   T r_ = 0;
+  int ignoreMask_ = 0;
 
 public:
   MakeGeometry(SmartPointer<Domain<T, D>> ls, SmartPointer<Sphere<T, D>> s) : ls_(ls), kind_(0), a_(s->origin), r_(s->radius) {}
   MakeGeometry(SmartPointer<Domain<T, D>> ls, SmartPointer<Plane<T, D>> p) : ls_(ls), kind_(1), a_(p->origin), b_(p->normal) {}
   MakeGeometry(SmartPointer<Domain<T, D>> ls, SmartPointer<Box<T, D>> b) : ls_(ls), kind_(2), a_(b->minCorner), b_(b->maxCorner) {}
+  // lsMakeGeometry.hpp:125-141: spheres keep periodic indices unmapped, boxes
+  // drop their boundary elements along periodic axes; planes ignore it
+  void setIgnoreBoundaryConditions(bool ignore) { ignoreMask_ = ignore ? 7 : 0; }
+  template <std::size_t N> void setIgnoreBoundaryConditions(std::array<bool, N> ignore) {
+    ignoreMask_ = 0;
+    for (unsigned i = 0; i < D && i < N; ++i) ignoreMask_ |= ignore[i] ? (1 << i) : 0;
+  }
   void apply() {
-    if (kind_ == 0) detail::ok(lsg_make_sphere(ls_->handle(), a_.data(), r_), "MakeGeometry");
+    if (kind_ == 0) detail::ok(lsg_make_sphere_ignore_bc(ls_->handle(), a_.data(), r_, ignoreMask_), "MakeGeometry");
     else if (kind_ == 1) detail::ok(lsg_make_plane(ls_->handle(), a_.data(), b_.data()), "MakeGeometry");
-    else if (kind_ == 2) detail::ok(lsg_make_box(ls_->handle(), a_.data(), b_.data()), "MakeGeometry");
+    else if (kind_ == 2) detail::ok(lsg_make_box_ignore_bc(ls_->handle(), a_.data(), b_.data(), ignoreMask_), "MakeGeometry");
   }
 };
 
@@ -482,8 +508,8 @@ template <class T, int D> class Advect {
 
   // Between RK stages: the user's callback with the advected level set, then
   // the velocity field again, so built-in functors it changed take effect.
-  // An exception is held here (it must not cross the C library) and
-  // rethrown by apply().
+  // An exception is held here (it must not cross the C library), the
+  // library stops the apply at that stage, and apply() rethrows it.
   static int velocityUpdate(lsg_domain *, void *user) {
     auto *self = static_cast<Advect *>(user);
     try {
@@ -491,6 +517,7 @@ template <class T, int D> class Advect {
       (void)self->pushVelocity();
     } catch (...) {
       if (!self->callbackError_) self->callbackError_ = std::current_exception();
+      return -1;  // the library stops the apply at this stage
     }
     return 0;
   }
@@ -645,8 +672,9 @@ public:
     }
     if (!pushVelocity()) return;
     callbackError_ = nullptr;
-    detail::ok(lsg_advect_apply(h_), "Advect::apply");
+    const int status = lsg_advect_apply(h_);
     if (callbackError_) std::rethrow_exception(std::exchange(callbackError_, nullptr));
+    detail::ok(status, "Advect::apply");
   }
 };
 

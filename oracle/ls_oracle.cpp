@@ -7,6 +7,7 @@
 #include "ls_oracle.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -812,6 +813,319 @@ template <int D> void makeSphere(Band<D> &ls, const double *origin, double radiu
     for (int j = 0; j < D; ++j)
       if (ls.g.bc[j] == PERIODIC) pp.first[j] = ls.g.map(j, pp.first[j]);
   fromPoints<D>(ls, pts, 2);
+}
+
+// FromSurfaceMesh (lsFromSurfaceMesh.hpp:112-480) for the meshes of
+// MakeGeometry's planes and boxes.  ViennaHRLE's grid functions (absent from
+// the reference tree) are restated as: gridPosition = index * delta,
+// globalCoordinate2LocalIndex = coordinate / delta, getMinIndex /
+// getMaxIndex = the grid's end points (-+715827882 on infinite axes, as the
+// grid-minimum run start of tests/BooleanOperationExactZero/result.hpp
+// shows; the period's end, max grid point + 1, on periodic axes),
+// getMinLocalCoordinate = getMinIndex * delta, and index wrapping on
+// periodic axes for globalIndex(ices)2LocalIndex(ices).  The fixture pins
+// this choice for infinite axes (a box edge on the user's bound is kept).
+using Vec3 = std::array<double, 3>;
+
+// calculateGridlineTriangleIntersection, 2D (:118-155)
+static int gridlineLine(const Vec3 &Point, const Vec3 *c, int dir, double &intersection) {
+  bool inside_pos = true, inside_neg = true;
+  double A[2];
+  const int dirA = (dir + 1) % 2;
+  for (int k = 0; k < 2; k++) {
+    A[k] = c[(k + 1) % 2][dirA] - Point[dirA];
+    if (k == dir) A[k] = -A[k];
+    if (A[k] < 0.) inside_pos = false;
+    if (A[k] > 0.) inside_neg = false;
+  }
+  if ((!inside_pos && inside_neg) || (inside_pos && !inside_neg)) {
+    double sum = A[0] + A[1];
+    int k = 0;
+    for (int i = 1; i < 2; ++i) {
+      if (inside_pos) {
+        if (A[i] > A[k]) k = i;
+      } else {
+        if (A[i] < A[k]) k = i;
+      }
+    }
+    intersection = c[k][dir] + (c[(k + 1) % 2][dir] - c[k][dir]) * (A[(k + 1) % 2] / sum);
+    return inside_pos ? 1 : -1;
+  }
+  return 0;
+}
+
+// calculateGridlineTriangleIntersection, 3D (:163-215)
+static int gridlineTriangle(const Vec3 &Point, const Vec3 *c, int dir, double &intersection) {
+  bool inside_pos = true, inside_neg = true;
+  double A[3];
+  const int dirA = (dir + 1) % 3;
+  const int dirB = (dir + 2) % 3;
+  for (int k = 0; k < 3; k++) {
+    bool swapped = (c[(k + 1) % 3] < c[(k + 2) % 3]);  // std::array: lexicographic
+    const Vec3 &v1 = swapped ? c[(k + 2) % 3] : c[(k + 1) % 3];
+    const Vec3 &v2 = swapped ? c[(k + 1) % 3] : c[(k + 2) % 3];
+    A[k] = (v1[dirA] - Point[dirA]) * (v2[dirB] - Point[dirB]) - (v2[dirA] - Point[dirA]) * (v1[dirB] - Point[dirB]);
+    if (swapped) A[k] = -A[k];
+    if (A[k] < 0.) inside_pos = false;
+    if (A[k] > 0.) inside_neg = false;
+  }
+  if ((!inside_pos && inside_neg) || (inside_pos && !inside_neg)) {
+    double sum = A[0] + A[1] + A[2];
+    int k = 0;
+    for (int i = 1; i < 3; ++i) {
+      if (inside_pos) {
+        if (A[i] > A[k]) k = i;
+      } else {
+        if (A[i] < A[k]) k = i;
+      }
+    }
+    intersection = c[k][dir] + (c[(k + 1) % 3][dir] - c[k][dir]) * (A[(k + 1) % 3] / sum) +
+                   (c[(k + 2) % 3][dir] - c[k][dir]) * (A[(k + 2) % 3] / sum);
+    return inside_pos ? 1 : -1;
+  }
+  return 0;
+}
+
+template <int D> static int32_t getMinIndex(const Grid<D> &g, int d) { return g.lo[d]; }
+template <int D> static int32_t getMaxIndex(const Grid<D> &g, int d) {
+  return g.bc[d] == PERIODIC ? g.hi[d] + 1 : g.hi[d];
+}
+template <int D> static int32_t periodicLocal(const Grid<D> &g, int d, int64_t i) {
+  if (g.bc[d] != PERIODIC) return (int32_t)i;
+  return g.map(d, i);
+}
+
+// FromSurfaceMesh::apply (:243-480); elements hold D node numbers.
+template <int D>
+void fromSurfaceMesh(Band<D> &ls, const std::vector<Vec3> &meshNodes, const std::vector<std::array<unsigned, 3>> &elements,
+                     const bool *removeBoundaryTriangles) {
+  const Grid<D> &grid = ls.g;
+  const double boundaryEps = 1e-5, distanceEps = 1e-4, signEps = 1e-6;
+  bool removeBoundaries[D];
+  for (int i = 0; i < D; ++i)
+    removeBoundaries[i] = !(!removeBoundaryTriangles[i] && grid.bc[i] == PERIODIC);  // :272-279
+  auto maxGridPoint = [&](int i) { return grid.hi[i]; };
+  std::vector<std::pair<Idx<D>, std::pair<double, double>>> points;
+  const double gridDelta = grid.delta;
+  double gridMin[D], gridMax[D];
+  for (int i = 0; i < D; ++i) {
+    gridMin[i] = getMinIndex(grid, i) * gridDelta;
+    gridMax[i] = getMaxIndex(grid, i) * gridDelta;
+  }
+  for (const auto &element : elements) {
+    Vec3 nodes[D];
+    double center[D];
+    for (int q = 0; q < D; ++q) {
+      nodes[q] = Vec3{0, 0, 0};
+      center[q] = 0;
+    }
+    bool flags[2 * D];
+    for (int q = 0; q < 2 * D; ++q) flags[q] = true;
+    bool removeElement = false;
+    for (int dim = 0; dim < D; dim++) {  // :311-329
+      for (int q = 0; q < D; q++) {
+        nodes[q][dim] = meshNodes[element[q]][dim];
+        if (std::abs(nodes[q][dim] - gridMin[dim]) < boundaryEps * gridDelta) nodes[q][dim] = gridMin[dim];
+        if (std::abs(nodes[q][dim] - gridMax[dim]) < boundaryEps * gridDelta) nodes[q][dim] = gridMax[dim];
+        if (nodes[q][dim] > gridMin[dim]) flags[dim] = false;
+        if (nodes[q][dim] < gridMax[dim]) flags[dim + D] = false;
+        center[dim] += nodes[q][dim];
+      }
+      if (removeBoundaries[dim] && (flags[dim] || flags[dim + D])) removeElement = true;
+    }
+    if (removeElement) continue;
+    for (int q = 0; q < D; q++) center[q] /= static_cast<double>(D);
+    Vec3 minNode = nodes[0], maxNode = nodes[0];
+    for (int i = 1; i < D; ++i)
+      for (int q = 0; q < D; ++q) {
+        minNode[q] = std::min(minNode[q], nodes[i][q]);
+        maxNode[q] = std::max(maxNode[q], nodes[i][q]);
+      }
+    Idx<D> minIndex, maxIndex;  // :348-353
+    for (int q = 0; q < D; q++) {
+      minIndex[q] = static_cast<int32_t>(std::ceil(minNode[q] / gridDelta));
+      maxIndex[q] = static_cast<int32_t>(std::floor(maxNode[q] / gridDelta));
+    }
+    for (int i = 0; i < D; ++i)  // :357-363
+      if (removeBoundaries[i]) {
+        minIndex[i] = std::max(minIndex[i], getMinIndex(grid, i));
+        maxIndex[i] = std::min(maxIndex[i], maxGridPoint(i));
+      }
+    for (int z = 0; z < D; z++) {  // :366-451
+      // box(min_bb, max_bb) keeps the per-axis minimum and maximum of its two
+      // corners (:26-30), so it is never empty
+      int32_t bmin[2] = {0, 0}, bmax[2] = {0, 0};
+      for (int h = 0; h < D - 1; ++h) {
+        bmin[h] = std::min(minIndex[(z + h + 1) % D], maxIndex[(z + h + 1) % D]);
+        bmax[h] = std::max(minIndex[(z + h + 1) % D], maxIndex[(z + h + 1) % D]);
+      }
+      int32_t pos[2] = {bmin[0], bmin[1]};
+      while (pos[D - 2] <= bmax[D - 2]) {  // box::iterator, first axis fastest
+        Idx<D> it_b;
+        for (int h = 0; h < D - 1; ++h) it_b[(z + h + 1) % D] = pos[h];
+        Vec3 p = {0, 0, 0};
+        for (int k = 1; k < D; k++) p[(k + z) % D] = it_b[(k + z) % D] * gridDelta;
+        double intersection = 0;
+        const int status = (D == 3) ? gridlineTriangle(p, nodes, z, intersection)
+                                    : gridlineLine(p, nodes, z, intersection);
+        if (status != 0 &&
+            !(removeBoundaries[z] && intersection > gridDelta * getMaxIndex(grid, z)) &&
+            !(removeBoundaries[z] && intersection < getMinIndex(grid, z) * gridDelta)) {
+          const double intersection2 = intersection / gridDelta;
+          int64_t floorI = (int64_t)std::floor(intersection2 - distanceEps);
+          int64_t ceilI = (int64_t)std::ceil(intersection2 + distanceEps);
+          floorI = std::max<int64_t>(floorI, (int64_t)minIndex[z] - 1);
+          ceilI = std::min<int64_t>(ceilI, (int64_t)maxIndex[z] + 1);
+          floorI = std::max<int64_t>(floorI, getMinIndex(grid, z));
+          ceilI = std::min<int64_t>(ceilI, getMaxIndex(grid, z));
+          if (!removeBoundaries[z]) {
+            floorI = periodicLocal(grid, z, floorI);
+            ceilI = periodicLocal(grid, z, ceilI);
+          }
+          double t[D];
+          for (int q = 0; q < D; ++q) t[q] = center[q];
+          t[z] -= intersection;
+          for (int k = 1; k < D; k++) t[(z + k) % D] -= p[(z + k) % D];
+          double norm = 0;
+          for (int q = 0; q < D; ++q) norm += t[q] * t[q];
+          norm = std::sqrt(norm);
+          for (int q = 0; q < D; ++q) t[q] /= norm;
+          for (int64_t k = floorI; k <= ceilI; ++k) {  // :414-447
+            it_b[z] = (int32_t)k;
+            double RealDistance = it_b[z] - intersection2;
+            double SignDistance = RealDistance;
+            SignDistance -= signEps * t[z];
+            if (status < 0) {
+              RealDistance = -RealDistance;
+              SignDistance = -SignDistance;
+            }
+            if (RealDistance < 0.) SignDistance = -SignDistance;
+            RealDistance *= (1. - distanceEps * 1e-3);
+            if (RealDistance > 1.) RealDistance = 1.;
+            if (RealDistance < -1.) RealDistance = -1.;
+            if (RealDistance == 0.) RealDistance = 0.;
+            Idx<D> local;
+            for (int q = 0; q < D; ++q) local[q] = periodicLocal(grid, q, it_b[q]);
+            points.push_back({local, {SignDistance, RealDistance}});
+          }
+        }
+        int h = 0;
+        for (; h < D - 2; ++h) {
+          ++pos[h];
+          if (pos[h] <= bmax[h]) break;
+          pos[h] = bmin[h];
+        }
+        if (h == D - 2) ++pos[h];
+      }
+    }
+  }
+  std::sort(points.begin(), points.end());  // :455
+  std::vector<std::pair<Idx<D>, double>> pts;
+  for (size_t i = 0; i < points.size(); ++i) {  // :458-469: first (smallest) per index
+    if (i > 0 && points[i].first == points[i - 1].first) continue;
+    pts.push_back({points[i].first, points[i].second.second});
+  }
+  fromPoints<D>(ls, pts, 2);  // :472-474
+}
+
+// MakeGeometry::makePlane (lsMakeGeometry.hpp:245-360); returns 0, or a
+// negative code for the reference's error cases
+template <int D> int makePlane(Band<D> &ls, const double *origin, const double *passedNormal) {
+  const Grid<D> &grid = ls.g;
+  const double gridDelta = grid.delta;
+  double modulus = 0.;
+  double normal[3] = {0, 0, 0};
+  for (unsigned i = 0; i < D; ++i) {
+    normal[i] = passedNormal[i];
+    modulus += normal[i] * normal[i];
+  }
+  modulus = std::sqrt(modulus);
+  for (unsigned i = 0; i < D; ++i) normal[i] /= modulus;
+  unsigned i = 0;
+  bool infDimSet = false;
+  for (unsigned n = 0; n < D; ++n)
+    if (grid.bc[n] == INFINITE) {
+      if (!infDimSet) {
+        i = n;
+        infDimSet = true;
+      } else {
+        return -2;
+      }
+    }
+  if (!infDimSet) return -4;
+  if (passedNormal[i] == 0.) return -3;
+  std::vector<Vec3> cornerPoints(2 * (D - 1), Vec3{0, 0, 0});
+  unsigned j = (i + 1) % D;
+  unsigned k = (i + 2) % D;
+  double minCoord[2], maxCoord[2];
+  for (unsigned n = 0; n < D - 1; ++n) {
+    minCoord[n] = gridDelta * (getMinIndex(grid, (i + n + 1) % D) - 1);
+    maxCoord[n] = gridDelta * (getMaxIndex(grid, (i + n + 1) % D) + 1);
+  }
+  cornerPoints[0][j] = minCoord[0];
+  cornerPoints[1][j] = maxCoord[0];
+  if constexpr (D == 3) {
+    cornerPoints[0][k] = minCoord[1];
+    cornerPoints[1][k] = maxCoord[1];
+    cornerPoints[2][j] = minCoord[0];
+    cornerPoints[2][k] = maxCoord[1];
+    cornerPoints[3][j] = maxCoord[0];
+    cornerPoints[3][k] = minCoord[1];
+  }
+  for (unsigned n = 0; n < cornerPoints.size(); ++n) {
+    double numerator = (cornerPoints[n][j] - origin[j]) * normal[j];
+    if constexpr (D == 3) numerator += (cornerPoints[n][k] - origin[k]) * normal[k];
+    else cornerPoints[n][2] = 0.;
+    cornerPoints[n][i] = origin[i] - numerator / normal[i];
+  }
+  std::vector<std::array<unsigned, 3>> elements;
+  if (D == 2) {
+    std::array<unsigned, 3> line = {0, 1, 0};
+    if (normal[i] < 0.) std::swap(line[0], line[1]);
+    elements.push_back(line);
+  } else {
+    std::array<unsigned, 3> triangle = {0, 1, 2};
+    if (normal[i] < 0.) std::swap(triangle[0], triangle[1]);
+    elements.push_back(triangle);
+    triangle = {0, 3, 1};
+    if (normal[i] < 0.) std::swap(triangle[0], triangle[1]);
+    elements.push_back(triangle);
+  }
+  const bool removeAll[3] = {true, true, true};  // FromSurfaceMesh's default
+  fromSurfaceMesh<D>(ls, cornerPoints, elements, removeAll);
+  return 0;
+}
+
+// MakeGeometry::makeBox (lsMakeGeometry.hpp:363-419); ignoreBC[d]:
+// setIgnoreBoundaryConditions of axis d (passed as setRemoveBoundaryTriangles)
+template <int D> void makeBox(Band<D> &ls, const double *minCorner, const double *maxCorner, const bool *ignoreBC) {
+  std::vector<Vec3> corners(1u << D, Vec3{0, 0, 0});
+  for (unsigned i = 0; i < D; ++i) corners[0][i] = minCorner[i];
+  for (unsigned i = 0; i < D; ++i) corners.back()[i] = maxCorner[i];
+  corners[1] = corners[0];
+  corners[1][0] = corners.back()[0];
+  corners[2] = corners[0];
+  corners[2][1] = corners.back()[1];
+  if constexpr (D == 3) {
+    corners[3] = corners.back();
+    corners[3][2] = corners[0][2];
+    corners[4] = corners[0];
+    corners[4][2] = corners.back()[2];
+    corners[5] = corners.back();
+    corners[5][1] = corners[0][1];
+    corners[6] = corners.back();
+    corners[6][0] = corners[0][0];
+  }
+  std::vector<std::array<unsigned, 3>> elements;
+  if (D == 2) {
+    std::array<unsigned, 3> lines[4] = {{0, 2, 0}, {2, 3, 0}, {3, 1, 0}, {1, 0, 0}};
+    for (auto &line : lines) elements.push_back(line);
+  } else {
+    std::array<unsigned, 3> triangles[12] = {{0, 3, 1}, {0, 2, 3}, {0, 1, 5}, {0, 5, 4}, {0, 4, 2}, {4, 6, 2},
+                                             {7, 6, 4}, {7, 4, 5}, {7, 2, 6}, {7, 3, 2}, {1, 3, 5}, {3, 7, 5}};
+    for (auto &t : triangles) elements.push_back(t);
+  }
+  fromSurfaceMesh<D>(ls, corners, elements, ignoreBC);
 }
 
 // ------------------------------------------------------------- schemes ---
@@ -1937,6 +2251,71 @@ template <int D> struct Advector {
 
 using namespace lso;
 
+// ----------------------------------------------------------- judges ---
+
+// CompareVolume::apply (lsCompareVolume.hpp:176-289; no custom increments,
+// no mesh): both level sets expanded to width 3 on copies (:195-220), then
+// every cell of the comparison box is inside (corner average <= 0) or not in
+// each, corners summed in corner order (bit k of the corner number = +1
+// along axis k) and divided by 2^D (:257-268); returns the cells that differ
+// (getCellCount; the mismatch is that times gridDelta^D, :164-166).  The box
+// (:87-110) spans the grid along bounded axes (cells lo..hi-1 for reflective
+// ones, lo..hi for periodic ones, whose last cell wraps) and, along the
+// others, the cells touching the run starts of either level set (the
+// grid-minimum entry excluded) -- ViennaHRLE's run breaks, absent from the
+// reference tree, restated as that bounding box.
+template <int D> uint64_t compareVolumeCells(Band<D> A, Band<D> B) {
+  if (A.width < 3) expand<D>(A, 3);
+  if (B.width < 3) expand<D>(B, 3);
+  int64_t bmin[D], bmax[D];
+  bool any = false;
+  for (const Band<D> *b : {&A, &B})
+    for (size_t i = 1; i < b->idx.size(); ++i)
+      for (int d = 0; d < D; ++d) {
+        const int64_t c = b->idx[i][d];
+        if (!any || c < bmin[d]) bmin[d] = c;
+        if (!any || c > bmax[d]) bmax[d] = c;
+        if (d == D - 1) any = true;
+      }
+  int64_t lo[D], hi[D];  // inclusive cell range per axis
+  for (int d = 0; d < D; ++d) {
+    if (A.g.bc[d] == PERIODIC) {
+      lo[d] = A.g.lo[d];
+      hi[d] = A.g.hi[d];
+    } else if (A.g.bc[d] == REFLECTIVE) {
+      lo[d] = A.g.lo[d];
+      hi[d] = (int64_t)A.g.hi[d] - 1;
+    } else {
+      if (!any) return 0;
+      lo[d] = bmin[d] - 1;
+      hi[d] = bmax[d];
+    }
+  }
+  const int64_t nz = (D == 3) ? hi[2] - lo[2] + 1 : 1;
+  uint64_t count = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : count)
+  for (int64_t kz = 0; kz < nz; ++kz) {
+    Idx<D> c;
+    if (D == 3) c[2] = (int32_t)(lo[2] + kz);
+    for (int64_t y = lo[1]; y <= hi[1]; ++y)
+      for (int64_t x = lo[0]; x <= hi[0]; ++x) {
+        c[0] = (int32_t)x;
+        c[1] = (int32_t)y;
+        double ta = 0., tb = 0.;
+        for (int i = 0; i < (1 << D); ++i) {
+          Idx<D> p;
+          for (int d = 0; d < D; ++d) p[d] = c[d] + ((i >> d) & 1);
+          ta += A.valueMapped(p);
+          tb += B.valueMapped(p);
+        }
+        ta /= (1 << D);
+        tb /= (1 << D);
+        count += ((ta <= 0.) != (tb <= 0.)) ? 1u : 0u;
+      }
+  }
+  return count;
+}
+
 struct lso_domain {
   int D;
   Band<2> b2;
@@ -2014,6 +2393,17 @@ void lso_domain_set(lso_domain *d, int64_t n, const int32_t *idx, const double *
 void lso_domain_from_points(lso_domain *d, int64_t n, const int32_t *idx, const double *val, int width) {
   if (d->D == 2) fromPts(d->b2, n, idx, val, width);
   else fromPts(d->b3, n, idx, val, width);
+}
+
+// MakeGeometry with a Plane / a Box (FromSurfaceMesh); plane: 0 or -2 (more
+// than one infinite axis), -3 (normal parallel to it), -4 (none)
+int lso_make_plane(lso_domain *d, const double *origin, const double *normal) {
+  return d->D == 2 ? makePlane<2>(d->b2, origin, normal) : makePlane<3>(d->b3, origin, normal);
+}
+void lso_make_box(lso_domain *d, const double *minCorner, const double *maxCorner, int ignoreMask) {
+  const bool ig[3] = {(ignoreMask & 1) != 0, (ignoreMask & 2) != 0, (ignoreMask & 4) != 0};
+  if (d->D == 2) makeBox<2>(d->b2, minCorner, maxCorner, ig);
+  else makeBox<3>(d->b3, minCorner, maxCorner, ig);
 }
 
 void lso_make_sphere(lso_domain *d, const double *origin, double radius) {
@@ -2124,6 +2514,12 @@ int lso_check(const lso_domain *d, char *msg, int msglen) {
     msg[msglen - 1] = 0;
   }
   return f;
+}
+
+// CompareVolume: cells whose inside/outside state differs (same grids).
+uint64_t lso_compare_volume(const lso_domain *a, const lso_domain *b) {
+  if (a->D == 2) return compareVolumeCells<2>(a->b2, b->b2);
+  return compareVolumeCells<3>(a->b3, b->b3);
 }
 
 double lso_weno5(const double *x, double dx, int plus) { return weno5(x, dx, plus != 0, 1e-6); }

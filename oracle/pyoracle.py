@@ -83,6 +83,9 @@ def lib():
         L.lso_domain_set.argtypes = [P, ctypes.c_int64, i32p, dp, ctypes.c_int]
         L.lso_domain_from_points.argtypes = [P, ctypes.c_int64, i32p, dp, ctypes.c_int]
         L.lso_make_sphere.argtypes = [P, dp, ctypes.c_double]
+        L.lso_make_plane.argtypes = [P, dp, dp]
+        L.lso_make_plane.restype = ctypes.c_int
+        L.lso_make_box.argtypes = [P, dp, dp, ctypes.c_int]
         L.lso_expand.argtypes = [P, ctypes.c_int]
         L.lso_reduce.argtypes = [P, ctypes.c_int]
         L.lso_prune.argtypes = [P, ctypes.c_int]
@@ -92,6 +95,8 @@ def lib():
         L.lso_combine.argtypes = [P, P, ctypes.c_double, ctypes.c_double]
         L.lso_check.argtypes = [P, ctypes.c_char_p, ctypes.c_int]
         L.lso_check.restype = ctypes.c_int
+        L.lso_compare_volume.argtypes = [P, P]
+        L.lso_compare_volume.restype = ctypes.c_uint64
         L.lso_weno5.argtypes = [dp, ctypes.c_double, ctypes.c_int]
         L.lso_weno5.restype = ctypes.c_double
         L.lso_weno3.argtypes = [dp, ctypes.c_double, ctypes.c_int]
@@ -185,6 +190,20 @@ class Domain:
         a, pa = _i32(idx)
         b, pb = _f64(val)
         lib().lso_domain_from_points(self.h, len(b), pa, pb, int(width))
+
+    def make_plane(self, origin, normal):
+        """MakeGeometry with a Plane (lsMakeGeometry.hpp:245-360 through
+        FromSurfaceMesh); returns 0 or the negative error code."""
+        o, po_ = _f64(list(origin) + [0.0] * (3 - len(origin)))
+        n, pn = _f64(list(normal) + [0.0] * (3 - len(normal)))
+        return lib().lso_make_plane(self.h, po_, pn)
+
+    def make_box(self, min_corner, max_corner, ignore_mask=0):
+        """MakeGeometry with a Box (lsMakeGeometry.hpp:363-419 through
+        FromSurfaceMesh); ignore_mask: setIgnoreBoundaryConditions per axis."""
+        a, pa = _f64(list(min_corner) + [0.0] * (3 - len(min_corner)))
+        b, pb = _f64(list(max_corner) + [0.0] * (3 - len(max_corner)))
+        lib().lso_make_box(self.h, pa, pb, int(ignore_mask))
 
     def make_sphere(self, origin, radius):
         o, po = _f64(list(origin) + [0.0] * (3 - len(origin)))
@@ -424,3 +443,10 @@ def set_threads(n):
 
 def num_threads():
     return lib().lso_num_threads()
+
+
+def compare_volume(target, sample):
+    """CompareVolume (lsCompareVolume.hpp:176-289): (cells that differ,
+    mismatch volume = cells * gridDelta^D)."""
+    n = int(lib().lso_compare_volume(target.h, sample.h))
+    return n, n * target.delta ** target.D

@@ -4,7 +4,10 @@
 // the device algorithm (candidate bitmaps, rank lookups, compaction, index
 // rebuild, schemes) against the CPU oracle and catch out-of-range indexing
 // before any GPU run.  The product library never uses it.
+#ifndef LSG_HOST_ONLY
 #define LSG_HOST_ONLY
+#endif
+#include "../../viennals_amd/csrc/lsg_geometry.hpp"
 #include "../../viennals_amd/csrc/lsg_items.hpp"
 
 #include <algorithm>
@@ -185,7 +188,99 @@ void candidates(const EBand &in, const std::vector<uint64_t> &C, std::vector<uin
   }
 }
 
+// The single-pass band operation of the HIP kernels (kBandOp), tile by
+// tile in order: candidate words of BAND_TILE_WORDS words, the rule at every
+// slot (tileSlot), keep flags against the previous slot, compaction, bitmap
+// words and prefix counts.  Used instead of the candidate-list pipeline
+// when g_tiled.
+constexpr int TW = 32;
+bool g_tiled = true;
+
+enum { MODE_STAR = 0, MODE_UNION = 1, MODE_SELF = 2 };
+
+template <class At, int D>
+EBand tiledOp(const EBand &in, const At &at, int mode, const std::vector<uint64_t> *B = nullptr) {
+  const Frame &f = in.f;
+  const int64_t nw = f.nwords;
+  const uint32_t base = (uint32_t)f.base;
+  EBand o;
+  o.D = in.D;
+  o.g = in.g;
+  o.f = f;
+  o.bits.assign(nw + 1, 0);
+  o.pre.assign(nw + 1, 0);
+  o.val.assign(base, 0.0);
+  o.pos.assign(base, 0xFFFFFFFFu);
+  const uint32_t nTiles = (uint32_t)((nw + TW - 1) / TW);
+  uint32_t P = 0;
+  for (uint32_t t = 0; t < nTiles; ++t) {
+    const uint32_t w0 = t * TW;
+    uint64_t cw[TW], kw[TW];
+    uint32_t cpre[TW + 1], kpre[TW + 1];
+    uint32_t acc = 0;
+    for (int j = 0; j < TW; ++j) {
+      const int64_t w = (int64_t)w0 + j;
+      uint64_t C = 0;
+      if (w < nw) {
+        if (mode == MODE_STAR) C = candStarWord<D>(in.bits.data(), f, in.g, w);
+        else if (mode == MODE_UNION) C = in.bits[w] | (*B)[w];
+        else C = in.bits[w];
+      }
+      cw[j] = C;
+      kw[j] = 0;
+      cpre[j] = acc;
+      acc += (uint32_t)__builtin_popcountll(C);
+    }
+    cpre[TW] = acc;
+    const uint32_t nc = acc;
+    const bool hasPred = t > 0 || base;
+    const uint32_t nslots = (nc > 0 || (t == 0 && base)) ? nc + 1 : 0;
+    std::vector<double> stage;
+    double prev = 0.0;
+    for (uint32_t s = 0; s < nslots; ++s) {
+      uint32_t p;
+      bool eval, cand;
+      int j, b;
+      tileSlot<TW>(f, t, w0, cw, cpre, nc, s, p, eval, cand, j, b);
+      check(!eval || p == 0xFFFFFFFFu || (int64_t)(p >> 6) < nw, "slot position outside the frame");
+      const double v = eval ? at(p) : 0.0;
+      if (s == 0 && t == 0 && base) o.val[0] = v;
+      const bool keep = cand && ((s == 1 && !hasPred) || !isUndef(v) || v != prev);
+      if (keep) {
+        stage.push_back(v);
+        kw[j] |= 1ull << b;
+      }
+      prev = v;
+    }
+    acc = 0;
+    for (int j = 0; j < TW; ++j) {
+      kpre[j] = acc;
+      acc += (uint32_t)__builtin_popcountll(kw[j]);
+    }
+    kpre[TW] = acc;
+    check(acc == stage.size(), "keep words and kept values disagree");
+    for (uint32_t k = 0; k < acc; ++k) {
+      const int jj = wordOfItem<TW>(kpre, k);
+      const int bb = selectBit(kw[jj], (int)(k - kpre[jj]));
+      o.val.push_back(stage[k]);
+      o.pos.push_back(((w0 + (uint32_t)jj) << 6) | (uint32_t)bb);
+    }
+    for (int j = 0; j < TW; ++j) {
+      const int64_t w = (int64_t)w0 + j;
+      if (w < nw) {
+        o.bits[w] = kw[j];
+        o.pre[w] = P + kpre[j];
+      }
+    }
+    P += acc;
+  }
+  o.bits[nw] = 0;
+  o.pre[nw] = P;
+  return o;
+}
+
 template <int D> EBand starOp(const EBand &in, const OpParams &op) {
+  if (g_tiled) return tiledOp<AtStar<D>, D>(in, AtStar<D>{in.view(), op}, MODE_STAR);
   const int64_t nw = in.f.nwords;
   std::vector<uint64_t> C(nw + 1, 0);
   for (int64_t w = 0; w < nw; ++w) C[w] = candStarWord<D>(in.bits.data(), in.f, in.g, w);
@@ -204,6 +299,12 @@ template <int D> EBand starOp(const EBand &in, const OpParams &op) {
 }
 
 template <int D> EBand selfOp(const EBand &in, const OpParams &op, const std::vector<double> *rates, double dt) {
+  if (g_tiled) {
+    OpParams o2 = op;
+    o2.rates = rates ? rates->data() : nullptr;
+    o2.dt = &dt;
+    return tiledOp<AtSelf<D>, D>(in, AtSelf<D>{in.view(), o2}, MODE_SELF);
+  }
   BandView v = in.view();
   std::vector<double> out(in.val.size());
   for (uint32_t i = 0; i < in.val.size(); ++i) {
@@ -229,6 +330,12 @@ EBand booleanOp(const EBand &A, const EBand &B, int kind, double wT, double wS) 
   std::vector<uint32_t> cpos, cpre;
   candidates(A, C, cpos, cpre);
   BandView va = A.view(), vb = B.view();
+  if (g_tiled) {
+    EBand o = (A.D == 2) ? tiledOp<AtBool, 2>(A, AtBool{va, vb, kind, wT, wS, nullptr}, MODE_UNION, &B.bits)
+                         : tiledOp<AtBool, 3>(A, AtBool{va, vb, kind, wT, wS, nullptr}, MODE_UNION, &B.bits);
+    o.width = A.width;
+    return o;
+  }
   std::vector<double> out(cpos.size());
   for (uint32_t i = 0; i < cpos.size(); ++i) {
     int32_t c[3];
@@ -575,6 +682,45 @@ int emu_advect(emu_domain *d, const emu_params *p, double *out) {
   out[3] = a.pointUpdates;
   return 0;
 }
+
+// The library's MakeGeometry plane (kind 0: a = origin, b = normal) or box
+// (kind 1: a, b = corners; mask: setIgnoreBoundaryConditions) on a grid
+// built like lsg_domain_create; returns the number of defined points written
+// to idx / val (at most cap), or a negative error code.
+int64_t emu_geometry(int kind, int D, const int32_t *gmin, const int32_t *gmax, const int *bc, double dx,
+                     const double *a, const double *b, int mask, int32_t *idx, double *val, int64_t cap) {
+  GridInfo g{};
+  g.D = D;
+  g.dx = dx;
+  for (int k = 0; k < 3; ++k) g.bc[k] = REFLECTIVE;
+  for (int k = 0; k < D; ++k) {
+    g.bc[k] = bc[k];
+    switch (bc[k]) {
+    case INFINITE: g.lo[k] = -INF_IDX; g.hi[k] = INF_IDX; break;
+    case PERIODIC: g.lo[k] = gmin[k]; g.hi[k] = gmax[k] - 1; break;
+    case POS_INFINITE: g.lo[k] = gmin[k]; g.hi[k] = INF_IDX; break;
+    case NEG_INFINITE: g.lo[k] = -INF_IDX; g.hi[k] = gmax[k]; break;
+    default: g.lo[k] = gmin[k]; g.hi[k] = gmax[k]; break;
+    }
+  }
+  std::vector<int32_t> pi;
+  std::vector<double> pv;
+  if (kind == 0) {
+    const int e = lsg::planeBand(D, g, a, b, pi, pv);
+    if (e) return e;
+  } else {
+    lsg::boxBand(D, g, a, b, pi, pv, mask);
+  }
+  const int64_t n = (int64_t)pv.size();
+  for (int64_t i = 0; i < n && i < cap; ++i) {
+    val[i] = pv[i];
+    for (int k = 0; k < D; ++k) idx[i * D + k] = pi[i * D + k];
+  }
+  return n;
+}
+
+// 1: band operations through the single-pass tile pipeline; 0: candidate lists.
+void emu_set_tiled(int on) { g_tiled = on != 0; }
 
 // Bit helpers of the index kernels on the host.
 void emu_deposit(const uint64_t *x, const uint64_t *m, uint64_t *out, int64_t n) {

@@ -40,6 +40,10 @@ def lib():
         u64p = ctypes.POINTER(ctypes.c_uint64)
         L.emu_deposit.argtypes = [u64p, u64p, u64p, ctypes.c_int64]
         L.emu_select.argtypes = [u64p, i32p, i32p, ctypes.c_int64]
+        L.emu_set_tiled.argtypes = [ctypes.c_int]
+        L.emu_geometry.restype = ctypes.c_int64
+        L.emu_geometry.argtypes = [ctypes.c_int, ctypes.c_int, i32p, i32p, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.c_double, dp, dp, ctypes.c_int, i32p, dp, ctypes.c_int64]
         _lib = L
     return _lib
 
@@ -126,3 +130,30 @@ def select_bit(w, k):
     lib().emu_select(w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), k.ctypes.data_as(i32p),
                      out.ctypes.data_as(i32p), len(w))
     return out
+
+
+def set_tiled(on):
+    """Band operations through the single-pass tile pipeline (True) or the
+    candidate-list pipeline (False)."""
+    lib().emu_set_tiled(int(bool(on)))
+
+
+def geometry(kind, D, gmin, gmax, bc, dx, a, b, mask=0):
+    """The library's MakeGeometry plane (kind 0) or box (kind 1) on the
+    host: (idx, val) of the defined points, or the negative error code."""
+    g0 = np.ascontiguousarray(gmin, dtype=np.int32)
+    g1 = np.ascontiguousarray(gmax, dtype=np.int32)
+    c = (ctypes.c_int * D)(*bc)
+    pa = np.ascontiguousarray(list(a) + [0.0] * (3 - len(a)), dtype=np.float64)
+    pb = np.ascontiguousarray(list(b) + [0.0] * (3 - len(b)), dtype=np.float64)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    args = (int(kind), D, g0.ctypes.data_as(ip), g1.ctypes.data_as(ip), c, float(dx), pa.ctypes.data_as(dp),
+            pb.ctypes.data_as(dp), int(mask))
+    n = lib().emu_geometry(*args, None, None, 0)
+    if n < 0:
+        return n
+    idx = np.zeros((n, D), dtype=np.int32)
+    val = np.zeros(n, dtype=np.float64)
+    lib().emu_geometry(*args, idx.ctypes.data_as(ip), val.ctypes.data_as(dp), n)
+    return idx, val

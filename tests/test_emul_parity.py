@@ -10,6 +10,15 @@ import pyemul as pe
 import pyoracle as po
 
 
+@pytest.fixture(autouse=True, params=["tiled", "lists"])
+def pipeline(request):
+    """Every case through both band-operation pipelines of the device code:
+    the single-pass tiles (one level set) and the candidate lists (batches)."""
+    pe.set_tiled(request.param == "tiled")
+    yield request.param
+    pe.set_tiled(True)
+
+
 def to_emul(o):
     e = pe.Domain(o.D, o.gmin, o.gmax, o.bc, o.delta)
     e.set(*o.get(), o.width)

@@ -75,3 +75,33 @@ def test_reference_sllf_test_judged_on_the_gpu():
     cc = vls.CompareChamfer(ref, s)
     cc.apply()
     assert cc.getChamferDistance() < 0.035
+
+
+def test_chamfer_expands_narrow_level_sets_to_width_two():
+    """lsCompareChamfer.hpp:118-134: a level set narrower than 2 is expanded
+    (on a copy) before its surface is extracted, so a Reduce(1) input gives
+    the distances of its width-2 expansion and stays unchanged itself."""
+    import viennals_amd as vls
+    say("CompareChamfer of Reduce(1) inputs")
+    a = cases.sphere(r=5.0, dx=0.5, origin=[0.1, 0.0, 0.2])
+    b = cases.sphere(r=5.4, dx=0.5, origin=[0.1, 0.0, 0.2])
+    ga, gb = to_gpu(a), to_gpu(b)
+    vls.Reduce(ga, 1).apply()
+    vls.Reduce(gb, 1).apply()
+    ia, va = ga.get_runs()
+    wa, wb = to_gpu(a), to_gpu(b)  # width-2 copies of the same surfaces
+    for g in (wa, wb):
+        vls.Reduce(g, 1).apply()
+        vls.Expand(g, 2).apply()
+    narrow = vls.CompareChamfer(ga, gb)
+    narrow.apply()
+    wide = vls.CompareChamfer(wa, wb)
+    wide.apply()
+    assert narrow.getNumTargetPoints() == wide.getNumTargetPoints() > 0
+    assert narrow.getNumSamplePoints() == wide.getNumSamplePoints() > 0
+    for f in ("getForwardDistance", "getBackwardDistance", "getChamferDistance", "getRMSChamferDistance",
+              "getMaxDistance"):
+        assert getattr(narrow, f)() == getattr(wide, f)(), f
+    ia2, va2 = ga.get_runs()
+    assert np.array_equal(ia, ia2) and np.array_equal(va.view(np.int64), va2.view(np.int64))
+    assert ga.getLevelSetWidth() == 1

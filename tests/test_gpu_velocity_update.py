@@ -124,3 +124,26 @@ def test_refused_misuse():
     b.setVelocityUpdateCallback(None)
     b.apply()
     assert b.getAdvectedTime() == pytest.approx(0.2)
+
+
+def test_exception_in_callback_stops_apply():
+    """An exception raised by the callback ends apply() at that stage (as
+    pybind11 carries it out of the reference's apply), so the callback is not
+    called again by the remaining stages and steps."""
+    import viennals_amd as vls
+    say("velocity update callback: an exception stops apply at its stage")
+    g = to_gpu(cases.sphere(r=4.0, dx=0.5))
+    a = vls.Advect(g, vls.VelocityField(b=1.0))
+    a.setTemporalScheme(2)
+    a.setAdvectionTime(100.0)
+    calls = []
+
+    def cb(d):
+        calls.append(d.getNumberOfPoints())
+        if len(calls) == 3:
+            raise ValueError("stop here")
+
+    a.setVelocityUpdateCallback(cb)
+    with pytest.raises(ValueError, match="stop here"):
+        a.apply()
+    assert len(calls) == 3  # RK3: two calls in step 1, the first call of step 2

@@ -303,8 +303,9 @@ class MakeGeometry:
 
     def setIgnoreBoundaryConditions(self, ignore):
         """One flag for every dimension or a list of flags
-        (lsMakeGeometry.hpp:125-141); spheres only: a periodic dimension
-        whose flag is set keeps its indices unmapped (:227-235)."""
+        (lsMakeGeometry.hpp:125-141): a periodic dimension whose flag is set
+        keeps a sphere's indices unmapped (:227-235) and drops a box's mesh
+        elements on its boundary (:415-417); planes ignore it."""
         flags = list(ignore) if hasattr(ignore, "__len__") else [bool(ignore)] * 3
         self.ignoreMask = sum(1 << i for i, f in enumerate(flags[:3]) if f)
 
@@ -314,8 +315,6 @@ class MakeGeometry:
             raise LsgError("No level set was passed to MakeGeometry.")
         pad = lambda v: list(v) + [0.0] * (3 - len(v))
         mask = getattr(self, "ignoreMask", 0)
-        if mask and not isinstance(g, Sphere):
-            raise LsgError("MakeGeometry.setIgnoreBoundaryConditions is supported for spheres only")
         if isinstance(g, Sphere):
             a, pa = _f64(pad(g.origin))
             check(lib().lsg_make_sphere_ignore_bc(d._h, pa, g.radius, int(mask)))
@@ -326,7 +325,7 @@ class MakeGeometry:
         elif isinstance(g, Box):
             a, pa = _f64(pad(g.minPoint))
             b, pb = _f64(pad(g.maxPoint))
-            check(lib().lsg_make_box(d._h, pa, pb))
+            check(lib().lsg_make_box_ignore_bc(d._h, pa, pb, int(mask)))
         else:
             raise LsgError("Invalid geometry type was specified for MakeGeometry.")
 
@@ -452,6 +451,42 @@ class CompareChamfer:
 
     def getNumSamplePoints(self):
         return self._n[1]
+
+
+class CompareVolume:
+    """CompareVolume (lsCompareVolume.hpp:176-289) on the GPU: the cells whose
+    corner average is inside (<= 0) in exactly one of the two level sets
+    (narrower level sets are expanded to width 3 on copies); both must share
+    their grid.  CompareArea is the same class in 2D."""
+
+    def __init__(self, target=None, sample=None):
+        self.target, self.sample = target, sample
+        self._cells = 0
+        self._volume = float("nan")
+
+    def setLevelSetTarget(self, d):
+        self.target = d
+
+    def setLevelSetSample(self, d):
+        self.sample = d
+
+    def apply(self):
+        c = ctypes.c_uint64()
+        v = ctypes.c_double()
+        check(lib().lsg_compare_volume(self.target._h, self.sample._h, ctypes.byref(c), ctypes.byref(v)))
+        self._cells, self._volume = int(c.value), float(v.value)
+
+    def getVolumeMismatch(self):
+        return self._volume
+
+    def getAreaMismatch(self):
+        return self._volume
+
+    def getCellCount(self):
+        return self._cells
+
+
+CompareArea = CompareVolume
 
 
 class BooleanOperation:
@@ -742,8 +777,10 @@ class Advect:
         level set (lsAdvect.hpp:966-969, lsAdvectIntegrationSchemes.hpp:87-89,
         128-130,148-150); its return value is ignored.  It may read level sets
         and change the velocity field (setVelocityField or a host
-        VelocityField's state); an exception raised in it is raised again by
-        apply().  None removes it."""
+        VelocityField's state); an exception raised in it stops apply() at
+        that stage (the level set keeps the stages done) and is raised again
+        by apply(), as pybind11 carries it out of the reference's apply.
+        None removes it."""
         from ._lib import VELOCITY_UPDATE_FN
         if callback is None:
             self._updateCallback = None
@@ -756,8 +793,9 @@ class Advect:
             me = ref()
             try:
                 callback(me._domains[-1])
-            except BaseException as e:  # surfaced by apply()
+            except BaseException as e:  # raised again by apply(), which stops at this stage
                 me._updateError = e
+                return -1
             return 0
 
         self._updateCallback = VELOCITY_UPDATE_FN(trampoline)

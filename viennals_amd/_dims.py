@@ -3,7 +3,8 @@ the drop-in counterparts of the reference's Python modules viennals.d2 and
 viennals.d3 (python/viennals/d2.pyi, d3.pyi) for the classes of the
 advection path: Domain, Advect, MakeGeometry, Sphere, Plane, Box,
 BooleanOperation, Expand, Reduce, Prune, VelocityField and the enums."""
-from . import (Advect, BooleanOperation, BooleanOperationEnum, BoundaryConditionEnum, Box, CompareChamfer, Expand,
+from . import (Advect, BooleanOperation, BooleanOperationEnum, BoundaryConditionEnum, Box, CompareArea, CompareChamfer,
+               CompareVolume, Expand,
                FinalizeStencilLocalLaxFriedrichs, HostVelocityField, LsgError, MakeGeometry, MarkVoidPoints,
                MaterialVelocityField, Plane, PointData, PrepareStencilLocalLaxFriedrichs, Prune, Reduce, Sphere,
                SpatialSchemeEnum, TemporalSchemeEnum, VelocityField)

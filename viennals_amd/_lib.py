@@ -20,7 +20,7 @@ EXPORTED = [
     "lsg_last_error", "lsg_domain_create", "lsg_domain_destroy", "lsg_domain_deep_copy",
     "lsg_domain_insert_points", "lsg_domain_set_runs", "lsg_domain_num_runs", "lsg_domain_get_runs",
     "lsg_domain_num_points", "lsg_domain_width", "lsg_domain_set_width", "lsg_make_sphere", "lsg_make_plane", "lsg_make_box",
-    "lsg_make_sphere_ignore_bc",
+    "lsg_make_sphere_ignore_bc", "lsg_make_box_ignore_bc",
     "lsg_expand", "lsg_reduce", "lsg_prune", "lsg_boolean", "lsg_rebuild", "lsg_advect_create",
     "lsg_advect_destroy", "lsg_advect_insert_next_level_set", "lsg_advect_clear_level_sets",
     "lsg_advect_set_velocity_builtin", "lsg_advect_set_velocity_builtin_material",
@@ -42,6 +42,7 @@ EXPORTED = [
     "lsg_advect_set_save_advection_velocities", "lsg_domain_point_data_insert", "lsg_domain_point_data_count",
     "lsg_domain_point_data_label", "lsg_domain_point_data_size", "lsg_domain_point_data_get",
     "lsg_domain_point_data_clear", "lsg_mark_void_points", "lsg_surface_points", "lsg_compare_chamfer",
+    "lsg_compare_volume",
 ]
 
 
@@ -78,6 +79,7 @@ def lib():
         "lsg_make_sphere_ignore_bc": ([P, dp, c_double, c_int], c_int),
         "lsg_make_plane": ([P, dp, dp], c_int),
         "lsg_make_box": ([P, dp, dp], c_int),
+        "lsg_make_box_ignore_bc": ([P, dp, dp, c_int], c_int),
         "lsg_expand": ([P, c_int], c_int),
         "lsg_reduce": ([P, c_int], c_int),
         "lsg_prune": ([P, c_int], c_int),
@@ -112,6 +114,7 @@ def lib():
         "lsg_mark_void_points": ([P], c_int),
         "lsg_surface_points": ([P, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)], c_int),
         "lsg_compare_chamfer": ([P, P, ctypes.c_void_p, ctypes.c_void_p], c_int),
+        "lsg_compare_volume": ([P, P, ctypes.c_void_p, ctypes.c_void_p], c_int),
         "lsg_advect_prepare_ls": ([P], c_int),
         "lsg_advect_apply": ([P], c_int),
         "lsg_advect_get_advected_time": ([P, dp], c_int),

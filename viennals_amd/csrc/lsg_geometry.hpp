@@ -12,5 +12,5 @@ void sphereBand(int D, const GridInfo &g, const double *origin, double radius, s
 int planeBand(int D, const GridInfo &g, const double *origin, const double *normal, std::vector<int32_t> &idx,
               std::vector<double> &val);
 void boxBand(int D, const GridInfo &g, const double *mn, const double *mx, std::vector<int32_t> &idx,
-             std::vector<double> &val);
+             std::vector<double> &val, int ignoreBcMask = 0);
 }  // namespace lsg

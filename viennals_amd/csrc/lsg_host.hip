@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <set>
 #include <string>
 #include <vector>
@@ -105,6 +106,7 @@ struct WsStore {
     (void)hipFree(w.partials);
     (void)hipFree(w.ncand);
     (void)hipFree(w.cseg);
+    (void)hipFree(w.tileStat);
     w = Workspace{};
     nwAlloc = 0;
   }
@@ -122,6 +124,10 @@ struct WsStore {
     LSG_HIP(hipMalloc(&w.partials, (SCAN_BLOCKS_MAX + 1) * sizeof(uint32_t)));
     LSG_HIP(hipMalloc(&w.ncand, sizeof(uint32_t)));
     LSG_HIP(hipMalloc(&w.cseg, ccap * sizeof(uint16_t)));
+    // tile status words start as "not ready" (launch tag 0 is never used)
+    const size_t ntile = (size_t)(nw / BAND_TILE_WORDS) + 2;
+    LSG_HIP(hipMalloc(&w.tileStat, ntile * sizeof(uint64_t)));
+    LSG_HIP(hipMemset(w.tileStat, 0, ntile * sizeof(uint64_t)));
     w.ccap = ccap;
     nwAlloc = nw;
     return 0;
@@ -685,6 +691,21 @@ static void doBoolean(lsg_domain *a, const lsg_domain *b, int kind, double wT, d
   doPrune(a, false);
 }
 
+// A look-back wait of a single-pass band operation ran out (not expected:
+// it needs workgroups dispatched out of index order).  The single-pass
+// kernels are turned off for the rest of the process and the flag cleared;
+// the caller repeats its work from a saved copy with the multi-kernel
+// pipeline.
+static bool lookbackFailed(const DevScalars &h) {
+  if (!h.lookbackFail) return false;
+  disableSinglePassBandOps();
+  fprintf(stderr, "[viennals_gpu] a single-pass band operation did not complete; repeating with the multi-kernel "
+                  "band operations\n");
+  (void)hipMemsetAsync(&g_ctx.sc->lookbackFail, 0, sizeof(uint32_t), g_ctx.s);
+  (void)hipStreamSynchronize(g_ctx.s);
+  return true;
+}
+
 static int readOverflow(bool reset) {
   uint32_t ov = 0;
   LSG_HIP(hipMemcpyAsync(&ov, &g_ctx.sc->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
@@ -1052,8 +1073,12 @@ static int velocityUpdate(lsg_advect *a, lsg_domain *top) {
   const Frame f = top->f;
   const int width = top->width;
   g_inVelocityUpdate = true;
-  (void)a->velUpdate(top, a->velUpdateUser);  // the reference ignores the returned bool
+  // the reference ignores the returned bool (0 or 1); a negative value is a
+  // callback that failed (an exception in a language binding), which ends
+  // the apply here as the exception ends the reference's
+  const int r = a->velUpdate(top, a->velUpdateUser);
   g_inVelocityUpdate = false;
+  if (r < 0) return fail(-46, "the velocity update callback failed; apply stopped at this stage");
   if (std::memcmp(&b, &top->cur.b, sizeof(BandBufs)) || std::memcmp(&f, &top->f, sizeof(Frame)) ||
       width != top->width)
     return fail(-1, "the velocity update callback must not modify the advected level set");
@@ -1223,6 +1248,13 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
     LSG_HIP(hipMemcpyAsync(a->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
     LSG_HIP(hipStreamSynchronize(g_ctx.s));
     const DevScalars &h = *a->hostSc;
+    if (lookbackFailed(h)) {  // repeat the step from the saved band
+      a->dropGraphs();
+      if (int e = copyDomain(d, a->save)) return e;
+      d->width = widthBefore;
+      LSG_HIP(hipStreamSynchronize(g_ctx.s));
+      continue;
+    }
     if (!h.overflow) {
       out.stepTime = h.stepTime;
       out.dt1 = h.dt[0];
@@ -1409,6 +1441,14 @@ static int runStepMM(lsg_advect *a, double maxTimeStep, StepOut &out) {
     LSG_HIP(hipMemcpyAsync(a->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
     LSG_HIP(hipStreamSynchronize(g_ctx.s));
     const DevScalars &h = *a->hostSc;
+    if (lookbackFailed(h)) {  // repeat the step from the saved bands
+      if (int e = copyDomain(top, a->save)) return e;
+      for (size_t i = 0; i + 1 < L; ++i)
+        if (int e = copyDomain(a->levelSets[i], a->lowerSave[i])) return e;
+      for (size_t i = 0; i < L; ++i) a->levelSets[i]->width = widths[i];
+      LSG_HIP(hipStreamSynchronize(g_ctx.s));
+      continue;
+    }
     if (!h.overflow) {
       out.stepTime = h.stepTime;
       out.dt1 = h.dt[0];
@@ -1912,6 +1952,13 @@ static int batchStepOnce(lsg_batch *b) {
     LSG_HIP(hipMemcpyAsync(b->hostSc, g_ctx.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, g_ctx.s));
     LSG_HIP(hipStreamSynchronize(g_ctx.s));
     const DevScalars &hs = *b->hostSc;
+    if (lookbackFailed(hs)) {  // repeat the step from the saved band
+      b->dropGraphs();
+      launchCopyBandWords(g_ctx.s, b->save.b, b->cur.b, b->W);
+      b->width = widthBefore;
+      LSG_HIP(hipStreamSynchronize(g_ctx.s));
+      continue;
+    }
     if (!hs.overflow) {
       b->pointUpdates += (double)(hs.totalActive - hs.stepActive0);
       ++b->steps;
@@ -2185,14 +2232,36 @@ static int nearestAll(lsg_domain *from, const SurfBuf &sf, lsg_domain *to, const
   if (e != hipSuccess) return fail(-2, hipGetErrorString(e));
   return 0;
 }
+static int afterOp(lsg_domain *d);
+
+// The level set a judge reads: a copy expanded to minWidth when the level
+// set is narrower, else the level set itself (CompareChamfer: width 2, a zero
+// crossing needs both edge end points defined, lsCompareChamfer.hpp:118-134;
+// CompareVolume: width 3, lsCompareVolume.hpp:195-220).
+static int judgeInput(lsg_domain *d, int minWidth, std::unique_ptr<lsg_domain> &copy, lsg_domain *&use) {
+  use = d;
+  if (d->width >= minWidth) return 0;
+  copy.reset(new lsg_domain());
+  if (int e = copyDomain(copy.get(), d)) return e;
+  if (int e = ensureStores(copy.get(), copy->cur.b.cap)) return e;
+  doExpand(copy.get(), minWidth);
+  if (int e = afterOp(copy.get())) return e;
+  use = copy.get();
+  return 0;
+}
+
 // CompareChamfer::apply (lsCompareChamfer.hpp:104-245); out = forward,
 // backward, Chamfer, RMS and max distance; counts = target, sample points.
 // Sums run on the host in point order, as the reference's loops do.
-int lsg_compare_chamfer(lsg_domain *target, lsg_domain *sample, double *out, int64_t *counts) {
+int lsg_compare_chamfer(lsg_domain *target0, lsg_domain *sample0, double *out, int64_t *counts) {
   if (int e = ctxInit()) return e;
-  if (target->D != sample->D) return fail(-1, "CompareChamfer: dimensions differ");
-  if (!target->framed || !sample->framed) return fail(-7, "CompareChamfer: level set has no data");
-  if (target->g.dx != sample->g.dx) return fail(-1, "CompareChamfer: grid deltas differ");
+  if (target0->D != sample0->D) return fail(-1, "CompareChamfer: dimensions differ");
+  if (!target0->framed || !sample0->framed) return fail(-7, "CompareChamfer: level set has no data");
+  if (target0->g.dx != sample0->g.dx) return fail(-1, "CompareChamfer: grid deltas differ");
+  std::unique_ptr<lsg_domain> tcopy, scopy;
+  lsg_domain *target = nullptr, *sample = nullptr;
+  if (int e = judgeInput(target0, 2, tcopy, target)) return e;
+  if (int e = judgeInput(sample0, 2, scopy, sample)) return e;
   SurfBuf st, ss;
   int rc = surfaceOf(target, st);
   if (!rc) rc = surfaceOf(sample, ss);
@@ -2224,6 +2293,76 @@ int lsg_compare_chamfer(lsg_domain *target, lsg_domain *sample, double *out, int
   out[2] = (sf + sb) / (double)(fwd.size() + bwd.size());
   out[3] = std::sqrt((sf2 + sb2) / (double)(fwd.size() + bwd.size()));
   out[4] = std::max(mf, mb);
+  return 0;
+}
+
+// CompareVolume::apply (lsCompareVolume.hpp:176-289) without custom
+// increments or mesh output.  The box spans the grid along bounded axes
+// (reflective: cells lo..hi-1; periodic: lo..hi, the last cell wraps) and,
+// along the others, the cells touching a run start of either level set (the
+// grid-minimum entry excluded), which restates ViennaHRLE's run breaks.
+int lsg_compare_volume(lsg_domain *target0, lsg_domain *sample0, uint64_t *cells, double *volume) {
+  if (int e = ctxInit()) return e;
+  if (target0->D != sample0->D) return fail(-1, "CompareVolume: dimensions differ");
+  if (!target0->framed || !sample0->framed) return fail(-7, "CompareVolume: level set has no data");
+  if (target0->g.dx != sample0->g.dx)
+    return fail(-1, "CompareVolume: grid delta mismatch; the grid deltas of the two level sets must be equal");
+  for (int d = 0; d < target0->D; ++d)
+    if (target0->g.lo[d] != sample0->g.lo[d] || target0->g.hi[d] != sample0->g.hi[d] ||
+        target0->g.bc[d] != sample0->g.bc[d])
+      return fail(-1, "CompareVolume: the level sets must share their grid bounds and boundary conditions");
+  std::unique_ptr<lsg_domain> tcopy, scopy;
+  lsg_domain *A = nullptr, *B = nullptr;
+  if (int e = judgeInput(target0, 3, tcopy, A)) return e;
+  if (int e = judgeInput(sample0, 3, scopy, B)) return e;
+  const int D = A->D;
+  int64_t bmin[3], bmax[3];
+  bool any = false;
+  for (lsg_domain *d : {A, B}) {
+    launchBandStats(g_ctx.s, d->cur.b, d->f, g_ctx.stat, (int32_t *)(g_ctx.stat + 1), 1u);
+    int32_t h[7];
+    LSG_HIP(hipMemcpyAsync(h, g_ctx.stat, 7 * sizeof(int32_t), hipMemcpyDeviceToHost, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    if (h[1] > h[4]) continue;  // no in-frame entry after entry 0
+    for (int k = 0; k < D; ++k) {
+      const int64_t a = (int64_t)h[1 + k] + d->f.o[k], b = (int64_t)h[4 + k] + d->f.o[k];
+      bmin[k] = any ? std::min(bmin[k], a) : a;
+      bmax[k] = any ? std::max(bmax[k], b) : b;
+    }
+    any = true;
+  }
+  int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  bool empty = false;
+  for (int k = 0; k < D; ++k) {
+    if (A->g.bc[k] == PERIODIC) {
+      lo[k] = A->g.lo[k];
+      hi[k] = A->g.hi[k];
+    } else if (A->g.bc[k] == REFLECTIVE) {
+      lo[k] = A->g.lo[k];
+      hi[k] = (int64_t)A->g.hi[k] - 1;
+    } else if (any) {
+      lo[k] = bmin[k] - 1;
+      hi[k] = bmax[k];
+    } else {
+      empty = true;
+    }
+    if (hi[k] < lo[k]) empty = true;
+  }
+  unsigned long long n = 0;
+  if (!empty) {
+    unsigned long long *dn = nullptr;
+    LSG_HIP(hipMalloc(&dn, sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(dn, 0, sizeof(unsigned long long), g_ctx.s);
+    if (e == hipSuccess) {
+      launchCompareVolume(g_ctx.s, D, A->cur.b, A->f, B->cur.b, B->f, A->g, lo, hi, dn);
+      e = hipMemcpyAsync(&n, dn, sizeof(n), hipMemcpyDeviceToHost, g_ctx.s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g_ctx.s);
+    (void)hipFree(dn);
+    if (e != hipSuccess) return fail(-100, std::string("HIP error: ") + hipGetErrorString(e));
+  }
+  if (cells) *cells = n;
+  if (volume) *volume = (double)n * std::pow(A->g.dx, D);
   return 0;
 }
 
@@ -2286,16 +2425,18 @@ int lsg_make_plane(lsg_domain *d, const double *origin, const double *normal) {
   if (e == -1) return fail(-1, "MakeGeometry: plane normal is zero");
   if (e == -2) return fail(-1, "Planes can only be created with one Infinite Boundary Condition. More than one found!");
   if (e == -3) return fail(-1, "MakeGeometry: Plane cannot be parallel to Infinite Boundary direction!");
+  if (e == -4) return fail(-1, "Planes require exactly one Infinite Boundary Condition. None found!");
   std::vector<int32_t> ri;
   std::vector<double> rv;
   runsFromPoints(d, pi, pv, ri, rv);
   return uploadRuns(d, ri, rv, 2);
 }
 
-int lsg_make_box(lsg_domain *d, const double *mn, const double *mx) {
+int lsg_make_box(lsg_domain *d, const double *mn, const double *mx) { return lsg_make_box_ignore_bc(d, mn, mx, 0); }
+int lsg_make_box_ignore_bc(lsg_domain *d, const double *mn, const double *mx, int ignore_mask) {
   std::vector<int32_t> pi;
   std::vector<double> pv;
-  boxBand(d->D, d->g, mn, mx, pi, pv);
+  boxBand(d->D, d->g, mn, mx, pi, pv, ignore_mask);
   std::vector<int32_t> ri;
   std::vector<double> rv;
   runsFromPoints(d, pi, pv, ri, rv);
@@ -2303,7 +2444,16 @@ int lsg_make_box(lsg_domain *d, const double *mn, const double *mx) {
 }
 
 static int afterOp(lsg_domain *d) {
+  uint32_t lb = 0;
+  LSG_HIP(hipMemcpyAsync(&lb, &g_ctx.sc->lookbackFail, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
   if (readOverflow(true)) return fail(-40, "band capacity overflow");
+  if (lb) {
+    DevScalars h{};
+    h.lookbackFail = lb;
+    (void)lookbackFailed(h);
+    return fail(-47, "a single-pass band operation did not complete; the level set is invalid (the multi-kernel "
+                     "band operations are used from now on)");
+  }
   return maybeReframe(d, false);
 }
 

@@ -486,6 +486,120 @@ LSG_HD inline int selectBit(uint64_t w, int k) {
   return pos;
 }
 
+// Frame position of the grid point before in-frame position p (p > 0) in
+// HRLE order: the previous bit, or the last x of the previous row when p
+// starts a row (rows are padded to whole words, the padding never holds a
+// point).
+LSG_HD inline uint32_t realPred(const Frame &f, uint32_t p) {
+  const uint32_t w = p >> 6;
+  if ((p & 63u) != 0u) return p - 1u;
+  const uint32_t row = f.divW.div(w);
+  if (w != row * (uint32_t)f.wpr) return p - 1u;  // bit 63 of the previous word of the row
+  return ((w - 1u) << 6) | (uint32_t)((f.ext[0] - 1) & 63);
+}
+
+// Largest j < N with pre[j] <= c for a nondecreasing pre with pre[0] <= c:
+// the word of a tile that holds item c of the tile (N a power of two).
+template <int N> LSG_HD inline int wordOfItem(const uint32_t *pre, uint32_t c) {
+  int lo = 0;
+  LSG_UNROLL
+  for (int step = N / 2; step >= 1; step >>= 1)
+    if (pre[lo + step] <= c) lo += step;
+  return lo;
+}
+
+// Slot s of tile t (words w0.. of the frame, candidate words cw with
+// exclusive prefix counts cpre) of a single-pass band operation: slot 0 is
+// the grid point before the tile's first candidate (for tile 0 the
+// grid-minimum entry outside the frame, when there is one), slot s >= 1 is
+// candidate s - 1.  Sets the frame position p (POS_NONE for the grid-minimum
+// entry), whether the rule is evaluated there and whether it is a candidate
+// (word j, bit b of the tile).
+template <int N>
+LSG_HD inline void tileSlot(const Frame &f, uint32_t t, uint32_t w0, const uint64_t *cw, const uint32_t *cpre,
+                            uint32_t nc, uint32_t s, uint32_t &p, bool &eval, bool &cand, int &j, int &b) {
+  p = POS_NONE;
+  eval = cand = false;
+  j = b = 0;
+  if (s == 0) {
+    if (t > 0 || f.base) {
+      eval = true;
+      if (t > 0) {
+        const int j0 = wordOfItem<N>(cpre, 0);
+        p = realPred(f, ((w0 + (uint32_t)j0) << 6) | (uint32_t)__builtin_ctzll(cw[j0]));
+      }
+    }
+  } else if (s <= nc) {
+    const uint32_t c = s - 1;
+    j = wordOfItem<N>(cpre, c);
+    b = selectBit(cw[j], (int)(c - cpre[j]));
+    p = ((w0 + (uint32_t)j) << 6) | (uint32_t)b;
+    eval = cand = true;
+  }
+}
+
+// Rules of the band operations evaluated at a frame position p (POS_NONE:
+// the grid-minimum entry outside the frame).  Evaluated at a point that is
+// not a candidate of the operation they return the value of the result's
+// run holding that point, which is what the keep flag of the next candidate
+// compares against: such a point's boundary-mapped star holds no run start,
+// so the rule sees one undefined value everywhere and returns it (star
+// rules), and a self rule evaluated "at or before" p reads the entry whose
+// run holds p.
+template <int D> struct AtStar {
+  BandView in;
+  OpParams op;
+  LSG_HD double operator()(uint32_t p) const {
+    int32_t c[3];
+    candCoords(in, 0, p, c);
+    double center, nb[2 * D];
+    gatherStar<D>(in, c, center, nb);
+    return (op.kind == OP_EXPAND) ? ruleExpand<D>(center, nb, op.a, op.b) : ruleRebuild<D>(center, nb, op.a);
+  }
+};
+
+template <int D> struct AtSelf {
+  BandView in;
+  OpParams op;
+  LSG_HD double operator()(uint32_t p) const {
+    // the grid-minimum entry precedes the band's first in-frame entry (in a
+    // batch view: the level set's own, after the entries of earlier sets)
+    uint32_t i = in.rb + in.pre[0] - 1;
+    if (p != POS_NONE) {
+      const BandView::Loc l = locAt((int64_t)p);
+      i = in.entryOf(l, in.bits[l.w], in.pre[l.w]);
+    }
+    const double c = in.val[i];
+    if (op.kind == OP_REDUCE) return ruleReduce(c, op.a);
+    if (op.kind == OP_UPDATE) return ruleUpdate(c, *op.dt, op.rates[i]);
+    if (op.kind == OP_UPDATE_MM) return ruleUpdateMM(c, *op.dt, op.rates + (size_t)i * 2 * op.nmat);
+    int32_t cc[3];
+    candCoords(in, i, in.pos[i], cc);
+    double center, nb[2 * D];
+    gatherStar<D>(in, cc, center, nb);
+    return rulePrune<D>(c, nb, op.kind == OP_PRUNE_RSZ);
+  }
+};
+
+struct AtBool {
+  BandView A, B;
+  int kind;
+  double wT, wS;
+  uint32_t *movedDown;  // combine only: set where the result exceeds b (both defined)
+  LSG_HD double operator()(uint32_t p) const {
+    int32_t c[3];
+    candCoords(A, 0, p, c);
+    BandView::Loc l[2] = {A.locate(c[0], c[1], c[2]), B.locate(c[0], c[1], c[2])};
+    const uint64_t wa = A.bits[l[0].w], wb = B.bits[l[1].w];
+    const uint32_t pa = A.pre[l[0].w], pb = B.pre[l[1].w];
+    const double a = A.val[A.entryOf(l[0], wa, pa)];
+    const double b = B.val[B.entryOf(l[1], wb, pb)];
+    const double r = ruleBoolean(kind, a, b, wT, wS);
+    if (movedDown && kind == B_COMBINE && !isUndef(a) && !isUndef(b) && r > b) *movedDown = 1u;
+    return r;
+  }
+};
+
 // Keep flag of candidate i: first entry, defined, or start of a run whose
 // value differs from the previous candidate's.
 LSG_HD inline uint32_t keepFlag(const double *out, uint32_t i) {

@@ -8,6 +8,7 @@
 // buffers), so no kernel ever indexes past its buffers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 
@@ -202,7 +203,6 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t
   if (sc->overflow) return;  // uniform: after the block-wide sum
   if (!SEG && logicalBlock() == 0 && threadIdx.x == 0 && base) cpos[0] = POS_NONE;
   const int lane = threadIdx.x & 63;
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (uint32_t t = lo; t < hi; t += TPB) {
     const uint32_t w = t + threadIdx.x;
     const uint64_t c = cNext;
@@ -505,6 +505,505 @@ __global__ void kBuildIndex(const uint64_t *cwords, const uint32_t *cpre, const 
   }
 }
 
+// ------------------------------------------- single-pass band operation ----
+//
+// One launch per band operation of a single level set: workgroup t takes
+// tile t (BAND_TILE_WORDS bitmap words), builds its candidate words,
+// evaluates the rule at every candidate, compacts the kept results in LDS,
+// obtains the number of entries of all earlier tiles by decoupled look-back
+// over per-tile status words (Merrill & Garland, "Single-pass Parallel Prefix
+// Scan with Decoupled Look-back", 2016), and writes the tile's entries,
+// bitmap words and prefix counts.  A tile waits only on tiles of lower
+// workgroup index; workgroups are dispatched to each XCD in increasing index
+// order, so the smallest unfinished tile is always running and never waits.
+// Each wait is also bounded: when it runs out the kernel flags lookbackFail
+// and the host turns the single-pass kernels off and repeats the work with
+// the multi-kernel pipeline below.  No counter is shared between workgroups:
+// a global claim counter costs ~25 ns per atomic when all XCDs hit one
+// address (measured: 185 us for 7200 tiles).
+//
+// Keep flags: an entry is kept when it is defined or starts a run whose value
+// differs from the previous candidate's (keepOf).  Slot 0 of a tile holds the
+// rule evaluated at the grid point before the tile's first candidate: that is
+// the previous candidate's result (the At* evaluators' contract), so no tile
+// reads another tile's values.
+enum CandMode : int { CAND_STAR = 0, CAND_UNION = 1, CAND_SELF = 2 };
+
+static constexpr int TW = BAND_TILE_WORDS;
+static constexpr uint32_t LOOKBACK_SPIN_LIMIT = 1u << 15;  // about 30 ms of polling
+static constexpr int LOOKBACK_PER_LANE = 8;  // a look-back window spans 64 x 8 tiles
+
+__device__ inline uint32_t launchTag(uint32_t epoch) { return (epoch % 0x3FFFFFFFu) + 1u; }  // never 0
+__device__ inline uint64_t tileWord(uint32_t flag, uint32_t tag, uint32_t v) {
+  return ((uint64_t)flag << 62) | ((uint64_t)tag << 32) | (uint64_t)v;
+}
+constexpr uint32_t TILE_AGG = 1, TILE_INCL = 2;
+
+// Exclusive prefix of the counts of the tiles before tile t (one wave, every
+// lane active); publishes tile t's aggregate, then its inclusive count.  Lane
+// l reads the tiles look - 1 - (l + 64 q), q < LOOKBACK_PER_LANE, so one
+// round of loads covers 512 predecessors; the prefix is complete when every
+// tile up to the nearest inclusive count is ready.
+__device__ uint32_t lookBack(uint64_t *stat, uint32_t t, uint32_t agg, uint32_t tag, DevScalars *sc) {
+  const int lane = threadIdx.x & 63;
+  if (lane == 0)
+    __hip_atomic_store(&stat[t], tileWord(t == 0 ? TILE_INCL : TILE_AGG, tag, agg), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  if (t == 0) return 0;
+  constexpr int Q = LOOKBACK_PER_LANE;
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
+  uint32_t excl = 0;
+  int64_t look = t;  // the window holds tiles look - 1 - j, j < 64 Q
+  uint32_t spins = 0;
+  for (;;) {
+    uint32_t v[Q];
+    uint32_t jn = NONE;
+    bool failed = false;
+    for (;;) {  // poll until every tile up to the nearest inclusive count is ready
+      bool ready[Q], incl[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int64_t idx = look - 1 - lane - 64 * q;
+        uint64_t st = 0;
+        if (idx >= 0) st = __hip_atomic_load(&stat[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t fl = (uint32_t)(st >> 62);
+        ready[q] = idx < 0 || (fl != 0 && (uint32_t)((st >> 32) & 0x3FFFFFFFu) == tag);
+        incl[q] = idx < 0 || (ready[q] && fl == TILE_INCL);  // before tile 0: an inclusive count of 0
+        v[q] = (idx < 0) ? 0u : (uint32_t)st;
+      }
+      jn = NONE;
+#pragma unroll
+      for (int q = Q - 1; q >= 0; --q)
+        if (incl[q]) jn = (uint32_t)(lane + 64 * q);
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t y = (uint32_t)__shfl_xor((int)jn, o, 64);
+        jn = y < jn ? y : jn;
+      }
+      bool ok = true;
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if ((uint32_t)(lane + 64 * q) <= jn && !ready[q]) ok = false;
+      if (__all(ok)) break;
+      if (++spins > LOOKBACK_SPIN_LIMIT) {
+        if (lane == 0) atomicExch(&sc->lookbackFail, 1u);
+        failed = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (failed) break;
+    uint32_t x = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      if ((uint32_t)(lane + 64 * q) <= jn) x += v[q];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+    excl += x;
+    if (jn != NONE) break;  // reached an inclusive count
+    look -= 64 * Q;
+  }
+  if (lane == 0)
+    __hip_atomic_store(&stat[t], tileWord(TILE_INCL, tag, excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
+template <class At, int MODE, int D, bool STAR>
+__global__ __launch_bounds__(256) void kBandOp(At at, const uint64_t *A, const uint64_t *B, Frame f, GridInfo g,
+                                               uint32_t nw, BandBufs out, uint64_t *stat, DevScalars *sc,
+                                               const uint32_t *inPre, uint32_t inBase) {
+  __shared__ uint64_t cw[TW], kw[TW];
+  __shared__ uint32_t cpre[TW + 1], kpre[TW + 1];
+  __shared__ double stage[TW * 64];
+  __shared__ double wlast[TPB / 64];
+  __shared__ uint32_t wcnt[TPB / 64];
+  __shared__ uint32_t exclS;
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const uint32_t tag = launchTag(sc->epoch);
+  const bool skip = sc->overflow != 0;  // an earlier kernel of the step overflowed: no tile is claimed
+  const uint32_t nTiles = (nw + TW - 1) / TW;
+  const uint32_t base = (uint32_t)f.base;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t t = blockIdx.x; !skip && t < nTiles; t += nTiles) {  // one tile per workgroup
+    const uint32_t w0 = t * TW;
+    if (wid == 0) {
+      const uint32_t w = w0 + (uint32_t)lane;
+      uint64_t C = 0;
+      if (lane < TW && w < nw) {
+        if (MODE == CAND_STAR) C = candStarWord<D>(A, f, g, (int64_t)w);
+        else if (MODE == CAND_UNION) C = A[w] | B[w];
+        else C = A[w];
+      }
+      const uint32_t pc = (uint32_t)__builtin_popcountll(C);
+      uint32_t x = pc;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (lane < TW) {
+        cw[lane] = C;
+        cpre[lane] = x - pc;
+        kw[lane] = 0;
+      }
+      if (lane == TW - 1) cpre[TW] = x;
+    }
+    __syncthreads();
+    const uint32_t nc = cpre[TW];
+    // slot 0: the grid point before the first candidate (tile 0: the
+    // grid-minimum entry outside the frame, or nothing when the grid minimum
+    // is the first candidate); slot s >= 1: candidate s - 1
+    const bool hasPred = (t > 0) || base;
+    // tile 0 always evaluates the grid-minimum entry when it lies outside
+    // the frame (output entry 0), even without candidates of its own
+    const uint32_t nslots = (nc > 0 || (t == 0 && base)) ? nc + 1 : 0;
+    uint32_t kept = 0;
+    double carry = 0.0;
+    for (uint32_t s0 = 0; s0 < nslots; s0 += TPB) {
+      const uint32_t s = s0 + threadIdx.x;
+      uint32_t p;
+      bool eval, cand;
+      int j, bit;
+      tileSlot<TW>(f, t, w0, cw, cpre, nc, s, p, eval, cand, j, bit);
+      const double v = eval ? at(p) : 0.0;
+      if (s == 0 && t == 0 && base) {  // output entry 0: the grid-minimum entry
+        out.val[0] = v;
+        out.pos[0] = POS_NONE;
+      }
+      const double up = __shfl_up(v, 1, 64);
+      if (lane == 63) wlast[wid] = v;
+      __syncthreads();
+      const double prev = (lane > 0) ? up : (wid > 0 ? wlast[wid - 1] : carry);
+      const double nextCarry = wlast[TPB / 64 - 1];
+      bool keep = false;
+      if (cand) keep = (s == 1 && !hasPred) || !isUndef(v) || v != prev;
+      const uint64_t m = (uint64_t)__ballot(keep);
+      if (lane == 0) wcnt[wid] = (uint32_t)__builtin_popcountll(m);
+      __syncthreads();
+      uint32_t woff = 0, tot = 0;
+#pragma unroll
+      for (int q = 0; q < TPB / 64; ++q) {
+        const uint32_t cq = wcnt[q];
+        if (q < wid) woff += cq;
+        tot += cq;
+      }
+      if (keep) {
+        stage[kept + woff + (uint32_t)__builtin_popcountll(m & below)] = v;
+        atomicOr((unsigned long long *)&kw[j], 1ull << bit);
+      }
+      kept += tot;
+      carry = nextCarry;
+      __syncthreads();
+    }
+    if (wid == 0) {
+      const uint64_t K = (lane < TW) ? kw[lane] : 0ull;
+      const uint32_t pc = (uint32_t)__builtin_popcountll(K);
+      uint32_t x = pc;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (lane < TW) kpre[lane] = x - pc;
+      if (lane == TW - 1) kpre[TW] = x;
+      const uint32_t e = lookBack(stat, t, kept, tag, sc);
+      if (lane == 0) exclS = e;
+    }
+    __syncthreads();
+    const uint32_t P = exclS;
+    const uint64_t o0 = (uint64_t)base + P;
+    if (o0 + kept <= out.cap) {
+      for (uint32_t k = threadIdx.x; k < kept; k += TPB) {
+        const int jj = wordOfItem<TW>(kpre, k);
+        const int b = selectBit(kw[jj], (int)(k - kpre[jj]));
+        out.val[o0 + k] = stage[k];
+        out.pos[o0 + k] = ((w0 + (uint32_t)jj) << 6) | (uint32_t)b;
+      }
+    } else if (threadIdx.x == 0) {
+      sc->overflow = 1;
+    }
+    if (threadIdx.x < TW) {
+      const uint32_t w = w0 + threadIdx.x;
+      if (w < nw) {
+        out.bits[w] = kw[threadIdx.x];
+        out.pre[w] = P + kpre[threadIdx.x];
+      }
+    }
+    if (t == nTiles - 1 && threadIdx.x == 0) {
+      out.bits[nw] = 0;
+      out.pre[nw] = P + kept;
+      uint64_t n = (uint64_t)base + P + kept;
+      if (n > out.cap) {
+        sc->overflow = 1;
+        n = out.cap;
+      }
+      *out.n = (uint32_t)n;
+      // algorithmic bytes of a star operation: the input band's values and
+      // positions read once, the output band's written once, and both
+      // bitmaps with their prefix counts
+      if (STAR) sc->starBytes += 12.0 * ((double)inPre[nw] + inBase + (double)n) + 24.0 * (double)nw;
+      // every tile has published its count and every workgroup has read the
+      // launch tag: the next launch tags its status words anew
+      __hip_atomic_store(&sc->epoch, sc->epoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+}
+
+// Batches: the same single pass over the segmented bitmap of many level
+// sets.  A tile's items are, per word, the grid-minimum entry of the level
+// set that starts at the word (when its frame does not hold the grid
+// minimum), then the word's candidates.  The first item of a level set is
+// always kept; an in-frame entry of level set k goes to index
+// baseIncl(k) + (in-frame entries before it), its grid-minimum entry just
+// before its first in-frame entry.  The look-back counts in-frame entries.
+template <int D> struct AtStarSeg {
+  BandView all;
+  const SegInfo *segs;
+  OpParams op;
+  __device__ double operator()(uint32_t p, uint32_t k) const { return AtStar<D>{segView(all, segs[k]), op}(p); }
+};
+template <int D> struct AtSelfSeg {
+  BandView all;
+  const SegInfo *segs;
+  OpParams op;  // dt: one per level set
+  __device__ double operator()(uint32_t p, uint32_t k) const {
+    OpParams o = op;
+    o.dt = op.dt + k;
+    return AtSelf<D>{segView(all, segs[k]), o}(p);
+  }
+};
+struct AtBoolSeg {
+  BandView A, B;
+  const SegInfo *segs;
+  int kind;
+  double wT, wS;
+  __device__ double operator()(uint32_t p, uint32_t k) const {
+    const SegInfo &S = segs[k];
+    return AtBool{segView(A, S), segView(B, S), kind, wT, wS, nullptr}(p);
+  }
+};
+
+static constexpr int TW_ITEMS = TW * 64 + TW;  // candidates and grid-minimum entries of a tile
+static constexpr uint32_t NO_SEG = 0xFFFFFFFFu;
+
+template <class At, int MODE, int D>
+__global__ __launch_bounds__(256) void kBandOpSeg(At at, const uint64_t *A, const uint64_t *B, SegArgs sa,
+                                                  uint32_t nw, BandBufs out, uint64_t *stat, DevScalars *sc) {
+  __shared__ uint64_t cw[TW], kw[TW];
+  __shared__ uint32_t ipre[TW + 1], kpre[TW + 1], wk[TW];
+  __shared__ uint32_t extraMask;
+  __shared__ double sval[TW_ITEMS];
+  __shared__ uint32_t soff[TW_ITEMS], spos[TW_ITEMS];
+  __shared__ uint16_t sseg[TW_ITEMS];
+  __shared__ double wlast[TPB / 64];
+  __shared__ uint32_t wlastSeg[TPB / 64];
+  __shared__ uint32_t wcnt[TPB / 64], wcntIn[TPB / 64];
+  __shared__ uint32_t exclS;
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const uint32_t tag = launchTag(sc->epoch);
+  const bool skip = sc->overflow != 0;
+  const uint32_t nTiles = (nw + TW - 1) / TW;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t t = blockIdx.x; !skip && t < nTiles; t += nTiles) {  // one tile per workgroup
+    const uint32_t w0 = t * TW;
+    if (wid == 0) {
+      const uint32_t w = w0 + (uint32_t)lane;
+      uint64_t C = 0;
+      uint32_t k = 0, extra = 0;
+      if (lane < TW && w < nw) {
+        k = sa.wseg[w];
+        const SegInfo &S = sa.segs[k];
+        if (MODE == CAND_STAR) C = candStarWord<D>(A + S.wordOff, S.f, S.g, (int64_t)(w - S.wordOff));
+        else if (MODE == CAND_UNION) C = A[w] | B[w];
+        else C = A[w];
+        extra = (w == S.wordOff && S.f.base) ? 1u : 0u;
+      }
+      const uint32_t pc = (uint32_t)__builtin_popcountll(C) + extra;
+      uint32_t x = pc;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+        if (lane >= o) x += y;
+      }
+      const uint64_t em = (uint64_t)__ballot(extra != 0);
+      if (lane < TW) {
+        cw[lane] = C;
+        ipre[lane] = x - pc;
+        kw[lane] = 0;
+        wk[lane] = k;
+      }
+      if (lane == TW - 1) ipre[TW] = x;
+      if (lane == 0) extraMask = (uint32_t)em;
+    }
+    __syncthreads();
+    const uint32_t ni = ipre[TW];
+    uint32_t kept = 0, keptIn = 0;
+    double carry = 0.0;
+    uint32_t carrySeg = NO_SEG;
+    for (uint32_t s0 = 0; ni > 0 && s0 < ni + 1; s0 += TPB) {
+      const uint32_t s = s0 + threadIdx.x;
+      uint32_t p = POS_NONE, k = NO_SEG;
+      bool eval = false, item = false, isBase = false;
+      int j = 0, bit = 0;
+      if (s == 0) {  // the grid point before the first item, in its level set
+        const int j0 = wordOfItem<TW>(ipre, 0);
+        if (!((extraMask >> j0) & 1u)) {
+          const uint32_t kk = wk[j0];
+          const uint32_t lw = w0 + (uint32_t)j0 - sa.segs[kk].wordOff;
+          const uint32_t p1 = (lw << 6) | (uint32_t)__builtin_ctzll(cw[j0]);
+          if (p1 > 0) {
+            p = realPred(sa.segs[kk].f, p1);
+            k = kk;
+            eval = true;
+          }
+        }
+      } else if (s <= ni) {
+        const uint32_t c = s - 1;
+        j = wordOfItem<TW>(ipre, c);
+        k = wk[j];
+        const uint32_t ex = (extraMask >> j) & 1u;
+        const uint32_t r = c - ipre[j];
+        item = eval = true;
+        if (ex && r == 0) {
+          isBase = true;
+        } else {
+          bit = selectBit(cw[j], (int)(r - ex));
+          p = ((w0 + (uint32_t)j - sa.segs[k].wordOff) << 6) | (uint32_t)bit;
+        }
+      }
+      const double v = eval ? at(p, k) : 0.0;
+      const double up = __shfl_up(v, 1, 64);
+      const uint32_t upSeg = (uint32_t)__shfl_up((int)k, 1, 64);
+      if (lane == 63) {
+        wlast[wid] = v;
+        wlastSeg[wid] = k;
+      }
+      __syncthreads();
+      const double prev = (lane > 0) ? up : (wid > 0 ? wlast[wid - 1] : carry);
+      const uint32_t prevSeg = (lane > 0) ? upSeg : (wid > 0 ? wlastSeg[wid - 1] : carrySeg);
+      const double nextCarry = wlast[TPB / 64 - 1];
+      const uint32_t nextCarrySeg = wlastSeg[TPB / 64 - 1];
+      bool keep = false;
+      if (item) keep = isBase || prevSeg != k || !isUndef(v) || v != prev;
+      const bool keepIn = keep && !isBase;
+      const uint64_t m = (uint64_t)__ballot(keep);
+      const uint64_t mIn = (uint64_t)__ballot(keepIn);
+      if (lane == 0) {
+        wcnt[wid] = (uint32_t)__builtin_popcountll(m);
+        wcntIn[wid] = (uint32_t)__builtin_popcountll(mIn);
+      }
+      __syncthreads();
+      uint32_t woff = 0, tot = 0, woffIn = 0, totIn = 0;
+#pragma unroll
+      for (int q = 0; q < TPB / 64; ++q) {
+        const uint32_t cq = wcnt[q], cqIn = wcntIn[q];
+        if (q < wid) {
+          woff += cq;
+          woffIn += cqIn;
+        }
+        tot += cq;
+        totIn += cqIn;
+      }
+      if (keep) {
+        const uint32_t si = kept + woff + (uint32_t)__builtin_popcountll(m & below);
+        const uint32_t inBefore = keptIn + woffIn + (uint32_t)__builtin_popcountll(mIn & below);
+        sval[si] = v;
+        soff[si] = sa.segs[k].baseIncl - (isBase ? 1u : 0u) + inBefore;
+        spos[si] = p;
+        sseg[si] = (uint16_t)k;
+        if (!isBase) atomicOr((unsigned long long *)&kw[j], 1ull << bit);
+      }
+      kept += tot;
+      keptIn += totIn;
+      carry = nextCarry;
+      carrySeg = nextCarrySeg;
+      __syncthreads();
+    }
+    if (wid == 0) {
+      const uint64_t K = (lane < TW) ? kw[lane] : 0ull;
+      const uint32_t pc = (uint32_t)__builtin_popcountll(K);
+      uint32_t x = pc;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (lane < TW) kpre[lane] = x - pc;
+      if (lane == TW - 1) kpre[TW] = x;
+      const uint32_t e = lookBack(stat, t, keptIn, tag, sc);
+      if (lane == 0) exclS = e;
+    }
+    __syncthreads();
+    const uint32_t P = exclS;
+    bool over = false;
+    for (uint32_t q = threadIdx.x; q < kept; q += TPB) {
+      const uint64_t o = (uint64_t)P + soff[q];
+      if (o < out.cap) {
+        out.val[o] = sval[q];
+        out.pos[o] = spos[q];
+        out.seg[o] = sseg[q];
+      } else {
+        over = true;
+      }
+    }
+    if (over) sc->overflow = 1;
+    if (threadIdx.x < TW) {
+      const uint32_t w = w0 + threadIdx.x;
+      if (w < nw) {
+        out.bits[w] = kw[threadIdx.x];
+        out.pre[w] = P + kpre[threadIdx.x];
+      }
+    }
+    if (t == nTiles - 1 && threadIdx.x == 0) {
+      out.bits[nw] = 0;
+      out.pre[nw] = P + keptIn;
+      uint64_t n = (uint64_t)sa.totalBase + P + keptIn;
+      if (n > out.cap) {
+        sc->overflow = 1;
+        n = out.cap;
+      }
+      *out.n = (uint32_t)n;
+      __hip_atomic_store(&sc->epoch, sc->epoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+}
+
+
+// Single-pass kernels on (LSG_FUSED_BANDOPS=0 turns them off); turned off
+// for the rest of the process when a look-back wait runs out.
+static bool g_fused = [] {
+  const char *e = getenv("LSG_FUSED_BANDOPS");
+  return !(e && e[0] == '0');
+}();
+bool singlePassBandOps() { return g_fused; }
+void disableSinglePassBandOps() { g_fused = false; }
+
+template <class At, int MODE, int D, bool STAR>
+static void bandOp(hipStream_t s, const At &at, const uint64_t *A, const uint64_t *B, const Frame &f,
+                   const GridInfo &g, BandBufs &out, uint64_t *stat, DevScalars *sc, const BandBufs &in) {
+  const uint32_t nw = (uint32_t)f.nwords;
+  const uint32_t nTiles = (nw + TW - 1) / TW;
+  kBandOp<At, MODE, D, STAR><<<nTiles, TPB, 0, s>>>(at, A, B, f, g, nw, out, stat, sc, in.pre, (uint32_t)f.base);
+}
+
+void launchStarOpScratch(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f,
+                         const GridInfo &g, const OpParams &op, uint64_t *stat, DevScalars *sc) {
+  const BandView v = viewOf(in, f, g);
+  if (D == 2) bandOp<AtStar<2>, CAND_STAR, 2, true>(s, AtStar<2>{v, op}, in.bits, nullptr, f, g, out, stat, sc, in);
+  else bandOp<AtStar<3>, CAND_STAR, 3, true>(s, AtStar<3>{v, op}, in.bits, nullptr, f, g, out, stat, sc, in);
+}
+
+__global__ void kClearFlags(DevScalars *sc) {
+  sc->overflow = 0;
+  sc->lookbackFail = 0;
+  sc->reframe = 0;
+}
+void launchClearFlags(hipStream_t s, DevScalars *sc) { kClearFlags<<<1, 1, 0, s>>>(sc); }
+
 // ------------------------------------------------------- host wrappers ----
 
 static SegArgs noSeg() { return SegArgs{nullptr, 0, 0, nullptr, nullptr, nullptr}; }
@@ -575,9 +1074,24 @@ static void boolOp(hipStream_t s, const BandBufs &A, const BandBufs &B, BandBufs
   compactInto<EvalBool<SEG>, SEG>(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, nw, f.base, out, sc, sa);
 }
 
+template <class At, int MODE, int D>
+static void bandOpSeg(hipStream_t s, const At &at, const uint64_t *A, const uint64_t *B, const SegCtx &c,
+                      BandBufs &out, uint64_t *stat, DevScalars *sc) {
+  const uint32_t nTiles = (c.nw + TW - 1) / TW;
+  kBandOpSeg<At, MODE, D><<<nTiles, TPB, 0, s>>>(at, A, B, segArgs(c, nullptr, nullptr), c.nw, out, stat, sc);
+}
+
 void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc, hipEvent_t e0, hipEvent_t e1,
                   bool externalEvents) {
+  if (g_fused) {
+    const BandView v = viewOf(in, f, g);
+    recordEvent(s, e0, externalEvents);
+    if (D == 2) bandOp<AtStar<2>, CAND_STAR, 2, true>(s, AtStar<2>{v, op}, in.bits, nullptr, f, g, out, ws.tileStat, sc, in);
+    else bandOp<AtStar<3>, CAND_STAR, 3, true>(s, AtStar<3>{v, op}, in.bits, nullptr, f, g, out, ws.tileStat, sc, in);
+    recordEvent(s, e1, externalEvents);
+    return;
+  }
   const uint32_t nw = (uint32_t)f.nwords;
   if (D == 2) starOp<2, false>(s, in, out, f, g, nw, ws, op, sc, noSeg(), e0, e1, externalEvents);
   else starOp<3, false>(s, in, out, f, g, nw, ws, op, sc, noSeg(), e0, e1, externalEvents);
@@ -585,6 +1099,12 @@ void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const
 
 void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc) {
+  if (g_fused) {
+    const BandView v = viewOf(in, f, g);
+    if (D == 2) bandOp<AtSelf<2>, CAND_SELF, 2, false>(s, AtSelf<2>{v, op}, in.bits, nullptr, f, g, out, ws.tileStat, sc, in);
+    else bandOp<AtSelf<3>, CAND_SELF, 3, false>(s, AtSelf<3>{v, op}, in.bits, nullptr, f, g, out, ws.tileStat, sc, in);
+    return;
+  }
   const uint32_t nw = (uint32_t)f.nwords;
   if (D == 2) selfOp<2, false>(s, in, out, f, g, nw, ws, op, sc, noSeg());
   else selfOp<3, false>(s, in, out, f, g, nw, ws, op, sc, noSeg());
@@ -593,6 +1113,12 @@ void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const
 void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
                    const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc,
                    uint32_t *movedDown) {
+  if (g_fused) {
+    const AtBool at{viewOf(A, f, g), viewOf(B, f, g), kind, wTarget, wSource, movedDown};
+    if (D == 2) bandOp<AtBool, CAND_UNION, 2, false>(s, at, A.bits, B.bits, f, g, out, ws.tileStat, sc, A);
+    else bandOp<AtBool, CAND_UNION, 3, false>(s, at, A.bits, B.bits, f, g, out, ws.tileStat, sc, A);
+    return;
+  }
   const uint32_t nw = (uint32_t)f.nwords;
   if (D == 2) boolOp<2, false>(s, A, B, out, f, g, nw, ws, kind, wTarget, wSource, sc, noSeg(), movedDown);
   else boolOp<3, false>(s, A, B, out, f, g, nw, ws, kind, wTarget, wSource, sc, noSeg(), movedDown);
@@ -603,6 +1129,14 @@ void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, B
 // its level set's from the table).
 void launchStarOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const SegCtx &c, Workspace &ws,
                      const OpParams &op, DevScalars *sc) {
+  if (g_fused) {
+    const BandView all = viewOf(in, Frame{}, GridInfo{});
+    if (D == 2) bandOpSeg<AtStarSeg<2>, CAND_STAR, 2>(s, AtStarSeg<2>{all, c.segs, op}, in.bits, nullptr, c, out,
+                                                      ws.tileStat, sc);
+    else bandOpSeg<AtStarSeg<3>, CAND_STAR, 3>(s, AtStarSeg<3>{all, c.segs, op}, in.bits, nullptr, c, out,
+                                               ws.tileStat, sc);
+    return;
+  }
   Frame f{};
   GridInfo g{};
   if (D == 2) starOp<2, true>(s, in, out, f, g, c.nw, ws, op, sc, segArgs(c, nullptr, nullptr));
@@ -611,6 +1145,14 @@ void launchStarOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, co
 
 void launchSelfOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const SegCtx &c, Workspace &ws,
                      const OpParams &op, DevScalars *sc) {
+  if (g_fused) {
+    const BandView all = viewOf(in, Frame{}, GridInfo{});
+    if (D == 2) bandOpSeg<AtSelfSeg<2>, CAND_SELF, 2>(s, AtSelfSeg<2>{all, c.segs, op}, in.bits, nullptr, c, out,
+                                                      ws.tileStat, sc);
+    else bandOpSeg<AtSelfSeg<3>, CAND_SELF, 3>(s, AtSelfSeg<3>{all, c.segs, op}, in.bits, nullptr, c, out,
+                                               ws.tileStat, sc);
+    return;
+  }
   Frame f{};
   GridInfo g{};
   if (D == 2) selfOp<2, true>(s, in, out, f, g, c.nw, ws, op, sc, segArgs(c, nullptr, nullptr));
@@ -619,6 +1161,12 @@ void launchSelfOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, co
 
 void launchBooleanSeg(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const SegCtx &c,
                       Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc) {
+  if (g_fused) {
+    const AtBoolSeg at{viewOf(A, Frame{}, GridInfo{}), viewOf(B, Frame{}, GridInfo{}), c.segs, kind, wTarget, wSource};
+    if (D == 2) bandOpSeg<AtBoolSeg, CAND_UNION, 2>(s, at, A.bits, B.bits, c, out, ws.tileStat, sc);
+    else bandOpSeg<AtBoolSeg, CAND_UNION, 3>(s, at, A.bits, B.bits, c, out, ws.tileStat, sc);
+    return;
+  }
   Frame f{};
   GridInfo g{};
   if (D == 2) boolOp<2, true>(s, A, B, out, f, g, c.nw, ws, kind, wTarget, wSource, sc, segArgs(c, nullptr, nullptr));
@@ -1287,6 +1835,7 @@ __global__ void kStepBeginSeg(DevScalars *sc, double *maxDt, uint32_t nseg) {
 __global__ void kStepTimeSeg(const DevScalars *sc, const double *dt, uint32_t nseg, int temporal, double *stepTime,
                              double *advTime) {
   LSG_ABORT_IF_OVERFLOW(sc);
+  if (sc->lookbackFail) return;  // the step is repeated
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nseg) return;
   const double d1 = dt[k], d2 = dt[nseg + k], d3 = dt[2 * nseg + k];
@@ -1439,11 +1988,11 @@ __device__ inline int32_t blockMinI(int32_t v) {
 }
 
 __global__ void kBandStats(const double *val, const uint32_t *pos, const uint32_t *nptr, Frame f, uint32_t *count,
-                           int32_t *bbox) {
+                           int32_t *bbox, uint32_t first) {
   const uint32_t n = *nptr;
   int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
   unsigned long long c = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (!isUndef(val[i])) ++c;
     if (pos[i] == POS_NONE) continue;
     int32_t r[3];
@@ -1474,9 +2023,10 @@ __global__ void kStatsInit(uint32_t *count, int32_t *bbox) {
     bbox[3 + d] = INT32_MIN;
   }
 }
-void launchBandStats(hipStream_t s, const BandBufs &b, const Frame &f, uint32_t *outCount, int32_t *bbox) {
+void launchBandStats(hipStream_t s, const BandBufs &b, const Frame &f, uint32_t *outCount, int32_t *bbox,
+                     uint32_t first) {
   kStatsInit<<<1, 1, 0, s>>>(outCount, bbox);
-  kBandStats<<<gridFor(b.cap, 512), TPB, 0, s>>>(b.val, b.pos, b.n, f, outCount, bbox);
+  kBandStats<<<gridFor(b.cap, 512), TPB, 0, s>>>(b.val, b.pos, b.n, f, outCount, bbox, first);
 }
 
 // BooleanOperation INVERT (lsBooleanOperation.hpp:214-247): every defined
@@ -1919,6 +2469,41 @@ void launchNearest(hipStream_t s, int D, const double *P, const uint32_t *nP, si
   BandView v = viewOf(q, f, g);
   if (D == 2) kNearest<2><<<gridFor(cap), TPB, 0, s>>>(P, nP, v, qStart, qPts, maxShell, dist);
   else kNearest<3><<<gridFor(cap), TPB, 0, s>>>(P, nP, v, qStart, qPts, maxShell, dist);
+}
+
+// CompareVolume (lsCompareVolume.hpp:257-289): per cell of the box (lower
+// corner lo + (x, y, z), extents ext), the corner averages of both level
+// sets, corners summed in corner order and divided by 2^D; counts the cells
+// inside (average <= 0) in exactly one of them.
+template <int D>
+__global__ void kCompareVolume(BandView A, BandView B, int64_t lo0, int64_t lo1, int64_t lo2, uint64_t e0,
+                               uint64_t e1, uint64_t ncell, unsigned long long *count) {
+  unsigned long long c = 0;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < ncell;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    const int64_t cell[3] = {lo0 + (int64_t)(k % e0), lo1 + (int64_t)((k / e0) % e1), lo2 + (int64_t)(k / (e0 * e1))};
+    double ta = 0., tb = 0.;
+    for (int i = 0; i < (1 << D); ++i) {
+      int32_t p[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) p[d] = A.g.map(d, cell[d] + ((i >> d) & 1));
+      ta += A.value(p[0], p[1], p[2]);
+      tb += B.value(p[0], p[1], p[2]);
+    }
+    ta /= (1 << D);
+    tb /= (1 << D);
+    c += ((ta <= 0.) != (tb <= 0.)) ? 1u : 0u;
+  }
+  const unsigned long long t = blockSumU64(c);
+  if (threadIdx.x == 0 && t) atomicAdd(count, t);
+}
+void launchCompareVolume(hipStream_t s, int D, const BandBufs &a, const Frame &fa, const BandBufs &b, const Frame &fb,
+                         const GridInfo &g, const int64_t *lo, const int64_t *hi, unsigned long long *count) {
+  const uint64_t e0 = (uint64_t)(hi[0] - lo[0] + 1), e1 = (uint64_t)(hi[1] - lo[1] + 1);
+  const uint64_t e2 = (D == 3) ? (uint64_t)(hi[2] - lo[2] + 1) : 1u;
+  const uint64_t n = e0 * e1 * e2;
+  const BandView va = viewOf(a, fa, g), vb = viewOf(b, fb, g);
+  if (D == 2) kCompareVolume<2><<<gridFor(n), TPB, 0, s>>>(va, vb, lo[0], lo[1], 0, e0, e1, n, count);
+  else kCompareVolume<3><<<gridFor(n), TPB, 0, s>>>(va, vb, lo[0], lo[1], lo[2], e0, e1, n, count);
 }
 
 // Direct evaluation of FiniteDifferences::weno3/weno5 on n stencils.

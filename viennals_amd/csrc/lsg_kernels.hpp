@@ -13,6 +13,8 @@ constexpr int SCAN_BLOCKS = 1024;
 constexpr int SCAN_BLOCKS_MAX = 4096;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 1;
+// bitmap words per tile of a single-pass band operation
+constexpr int BAND_TILE_WORDS = 32;
 
 // Device scalars shared by the kernels of one advection handle.
 struct DevScalars {
@@ -33,6 +35,10 @@ struct DevScalars {
   // stencil local Lax-Friedrichs: ((j + 1) << 4) | material of the last
   // active point j (active-list order) whose scheme call set its alphas
   unsigned long long sllfKey;
+  // single-pass band operations: the launch counter tagging tile status
+  // words, and a flag set when a look-back wait ran out (never expected)
+  uint32_t epoch;
+  uint32_t lookbackFail;
 };
 
 // Mutable device buffers of a band.
@@ -58,6 +64,7 @@ struct Workspace {
   uint32_t *ncand;     // device candidate count
   size_t ccap;
   uint16_t *cseg;      // ccap (batch: level set of each candidate)
+  uint64_t *tileStat;  // nwords / BAND_TILE_WORDS + 1 (single-pass band operations: tile status)
 };
 
 // A batch launch: level-set table and per-word level set (device), words.
@@ -71,6 +78,10 @@ struct SegCtx {
 
 BandView viewOf(const BandBufs &b, const Frame &f, const GridInfo &g);
 
+// The single-pass band-operation kernels: whether they are used, and their
+// switch-off after a look-back wait ran out (DevScalars::lookbackFail).
+bool singlePassBandOps();
+void disableSinglePassBandOps();
 void setScanBlocks(int blocks);
 int defaultScanBlocks();
 int scanBlocks();
@@ -122,6 +133,12 @@ void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const
 void launchBoolean(hipStream_t s, int D, const BandBufs &A, const BandBufs &B, BandBufs &out, const Frame &f,
                    const GridInfo &g, Workspace &ws, int kind, double wTarget, double wSource, DevScalars *sc,
                    uint32_t *movedDown = nullptr);
+// Star operation with caller-provided scratch: the counters in *sc and the
+// tile status words in stat (at least nwords / BAND_TILE_WORDS + 2).
+void launchStarOpScratch(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f,
+                         const GridInfo &g, const OpParams &op, uint64_t *stat, DevScalars *sc);
+// Clears the overflow, look-back and re-frame flags of *sc.
+void launchClearFlags(hipStream_t s, DevScalars *sc);
 // Rate pass: Hamiltonian + dissipation per active entry, CFL reduction.
 void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
                  double *rates, DevScalars *sc, const double *maxTimeStep, double *dtOut, Workspace &ws,
@@ -137,7 +154,9 @@ void launchStepBegin(hipStream_t s, DevScalars *sc, const double *maxDt);
 void launchCopyBand(hipStream_t s, const BandBufs &src, BandBufs &dst, const Frame &f);
 // Count defined entries (to *outCount) and the bounding box of in-frame
 // entries (frame-relative, to bbox[6]: min x,y,z then max x,y,z).
-void launchBandStats(hipStream_t s, const BandBufs &b, const Frame &f, uint32_t *outCount, int32_t *bbox);
+// Entries before `first` are skipped.
+void launchBandStats(hipStream_t s, const BandBufs &b, const Frame &f, uint32_t *outCount, int32_t *bbox,
+                     uint32_t first = 0);
 void launchFill32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void launchNegate(hipStream_t s, const BandBufs &b);
 
@@ -160,6 +179,11 @@ void launchSurfacePoints(hipStream_t s, int D, const BandBufs &b, const Frame &f
 void launchNearest(hipStream_t s, int D, const double *P, const uint32_t *nP, size_t cap, const BandBufs &q,
                    const Frame &f, const GridInfo &g, const uint32_t *qStart, const double *qPts, int maxShell,
                    double *dist);
+// CompareVolume: cells of the box lo..hi (inclusive lower-corner indices)
+// whose corner average is inside (<= 0) in exactly one of a, b, added to
+// *count.
+void launchCompareVolume(hipStream_t s, int D, const BandBufs &a, const Frame &fa, const BandBufs &b, const Frame &fb,
+                         const GridInfo &g, const int64_t *lo, const int64_t *hi, unsigned long long *count);
 // MarkVoidPoints: the marker (1 void, 0 not) of every defined point of b
 // (dlist / ndef: the defined entries in order); sign / par hold one byte /
 // one word per frame point.
