@@ -9,10 +9,12 @@ directional etch velocity -(max(0,n_z) + 0.1)), WENO5 + RK3, CFL 0.4, FP64.
 value = point-updates/s: active points (|phi| <= 0.5) summed over the RK
 stages of the K timed steps, divided by the wall time of those steps with
 the band resident in HBM.  Two kernels of the step are then timed with HIP
-events on the library's stream over K more steps: the single-pass star band
-operation (Expand cycles and rebuilds; algorithmic bytes: 12 B per input
-entry read and per output entry written, value and position, plus 24 B per
-bitmap word for both bitmaps with their prefix counts) and the rate
+events on the library's stream over K more steps: the rule evaluation of
+the star band operations (Expand cycles and rebuilds; kEvalKeep<EvalStar>,
+algorithmic bytes 8 B per input entry and 8 B per candidate; with
+--single-pass the whole single-pass operation kBandOp<AtStar>, 12 B per
+input entry read and per output entry written, value and position, plus
+24 B per bitmap word for both bitmaps with their prefix counts) and the rate
 (Hamiltonian) kernel (8 B per band entry read plus 8 B per active point
 written).  The one with more time is reported as `roofline`.
 """
@@ -272,18 +274,25 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-size", type=int, default=512)
     ap.add_argument("--seq-steps", type=int, default=2)
+    ap.add_argument("--single-pass", action="store_true",
+                    help="band operations as single-pass kernels (LSG_FUSED_BANDOPS=1 does the same)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU; "
               "multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
         return 2
 
+    single_pass = args.single_pass or os.environ.get("LSG_FUSED_BANDOPS", "") == "1"
+    if single_pass:
+        from viennals_amd._lib import check, lib
+        check(lib().lsg_set_single_pass_band_ops(1))
     if args.workload == "batch512":
         return run_batch(args)
 
     import viennals_amd as vls
 
     d, vel, spatial, temporal, cfl, cfg = build_workload(vls, args.workload)
+    cfg["band_operations"] = "single-pass" if single_pass else "multi-kernel"
     idx0, val0 = d.get_runs()
     width0 = d.getLevelSetWidth()
 
@@ -335,8 +344,12 @@ def main():
 
     # the dominant kernel of the step (most time over the timed steps) gets
     # "roofline"; the other measured kernel is reported beside it
-    star = roofline("kBandOpStar", "kBandOp<AtStar> (single-pass Expand / rebuild band operation)",
-                    adv.getStarKernelStats())
+    if single_pass:
+        star = roofline("kBandOpStar", "kBandOp<AtStar> (single-pass Expand / rebuild band operation)",
+                        adv.getStarKernelStats())
+    else:
+        star = roofline("kEvalKeepStar", "kEvalKeep<EvalStar> (Expand / rebuild rule evaluation)",
+                        adv.getStarKernelStats())
     rate = roofline("kRates", "kRates (Hamiltonian + CFL reduction)", adv.getRateKernelStats())
     dominant, other = (star, rate) if star["total_ms"] >= rate["total_ms"] else (rate, star)
 

@@ -311,6 +311,12 @@ int lsg_batch_autotune(lsg_batch *b, int *chosen, double *ms);
 
 /* Device synchronisation (for timing by callers). */
 int lsg_device_synchronize(void);
+/* Band operations of single level sets and batches as one single-pass
+ * kernel each (on != 0) or as the multi-kernel pipeline (0, the default;
+ * also set by LSG_FUSED_BANDOPS=1).  A tuning control with no counterpart
+ * in the reference; both give the same level sets.  Advect objects keep
+ * the kernels of the steps they have captured. */
+int lsg_set_single_pass_band_ops(int on);
 
 #ifdef __cplusplus
 }

@@ -42,7 +42,7 @@ EXPORTED = [
     "lsg_advect_set_save_advection_velocities", "lsg_domain_point_data_insert", "lsg_domain_point_data_count",
     "lsg_domain_point_data_label", "lsg_domain_point_data_size", "lsg_domain_point_data_get",
     "lsg_domain_point_data_clear", "lsg_mark_void_points", "lsg_surface_points", "lsg_compare_chamfer",
-    "lsg_compare_volume",
+    "lsg_compare_volume", "lsg_set_single_pass_band_ops",
 ]
 
 
@@ -115,6 +115,7 @@ def lib():
         "lsg_surface_points": ([P, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)], c_int),
         "lsg_compare_chamfer": ([P, P, ctypes.c_void_p, ctypes.c_void_p], c_int),
         "lsg_compare_volume": ([P, P, ctypes.c_void_p, ctypes.c_void_p], c_int),
+        "lsg_set_single_pass_band_ops": ([c_int], c_int),
         "lsg_advect_prepare_ls": ([P], c_int),
         "lsg_advect_apply": ([P], c_int),
         "lsg_advect_get_advected_time": ([P, dp], c_int),

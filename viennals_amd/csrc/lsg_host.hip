@@ -2462,6 +2462,7 @@ int lsg_expand(lsg_domain *d, int width) {
   doExpand(d, width);
   return afterOp(d);
 }
+
 int lsg_reduce(lsg_domain *d, int width) {
   if (int e = ensureStores(d, d->cur.b.cap)) return e;
   doReduce(d, width);
@@ -2913,6 +2914,12 @@ int lsg_batch_get_point_updates(lsg_batch *b, double *n, unsigned *steps) {
 
 int lsg_batch_write_back(lsg_batch *b) { return batchWriteBack(b); }
 
+int lsg_set_single_pass_band_ops(int on) {
+  if (int e = ctxInit()) return e;
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  setSinglePassBandOps(on != 0);
+  return 0;
+}
 int lsg_batch_set_scan_blocks(lsg_batch *b, int blocks) {
   if (blocks < 8 || blocks > SCAN_BLOCKS_MAX) return fail(-1, "batch: scan grid out of range");
   b->scanBlocks = blocks;

@@ -529,7 +529,7 @@ __global__ void kBuildIndex(const uint64_t *cwords, const uint32_t *cpre, const 
 // reads another tile's values.
 enum CandMode : int { CAND_STAR = 0, CAND_UNION = 1, CAND_SELF = 2 };
 
-static constexpr int TW = BAND_TILE_WORDS;
+static constexpr int TWS = BAND_TILE_WORDS;  // tile width of the batch kernel
 static constexpr uint32_t LOOKBACK_SPIN_LIMIT = 1u << 15;  // about 30 ms of polling
 static constexpr int LOOKBACK_PER_LANE = 8;  // a look-back window spans 64 x 8 tiles
 
@@ -544,7 +544,14 @@ constexpr uint32_t TILE_AGG = 1, TILE_INCL = 2;
 // l reads the tiles look - 1 - (l + 64 q), q < LOOKBACK_PER_LANE, so one
 // round of loads covers 512 predecessors; the prefix is complete when every
 // tile up to the nearest inclusive count is ready.
-__device__ uint32_t lookBack(uint64_t *stat, uint32_t t, uint32_t agg, uint32_t tag, DevScalars *sc) {
+// Look-back tuning: the first poll reads 64 predecessors; while none of
+// them holds an inclusive count (and all are ready) the window grows to
+// 64 x lbq tiles; between polls a wave sleeps lbSleep x 128 clocks.
+struct Lookback {
+  uint32_t q;      // 1..LOOKBACK_PER_LANE
+  uint32_t sleep;  // >= 1
+};
+__device__ uint32_t lookBack(uint64_t *stat, uint32_t t, uint32_t agg, uint32_t tag, DevScalars *sc, Lookback lb) {
   const int lane = threadIdx.x & 63;
   if (lane == 0)
     __hip_atomic_store(&stat[t], tileWord(t == 0 ? TILE_INCL : TILE_AGG, tag, agg), __ATOMIC_RELAXED,
@@ -553,7 +560,8 @@ __device__ uint32_t lookBack(uint64_t *stat, uint32_t t, uint32_t agg, uint32_t 
   constexpr int Q = LOOKBACK_PER_LANE;
   constexpr uint32_t NONE = 0xFFFFFFFFu;
   uint32_t excl = 0;
-  int64_t look = t;  // the window holds tiles look - 1 - j, j < 64 Q
+  int64_t look = t;  // the window holds tiles look - 1 - j, j < 64 qw
+  uint32_t qw = 1;   // the window's width in 64-tile rows
   uint32_t spins = 0;
   for (;;) {
     uint32_t v[Q];
@@ -563,13 +571,17 @@ __device__ uint32_t lookBack(uint64_t *stat, uint32_t t, uint32_t agg, uint32_t 
       bool ready[Q], incl[Q];
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
+        // rows outside the window count as ready, hold no count and are
+        // never inclusive; before tile 0 lies an inclusive count of 0
         const int64_t idx = look - 1 - lane - 64 * q;
+        const bool in = (uint32_t)q < qw;
         uint64_t st = 0;
-        if (idx >= 0) st = __hip_atomic_load(&stat[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (in && idx >= 0) st = __hip_atomic_load(&stat[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t fl = (uint32_t)(st >> 62);
-        ready[q] = idx < 0 || (fl != 0 && (uint32_t)((st >> 32) & 0x3FFFFFFFu) == tag);
-        incl[q] = idx < 0 || (ready[q] && fl == TILE_INCL);  // before tile 0: an inclusive count of 0
-        v[q] = (idx < 0) ? 0u : (uint32_t)st;
+        const bool rd = idx < 0 || (fl != 0 && (uint32_t)((st >> 32) & 0x3FFFFFFFu) == tag);
+        ready[q] = !in || rd;
+        incl[q] = in && (idx < 0 || (rd && fl == TILE_INCL));
+        v[q] = (in && idx >= 0) ? (uint32_t)st : 0u;
       }
       jn = NONE;
 #pragma unroll
@@ -590,7 +602,7 @@ __device__ uint32_t lookBack(uint64_t *stat, uint32_t t, uint32_t agg, uint32_t 
         failed = true;
         break;
       }
-      __builtin_amdgcn_s_sleep(1);
+      for (uint32_t z = 0; z < lb.sleep; ++z) __builtin_amdgcn_s_sleep(2);
     }
     if (failed) break;
     uint32_t x = 0;
@@ -601,17 +613,19 @@ __device__ uint32_t lookBack(uint64_t *stat, uint32_t t, uint32_t agg, uint32_t 
     for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
     excl += x;
     if (jn != NONE) break;  // reached an inclusive count
-    look -= 64 * Q;
+    look -= 64 * (int64_t)qw;
+    qw = lb.q;
   }
   if (lane == 0)
     __hip_atomic_store(&stat[t], tileWord(TILE_INCL, tag, excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return excl;
 }
 
-template <class At, int MODE, int D, bool STAR>
+template <class At, int MODE, int D, bool STAR, int TW>
 __global__ __launch_bounds__(256) void kBandOp(At at, const uint64_t *A, const uint64_t *B, Frame f, GridInfo g,
                                                uint32_t nw, BandBufs out, uint64_t *stat, DevScalars *sc,
-                                               const uint32_t *inPre, uint32_t inBase) {
+                                               const uint32_t *inPre, uint32_t inBase, Lookback lb) {
+  static_assert(TW == 32 || TW == 64, "wave 0 builds one word per lane");
   __shared__ uint64_t cw[TW], kw[TW];
   __shared__ uint32_t cpre[TW + 1], kpre[TW + 1];
   __shared__ double stage[TW * 64];
@@ -707,7 +721,7 @@ __global__ __launch_bounds__(256) void kBandOp(At at, const uint64_t *A, const u
       }
       if (lane < TW) kpre[lane] = x - pc;
       if (lane == TW - 1) kpre[TW] = x;
-      const uint32_t e = lookBack(stat, t, kept, tag, sc);
+      const uint32_t e = lookBack(stat, t, kept, tag, sc, lb);
       if (lane == 0) exclS = e;
     }
     __syncthreads();
@@ -785,12 +799,14 @@ struct AtBoolSeg {
   }
 };
 
-static constexpr int TW_ITEMS = TW * 64 + TW;  // candidates and grid-minimum entries of a tile
+static constexpr int TW_ITEMS = TWS * 64 + TWS;  // candidates and grid-minimum entries of a tile
 static constexpr uint32_t NO_SEG = 0xFFFFFFFFu;
 
 template <class At, int MODE, int D>
 __global__ __launch_bounds__(256) void kBandOpSeg(At at, const uint64_t *A, const uint64_t *B, SegArgs sa,
-                                                  uint32_t nw, BandBufs out, uint64_t *stat, DevScalars *sc) {
+                                                  uint32_t nw, BandBufs out, uint64_t *stat, DevScalars *sc,
+                                                  Lookback lb) {
+  constexpr int TW = TWS;
   __shared__ uint64_t cw[TW], kw[TW];
   __shared__ uint32_t ipre[TW + 1], kpre[TW + 1], wk[TW];
   __shared__ uint32_t extraMask;
@@ -933,7 +949,7 @@ __global__ __launch_bounds__(256) void kBandOpSeg(At at, const uint64_t *A, cons
       }
       if (lane < TW) kpre[lane] = x - pc;
       if (lane == TW - 1) kpre[TW] = x;
-      const uint32_t e = lookBack(stat, t, keptIn, tag, sc);
+      const uint32_t e = lookBack(stat, t, keptIn, tag, sc, lb);
       if (lane == 0) exclS = e;
     }
     __syncthreads();
@@ -973,21 +989,36 @@ __global__ __launch_bounds__(256) void kBandOpSeg(At at, const uint64_t *A, cons
 }
 
 
-// Single-pass kernels on (LSG_FUSED_BANDOPS=0 turns them off); turned off
-// for the rest of the process when a look-back wait runs out.
-static bool g_fused = [] {
-  const char *e = getenv("LSG_FUSED_BANDOPS");
-  return !(e && e[0] == '0');
-}();
+// Single-pass band operations: off by default (the multi-kernel pipeline
+// below is faster at the measured sizes); LSG_FUSED_BANDOPS=1 or
+// setSinglePassBandOps turns them on; a look-back wait that runs out turns
+// them off for the rest of the process.  LSG_LB_WINDOW (1..8 rows of 64
+// tiles), LSG_LB_SLEEP (polls' backoff) and LSG_BAND_TILE_WORDS (32 or 64)
+// tune them.
+static uint32_t envU32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi) {
+  const char *e = getenv(name);
+  if (!e || !e[0]) return dflt;
+  const unsigned long v = strtoul(e, nullptr, 10);
+  return (uint32_t)std::min<unsigned long>(std::max<unsigned long>(v, lo), hi);
+}
+static bool g_fused = envU32("LSG_FUSED_BANDOPS", 0, 0, 1) != 0;
+static const Lookback g_lb{envU32("LSG_LB_WINDOW", LOOKBACK_PER_LANE, 1, LOOKBACK_PER_LANE),
+                           envU32("LSG_LB_SLEEP", 1, 1, 64)};
+static const int g_bandTW = envU32("LSG_BAND_TILE_WORDS", 32, 32, 64) >= 64 ? 64 : 32;
 bool singlePassBandOps() { return g_fused; }
 void disableSinglePassBandOps() { g_fused = false; }
+void setSinglePassBandOps(bool on) { g_fused = on; }
 
 template <class At, int MODE, int D, bool STAR>
 static void bandOp(hipStream_t s, const At &at, const uint64_t *A, const uint64_t *B, const Frame &f,
                    const GridInfo &g, BandBufs &out, uint64_t *stat, DevScalars *sc, const BandBufs &in) {
   const uint32_t nw = (uint32_t)f.nwords;
-  const uint32_t nTiles = (nw + TW - 1) / TW;
-  kBandOp<At, MODE, D, STAR><<<nTiles, TPB, 0, s>>>(at, A, B, f, g, nw, out, stat, sc, in.pre, (uint32_t)f.base);
+  if (g_bandTW == 64)
+    kBandOp<At, MODE, D, STAR, 64><<<(nw + 63) / 64, TPB, 0, s>>>(at, A, B, f, g, nw, out, stat, sc, in.pre,
+                                                                  (uint32_t)f.base, g_lb);
+  else
+    kBandOp<At, MODE, D, STAR, 32><<<(nw + 31) / 32, TPB, 0, s>>>(at, A, B, f, g, nw, out, stat, sc, in.pre,
+                                                                  (uint32_t)f.base, g_lb);
 }
 
 void launchStarOpScratch(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f,
@@ -1077,8 +1108,8 @@ static void boolOp(hipStream_t s, const BandBufs &A, const BandBufs &B, BandBufs
 template <class At, int MODE, int D>
 static void bandOpSeg(hipStream_t s, const At &at, const uint64_t *A, const uint64_t *B, const SegCtx &c,
                       BandBufs &out, uint64_t *stat, DevScalars *sc) {
-  const uint32_t nTiles = (c.nw + TW - 1) / TW;
-  kBandOpSeg<At, MODE, D><<<nTiles, TPB, 0, s>>>(at, A, B, segArgs(c, nullptr, nullptr), c.nw, out, stat, sc);
+  const uint32_t nTiles = (c.nw + TWS - 1) / TWS;
+  kBandOpSeg<At, MODE, D><<<nTiles, TPB, 0, s>>>(at, A, B, segArgs(c, nullptr, nullptr), c.nw, out, stat, sc, g_lb);
 }
 
 void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,

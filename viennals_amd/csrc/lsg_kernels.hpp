@@ -82,6 +82,7 @@ BandView viewOf(const BandBufs &b, const Frame &f, const GridInfo &g);
 // switch-off after a look-back wait ran out (DevScalars::lookbackFail).
 bool singlePassBandOps();
 void disableSinglePassBandOps();
+void setSinglePassBandOps(bool on);
 void setScanBlocks(int blocks);
 int defaultScanBlocks();
 int scanBlocks();
