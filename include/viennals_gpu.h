@@ -155,6 +155,19 @@ int lsg_make_box_ignore_bc(lsg_domain *d, const double *min_corner, const double
 /* ---- band maintenance ----------------------------------------------------- */
 /* Expand<T,D>(ls, width).apply() (lsExpand.hpp:44-172). */
 int lsg_expand(lsg_domain *d, int width);
+/* lsg_expand without allocating device memory.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under graph
+ * capture, serving loops, harnesses that forbid it).  The level set's two
+ * band stores are the input and the output; the kernels' flags, statistics
+ * and tile status words live in a static device array of about 1 MiB per
+ * GPU, kept until the process ends.  Limits: a frame of at most 2^22 bitmap
+ * words (2^28 grid points), no point data, the expanded band at least 12
+ * cells from its frame edge along infinite axes; otherwise an error with
+ * the level set unchanged.  When the output exceeds the stores' capacity
+ * (-40) the level set holds the cycles completed before, its width says
+ * how many.  Calls share the array: calls that overlap on one GPU (from
+ * several threads) race.  Synchronises once per cycle. */
+int lsg_expand_noalloc(lsg_domain *d, int width);
 /* Reduce<T,D>(ls, width).apply() (lsReduce.hpp:55-134). */
 int lsg_reduce(lsg_domain *d, int width);
 /* Prune<T,D>(ls) with setRemoveStrayZeros (lsPrune.hpp:74-194). */

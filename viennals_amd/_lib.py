@@ -21,7 +21,7 @@ EXPORTED = [
     "lsg_domain_insert_points", "lsg_domain_set_runs", "lsg_domain_num_runs", "lsg_domain_get_runs",
     "lsg_domain_num_points", "lsg_domain_width", "lsg_domain_set_width", "lsg_make_sphere", "lsg_make_plane", "lsg_make_box",
     "lsg_make_sphere_ignore_bc", "lsg_make_box_ignore_bc",
-    "lsg_expand", "lsg_reduce", "lsg_prune", "lsg_boolean", "lsg_rebuild", "lsg_advect_create",
+    "lsg_expand", "lsg_expand_noalloc", "lsg_reduce", "lsg_prune", "lsg_boolean", "lsg_rebuild", "lsg_advect_create",
     "lsg_advect_destroy", "lsg_advect_insert_next_level_set", "lsg_advect_clear_level_sets",
     "lsg_advect_set_velocity_builtin", "lsg_advect_set_velocity_builtin_material",
     "lsg_advect_set_dissipation_alpha_builtin", "lsg_advect_set_spatial_scheme",
@@ -81,6 +81,7 @@ def lib():
         "lsg_make_box": ([P, dp, dp], c_int),
         "lsg_make_box_ignore_bc": ([P, dp, dp, c_int], c_int),
         "lsg_expand": ([P, c_int], c_int),
+        "lsg_expand_noalloc": ([P, c_int], c_int),
         "lsg_reduce": ([P, c_int], c_int),
         "lsg_prune": ([P, c_int], c_int),
         "lsg_boolean": ([P, P, c_int], c_int),

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -40,13 +41,33 @@ static int fail(int code, const std::string &m) {
 static constexpr int MARGIN = 24;        // frame margin (cells) in infinite dimensions
 static constexpr int REFRAME_SLACK = 12; // re-frame when the band comes closer to the frame edge
 
+// Scratch of lsg_expand_noalloc, which allocates nothing: a block of device
+// scalars (flags, launch counter), the band statistics and the tile status
+// words of the single-pass star kernel for frames of up to
+// NOALLOC_MAX_WORDS bitmap words.  Zero at load: tile status words then
+// read as "not ready" and the first launch tag is 1.
+static constexpr int64_t NOALLOC_MAX_WORDS = (int64_t)1 << 22;  // 2^28 frame points
+static constexpr size_t NOALLOC_SC_BYTES = (sizeof(DevScalars) + 255) / 256 * 256;
+static constexpr size_t NOALLOC_STAT_BYTES = 256;
+static constexpr size_t NOALLOC_TILES = (size_t)(NOALLOC_MAX_WORDS / BAND_TILE_WORDS) + 2;
+static constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_SC_BYTES + NOALLOC_STAT_BYTES + NOALLOC_TILES * 8;
+static_assert(NOALLOC_SCRATCH_BYTES <= ((size_t)2 << 20), "no-allocation scratch stays near 1 MiB");
+
+}  // namespace lsg
+
+static __device__ __align__(256) unsigned char scratch[lsg::NOALLOC_SCRATCH_BYTES];
+
+namespace lsg {
+
 struct WsStore;
 struct Ctx {
   bool init = false;
+  int dev = -1;  // the device current at the library's first call; every stream and buffer is on it
   hipStream_t s = nullptr;
   DevScalars *sc = nullptr;
   uint32_t *stat = nullptr;  // [0] count, [1..6] bbox (int32)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  unsigned char *scratch = nullptr;  // device address of `scratch` on dev
 };
 static Ctx g_ctx;
 static WsStore *g_ws = nullptr;
@@ -55,6 +76,9 @@ static int ctxInit() {
   if (g_ctx.init) return 0;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-101, "no HIP device available");
+  LSG_HIP(hipGetDevice(&g_ctx.dev));
+  // the array's name is a host-side stand-in: kernels get this address
+  LSG_HIP(hipGetSymbolAddress((void **)&g_ctx.scratch, HIP_SYMBOL(scratch)));
   LSG_HIP(hipStreamCreateWithFlags(&g_ctx.s, hipStreamNonBlocking));
   LSG_HIP(hipMalloc(&g_ctx.sc, sizeof(DevScalars)));
   LSG_HIP(hipMemset(g_ctx.sc, 0, sizeof(DevScalars)));
@@ -2463,6 +2487,77 @@ int lsg_expand(lsg_domain *d, int width) {
   return afterOp(d);
 }
 
+// Expand without allocating: the cycles of lsg_expand (doExpand) as
+// single-pass star kernels whose flags, statistics and tile status words
+// live in the file-scope `scratch` array, the level set's own two band
+// stores as input and output.  One synchronisation per cycle reads the
+// flags: after an overflow or an unfinished look-back the level set keeps
+// the last complete cycle (the width says which).
+int lsg_expand_noalloc(lsg_domain *d, int width) {
+  if (!g_ctx.init || !d->framed) return fail(-7, "lsg_expand_noalloc: level set has no data");
+  int cur = -1;
+  LSG_HIP(hipGetDevice(&cur));
+  if (cur != g_ctx.dev) return fail(-50, "lsg_expand_noalloc: the library runs on device " + std::to_string(g_ctx.dev));
+  if (width <= d->width) return 0;
+  if (!d->pd.empty()) return fail(-48, "lsg_expand_noalloc: point data is translated with allocated buffers; use lsg_expand");
+  const int64_t nw = d->f.nwords;
+  if (nw > NOALLOC_MAX_WORDS)
+    return fail(-49, "lsg_expand_noalloc: frame of " + std::to_string(nw) + " bitmap words exceeds the limit of " +
+                         std::to_string(NOALLOC_MAX_WORDS));
+  if (!d->tmp.b.val || d->tmp.nwAlloc < nw || !d->cur.b.val)
+    return fail(-7, "lsg_expand_noalloc: level set has no output store");
+  DevScalars *sc = (DevScalars *)g_ctx.scratch;
+  uint32_t *stat = (uint32_t *)(g_ctx.scratch + NOALLOC_SC_BYTES);
+  uint64_t *tiles = (uint64_t *)(g_ctx.scratch + NOALLOC_SC_BYTES + NOALLOC_STAT_BYTES);
+  const int cycles = width - d->width;
+  // the expanded band keeps REFRAME_SLACK cells to every frame edge that
+  // is not a grid end (the state lsg_expand leaves without moving the
+  // frame); each cycle grows the band by at most one cell per axis
+  launchBandStats(g_ctx.s, d->cur.b, d->f, stat, (int32_t *)(stat + 1));
+  uint32_t h[7];
+  LSG_HIP(hipMemcpyAsync(h, stat, sizeof(h), hipMemcpyDeviceToHost, g_ctx.s));
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  const int32_t *bb = (const int32_t *)(h + 1);
+  if (bb[0] <= bb[3])
+    for (int k = 0; k < d->D; ++k) {
+      if (d->g.bounded(k)) continue;
+      const int64_t top = (int64_t)d->f.o[k] + d->f.ext[k] - 1;
+      const int64_t lo = (d->f.o[k] == d->g.lo[k]) ? INT64_MAX : bb[k];
+      const int64_t hi = (top == d->g.hi[k]) ? INT64_MAX : (int64_t)d->f.ext[k] - 1 - bb[3 + k];
+      if (std::min(lo, hi) < (int64_t)cycles + REFRAME_SLACK)
+        return fail(-44, "lsg_expand_noalloc: the expanded band would come within " + std::to_string(REFRAME_SLACK) +
+                             " cells of its frame edge; lsg_expand moves the frame");
+    }
+  launchClearFlags(g_ctx.s, sc);
+  const double totalLimit = width * 0.5;
+  const int startWidth = d->width;
+  for (int cyc = 0; cyc < cycles; ++cyc) {
+    OpParams op{};
+    op.kind = OP_EXPAND;
+    op.a = (startWidth + cyc + 1) * 0.5;
+    op.b = totalLimit;
+    launchStarOpScratch(g_ctx.s, d->D, d->cur.b, d->tmp.b, d->f, d->g, op, tiles, sc);
+    uint32_t flags[2];
+    static_assert(offsetof(DevScalars, reframe) == offsetof(DevScalars, overflow) + 4, "flag layout");
+    LSG_HIP(hipMemcpyAsync(flags, &sc->overflow, sizeof(flags), hipMemcpyDeviceToHost, g_ctx.s));
+    uint32_t lb = 0;
+    LSG_HIP(hipMemcpyAsync(&lb, &sc->lookbackFail, sizeof(lb), hipMemcpyDeviceToHost, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    if (flags[0] || lb) {
+      // status words of tiles that published after the launch counter moved
+      // on would read as ready in the next launch: clear them all
+      if (lb) launchFill32(g_ctx.s, (uint32_t *)tiles, 0u, 2 * NOALLOC_TILES);
+      launchClearFlags(g_ctx.s, sc);
+      LSG_HIP(hipStreamSynchronize(g_ctx.s));
+      d->width = startWidth + cyc;
+      if (lb) return fail(-47, "lsg_expand_noalloc: a single-pass band operation did not complete");
+      return fail(-40, "lsg_expand_noalloc: band capacity overflow");
+    }
+    swapCur(d);
+  }
+  d->width = width;
+  return 0;
+}
 int lsg_reduce(lsg_domain *d, int width) {
   if (int e = ensureStores(d, d->cur.b.cap)) return e;
   doReduce(d, width);
