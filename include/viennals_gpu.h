@@ -107,6 +107,22 @@ int lsg_domain_point_data_label(const lsg_domain *d, int components, int i, char
 int64_t lsg_domain_point_data_size(const lsg_domain *d, int components, int i);
 int lsg_domain_point_data_get(const lsg_domain *d, int components, int i, double *out, int64_t n);
 int lsg_domain_point_data_clear(lsg_domain *d);
+/* Domain::serialize (lsDomain.hpp:188-215), the bytes Writer puts in a
+ * .lvst file (lsWriter.hpp:35-60): "lsDomain", format version 0, the grid,
+ * the run list, the width as uint32 and a point-data flag byte with the
+ * point data after it.  The grid, run-list and point-data sections are
+ * this library's layout (the reference takes them from ViennaHRLE and
+ * ViennaCore, which it does not contain).  *size receives the byte count;
+ * with buf null nothing is written, else at most cap bytes. */
+int lsg_domain_serialize(lsg_domain *d, uint8_t *buf, int64_t cap, int64_t *size);
+/* Domain::deserialize (lsDomain.hpp:218-268), Reader (lsReader.hpp): the
+ * grid, band, width and (when the stream has any) point data are replaced;
+ * a missing header (-60), a newer version (-62), another dimension (-63)
+ * or a short or malformed stream (-61) leave the level set unchanged. */
+int lsg_domain_deserialize(lsg_domain *d, const uint8_t *buf, int64_t size);
+/* Domain::getGrid (lsDomain.hpp:141-144): dimension, index bounds as given,
+ * boundary conditions (BoundaryConditionEnum values) and grid delta. */
+int lsg_domain_get_grid(const lsg_domain *d, int *D, int32_t *gmin, int32_t *gmax, int *bc, double *dx);
 /* MarkVoidPoints<T,D>::apply with its default options, as Advect uses it
  * (lsMarkVoidPoints.hpp:156-331, lsAdvect.hpp:443-447): point data
  * "VoidPointMarkers", 1 for a point enclosed away from the top region. */

@@ -6,6 +6,7 @@ Sphere, Plane, Box, Expand, Reduce, Prune, BooleanOperation, Advect;
 python/pyWrap.hpp:324-401).  All level-set data lives in GPU memory.
 """
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -16,7 +17,7 @@ __all__ = [
     "Domain", "Sphere", "Plane", "Box", "MakeGeometry", "Expand", "Reduce", "Prune", "BooleanOperation",
     "VelocityField", "HostVelocityField", "Advect", "BatchAdvect", "LsgError", "weno", "PointData",
     "MarkVoidPoints", "CompareChamfer", "surface_points", "PrepareStencilLocalLaxFriedrichs",
-    "FinalizeStencilLocalLaxFriedrichs",
+    "FinalizeStencilLocalLaxFriedrichs", "Writer", "Reader",
 ]
 
 POS_VALUE = float(np.finfo(np.float64).max)
@@ -267,6 +268,34 @@ class Domain:
                                         val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n.value))
         return idx, val
 
+    def serialize(self):
+        """Domain::serialize (lsDomain.hpp:188-215): the level set as bytes."""
+        n = ctypes.c_int64()
+        check(lib().lsg_domain_serialize(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        check(lib().lsg_domain_serialize(self._h, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value]
+
+    def deserialize(self, data):
+        """Domain::deserialize (lsDomain.hpp:218-268): grid, band, width and
+        point data from bytes written by serialize."""
+        data = bytes(data)
+        check(lib().lsg_domain_deserialize(self._h, data, len(data)))
+        self._read_grid()
+
+    def _read_grid(self):
+        D = ctypes.c_int()
+        gmin = (ctypes.c_int32 * 3)()
+        gmax = (ctypes.c_int32 * 3)()
+        bc = (ctypes.c_int * 3)()
+        dx = ctypes.c_double()
+        check(lib().lsg_domain_get_grid(self._h, ctypes.byref(D), gmin, gmax, bc, ctypes.byref(dx)))
+        self.D = D.value
+        self.gmin = [gmin[k] for k in range(self.D)]
+        self.gmax = [gmax[k] for k in range(self.D)]
+        self.bc = [bc[k] for k in range(self.D)]
+        self.gridDelta = dx.value
+
     def defined_points(self):
         idx, val = self.get_runs()
         m = np.abs(val) != POS_VALUE
@@ -328,6 +357,60 @@ class MakeGeometry:
             check(lib().lsg_make_box_ignore_bc(d._h, pa, pb, int(mask)))
         else:
             raise LsgError("Invalid geometry type was specified for MakeGeometry.")
+
+
+def _lvst_name(fileName):
+    # Writer::apply / Reader::apply append ".lvst" to other names
+    # (lsWriter.hpp:46-50, lsReader.hpp:44-48)
+    name = str(fileName)
+    if not name.endswith(".lvst"):
+        warnings.warn("File name does not end in '.lvst', appending it.")
+        name += ".lvst"
+    return name
+
+
+class Writer:
+    """Writer<T,D> (lsWriter.hpp:12-60): Domain::serialize into a .lvst file."""
+
+    def __init__(self, levelSet=None, fileName=""):
+        self.levelSet, self.fileName = levelSet, fileName
+
+    def setLevelSet(self, levelSet):
+        self.levelSet = levelSet
+
+    def setFileName(self, fileName):
+        self.fileName = fileName
+
+    def apply(self):
+        if self.levelSet is None:
+            raise LsgError("No level-set was passed to Writer.")
+        if not self.fileName:
+            raise LsgError("No file name specified for Writer.")
+        self.fileName = _lvst_name(self.fileName)
+        with open(self.fileName, "wb") as f:
+            f.write(self.levelSet.serialize())
+
+
+class Reader:
+    """Reader<T,D> (lsReader.hpp:12-60): Domain::deserialize from a .lvst file."""
+
+    def __init__(self, levelSet=None, fileName=""):
+        self.levelSet, self.fileName = levelSet, fileName
+
+    def setLevelSet(self, levelSet):
+        self.levelSet = levelSet
+
+    def setFileName(self, fileName):
+        self.fileName = fileName
+
+    def apply(self):
+        if self.levelSet is None:
+            raise LsgError("No level-set was passed to Reader.")
+        if not self.fileName:
+            raise LsgError("No file name specified for Reader.")
+        self.fileName = _lvst_name(self.fileName)
+        with open(self.fileName, "rb") as f:
+            self.levelSet.deserialize(f.read())
 
 
 class Expand:

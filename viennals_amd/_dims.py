@@ -7,7 +7,7 @@ from . import (Advect, BooleanOperation, BooleanOperationEnum, BoundaryCondition
                CompareVolume, Expand,
                FinalizeStencilLocalLaxFriedrichs, HostVelocityField, LsgError, MakeGeometry, MarkVoidPoints,
                MaterialVelocityField, Plane, PointData, PrepareStencilLocalLaxFriedrichs, Prune, Reduce, Sphere,
-               SpatialSchemeEnum, TemporalSchemeEnum, VelocityField)
+               SpatialSchemeEnum, TemporalSchemeEnum, VelocityField, Reader, Writer)
 from . import Domain as _Domain
 
 

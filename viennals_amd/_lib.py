@@ -41,7 +41,8 @@ EXPORTED = [
     "lsg_advect_set_velocity_update_callback",
     "lsg_advect_set_save_advection_velocities", "lsg_domain_point_data_insert", "lsg_domain_point_data_count",
     "lsg_domain_point_data_label", "lsg_domain_point_data_size", "lsg_domain_point_data_get",
-    "lsg_domain_point_data_clear", "lsg_mark_void_points", "lsg_surface_points", "lsg_compare_chamfer",
+    "lsg_domain_point_data_clear", "lsg_domain_serialize", "lsg_domain_deserialize", "lsg_domain_get_grid",
+    "lsg_mark_void_points", "lsg_surface_points", "lsg_compare_chamfer",
     "lsg_compare_volume", "lsg_set_single_pass_band_ops",
 ]
 
@@ -112,6 +113,9 @@ def lib():
         "lsg_domain_point_data_size": ([P, c_int, c_int], ctypes.c_int64),
         "lsg_domain_point_data_get": ([P, c_int, c_int, ctypes.c_void_p, ctypes.c_int64], c_int),
         "lsg_domain_point_data_clear": ([P], c_int),
+        "lsg_domain_serialize": ([P, ctypes.c_void_p, ctypes.c_int64, i64p], c_int),
+        "lsg_domain_deserialize": ([P, ctypes.c_void_p, ctypes.c_int64], c_int),
+        "lsg_domain_get_grid": ([P, ip, i32p, i32p, ip, dp], c_int),
         "lsg_mark_void_points": ([P], c_int),
         "lsg_surface_points": ([P, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)], c_int),
         "lsg_compare_chamfer": ([P, P, ctypes.c_void_p, ctypes.c_void_p], c_int),

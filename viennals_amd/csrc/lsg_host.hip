@@ -175,6 +175,7 @@ struct PdArray {
 struct lsg_domain {
   int D = 3;
   GridInfo g{};
+  int32_t umin[3] = {0, 0, 0}, umax[3] = {0, 0, 0};  // the grid bounds as given (kept for serialisation)
   Frame f{};
   int width = 1;
   Store cur, tmp;  // current band and the output buffer of the next op
@@ -1069,6 +1070,10 @@ static void copyDomainNoAlloc(lsg_domain *dst, const lsg_domain *src) {
 static int copyDomain(lsg_domain *dst, const lsg_domain *src) {
   dst->D = src->D;
   dst->g = src->g;
+  for (int k = 0; k < 3; ++k) {
+    dst->umin[k] = src->umin[k];
+    dst->umax[k] = src->umax[k];
+  }
   dst->f = src->f;
   dst->framed = src->framed;
   dst->width = src->width;
@@ -2052,31 +2057,48 @@ extern "C" {
 
 const char *lsg_last_error(void) { return g_err.c_str(); }
 
+// The grid of a level set (hrleGrid's index range per boundary condition:
+// infinite directions reach the grid end points, a periodic direction ends
+// one point before its upper bound).
+static int setGrid(lsg_domain *d, int D, const int32_t *gmin, const int32_t *gmax, const int *bc, double dx) {
+  GridInfo g{};
+  for (int k = 0; k < 3; ++k) {
+    g.lo[k] = g.hi[k] = 0;
+    g.bc[k] = REFLECTIVE;
+  }
+  for (int k = 0; k < D; ++k) {
+    const int b = bc[k];
+    g.bc[k] = b;
+    switch (b) {
+    case INFINITE: g.lo[k] = -INF_IDX; g.hi[k] = INF_IDX; break;
+    case PERIODIC: g.lo[k] = gmin[k]; g.hi[k] = gmax[k] - 1; break;
+    case POS_INFINITE: g.lo[k] = gmin[k]; g.hi[k] = INF_IDX; break;
+    case NEG_INFINITE: g.lo[k] = -INF_IDX; g.hi[k] = gmax[k]; break;
+    case REFLECTIVE: g.lo[k] = gmin[k]; g.hi[k] = gmax[k]; break;
+    default: return fail(-1, "unknown boundary condition");
+    }
+  }
+  g.D = D;
+  g.dx = dx;
+  d->D = D;
+  d->g = g;
+  for (int k = 0; k < 3; ++k) {
+    d->umin[k] = (k < D) ? gmin[k] : 0;
+    d->umax[k] = (k < D) ? gmax[k] : 0;
+  }
+  return 0;
+}
+
 int lsg_domain_create(int D, const int32_t *gmin, const int32_t *gmax, const int *bc, double grid_delta,
                       lsg_domain **out) {
   if (D != 2 && D != 3) return fail(-1, "dimension must be 2 or 3");
   if (!(grid_delta > 0)) return fail(-1, "gridDelta must be positive");
   if (int e = ctxInit()) return e;
   auto *d = new lsg_domain();
-  d->D = D;
-  for (int k = 0; k < 3; ++k) {
-    d->g.lo[k] = d->g.hi[k] = 0;
-    d->g.bc[k] = REFLECTIVE;
+  if (int e = setGrid(d, D, gmin, gmax, bc, grid_delta)) {
+    delete d;
+    return e;
   }
-  for (int k = 0; k < D; ++k) {
-    int b = bc[k];
-    d->g.bc[k] = b;
-    switch (b) {
-    case INFINITE: d->g.lo[k] = -INF_IDX; d->g.hi[k] = INF_IDX; break;
-    case PERIODIC: d->g.lo[k] = gmin[k]; d->g.hi[k] = gmax[k] - 1; break;
-    case POS_INFINITE: d->g.lo[k] = gmin[k]; d->g.hi[k] = INF_IDX; break;
-    case NEG_INFINITE: d->g.lo[k] = -INF_IDX; d->g.hi[k] = gmax[k]; break;
-    case REFLECTIVE: d->g.lo[k] = gmin[k]; d->g.hi[k] = gmax[k]; break;
-    default: delete d; return fail(-1, "unknown boundary condition");
-    }
-  }
-  d->g.D = D;
-  d->g.dx = grid_delta;
   // empty level set: one positive undefined run (Domain constructor, lsDomain.hpp:76-77)
   std::vector<int32_t> idx(D);
   for (int k = 0; k < D; ++k) idx[k] = d->g.lo[k];
@@ -2197,6 +2219,201 @@ int lsg_domain_point_data_get(const lsg_domain *d, int components, int i, double
 }
 int lsg_domain_point_data_clear(lsg_domain *d) {
   pdClear(d);
+  return 0;
+}
+
+// ------------------------------------------------------- serialisation ---
+// Domain::serialize / deserialize (lsDomain.hpp:188-268).  The envelope is
+// the reference's: "lsDomain", the format version byte (0), the grid, the
+// HRLE domain, the level-set width as uint32 and a byte saying whether
+// point data follows.  ViennaHRLE's grid and domain serialisers and
+// ViennaCore's PointData serialiser write the three inner sections; they
+// are not part of the reference, so these sections have a layout of this
+// library (native byte order):
+//   grid:  "lsgGrid" NUL, uint8 D, per direction int32 minimum, int32
+//          maximum (the bounds as given), uint8 boundary condition, then
+//          float64 gridDelta
+//   runs:  "lsgRuns" NUL, uint64 n, n x D int32 run starts (HRLE order,
+//          entry 0 at the grid minimum), n float64 run values
+//   point data: uint32 number of scalar arrays, each uint32 label length,
+//          the label, uint64 tuples, the float64 values; then the vector
+//          arrays alike with 3 float64 per tuple
+extern "C++" {
+namespace {
+struct ByteOut {
+  std::vector<uint8_t> b;
+  void put(const void *p, size_t n) {
+    const uint8_t *c = (const uint8_t *)p;
+    b.insert(b.end(), c, c + n);
+  }
+  template <class T> void val(T x) { put(&x, sizeof(T)); }
+};
+struct ByteIn {
+  const uint8_t *p;
+  size_t n, o;
+  bool get(void *dst, size_t k) {
+    if (k > n - o) return false;
+    std::memcpy(dst, p + o, k);
+    o += k;
+    return true;
+  }
+  template <class T> bool val(T &x) { return get(&x, sizeof(T)); }
+};
+}  // namespace
+static constexpr char DOMAIN_FORMAT_VERSION = 0;  // LS_DOMAIN_SERIALIZATION_VERSION (lsDomain.hpp:16)
+
+static int serializeDomain(lsg_domain *d, std::vector<uint8_t> &bytes) {
+  ByteOut o;
+  o.put("lsDomain", 8);
+  o.val<char>(DOMAIN_FORMAT_VERSION);
+  o.put("lsgGrid", 8);
+  o.val<uint8_t>((uint8_t)d->D);
+  for (int k = 0; k < d->D; ++k) {
+    o.val<int32_t>(d->umin[k]);
+    o.val<int32_t>(d->umax[k]);
+    o.val<uint8_t>((uint8_t)d->g.bc[k]);
+  }
+  o.val<double>(d->g.dx);
+  std::vector<int32_t> idx;
+  std::vector<double> val;
+  if (int e = downloadRuns(d, idx, val)) return e;
+  o.put("lsgRuns", 8);
+  o.val<uint64_t>((uint64_t)val.size());
+  o.put(idx.data(), idx.size() * sizeof(int32_t));
+  o.put(val.data(), val.size() * sizeof(double));
+  o.val<uint32_t>((uint32_t)d->width);
+  const char hasPointData = d->pd.empty() ? 0 : 1;
+  o.val<char>(hasPointData);
+  if (hasPointData) {
+    for (int comps : {1, 3}) {
+      const int na = lsg_domain_point_data_count(d, comps);
+      o.val<uint32_t>((uint32_t)na);
+      for (int i = 0; i < na; ++i) {
+        const std::string &label = d->pd[pdIndex(d, comps, i)].label;
+        o.val<uint32_t>((uint32_t)label.size());
+        o.put(label.data(), label.size());
+        const int64_t len = lsg_domain_point_data_size(d, comps, i);
+        if (len < 0) return (int)len;
+        std::vector<double> v((size_t)len * comps);
+        if (int e = lsg_domain_point_data_get(d, comps, i, v.data(), len)) return e;
+        o.val<uint64_t>((uint64_t)len);
+        o.put(v.data(), v.size() * sizeof(double));
+      }
+    }
+  }
+  bytes.swap(o.b);
+  return 0;
+}
+}  // extern "C++"
+
+int lsg_domain_serialize(lsg_domain *d, uint8_t *buf, int64_t cap, int64_t *size) {
+  if (int e = ctxInit()) return e;
+  std::vector<uint8_t> bytes;
+  if (int e = serializeDomain(d, bytes)) return e;
+  if (size) *size = (int64_t)bytes.size();
+  if (!buf) return 0;
+  if (cap < (int64_t)bytes.size()) return fail(-1, "serialize: buffer too small");
+  std::memcpy(buf, bytes.data(), bytes.size());
+  return 0;
+}
+
+// The stream is read completely before the level set changes: a stream
+// that is cut short or malformed leaves it as it was.  As in the
+// reference, point data the level set holds stays when the stream has
+// none (lsDomain.hpp:262-266).
+int lsg_domain_deserialize(lsg_domain *d, const uint8_t *buf, int64_t size) {
+  if (int e = ctxInit()) return e;
+  if (!buf || size < 0) return fail(-1, "deserialize: no data");
+  ByteIn in{buf, (size_t)size, 0};
+  char tag[8];
+  if (!in.get(tag, 8) || std::memcmp(tag, "lsDomain", 8) != 0)
+    return fail(-60, "Reading Domain from stream failed. Header could not be found.");
+  char version = 0;
+  if (!in.val(version)) return fail(-61, "Reading Domain from stream failed: stream ends early.");
+  if (version > DOMAIN_FORMAT_VERSION)
+    return fail(-62, "Reading Domain of version " + std::to_string((int)version) + " with reader of version " +
+                         std::to_string((int)DOMAIN_FORMAT_VERSION) + " failed.");
+  const auto early = [] { return fail(-61, "Reading Domain from stream failed: stream ends early or is malformed."); };
+  uint8_t D = 0;
+  if (!in.get(tag, 8) || std::memcmp(tag, "lsgGrid", 8) != 0 || !in.val(D)) return early();
+  if ((int)D != d->D) return fail(-63, "Reading Domain from stream failed: the stream holds a " + std::to_string(D) +
+                                           "D level set, this one is " + std::to_string(d->D) + "D.");
+  int32_t gmin[3] = {0, 0, 0}, gmax[3] = {0, 0, 0};
+  int bc[3] = {0, 0, 0};
+  for (int k = 0; k < D; ++k) {
+    uint8_t b = 0;
+    if (!in.val(gmin[k]) || !in.val(gmax[k]) || !in.val(b)) return early();
+    bc[k] = b;
+  }
+  double dx = 0;
+  if (!in.val(dx) || !(dx > 0)) return early();
+  uint64_t n = 0;
+  if (!in.get(tag, 8) || std::memcmp(tag, "lsgRuns", 8) != 0 || !in.val(n)) return early();
+  if (n == 0 || n > (uint64_t)(size / (int64_t)(D * sizeof(int32_t) + sizeof(double)))) return early();
+  std::vector<int32_t> idx((size_t)n * D);
+  std::vector<double> val((size_t)n);
+  uint32_t width = 0;
+  char hasPointData = 0;
+  if (!in.get(idx.data(), idx.size() * sizeof(int32_t)) || !in.get(val.data(), val.size() * sizeof(double)) ||
+      !in.val(width) || !in.val(hasPointData))
+    return early();
+  struct Arr {
+    std::string label;
+    int comps;
+    std::vector<double> v;
+  };
+  std::vector<Arr> arrays;
+  if (hasPointData == 1) {
+    for (int comps : {1, 3}) {
+      uint32_t na = 0;
+      if (!in.val(na)) return early();
+      for (uint32_t i = 0; i < na; ++i) {
+        uint32_t len = 0;
+        if (!in.val(len) || len > size) return early();
+        std::string label(len, '\0');
+        uint64_t tuples = 0;
+        if (!in.get(&label[0], len) || !in.val(tuples) || tuples > (uint64_t)size / (8u * comps)) return early();
+        std::vector<double> v((size_t)tuples * comps);
+        if (!in.get(v.data(), v.size() * sizeof(double))) return early();
+        arrays.push_back(Arr{label, comps, std::move(v)});
+      }
+    }
+  }
+  // build the level set on its own, then take it over
+  std::unique_ptr<lsg_domain> t(new lsg_domain());
+  if (int e = setGrid(t.get(), D, gmin, gmax, bc, dx)) return e;
+  if (int e = uploadRuns(t.get(), idx, val, (int)width)) return e;
+  for (const Arr &a : arrays)
+    if (int e = lsg_domain_point_data_insert(t.get(), a.label.c_str(), a.comps, a.v.data(), (int64_t)(a.v.size() / a.comps), 0))
+      return e;
+  d->D = t->D;
+  d->g = t->g;
+  for (int k = 0; k < 3; ++k) {
+    d->umin[k] = t->umin[k];
+    d->umax[k] = t->umax[k];
+  }
+  d->f = t->f;
+  d->framed = t->framed;
+  d->width = t->width;
+  if (int e = ensureCur(d, t->cur.nwAlloc, t->cur.b.cap)) return e;
+  launchCopyBand(g_ctx.s, t->cur.b, d->cur.b, t->f);
+  d->hostN = t->hostN;
+  if (hasPointData == 1) {
+    pdClear(d);
+    if (int e = pdCopy(d, t.get())) return e;
+  }
+  LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  return 0;
+}
+
+int lsg_domain_get_grid(const lsg_domain *d, int *D, int32_t *gmin, int32_t *gmax, int *bc, double *dx) {
+  if (D) *D = d->D;
+  for (int k = 0; k < d->D; ++k) {
+    if (gmin) gmin[k] = d->umin[k];
+    if (gmax) gmax[k] = d->umax[k];
+    if (bc) bc[k] = d->g.bc[k];
+  }
+  if (dx) *dx = d->g.dx;
   return 0;
 }
 // ------------------------------------------------------ surface & Chamfer ---

@@ -5,11 +5,11 @@ from ._dims import (Advect, BooleanOperation, BooleanOperationEnum, BoundaryCond
                     CompareChamfer, CompareVolume,
                     Expand, FinalizeStencilLocalLaxFriedrichs, HostVelocityField, MakeGeometry, MarkVoidPoints,
                     MaterialVelocityField, Plane, PointData, PrepareStencilLocalLaxFriedrichs, Prune, Reduce, Sphere,
-                    SpatialSchemeEnum, TemporalSchemeEnum, VelocityField, domain_class)
+                    SpatialSchemeEnum, TemporalSchemeEnum, VelocityField, Reader, Writer, domain_class)
 
 D = 2
 Domain = domain_class(2)
 
 __all__ = ["Advect", "BooleanOperation", "BooleanOperationEnum", "BoundaryConditionEnum", "Box", "CompareArea", "CompareChamfer", "CompareVolume",
            "Domain", "Expand", "FinalizeStencilLocalLaxFriedrichs", "HostVelocityField", "MakeGeometry", "MarkVoidPoints", "MaterialVelocityField", "Plane",
-           "PointData", "PrepareStencilLocalLaxFriedrichs", "Prune", "Reduce", "Sphere", "SpatialSchemeEnum", "TemporalSchemeEnum", "VelocityField"]
+           "PointData", "PrepareStencilLocalLaxFriedrichs", "Prune", "Reduce", "Sphere", "SpatialSchemeEnum", "TemporalSchemeEnum", "VelocityField", "Reader", "Writer"]
