@@ -33,7 +33,8 @@ def sphere(vls, r=6.0):
     return d
 
 
-@pytest.mark.parametrize("spatial,temporal", [(0, 0), (1, 2), (11, 2), (6, 1), (10, 0)])
+@pytest.mark.parametrize("spatial,temporal", [(0, 0), (1, 2), (11, 2), (6, 1), (10, 0), (2, 0), (3, 2), (7, 0),
+                                              (8, 1)])
 def test_host_callback_matches_builtin(spatial, temporal):
     import viennals_amd as vls
     f = NormalPower(dir=0, k=2, b=1.0, a_pos=2.3, a_neg=0.5, vec=(0.2, 0.0, -0.1))
@@ -90,10 +91,79 @@ def test_host_callback_arguments():
     np.testing.assert_allclose(norms, 1.0, rtol=1e-12)
 
 
-def test_host_callback_rejected_with_lax_friedrichs():
+def test_host_callback_llf_box_calls():
+    """LLF calls the velocity at the 3^D box around every active point with
+    coordinate + offset * gridDelta, the box point's normal and the centre's
+    point id (lsLocalLaxFriedrichs.hpp:193-222), after the scheme's call."""
+    import viennals_amd as vls
+    d = sphere(vls, 4.0)
+    calls = []
+
+    class Rec(vls.HostVelocityField):
+        def getScalarVelocity(self, coordinate, material, normal, pointId):
+            calls.append((tuple(coordinate), pointId))
+            return 1.0
+
+    a = vls.Advect(d, Rec())
+    a.setSpatialScheme(vls.SpatialSchemeEnum.LOCAL_LAX_FRIEDRICHS_1ST_ORDER)
+    a.setTemporalScheme(vls.TemporalSchemeEnum.FORWARD_EULER)
+    a.prepareLS()
+    idx, val = d.get_runs()
+    active = (np.abs(val) < 1e300) & (np.abs(val) <= 0.5)
+    a.setSingleStep(True)
+    a.apply()
+    n = int(active.sum())
+    assert len(calls) == 28 * n
+    centres = np.array([calls[28 * j][0] for j in range(n)])
+    np.testing.assert_array_equal(centres, idx[active] * 0.5)
+    for j in (0, n // 2, n - 1):
+        c = np.array(calls[28 * j][0])
+        for q in range(27):
+            off = np.array([q % 3 - 1, (q // 3) % 3 - 1, q // 9 - 1]) * 0.5
+            np.testing.assert_array_equal(np.array(calls[28 * j + 1 + q][0]), c + off)
+            assert calls[28 * j + 1 + q][1] == calls[28 * j][1]
+
+
+@pytest.mark.parametrize("spatial", [4, 9])
+def test_host_callback_rejected_with_analytical_and_stencil_llf(spatial):
     import viennals_amd as vls
     d = sphere(vls, 3.0)
     a = vls.Advect(d, vls.HostVelocityField())
-    a.setSpatialScheme(vls.SpatialSchemeEnum.LAX_FRIEDRICHS_1ST_ORDER)
+    a.setSpatialScheme(spatial)
     with pytest.raises(vls.LsgError, match="host-evaluated"):
         a.apply()
+
+
+@pytest.mark.parametrize("spatial,temporal", [(3, 2), (8, 1), (2, 0)])
+def test_host_callback_several_level_sets(spatial, temporal):
+    """A host callback with a material-dependent velocity equals the
+    built-in per-material functors with LF (global alphas over the points'
+    materials) and LLF (box velocities per material) on an etch stop."""
+    import viennals_amd as vls
+    from test_gpu_multimaterial import plane2
+    from test_gpu_parity import to_gpu
+    rates = [(-0.05, -0.2), (-0.1, -1.0)]
+    fs = [NormalPower(dir=1, k=1, b=b, a_pos=ap, a_neg=0.0) for b, ap in rates]
+
+    class Host(vls.HostVelocityField):
+        def getScalarVelocity(self, coordinate, material, normal, pointId):
+            return fs[material].scalar(normal)
+
+    builtin = vls.MaterialVelocityField([vls.VelocityField.normal_power(dir=1, k=1, b=b, a_pos=ap, a_neg=0.0)
+                                         for b, ap in rates])
+    out = []
+    for vel in (builtin, Host()):
+        lv = [to_gpu(plane2(-2.3)), to_gpu(plane2(0.0))]
+        a = vls.Advect()
+        for g in lv:
+            a.insertNextLevelSet(g)
+        a.setVelocityField(vel)
+        a.setSpatialScheme(spatial)
+        a.setTemporalScheme(temporal)
+        a.setTimeStepRatio(0.4)
+        a.setAdvectionTime(3.0)
+        a.apply()
+        out.append(([g.get_runs() for g in lv], a.getAdvectedTime()))
+    assert out[0][1] == out[1][1]
+    for (i0, v0), (i1, v1) in zip(out[0][0], out[1][0]):
+        assert np.array_equal(i0, i1) and np.array_equal(v0.view(np.int64), v1.view(np.int64))

@@ -103,6 +103,27 @@ def test_trench_mask_stop_layer_3d():
     run_both([stop, mask, top], 11, 2, 0.4, 0.0, mat, steps=6)
 
 
+@pytest.mark.parametrize("spatial,temporal", [(2, 0), (3, 2)])
+def test_lax_friedrichs_etch_stop_layer(spatial, temporal):
+    """LF1/LF2 with a stop layer: findGlobalAlphas takes every active
+    point's material (lsAdvect.hpp:133-171), the global alphas bound dt."""
+    say(f"two level sets, etch stop, Lax-Friedrichs: spatial {spatial} temporal {temporal}")
+    mats = [dict(kind=1, dir=2, k=1, b=-0.05, a_pos=-0.2, a_neg=0.0),
+            dict(kind=1, dir=2, k=1, b=-0.1, a_pos=-1.0, a_neg=0.0)]
+    run_both([plane2(-2.3), plane2(0.0)], spatial, temporal, 0.4, 4.0, mats)
+
+
+def test_lax_friedrichs_mask_2d():
+    say("mask over a plane (2D), LF2 + RK2")
+    gmin, gmax, bc = [-16, -20], [16, 20], [REF, INF]
+    mask = po.Domain(2, gmin, gmax, bc, 1.0)
+    i, v = cases.box_points(2, [2.5, -1.5], [12.5, 2.0], 1.0, gmin, gmax)
+    mask.from_points(i, v, 2)
+    top = plane2(0.0)
+    top.boolean(mask, 1)
+    run_both([mask, top], 3, 1, 0.4, 3.0, [dict(b=0.0), dict(kind=1, dir=1, k=1, b=-0.2, a_pos=-1.0, a_neg=0.0)])
+
+
 def test_too_many_level_sets_and_unsupported_scheme():
     import viennals_amd as vls
     say("multi-material argument errors")
@@ -111,7 +132,7 @@ def test_too_many_level_sets_and_unsupported_scheme():
     for g in d:
         a.insertNextLevelSet(g)
     a.setVelocityField(vls.VelocityField.constant(-1.0))
-    a.setSpatialScheme(3)  # LF2: global alphas per material are not supported
+    a.setSpatialScheme(12)  # learned WENO5 takes one level set
     with pytest.raises(vls.LsgError, match="several level sets"):
         a.apply()
 

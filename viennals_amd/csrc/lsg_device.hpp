@@ -240,9 +240,12 @@ struct RateParams {
   double velAlpha[3];  // getDissipationAlpha of the built-in functor (LLFA)
   // learned weno5 weights: network parameters as bf16 (WENO5 learned only)
   const uint16_t *mlp;
-  // host-evaluated velocities: scalar and vector per active point, in
-  // active-list order (4 doubles each); null for the built-in functor
+  // host-evaluated velocities: scalar and vector (4 doubles) per record
+  // and material, hostRec records per active point in active-list order
+  // (the scheme's call, then the LF global-alpha call or the 3^D LLF box);
+  // null for the built-in functor
   const double *hostVel;
+  int hostRec;
   // several level sets: built-in functor of material m where matSet[m]
   // (the others use the functor above); kind, dir, k, b, aPos, aNeg, vec
   int matSet[8];

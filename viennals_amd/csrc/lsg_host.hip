@@ -871,7 +871,7 @@ struct lsg_advect {
   lsg_vector_velocity_fn hostVector = nullptr;
   void *hostUser = nullptr;
   double *velIn = nullptr, *velOut = nullptr;  // device: 7 / 4 (x materials) doubles per active point
-  int velMat = 1;
+  int velMat = 1, velRec = 1;
   uint32_t *dlist = nullptr, *ndef = nullptr;
   size_t velCap = 0;
   std::vector<double> velInH, velOutH;
@@ -972,41 +972,67 @@ static void pdUpdate(lsg_advect *a, lsg_domain *d, const double *dt, int nmat) {
 // With several level sets the callbacks run for every material m of every
 // active point (velocities per (point, m), 4 doubles each): the material a
 // point's scheme call uses is only known on the device.
-static int hostVelocities(lsg_advect *a, lsg_domain *d, int nmat = 1) {
+// The LF schemes get findGlobalAlphas' call as a second record per point
+// and the LLF schemes the 3^D alpha box (kVelInputs): the global alphas are
+// then reduced here from the callbacks' results -- the maximum does not
+// depend on the order -- and written to the device scalars.  The box and
+// global-alpha calls are made for every active point, also where the
+// reference skips the box (a zero Hamiltonian, lsLocalLaxFriedrichs.hpp:
+// 178-180): the velocities used are the same, a callback counting its calls
+// sees more of them.
+static int hostVelocities(lsg_advect *a, lsg_domain *d, int nmat = 1, const LowerViews *lower = nullptr) {
   const size_t cap = d->cur.b.cap;
-  if (a->velCap < cap || a->velMat < nmat) {
+  const int R = hostVelocityRecords(a->spatial, d->D);
+  if (a->velCap < cap || a->velMat < nmat || a->velRec < R) {
     for (void *p : {(void *)a->velIn, (void *)a->velOut, (void *)a->dlist}) if (p) (void)hipFree(p);
     a->velIn = a->velOut = nullptr;
     a->dlist = nullptr;
     a->velCap = 0;
-    LSG_HIP(hipMalloc(&a->velIn, cap * 7 * sizeof(double)));
-    LSG_HIP(hipMalloc(&a->velOut, cap * 4 * (size_t)nmat * sizeof(double)));
+    LSG_HIP(hipMalloc(&a->velIn, cap * (size_t)R * 8 * sizeof(double)));
+    LSG_HIP(hipMalloc(&a->velOut, cap * (size_t)R * 4 * (size_t)nmat * sizeof(double)));
     LSG_HIP(hipMalloc(&a->dlist, cap * sizeof(uint32_t)));
     a->velCap = cap;
     a->velMat = nmat;
+    a->velRec = R;
   }
   if (!a->ndef) LSG_HIP(hipMalloc(&a->ndef, sizeof(uint32_t)));
-  launchVelInputs(g_ctx.s, d->D, d->cur.b, d->f, d->g, a->rp, g_ws->w, a->dlist, a->ndef, a->velIn, g_ctx.sc);
+  LowerViews none{};
+  launchVelInputs(g_ctx.s, d->D, d->cur.b, d->f, d->g, a->rp, lower ? *lower : none, g_ws->w, a->dlist, a->ndef,
+                  a->velIn, g_ctx.sc);
   uint32_t nact = 0;
   LSG_HIP(hipMemcpyAsync(&nact, g_ws->w.ncand, sizeof(uint32_t), hipMemcpyDeviceToHost, g_ctx.s));
   LSG_HIP(hipStreamSynchronize(g_ctx.s));
-  a->velInH.resize((size_t)nact * 7);
-  a->velOutH.resize((size_t)nact * 4 * nmat);
-  if (nact) {
-    LSG_HIP(hipMemcpy(a->velInH.data(), a->velIn, (size_t)nact * 7 * sizeof(double), hipMemcpyDeviceToHost));
-  }
-  for (uint32_t j = 0; j < nact; ++j) {
-    const double *in = &a->velInH[(size_t)j * 7];
+  const size_t nrec = (size_t)nact * R;
+  a->velInH.resize(nrec * 8);
+  a->velOutH.resize(nrec * 4 * nmat);
+  if (nact) LSG_HIP(hipMemcpy(a->velInH.data(), a->velIn, nrec * 8 * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t r = 0; r < nrec; ++r) {
+    const double *in = &a->velInH[r * 8];
     const uint64_t pid = (uint64_t)in[6];
     for (int m = 0; m < nmat; ++m) {
-      double *o = &a->velOutH[((size_t)j * nmat + m) * 4];
+      double *o = &a->velOutH[(r * nmat + m) * 4];
       o[0] = a->hostScalar ? a->hostScalar(in, m, in + 3, pid, a->hostUser) : 0.0;
       o[1] = o[2] = o[3] = 0.0;
       if (a->hostVector) a->hostVector(in, m, in + 3, pid, a->hostUser, o + 1);
     }
   }
-  if (nact)
-    LSG_HIP(hipMemcpy(a->velOut, a->velOutH.data(), (size_t)nact * 4 * nmat * sizeof(double), hipMemcpyHostToDevice));
+  if (nact) LSG_HIP(hipMemcpy(a->velOut, a->velOutH.data(), nrec * 4 * nmat * sizeof(double), hipMemcpyHostToDevice));
+  if (R == 2) {  // LF: findGlobalAlphas from the second record of every point (lsAdvect.hpp:164-171)
+    double al[3] = {0, 0, 0};
+    for (uint32_t j = 0; j < nact; ++j) {
+      const double *in = &a->velInH[((size_t)j * R + 1) * 8];
+      const int m = std::max(0, std::min(nmat - 1, (int)in[7]));
+      const double *o = &a->velOutH[(((size_t)j * R + 1) * nmat + m) * 4];
+      for (int k = 0; k < d->D; ++k) {
+        const double t = std::fabs((o[0] + o[1 + k]) * in[3 + k]);
+        al[k] = (al[k] < t) ? t : al[k];  // std::max
+      }
+    }
+    unsigned long long bits[3];
+    std::memcpy(bits, al, sizeof(bits));
+    LSG_HIP(hipMemcpyAsync(g_ctx.sc->alphaBits, bits, sizeof(bits), hipMemcpyHostToDevice, g_ctx.s));
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+  }
   return 0;
 }
 
@@ -1014,12 +1040,16 @@ static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *d
   doExpand(d, prepareWidth(a->spatial));  // Advect::prepareLS (lsAdvect.hpp:973-1015)
   RateParams rp = a->rp;
   rp.hostVel = nullptr;
+  rp.hostRec = hostVelocityRecords(a->spatial, d->D);
   rp.dissOut = a->saveVelocities ? a->diss : nullptr;
-  if (a->hostScalar || a->hostVector) {
+  const bool host = a->hostScalar || a->hostVector;
+  if (host) {
     if (!a->hostErr) a->hostErr = hostVelocities(a, d);
     rp.hostVel = a->velOut;
   }
-  if (a->spatial == 2 || a->spatial == 3) launchGlobalAlphas(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, g_ctx.sc);
+  // LF global alphas: from the host calls, or the built-in functor here
+  if ((a->spatial == 2 || a->spatial == 3) && !host)
+    launchGlobalAlphas(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, g_ctx.sc);
   if (a->ignoreVoids && !a->hostErr) a->hostErr = markVoids(d, rp);
   if (a->profiling && a->evUsed + 2 <= (int)a->evPool.size()) {
     // events bracket the rate kernel itself
@@ -1353,10 +1383,14 @@ static void stageMM(lsg_advect *a, const double *maxDt, double *dtOut) {
   RateParams rp = a->rp;
   rp.dissOut = a->saveVelocities ? a->diss : nullptr;
   rp.hostVel = nullptr;
-  if (a->hostScalar || a->hostVector) {
-    if (!a->hostErr) a->hostErr = hostVelocities(a, top, L);
+  rp.hostRec = hostVelocityRecords(a->spatial, top->D);
+  const bool host = a->hostScalar || a->hostVector;
+  if (host) {
+    if (!a->hostErr) a->hostErr = hostVelocities(a, top, L, &lw);
     rp.hostVel = a->velOut;
   }
+  if ((a->spatial == 2 || a->spatial == 3) && !host)
+    launchGlobalAlphasMM(g_ctx.s, top->D, top->cur.b, top->f, top->g, lw, rp, g_ctx.sc);
   if (a->ignoreVoids && !a->hostErr) a->hostErr = markVoids(top, rp);
   launchRatesMM(g_ctx.s, top->D, top->cur.b, top->f, top->g, lw, rp, a->rates, g_ctx.sc, maxDt, dtOut, g_ws->w);
   OpParams op{};
@@ -1503,9 +1537,7 @@ static int advectApply(lsg_advect *a) {
   const bool multi = a->levelSets.size() > 1;
   if (multi) {
     if (a->levelSets.size() > (size_t)MAX_LEVEL_SETS) return fail(-3, "Advect: at most 8 level sets are supported");
-    if (a->spatial == 2 || a->spatial == 3 || a->spatial == 12)
-      return fail(-3, "Advect: with several level sets the Lax-Friedrichs (global alpha) and learned WENO5 schemes "
-                      "are not supported");
+    if (a->spatial == 12) return fail(-3, "Advect: with several level sets the learned WENO5 scheme is not supported");
   }
   if (prepareWidth(a->spatial) < 0) return fail(-4, "Advect: Discretization scheme not found.");
   if (!(a->cfl > 0)) return fail(-6, "Advect: time step ratio must be positive");
@@ -1521,10 +1553,9 @@ static int advectApply(lsg_advect *a) {
   a->rp.sllfBaseEps = std::cbrt(std::numeric_limits<double>::epsilon());
   // WENO receives dissipationAlpha as its calcNormal flag (lsAdvect.hpp:670-676)
   a->rp.calcNormals = (a->spatial >= 10) ? (a->dissipationAlpha != 0.0) : a->calcNormals;
-  if ((a->hostScalar || a->hostVector) &&
-      !(a->spatial == 0 || a->spatial == 1 || a->spatial == 5 || a->spatial == 6 || a->spatial >= 10))
-    return fail(-9, "Advect: host-evaluated velocities are supported with the Engquist-Osher, local-local "
-                    "Lax-Friedrichs and WENO schemes");
+  if ((a->hostScalar || a->hostVector) && (a->spatial == 4 || a->spatial == 9 || a->spatial == 12))
+    return fail(-9, "Advect: host-evaluated velocities are supported with the Engquist-Osher, Lax-Friedrichs, "
+                    "local Lax-Friedrichs, local-local Lax-Friedrichs and WENO schemes");
   if (a->spatial == 12 && !a->mlpDev)
     return fail(-8, "Advect: learned WENO5 selected but no network parameters were set");
   a->rp.mlp = a->mlpDev;

@@ -635,6 +635,10 @@ struct Vel {
   double b, aPos, aNeg;
   double vec[3];
   double alpha[3];  // getDissipationAlpha (LLFA): constant per direction
+  // host-evaluated velocities at the 3^D points of the local Lax-Friedrichs
+  // alpha box: box point q has (scalar, vector[3]) at box[q * boxStride]
+  const double *box = nullptr;
+  int boxStride = 4;
   // built-in VelocityField::getScalarVelocity (lsVelocityField.hpp:17-21)
   LSG_HD double scalar(const double n[3]) const {
     if (kind == 0) return b;
@@ -1187,6 +1191,46 @@ LSG_HD inline RateOut schemeLLFA(const BandView &b, const int32_t c[3], double p
   return {totalGrad, dissipation};
 }
 
+// Normal at point q of the local Lax-Friedrichs alpha box around c
+// (lsLocalLaxFriedrichs.hpp:193-215): central differences of the boundary
+// mapped neighbours, normalised; q = (ox + 1) + 3 (oy + 1) + 9 (oz + 1).
+template <int D> LSG_HD inline void llfBoxNormal(const BandView &b, const int32_t c[3], int q, double normal[3]) {
+  const int ni[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
+  int32_t nc[3] = {0, 0, 0};
+  LSG_UNROLL
+  for (int d = 0; d < D; ++d) nc[d] = b.g.map(d, (int64_t)c[d] + ni[d]);
+  normal[0] = normal[1] = normal[2] = 0.;
+  double nm = 0.;
+  BandView::Loc l[2 * D + 1];
+  l[2 * D] = b.locate(nc[0], nc[1], nc[2]);
+  LSG_UNROLL
+  for (int dir = 0; dir < D; ++dir) {
+    int32_t pp[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d) {
+      pp[d] = b.g.map(d, (int64_t)c[d] + ni[d] + (d == dir ? 1 : 0));
+      pn[d] = b.g.map(d, (int64_t)c[d] + ni[d] - (d == dir ? 1 : 0));
+    }
+    l[dir] = b.locate(pp[0], pp[1], pp[2]);
+    l[D + dir] = b.locate(pn[0], pn[1], pn[2]);
+  }
+  double bv[2 * D + 1];
+  b.values<2 * D + 1>(l, bv);
+  const double center = bv[2 * D];
+  LSG_UNROLL
+  for (int dir = 0; dir < D; ++dir) {
+    const double neg = bv[D + dir];
+    const double pos = bv[dir];
+    const double dP = (pos - center) / b.g.dx;
+    const double dN = (center - neg) / b.g.dx;
+    normal[dir] = (dP + dN) * 0.5;
+    nm += normal[dir] * normal[dir];
+  }
+  nm = 1. / sqrt(nm);
+  LSG_UNROLL
+  for (int dir = 0; dir < D; ++dir) normal[dir] *= nm;
+}
+
 // LocalLaxFriedrichs::operator() (lsLocalLaxFriedrichs.hpp:65-241)
 template <int D>
 LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double phi0, int order, const Vel &vel,
@@ -1225,44 +1269,13 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
   constexpr int numNeighbors = (D == 3) ? 27 : 9;
   LSG_UNROLL
   for (int q = 0; q < numNeighbors; ++q) {  // box loop (lsLocalLaxFriedrichs.hpp:193-231), x fastest
-    const int ni[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
-    int32_t nc[3] = {0, 0, 0};
-    LSG_UNROLL
-    for (int d = 0; d < D; ++d) nc[d] = b.g.map(d, (int64_t)c[d] + ni[d]);
-    double normal[3] = {0, 0, 0};
-    double nm = 0.;
-    BandView::Loc l[2 * D + 1];
-    l[2 * D] = b.locate(nc[0], nc[1], nc[2]);
+    double normal[3];
+    llfBoxNormal<D>(b, c, q, normal);
+    const double scaVel = vel.box ? vel.box[q * vel.boxStride] : vel.scalar(normal);
     LSG_UNROLL
     for (int dir = 0; dir < D; ++dir) {
-      int32_t pp[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
-      LSG_UNROLL
-      for (int d = 0; d < D; ++d) {
-        pp[d] = b.g.map(d, (int64_t)c[d] + ni[d] + (d == dir ? 1 : 0));
-        pn[d] = b.g.map(d, (int64_t)c[d] + ni[d] - (d == dir ? 1 : 0));
-      }
-      l[dir] = b.locate(pp[0], pp[1], pp[2]);
-      l[D + dir] = b.locate(pn[0], pn[1], pn[2]);
-    }
-    double bv[2 * D + 1];
-    b.values<2 * D + 1>(l, bv);
-    const double center = bv[2 * D];
-    LSG_UNROLL
-    for (int dir = 0; dir < D; ++dir) {
-      const double neg = bv[D + dir];
-      const double pos = bv[dir];
-      const double dP = (pos - center) / b.g.dx;
-      const double dN = (center - neg) / b.g.dx;
-      normal[dir] = (dP + dN) * 0.5;
-      nm += normal[dir] * normal[dir];
-    }
-    nm = 1. / sqrt(nm);
-    LSG_UNROLL
-    for (int dir = 0; dir < D; ++dir) normal[dir] *= nm;
-    const double scaVel = vel.scalar(normal);
-    LSG_UNROLL
-    for (int dir = 0; dir < D; ++dir) {
-      double tempAlpha = fabs((scaVel + vel.vec[dir]) * normal[dir]);
+      const double vv = vel.box ? vel.box[q * vel.boxStride + 1 + dir] : vel.vec[dir];
+      double tempAlpha = fabs((scaVel + vv) * normal[dir]);
       alpha[dir] = dmaxs(alpha[dir], tempAlpha);
       alphaAcc[dir] = dmaxs(alphaAcc[dir], tempAlpha);
     }
@@ -1416,7 +1429,10 @@ LSG_HD inline RateOut schemeSLLF(const BandView &b, const int32_t c[3], const Ve
 template <int D> LSG_HD inline void schemeNormal(const BandView &b, const int32_t c[3], double phi0, int scheme,
                                                  int order, bool calcNormals, double n[3]) {
   n[0] = n[1] = n[2] = 0.;
-  if (scheme == 3 /* S_LLLF */) {
+  // upwind-difference normal: LLLF and LLF always (lsLocalLocalLaxFriedrichs
+  // .hpp:118-124, lsLocalLaxFriedrichs.hpp:140-155), LF with
+  // calculateNormalVectors (lsLaxFriedrichs.hpp:127-141)
+  if (scheme == 3 /* S_LLLF */ || scheme == 4 /* S_LLF */ || (scheme == 2 /* S_LF */ && calcNormals)) {
     double st[D][5];
     gatherUpwind<D>(b, c, order, st);
     double nm = 0;
@@ -1432,7 +1448,7 @@ template <int D> LSG_HD inline void schemeNormal(const BandView &b, const int32_
     for (int i = 0; i < D; ++i) n[i] *= nm;
     return;
   }
-  if (!calcNormals) return;
+  if (!calcNormals || scheme == 2 /* S_LF */) return;
   double st[D][3];
   gatherAxes<D, 1>(b, c, st);
   double den = 0;
@@ -1509,9 +1525,10 @@ LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi
   return v;
 }
 
-// Per-point alphas of Advect::findGlobalAlphas (lsAdvect.hpp:150-178).
-template <int D> LSG_HD inline void globalAlphaPoint(const BandView &b, const int32_t c[3], const Vel &vel, double *la) {
-  double normal[3] = {0, 0, 0};
+// Normal of Advect::findGlobalAlphas (lsAdvect.hpp:150-162): phiPos -
+// phiNeg per direction, normalised.
+template <int D> LSG_HD inline void globalAlphaNormal(const BandView &b, const int32_t c[3], double normal[3]) {
+  normal[0] = normal[1] = normal[2] = 0.;
   double nm = 0.;
   double st[D][3];
   gatherAxes<D, 1>(b, c, st);
@@ -1523,6 +1540,11 @@ template <int D> LSG_HD inline void globalAlphaPoint(const BandView &b, const in
   nm = 1. / sqrt(nm);
   LSG_UNROLL
   for (int k = 0; k < D; ++k) normal[k] *= nm;
+}
+// Per-point alphas of Advect::findGlobalAlphas (lsAdvect.hpp:150-178).
+template <int D> LSG_HD inline void globalAlphaPoint(const BandView &b, const int32_t c[3], const Vel &vel, double *la) {
+  double normal[3];
+  globalAlphaNormal<D>(b, c, normal);
   const double sv = vel.scalar(normal);
   LSG_UNROLL
   for (int k = 0; k < D; ++k) {

@@ -1358,31 +1358,92 @@ __global__ __launch_bounds__(256) void kDefinedList(const double *val, const uin
   if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) *ndef = off;
 }
 
+// Values of the lower level sets and the top at grid point c (the lower
+// ones share the top's frame); lv[lw.n] = phiTop.
+__device__ inline void levelValuesAt(const BandView &b, const LowerViews &lw, const int32_t c[3], double phiTop,
+                                     double *lv) {
+  const BandView::Loc l = b.locate(c[0], c[1], c[2]);
+  for (int m = 0; m < lw.n; ++m) {
+    BandView v = b;
+    v.bits = lw.bits[m];
+    v.pre = lw.pre[m];
+    v.val = lw.val[m];
+    lv[m] = v.val[v.entryOf(l, v.bits[l.w], v.pre[l.w])];
+  }
+  lv[lw.n] = phiTop;
+}
+// Material of findGlobalAlphas (lsAdvect.hpp:133-141): the first level set,
+// bottom up, whose value is <= value + wrappingLayerEpsilon.
+__device__ inline int alphaMaterial(const double *lv, int L, double value) {
+  for (int m = 0; m < L; ++m)
+    if (lv[m] <= value + 1e-4) return m;
+  return L - 1;
+}
+
+LSG_HD inline int hostRecordsOf(int scheme, int D) {
+  if (scheme == S_LF) return 2;
+  if (scheme == S_LLF) return 1 + ((D == 3) ? 27 : 9);
+  return 1;
+}
+int hostVelocityRecords(int spatial, int D) { return hostRecordsOf(schemeIdOf(spatial), D); }
+
 // Inputs of VelocityField::getScalarVelocity / getVectorVelocity for every
-// active point (lsEngquistOsher.hpp:140-145, lsWENO.hpp:132-137): coordinate
-// index * gridDelta (3 components), the scheme's normal, and the point id
-// (rank among the defined points of the expanded band); 7 doubles each.
+// active point: coordinate index * gridDelta, the scheme's normal, the point
+// id (rank among the defined points of the expanded band) and -1 (called
+// for every material) -- the scheme's call (lsEngquistOsher.hpp:140-145,
+// lsWENO.hpp:132-137, lsLaxFriedrichs.hpp:136-141,
+// lsLocalLaxFriedrichs.hpp:157-162); for LF then findGlobalAlphas' call
+// with its own normal and the point's material (lsAdvect.hpp:145-171); for
+// LLF then the 3^D alpha box, coordinate + offset * gridDelta with the box
+// point's normal (lsLocalLaxFriedrichs.hpp:193-222).  8 doubles per record.
 template <int D>
-__global__ void kVelInputs(BandView b, const uint32_t *alist, const uint32_t *nact, const uint32_t *dlist,
-                           const uint32_t *ndef, RateParams rp, double *out) {
+__global__ void kVelInputs(BandView b, LowerViews lw, const uint32_t *alist, const uint32_t *nact,
+                           const uint32_t *dlist, const uint32_t *ndef, RateParams rp, double *out) {
   const uint32_t n = *nact, nd = *ndef;
   const int scheme = schemeIdOf(rp.spatial);
+  const int R = hostRecordsOf(scheme, D);
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const uint32_t i = alist[j];
     int32_t c[3];
     candCoords(b, i, b.pos[i], c);
+    const double phi0 = b.val[i];
     double nrm[3];
-    schemeNormal<D>(b, c, b.val[i], scheme, rp.order, rp.calcNormals != 0, nrm);
+    schemeNormal<D>(b, c, phi0, scheme, rp.order, rp.calcNormals != 0, nrm);
     uint32_t lo = 0, hi = nd;  // first defined entry >= i
     while (lo < hi) {
       const uint32_t m = (lo + hi) >> 1;
       if (dlist[m] < i) lo = m + 1;
       else hi = m;
     }
-    double *o = out + 7 * (size_t)j;
-    for (int d = 0; d < 3; ++d) o[d] = (d < D) ? (double)c[d] * b.g.dx : 0.0;
+    double coord[3];
+    for (int d = 0; d < 3; ++d) coord[d] = (d < D) ? (double)c[d] * b.g.dx : 0.0;
+    double *o = out + 8 * (size_t)j * R;
+    for (int d = 0; d < 3; ++d) o[d] = coord[d];
     for (int d = 0; d < 3; ++d) o[3 + d] = nrm[d];
     o[6] = (double)lo;
+    o[7] = -1.0;
+    if (scheme == S_LF) {
+      double gn[3];
+      globalAlphaNormal<D>(b, c, gn);
+      double lv[MAX_LEVEL_SETS];
+      levelValuesAt(b, lw, c, phi0, lv);
+      double *r = o + 8;
+      for (int d = 0; d < 3; ++d) r[d] = coord[d];
+      for (int d = 0; d < 3; ++d) r[3 + d] = gn[d];
+      r[6] = (double)lo;
+      r[7] = (double)alphaMaterial(lv, lw.n + 1, phi0);
+    } else if (scheme == S_LLF) {
+      for (int q = 0; q < R - 1; ++q) {
+        const int ni[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
+        double bn[3];
+        llfBoxNormal<D>(b, c, q, bn);
+        double *r = o + 8 * (1 + q);
+        for (int d = 0; d < 3; ++d) r[d] = (d < D) ? coord[d] + (double)ni[d] * b.g.dx : 0.0;
+        for (int d = 0; d < 3; ++d) r[3 + d] = bn[d];
+        r[6] = (double)lo;
+        r[7] = -1.0;
+      }
+    }
   }
 }
 
@@ -1410,12 +1471,17 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
     ++act;
     BandView v = b;
     Vel ve = vel;
-    if (!SEG && rp.hostVel) {  // velocity evaluated on the host for active point j
+    if (!SEG && rp.hostVel) {  // velocities evaluated on the host for active point j
+      const double *h = rp.hostVel + 4 * (size_t)j * rp.hostRec;
       ve.kind = 0;
-      ve.b = rp.hostVel[4 * (size_t)j];
-      ve.vec[0] = rp.hostVel[4 * (size_t)j + 1];
-      ve.vec[1] = rp.hostVel[4 * (size_t)j + 2];
-      ve.vec[2] = rp.hostVel[4 * (size_t)j + 3];
+      ve.b = h[0];
+      ve.vec[0] = h[1];
+      ve.vec[1] = h[2];
+      ve.vec[2] = h[3];
+      if (SCHEME == S_LLF) {  // the alpha box follows the scheme's call
+        ve.box = h + 4;
+        ve.boxStride = 4;
+      }
     }
     if (SEG) {
       const uint32_t k = sa.iseg[i];
@@ -1627,6 +1693,8 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
   const int L = lw.n + 1;
   const double MAXV = 1.7976931348623157e308;
   double ga[3] = {0, 0, 0}, la[3] = {0, 0, 0};
+  if (SCHEME == S_LF)
+    for (int i = 0; i < 3; ++i) ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
   double minStep = MAXV;
   unsigned long long act = 0;
   unsigned long long sllfKey = 0;  // SLLF: (j + 1) << 4 | material of this thread's last call that set alphas
@@ -1665,12 +1733,16 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
       if (mat >= 0 && !isVoid) {
         Vel vm = velOfMaterial(rp, base, mat);
         if (rp.hostVel) {  // evaluated on the host for (active point j, material mat)
-          const double *h = rp.hostVel + ((size_t)j * L + mat) * 4;
+          const double *h = rp.hostVel + ((size_t)j * rp.hostRec * L + mat) * 4;
           vm.kind = 0;
           vm.b = h[0];
           vm.vec[0] = h[1];
           vm.vec[1] = h[2];
           vm.vec[2] = h[3];
+          if (SCHEME == S_LLF) {
+            vm.box = h + 4 * L;
+            vm.boxStride = 4 * L;
+          }
         }
         bool used = false;
         v = ratePoint<D, SCHEME>(b, c, phi0, rp, vm, ga, la, velocity, nullptr, nullptr, &used, &dn);
@@ -1751,7 +1823,7 @@ void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, cons
                    const RateParams &rp, double *lists, DevScalars *sc, const double *maxTimeStep, double *dtOut,
                    Workspace &ws) {
   const int scheme = schemeIdOf(rp.spatial);
-  activeList(s, b, ws, sc, 1, 0);  // resets the rate maxima too
+  activeList(s, b, ws, sc, 1, scheme == S_LF ? 1 : 0);  // resets the rate maxima (not LF's global alphas)
   kResetMM<<<1, 1, 0, s>>>(sc);
   BandView v = viewOf(b, f, g);
   const int ge = rateBlocks();
@@ -1759,6 +1831,7 @@ void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, cons
   if (D == 2) {
     switch (scheme) {
     case S_EO: LSG_RATES_MM(2, S_EO); break;
+    case S_LF: LSG_RATES_MM(2, S_LF); break;
     case S_WENO5: LSG_RATES_MM(2, S_WENO5); break;
     case S_LLLF: LSG_RATES_MM(2, S_LLLF); break;
     case S_LLFA: LSG_RATES_MM(2, S_LLFA); break;
@@ -1769,6 +1842,7 @@ void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, cons
   } else {
     switch (scheme) {
     case S_EO: LSG_RATES_MM(3, S_EO); break;
+    case S_LF: LSG_RATES_MM(3, S_LF); break;
     case S_WENO5: LSG_RATES_MM(3, S_WENO5); break;
     case S_LLLF: LSG_RATES_MM(3, S_LLLF); break;
     case S_LLFA: LSG_RATES_MM(3, S_LLFA); break;
@@ -1782,7 +1856,8 @@ void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, cons
     if (D == 2) kSllfAlphas<2><<<1, 1, 0, s>>>(v, ws.cpos, rp, sc, 1);
     else kSllfAlphas<3><<<1, 1, 0, s>>>(v, ws.cpos, rp, sc, 1);
   }
-  const int useAlphas = (scheme == S_LLLF || scheme == S_LLF || scheme == S_LLFA || scheme == S_SLLF) ? 1 : 0;
+  const int useAlphas =
+      (scheme == S_LF || scheme == S_LLLF || scheme == S_LLF || scheme == S_LLFA || scheme == S_SLLF) ? 1 : 0;
   kTimeStepMM<<<1, 1, 0, s>>>(sc, D, g.dx, maxTimeStep, useAlphas, dtOut);
 }
 
@@ -1791,13 +1866,14 @@ void launchSetFlag(hipStream_t s, uint32_t *flag, uint32_t v) { kSetFlag<<<1, 1,
 
 // Active list, defined list and the velocity inputs of the active points.
 void launchVelInputs(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
-                     Workspace &ws, uint32_t *dlist, uint32_t *ndef, double *out, DevScalars *sc) {
+                     const LowerViews &lw, Workspace &ws, uint32_t *dlist, uint32_t *ndef, double *out,
+                     DevScalars *sc) {
   activeList(s, b, ws, sc);
   kDefinedCount<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials);
   kDefinedList<<<g_scanBlocks, TPB, 0, s>>>(b.val, b.n, ws.partials, dlist, ndef);
   BandView v = viewOf(b, f, g);
-  if (D == 2) kVelInputs<2><<<rateBlocks(), TPB, 0, s>>>(v, ws.cpos, ws.ncand, dlist, ndef, rp, out);
-  else kVelInputs<3><<<rateBlocks(), TPB, 0, s>>>(v, ws.cpos, ws.ncand, dlist, ndef, rp, out);
+  if (D == 2) kVelInputs<2><<<rateBlocks(), TPB, 0, s>>>(v, lw, ws.cpos, ws.ncand, dlist, ndef, rp, out);
+  else kVelInputs<3><<<rateBlocks(), TPB, 0, s>>>(v, lw, ws.cpos, ws.ncand, dlist, ndef, rp, out);
 }
 
 // Batch rate pass: every level set's stage velocities and max |H - D| per
@@ -1949,6 +2025,40 @@ template <int D> __global__ __launch_bounds__(256) void kGlobalAlphas(BandView b
 }
 
 __global__ void kResetAlphas(DevScalars *sc) { sc->alphaBits[0] = sc->alphaBits[1] = sc->alphaBits[2] = 0; }
+
+template <int D> struct AlphaPointMMFn {
+  BandView b;
+  LowerViews lw;
+  RateParams rp;
+  Vel base;
+  double la[3];
+  __device__ __attribute__((always_inline)) void operator()(uint32_t i) {
+    int32_t c[3];
+    candCoords(b, i, b.pos[i], c);
+    const double phi0 = b.val[i];
+    double lv[MAX_LEVEL_SETS];
+    levelValuesAt(b, lw, c, phi0, lv);
+    const Vel vel = velOfMaterial(rp, base, alphaMaterial(lv, lw.n + 1, phi0));
+    globalAlphaPoint<D>(b, c, vel, la);
+  }
+};
+template <int D>
+__global__ __launch_bounds__(256) void kGlobalAlphasMM(BandView b, LowerViews lw, const uint32_t *nptr, RateParams rp,
+                                                       DevScalars *sc) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const uint32_t n = *nptr;
+  AlphaPointMMFn<D> fn{b, lw, rp, velOf(rp), {0, 0, 0}};
+  forEachActive(b.val, n, fn);
+  for (int d = 0; d < D; ++d) blockMaxAtomic((unsigned long long)__double_as_longlong(fn.la[d]), &sc->alphaBits[d]);
+}
+void launchGlobalAlphasMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g,
+                          const LowerViews &lw, const RateParams &rp, DevScalars *sc) {
+  kResetAlphas<<<1, 1, 0, s>>>(sc);
+  BandView v = viewOf(b, f, g);
+  const int ge = gridFor(b.cap, 8192);
+  if (D == 2) kGlobalAlphasMM<2><<<ge, TPB, 0, s>>>(v, lw, b.n, rp, sc);
+  else kGlobalAlphasMM<3><<<ge, TPB, 0, s>>>(v, lw, b.n, rp, sc);
+}
 
 void launchGlobalAlphas(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g,
                         const RateParams &rp, DevScalars *sc) {

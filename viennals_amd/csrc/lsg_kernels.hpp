@@ -90,8 +90,6 @@ void launchStarOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, co
                      const OpParams &op, DevScalars *sc);
 void launchSelfOpSeg(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const SegCtx &c, Workspace &ws,
                      const OpParams &op, DevScalars *sc);
-void launchVelInputs(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
-                     Workspace &ws, uint32_t *dlist, uint32_t *ndef, double *out, DevScalars *sc);
 void launchRatesSeg(hipStream_t s, int D, const BandBufs &b, const SegCtx &c, const RateParams &rp, double *rates,
                     DevScalars *sc, unsigned long long *segMax, const double *maxDt, double *dtOut, Workspace &ws);
 void launchStepBeginSeg(hipStream_t s, DevScalars *sc, double *maxDt, uint32_t nseg);
@@ -114,11 +112,25 @@ struct LowerViews {
 };
 // Rate pass with several level sets: per active point of the top a list of
 // (velocity, stop value) pairs, (lw.n + 1) pairs per entry of `lists`; dt
-// from the per-point step limits.  LF1/LF2 and the learned scheme are not
-// supported here.
+// from the per-point step limits.  The learned scheme is not supported here.
 void launchRatesMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const LowerViews &lw,
                    const RateParams &rp, double *lists, DevScalars *sc, const double *maxTimeStep, double *dtOut,
                    Workspace &ws);
+// Advect::findGlobalAlphas with the material of each active point (the
+// first level set, bottom up, whose value there is <= value + 1e-4) and its
+// built-in functor (lsAdvect.hpp:90-195).
+void launchGlobalAlphasMM(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g,
+                          const LowerViews &lw, const RateParams &rp, DevScalars *sc);
+// Records of host velocity calls per active point for a scheme: 1 (the
+// scheme's call), 2 for LF (and findGlobalAlphas' call), 1 + 3^D for LLF
+// (and the alpha box).
+int hostVelocityRecords(int spatial, int D);
+// Inputs of the host velocity calls: 8 doubles per record (coordinate,
+// normal, point id, material or -1 for every material), records of active
+// point j at [j * hostVelocityRecords, ...).
+void launchVelInputs(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
+                     const LowerViews &lw, Workspace &ws, uint32_t *dlist, uint32_t *ndef, double *out,
+                     DevScalars *sc);
 void launchSetFlag(hipStream_t s, uint32_t *flag, uint32_t v);
 
 // Band operations (all asynchronous on `s`; sizes live in device memory).
