@@ -2574,29 +2574,55 @@ __global__ void kNearest(const double *P, const uint32_t *nP, BandView q, const 
       c[d] = (int32_t)floor(p[d] / q.g.dx);
     }
     double best2 = __builtin_inf();
+    // the surface points of the band entry at grid point c + o
+    auto visit = [&](int ox, int oy, int oz) {
+      const int32_t g[3] = {c[0] + ox, c[1] + oy, c[2] + oz};
+      bool in = true;
+      for (int d = 0; d < D; ++d) in = in && g[d] >= q.g.lo[d] && g[d] <= q.g.hi[d];
+      if (!in) return;
+      const BandView::Loc l = q.locate(g[0], g[1], g[2]);
+      if (!l.inside || !((q.bits[l.w] >> l.bt) & 1ull)) return;
+      const uint32_t e = q.entryOf(l, q.bits[l.w], q.pre[l.w]);
+      for (uint32_t s = qStart[e]; s < qStart[e + 1]; ++s) {
+        double d2 = 0;
+        for (int d = 0; d < D; ++d) {
+          const double t = p[d] - qPts[(size_t)s * D + d];
+          d2 += t * t;
+        }
+        if (d2 < best2) best2 = d2;
+      }
+    };
     for (int r = 0; r <= maxShell; ++r) {
-      const int zr = (D == 3) ? r : 0;
-      for (int oz = -zr; oz <= zr; ++oz)
+      // the grid points at Chebyshev distance r only: the faces of the
+      // (2r+1)^D cube, each point once ((2r+1)^D - (2r-1)^D of them)
+      if (r == 0) {
+        visit(0, 0, 0);
+      } else if (D == 2) {
+        for (int ox = -r; ox <= r; ++ox) {
+          visit(ox, -r, 0);
+          visit(ox, r, 0);
+        }
+        for (int oy = -r + 1; oy <= r - 1; ++oy) {
+          visit(-r, oy, 0);
+          visit(r, oy, 0);
+        }
+      } else {
         for (int oy = -r; oy <= r; ++oy)
           for (int ox = -r; ox <= r; ++ox) {
-            const int m = max(abs(ox), max(abs(oy), abs(oz)));
-            if (m != r) continue;  // shell r only
-            const int32_t g[3] = {c[0] + ox, c[1] + oy, c[2] + oz};
-            bool in = true;
-            for (int d = 0; d < D; ++d) in = in && g[d] >= q.g.lo[d] && g[d] <= q.g.hi[d];
-            if (!in) continue;
-            const BandView::Loc l = q.locate(g[0], g[1], g[2]);
-            if (!l.inside || !((q.bits[l.w] >> l.bt) & 1ull)) continue;
-            const uint32_t e = q.entryOf(l, q.bits[l.w], q.pre[l.w]);
-            for (uint32_t s = qStart[e]; s < qStart[e + 1]; ++s) {
-              double d2 = 0;
-              for (int d = 0; d < D; ++d) {
-                const double t = p[d] - qPts[(size_t)s * D + d];
-                d2 += t * t;
-              }
-              if (d2 < best2) best2 = d2;
-            }
+            visit(ox, oy, -r);
+            visit(ox, oy, r);
           }
+        for (int oz = -r + 1; oz <= r - 1; ++oz) {
+          for (int ox = -r; ox <= r; ++ox) {
+            visit(ox, -r, oz);
+            visit(ox, r, oz);
+          }
+          for (int oy = -r + 1; oy <= r - 1; ++oy) {
+            visit(-r, oy, oz);
+            visit(r, oy, oz);
+          }
+        }
+      }
       // (one shell of margin: p / dx may round up to the next cell)
       const double rr = (double)(r - 1) * q.g.dx;
       if (r >= 1 && best2 <= rr * rr) break;
