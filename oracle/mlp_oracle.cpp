@@ -6,7 +6,7 @@
 // kernels (loss and gradients).
 //
 // Network: inputs are the five one-sided differences d[0..4] of weno5,
-// divided by max|d|; 5 -> 32 -> 32 -> 3, GELU (tanh form) after the hidden
+// divided by max|d|; 5 -> 32 -> 32 -> 3, GELU (tanh approximation) after the hidden
 // layers, softmax output = weights of weno5's three candidate polynomials.
 // The weighted sum replaces sum(alpha_k p_k)/sum(alpha_k).  Where the
 // regularised smoothness indicators (eps*dx^2 + beta_k) differ by more than a
@@ -41,31 +41,36 @@ float bf16Round(float x) {
   return r;
 }
 
+// GELU with the tanh approximation, 0.5 x (1 + tanh(u)), written as
+// x sigmoid(2u), u = sqrt(2/pi) (x + 0.044715 x^3); derivative
+// sigmoid(2u) + 2 x sigmoid(2u) (1 - sigmoid(2u)) u'.
 template <class R> static inline R geluT(R x, R &dgelu) {
   const R c = (R)0.7978845608028654;  // sqrt(2/pi)
   const R a = (R)0.044715;
   const R x2 = x * x;
   const R u = c * (x + a * x2 * x);
-  const R t = std::tanh(u);
-  const R half = (R)0.5;
-  dgelu = half * ((R)1 + t) + half * x * ((R)1 - t * t) * c * ((R)1 + (R)3 * a * x2);
-  return half * x * ((R)1 + t);
+  const R sg = (R)1 / ((R)1 + std::exp((R)-2 * u));
+  dgelu = sg + (R)2 * x * sg * ((R)1 - sg) * c * ((R)1 + (R)3 * a * x2);
+  return x * sg;
 }
 
 // Inference: P holds the network parameters as floats (the bf16 values the
-// GPU uses); fp32 accumulation in index order, bias first.
+// GPU uses); fp32 accumulation in index order.  The forward pass of the
+// training kernel: layer 1 bias first, its activations rounded to bf16 for
+// the 32x32 product, which is summed before its bias; layer 3 bias first.
 void mlpForward(const float *P, const float in[MLP_IN], float w[MLP_OUT]) {
-  float h1[MLP_H], h2[MLP_H], z3[MLP_OUT];
+  float x[MLP_IN], h1[MLP_H], h2[MLP_H], z3[MLP_OUT];
   float dg;
+  for (int i = 0; i < MLP_IN; ++i) x[i] = bf16Round(in[i]);
   for (int j = 0; j < MLP_H; ++j) {
     float z = P[MLP_B1 + j];
-    for (int i = 0; i < MLP_IN; ++i) z += P[MLP_W1 + j * MLP_IN + i] * in[i];
-    h1[j] = geluT<float>(z, dg);
+    for (int i = 0; i < MLP_IN; ++i) z += P[MLP_W1 + j * MLP_IN + i] * x[i];
+    h1[j] = bf16Round(geluT<float>(z, dg));
   }
   for (int j = 0; j < MLP_H; ++j) {
-    float z = P[MLP_B2 + j];
+    float z = 0.f;
     for (int i = 0; i < MLP_H; ++i) z += P[MLP_W2 + j * MLP_H + i] * h1[i];
-    h2[j] = geluT<float>(z, dg);
+    h2[j] = bf16Round(geluT<float>(z + P[MLP_B2 + j], dg));
   }
   for (int k = 0; k < MLP_OUT; ++k) {
     float z = P[MLP_B3 + k];
@@ -138,9 +143,9 @@ double weno5L(const double *x, double dx, bool plus, const float *P) {
 // network inputs, X[8s+5..8s+7] candidate polynomials divided by max|d|,
 // T[s] target derivative times dx / max|d|.  loss = (1/n) sum (sum_k w_k
 // p_k - t)^2.  emulate != 0 reproduces the rounding points of the GPU
-// training kernel: weights used as their bf16 values, the hidden activation
-// of layer 1 and the layer-2 pre-activation gradient rounded to bf16 where
-// they enter the 32x32 products; everything else in R.
+// training kernel: weights used as their bf16 values, the inputs, both
+// hidden activations and the layer-2 pre-activation gradient rounded to
+// bf16 where they enter the products; everything else in R.
 template <class R>
 static double lossGrad(const float *Pm, int64_t n, const float *X, const float *T, double *G, int emulate) {
   std::vector<R> P(MLP_NPARAM);
@@ -153,11 +158,13 @@ static double lossGrad(const float *Pm, int64_t n, const float *X, const float *
   double loss = 0;
   const R invn = (R)1 / (R)n;
   for (int64_t s = 0; s < n; ++s) {
-    const float *x = X + 8 * s;
+    const float *xs = X + 8 * s;
+    R x[8];
+    for (int i = 0; i < 8; ++i) x[i] = (i < MLP_IN) ? rb((R)xs[i]) : (R)xs[i];
     R h1[MLP_H], d1[MLP_H], h1b[MLP_H], h2[MLP_H], d2[MLP_H], z3[MLP_OUT], w[MLP_OUT];
     for (int j = 0; j < MLP_H; ++j) {
       R z = P[MLP_B1 + j];
-      for (int i = 0; i < MLP_IN; ++i) z += P[MLP_W1 + j * MLP_IN + i] * (R)x[i];
+      for (int i = 0; i < MLP_IN; ++i) z += P[MLP_W1 + j * MLP_IN + i] * x[i];
       h1[j] = geluT<R>(z, d1[j]);
       h1b[j] = rb(h1[j]);
     }
@@ -165,7 +172,7 @@ static double lossGrad(const float *Pm, int64_t n, const float *X, const float *
       R z = 0;
       for (int i = 0; i < MLP_H; ++i) z += P[MLP_W2 + j * MLP_H + i] * h1b[i];
       z = z + P[MLP_B2 + j];
-      h2[j] = geluT<R>(z, d2[j]);
+      h2[j] = rb(geluT<R>(z, d2[j]));
     }
     for (int k = 0; k < MLP_OUT; ++k) {
       R z = P[MLP_B3 + k];

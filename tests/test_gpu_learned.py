@@ -90,7 +90,10 @@ def test_training_reduces_loss_on_teacher_task():
 
 def test_learned_scheme_step_matches_oracle():
     """One RK3 step of a star shape with the learned scheme: GPU vs oracle
-    (fp32 network: values agree to 1e-6 grid units, same defined set)."""
+    (values agree to 1e-5 grid units, same defined set: the network takes
+    its inputs and activations as bf16, and a last-bit difference of an fp32
+    value -- fused multiply-adds on the GPU, the hardware exp and reciprocal
+    -- can move one bf16 rounding)."""
     import viennals_amd as vls
     from viennals_amd.learned import StarShape
     shape = StarShape(12.0, seed=5)
@@ -115,7 +118,7 @@ def test_learned_scheme_step_matches_oracle():
     assert np.array_equal(gi, oi)
     both = (np.abs(gv) < 1e300) & (np.abs(ov) < 1e300)
     assert np.array_equal(np.abs(gv) < 1e300, np.abs(ov) < 1e300)
-    assert np.max(np.abs(gv[both] - ov[both])) < 1e-6
+    assert np.max(np.abs(gv[both] - ov[both])) < 1e-5
 
 
 def test_learned_scheme_requires_weights():

@@ -246,6 +246,9 @@ struct RateParams {
   // null for the built-in functor
   const double *hostVel;
   int hostRec;
+  // learned weno5: the network of a wave's points on the matrix cores (1)
+  // or per lane on the vector ALUs (0)
+  int learnedWave;
   // several level sets: built-in functor of material m where matSet[m]
   // (the others use the functor above); kind, dir, k, b, aPos, aNeg, vec
   int matSet[8];

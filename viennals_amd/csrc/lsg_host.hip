@@ -1559,6 +1559,12 @@ static int advectApply(lsg_advect *a) {
   if (a->spatial == 12 && !a->mlpDev)
     return fail(-8, "Advect: learned WENO5 selected but no network parameters were set");
   a->rp.mlp = a->mlpDev;
+  // the network on the matrix cores per wave (default) or per lane
+  // (LSG_LEARNED_WAVE=0)
+  {
+    const char *e = getenv("LSG_LEARNED_WAVE");
+    a->rp.learnedWave = (e && e[0] == '0') ? 0 : 1;
+  }
   if (int e = ensureStores(d, d->cur.b.cap)) return e;
   if (int e = ensureRates(a, d->cur.b.cap)) return e;
   if (a->temporal != 0) {

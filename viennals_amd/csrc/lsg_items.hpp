@@ -640,7 +640,7 @@ struct Vel {
   const double *box = nullptr;
   int boxStride = 4;
   // built-in VelocityField::getScalarVelocity (lsVelocityField.hpp:17-21)
-  LSG_HD double scalar(const double n[3]) const {
+  LSG_HD __attribute__((always_inline)) double scalar(const double n[3]) const {
     if (kind == 0) return b;
     if (kind == 2) {  // b + aPos * sum |n_i| (examples/AirGapDeposition/AirGapDeposition.cpp:20-27)
       double s = 0.0;
@@ -834,33 +834,59 @@ LSG_HD inline double weno5(const double *x, double dx, bool plus) {
 // the three candidate polynomials come from a small network fed with the
 // differences d / max|d|; elsewhere (kinks) plain weno5 is used.  P holds
 // the parameters as floats (bf16 values); fp32 accumulation, bias first.
+// exp(x) and a / b of the network's activations: the hardware's
+// single-instruction forms on the GPU (v_exp_f32, v_rcp_f32), the library
+// functions in a host build.
+LSG_HD inline float mlpExp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __expf(x);
+#else
+  return std::exp(x);
+#endif
+}
+LSG_HD inline float mlpDiv(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __fdividef(a, b);
+#else
+  return a / b;
+#endif
+}
+// GELU, tanh approximation 0.5 x (1 + tanh(u)) = x sigmoid(2u) with
+// u = sqrt(2/pi) (x + 0.044715 x^3); sigmoid(2u) = 1 / (1 + e^-2u).
+LSG_HD inline float mlpSigmoid2(float u) { return mlpDiv(1.0f, 1.0f + mlpExp(-2.0f * u)); }
 LSG_HD inline float mlpGelu(float x) {
   const float c = 0.7978845608028654f, a = 0.044715f;
   const float u = c * (x + a * (x * x) * x);
-  return 0.5f * x * (1.0f + tanhf(u));
+  return x * mlpSigmoid2(u);
 }
 
+// The forward pass of the training kernel and of the matrix-core inference:
+// the inputs and both hidden activations enter their products as bf16,
+// fp32 accumulation; layers 1 and 3 start from their biases, layer 2 adds
+// its bias after the sum; softmax.
 LSG_HD inline void mlpForward(const float *P, const float in[MLP_IN], float w[MLP_OUT]) {
-  float h1[MLP_H];
+  float x[MLP_IN], h1[MLP_H];
+  LSG_UNROLL
+  for (int i = 0; i < MLP_IN; ++i) x[i] = bf16Float(bf16Bits(in[i]));
   LSG_UNROLL
   for (int j = 0; j < MLP_H; ++j) {
     float z = P[MLP_B1 + j];
     LSG_UNROLL
-    for (int i = 0; i < MLP_IN; ++i) z += P[MLP_W1 + j * MLP_IN + i] * in[i];
-    h1[j] = mlpGelu(z);
+    for (int i = 0; i < MLP_IN; ++i) z += P[MLP_W1 + j * MLP_IN + i] * x[i];
+    h1[j] = bf16Float(bf16Bits(mlpGelu(z)));
   }
   float z3[MLP_OUT] = {P[MLP_B3], P[MLP_B3 + 1], P[MLP_B3 + 2]};
   for (int j = 0; j < MLP_H; ++j) {
-    float z = P[MLP_B2 + j];
+    float z = 0.f;
     LSG_UNROLL
     for (int i = 0; i < MLP_H; ++i) z += P[MLP_W2 + j * MLP_H + i] * h1[i];
-    const float h2 = mlpGelu(z);
+    const float h2 = bf16Float(bf16Bits(mlpGelu(z + P[MLP_B2 + j])));
     z3[0] += P[MLP_W3 + j] * h2;
     z3[1] += P[MLP_W3 + MLP_H + j] * h2;
     z3[2] += P[MLP_W3 + 2 * MLP_H + j] * h2;
   }
   const float m = fmaxf(z3[0], fmaxf(z3[1], z3[2]));
-  const float e0 = expf(z3[0] - m), e1 = expf(z3[1] - m), e2 = expf(z3[2] - m);
+  const float e0 = mlpExp(z3[0] - m), e1 = mlpExp(z3[1] - m), e2 = mlpExp(z3[2] - m);
   const float sum = (e0 + e1) + e2;
   w[0] = e0 / sum;
   w[1] = e1 / sum;
@@ -950,8 +976,12 @@ LSG_HD inline RateOut schemeEO(const BandView &b, const int32_t c[3], double phi
 // (weno5Learned for each): the stencil work is unrolled, the network runs in
 // one loop over the evaluations that need it (one copy of its code), its
 // inputs and outputs staged in the caller's scratch `io` (2D*5 floats).
+// phase 1 only writes the network inputs (zeros where the network is not
+// used) for a caller that runs the network for a whole wave; phase 2 then
+// takes the weights from io[5e .. 5e+2] instead of running it.
 template <int D>
-LSG_HD inline void weno5LearnedAxes(double sts[][7], double dx, const float *P, float *io, double *gm, double *gp) {
+LSG_HD inline void weno5LearnedAxes(double sts[][7], double dx, const float *P, float *io, double *gm, double *gp,
+                                    int phase = 0) {
   double res[2 * D];
   double pe[2 * D][3];
   uint32_t net = 0;
@@ -979,21 +1009,29 @@ LSG_HD inline void weno5LearnedAxes(double sts[][7], double dx, const float *P, 
     for (int i = 0; i < 5; ++i) m = dmaxs(m, fabs(d[i]));
     res[e] = 0.;
     if (m == 0. || mx > MLP_KINK_RATIO * mn) {
-      res[e] = weno5(x, dx, plus);
+      if (phase != 1) {
+        res[e] = weno5(x, dx, plus);
+      } else {
+        LSG_UNROLL
+        for (int i = 0; i < 5; ++i) io[e * 5 + i] = 0.f;
+      }
     } else {
       net |= 1u << e;
-      LSG_UNROLL
-      for (int i = 0; i < 5; ++i) io[e * 5 + i] = (float)(d[i] / m);
+      if (phase != 2) {
+        LSG_UNROLL
+        for (int i = 0; i < 5; ++i) io[e * 5 + i] = (float)(d[i] / m);
+      }
     }
     pe[e][0] = (1.0 / 6.0) * (2 * d[0] - 7 * d[1] + 11 * d[2]);
     pe[e][1] = (1.0 / 6.0) * (-d[1] + 5 * d[2] + 2 * d[3]);
     pe[e][2] = (1.0 / 6.0) * (2 * d[2] + 5 * d[3] - d[4]);
   }
+  if (phase == 1) return;
 #if defined(__clang__)
 #pragma unroll 1
 #endif
   for (int e = 0; e < 2 * D; ++e) {
-    if (!((net >> e) & 1u)) continue;
+    if (phase == 2 || !((net >> e) & 1u)) continue;
     float in[MLP_IN], w[MLP_OUT];
     LSG_UNROLL
     for (int i = 0; i < MLP_IN; ++i) in[i] = io[e * 5 + i];
@@ -1012,7 +1050,7 @@ LSG_HD inline void weno5LearnedAxes(double sts[][7], double dx, const float *P, 
 
 template <int D, int ORDER, bool LEARNED = false>
 LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double phi0, bool calcNormals, const Vel &vel,
-                                 const float *mlp = nullptr, float *io = nullptr) {
+                                 const float *mlp = nullptr, float *io = nullptr, int phase = 0) {
   constexpr int r = (ORDER + 1) / 2;
   double gradPosTotal = 0, gradNegTotal = 0;
   double gm[D], gp[D];
@@ -1024,7 +1062,8 @@ LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double p
   if constexpr (LEARNED) {
     LSG_UNROLL
     for (int i = 0; i < D; i++) sts[i][r] = phi0;
-    weno5LearnedAxes<D>(sts, b.g.dx, mlp, io, gm, gp);
+    weno5LearnedAxes<D>(sts, b.g.dx, mlp, io, gm, gp, phase);
+    if (phase == 1) return {0., 0.};
   }
   LSG_UNROLL
   for (int i = 0; i < D; i++) {
@@ -1504,7 +1543,7 @@ template <int D, int SCHEME>
 LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi0, const RateParams &rp, const Vel &vel,
                                const double *globalAlphas, double *alphaAcc, double &velocity,
                                const float *mlp = nullptr, float *io = nullptr, bool *sllfUsed = nullptr,
-                               double *dnOut = nullptr) {
+                               double *dnOut = nullptr, int learnedPhase = 0) {
   RateOut r;
   if (SCHEME == S_SLLF) {
     bool used = false;
@@ -1513,7 +1552,7 @@ LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi
   } else if (SCHEME == S_EO) r = schemeEO<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO5) r = schemeWENO<D, 5>(b, c, phi0, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO3) r = schemeWENO<D, 3>(b, c, phi0, rp.calcNormals != 0, vel);
-  else if (SCHEME == S_WENO5L) r = schemeWENO<D, 5, true>(b, c, phi0, rp.calcNormals != 0, vel, mlp, io);
+  else if (SCHEME == S_WENO5L) r = schemeWENO<D, 5, true>(b, c, phi0, rp.calcNormals != 0, vel, mlp, io, learnedPhase);
   else if (SCHEME == S_LF) r = schemeLF<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel, rp.alphaFactor, globalAlphas);
   else if (SCHEME == S_LLLF) r = schemeLLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
   else if (SCHEME == S_LLFA) r = schemeLLFA<D>(b, c, phi0, vel, alphaAcc);

@@ -1467,6 +1467,19 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
   unsigned long long *segMax;
   uint32_t curSeg;
   unsigned long long sllfKey;  // SLLF: key of this lane's last active point that set its alphas
+  int learnedPhase;            // learned weno5: 2 when the wave has run the network (waveMlp)
+  // learned weno5, first phase: this lane's network inputs for entry i
+  __device__ __attribute__((always_inline)) void prepareLearned(uint32_t i) {
+    int32_t c[3];
+    candCoords(b, i, b.pos[i], c);
+    double sts[D][7];
+    gatherAxes<D, 3>(b, c, sts);
+    const double phi0 = b.val[i];
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d) sts[d][3] = phi0;
+    double gm[D], gp[D];
+    weno5LearnedAxes<D>(sts, b.g.dx, mlp, io, gm, gp, 1);
+  }
   __device__ __attribute__((always_inline)) void operator()(uint32_t i, uint32_t j) {
     ++act;
     BandView v = b;
@@ -1504,7 +1517,7 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
     double velocity;
     bool used = false;
     double dn = 0.;
-    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp, io, &used, &dn);
+    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp, io, &used, &dn, learnedPhase);
     if (!SEG && rp.dissOut) rp.dissOut[i] = dn;
     if (SCHEME == S_SLLF && used) sllfKey = (unsigned long long)(j + 1) << 4;  // lanes visit j in increasing order
     if (velocity != 0.) {
@@ -1514,7 +1527,141 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
   }
 };
 
-template <int D, int SCHEME, bool SEG>
+// ------------------------------------------- learned weno5 on the matrix cores
+//
+// The network evaluations of a wave (64 lanes x 2D, one row of 5 inputs each
+// in LDS, zeros where a lane does not use the network) in blocks of 32
+// rows, every layer a v_mfma_f32_32x32x16_bf16 on the transposed
+// activations: D[m][n] = sum_k A[m][k] B[k][n] with m = unit, n = sample.
+// Accumulator element r of lane l holds unit U(h, r) = 8(r/4) + 4h + r%4
+// (h = l/32) of sample l%32, and the B operand of the next layer is taken
+// from those elements directly: its K slot t of step q (lane half h) stands
+// for unit U(h, 8q + t), and the weights (A operand, row l%32) are held in
+// the same permuted order, in registers from the kernel start.  Inputs and
+// activations enter each product as bf16, accumulation is fp32; layer 1 and
+// layer 3 start from their biases, layer 2 adds its bias after the sum --
+// the rounding points of the training kernel.  The three softmax weights
+// overwrite the row's first three inputs.
+typedef __bf16 mlpBf16x8 __attribute__((ext_vector_type(8)));
+typedef short mlpI16x8 __attribute__((ext_vector_type(8)));
+typedef float mlpF32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline int mlpUnit(int h, int r) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+// fp32 -> bf16 bits, round to nearest even: the hardware conversion
+// (v_cvt_pk_bf16_f32), the same value as bf16Bits for finite inputs
+__device__ inline short mlpBf16(float x) { return __builtin_bit_cast(short, (__bf16)x); }
+
+// LDS written by one lane of a wave and read by another: the wave's own
+// accesses in order
+__device__ inline void waveSyncLds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// This lane's operands: the three weight matrices in the permuted order
+// and the biases of its accumulator elements.
+struct MlpLane {
+  mlpBf16x8 w1, w2[2], w3[2];
+  float b1[16], b2[16], b3[16];
+};
+__device__ __attribute__((always_inline)) inline MlpLane mlpLane(const uint16_t *P) {
+  const int lane = threadIdx.x & 63, col = lane & 31, h = lane >> 5;
+  MlpLane o;
+  mlpI16x8 a;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int i = 8 * h + t;
+    a[t] = (short)((i < MLP_IN) ? P[MLP_W1 + col * MLP_IN + i] : 0);
+  }
+  o.w1 = __builtin_bit_cast(mlpBf16x8, a);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    mlpI16x8 b, c;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int u = mlpUnit(h, 8 * q + t);
+      b[t] = (short)P[MLP_W2 + col * MLP_H + u];
+      c[t] = (short)((col < MLP_OUT) ? P[MLP_W3 + col * MLP_H + u] : 0);
+    }
+    o.w2[q] = __builtin_bit_cast(mlpBf16x8, b);
+    o.w3[q] = __builtin_bit_cast(mlpBf16x8, c);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int u = mlpUnit(h, r);
+    o.b1[r] = bf16Float(P[MLP_B1 + u]);
+    o.b2[r] = bf16Float(P[MLP_B2 + u]);
+    o.b3[r] = (u < MLP_OUT) ? bf16Float(P[MLP_B3 + u]) : 0.f;
+  }
+  return o;
+}
+
+// Two blocks of 32 rows at a time, so that the matrix and activation work
+// of one overlaps the other's latency.
+template <int NROWS>
+__device__ __attribute__((always_inline)) void waveMlp(float *rows, const MlpLane &W) {
+  static_assert(NROWS % 64 == 0, "pairs of 32-row blocks");
+  const int lane = threadIdx.x & 63, col = lane & 31, h = lane >> 5;
+  for (int rb = 0; rb < NROWS / 32; rb += 2) {
+    float *row[2] = {rows + (rb * 32 + col) * MLP_IN, rows + ((rb + 1) * 32 + col) * MLP_IN};
+    // rows without the network hold zeros: a pair of blocks without any
+    // network row is skipped
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < MLP_IN; ++t) any = any || row[0][t] != 0.f || row[1][t] != 0.f;
+    if (!__any(any)) continue;
+    mlpF32x16 acc[2];
+    mlpI16x8 g0[2], g1[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      mlpI16x8 xb;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) xb[t] = (h == 0 && t < MLP_IN) ? mlpBf16(row[p][t]) : (short)0;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[p][r] = W.b1[r];
+      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.w1, __builtin_bit_cast(mlpBf16x8, xb), acc[p], 0, 0, 0);
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        g0[p][t] = mlpBf16(mlpGelu(acc[p][t]));
+        g1[p][t] = mlpBf16(mlpGelu(acc[p][8 + t]));
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[p][r] = 0.f;
+      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.w2[0], __builtin_bit_cast(mlpBf16x8, g0[p]), acc[p], 0, 0, 0);
+      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.w2[1], __builtin_bit_cast(mlpBf16x8, g1[p]), acc[p], 0, 0, 0);
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        g0[p][t] = mlpBf16(mlpGelu(acc[p][t] + W.b2[t]));
+        g1[p][t] = mlpBf16(mlpGelu(acc[p][8 + t] + W.b2[8 + t]));
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[p][r] = W.b3[r];
+      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.w3[0], __builtin_bit_cast(mlpBf16x8, g0[p]), acc[p], 0, 0, 0);
+      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.w3[1], __builtin_bit_cast(mlpBf16x8, g1[p]), acc[p], 0, 0, 0);
+    }
+    if (h == 0) {  // logits of sample col: units 0, 1, 2 = elements 0, 1, 2
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const float m = fmaxf(acc[p][0], fmaxf(acc[p][1], acc[p][2]));
+        const float e0 = mlpExp(acc[p][0] - m), e1 = mlpExp(acc[p][1] - m), e2 = mlpExp(acc[p][2] - m);
+        const float sum = (e0 + e1) + e2;
+        row[p][0] = e0 / sum;
+        row[p][1] = e1 / sum;
+        row[p][2] = e2 / sum;
+      }
+    }
+  }
+  waveSyncLds();
+}
+
+template <int D, int SCHEME, bool SEG, bool LWAVE = false>  // LWAVE: learned weno5, network per wave
 __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, const uint32_t *nact, RateParams rp,
                                               double *rates, DevScalars *sc, SegArgs sa, unsigned long long *segMax) {
   const uint32_t n = *nact;
@@ -1531,10 +1678,10 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
   const uint32_t segFirst = (SEG && lo < hi) ? sa.iseg[list[lo]] : 0u;
   __shared__ float mlpIO[SCHEME == S_WENO5L ? TPB * 2 * D * MLP_IN : 1];
   float *io = (SCHEME == S_WENO5L) ? &mlpIO[threadIdx.x * 2 * D * MLP_IN] : nullptr;
-  RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, io, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst, 0ull};
+  RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, io, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst, 0ull, 0};
   if (SCHEME == S_LF)
     for (int i = 0; i < 3; ++i) fn.ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
-  if (SEG) {
+  if constexpr (SEG) {
     for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) fn(list[j], j);
   } else {
     // Each XCD takes a contiguous slab of the active list in proportion to
@@ -1548,9 +1695,26 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
     const uint32_t first = (uint32_t)((uint64_t)n * b0 / gridDim.x);
     const uint32_t last = (uint32_t)((uint64_t)n * (b0 + nbx) / gridDim.x);
     const uint32_t wave = k * (TPB / 64) + (threadIdx.x >> 6), waves = nbx * (TPB / 64);
-    for (uint32_t base = first + wave * 64u; base < last; base += waves * 64u) {
-      const uint32_t j = base + lane;
-      if (j < last) fn(list[j], j);
+    if constexpr (SCHEME == S_WENO5L && LWAVE) {
+      // every lane of the wave takes part in the network of each 64 points
+      const MlpLane W = mlpLane(rp.mlp);
+      float *rows = &mlpIO[(threadIdx.x & ~63u) * 2 * D * MLP_IN];
+      fn.learnedPhase = 2;
+      for (uint32_t base = first + wave * 64u; base < last; base += waves * 64u) {
+        const uint32_t j = base + lane;
+        if (j < last) fn.prepareLearned(list[j]);
+        else
+          for (int q = 0; q < 2 * D * MLP_IN; ++q) io[q] = 0.f;
+        waveSyncLds();
+        waveMlp<64 * 2 * D>(rows, W);
+        if (j < last) fn(list[j], j);
+        waveSyncLds();  // the rows are rewritten by the next points
+      }
+    } else {
+      for (uint32_t base = first + wave * 64u; base < last; base += waves * 64u) {
+        const uint32_t j = base + lane;
+        if (j < last) fn(list[j], j);
+      }
     }
   }
   if (!SEG) {
@@ -1643,7 +1807,12 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
     switch (scheme) {
     case S_EO: LSG_RATES(2, S_EO); break;
     case S_WENO5: LSG_RATES(2, S_WENO5); break;
-    case S_WENO5L: LSG_RATES(2, S_WENO5L); break;
+    case S_WENO5L:
+      if (rp.learnedWave)
+        kRates<2, S_WENO5L, false, true><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, rp, rates, sc, noSeg(), nullptr);
+      else
+        LSG_RATES(2, S_WENO5L);
+      break;
     case S_LF: LSG_RATES(2, S_LF); break;
     case S_LLLF: LSG_RATES(2, S_LLLF); break;
     case S_LLFA: LSG_RATES(2, S_LLFA); break;
@@ -1655,7 +1824,12 @@ void launchRates(hipStream_t s, int D, const BandBufs &b, const Frame &f, const 
     switch (scheme) {
     case S_EO: LSG_RATES(3, S_EO); break;
     case S_WENO5: LSG_RATES(3, S_WENO5); break;
-    case S_WENO5L: LSG_RATES(3, S_WENO5L); break;
+    case S_WENO5L:
+      if (rp.learnedWave)
+        kRates<3, S_WENO5L, false, true><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, rp, rates, sc, noSeg(), nullptr);
+      else
+        LSG_RATES(3, S_WENO5L);
+      break;
     case S_LF: LSG_RATES(3, S_LF); break;
     case S_LLLF: LSG_RATES(3, S_LLLF); break;
     case S_LLFA: LSG_RATES(3, S_LLFA); break;

@@ -13,7 +13,8 @@
 // forward product H1 W2^T, the input gradient dZ2 W2 and the weight
 // gradient dZ2^T H1 (accumulated over all tiles of the wave in the MFMA
 // accumulators).  The 5->32 and 32->3 layers, GELU, softmax and loss run on
-// the vector ALUs in fp32.  Per-wave partial gradients go to a buffer that a
+// the vector ALUs in fp32, with the inputs and the layer-2 activations
+// rounded to bf16 where the inference's matrix products take them.  Per-wave partial gradients go to a buffer that a
 // second kernel sums in a fixed order (deterministic), then Adam updates the
 // fp32 master parameters.
 //
@@ -33,13 +34,15 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short i16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// GELU x sigmoid(2u) (mlpGelu) and its derivative
+// sigmoid(2u) + 2 x sigmoid(2u) (1 - sigmoid(2u)) u'.
 __device__ inline float geluD(float x, float &dg) {
   const float c = 0.7978845608028654f, a = 0.044715f;
   const float x2 = x * x;
   const float u = c * (x + a * x2 * x);
-  const float t = tanhf(u);
-  dg = 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * a * x2);
-  return 0.5f * x * (1.0f + t);
+  const float sg = mlpSigmoid2(u);
+  dg = sg + 2.0f * x * sg * (1.0f - sg) * c * (1.0f + 3.0f * a * x2);
+  return x * sg;
 }
 
 __device__ inline bf16x8 packBf16(const float *v) {
@@ -116,6 +119,8 @@ __global__ __launch_bounds__(64) void kMlpGrad(const float *X, const float *T, u
       for (int q = 0; q < 8; ++q) x[q] = X[(size_t)s * 8 + q];
       t = T[s];
     }
+    // the inputs enter layer 1 as bf16 (as in the matrix-core inference)
+    for (int i = 0; i < MLP_IN; ++i) x[i] = bf16Float(bf16Bits(x[i]));
     // layer 1 (VALU), keep gelu'
     float d1[MLP_H];
     for (int j = 0; j < MLP_H; ++j) {
@@ -143,14 +148,14 @@ __global__ __launch_bounds__(64) void kMlpGrad(const float *X, const float *T, u
     float z3[MLP_OUT] = {L.b3[0], L.b3[1], L.b3[2]};
     for (int j = 0; j < MLP_H; ++j) {
       const float z = L.buf1[l][j] + L.b2[j];
-      const float h = geluD(z, d2[j]);
+      const float h = bf16Float(bf16Bits(geluD(z, d2[j])));  // enters layer 3 as bf16
       L.buf2[l][j] = h;
       for (int k = 0; k < MLP_OUT; ++k) z3[k] += L.w3[k * MLP_H + j] * h;
     }
     const float m = fmaxf(z3[0], fmaxf(z3[1], z3[2]));
     float w[MLP_OUT], se = 0;
     for (int k = 0; k < MLP_OUT; ++k) {
-      w[k] = expf(z3[k] - m);
+      w[k] = mlpExp(z3[k] - m);
       se += w[k];
     }
     for (int k = 0; k < MLP_OUT; ++k) w[k] = w[k] / se;

@@ -474,6 +474,7 @@ def main(argv=None):
     c.add_argument("--shapes", type=int, default=2)
     c.add_argument("--time", type=float, default=8.0)
     c.add_argument("--seed", type=int, default=1234)
+    c.add_argument("--radius", type=float, default=32.0, help="mean shape radius in coarse cells")
     c.add_argument("--report")
     args = ap.parse_args(argv)
 
@@ -498,7 +499,7 @@ def main(argv=None):
     rng = np.random.default_rng(args.seed)
     out = []
     for s in range(args.shapes):
-        shape = StarShape(32.0, seed=int(rng.integers(1 << 30)))
+        shape = StarShape(args.radius, seed=int(rng.integers(1 << 30)))
         vel = random_velocity(rng)
         r = compare(shape, vel, args.time, params, log=log)
         out.append(r)
