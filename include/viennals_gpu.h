@@ -282,6 +282,11 @@ int lsg_advect_get_rate_kernel_stats(lsg_advect *a, double *ms, int64_t *launche
  * that takes most of a step; algorithmic bytes per launch are 8 per input
  * band entry plus 8 per candidate. */
 int lsg_advect_get_star_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes);
+/* No counterpart in the reference: the same figures for the evaluation
+ * kernel of two Expand cycles run as one band operation (the cycles of an
+ * Expand go two at a time on one level set without point data); bytes
+ * counted as above. */
+int lsg_advect_get_expand2_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes);
 
 /* Finite-difference kernels FiniteDifferences::weno3/weno5
  * (lsFiniteDifferences.hpp:58-123) evaluated on the GPU for n stencils:

@@ -298,6 +298,43 @@ template <int D> EBand starOp(const EBand &in, const OpParams &op) {
   return compact(in, cpos, cpre, out);
 }
 
+// Two Expand cycles in one operation (the candidate-list pipeline's
+// OP_EXPAND2): candidates from the star dilation of the star dilation.
+bool g_pairs = false;
+template <int D> EBand starOp2(const EBand &in, const OpParams &op) {
+  const int64_t nw = in.f.nwords;
+  std::vector<uint64_t> T(nw + 1, 0), C(nw + 1, 0);
+  for (int64_t w = 0; w < nw; ++w) T[w] = candStarWord<D>(in.bits.data(), in.f, in.g, w);
+  for (int64_t w = 0; w < nw; ++w) C[w] = candStarWord<D>(T.data(), in.f, in.g, w);
+  std::vector<uint32_t> cpos, cpre;
+  candidates(in, C, cpos, cpre);
+  BandView v = in.view();
+  std::vector<double> out(cpos.size());
+  for (uint32_t i = 0; i < cpos.size(); ++i) {
+    int32_t c[3];
+    candCoords(v, i, cpos[i], c);
+    out[i] = expandTwice<D>(v, c, op.a, op.a2, op.b);
+  }
+  return compact(in, cpos, cpre, out);
+}
+
+// updateLevelSet + rebuildLS as one star operation (OP_UPDATE_REBUILD).
+template <int D> EBand updRebuildOp(const EBand &in, const std::vector<double> &rates, double dt, double cutoff) {
+  const int64_t nw = in.f.nwords;
+  std::vector<uint64_t> C(nw + 1, 0);
+  for (int64_t w = 0; w < nw; ++w) C[w] = candStarWord<D>(in.bits.data(), in.f, in.g, w);
+  std::vector<uint32_t> cpos, cpre;
+  candidates(in, C, cpos, cpre);
+  BandView v = in.view();
+  std::vector<double> out(cpos.size());
+  for (uint32_t i = 0; i < cpos.size(); ++i) {
+    int32_t c[3];
+    candCoords(v, i, cpos[i], c);
+    out[i] = updateRebuild<D>(v, c, rates.data(), dt, cutoff);
+  }
+  return compact(in, cpos, cpre, out);
+}
+
 template <int D> EBand selfOp(const EBand &in, const OpParams &op, const std::vector<double> *rates, double dt) {
   if (g_tiled) {
     OpParams o2 = op;
@@ -353,12 +390,21 @@ template <int D> void doExpand(EBand &d, int width) {
   if (width <= d.width) return;
   const double totalLimit = width * 0.5;
   const int startWidth = d.width;
-  for (int cyc = 0; cyc < width - startWidth; ++cyc) {
+  const int cycles = width - startWidth;
+  for (int cyc = 0; cyc < cycles;) {
     OpParams op{};
-    op.kind = OP_EXPAND;
     op.a = (startWidth + cyc + 1) * 0.5;
     op.b = totalLimit;
-    d = starOp<D>(d, op);
+    if (g_pairs && !g_tiled && cycles - cyc >= 2) {
+      op.kind = OP_EXPAND2;
+      op.a2 = (startWidth + cyc + 2) * 0.5;
+      d = starOp2<D>(d, op);
+      cyc += 2;
+    } else {
+      op.kind = OP_EXPAND;
+      d = starOp<D>(d, op);
+      cyc += 1;
+    }
   }
   d.width = width;
 }
@@ -511,6 +557,11 @@ struct EAdvect {
     default: dt = ratesPass<D, S_WENO5>(d, maxTimeStep, ga); break;
     }
     OpParams op{};
+    if (g_pairs && !g_tiled) {  // the fused update and rebuild (OP_UPDATE_REBUILD)
+      d = updRebuildOp<D>(d, rates, dt, rp.spatial == 9 ? 1.5 : 1.0);
+      d.width = rp.spatial == 9 ? 3 : 2;
+      return dt;
+    }
     op.kind = OP_UPDATE;
     d = selfOp<D>(d, op, &rates, dt);
     d.width = 1;
@@ -721,6 +772,7 @@ int64_t emu_geometry(int kind, int D, const int32_t *gmin, const int32_t *gmax, 
 
 // 1: band operations through the single-pass tile pipeline; 0: candidate lists.
 void emu_set_tiled(int on) { g_tiled = on != 0; }
+void emu_set_expand_pairs(int on) { g_pairs = on != 0; }
 
 // Bit helpers of the index kernels on the host.
 void emu_deposit(const uint64_t *x, const uint64_t *m, uint64_t *out, int64_t n) {

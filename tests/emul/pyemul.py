@@ -41,6 +41,7 @@ def lib():
         L.emu_deposit.argtypes = [u64p, u64p, u64p, ctypes.c_int64]
         L.emu_select.argtypes = [u64p, i32p, i32p, ctypes.c_int64]
         L.emu_set_tiled.argtypes = [ctypes.c_int]
+        L.emu_set_expand_pairs.argtypes = [ctypes.c_int]
         L.emu_geometry.restype = ctypes.c_int64
         L.emu_geometry.argtypes = [ctypes.c_int, ctypes.c_int, i32p, i32p, ctypes.POINTER(ctypes.c_int),
                                    ctypes.c_double, dp, dp, ctypes.c_int, i32p, dp, ctypes.c_int64]
@@ -130,6 +131,12 @@ def select_bit(w, k):
     lib().emu_select(w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), k.ctypes.data_as(i32p),
                      out.ctypes.data_as(i32p), len(w))
     return out
+
+
+def set_expand_pairs(on):
+    """Expand two cycles per band operation and fuse each stage's update
+    with its rebuild (candidate-list pipeline)."""
+    lib().emu_set_expand_pairs(int(bool(on)))
 
 
 def set_tiled(on):

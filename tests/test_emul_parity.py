@@ -10,13 +10,17 @@ import pyemul as pe
 import pyoracle as po
 
 
-@pytest.fixture(autouse=True, params=["tiled", "lists"])
+@pytest.fixture(autouse=True, params=["tiled", "lists", "pairs"])
 def pipeline(request):
-    """Every case through both band-operation pipelines of the device code:
-    the single-pass tiles (one level set) and the candidate lists (batches)."""
+    """Every case through the band-operation pipelines of the device code:
+    the single-pass tiles, the candidate lists, and the candidate lists with
+    Expand cycles two at a time (OP_EXPAND2) and each stage's update fused
+    with its rebuild (OP_UPDATE_REBUILD)."""
     pe.set_tiled(request.param == "tiled")
+    pe.set_expand_pairs(request.param == "pairs")
     yield request.param
     pe.set_tiled(True)
+    pe.set_expand_pairs(False)
 
 
 def to_emul(o):

@@ -134,8 +134,10 @@ def test_learned_scheme_requires_weights():
 def test_trained_network_gain_over_plain_coarse():
     """The committed network (trained on fine WENO5-RK3 runs by the GPU
     trainer) against plain WENO5 on the same coarse grid, both judged against
-    the fine plain run of the same star shape: it must not lose accuracy,
-    and on this shape it reduces the surface distance."""
+    the fine plain run of the same star shape sampled on the coarse grid,
+    with CompareChamfer and CompareVolume: on this shape it reduces both the
+    Chamfer distance and the volume mismatch (the wall-time ratio to the fine
+    run is printed, not asserted: it is a timing)."""
     import os
     import viennals_amd as vls
     from viennals_amd import learned as L
@@ -145,7 +147,8 @@ def test_trained_network_gain_over_plain_coarse():
     vel = L.random_velocity(rng)
     r = L.compare(shape, vel, 6.0, P)
     plain, learned = r["coarse_plain"], r["coarse_learned"]
-    print("surface distance plain %.5f learned %.5f; volume error plain %.2e learned %.2e" %
-          (plain["surface_distance"], learned["surface_distance"], plain["volume_error"], learned["volume_error"]))
-    assert learned["surface_distance"] < plain["surface_distance"]
-    assert learned["chamfer"] <= plain["chamfer"] * 1.01
+    print("chamfer plain %.5f learned %.5f; volume error plain %.3e learned %.3e; speedup vs fine %.2f" %
+          (plain["chamfer"], learned["chamfer"], plain["volume_error"], learned["volume_error"],
+           learned["speedup_vs_fine"]))
+    assert learned["chamfer"] < plain["chamfer"]
+    assert learned["volume_mismatch"] <= plain["volume_mismatch"]

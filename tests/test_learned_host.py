@@ -38,3 +38,19 @@ def test_init_params_start_at_linear_weights():
     P = L.init_params(0)
     w = po.mlp_forward(P, np.array([0.2, 1.0, -0.3, 0.5, 0.1], np.float32))
     np.testing.assert_allclose(w, L.IDEAL, rtol=1e-6)
+
+
+def test_star_perturbation_matches_scipy_harmonics():
+    """The star radius perturbation (recurrence in l per order m) against
+    the same sum over scipy's complex spherical harmonics."""
+    from scipy.special import sph_harm_y
+    shape = L.StarShape(10.0, seed=11)
+    rng = np.random.default_rng(0)
+    th = np.arccos(rng.uniform(-1, 1, 2000))
+    ph = rng.uniform(0, 2 * np.pi, 2000)
+    th[:4] = [0.0, np.pi, np.pi / 2, 1e-9]  # poles and the equator
+    ref = np.zeros_like(th)
+    for l, m, a in shape.terms:
+        y = sph_harm_y(l, abs(m), th, ph)
+        ref += a * (np.real(y) if m == 0 else np.sqrt(2.0) * (np.real(y) if m > 0 else np.imag(y)))
+    np.testing.assert_allclose(shape._pert(th, ph), ref, rtol=0, atol=1e-12)

@@ -951,6 +951,15 @@ class Advect:
         check(lib().lsg_advect_get_star_kernel_stats(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)))
         return ms.value, n.value, b.value
 
+    def getExpand2KernelStats(self):
+        """(ms, launches, algorithmic bytes) of the kernel that evaluates two
+        Expand cycles at once, over the steps since setProfiling(True)."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        b = ctypes.c_double()
+        check(lib().lsg_advect_get_expand2_kernel_stats(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)))
+        return ms.value, n.value, b.value
+
 
 class BatchAdvect:
     """Many independent Advect instances (one level set each) stepped

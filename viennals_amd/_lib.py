@@ -32,7 +32,8 @@ EXPORTED = [
     "lsg_advect_set_adaptive_time_stepping", "lsg_advect_set_update_point_data", "lsg_advect_prepare_ls",
     "lsg_advect_apply", "lsg_advect_get_advected_time", "lsg_advect_get_current_time_step",
     "lsg_advect_get_number_of_time_steps", "lsg_advect_get_point_updates", "lsg_advect_set_profiling",
-    "lsg_advect_get_rate_kernel_stats", "lsg_advect_get_star_kernel_stats", "lsg_weno_eval", "lsg_device_synchronize",
+    "lsg_advect_get_rate_kernel_stats", "lsg_advect_get_star_kernel_stats", "lsg_advect_get_expand2_kernel_stats",
+    "lsg_weno_eval", "lsg_device_synchronize",
     "lsg_advect_set_learned_weights", "lsg_weno5_learned_eval", "lsg_mlp_create", "lsg_mlp_destroy",
     "lsg_mlp_set_samples", "lsg_mlp_loss_grad", "lsg_mlp_train", "lsg_mlp_get_params",
     "lsg_batch_create", "lsg_batch_destroy", "lsg_batch_set_velocity_builtin", "lsg_batch_set_schemes",
@@ -130,6 +131,7 @@ def lib():
         "lsg_advect_set_profiling": ([P, c_int], c_int),
         "lsg_advect_get_rate_kernel_stats": ([P, dp, i64p, dp], c_int),
         "lsg_advect_get_star_kernel_stats": ([P, dp, i64p, dp], c_int),
+        "lsg_advect_get_expand2_kernel_stats": ([P, dp, i64p, dp], c_int),
         "lsg_weno_eval": ([c_int, ctypes.c_int64, dp, c_double, c_int, dp], c_int),
         "lsg_advect_set_learned_weights": ([P, fp, c_int], c_int),
         "lsg_weno5_learned_eval": ([ctypes.c_int64, dp, c_double, c_int, fp, dp], c_int),

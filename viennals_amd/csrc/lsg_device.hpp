@@ -21,6 +21,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 
 // Full unrolling of the short per-dimension loops keeps the small local
@@ -55,25 +56,53 @@ struct GridInfo {
   double dx;
 
   LSG_HD bool bounded(int d) const { return bc[d] == REFLECTIVE || bc[d] == PERIODIC; }
+  // Where a single step from inside [lo, hi] lands when it leaves the
+  // axis below lo and above hi: one wrap, mirror or clamp is the whole of
+  // map() then (an axis of one point maps everything to it).  With these,
+  // map(d, x + s) = stepTo(x + s, ...) for x in [lo, hi] and s = +-1.
+  LSG_HD void stepTargets(int d, int32_t &below, int32_t &above) const {
+    if (hi[d] == lo[d]) {
+      below = above = lo[d];
+    } else if (bc[d] == PERIODIC) {
+      below = hi[d];
+      above = lo[d];
+    } else {
+      below = (bc[d] == REFLECTIVE || bc[d] == POS_INFINITE) ? lo[d] + 1 : lo[d];
+      above = (bc[d] == REFLECTIVE || bc[d] == NEG_INFINITE) ? hi[d] - 1 : hi[d];
+    }
+  }
+  LSG_HD static int32_t stepTo(int64_t y, int32_t lo_, int32_t hi_, int32_t below, int32_t above) {
+    return y < lo_ ? below : (y > hi_ ? above : (int32_t)y);
+  }
   // periodic wrap / mirror about the boundary point / clamp (same convention
-  // as the CPU restatement)
+  // as the CPU restatement).  A stencil step leaves the grid by a cell or
+  // two: one period or one mirror brings it back, so those come first and
+  // the division and the repeated mirrors only run for points further out.
   LSG_HD int32_t map(int d, int64_t c) const {
     if (c >= lo[d] && c <= hi[d]) return (int32_t)c;
+    const int64_t l = lo[d], h = hi[d];
     if (bc[d] == PERIODIC) {
-      int64_t L = (int64_t)hi[d] - lo[d] + 1;
-      int64_t m = (c - lo[d]) % L;
+      const int64_t L = h - l + 1;
+      if (c > h && c - L <= h) return (int32_t)(c - L);
+      if (c < l && c + L >= l) return (int32_t)(c + L);
+      int64_t m = (c - l) % L;
       if (m < 0) m += L;
-      return (int32_t)(lo[d] + m);
+      return (int32_t)(l + m);
     }
-    bool reflLo = bc[d] == REFLECTIVE || bc[d] == POS_INFINITE;
-    bool reflHi = bc[d] == REFLECTIVE || bc[d] == NEG_INFINITE;
-    for (int it = 0; it < 8; ++it) {
-      if (c < lo[d]) c = reflLo ? 2 * (int64_t)lo[d] - c : lo[d];
-      else if (c > hi[d]) c = reflHi ? 2 * (int64_t)hi[d] - c : hi[d];
+    const bool reflLo = bc[d] == REFLECTIVE || bc[d] == POS_INFINITE;
+    const bool reflHi = bc[d] == REFLECTIVE || bc[d] == NEG_INFINITE;
+    c = (c < l) ? (reflLo ? 2 * l - c : l) : (reflHi ? 2 * h - c : h);
+    if (c >= l && c <= h) return (int32_t)c;
+#if defined(__clang__)
+#pragma unroll 1
+#endif
+    for (int it = 1; it < 8; ++it) {
+      if (c < l) c = reflLo ? 2 * l - c : l;
+      else if (c > h) c = reflHi ? 2 * h - c : h;
       else break;
     }
-    if (c < lo[d]) c = lo[d];
-    if (c > hi[d]) c = hi[d];
+    if (c < l) c = l;
+    if (c > h) c = h;
     return (int32_t)c;
   }
 };
@@ -306,7 +335,9 @@ enum OpKind : int {
   OP_PRUNE_RSZ = 4, // Prune with removeStrayZeros
   OP_UPDATE = 5,    // Reduce(1) + value -= dt * rate
   OP_CHECK = 6,
-  OP_UPDATE_MM = 7  // Reduce(1) + update through the rate list of a point (several level sets)
+  OP_UPDATE_MM = 7, // Reduce(1) + update through the rate list of a point (several level sets)
+  OP_EXPAND2 = 8,   // two Expand cycles (limits a, then a2; totalLimit b)
+  OP_UPDATE_REBUILD = 9  // OP_UPDATE then Advect::rebuildLS (cutoff a) as one star operation
 };
 enum BoolKind : int { B_INTERSECT = 0, B_UNION = 1, B_RELCOMP = 2, B_COMBINE = 3 };
 
@@ -316,6 +347,7 @@ struct OpParams {
   const double *rates;
   const double *dt;  // device pointer (update)
   int nmat;          // OP_UPDATE_MM: rate-list slots per entry (level sets)
+  double a2;         // OP_EXPAND2: limit of the second cycle
 };
 
 }  // namespace lsg

@@ -121,7 +121,7 @@ struct WsStore {
   int64_t nwAlloc = 0;
   void release() {
     (void)hipFree(w.cbits);
-    (void)hipFree(w.cnt);
+    (void)hipFree(w.cbits2);
     (void)hipFree(w.cpre);
     (void)hipFree(w.cpos);
     (void)hipFree(w.out);
@@ -139,7 +139,7 @@ struct WsStore {
     // the candidate list kernel addresses its slots with 32-bit byte offsets
     if (ccap >= ((size_t)1 << 30)) return fail(-12, "band capacity exceeds 2^30 entries");
     LSG_HIP(hipMalloc(&w.cbits, (nw + 1) * sizeof(uint64_t)));
-    LSG_HIP(hipMalloc(&w.cnt, (nw + 1) * sizeof(uint32_t)));
+    LSG_HIP(hipMalloc(&w.cbits2, (nw + 1) * sizeof(uint64_t)));
     LSG_HIP(hipMalloc(&w.cpre, (nw + 1) * sizeof(uint32_t)));
     LSG_HIP(hipMalloc(&w.cpos, ccap * sizeof(uint32_t)));
     LSG_HIP(hipMalloc(&w.out, ccap * sizeof(double)));
@@ -458,20 +458,29 @@ struct StarTiming {
   int cap = 0, used = 0;
   bool external = false;
 };
-static StarTiming *g_starTiming = nullptr;
+static StarTiming *g_starTiming = nullptr;   // one-cycle star operations
+static StarTiming *g_star2Timing = nullptr;  // two Expand cycles at once
 
+static void takeEvents(StarTiming *t, hipEvent_t &e0, hipEvent_t &e1, bool &external) {
+  if (t && t->used + 2 <= t->cap) {
+    e0 = t->ev[t->used];
+    e1 = t->ev[t->used + 1];
+    t->used += 2;
+    external = t->external;
+  }
+}
 static void opStar(lsg_domain *d, const OpParams &op) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   bool external = false;
-  if (StarTiming *t = g_starTiming) {
-    if (t->used + 2 <= t->cap) {
-      e0 = t->ev[t->used];
-      e1 = t->ev[t->used + 1];
-      t->used += 2;
-      external = t->external;
-    }
-  }
+  takeEvents(g_starTiming, e0, e1, external);
   launchStarOp(g_ctx.s, d->D, d->cur.b, d->tmp.b, d->f, d->g, g_ws->w, op, g_ctx.sc, e0, e1, external);
+  swapCur(d);
+}
+static void opStar2(lsg_domain *d, const OpParams &op) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool external = false;
+  takeEvents(g_star2Timing, e0, e1, external);
+  launchStarOp2(g_ctx.s, d->D, d->cur.b, d->tmp.b, d->f, d->g, g_ws->w, op, g_ctx.sc, e0, e1, external);
   swapCur(d);
 }
 static void opSelf(lsg_domain *d, const OpParams &op) {
@@ -663,17 +672,49 @@ static int pdCopy(lsg_domain *dst, const lsg_domain *src) {
 
 // Expand<T,D>::apply (lsExpand.hpp:44-172); the early exit for a band
 // without defined points is not taken (its result is then unchanged).
+// LSG_EXPAND_PAIRS=1: with the multi-kernel band operations and no point
+// data to carry, the cycles run two at a time (OP_EXPAND2), the pairs
+// first: the candidates of a pair reach two cells beyond the band, so the
+// widest candidate set is the one of the last single cycle or pair, as with
+// single cycles.  Off by default: the pair's evaluation reads 31 points per
+// candidate and costs what the two operations it replaces cost.
+static bool expandPairs() {
+  static const bool on = [] {
+    const char *e = getenv("LSG_EXPAND_PAIRS");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+// The update of a stage and the rebuild after it as one band operation;
+// LSG_FUSE_UPDATE=0 keeps them apart.
+static bool fuseUpdateRebuild() {
+  static const bool on = [] {
+    const char *e = getenv("LSG_FUSE_UPDATE");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
 static void doExpand(lsg_domain *d, int width) {
   if (width <= d->width) return;
   const double totalLimit = width * 0.5;
   const int startWidth = d->width;
-  for (int cyc = 0; cyc < width - startWidth; ++cyc) {
+  const int cycles = width - startWidth;
+  const bool pairs = expandPairs() && !singlePassBandOps() && d->pd.empty();
+  for (int cyc = 0; cyc < cycles;) {
     OpParams op{};
-    op.kind = OP_EXPAND;
     op.a = (startWidth + cyc + 1) * 0.5;
     op.b = totalLimit;
-    opStar(d, op);
-    pdTranslate(d, PdSrcParams{SRC_EXPAND, totalLimit, 0});
+    if (pairs && cycles - cyc >= 2) {
+      op.kind = OP_EXPAND2;
+      op.a2 = (startWidth + cyc + 2) * 0.5;
+      opStar2(d, op);
+      cyc += 2;
+    } else {
+      op.kind = OP_EXPAND;
+      opStar(d, op);
+      pdTranslate(d, PdSrcParams{SRC_EXPAND, totalLimit, 0});
+      cyc += 1;
+    }
   }
   d->width = width;
 }
@@ -847,10 +888,10 @@ struct lsg_advect {
   int evUsed = 0;
   bool capturing = false;
   // star-operation evaluation kernel (profiling)
-  std::vector<hipEvent_t> starPool;
-  lsg::StarTiming starT;
-  double starMs = 0, starBytes = 0;
-  int64_t starLaunches = 0;
+  std::vector<hipEvent_t> starPool, star2Pool;
+  lsg::StarTiming starT, star2T;
+  double starMs = 0, starBytes = 0, star2Ms = 0, star2Bytes = 0;
+  int64_t starLaunches = 0, star2Launches = 0;
   // one captured graph per buffer parity of the step
   struct StepGraph {
     uint64_t key = 0;
@@ -894,6 +935,7 @@ struct lsg_advect {
     if (diss) (void)hipFree(diss);
     for (auto e : evPool) (void)hipEventDestroy(e);
     for (auto e : starPool) (void)hipEventDestroy(e);
+    for (auto e : star2Pool) (void)hipEventDestroy(e);
     if (hostMaxDt) (void)hipHostFree(hostMaxDt);
     if (hostSc) (void)hipHostFree(hostSc);
     if (hostStat) (void)hipHostFree(hostStat);
@@ -1060,11 +1102,20 @@ static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *d
   } else {
     launchRates(g_ctx.s, d->D, d->cur.b, d->f, d->g, rp, a->rates, g_ctx.sc, maxDt, dtOut, g_ws->w);
   }
-  // updateLevelSet: Reduce(1) + update (lsAdvect.hpp:686-800)
+  // updateLevelSet: Reduce(1) + update (lsAdvect.hpp:686-800), then
+  // rebuildLS; one star operation when no point data follows the band
   OpParams op{};
-  op.kind = OP_UPDATE;
   op.rates = a->rates;
   op.dt = dtOut;
+  if (fuseUpdateRebuild() && !singlePassBandOps() && d->pd.empty() && !a->saveVelocities) {
+    op.kind = OP_UPDATE_REBUILD;
+    op.a = (a->spatial == 9) ? 1.5 : 1.0;  // doRebuild's cutoff
+    launchUpdRebuildOp(g_ctx.s, d->D, d->cur.b, d->tmp.b, d->f, d->g, g_ws->w, op, g_ctx.sc);
+    swapCur(d);
+    d->width = (a->spatial == 9) ? 3 : 2;
+    return;
+  }
+  op.kind = OP_UPDATE;
   opSelf(d, op);
   pdUpdate(a, d, dtOut, 1);
   d->width = 1;
@@ -1203,6 +1254,8 @@ static uint64_t stepKey(lsg_advect *a, lsg_domain *d) {
   add(&a->cfl, sizeof(double));
   add(&a->rates, sizeof(double *));
   add(&a->profiling, sizeof(bool));
+  const bool singlePass = singlePassBandOps();  // the band-operation kernels a capture records
+  add(&singlePass, sizeof(bool));
   return h;
 }
 
@@ -1213,9 +1266,14 @@ static int enqueueStepBody(lsg_advect *a, lsg_domain *d) {
   copyDomainNoAlloc(a->save, d);
   a->evUsed = 0;
   a->starT = StarTiming{a->starPool.data(), (int)a->starPool.size(), 0, a->capturing};
-  if (a->profiling) g_starTiming = &a->starT;
+  a->star2T = StarTiming{a->star2Pool.data(), (int)a->star2Pool.size(), 0, a->capturing};
+  if (a->profiling) {
+    g_starTiming = &a->starT;
+    g_star2Timing = &a->star2T;
+  }
   const int r = enqueueStep(a, d);
   g_starTiming = nullptr;
+  g_star2Timing = nullptr;
   if (r) return r;
   launchFrameCheck(g_ctx.s, d->cur.b, d->f, d->g, d->D, REFRAME_SLACK, g_ctx.sc);
   return 0;
@@ -1331,6 +1389,13 @@ static int runStep(lsg_advect *a, lsg_domain *d, double maxTimeStep, StepOut &ou
           if (hipEventElapsedTime(&ms, a->starPool[k], a->starPool[k + 1]) == hipSuccess) {
             a->starMs += ms;
             a->starLaunches += 1;
+          }
+        }
+        for (int k = 0; k + 1 < a->star2T.used; k += 2) {
+          float ms = 0;
+          if (hipEventElapsedTime(&ms, a->star2Pool[k], a->star2Pool[k + 1]) == hipSuccess) {
+            a->star2Ms += ms;
+            a->star2Launches += 1;
           }
         }
       }
@@ -1583,6 +1648,8 @@ static int advectApply(lsg_advect *a) {
     for (auto &e : a->evPool) LSG_HIP(hipEventCreate(&e));
     a->starPool.resize(96);
     for (auto &e : a->starPool) LSG_HIP(hipEventCreate(&e));
+    a->star2Pool.resize(64);
+    for (auto &e : a->star2Pool) LSG_HIP(hipEventCreate(&e));
   }
   if (const char *ng = getenv("LSG_NO_GRAPH")) a->useGraphs = (ng[0] == '0');
   a->pointUpdates = 0;
@@ -1611,6 +1678,7 @@ static int advectApply(lsg_advect *a) {
   if (a->profiling) {
     a->rateBytes = a->hostSc->rateBytes;
     a->starBytes = a->hostSc->starBytes;
+    a->star2Bytes = a->hostSc->star2Bytes;
   }
   return 0;
 }
@@ -3017,6 +3085,7 @@ int lsg_advect_set_profiling(lsg_advect *a, int on) {
   double z = 0;
   LSG_HIP(hipMemcpy(&g_ctx.sc->rateBytes, &z, sizeof(double), hipMemcpyHostToDevice));
   LSG_HIP(hipMemcpy(&g_ctx.sc->starBytes, &z, sizeof(double), hipMemcpyHostToDevice));
+  LSG_HIP(hipMemcpy(&g_ctx.sc->star2Bytes, &z, sizeof(double), hipMemcpyHostToDevice));
   a->profiling = on != 0;
   a->rateMs = 0;
   a->rateLaunches = 0;
@@ -3024,6 +3093,9 @@ int lsg_advect_set_profiling(lsg_advect *a, int on) {
   a->starMs = 0;
   a->starLaunches = 0;
   a->starBytes = 0;
+  a->star2Ms = 0;
+  a->star2Launches = 0;
+  a->star2Bytes = 0;
   return 0;
 }
 int lsg_advect_get_rate_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes) {
@@ -3036,6 +3108,12 @@ int lsg_advect_get_star_kernel_stats(lsg_advect *a, double *ms, int64_t *launche
   *ms = a->starMs;
   *launches = a->starLaunches;
   *bytes = a->starBytes;
+  return 0;
+}
+int lsg_advect_get_expand2_kernel_stats(lsg_advect *a, double *ms, int64_t *launches, double *bytes) {
+  *ms = a->star2Ms;
+  *launches = a->star2Launches;
+  *bytes = a->star2Bytes;
   return 0;
 }
 

@@ -273,23 +273,44 @@ LSG_HD inline double updateRateMM(double c, double dt, const double *list, int &
 
 // ------------------------------------------------------- neighbourhood ----
 
-// Centre value and the 2D star neighbours (boundary mapped) of point c:
-// nb[d] = +e_d, nb[D + d] = -e_d.  All 2D + 1 lookups are issued together.
-// Frame bit index of point c when c and every point up to R steps from it
-// along each axis lie inside the grid bounds (so boundary mapping is the
-// identity) and inside the frame; -1 otherwise.  Such neighbours are at
-// fixed bit offsets (1, rowBits, ext[1] rowBits) from c and their locate()
-// is exactly that index with inside = true.
-template <int D> LSG_HD inline int64_t interiorLin(const BandView &b, const int32_t c[3], int R) {
+// Frame bit index of grid point c and, per axis d and step k (|k| <= R),
+// the frame-bit offset off[d][R + k] of map(d, c_d + k) from c, when every
+// such point is a frame bit: c inside the grid, each bounded axis (which
+// spans the frame) at least R + 1 points long, so that one wrap or mirror
+// is the whole mapping of a step, and c at least R cells inside the frame
+// along each unbounded axis, where the mapping is the identity.  -1
+// otherwise.  A located point is then exactly that index with inside =
+// true, and a boundary costs a few selects instead of locate() and map().
+template <int D> LSG_HD inline int64_t axisStride(const BandView &b, int d);
+template <int D, int R> LSG_HD inline int64_t mappedLin(const BandView &b, const int32_t c[3], int64_t off[][2 * R + 1]) {
   bool ok = true;
   LSG_UNROLL
-  for (int d = 0; d < D; ++d) ok = ok && c[d] - R >= b.g.lo[d] && c[d] + R <= b.g.hi[d];
+  for (int d = 0; d < D; ++d) {
+    const int64_t lo = b.g.lo[d], hi = b.g.hi[d];
+    const int64_t st = axisStride<D>(b, d);
+    if (b.g.bounded(d)) {
+      ok = ok && c[d] >= lo && c[d] <= hi && hi - lo >= R;
+      const bool per = b.g.bc[d] == PERIODIC;
+      const int64_t L = hi - lo + 1;
+      LSG_UNROLL
+      for (int k = -R; k <= R; ++k) {
+        const int64_t y = (int64_t)c[d] + k;
+        int64_t m = y;
+        if (k < 0 && y < lo) m = per ? y + L : 2 * lo - y;
+        if (k > 0 && y > hi) m = per ? y - L : 2 * hi - y;
+        off[d][R + k] = (m - c[d]) * st;
+      }
+    } else {
+      const int64_t r = (int64_t)c[d] - b.f.o[d];
+      ok = ok && r >= R && r + R < b.f.ext[d];
+      LSG_UNROLL
+      for (int k = -R; k <= R; ++k) off[d][R + k] = k * st;
+    }
+  }
   const int64_t xr = (int64_t)c[0] - b.f.o[0];
   const int64_t yr = (int64_t)c[1] - b.f.o[1];
   const int64_t zr = (int64_t)c[2] - b.f.o[2];
-  ok = ok && xr >= R && xr + R < b.f.ext[0] && yr >= R && yr + R < b.f.ext[1];
-  if (D == 3) ok = ok && zr >= R && zr + R < b.f.ext[2];
-  else ok = ok && zr >= 0 && zr < b.f.ext[2];
+  if (D == 2) ok = ok && zr >= 0 && zr < b.f.ext[2];
   return ok ? ((zr * b.f.ext[1] + yr) * (int64_t)b.f.rowBits() + xr) : -1;
 }
 LSG_HD inline BandView::Loc locAt(int64_t lin) {
@@ -303,15 +324,17 @@ template <int D> LSG_HD inline int64_t axisStride(const BandView &b, int d) {
   return d == 0 ? 1 : (d == 1 ? (int64_t)b.f.rowBits() : (int64_t)b.f.ext[1] * b.f.rowBits());
 }
 
+// Centre value and the 2D star neighbours (boundary mapped) of point c:
+// nb[d] = +e_d, nb[D + d] = -e_d.  All 2D + 1 lookups are issued together.
 template <int D> LSG_HD inline void starLocs(const BandView &b, const int32_t c[3], BandView::Loc *l) {
-  const int64_t lin = interiorLin<D>(b, c, 1);
+  int64_t off[3][3];
+  const int64_t lin = mappedLin<D, 1>(b, c, off);
   if (lin >= 0) {
     l[2 * D] = locAt(lin);
     LSG_UNROLL
     for (int d = 0; d < D; ++d) {
-      const int64_t st = axisStride<D>(b, d);
-      l[d] = locAt(lin + st);
-      l[D + d] = locAt(lin - st);
+      l[d] = locAt(lin + off[d][2]);
+      l[D + d] = locAt(lin + off[d][0]);
     }
     return;
   }
@@ -340,15 +363,15 @@ template <int D> LSG_HD inline void gatherStar(const BandView &b, const int32_t 
 // caller (the centre value).  All 2 D R lookups are issued together.
 template <int D, int R> LSG_HD inline void gatherAxes(const BandView &b, const int32_t c[3], double st[][2 * R + 1]) {
   BandView::Loc l[2 * D * R];
-  const int64_t lin = interiorLin<D>(b, c, R);
+  int64_t off[3][2 * R + 1];
+  const int64_t lin = mappedLin<D, R>(b, c, off);
   if (lin >= 0) {
     LSG_UNROLL
     for (int d = 0; d < D; ++d) {
-      const int64_t sd = axisStride<D>(b, d);
       LSG_UNROLL
       for (int k = 1; k <= R; ++k) {
-        l[(d * R + k - 1) * 2] = locAt(lin + k * sd);
-        l[(d * R + k - 1) * 2 + 1] = locAt(lin - k * sd);
+        l[(d * R + k - 1) * 2] = locAt(lin + off[d][R + k]);
+        l[(d * R + k - 1) * 2 + 1] = locAt(lin + off[d][R - k]);
       }
     }
   } else {
@@ -375,6 +398,205 @@ template <int D, int R> LSG_HD inline void gatherAxes(const BandView &b, const i
       st[d][R - k] = v[(d * R + k - 1) * 2 + 1];
     }
   }
+}
+
+// updateLevelSet followed by rebuildLS (ruleUpdate, then ruleRebuild with
+// the given cutoff) at point c: the rebuild rule on the updated values of
+// c's star, each the update rule of the input entry there and its rate
+// (rates are indexed by input entry, like the update operation's).  A
+// point that is not a candidate of the rebuild sees undefined values only
+// (the update keeps an undefined entry's sign), so it gets the value its
+// run holds in the rebuilt band.
+template <int D>
+LSG_HD inline double updateRebuild(const BandView &b, const int32_t c[3], const double *rates, double dt,
+                                   double cutoff) {
+  constexpr int N = 2 * D + 1;
+  BandView::Loc l[N];
+  starLocs<D>(b, c, l);
+  uint64_t wd[N];
+  uint32_t pr[N];
+  LSG_UNROLL
+  for (int k = 0; k < N; ++k) {
+    wd[k] = b.bits[l[k].w];
+    pr[k] = b.pre[l[k].w];
+  }
+  double u[N];
+  LSG_UNROLL
+  for (int k = 0; k < N; ++k) {
+    const uint32_t e = b.entryOf(l[k], wd[k], pr[k]);
+    u[k] = ruleUpdate(b.val[e], dt, rates[e]);
+  }
+  return ruleRebuild<D>(u[2 * D], u, cutoff);
+}
+
+// ruleExpand from the minimum and the maximum of the star's values: the
+// rule's minimum of (neighbour + 1) started at POS is (minimum + 1) and its
+// maximum of (neighbour - 1) started at NEG is (maximum - 1), exactly, since
+// rounding is monotone (POS + 1 rounds to POS).
+LSG_HD inline double ruleExpandMinMax(double c, double vmin, double vmax, double limit, double totalLimit) {
+  if (fabs(c) <= totalLimit) return c;
+  if (c > -DEPS) {
+    const double dist = vmin + 1.0;
+    return (dist <= limit) ? dist : POS;
+  }
+  const double dist = vmax - 1.0;
+  return (dist >= -limit) ? dist : NEG;
+}
+LSG_HD inline void minMaxAcc(double v, double &vmin, double &vmax) {
+  vmin = (v < vmin) ? v : vmin;
+  vmax = (v > vmax) ? v : vmax;
+}
+
+// Two Expand cycles at once (ruleExpand with limit lim1, then lim2; final
+// limit tot) at point c of the grid, from the input band within two star
+// steps of c: the first cycle's rule at c and at each of its boundary-mapped
+// star neighbours, then the second cycle's rule at c on those values -- the
+// composition of two cycles.  A point that is not a candidate of the first
+// cycle sees undefined values only, so the rule gives it the value its run
+// holds in the first cycle's result.  The steps are boundary mapped one at
+// a time (a mirror boundary makes two steps along one axis depend on the
+// path): m1[d][s] = map(c_d +- 1) (s = 0: +, 1: -) is a star point's
+// coordinate along d, m2[d][s][t] = map(m1[d][s] +- 1) the steps along d
+// from it (the centre and c +- 2 e_d away from boundaries), and the
+// diagonals have both axes at their m1 (the same from either star point:
+// the axes map independently).  Bounded axes span the frame, so wherever c
+// lies two cells inside the frame along the unbounded axes every point is
+// a frame bit at a fixed offset from c; elsewhere each point is located.
+// The points are read in two batches to bound the loads in flight.
+template <int D>
+LSG_HD inline double expandTwice(const BandView &b, const int32_t c[3], double lim1, double lim2, double tot) {
+  int32_t m1[3][2], m2[3][2][2];
+  int64_t o1[3][2], o2[3][2][2];  // frame-bit offsets of the mapped coordinates from c
+  bool fast = true;
+  LSG_UNROLL
+  for (int d = 0; d < D; ++d) {
+#ifdef LSG_HOST_ONLY
+    if (c[d] < b.g.lo[d] || c[d] > b.g.hi[d]) std::abort();  // the host build checks that c is a grid point
+#endif
+    int32_t below, above;
+    b.g.stepTargets(d, below, above);
+    const int64_t st = axisStride<D>(b, d);
+    LSG_UNROLL
+    for (int s = 0; s < 2; ++s) {
+      m1[d][s] = GridInfo::stepTo((int64_t)c[d] + (s ? -1 : 1), b.g.lo[d], b.g.hi[d], below, above);
+      o1[d][s] = (int64_t)(m1[d][s] - c[d]) * st;
+      LSG_UNROLL
+      for (int t = 0; t < 2; ++t) {
+        m2[d][s][t] = GridInfo::stepTo((int64_t)m1[d][s] + (t ? -1 : 1), b.g.lo[d], b.g.hi[d], below, above);
+        o2[d][s][t] = (int64_t)(m2[d][s][t] - c[d]) * st;
+      }
+    }
+    if (!b.g.bounded(d)) {
+      const int64_t r = (int64_t)c[d] - b.f.o[d];
+      fast = fast && r >= 2 && r + 2 < b.f.ext[d];
+    }
+  }
+  const int64_t lin0 = ((int64_t)(c[2] - b.f.o[2]) * b.f.ext[1] + (c[1] - b.f.o[1])) * (int64_t)b.f.rowBits() +
+                       (c[0] - b.f.o[0]);
+  // batch 1: slot 0 the centre, 1 + 2d + s the star points, then per axis
+  // pair lo < hi the four diagonals (lo at m1[lo][k >> 1], hi at
+  // m1[hi][k & 1])
+  constexpr int N1 = 1 + 2 * D + 2 * D * (D - 1);
+  double v1c, qm[3][2], qx[3][2], sv[3][2];
+  {
+    BandView::Loc l[N1];
+    if (fast) {
+      l[0] = locAt(lin0);
+      LSG_UNROLL
+      for (int d = 0; d < D; ++d)
+        LSG_UNROLL
+        for (int s = 0; s < 2; ++s) l[1 + 2 * d + s] = locAt(lin0 + o1[d][s]);
+      LSG_UNROLL
+      for (int lo = 0; lo < D; ++lo)
+        LSG_UNROLL
+        for (int hi = lo + 1; hi < D; ++hi) {
+          const int q = 1 + 2 * D + 4 * (lo * (2 * D - lo - 1) / 2 + (hi - lo - 1));
+          LSG_UNROLL
+          for (int k = 0; k < 4; ++k) l[q + k] = locAt(lin0 + o1[lo][k >> 1] + o1[hi][k & 1]);
+        }
+    } else {
+      l[0] = b.locate(c[0], c[1], c[2]);
+      LSG_UNROLL
+      for (int d = 0; d < D; ++d)
+        LSG_UNROLL
+        for (int s = 0; s < 2; ++s) {
+          int32_t p[3] = {c[0], c[1], c[2]};
+          p[d] = m1[d][s];
+          l[1 + 2 * d + s] = b.locate(p[0], p[1], p[2]);
+        }
+      LSG_UNROLL
+      for (int lo = 0; lo < D; ++lo)
+        LSG_UNROLL
+        for (int hi = lo + 1; hi < D; ++hi) {
+          const int q = 1 + 2 * D + 4 * (lo * (2 * D - lo - 1) / 2 + (hi - lo - 1));
+          LSG_UNROLL
+          for (int k = 0; k < 4; ++k) {
+            int32_t p[3] = {c[0], c[1], c[2]};
+            p[lo] = m1[lo][k >> 1];
+            p[hi] = m1[hi][k & 1];
+            l[q + k] = b.locate(p[0], p[1], p[2]);
+          }
+        }
+    }
+    double v[N1];
+    b.values<N1>(l, v);
+    double mn = POS, mx = NEG;
+    LSG_UNROLL
+    for (int k = 0; k < 2 * D; ++k) minMaxAcc(v[1 + k], mn, mx);
+    v1c = ruleExpandMinMax(v[0], mn, mx, lim1, tot);
+    LSG_UNROLL
+    for (int a = 0; a < D; ++a)
+      LSG_UNROLL
+      for (int s = 0; s < 2; ++s) {
+        sv[a][s] = v[1 + 2 * a + s];
+        qm[a][s] = POS;
+        qx[a][s] = NEG;
+        LSG_UNROLL
+        for (int d = 0; d < D; ++d) {
+          if (d == a) continue;
+          const int lo = a < d ? a : d, hi = a < d ? d : a;
+          const int q = 1 + 2 * D + 4 * (lo * (2 * D - lo - 1) / 2 + (hi - lo - 1));
+          // axis a fixed at m1[a][s], axis d at both of its m1
+          if (a < d) {
+            minMaxAcc(v[q + 2 * s], qm[a][s], qx[a][s]);
+            minMaxAcc(v[q + 2 * s + 1], qm[a][s], qx[a][s]);
+          } else {
+            minMaxAcc(v[q + s], qm[a][s], qx[a][s]);
+            minMaxAcc(v[q + 2 + s], qm[a][s], qx[a][s]);
+          }
+        }
+      }
+  }
+  // batch 2: slot 4d + 2s + t the point with axis d at m2[d][s][t]
+  double mn2 = POS, mx2 = NEG;
+  {
+    BandView::Loc l[4 * D];
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d)
+      LSG_UNROLL
+      for (int s = 0; s < 2; ++s)
+        LSG_UNROLL
+        for (int t = 0; t < 2; ++t) {
+          if (fast) {
+            l[4 * d + 2 * s + t] = locAt(lin0 + o2[d][s][t]);
+          } else {
+            int32_t p[3] = {c[0], c[1], c[2]};
+            p[d] = m2[d][s][t];
+            l[4 * d + 2 * s + t] = b.locate(p[0], p[1], p[2]);
+          }
+        }
+    double w[4 * D];
+    b.values<4 * D>(l, w);
+    LSG_UNROLL
+    for (int a = 0; a < D; ++a)
+      LSG_UNROLL
+      for (int s = 0; s < 2; ++s) {
+        minMaxAcc(w[4 * a + 2 * s], qm[a][s], qx[a][s]);
+        minMaxAcc(w[4 * a + 2 * s + 1], qm[a][s], qx[a][s]);
+        minMaxAcc(ruleExpandMinMax(sv[a][s], qm[a][s], qx[a][s], lim1, tot), mn2, mx2);
+      }
+  }
+  return ruleExpandMinMax(v1c, mn2, mx2, lim2, tot);
 }
 
 // Coordinates of candidate / entry i at frame position p (entry 0 is the
@@ -1048,14 +1270,25 @@ LSG_HD inline void weno5LearnedAxes(double sts[][7], double dx, const float *P, 
   }
 }
 
+// Learned weno5 in two phases can keep the axis values of the first in
+// `stc` (element q of the D x 7 values at stc[q * stcStride]): the second
+// phase then reads them there instead of gathering them again.
 template <int D, int ORDER, bool LEARNED = false>
 LSG_HD inline RateOut schemeWENO(const BandView &b, const int32_t c[3], double phi0, bool calcNormals, const Vel &vel,
-                                 const float *mlp = nullptr, float *io = nullptr, int phase = 0) {
+                                 const float *mlp = nullptr, float *io = nullptr, int phase = 0,
+                                 const double *stc = nullptr, int stcStride = 0) {
   constexpr int r = (ORDER + 1) / 2;
   double gradPosTotal = 0, gradNegTotal = 0;
   double gm[D], gp[D];
   double sts[D][2 * r + 1];
-  gatherAxes<D, r>(b, c, sts);
+  if (LEARNED && phase == 2 && stc) {
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d)
+      LSG_UNROLL
+      for (int k = 0; k < 2 * r + 1; ++k) sts[d][k] = stc[(d * (2 * r + 1) + k) * stcStride];
+  } else {
+    gatherAxes<D, r>(b, c, sts);
+  }
   double cd[D];  // central differences of the normal (lsWENO.hpp:113-129)
   LSG_UNROLL
   for (int i = 0; i < D; i++) cd[i] = sts[i][r + 1] - sts[i][r - 1];
@@ -1330,11 +1563,12 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
 template <int D> LSG_HD inline void gatherBox(const BandView &b, const int32_t c[3], double *box) {
   constexpr int NB = (D == 3) ? 27 : 9;
   BandView::Loc l[NB];
-  const int64_t lin = interiorLin<D>(b, c, 1);
+  int64_t off[3][3];
+  const int64_t lin = mappedLin<D, 1>(b, c, off);
   if (lin >= 0) {
-    const int64_t sy = axisStride<D>(b, 1), sz = (D == 3) ? axisStride<D>(b, 2) : 0;
     LSG_UNROLL
-    for (int q = 0; q < NB; ++q) l[q] = locAt(lin + (q % 3 - 1) + ((q / 3) % 3 - 1) * sy + (q / 9 - 1) * sz);
+    for (int q = 0; q < NB; ++q)
+      l[q] = locAt(lin + off[0][q % 3] + off[1][(q / 3) % 3] + (D == 3 ? off[2][q / 9] : 0));
   } else {
     LSG_UNROLL
     for (int q = 0; q < NB; ++q) {
@@ -1543,7 +1777,8 @@ template <int D, int SCHEME>
 LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi0, const RateParams &rp, const Vel &vel,
                                const double *globalAlphas, double *alphaAcc, double &velocity,
                                const float *mlp = nullptr, float *io = nullptr, bool *sllfUsed = nullptr,
-                               double *dnOut = nullptr, int learnedPhase = 0) {
+                               double *dnOut = nullptr, int learnedPhase = 0, const double *stc = nullptr,
+                               int stcStride = 0) {
   RateOut r;
   if (SCHEME == S_SLLF) {
     bool used = false;
@@ -1552,7 +1787,8 @@ LSG_HD inline double ratePoint(const BandView &b, const int32_t c[3], double phi
   } else if (SCHEME == S_EO) r = schemeEO<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO5) r = schemeWENO<D, 5>(b, c, phi0, rp.calcNormals != 0, vel);
   else if (SCHEME == S_WENO3) r = schemeWENO<D, 3>(b, c, phi0, rp.calcNormals != 0, vel);
-  else if (SCHEME == S_WENO5L) r = schemeWENO<D, 5, true>(b, c, phi0, rp.calcNormals != 0, vel, mlp, io, learnedPhase);
+  else if (SCHEME == S_WENO5L) r = schemeWENO<D, 5, true>(b, c, phi0, rp.calcNormals != 0, vel, mlp, io, learnedPhase,
+                                                             stc, stcStride);
   else if (SCHEME == S_LF) r = schemeLF<D>(b, c, phi0, rp.order, rp.calcNormals != 0, vel, rp.alphaFactor, globalAlphas);
   else if (SCHEME == S_LLLF) r = schemeLLLF<D>(b, c, phi0, rp.order, vel, rp.alphaFactor, alphaAcc);
   else if (SCHEME == S_LLFA) r = schemeLLFA<D>(b, c, phi0, vel, alphaAcc);

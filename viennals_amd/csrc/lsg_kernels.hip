@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t
 // ------------------------------------------------- evaluators of rules ----
 
 template <int D, bool SEG> struct EvalStar {
-  static constexpr bool kStar = !SEG;  // single band: algorithmic bytes are counted
+  static constexpr int kCount = SEG ? 0 : 1;  // single band: algorithmic bytes counted in starBytes
   BandView in;
   SegArgs sa;
   const uint32_t *cpos;
@@ -287,8 +287,40 @@ template <int D, bool SEG> struct EvalStar {
   }
 };
 
+// Two Expand cycles (OP_EXPAND2) at every candidate of the twice-dilated
+// candidate bitmap (one level set).
+template <int D> struct EvalStar2 {
+  static constexpr int kCount = 2;  // algorithmic bytes counted in star2Bytes
+  BandView in;
+  const uint32_t *cpos;
+  OpParams op;
+  __device__ uint32_t inputEntries() const { return in.pre[in.f.nwords] + in.rb; }
+  // inlined into the kernel: as a call it passed the evaluator through
+  // scratch memory
+  __device__ __attribute__((always_inline)) double operator()(uint32_t i) const {
+    int32_t c[3];
+    candCoords(in, i, cpos[i], c);
+    return expandTwice<D>(in, c, op.a, op.a2, op.b);
+  }
+};
+
+// updateLevelSet and rebuildLS (OP_UPDATE_REBUILD) at every candidate of
+// the star dilation of the band the rates belong to (one level set).
+template <int D> struct EvalUpdRebuild {
+  static constexpr int kCount = 0;
+  BandView in;
+  const uint32_t *cpos;
+  OpParams op;
+  __device__ uint32_t inputEntries() const { return 0; }
+  __device__ __attribute__((always_inline)) double operator()(uint32_t i) const {
+    int32_t c[3];
+    candCoords(in, i, cpos[i], c);
+    return updateRebuild<D>(in, c, op.rates, *op.dt, op.a);
+  }
+};
+
 template <int D, bool SEG> struct EvalSelf {
-  static constexpr bool kStar = false;
+  static constexpr int kCount = 0;
   __device__ uint32_t inputEntries() const { return 0; }
   BandView in;
   SegArgs sa;
@@ -309,7 +341,7 @@ template <int D, bool SEG> struct EvalSelf {
 };
 
 template <bool SEG> struct EvalBool {
-  static constexpr bool kStar = false;
+  static constexpr int kCount = 0;
   __device__ uint32_t inputEntries() const { return 0; }
   BandView A, B;
   SegArgs sa;
@@ -387,6 +419,8 @@ __global__ __launch_bounds__(256) void kEvalKeep(Ev ev, const uint32_t *nptr, do
   const uint32_t from = (lo > 0 && lo < hi) ? lo - 1 : lo;
   double carry = 0.0;  // last value of the previous tile
   uint32_t acc = 0;
+  // one tile at a time: the larger evaluators hold many loads in flight
+#pragma unroll 1
   for (uint32_t t = from; t < hi; t += TPB) {
     const uint32_t i = t + threadIdx.x;
     const bool ok = i >= lo && i < hi;
@@ -400,8 +434,10 @@ __global__ __launch_bounds__(256) void kEvalKeep(Ev ev, const uint32_t *nptr, do
   // algorithmic bytes of a star operation's evaluation: every input entry
   // read once (the star supports lie in the band), one value written per
   // candidate
-  if constexpr (Ev::kStar)
+  if constexpr (Ev::kCount == 1)
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->starBytes += 8.0 * ((double)ev.inputEntries() + (double)n);
+  if constexpr (Ev::kCount == 2)
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->star2Bytes += 8.0 * ((double)ev.inputEntries() + (double)n);
 }
 
 // Phase 2: rescan the keep flags from the block offset, scatter the kept
@@ -1083,6 +1119,37 @@ static void starOp(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame
                                      externalEvents);
 }
 
+// Two Expand cycles in one band operation: the candidates are the star
+// dilation of the star dilation of the input's run starts (a superset of
+// the second cycle's visits: the first cycle's run starts are among its
+// candidates), each evaluated with expandTwice.
+template <int D>
+static void starOp2(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g, uint32_t nw,
+                    Workspace &ws, const OpParams &op, DevScalars *sc, hipEvent_t e0, hipEvent_t e1,
+                    bool externalEvents) {
+  const SegArgs sa0 = noSeg();
+  kCandCount<D, 0, false><<<g_scanBlocks, TPB, 0, s>>>(in.bits, nullptr, f, g, nw, ws.cbits2, ws.partials, sc, sa0);
+  candidates<D, 0, false>(s, ws.cbits2, nullptr, f, g, nw, ws, sc, sa0);
+  SegArgs sa = sa0;
+  sa.iseg = ws.cseg;
+  sa.oseg = out.seg;
+  EvalStar2<D> ev{viewOf(in, f, g), ws.cpos, op};
+  compactInto<EvalStar2<D>, false>(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, nw, f.base, out, sc, sa, e0, e1,
+                                   externalEvents);
+}
+
+template <int D>
+static void updRebuildOp(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
+                         uint32_t nw, Workspace &ws, const OpParams &op, DevScalars *sc) {
+  const SegArgs sa0 = noSeg();
+  candidates<D, 0, false>(s, in.bits, nullptr, f, g, nw, ws, sc, sa0);
+  SegArgs sa = sa0;
+  sa.iseg = ws.cseg;
+  sa.oseg = out.seg;
+  EvalUpdRebuild<D> ev{viewOf(in, f, g), ws.cpos, op};
+  compactInto<EvalUpdRebuild<D>, false>(s, ev, ws, ws.ncand, ws.cpos, ws.cpre, ws.cbits, nw, f.base, out, sc, sa);
+}
+
 template <int D, bool SEG>
 static void selfOp(hipStream_t s, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g, uint32_t nw,
                    Workspace &ws, const OpParams &op, DevScalars *sc, const SegArgs &sa0) {
@@ -1126,6 +1193,21 @@ void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const
   const uint32_t nw = (uint32_t)f.nwords;
   if (D == 2) starOp<2, false>(s, in, out, f, g, nw, ws, op, sc, noSeg(), e0, e1, externalEvents);
   else starOp<3, false>(s, in, out, f, g, nw, ws, op, sc, noSeg(), e0, e1, externalEvents);
+}
+
+void launchStarOp2(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
+                   Workspace &ws, const OpParams &op, DevScalars *sc, hipEvent_t e0, hipEvent_t e1,
+                   bool externalEvents) {
+  const uint32_t nw = (uint32_t)f.nwords;
+  if (D == 2) starOp2<2>(s, in, out, f, g, nw, ws, op, sc, e0, e1, externalEvents);
+  else starOp2<3>(s, in, out, f, g, nw, ws, op, sc, e0, e1, externalEvents);
+}
+
+void launchUpdRebuildOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
+                        Workspace &ws, const OpParams &op, DevScalars *sc) {
+  const uint32_t nw = (uint32_t)f.nwords;
+  if (D == 2) updRebuildOp<2>(s, in, out, f, g, nw, ws, op, sc);
+  else updRebuildOp<3>(s, in, out, f, g, nw, ws, op, sc);
 }
 
 void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
@@ -1468,7 +1550,9 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
   uint32_t curSeg;
   unsigned long long sllfKey;  // SLLF: key of this lane's last active point that set its alphas
   int learnedPhase;            // learned weno5: 2 when the wave has run the network (waveMlp)
-  // learned weno5, first phase: this lane's network inputs for entry i
+  double *stc;                 // learned weno5: this lane's axis values (LDS, stride 64)
+  // learned weno5, first phase: this lane's network inputs for entry i, and
+  // its axis values kept for the second phase
   __device__ __attribute__((always_inline)) void prepareLearned(uint32_t i) {
     int32_t c[3];
     candCoords(b, i, b.pos[i], c);
@@ -1477,6 +1561,10 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
     const double phi0 = b.val[i];
     LSG_UNROLL
     for (int d = 0; d < D; ++d) sts[d][3] = phi0;
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d)
+      LSG_UNROLL
+      for (int k = 0; k < 7; ++k) stc[(d * 7 + k) * 64] = sts[d][k];
     double gm[D], gp[D];
     weno5LearnedAxes<D>(sts, b.g.dx, mlp, io, gm, gp, 1);
   }
@@ -1517,7 +1605,8 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
     double velocity;
     bool used = false;
     double dn = 0.;
-    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp, io, &used, &dn, learnedPhase);
+    rates[i] = ratePoint<D, SCHEME>(v, c, v.val[i], rp, ve, ga, la, velocity, mlp, io, &used, &dn, learnedPhase, stc,
+                                    64);
     if (!SEG && rp.dissOut) rp.dissOut[i] = dn;
     if (SCHEME == S_SLLF && used) sllfKey = (unsigned long long)(j + 1) << 4;  // lanes visit j in increasing order
     if (velocity != 0.) {
@@ -1669,16 +1758,18 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
   uint32_t lo, hi;
   blockChunk(n, lo, hi, 1);
   const Vel vel = velOf(rp);
-  // learned weno5: the bf16 parameters are widened to fp32 once per block
-  __shared__ float mlpW[SCHEME == S_WENO5L ? MLP_NPARAM : 1];
-  if (SCHEME == S_WENO5L) {
+  // learned weno5 with the network per lane: the bf16 parameters are
+  // widened to fp32 once per block
+  __shared__ float mlpW[(SCHEME == S_WENO5L && !LWAVE) ? MLP_NPARAM : 1];
+  if (SCHEME == S_WENO5L && !LWAVE) {
     for (int i = threadIdx.x; i < MLP_NPARAM; i += TPB) mlpW[i] = bf16Float(rp.mlp[i]);
     __syncthreads();
   }
   const uint32_t segFirst = (SEG && lo < hi) ? sa.iseg[list[lo]] : 0u;
   __shared__ float mlpIO[SCHEME == S_WENO5L ? TPB * 2 * D * MLP_IN : 1];
   float *io = (SCHEME == S_WENO5L) ? &mlpIO[threadIdx.x * 2 * D * MLP_IN] : nullptr;
-  RatePointFn<D, SCHEME, SEG> fn{b, rp, vel, rates, mlpW, io, {0, 0, 0}, {0, 0, 0}, 0., 0ull, sa, segMax, segFirst, 0ull, 0};
+  RatePointFn<D, SCHEME, SEG> fn{b,   rp,     vel,      rates, mlpW, io, {0, 0, 0}, {0, 0, 0}, 0., 0ull,
+                                 sa,  segMax, segFirst, 0ull,  0,    nullptr};
   if (SCHEME == S_LF)
     for (int i = 0; i < 3; ++i) fn.ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
   if constexpr (SEG) {
@@ -1699,6 +1790,8 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
       // every lane of the wave takes part in the network of each 64 points
       const MlpLane W = mlpLane(rp.mlp);
       float *rows = &mlpIO[(threadIdx.x & ~63u) * 2 * D * MLP_IN];
+      __shared__ double stCache[LWAVE ? TPB * D * 7 : 1];  // lane-interleaved per wave
+      fn.stc = &stCache[(threadIdx.x & ~63u) * D * 7 + lane];
       fn.learnedPhase = 2;
       for (uint32_t base = first + wave * 64u; base < last; base += waves * 64u) {
         const uint32_t j = base + lane;

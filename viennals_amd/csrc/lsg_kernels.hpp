@@ -29,6 +29,7 @@ struct DevScalars {
   double maxTimeStep; // bound passed to the first stage
   double rateBytes;   // accumulated algorithmic bytes of the rate kernel
   double starBytes;   // accumulated algorithmic bytes of the star-operation evaluation (single band)
+  double star2Bytes;  // the same for the evaluation of two Expand cycles at once
   unsigned long long minStepBits;  // several level sets: min over points of the step limit (bits of a double >= 0)
   uint32_t movedDown;              // combine: the weighted value exceeded the start value somewhere
   unsigned long long stepActive0;  // totalActive at the start of the step
@@ -54,7 +55,7 @@ struct BandBufs {
 
 struct Workspace {
   uint64_t *cbits;     // nwords + 1
-  uint32_t *cnt;       // nwords + 1
+  uint64_t *cbits2;    // nwords + 1 (two Expand cycles: the first star dilation)
   uint32_t *cpre;      // nwords + 1
   uint32_t *cpos;      // ccap
   double *out;         // ccap
@@ -139,6 +140,14 @@ void launchSetFlag(hipStream_t s, uint32_t *flag, uint32_t v);
 void launchStarOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr, bool externalEvents = false);
+// Two Expand cycles as one band operation (multi-kernel pipeline, one level set).
+void launchStarOp2(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
+                   Workspace &ws, const OpParams &op, DevScalars *sc, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr, bool externalEvents = false);
+// updateLevelSet and rebuildLS as one star operation (multi-kernel
+// pipeline, one level set; op.rates, op.dt, op.a = cutoff).
+void launchUpdRebuildOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
+                        Workspace &ws, const OpParams &op, DevScalars *sc);
 void launchSelfOp(hipStream_t s, int D, const BandBufs &in, BandBufs &out, const Frame &f, const GridInfo &g,
                   Workspace &ws, const OpParams &op, DevScalars *sc);
 // movedDown (optional): set to 1 where a combine's weighted value exceeds

@@ -14,16 +14,17 @@ wherever the smoothness indicators disagree by more than a factor 10
   same point (target);
 * training: the library's forward/backward kernels (matrix cores for the
   hidden layer) with Adam;
-* judging: Chamfer distance and volume error of the coarse runs (plain
-  and learned) against the fine plain run.
+* judging: CompareChamfer and CompareVolume of the coarse runs (plain and
+  learned) against the fine plain run sampled on the coarse grid.
 """
 import ctypes
+import os
 import time
 
 import numpy as np
 
-from . import (Advect, BoundaryConditionEnum, Domain, Expand, SpatialSchemeEnum, TemporalSchemeEnum, VelocityField,
-               POS_VALUE, check, rebuild, weno)
+from . import (Advect, CompareChamfer, CompareVolume, Domain, Expand, SpatialSchemeEnum, TemporalSchemeEnum,
+               VelocityField, check, rebuild, weno)
 from ._lib import lib
 
 NPARAM = 1347
@@ -52,12 +53,10 @@ def init_params(seed=0):
 
 # ------------------------------------------------------------ geometry ----
 
-def _real_sph_harm(l, m, theta, phi):
-    from scipy.special import sph_harm_y
-    if m == 0:
-        return np.real(sph_harm_y(l, 0, theta, phi))
-    y = sph_harm_y(l, abs(m), theta, phi)
-    return np.sqrt(2.0) * (np.real(y) if m > 0 else np.imag(y))
+def _sph_norm(l, m):
+    """sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) for m >= 0."""
+    from math import factorial, pi, sqrt
+    return sqrt((2 * l + 1) / (4 * pi) * factorial(l - m) / factorial(l + m))
 
 
 class StarShape:
@@ -84,10 +83,48 @@ class StarShape:
         return th, np.mod(ph, 2 * np.pi)
 
     def _pert(self, th, ph):
-        s = np.zeros_like(th)
+        """sum a_lm Y_lm(th, ph) with the real harmonics Y_l0 = Re Y_l^0,
+        Y_lm = sqrt2 Re Y_l^m (m > 0), sqrt2 Im Y_l^|m| (m < 0): per order m
+        the Legendre sums C_m = sum_l c_lm P_l^m (cosine) and S_m (sine) are
+        accumulated along the recurrence in l, then
+        sum_m C_m cos(m ph) + S_m sin(m ph)."""
+        coef = {}
         for l, m, a in self.terms:
-            s += a * _real_sph_harm(l, m, th, ph)
-        return s
+            coef[(l, m)] = a * _sph_norm(l, abs(m)) * (np.sqrt(2.0) if m != 0 else 1.0)
+        lmax = max(l for l, _, _ in self.terms)
+        x = np.cos(th)
+        sn = np.sin(th)  # th in [0, pi]
+        out = np.zeros_like(th)
+        pmm = np.ones_like(x)
+        tmp = np.empty_like(x)
+        for m in range(lmax + 1):
+            if m > 0:
+                pmm *= sn
+                pmm *= -(2 * m - 1)
+            C = np.zeros_like(x)
+            S = np.zeros_like(x) if m > 0 else None
+
+            def acc(l, P):
+                if (l, m) in coef:
+                    np.multiply(P, coef[(l, m)], out=tmp)
+                    C.__iadd__(tmp)
+                if m > 0 and (l, -m) in coef:
+                    np.multiply(P, coef[(l, -m)], out=tmp)
+                    S.__iadd__(tmp)
+
+            acc(m, pmm)
+            if m < lmax:
+                p2, p1 = pmm, (2 * m + 1) * x * pmm
+                acc(m + 1, p1)
+                for l in range(m + 2, lmax + 1):
+                    p0 = ((2 * l - 1) * x * p1 - (l + m - 1) * p2) / (l - m)
+                    acc(l, p0)
+                    p2, p1 = p1, p0
+            if m == 0:
+                out += C
+            else:
+                out += C * np.cos(m * ph) + S * np.sin(m * ph)
+        return out
 
     def implicit(self, x):
         """g(x) = |x| - r(x/|x|) for points x (n, 3)."""
@@ -100,18 +137,29 @@ class StarShape:
         """Defined points (index, phi in grid units) with |phi| <= halfwidth;
         phi = g / |grad g| (first-order signed distance), gradient by central
         differences."""
-        rmax = self.R * (1.0 + self.scale * 1.01 * np.max(np.abs(self._pert(*self._fibonacci(20000))))) + 3 * dx
+        pmax = np.max(np.abs(self._pert(*self._fibonacci(20000))))
+        rmax = self.R * (1.0 + self.scale * 1.01 * pmax) + 3 * dx
         n = int(np.ceil(rmax / dx)) + 2
         ax = np.arange(-n, n + 1)
         pts, vals = [], []
         h = 1e-4 * self.R
-        for iz in ax:  # slab by slab keeps memory small
-            X, Y = np.meshgrid(ax, ax, indexing="ij")
+        cut = (halfwidth + 2.0) * dx * 2.0
+        # |g| <= cut needs |x| within cut of the radius range; 1.1 covers the
+        # maximum between the sampled directions of this band-limited radius
+        r_lo = self.R * (1.0 - self.scale * 1.1 * pmax) - cut - dx
+        r_hi = self.R * (1.0 + self.scale * 1.1 * pmax) + cut + dx
+        X, Y = np.meshgrid(ax, ax, indexing="ij")
+
+        def slab(iz):
             P = np.stack([X.ravel(), Y.ravel(), np.full(X.size, iz)], 1).astype(np.float64) * dx
+            rr = np.linalg.norm(P, axis=1)
+            P = P[(rr >= r_lo) & (rr <= r_hi)]
+            if not len(P):
+                return None
             g = self.implicit(P)
-            keep = np.abs(g) <= (halfwidth + 2.0) * dx * 2.0
+            keep = np.abs(g) <= cut
             if not keep.any():
-                continue
+                return None
             P = P[keep]
             g = g[keep]
             grad = np.zeros_like(P)
@@ -121,9 +169,17 @@ class StarShape:
                 grad[:, a] = (self.implicit(P + e) - self.implicit(P - e)) / (2 * h)
             phi = g / np.linalg.norm(grad, axis=1) / dx
             m = np.abs(phi) <= halfwidth
-            idx = np.rint(P[m] / dx).astype(np.int32)
-            pts.append(idx)
-            vals.append(phi[m])
+            return np.rint(P[m] / dx).astype(np.int32), phi[m]
+
+        # slabs in threads: numpy's array work runs outside the interpreter
+        # lock; each slab keeps its memory small
+        from concurrent.futures import ThreadPoolExecutor
+        workers = max(1, min(16, os.cpu_count() or 1)) if len(ax) > 160 else 1
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            for r in ex.map(slab, ax):
+                if r is not None:
+                    pts.append(r[0])
+                    vals.append(r[1])
         idx = np.concatenate(pts)
         val = np.concatenate(vals)
         order = np.lexsort((idx[:, 0], idx[:, 1], idx[:, 2]))
@@ -202,18 +258,19 @@ def _weno5_parts(st):
     return d, np.stack([b1, b2, b3], 1), np.stack([p1, p2, p3], 1)
 
 
-def samples_from_fine(fine, dxc=1.0):
-    """Training samples of one fine snapshot (gridDelta dxc/2).
-
-    Returns X (n, 8) float32 and T (n,) float32."""
+def coarse_sample(fine, dxc=1.0, fine_width=5):
+    """The fine level set (gridDelta dxc/2) sampled at the coarse grid points
+    and prepared like the coarse solver's band: a copy expanded to
+    `fine_width`, its points of even index at index/2 with phi in coarse
+    grid units, the points with |phi| <= 1 inserted and the band rebuilt
+    (rebuildLS).  Returns the coarse level set and the expanded fine
+    points (indices, values)."""
     dxf = fine.gridDelta
     assert abs(dxc - 2 * dxf) < 1e-12
     f = Domain(gridDelta=dxf, dim=3)
     f.deepCopy(fine)
-    Expand(f, 9).apply()
+    Expand(f, fine_width).apply()
     fi, fv = f.defined_points()
-    F, flo = _dense(fi, fv)
-    # coarse band sampled from the fine one, prepared like the coarse solver
     even = np.all(fi % 2 == 0, axis=1) & (np.abs(fv) <= 2.0)
     ci = (fi[even] // 2).astype(np.int32)
     cv = fv[even] * (dxf / dxc)
@@ -221,6 +278,16 @@ def samples_from_fine(fine, dxc=1.0):
     c = Domain(gridDelta=dxc, dim=3)
     c.insertPoints(ci[m], cv[m], 2)
     rebuild(c)
+    return c, fi, fv
+
+
+def samples_from_fine(fine, dxc=1.0):
+    """Training samples of one fine snapshot (gridDelta dxc/2).
+
+    Returns X (n, 8) float32 and T (n,) float32."""
+    dxf = fine.gridDelta
+    c, fi, fv = coarse_sample(fine, dxc, fine_width=9)
+    F, flo = _dense(fi, fv)
     Expand(c, 7).apply()
     ki, kv = c.defined_points()
     C, clo = _dense(ki, kv)
@@ -358,96 +425,52 @@ def train(X, T, iters=3000, batch=16384, lr=2e-3, seed=0, val_frac=0.1, log=None
 
 # -------------------------------------------------------------- judges ----
 
-def surface_points(domain):
-    """Zero crossings of phi along grid edges between defined neighbours
-    (physical coordinates)."""
-    idx, val = domain.defined_points()
-    A, lo = _dense(idx, val, pad=1)
-    dx = domain.gridDelta
-    out = []
-    for a in range(domain.D):
-        sl0 = [slice(None)] * domain.D
-        sl1 = [slice(None)] * domain.D
-        sl0[a] = slice(0, -1)
-        sl1[a] = slice(1, None)
-        v, w = A[tuple(sl0)], A[tuple(sl1)]
-        m = ~np.isnan(v) & ~np.isnan(w) & ((v < 0) != (w < 0)) & (v != w)
-        p = np.argwhere(m).astype(np.float64)
-        t = v[m] / (v[m] - w[m])
-        p[:, a] += t
-        out.append((p + lo) * dx)
-    return np.concatenate(out)
+def volume(d):
+    """Volume inside a closed level set under CompareVolume's cell rule
+    (lsCompareVolume.hpp:176-289): half the mismatch against a copy of
+    itself translated along x beyond its own extent, where the two insides
+    are disjoint and the rule is translation invariant."""
+    idx, val = d.defined_points()
+    shift = np.zeros(d.D, np.int32)
+    shift[0] = idx[:, 0].max() - idx[:, 0].min() + 8
+    t = Domain(gridDelta=d.gridDelta, dim=d.D)
+    t.insertPoints(idx + shift, val, d.getLevelSetWidth())
+    cv = CompareVolume(d, t)
+    cv.apply()
+    return cv.getVolumeMismatch() / 2.0
 
 
-def chamfer(a, b):
-    """Mean of the nearest-neighbour distances in both directions."""
-    from scipy.spatial import cKDTree
-    da, _ = cKDTree(b).query(a)
-    db, _ = cKDTree(a).query(b)
-    return (da.sum() + db.sum()) / (len(a) + len(b))
-
-
-def enclosed_volume(domain):
-    """Volume of phi < 0 for a closed surface: band points count with the
-    smeared Heaviside clamp(0.5 - phi, 0, 1), undefined cells enclosed by
-    the band count fully; times dx^3."""
-    from scipy import ndimage
-    idx, val = domain.defined_points()
-    A, lo = _dense(idx, val, pad=2)
-    band = ~np.isnan(A)
-    frac = np.clip(0.5 - A[band], 0.0, 1.0).sum()
-    interior = ndimage.binary_fill_holes(band) & ~band
-    return (frac + interior.sum()) * domain.gridDelta ** 3
-
-
-def surface_distance(a, b):
-    """Symmetric mean distance between the zero level sets of a and b: the
-    surface points of one evaluated in the other's level set (trilinear
-    interpolation of phi times its gridDelta), averaged over both."""
-    from scipy.ndimage import map_coordinates
-
-    from scipy.ndimage import distance_transform_edt
-
-    def one_way(src, dst):
-        pts = surface_points(src)
-        idx, val = dst.defined_points()
-        A, lo = _dense(idx, val, pad=2)
-        # cells outside the narrow band take their nearest defined value, so
-        # that every interpolation cell around the surface is complete
-        miss = np.isnan(A)
-        if miss.any():
-            near = distance_transform_edt(miss, return_distances=False, return_indices=True)
-            A = A[tuple(near)]
-        coords = (pts / dst.gridDelta - lo).T
-        phi = map_coordinates(A, coords, order=1, mode="nearest")
-        return np.abs(phi) * dst.gridDelta, 0
-
-    da, ma = one_way(a, b)
-    db, mb = one_way(b, a)
-    return (da.sum() + db.sum()) / (len(da) + len(db)), ma + mb
+def judge(run, ref):
+    """A coarse run against the fine run sampled on the coarse grid
+    (coarse_sample), with the library's CompareChamfer and CompareVolume
+    (the reference's rules, lsCompareChamfer.hpp, lsCompareVolume.hpp):
+    Chamfer distance, volume mismatch and the mismatch relative to the
+    reference's volume."""
+    ch = CompareChamfer(ref, run)
+    ch.apply()
+    cv = CompareVolume(ref, run)
+    cv.apply()
+    return dict(chamfer=ch.getChamferDistance(), volume_mismatch=cv.getVolumeMismatch(),
+                volume_error=cv.getVolumeMismatch() / volume(ref))
 
 
 def compare(shape, vel, T, params, log=None):
-    """Fine plain (dx 0.5) vs coarse plain and coarse learned (dx 1.0):
-    Chamfer distance of the surface point sets, surface distance, relative
-    volume error, wall time of Advect::apply(T) on the GPU."""
+    """Fine plain (dx 0.5) vs coarse plain and coarse learned (dx 1.0): the
+    coarse runs judged against the fine run sampled on the coarse grid
+    (judge), wall time of Advect::apply(T) on the GPU and the fine run's
+    wall time over each coarse one."""
     res = {}
     fine = star_domain(shape, 0.5)
     sf, wf, tf = advect(fine, vel, T)
     res["fine"] = dict(steps=sf, wall=wf, time=tf)
     if log:
         log(f"fine: {res['fine']}")
-    pf = surface_points(fine)
-    vf = enclosed_volume(fine)
+    ref, _, _ = coarse_sample(fine, 1.0)
     for name, spatial, prm in (("coarse_plain", SpatialSchemeEnum.WENO_5TH_ORDER, None),
                                ("coarse_learned", SpatialSchemeEnum.WENO_5TH_ORDER_LEARNED, params)):
         c = star_domain(shape, 1.0)
         s, w, t = advect(c, vel, T, spatial=spatial, params=prm)
-        pc = surface_points(c)
-        vc = enclosed_volume(c)
-        sd, miss = surface_distance(c, fine)
-        res[name] = dict(steps=s, wall=w, time=t, chamfer=float(chamfer(pc, pf)), surface_distance=float(sd),
-                         unmatched=miss, volume_error=float(abs(vc - vf) / vf), speedup_vs_fine=wf / w)
+        res[name] = dict(steps=s, wall=w, time=t, speedup_vs_fine=wf / w, **judge(c, ref))
         if log:
             log(f"{name}: {res[name]}")
     return res
