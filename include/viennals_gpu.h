@@ -226,13 +226,28 @@ int lsg_advect_set_dissipation_alpha_builtin(lsg_advect *a, int material, const 
  * (3 components), material 0, the normal the scheme computes and pointId =
  * rank among the defined points of the expanded band, in point order, on the
  * calling thread.  Either callback may be null (zero).  Supported with the
- * Engquist-Osher, local-local Lax-Friedrichs and WENO schemes. */
+ * Engquist-Osher, Lax-Friedrichs (also findGlobalAlphas' calls, lsAdvect.hpp:
+ * 90-195), local Lax-Friedrichs (also the 3^D alpha box, lsLocalLaxFriedrichs
+ * .hpp:184-232), analytical local Lax-Friedrichs (with
+ * lsg_advect_set_dissipation_alpha_host), local-local Lax-Friedrichs and WENO
+ * schemes; not with the stencil local Lax-Friedrichs scheme. */
 typedef double (*lsg_scalar_velocity_fn)(const double *coordinate, int material, const double *normal,
                                          uint64_t point_id, void *user);
 typedef void (*lsg_vector_velocity_fn)(const double *coordinate, int material, const double *normal,
                                        uint64_t point_id, void *user, double *out3);
 int lsg_advect_set_velocity_host(lsg_advect *a, lsg_scalar_velocity_fn scalar, lsg_vector_velocity_fn vector,
                                  void *user);
+/* VelocityField::getDissipationAlpha (lsVelocityField.hpp:32-38) of a
+ * host-evaluated velocity field, for LOCAL_LAX_FRIEDRICHS_ANALYTICAL_1ST_ORDER
+ * (lsLocalLaxFriedrichsAnalytical.hpp:186-214): called for every direction
+ * at each of the 3^D box points of every active point, with the box point's
+ * central differences (not normalised), after the velocity callbacks of
+ * lsg_advect_set_velocity_host; alpha(direction) is the maximum over the box.
+ * fn = NULL gives the reference's default, 0.  A built-in velocity setter
+ * removes it. */
+typedef double (*lsg_dissipation_alpha_fn)(int direction, int material, const double *central_differences,
+                                           void *user);
+int lsg_advect_set_dissipation_alpha_host(lsg_advect *a, lsg_dissipation_alpha_fn fn, void *user);
 int lsg_advect_set_spatial_scheme(lsg_advect *a, int scheme);     /* :936 */
 int lsg_advect_set_temporal_scheme(lsg_advect *a, int scheme);    /* :949 */
 int lsg_advect_set_advection_time(lsg_advect *a, double time);    /* :873 */

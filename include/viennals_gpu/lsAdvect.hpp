@@ -538,6 +538,11 @@ template <class T, int D> class Advect {
     out[2] = (double)r[2];
   }
 
+  static double hostAlpha(int direction, int material, const double *cd, void *user) {
+    auto *v = static_cast<VelocityField<T> *>(user);
+    return (double)v->getDissipationAlpha(direction, material, Vec3D<T>{(T)cd[0], (T)cd[1], (T)cd[2]});
+  }
+
   bool pushVelocity() {
     if (!velocities_) {
       std::fprintf(stderr, "[viennals_gpu] No velocity field passed to Advect. Not advecting.\n");
@@ -570,6 +575,8 @@ template <class T, int D> class Advect {
     auto *b = dynamic_cast<BuiltinVelocityField<T> *>(velocities_.get());
     if (!b)
       return detail::ok(lsg_advect_set_velocity_host(h_, &Advect::hostScalar, &Advect::hostVector, velocities_.get()),
+                        "setVelocityField") &&
+             detail::ok(lsg_advect_set_dissipation_alpha_host(h_, &Advect::hostAlpha, velocities_.get()),
                         "setVelocityField");
     double vec[3] = {b->vec[0], b->vec[1], b->vec[2]};
     double al[3] = {b->alpha[0], b->alpha[1], b->alpha[2]};

@@ -124,33 +124,119 @@ def test_host_callback_llf_box_calls():
             assert calls[28 * j + 1 + q][1] == calls[28 * j][1]
 
 
-@pytest.mark.parametrize("spatial", [4, 9])
-def test_host_callback_rejected_with_analytical_and_stencil_llf(spatial):
+def test_host_callback_rejected_with_stencil_llf():
     import viennals_amd as vls
     d = sphere(vls, 3.0)
     a = vls.Advect(d, vls.HostVelocityField())
-    a.setSpatialScheme(spatial)
+    a.setSpatialScheme(9)
     with pytest.raises(vls.LsgError, match="host-evaluated"):
         a.apply()
 
 
-@pytest.mark.parametrize("spatial,temporal", [(3, 2), (8, 1), (2, 0)])
+@pytest.mark.parametrize("temporal", [0, 2])
+def test_host_llfa_alphas_match_builtin(temporal):
+    """LOCAL_LAX_FRIEDRICHS_ANALYTICAL with getDissipationAlpha on the host
+    (constant per direction, like the built-in functors' alphas): bit-identical
+    to the built-in functor with the same alphas."""
+    import viennals_amd as vls
+    f = NormalPower(dir=0, k=2, b=1.0, a_pos=2.3, a_neg=0.5, vec=(0.2, 0.0, -0.1))
+    al = (0.7, 0.4, 0.9)
+
+    class Host(vls.HostVelocityField):
+        def getScalarVelocity(self, coordinate, material, normal, pointId):
+            return f.scalar(normal)
+
+        def getVectorVelocity(self, coordinate, material, normal, pointId):
+            return f.vec
+
+        def getDissipationAlpha(self, direction, material, centralDifferences):
+            return al[direction]
+
+    builtin = vls.VelocityField(kind=1, dir=0, k=2, b=1.0, a_pos=2.3, a_neg=0.5, vec=f.vec, alpha=al)
+    results = []
+    for vel in (builtin, Host()):
+        d = sphere(vls)
+        a = vls.Advect(d, vel)
+        a.setSpatialScheme(4)
+        a.setTemporalScheme(temporal)
+        a.setTimeStepRatio(0.4)
+        for _ in range(2):
+            a.apply()
+        results.append((d.get_runs(), a.getAdvectedTime()))
+    (i0, v0), t0 = results[0]
+    (i1, v1), t1 = results[1]
+    assert t0 == t1
+    assert np.array_equal(i0, i1) and np.array_equal(v0, v1)
+
+
+def test_host_llfa_alpha_box_calls():
+    """getDissipationAlpha is called for every direction at the 27 box
+    points of every active point, right after the point's velocity call; at
+    the box centre its central differences, normalised, are the normal the
+    velocity call received (two separate device computations of the same
+    differences); alphas that grow with the differences advect the surface
+    differently from constant ones."""
+    import viennals_amd as vls
+    events = []
+
+    class Host(vls.HostVelocityField):
+        def getScalarVelocity(self, coordinate, material, normal, pointId):
+            events.append(("v", normal))
+            return 1.0
+
+        def getDissipationAlpha(self, direction, material, centralDifferences):
+            events.append(("a", direction, centralDifferences))
+            return 0.5 + min(abs(centralDifferences[direction]), 4.0)
+
+    d = sphere(vls, 4.0)
+    a = vls.Advect(d, Host())
+    a.setSpatialScheme(4)
+    a.setTemporalScheme(0)
+    a.setTimeStepRatio(0.4)
+    a.apply()
+    per = 1 + 27 * 3
+    assert len(events) > 0 and len(events) % per == 0
+    for k in range(0, len(events), per):
+        assert events[k][0] == "v"
+        box = events[k + 1:k + per]
+        assert [e[0] for e in box] == ["a"] * 81 and [e[1] for e in box] == [0, 1, 2] * 27
+        cd = np.array(box[13 * 3][2])  # box point q = 13: offset (0, 0, 0)
+        nm = 1.0 / np.sqrt(cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2])
+        assert tuple(cd * nm) == tuple(events[k][1])
+    d2 = sphere(vls, 4.0)
+    b = vls.Advect(d2, vls.VelocityField(kind=0, b=1.0, alpha=(0.5, 0.5, 0.5)))
+    b.setSpatialScheme(4)
+    b.setTemporalScheme(0)
+    b.setTimeStepRatio(0.4)
+    b.apply()
+    _, v1 = d.get_runs()
+    _, v2 = d2.get_runs()
+    assert not (len(v1) == len(v2) and np.array_equal(v1, v2))
+
+
+@pytest.mark.parametrize("spatial,temporal", [(3, 2), (8, 1), (2, 0), (4, 0), (4, 2)])
 def test_host_callback_several_level_sets(spatial, temporal):
     """A host callback with a material-dependent velocity equals the
     built-in per-material functors with LF (global alphas over the points'
-    materials) and LLF (box velocities per material) on an etch stop."""
+    materials), LLF (box velocities per material) and the analytical LLF
+    (box alphas per material) on an etch stop."""
     import viennals_amd as vls
     from test_gpu_multimaterial import plane2
     from test_gpu_parity import to_gpu
     rates = [(-0.05, -0.2), (-0.1, -1.0)]
     fs = [NormalPower(dir=1, k=1, b=b, a_pos=ap, a_neg=0.0) for b, ap in rates]
 
+    alphas = [(0.0, 0.3, 0.1), (0.2, 1.2, 0.0)]
+
     class Host(vls.HostVelocityField):
         def getScalarVelocity(self, coordinate, material, normal, pointId):
             return fs[material].scalar(normal)
 
-    builtin = vls.MaterialVelocityField([vls.VelocityField.normal_power(dir=1, k=1, b=b, a_pos=ap, a_neg=0.0)
-                                         for b, ap in rates])
+        def getDissipationAlpha(self, direction, material, centralDifferences):
+            return alphas[material][direction]
+
+    builtin = vls.MaterialVelocityField([vls.VelocityField(kind=1, dir=1, k=1, b=b, a_pos=ap, a_neg=0.0, alpha=al)
+                                         for (b, ap), al in zip(rates, alphas)])
     out = []
     for vel in (builtin, Host()):
         lv = [to_gpu(plane2(-2.3)), to_gpu(plane2(0.0))]

@@ -752,6 +752,9 @@ class HostVelocityField:
     def getVectorVelocity(self, coordinate, material, normal, pointId):
         return (0.0, 0.0, 0.0)
 
+    def getDissipationAlpha(self, direction, material, centralDifferences):
+        return 0.0
+
 
 class Advect:
     """Advect<T,D> (lsAdvect.hpp:54-1044) running on the GPU."""
@@ -785,7 +788,7 @@ class Advect:
 
     def setVelocityField(self, v):
         if isinstance(v, HostVelocityField) or (isinstance(v, VelocityField) and v._overridden()):
-            from ._lib import SCALAR_VELOCITY_FN, VECTOR_VELOCITY_FN
+            from ._lib import DISSIPATION_ALPHA_FN, SCALAR_VELOCITY_FN, VECTOR_VELOCITY_FN
 
             def scalar(c, m, n, pid, user):
                 return float(v.getScalarVelocity((c[0], c[1], c[2]), m, (n[0], n[1], n[2]), pid))
@@ -794,9 +797,14 @@ class Advect:
                 r = v.getVectorVelocity((c[0], c[1], c[2]), m, (n[0], n[1], n[2]), pid)
                 out[0], out[1], out[2] = float(r[0]), float(r[1]), float(r[2])
 
-            self._callbacks = (SCALAR_VELOCITY_FN(scalar), VECTOR_VELOCITY_FN(vector))
+            def alpha(direction, m, cd, user):
+                return float(v.getDissipationAlpha(direction, m, (cd[0], cd[1], cd[2])))
+
+            self._callbacks = (SCALAR_VELOCITY_FN(scalar), VECTOR_VELOCITY_FN(vector), DISSIPATION_ALPHA_FN(alpha))
             check(lib().lsg_advect_set_velocity_host(self._h, ctypes.cast(self._callbacks[0], ctypes.c_void_p),
                                                      ctypes.cast(self._callbacks[1], ctypes.c_void_p), None))
+            check(lib().lsg_advect_set_dissipation_alpha_host(self._h, ctypes.cast(self._callbacks[2],
+                                                                                   ctypes.c_void_p), None))
             return
         self._callbacks = None
         if isinstance(v, MaterialVelocityField):

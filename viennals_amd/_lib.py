@@ -39,6 +39,7 @@ EXPORTED = [
     "lsg_batch_create", "lsg_batch_destroy", "lsg_batch_set_velocity_builtin", "lsg_batch_set_schemes",
     "lsg_batch_apply", "lsg_batch_get_advected_times", "lsg_batch_get_point_updates", "lsg_batch_write_back",
     "lsg_batch_set_scan_blocks", "lsg_batch_autotune", "lsg_advect_set_velocity_host",
+    "lsg_advect_set_dissipation_alpha_host",
     "lsg_advect_set_velocity_update_callback",
     "lsg_advect_set_save_advection_velocities", "lsg_domain_point_data_insert", "lsg_domain_point_data_count",
     "lsg_domain_point_data_label", "lsg_domain_point_data_size", "lsg_domain_point_data_get",
@@ -152,6 +153,7 @@ def lib():
         "lsg_batch_set_scan_blocks": ([P, c_int], c_int),
         "lsg_batch_autotune": ([P, ip, dp], c_int),
         "lsg_advect_set_velocity_host": ([P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], c_int),
+        "lsg_advect_set_dissipation_alpha_host": ([P, ctypes.c_void_p, ctypes.c_void_p], c_int),
         "lsg_advect_set_velocity_update_callback": ([P, ctypes.c_void_p, ctypes.c_void_p], c_int),
         "lsg_device_synchronize": ([], c_int),
     }
@@ -178,3 +180,5 @@ VECTOR_VELOCITY_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), cty
                                       ctypes.POINTER(ctypes.c_double), ctypes.c_uint64, ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_double))
 VELOCITY_UPDATE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
+DISSIPATION_ALPHA_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.c_void_p)

@@ -911,6 +911,8 @@ struct lsg_advect {
   lsg_scalar_velocity_fn hostScalar = nullptr;
   lsg_vector_velocity_fn hostVector = nullptr;
   void *hostUser = nullptr;
+  lsg_dissipation_alpha_fn hostAlpha = nullptr;  // analytical LLF: getDissipationAlpha on the host
+  void *hostAlphaUser = nullptr;
   double *velIn = nullptr, *velOut = nullptr;  // device: 7 / 4 (x materials) doubles per active point
   int velMat = 1, velRec = 1;
   uint32_t *dlist = nullptr, *ndef = nullptr;
@@ -1048,13 +1050,21 @@ static int hostVelocities(lsg_advect *a, lsg_domain *d, int nmat = 1, const Lowe
   a->velInH.resize(nrec * 8);
   a->velOutH.resize(nrec * 4 * nmat);
   if (nact) LSG_HIP(hipMemcpy(a->velInH.data(), a->velIn, nrec * 8 * sizeof(double), hipMemcpyDeviceToHost));
+  // the analytical LLF's box records take getDissipationAlpha per direction
+  // (its central differences in the normal slots), the others the velocities
+  const bool alphaBox = a->spatial == 4;
   for (size_t r = 0; r < nrec; ++r) {
     const double *in = &a->velInH[r * 8];
     const uint64_t pid = (uint64_t)in[6];
     for (int m = 0; m < nmat; ++m) {
       double *o = &a->velOutH[(r * nmat + m) * 4];
+      o[0] = o[1] = o[2] = o[3] = 0.0;
+      if (alphaBox && r % (size_t)R != 0) {
+        if (a->hostAlpha)
+          for (int dir = 0; dir < d->D; ++dir) o[1 + dir] = a->hostAlpha(dir, m, in + 3, a->hostAlphaUser);
+        continue;
+      }
       o[0] = a->hostScalar ? a->hostScalar(in, m, in + 3, pid, a->hostUser) : 0.0;
-      o[1] = o[2] = o[3] = 0.0;
       if (a->hostVector) a->hostVector(in, m, in + 3, pid, a->hostUser, o + 1);
     }
   }
@@ -1618,9 +1628,9 @@ static int advectApply(lsg_advect *a) {
   a->rp.sllfBaseEps = std::cbrt(std::numeric_limits<double>::epsilon());
   // WENO receives dissipationAlpha as its calcNormal flag (lsAdvect.hpp:670-676)
   a->rp.calcNormals = (a->spatial >= 10) ? (a->dissipationAlpha != 0.0) : a->calcNormals;
-  if ((a->hostScalar || a->hostVector) && (a->spatial == 4 || a->spatial == 9 || a->spatial == 12))
+  if ((a->hostScalar || a->hostVector) && (a->spatial == 9 || a->spatial == 12))
     return fail(-9, "Advect: host-evaluated velocities are supported with the Engquist-Osher, Lax-Friedrichs, "
-                    "local Lax-Friedrichs, local-local Lax-Friedrichs and WENO schemes");
+                    "local Lax-Friedrichs (also analytical), local-local Lax-Friedrichs and WENO schemes");
   if (a->spatial == 12 && !a->mlpDev)
     return fail(-8, "Advect: learned WENO5 selected but no network parameters were set");
   a->rp.mlp = a->mlpDev;
@@ -2961,11 +2971,19 @@ int lsg_advect_set_velocity_host(lsg_advect *a, lsg_scalar_velocity_fn scalar, l
   a->velSet = true;
   return 0;
 }
+int lsg_advect_set_dissipation_alpha_host(lsg_advect *a, lsg_dissipation_alpha_fn fn, void *user) {
+  a->hostAlpha = fn;
+  a->hostAlphaUser = user;
+  a->dropGraphs();
+  return 0;
+}
 int lsg_advect_set_velocity_builtin(lsg_advect *a, int kind, int dir, int k, double b, double aPos, double aNeg,
                                     const double *vec3) {
   a->hostScalar = nullptr;
   a->hostVector = nullptr;
   a->hostUser = nullptr;
+  a->hostAlpha = nullptr;
+  a->hostAlphaUser = nullptr;
   if (kind < 0 || kind > 2) return fail(-1, "unknown velocity kind");
   if (dir < 0 || dir > 2) return fail(-1, "velocity direction must be 0, 1 or 2");
   a->rp.velKind = kind;

@@ -858,7 +858,9 @@ struct Vel {
   double vec[3];
   double alpha[3];  // getDissipationAlpha (LLFA): constant per direction
   // host-evaluated velocities at the 3^D points of the local Lax-Friedrichs
-  // alpha box: box point q has (scalar, vector[3]) at box[q * boxStride]
+  // alpha box: box point q has (scalar, vector[3]) at box[q * boxStride];
+  // for the analytical scheme the host's getDissipationAlpha values per
+  // direction at box[q * boxStride + 1 + direction]
   const double *box = nullptr;
   int boxStride = 4;
   // built-in VelocityField::getScalarVelocity (lsVelocityField.hpp:17-21)
@@ -1455,12 +1457,54 @@ LSG_HD inline RateOut schemeLLFA(const BandView &b, const int32_t c[3], double p
   if (totalGrad == 0.) return {0., 0.};
   LSG_UNROLL
   for (int i = 0; i < D; ++i) {
-    const double a = vel.alpha[i];
+    // the maximum of getDissipationAlpha over the 3^D box (constant for the
+    // built-in functors; the host's values at the box points otherwise)
+    double a = vel.alpha[i];
+    if (vel.box) {
+      constexpr int NB = (D == 3) ? 27 : 9;
+      a = vel.box[1 + i];
+      for (int q = 1; q < NB; ++q) a = dmaxs(a, vel.box[q * vel.boxStride + 1 + i]);
+    }
     const double alpha = dmaxs(0.0, a);
     alphaAcc[i] = dmaxs(alphaAcc[i], a);
     dissipation += alpha * (gradNeg[i] - gradPos[i]) * 0.5;
   }
   return {totalGrad, dissipation};
+}
+
+// The central differences of the analytical local Lax-Friedrichs alpha box
+// at box point q around c (lsLocalLaxFriedrichsAnalytical.hpp:191-208,
+// calculateNormalComponent :30-34): ((pos - centre) / dx + (centre - neg)
+// / dx) / 2 per direction, the neighbours boundary mapped per coordinate of
+// their offset from c, not normalised.
+template <int D>
+LSG_HD inline void llfaBoxDiffs(const BandView &b, const int32_t c[3], int q, double out[3]) {
+  const int ni[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
+  BandView::Loc l[2 * D + 1];
+  int32_t nc[3] = {0, 0, 0};
+  LSG_UNROLL
+  for (int d = 0; d < D; ++d) nc[d] = b.g.map(d, (int64_t)c[d] + ni[d]);
+  l[2 * D] = b.locate(nc[0], nc[1], nc[2]);
+  LSG_UNROLL
+  for (int dir = 0; dir < D; ++dir) {
+    int32_t pp[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d) {
+      pp[d] = b.g.map(d, (int64_t)c[d] + ni[d] + (d == dir ? 1 : 0));
+      pn[d] = b.g.map(d, (int64_t)c[d] + ni[d] - (d == dir ? 1 : 0));
+    }
+    l[dir] = b.locate(pp[0], pp[1], pp[2]);
+    l[D + dir] = b.locate(pn[0], pn[1], pn[2]);
+  }
+  double v[2 * D + 1];
+  b.values<2 * D + 1>(l, v);
+  out[0] = out[1] = out[2] = 0.;
+  LSG_UNROLL
+  for (int dir = 0; dir < D; ++dir) {
+    const double diffPos = (v[dir] - v[2 * D]) / b.g.dx;
+    const double diffNeg = (v[2 * D] - v[D + dir]) / b.g.dx;
+    out[dir] = (diffPos + diffNeg) * 0.5;
+  }
 }
 
 // Normal at point q of the local Lax-Friedrichs alpha box around c
@@ -1702,17 +1746,20 @@ LSG_HD inline RateOut schemeSLLF(const BandView &b, const int32_t c[3], const Ve
 template <int D> LSG_HD inline void schemeNormal(const BandView &b, const int32_t c[3], double phi0, int scheme,
                                                  int order, bool calcNormals, double n[3]) {
   n[0] = n[1] = n[2] = 0.;
-  // upwind-difference normal: LLLF and LLF always (lsLocalLocalLaxFriedrichs
-  // .hpp:118-124, lsLocalLaxFriedrichs.hpp:140-155), LF with
-  // calculateNormalVectors (lsLaxFriedrichs.hpp:127-141)
-  if (scheme == 3 /* S_LLLF */ || scheme == 4 /* S_LLF */ || (scheme == 2 /* S_LF */ && calcNormals)) {
+  // upwind-difference normal: LLLF, LLF and the analytical LLF (first
+  // order) always (lsLocalLocalLaxFriedrichs.hpp:118-124,
+  // lsLocalLaxFriedrichs.hpp:140-155, lsLocalLaxFriedrichsAnalytical.hpp:
+  // 122-159), LF with calculateNormalVectors (lsLaxFriedrichs.hpp:127-141)
+  if (scheme == 3 /* S_LLLF */ || scheme == 4 /* S_LLF */ || scheme == 7 /* S_LLFA */ ||
+      (scheme == 2 /* S_LF */ && calcNormals)) {
+    const int ord = (scheme == 7) ? 1 : order;
     double st[D][5];
-    gatherUpwind<D>(b, c, order, st);
+    gatherUpwind<D>(b, c, ord, st);
     double nm = 0;
     LSG_UNROLL
     for (int i = 0; i < D; i++) {
       double diffPos, diffNeg;
-      upwindDiffs(st[i], b.g.dx, order, phi0, diffPos, diffNeg);
+      upwindDiffs(st[i], b.g.dx, ord, phi0, diffPos, diffNeg);
       n[i] = (diffNeg + diffPos) * 0.5;
       nm += n[i] * n[i];
     }

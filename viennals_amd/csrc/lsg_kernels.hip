@@ -1464,7 +1464,7 @@ __device__ inline int alphaMaterial(const double *lv, int L, double value) {
 
 LSG_HD inline int hostRecordsOf(int scheme, int D) {
   if (scheme == S_LF) return 2;
-  if (scheme == S_LLF) return 1 + ((D == 3) ? 27 : 9);
+  if (scheme == S_LLF || scheme == S_LLFA) return 1 + ((D == 3) ? 27 : 9);
   return 1;
 }
 int hostVelocityRecords(int spatial, int D) { return hostRecordsOf(schemeIdOf(spatial), D); }
@@ -1477,7 +1477,9 @@ int hostVelocityRecords(int spatial, int D) { return hostRecordsOf(schemeIdOf(sp
 // lsLocalLaxFriedrichs.hpp:157-162); for LF then findGlobalAlphas' call
 // with its own normal and the point's material (lsAdvect.hpp:145-171); for
 // LLF then the 3^D alpha box, coordinate + offset * gridDelta with the box
-// point's normal (lsLocalLaxFriedrichs.hpp:193-222).  8 doubles per record.
+// point's normal (lsLocalLaxFriedrichs.hpp:193-222); for the analytical LLF
+// the 3^D box with the central differences getDissipationAlpha receives
+// (lsLocalLaxFriedrichsAnalytical.hpp:191-214).  8 doubles per record.
 template <int D>
 __global__ void kVelInputs(BandView b, LowerViews lw, const uint32_t *alist, const uint32_t *nact,
                            const uint32_t *dlist, const uint32_t *ndef, RateParams rp, double *out) {
@@ -1514,11 +1516,12 @@ __global__ void kVelInputs(BandView b, LowerViews lw, const uint32_t *alist, con
       for (int d = 0; d < 3; ++d) r[3 + d] = gn[d];
       r[6] = (double)lo;
       r[7] = (double)alphaMaterial(lv, lw.n + 1, phi0);
-    } else if (scheme == S_LLF) {
+    } else if (scheme == S_LLF || scheme == S_LLFA) {
       for (int q = 0; q < R - 1; ++q) {
         const int ni[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
         double bn[3];
-        llfBoxNormal<D>(b, c, q, bn);
+        if (scheme == S_LLF) llfBoxNormal<D>(b, c, q, bn);
+        else llfaBoxDiffs<D>(b, c, q, bn);
         double *r = o + 8 * (1 + q);
         for (int d = 0; d < 3; ++d) r[d] = (d < D) ? coord[d] + (double)ni[d] * b.g.dx : 0.0;
         for (int d = 0; d < 3; ++d) r[3 + d] = bn[d];
@@ -1579,7 +1582,7 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
       ve.vec[0] = h[1];
       ve.vec[1] = h[2];
       ve.vec[2] = h[3];
-      if (SCHEME == S_LLF) {  // the alpha box follows the scheme's call
+      if (SCHEME == S_LLF || SCHEME == S_LLFA) {  // the alpha box follows the scheme's call
         ve.box = h + 4;
         ve.boxStride = 4;
       }
@@ -2006,7 +2009,7 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
           vm.vec[0] = h[1];
           vm.vec[1] = h[2];
           vm.vec[2] = h[3];
-          if (SCHEME == S_LLF) {
+          if (SCHEME == S_LLF || SCHEME == S_LLFA) {
             vm.box = h + 4 * L;
             vm.boxStride = 4 * L;
           }

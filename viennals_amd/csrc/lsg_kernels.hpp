@@ -9,7 +9,10 @@
 
 namespace lsg {
 
-constexpr int SCAN_BLOCKS = 1024;
+// default grid of the band-operation scans: 1536 blocks measured best at
+// the trench benchmark (1.479 ms per step against 1.506 at 1024, 1.683 at
+// 2048, 2.70 at 256)
+constexpr int SCAN_BLOCKS = 1536;
 constexpr int SCAN_BLOCKS_MAX = 4096;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 1;
@@ -124,7 +127,8 @@ void launchGlobalAlphasMM(hipStream_t s, int D, const BandBufs &b, const Frame &
                           const LowerViews &lw, const RateParams &rp, DevScalars *sc);
 // Records of host velocity calls per active point for a scheme: 1 (the
 // scheme's call), 2 for LF (and findGlobalAlphas' call), 1 + 3^D for LLF
-// (and the alpha box).
+// (and the alpha box) and for the analytical LLF (and its box of
+// getDissipationAlpha calls).
 int hostVelocityRecords(int spatial, int D);
 // Inputs of the host velocity calls: 8 doubles per record (coordinate,
 // normal, point id, material or -1 for every material), records of active
