@@ -338,8 +338,9 @@ int lsg_mlp_get_params(lsg_mlp *m, float *out);
  * the device, every kernel of a step is one launch for all level sets, the
  * step is captured into a HIP graph, and each level set keeps its own grid,
  * velocity and CFL time step.  lsg_batch_apply is Advect::apply with
- * advectionTime 0 (one step) for every level set.  Schemes: Engquist-Osher
- * and WENO.  Results are written back into the domains by
+ * advectionTime 0 (one step) for every level set.  Schemes: Engquist-Osher,
+ * WENO and the Lax-Friedrichs schemes LF, LLF and LLLF (alphas and CFL
+ * bound per level set).  Results are written back into the domains by
  * lsg_batch_write_back. */
 typedef struct lsg_batch lsg_batch;
 int lsg_batch_create(lsg_batch **out, lsg_domain *const *domains, int n);

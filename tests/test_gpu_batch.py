@@ -55,7 +55,8 @@ def individual(vls, sets, vels, steps, spatial, temporal):
     return np.array(times)
 
 
-@pytest.mark.parametrize("spatial,temporal,steps", [(11, 2, 3), (0, 0, 4), (10, 1, 2)])
+@pytest.mark.parametrize("spatial,temporal,steps", [(11, 2, 3), (0, 0, 4), (10, 1, 2), (2, 0, 3), (3, 2, 2),
+                                                    (5, 0, 3), (6, 2, 2), (7, 0, 2), (8, 2, 2)])
 def test_batch_matches_individual_advection(spatial, temporal, steps):
     import viennals_amd as vls
     sets, vels = make_sets(vls)

@@ -295,6 +295,11 @@ struct RateParams {
   // band entry; a void point gets no scheme call (lsAdvect.hpp:510-528)
   const double *voidMarks;
   const uint32_t *voidRank;
+  // batches: per level set the local alpha maxima of the rate pass (LLF,
+  // LLLF) and the global alphas found before it (LF), 3 bit patterns of
+  // non-negative doubles each
+  unsigned long long *segAlpha;
+  unsigned long long *segGA;
 };
 
 // Which point a band operation's result takes its point data from.

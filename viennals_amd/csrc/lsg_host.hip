@@ -1761,6 +1761,7 @@ struct lsg_batch {
   double *rates = nullptr;
   size_t ratesCap = 0;
   unsigned long long *segMax = nullptr;
+  unsigned long long *segAlpha = nullptr, *segGA = nullptr;  // Lax-Friedrichs alphas per level set (3 each)
   double *segMaxDt = nullptr, *segDt = nullptr, *segStepTime = nullptr, *segAdvTime = nullptr;
   uint32_t *bounds = nullptr, *boundsIdx = nullptr;  // pre[] at the level-set word boundaries (write-back)
   double pointUpdates = 0;
@@ -1790,13 +1791,14 @@ struct lsg_batch {
     itmp.release();
     save.release();
     for (void *p : {(void *)segsD, (void *)wsegD, (void *)rates, (void *)segMax, (void *)segMaxDt, (void *)segDt,
-                    (void *)segStepTime, (void *)bounds, (void *)boundsIdx})
+                    (void *)segStepTime, (void *)bounds, (void *)boundsIdx, (void *)segAlpha, (void *)segGA})
       if (p) (void)hipFree(p);
     segsD = nullptr;
     wsegD = nullptr;
     rates = nullptr;
     ratesCap = 0;
     segMax = nullptr;
+    segAlpha = segGA = nullptr;
     segMaxDt = segDt = segStepTime = nullptr;
     bounds = boundsIdx = nullptr;
   }
@@ -1866,6 +1868,8 @@ static int batchAssemble(lsg_batch *b) {
   LSG_HIP(hipMalloc(&b->rates, cap * sizeof(double)));
   b->ratesCap = cap;
   LSG_HIP(hipMalloc(&b->segMax, S * sizeof(unsigned long long)));
+  LSG_HIP(hipMalloc(&b->segAlpha, 3 * S * sizeof(unsigned long long)));
+  LSG_HIP(hipMalloc(&b->segGA, 3 * S * sizeof(unsigned long long)));
   LSG_HIP(hipMalloc(&b->segMaxDt, S * sizeof(double)));
   LSG_HIP(hipMalloc(&b->segDt, 3 * S * sizeof(double)));
   LSG_HIP(hipMalloc(&b->segStepTime, S * sizeof(double)));
@@ -2145,8 +2149,10 @@ static int batchPrepare(lsg_batch *b) {
   for (size_t k = 0; k < b->doms.size(); ++k)
     if (!b->velSet[k]) return fail(-2, "batch: a level set has no velocity");
   const int sid = schemeIdOf(b->spatial);
-  if (prepareWidth(b->spatial) < 0 || !(sid == S_EO || sid == S_WENO3 || sid == S_WENO5))
-    return fail(-4, "batch: only the Engquist-Osher and WENO schemes are supported");
+  if (prepareWidth(b->spatial) < 0 ||
+      !(sid == S_EO || sid == S_WENO3 || sid == S_WENO5 || sid == S_LF || sid == S_LLLF || sid == S_LLF))
+    return fail(-4, "batch: the Engquist-Osher, Lax-Friedrichs (global, local, local-local) and WENO schemes are "
+                    "supported");
   if (!(b->cfl > 0)) return fail(-6, "batch: time step ratio must be positive");
   b->rp = RateParams{};
   b->rp.spatial = b->spatial;
@@ -2161,6 +2167,8 @@ static int batchPrepare(lsg_batch *b) {
       if (!d->framed) return fail(-7, "batch: level set has no data");
     if (int e = batchAssemble(b)) return e;
   }
+  b->rp.segAlpha = b->segAlpha;
+  b->rp.segGA = b->segGA;
   return 0;
 }
 

@@ -1510,14 +1510,30 @@ LSG_HD inline void llfaBoxDiffs(const BandView &b, const int32_t c[3], int q, do
 // Normal at point q of the local Lax-Friedrichs alpha box around c
 // (lsLocalLaxFriedrichs.hpp:193-215): central differences of the boundary
 // mapped neighbours, normalised; q = (ox + 1) + 3 (oy + 1) + 9 (oz + 1).
-template <int D> LSG_HD inline void llfBoxNormal(const BandView &b, const int32_t c[3], int q, double normal[3]) {
+// The 2D + 1 lookups of box point q's normal: slot 2D the box point, dir
+// and D + dir its neighbours along dir (+, -).  With lin >= 0 (mappedLin
+// with R = 2) every point is a frame bit at its mapped offset; otherwise
+// each is located.
+template <int D>
+LSG_HD inline void llfBoxLocs(const BandView &b, const int32_t c[3], int q, int64_t lin, const int64_t off[][5],
+                              BandView::Loc *l) {
   const int ni[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
+  if (lin >= 0) {
+    int64_t base = lin;
+    LSG_UNROLL
+    for (int d = 0; d < D; ++d) base += off[d][2 + ni[d]];
+    l[2 * D] = locAt(base);
+    LSG_UNROLL
+    for (int dir = 0; dir < D; ++dir) {
+      const int64_t here = off[dir][2 + ni[dir]];
+      l[dir] = locAt(base - here + off[dir][3 + ni[dir]]);
+      l[D + dir] = locAt(base - here + off[dir][1 + ni[dir]]);
+    }
+    return;
+  }
   int32_t nc[3] = {0, 0, 0};
   LSG_UNROLL
   for (int d = 0; d < D; ++d) nc[d] = b.g.map(d, (int64_t)c[d] + ni[d]);
-  normal[0] = normal[1] = normal[2] = 0.;
-  double nm = 0.;
-  BandView::Loc l[2 * D + 1];
   l[2 * D] = b.locate(nc[0], nc[1], nc[2]);
   LSG_UNROLL
   for (int dir = 0; dir < D; ++dir) {
@@ -1530,21 +1546,31 @@ template <int D> LSG_HD inline void llfBoxNormal(const BandView &b, const int32_
     l[dir] = b.locate(pp[0], pp[1], pp[2]);
     l[D + dir] = b.locate(pn[0], pn[1], pn[2]);
   }
-  double bv[2 * D + 1];
-  b.values<2 * D + 1>(l, bv);
+}
+// The normal from those values: central differences, normalised.
+template <int D> LSG_HD inline void llfBoxNormalFrom(const double *bv, double dx, double normal[3]) {
+  normal[0] = normal[1] = normal[2] = 0.;
+  double nm = 0.;
   const double center = bv[2 * D];
   LSG_UNROLL
   for (int dir = 0; dir < D; ++dir) {
     const double neg = bv[D + dir];
     const double pos = bv[dir];
-    const double dP = (pos - center) / b.g.dx;
-    const double dN = (center - neg) / b.g.dx;
+    const double dP = (pos - center) / dx;
+    const double dN = (center - neg) / dx;
     normal[dir] = (dP + dN) * 0.5;
     nm += normal[dir] * normal[dir];
   }
   nm = 1. / sqrt(nm);
   LSG_UNROLL
   for (int dir = 0; dir < D; ++dir) normal[dir] *= nm;
+}
+template <int D> LSG_HD inline void llfBoxNormal(const BandView &b, const int32_t c[3], int q, double normal[3]) {
+  BandView::Loc l[2 * D + 1];
+  llfBoxLocs<D>(b, c, q, -1, nullptr, l);
+  double bv[2 * D + 1];
+  b.values<2 * D + 1>(l, bv);
+  llfBoxNormalFrom<D>(bv, b.g.dx, normal);
 }
 
 // LocalLaxFriedrichs::operator() (lsLocalLaxFriedrichs.hpp:65-241)
@@ -1583,17 +1609,49 @@ LSG_HD inline RateOut schemeLLF(const BandView &b, const int32_t c[3], double ph
   LSG_UNROLL
   for (int i = 0; i < D; ++i) alpha[i] = 0;
   constexpr int numNeighbors = (D == 3) ? 27 : 9;
-  LSG_UNROLL
-  for (int q = 0; q < numNeighbors; ++q) {  // box loop (lsLocalLaxFriedrichs.hpp:193-231), x fastest
-    double normal[3];
-    llfBoxNormal<D>(b, c, q, normal);
-    const double scaVel = vel.box ? vel.box[q * vel.boxStride] : vel.scalar(normal);
+  // box loop (lsLocalLaxFriedrichs.hpp:193-231), x fastest; a box point's
+  // 2D + 1 lookups are at fixed offsets from c (mappedLin, radius 2) away
+  // from the frame edge.  G box points per batch of lookups: 1 measured
+  // best (rough1024 LLF2: 534 us per rate pass against 742 us with 3, whose
+  // registers halve the occupancy)
+  int64_t off[3][5];
+  const int64_t lin = mappedLin<D, 2>(b, c, off);
+  constexpr int G = 1, NL = 2 * D + 1;
+#if defined(__clang__)
+#pragma unroll 1
+#endif
+  for (int q0 = 0; q0 < numNeighbors; q0 += G) {
+    BandView::Loc l[G * NL];
     LSG_UNROLL
-    for (int dir = 0; dir < D; ++dir) {
-      const double vv = vel.box ? vel.box[q * vel.boxStride + 1 + dir] : vel.vec[dir];
-      double tempAlpha = fabs((scaVel + vv) * normal[dir]);
-      alpha[dir] = dmaxs(alpha[dir], tempAlpha);
-      alphaAcc[dir] = dmaxs(alphaAcc[dir], tempAlpha);
+    for (int g = 0; g < G; ++g) llfBoxLocs<D>(b, c, q0 + g, lin, off, l + g * NL);
+    double bv[G * NL];
+    b.values<G * NL>(l, bv);
+    LSG_UNROLL
+    for (int g = 0; g < G; ++g) {
+      const int q = q0 + g;
+      double normal[3];
+      llfBoxNormalFrom<D>(bv + g * NL, b.g.dx, normal);
+      // the box velocities: the functor at the box normal, or the host's
+      // values for this box point
+      double sv, vv[3];
+      if (vel.box) {
+        const double *h = vel.box + q * vel.boxStride;
+        sv = h[0];
+        vv[0] = h[1];
+        vv[1] = h[2];
+        vv[2] = h[3];
+      } else {
+        sv = vel.scalar(normal);
+        vv[0] = vel.vec[0];
+        vv[1] = vel.vec[1];
+        vv[2] = vel.vec[2];
+      }
+      LSG_UNROLL
+      for (int dir = 0; dir < D; ++dir) {
+        double tempAlpha = fabs((sv + vv[dir]) * normal[dir]);
+        alpha[dir] = dmaxs(alpha[dir], tempAlpha);
+        alphaAcc[dir] = dmaxs(alphaAcc[dir], tempAlpha);
+      }
     }
   }
   LSG_UNROLL

@@ -1595,6 +1595,14 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
       if (k != curSeg) {
         if (lmax > 0.) atomicMax(&segMax[curSeg], (unsigned long long)__double_as_longlong(lmax));
         lmax = 0.;
+        if (SCHEME == S_LLLF || SCHEME == S_LLF) {  // the level set's local alpha maxima
+          for (int d = 0; d < D; ++d) {
+            if (la[d] > 0.) atomicMax(&rp.segAlpha[3 * curSeg + d], (unsigned long long)__double_as_longlong(la[d]));
+            la[d] = 0.;
+          }
+        }
+        if (SCHEME == S_LF)  // the level set's global alphas
+          for (int d = 0; d < 3; ++d) ga[d] = __longlong_as_double((long long)rp.segGA[3 * k + d]);
         curSeg = k;
       }
     }
@@ -1774,7 +1782,8 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
   RatePointFn<D, SCHEME, SEG> fn{b,   rp,     vel,      rates, mlpW, io, {0, 0, 0}, {0, 0, 0}, 0., 0ull,
                                  sa,  segMax, segFirst, 0ull,  0,    nullptr};
   if (SCHEME == S_LF)
-    for (int i = 0; i < 3; ++i) fn.ga[i] = __longlong_as_double((long long)sc->alphaBits[i]);
+    for (int i = 0; i < 3; ++i)
+      fn.ga[i] = __longlong_as_double((long long)(SEG ? rp.segGA[3 * segFirst + i] : sc->alphaBits[i]));
   if constexpr (SEG) {
     for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) fn(list[j], j);
   } else {
@@ -1833,6 +1842,10 @@ __global__ __launch_bounds__(256) void kRates(BandView b, const uint32_t *list, 
     }
     __syncthreads();
     if (threadIdx.x < NSLOT && slot[threadIdx.x]) atomicMax(&segMax[segFirst + threadIdx.x], slot[threadIdx.x]);
+    if (SCHEME == S_LLLF || SCHEME == S_LLF)
+      for (int d = 0; d < D; ++d)
+        if (fn.la[d] > 0.)
+          atomicMax(&rp.segAlpha[3 * fn.curSeg + d], (unsigned long long)__double_as_longlong(fn.la[d]));
   }
   unsigned long long t = blockSumU64(fn.act);
   if (threadIdx.x == 0 && t) atomicAdd(&sc->activeCount, t);
@@ -2147,43 +2160,98 @@ void launchVelInputs(hipStream_t s, int D, const BandBufs &b, const Frame &f, co
 }
 
 // Batch rate pass: every level set's stage velocities and max |H - D| per
-// level set; then the per-level-set time steps (EO and WENO schemes).
+// level set; then the per-level-set time steps (EO, WENO and the
+// Lax-Friedrichs schemes LF, LLF and LLLF).
 __global__ void kTimeStepSeg(DevScalars *sc, const SegInfo *segs, uint32_t nseg, double cfl, int D,
-                             const unsigned long long *segMax, const double *maxDt, double *dtOut, const uint32_t *nb) {
+                             const unsigned long long *segMax, const double *maxDt, double *dtOut, const uint32_t *nb,
+                             const unsigned long long *alphas) {
   LSG_ABORT_IF_OVERFLOW(sc);
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < nseg) {
     const double m = __longlong_as_double((long long)segMax[k]);
-    dtOut[k] = stageTimeStep(maxDt[k], cfl, m, D, nullptr, false, segs[k].g.dx);
+    double al[3] = {0., 0., 0.};
+    if (alphas)
+      for (int d = 0; d < 3; ++d) al[d] = __longlong_as_double((long long)alphas[3 * k + d]);
+    dtOut[k] = stageTimeStep(maxDt[k], cfl, m, D, al, alphas != nullptr, segs[k].g.dx);
   }
   if (k == 0) {
     sc->totalActive += sc->activeCount;
     sc->rateBytes += 8.0 * (double)(*nb) + 8.0 * (double)sc->activeCount;
   }
 }
-__global__ void kResetRatesSeg(DevScalars *sc, unsigned long long *segMax, uint32_t nseg) {
+__global__ void kResetRatesSeg(DevScalars *sc, unsigned long long *segMax, uint32_t nseg, unsigned long long *segAlpha,
+                               unsigned long long *segGA) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nseg) segMax[k] = 0ull;
+  if (k < nseg) {
+    segMax[k] = 0ull;
+    for (int d = 0; d < 3; ++d) {
+      if (segAlpha) segAlpha[3 * k + d] = 0ull;
+      if (segGA) segGA[3 * k + d] = 0ull;
+    }
+  }
   if (k == 0) sc->activeCount = 0;
+}
+
+// Advect::findGlobalAlphas (lsAdvect.hpp:90-195) per level set of a batch:
+// the maxima over each level set's active points with its own functor.
+// A lane's running maxima belong to its current level set and are flushed
+// when that changes (the active list holds each level set's points
+// together).
+template <int D>
+__global__ __launch_bounds__(256) void kGlobalAlphasSeg(BandView b, const uint32_t *list, const uint32_t *nact,
+                                                        SegArgs sa, unsigned long long *segGA, DevScalars *sc) {
+  LSG_ABORT_IF_OVERFLOW(sc);
+  const uint32_t n = *nact;
+  uint32_t lo, hi;
+  blockChunk(n, lo, hi, 1);
+  uint32_t cur = 0xFFFFFFFFu;
+  double la[3] = {0., 0., 0.};
+  auto flush = [&]() {
+    if (cur == 0xFFFFFFFFu) return;
+    for (int d = 0; d < D; ++d)
+      if (la[d] > 0.) atomicMax(&segGA[3 * cur + d], (unsigned long long)__double_as_longlong(la[d]));
+  };
+  for (uint32_t j = lo + threadIdx.x; j < hi; j += TPB) {
+    const uint32_t i = list[j];
+    const uint32_t k = sa.iseg[i];
+    if (k != cur) {
+      flush();
+      la[0] = la[1] = la[2] = 0.;
+      cur = k;
+    }
+    const SegInfo &S = sa.segs[k];
+    const BandView v = segView(b, S);
+    int32_t c[3];
+    candCoords(v, i, v.pos[i], c);
+    globalAlphaPoint<D>(v, c, S.vel, la);
+  }
+  flush();
 }
 
 void launchRatesSeg(hipStream_t s, int D, const BandBufs &b, const SegCtx &c, const RateParams &rp, double *rates,
                     DevScalars *sc, unsigned long long *segMax, const double *maxDt, double *dtOut, Workspace &ws) {
   const int scheme = schemeIdOf(rp.spatial);
   const int gs = (int)((c.nseg + TPB - 1) / TPB);
-  kResetRatesSeg<<<gs, TPB, 0, s>>>(sc, segMax, c.nseg);
+  kResetRatesSeg<<<gs, TPB, 0, s>>>(sc, segMax, c.nseg, rp.segAlpha, rp.segGA);
   activeList(s, b, ws, sc);
   Frame f{};
   GridInfo g{};
   BandView v = viewOf(b, f, g);
   SegArgs sa = segArgs(c, b.seg, nullptr);
   const int ge = rateBlocks();
+  if (scheme == S_LF) {
+    if (D == 2) kGlobalAlphasSeg<2><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, sa, rp.segGA, sc);
+    else kGlobalAlphasSeg<3><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, sa, rp.segGA, sc);
+  }
 #define LSG_RATES_SEG(DD, SS) kRates<DD, SS, true><<<ge, TPB, 0, s>>>(v, ws.cpos, ws.ncand, rp, rates, sc, sa, segMax)
   if (D == 2) {
     switch (scheme) {
     case S_EO: LSG_RATES_SEG(2, S_EO); break;
     case S_WENO3: LSG_RATES_SEG(2, S_WENO3); break;
     case S_WENO5L: LSG_RATES_SEG(2, S_WENO5L); break;
+    case S_LF: LSG_RATES_SEG(2, S_LF); break;
+    case S_LLLF: LSG_RATES_SEG(2, S_LLLF); break;
+    case S_LLF: LSG_RATES_SEG(2, S_LLF); break;
     default: LSG_RATES_SEG(2, S_WENO5); break;
     }
   } else {
@@ -2191,11 +2259,18 @@ void launchRatesSeg(hipStream_t s, int D, const BandBufs &b, const SegCtx &c, co
     case S_EO: LSG_RATES_SEG(3, S_EO); break;
     case S_WENO3: LSG_RATES_SEG(3, S_WENO3); break;
     case S_WENO5L: LSG_RATES_SEG(3, S_WENO5L); break;
+    case S_LF: LSG_RATES_SEG(3, S_LF); break;
+    case S_LLLF: LSG_RATES_SEG(3, S_LLLF); break;
+    case S_LLF: LSG_RATES_SEG(3, S_LLF); break;
     default: LSG_RATES_SEG(3, S_WENO5); break;
     }
   }
 #undef LSG_RATES_SEG
-  kTimeStepSeg<<<gs, TPB, 0, s>>>(sc, c.segs, c.nseg, rp.cfl, D, segMax, maxDt, dtOut, b.n);
+  // the per-level-set step limits of the Lax-Friedrichs schemes use their
+  // alphas (global ones for LF, the pass's local maxima for LLF and LLLF)
+  const unsigned long long *alphas =
+      (scheme == S_LF) ? rp.segGA : ((scheme == S_LLLF || scheme == S_LLF) ? rp.segAlpha : nullptr);
+  kTimeStepSeg<<<gs, TPB, 0, s>>>(sc, c.segs, c.nseg, rp.cfl, D, segMax, maxDt, dtOut, b.n, alphas);
 }
 
 // Per-level-set step bookkeeping of a batch: stage-1 bound, returned time of
