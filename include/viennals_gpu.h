@@ -229,8 +229,11 @@ int lsg_advect_set_dissipation_alpha_builtin(lsg_advect *a, int material, const 
  * Engquist-Osher, Lax-Friedrichs (also findGlobalAlphas' calls, lsAdvect.hpp:
  * 90-195), local Lax-Friedrichs (also the 3^D alpha box, lsLocalLaxFriedrichs
  * .hpp:184-232), analytical local Lax-Friedrichs (with
- * lsg_advect_set_dissipation_alpha_host), local-local Lax-Friedrichs and WENO
- * schemes; not with the stencil local Lax-Friedrichs scheme. */
+ * lsg_advect_set_dissipation_alpha_host), local-local Lax-Friedrichs, WENO
+ * and stencil local Lax-Friedrichs schemes (the point and its 3^D box, then
+ * scalar() at every box normal moved by -DN and +DN per axis,
+ * lsStencilLocalLaxFriedrichsScalar.hpp:252-338); not with the learned
+ * WENO5 scheme. */
 typedef double (*lsg_scalar_velocity_fn)(const double *coordinate, int material, const double *normal,
                                          uint64_t point_id, void *user);
 typedef void (*lsg_vector_velocity_fn)(const double *coordinate, int material, const double *normal,

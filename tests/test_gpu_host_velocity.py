@@ -34,7 +34,7 @@ def sphere(vls, r=6.0):
 
 
 @pytest.mark.parametrize("spatial,temporal", [(0, 0), (1, 2), (11, 2), (6, 1), (10, 0), (2, 0), (3, 2), (7, 0),
-                                              (8, 1)])
+                                              (8, 1), (9, 0), (9, 2)])
 def test_host_callback_matches_builtin(spatial, temporal):
     import viennals_amd as vls
     f = NormalPower(dir=0, k=2, b=1.0, a_pos=2.3, a_neg=0.5, vec=(0.2, 0.0, -0.1))
@@ -124,11 +124,57 @@ def test_host_callback_llf_box_calls():
             assert calls[28 * j + 1 + q][1] == calls[28 * j][1]
 
 
-def test_host_callback_rejected_with_stencil_llf():
+def test_host_callback_sllf_calls():
+    """The stencil local Lax-Friedrichs scheme calls the velocity at the
+    point (its centre normal), at the 3^D box points (coordinate + offset *
+    gridDelta, the box point's normal) and then, per box point and axis k, at
+    the box normal moved by -DN and +DN along k, DN = cbrt(eps) * gridDelta
+    for a unit velocity (lsStencilLocalLaxFriedrichsScalar.hpp:62-74,
+    252-338): 28 + 27 * 6 calls per active point, the second round after
+    the first."""
+    import viennals_amd as vls
+    d = sphere(vls, 4.0)
+    calls = []
+
+    class Rec(vls.HostVelocityField):
+        def getScalarVelocity(self, coordinate, material, normal, pointId):
+            calls.append((tuple(coordinate), tuple(normal), pointId))
+            return 1.0
+
+    a = vls.Advect(d, Rec())
+    a.setSpatialScheme(9)
+    a.setTemporalScheme(vls.TemporalSchemeEnum.FORWARD_EULER)
+    a.prepareLS()
+    idx, val = d.get_runs()
+    active = (np.abs(val) < 1e300) & (np.abs(val) <= 0.5)
+    a.setSingleStep(True)
+    a.apply()
+    n = int(active.sum())
+    assert len(calls) == (28 + 162) * n
+    first, second = calls[:28 * n], calls[28 * n:]
+    np.testing.assert_array_equal(np.array([first[28 * j][0] for j in range(n)]), idx[active] * 0.5)
+    DN = np.cbrt(np.finfo(np.float64).eps) * 0.5
+    for j in (0, n // 2, n - 1):
+        for q in range(27):
+            coord, ln, pid = first[28 * j + 1 + q]
+            assert pid == first[28 * j][2]
+            for k in range(3):
+                cp, npn, pp = second[162 * j + 6 * q + 2 * k]
+                cn, nnn, pn = second[162 * j + 6 * q + 2 * k + 1]
+                assert cp == coord and cn == coord and pp == pid and pn == pid
+                assert abs(npn[k] - (ln[k] - DN)) <= 1e-15 and abs(nnn[k] - (ln[k] + DN)) <= 1e-15
+                for i in range(3):
+                    if i != k:  # earlier axes moved and moved back, later ones untouched
+                        assert abs(npn[i] - ln[i]) <= 1e-15 and abs(nnn[i] - ln[i]) <= 1e-15
+                        if i > k:
+                            assert npn[i] == ln[i] and nnn[i] == ln[i]
+
+
+def test_host_callback_rejected_with_learned():
     import viennals_amd as vls
     d = sphere(vls, 3.0)
     a = vls.Advect(d, vls.HostVelocityField())
-    a.setSpatialScheme(9)
+    a.setSpatialScheme(12)
     with pytest.raises(vls.LsgError, match="host-evaluated"):
         a.apply()
 
@@ -214,12 +260,13 @@ def test_host_llfa_alpha_box_calls():
     assert not (len(v1) == len(v2) and np.array_equal(v1, v2))
 
 
-@pytest.mark.parametrize("spatial,temporal", [(3, 2), (8, 1), (2, 0), (4, 0), (4, 2)])
+@pytest.mark.parametrize("spatial,temporal", [(3, 2), (8, 1), (2, 0), (4, 0), (4, 2), (9, 0), (9, 2)])
 def test_host_callback_several_level_sets(spatial, temporal):
     """A host callback with a material-dependent velocity equals the
     built-in per-material functors with LF (global alphas over the points'
-    materials), LLF (box velocities per material) and the analytical LLF
-    (box alphas per material) on an etch stop."""
+    materials), LLF (box velocities per material), the analytical LLF
+    (box alphas per material) and the stencil local LF (box and perturbed
+    normal velocities per material) on an etch stop."""
     import viennals_amd as vls
     from test_gpu_multimaterial import plane2
     from test_gpu_parity import to_gpu

@@ -74,14 +74,40 @@ def test_sllf_mask_keeps_lower_level_set(temporal):
     assert cases.bit_equal(before[0], before[1], after[0], after[1])
 
 
-def test_sllf_rejects_host_velocity():
+@pytest.mark.parametrize("temporal", [0, 2])
+def test_sllf_host_velocity_matches_oracle(temporal):
+    """SLLF with a host VelocityField subclass computing the egg velocity
+    (same operations as the built-in functor): the scheme's call, the 3^D
+    box calls and the calls at the box normals moved by -DN and +DN per axis
+    all come from the host -- bit-exact against the oracle's built-in run,
+    so the perturbed normals and DN are the reference's."""
     import viennals_amd as vls
-    say("SLLF with a host velocity field: error")
-    d = to_gpu(cases.sphere(r=3.0, dx=0.5))
-    a = vls.Advect(d, vls.HostVelocityField())
+    vel = dict(cases.VEL_EGG, vec=(0.2, 0.0, -0.1))
+    say(f"SLLF host velocity (egg + vector), temporal {temporal}")
+
+    class Egg(vls.HostVelocityField):
+        def getScalarVelocity(self, coordinate, material, normal, pointId):
+            c = normal[0]
+            a = vel["a_pos"] if c > 0 else vel["a_neg"]
+            ac = abs(c)
+            return vel["b"] + a * (ac * ac)
+
+        def getVectorVelocity(self, coordinate, material, normal, pointId):
+            return vel["vec"]
+
+    o = cases.sphere(r=4.0, dx=0.5, origin=[0.3, 0.1, -0.2])
+    g = to_gpu(o)
+    p = po.make_params(spatial=SLLF, temporal=temporal, cfl=0.4, time=0.6, vel=vel)
+    st, r = po.advect(o, p)
+    assert st == 0
+    a = vls.Advect(g, Egg())
     a.setSpatialScheme(SLLF)
-    with pytest.raises(vls.LsgError, match="host-evaluated"):
-        a.apply()
+    a.setTemporalScheme(temporal)
+    a.setTimeStepRatio(0.4)
+    a.setAdvectionTime(0.6)
+    a.apply()
+    assert a.getAdvectedTime() == r["advected_time"] and a.getNumberOfTimeSteps() == r["steps"]
+    assert_same(o, g, "SLLF host velocity")
 
 
 def _oracle_prepare(levelSets, isDepo):

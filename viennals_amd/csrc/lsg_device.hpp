@@ -275,6 +275,10 @@ struct RateParams {
   // null for the built-in functor
   const double *hostVel;
   int hostRec;
+  // stencil local Lax-Friedrichs with host-evaluated velocities: the
+  // scalar velocities at the perturbed box normals, 2 D per box point and
+  // 2 D 3^D per (active point, material), active point major
+  const double *hostSllf;
   // learned weno5: the network of a wave's points on the matrix cores (1)
   // or per lane on the vector ALUs (0)
   int learnedWave;

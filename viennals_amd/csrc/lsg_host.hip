@@ -918,6 +918,10 @@ struct lsg_advect {
   uint32_t *dlist = nullptr, *ndef = nullptr;
   size_t velCap = 0;
   std::vector<double> velInH, velOutH;
+  // stencil local Lax-Friedrichs: second-round records / results (device)
+  double *sllfIn = nullptr, *sllfOut = nullptr;
+  size_t sllfCap = 0;
+  std::vector<double> sllfInH, sllfOutH;
   int hostErr = 0;
   // Advect::setVelocityUpdateCallback (lsAdvect.hpp:82,966-969)
   lsg_velocity_update_fn velUpdate = nullptr;
@@ -942,7 +946,7 @@ struct lsg_advect {
     if (hostSc) (void)hipHostFree(hostSc);
     if (hostStat) (void)hipHostFree(hostStat);
     if (mlpDev) (void)hipFree(mlpDev);
-    for (void *p : {(void *)velIn, (void *)velOut, (void *)dlist, (void *)ndef})
+    for (void *p : {(void *)velIn, (void *)velOut, (void *)dlist, (void *)ndef, (void *)sllfIn, (void *)sllfOut})
       if (p) (void)hipFree(p);
   }
 };
@@ -1069,6 +1073,33 @@ static int hostVelocities(lsg_advect *a, lsg_domain *d, int nmat = 1, const Lowe
     }
   }
   if (nact) LSG_HIP(hipMemcpy(a->velOut, a->velOutH.data(), nrec * 4 * nmat * sizeof(double), hipMemcpyHostToDevice));
+  if (a->spatial == 9 && nact) {
+    // stencil local Lax-Friedrichs: getScalarVelocity at the box normals
+    // moved by -DN and +DN per axis, DN from the first round's box velocity
+    // (lsStencilLocalLaxFriedrichsScalar.hpp:300-338); records without a
+    // call in the reference (material -2) give 0
+    const size_t nb = (size_t)nact * nmat * (R - 1) * 2 * d->D;
+    if (a->sllfCap < nb) {
+      for (void *p : {(void *)a->sllfIn, (void *)a->sllfOut}) if (p) (void)hipFree(p);
+      a->sllfIn = a->sllfOut = nullptr;
+      a->sllfCap = 0;
+      LSG_HIP(hipMalloc(&a->sllfIn, nb * 8 * sizeof(double)));
+      LSG_HIP(hipMalloc(&a->sllfOut, nb * sizeof(double)));
+      a->sllfCap = nb;
+    }
+    launchSllfInputs(g_ctx.s, d->D, g_ws->w.ncand, nmat, a->velIn, a->velOut, a->rp.sllfBaseEps * d->g.dx,
+                     a->sllfIn);
+    LSG_HIP(hipStreamSynchronize(g_ctx.s));
+    a->sllfInH.resize(nb * 8);
+    a->sllfOutH.resize(nb);
+    LSG_HIP(hipMemcpy(a->sllfInH.data(), a->sllfIn, nb * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < nb; ++r) {
+      const double *in = &a->sllfInH[r * 8];
+      const int m = (int)in[7];
+      a->sllfOutH[r] = (m >= 0 && a->hostScalar) ? a->hostScalar(in, m, in + 3, (uint64_t)in[6], a->hostUser) : 0.0;
+    }
+    LSG_HIP(hipMemcpy(a->sllfOut, a->sllfOutH.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (R == 2) {  // LF: findGlobalAlphas from the second record of every point (lsAdvect.hpp:164-171)
     double al[3] = {0, 0, 0};
     for (uint32_t j = 0; j < nact; ++j) {
@@ -1092,12 +1123,14 @@ static void stageFE(lsg_advect *a, lsg_domain *d, const double *maxDt, double *d
   doExpand(d, prepareWidth(a->spatial));  // Advect::prepareLS (lsAdvect.hpp:973-1015)
   RateParams rp = a->rp;
   rp.hostVel = nullptr;
+  rp.hostSllf = nullptr;
   rp.hostRec = hostVelocityRecords(a->spatial, d->D);
   rp.dissOut = a->saveVelocities ? a->diss : nullptr;
   const bool host = a->hostScalar || a->hostVector;
   if (host) {
     if (!a->hostErr) a->hostErr = hostVelocities(a, d);
     rp.hostVel = a->velOut;
+    rp.hostSllf = a->sllfOut;
   }
   // LF global alphas: from the host calls, or the built-in functor here
   if ((a->spatial == 2 || a->spatial == 3) && !host)
@@ -1458,11 +1491,13 @@ static void stageMM(lsg_advect *a, const double *maxDt, double *dtOut) {
   RateParams rp = a->rp;
   rp.dissOut = a->saveVelocities ? a->diss : nullptr;
   rp.hostVel = nullptr;
+  rp.hostSllf = nullptr;
   rp.hostRec = hostVelocityRecords(a->spatial, top->D);
   const bool host = a->hostScalar || a->hostVector;
   if (host) {
     if (!a->hostErr) a->hostErr = hostVelocities(a, top, L, &lw);
     rp.hostVel = a->velOut;
+    rp.hostSllf = a->sllfOut;
   }
   if ((a->spatial == 2 || a->spatial == 3) && !host)
     launchGlobalAlphasMM(g_ctx.s, top->D, top->cur.b, top->f, top->g, lw, rp, g_ctx.sc);
@@ -1628,9 +1663,8 @@ static int advectApply(lsg_advect *a) {
   a->rp.sllfBaseEps = std::cbrt(std::numeric_limits<double>::epsilon());
   // WENO receives dissipationAlpha as its calcNormal flag (lsAdvect.hpp:670-676)
   a->rp.calcNormals = (a->spatial >= 10) ? (a->dissipationAlpha != 0.0) : a->calcNormals;
-  if ((a->hostScalar || a->hostVector) && (a->spatial == 9 || a->spatial == 12))
-    return fail(-9, "Advect: host-evaluated velocities are supported with the Engquist-Osher, Lax-Friedrichs, "
-                    "local Lax-Friedrichs (also analytical), local-local Lax-Friedrichs and WENO schemes");
+  if ((a->hostScalar || a->hostVector) && a->spatial == 12)
+    return fail(-9, "Advect: host-evaluated velocities are not supported with the learned WENO5 scheme");
   if (a->spatial == 12 && !a->mlpDev)
     return fail(-8, "Advect: learned WENO5 selected but no network parameters were set");
   a->rp.mlp = a->mlpDev;

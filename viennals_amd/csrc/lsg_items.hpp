@@ -863,6 +863,10 @@ struct Vel {
   // direction at box[q * boxStride + 1 + direction]
   const double *box = nullptr;
   int boxStride = 4;
+  // stencil local Lax-Friedrichs: the host's getScalarVelocity at the
+  // perturbed normals of box point q, axis k: sllf[(q * D + k) * 2] at
+  // normal - DN e_k, sllf[(q * D + k) * 2 + 1] at normal + DN e_k
+  const double *sllf = nullptr;
   // built-in VelocityField::getScalarVelocity (lsVelocityField.hpp:17-21)
   LSG_HD __attribute__((always_inline)) double scalar(const double n[3]) const {
     if (kind == 0) return b;
@@ -1692,6 +1696,43 @@ template <int D> LSG_HD inline void gatherBox(const BandView &b, const int32_t c
 // A box point whose normal is below 1e-6 in every component ends the alpha
 // loop (its `continue` skips incrementIndex: the later slots stay zero).
 // `used` is set when the point sets finalAlphas (scalar velocity != 0).
+#define LSG_BOX(x, y, z) box[((x) + 1) + 3 * ((y) + 1) + (D == 3 ? 9 * ((z) + 1) : 0)]
+// FiniteDifferences<T, FIRST_ORDER>::calculateGradient at a box offset
+// (lsFiniteDifferences.hpp:177-181), over the 3^D box values of gatherBox
+template <int D>
+LSG_HD inline void sllfGradient(const double *box, double dx, int ox, int oy, int oz, double *g) {
+  g[0] = ((LSG_BOX(1, oy, oz) - LSG_BOX(0, oy, oz)) / dx + (LSG_BOX(0, oy, oz) - LSG_BOX(-1, oy, oz)) / dx) * 0.5;
+  g[1] = ((LSG_BOX(ox, 1, oz) - LSG_BOX(ox, 0, oz)) / dx + (LSG_BOX(ox, 0, oz) - LSG_BOX(ox, -1, oz)) / dx) * 0.5;
+  g[2] = 0.;
+  if (D == 3)
+    g[2] = ((LSG_BOX(ox, oy, 1) - LSG_BOX(ox, oy, 0)) / dx + (LSG_BOX(ox, oy, 0) - LSG_BOX(ox, oy, -1)) / dx) * 0.5;
+}
+// calculateNormal (lsStencilLocalLaxFriedrichsScalar.hpp:142-163)
+template <int D>
+LSG_HD inline void sllfNormal(const double *box, double dx, int ox, int oy, int oz, double *n) {
+  double g[3];
+  sllfGradient<D>(box, dx, ox, oy, oz, g);
+  double modulus = 0.;
+  LSG_UNROLL
+  for (int i = 0; i < D; ++i) modulus += g[i] * g[i];
+  modulus = 1. / sqrt(modulus);
+  n[0] = n[1] = n[2] = 0.;
+  LSG_UNROLL
+  for (int i = 0; i < D; ++i) n[i] = g[i] * modulus;
+}
+// A box normal below 1e-6 in every component (:283-288)
+LSG_HD inline bool sllfCorrupt(const double n[3]) {
+  return fabs(n[0]) < 1e-6 && fabs(n[1]) < 1e-6 && fabs(n[2]) < 1e-6;
+}
+// Normal offset of the simple adaptive epsilon (:62-74) for a box point's
+// scalar velocity
+LSG_HD inline double sllfEpsilon(double baseEps, double lsv) {
+  const double absVel = fabs(lsv);
+  double DN = baseEps;
+  if (absVel > 1e-10) DN = baseEps * dmaxs(0.1, dmins(10.0, absVel));
+  return DN;
+}
+
 template <int D>
 LSG_HD inline RateOut schemeSLLF(const BandView &b, const int32_t c[3], const Vel &vel, double baseNormalEps,
                                  double *finalAlphas, bool &used) {
@@ -1700,27 +1741,8 @@ LSG_HD inline RateOut schemeSLLF(const BandView &b, const int32_t c[3], const Ve
   gatherBox<D>(b, c, box);
   const double dx = b.g.dx;
   used = false;
-#define LSG_BOX(x, y, z) box[((x) + 1) + 3 * ((y) + 1) + (D == 3 ? 9 * ((z) + 1) : 0)]
-  // FiniteDifferences<T, FIRST_ORDER>::calculateGradient (lsFiniteDifferences.hpp:177-181)
-  auto gradient = [&](int ox, int oy, int oz, double *g) {
-    g[0] = ((LSG_BOX(1, oy, oz) - LSG_BOX(0, oy, oz)) / dx + (LSG_BOX(0, oy, oz) - LSG_BOX(-1, oy, oz)) / dx) * 0.5;
-    g[1] = ((LSG_BOX(ox, 1, oz) - LSG_BOX(ox, 0, oz)) / dx + (LSG_BOX(ox, 0, oz) - LSG_BOX(ox, -1, oz)) / dx) * 0.5;
-    g[2] = 0.;
-    if (D == 3)
-      g[2] = ((LSG_BOX(ox, oy, 1) - LSG_BOX(ox, oy, 0)) / dx + (LSG_BOX(ox, oy, 0) - LSG_BOX(ox, oy, -1)) / dx) * 0.5;
-  };
-  // calculateNormal (:142-163)
-  auto normalAt = [&](int ox, int oy, int oz, double *n) {
-    double g[3];
-    gradient(ox, oy, oz, g);
-    double modulus = 0.;
-    LSG_UNROLL
-    for (int i = 0; i < D; ++i) modulus += g[i] * g[i];
-    modulus = 1. / sqrt(modulus);
-    n[0] = n[1] = n[2] = 0.;
-    LSG_UNROLL
-    for (int i = 0; i < D; ++i) n[i] = g[i] * modulus;
-  };
+  auto gradient = [&](int ox, int oy, int oz, double *g) { sllfGradient<D>(box, dx, ox, oy, oz, g); };
+  auto normalAt = [&](int ox, int oy, int oz, double *n) { sllfNormal<D>(box, dx, ox, oy, oz, n); };
   double normal[3];
   normalAt(0, 0, 0, normal);
   double sv = vel.scalar(normal);
@@ -1741,22 +1763,22 @@ LSG_HD inline RateOut schemeSLLF(const BandView &b, const int32_t c[3], const Ve
     const int ox = q % 3 - 1, oy = (q / 3) % 3 - 1, oz = (D == 3) ? q / 9 - 1 : 0;
     double ln[3];
     normalAt(ox, oy, oz, ln);
-    alive = alive && !(fabs(ln[0]) < 1e-6 && fabs(ln[1]) < 1e-6 && fabs(ln[2]) < 1e-6);
+    alive = alive && !sllfCorrupt(ln);
     if (!alive) continue;
-    double lsv = vel.scalar(ln);
+    // the box point's velocities: the functor's, or the host's box record
+    const double *hb = vel.box ? vel.box + q * vel.boxStride : nullptr;
+    double lsv = hb ? hb[0] : vel.scalar(ln);
     LSG_UNROLL
-    for (int d = 0; d < D; ++d) lsv += vel.vec[d] * ln[d];
-    const double absVel = fabs(lsv);
-    double DN = baseEps;
-    if (absVel > 1e-10) DN = baseEps * dmaxs(0.1, dmins(10.0, absVel));
+    for (int d = 0; d < D; ++d) lsv += (hb ? hb[1 + d] : vel.vec[d]) * ln[d];
+    const double DN = sllfEpsilon(baseEps, lsv);
     double np[3] = {ln[0], ln[1], ln[2]}, nn[3] = {ln[0], ln[1], ln[2]};
     double vd[3] = {0., 0., 0.};
     LSG_UNROLL
     for (int k = 0; k < D; ++k) {  // :324-338
       np[k] -= DN;
       nn[k] += DN;
-      const double vp = vel.scalar(np);
-      const double vn = vel.scalar(nn);
+      const double vp = vel.sllf ? vel.sllf[(q * D + k) * 2] : vel.scalar(np);
+      const double vn = vel.sllf ? vel.sllf[(q * D + k) * 2 + 1] : vel.scalar(nn);
       vd[k] = (vn - vp) / (2.0 * DN);
       np[k] += DN;
       nn[k] -= DN;
@@ -1824,6 +1846,12 @@ template <int D> LSG_HD inline void schemeNormal(const BandView &b, const int32_
     nm = 1. / sqrt(nm);
     LSG_UNROLL
     for (int i = 0; i < D; ++i) n[i] *= nm;
+    return;
+  }
+  if (scheme == 8 /* S_SLLF */) {  // calculateNormal at the centre (lsStencilLocalLaxFriedrichsScalar.hpp:252)
+    double box[(D == 3) ? 27 : 9];
+    gatherBox<D>(b, c, box);
+    sllfNormal<D>(box, b.g.dx, 0, 0, 0, n);
     return;
   }
   if (!calcNormals || scheme == 2 /* S_LF */) return;

@@ -1464,7 +1464,7 @@ __device__ inline int alphaMaterial(const double *lv, int L, double value) {
 
 LSG_HD inline int hostRecordsOf(int scheme, int D) {
   if (scheme == S_LF) return 2;
-  if (scheme == S_LLF || scheme == S_LLFA) return 1 + ((D == 3) ? 27 : 9);
+  if (scheme == S_LLF || scheme == S_LLFA || scheme == S_SLLF) return 1 + ((D == 3) ? 27 : 9);
   return 1;
 }
 int hostVelocityRecords(int spatial, int D) { return hostRecordsOf(schemeIdOf(spatial), D); }
@@ -1479,7 +1479,9 @@ int hostVelocityRecords(int spatial, int D) { return hostRecordsOf(schemeIdOf(sp
 // LLF then the 3^D alpha box, coordinate + offset * gridDelta with the box
 // point's normal (lsLocalLaxFriedrichs.hpp:193-222); for the analytical LLF
 // the 3^D box with the central differences getDissipationAlpha receives
-// (lsLocalLaxFriedrichsAnalytical.hpp:191-214).  8 doubles per record.
+// (lsLocalLaxFriedrichsAnalytical.hpp:191-214); for the stencil local
+// Lax-Friedrichs scheme the 3^D box with the box points' first-order normals
+// (lsStencilLocalLaxFriedrichsScalar.hpp:276-298).  8 doubles per record.
 template <int D>
 __global__ void kVelInputs(BandView b, LowerViews lw, const uint32_t *alist, const uint32_t *nact,
                            const uint32_t *dlist, const uint32_t *ndef, RateParams rp, double *out) {
@@ -1516,12 +1518,15 @@ __global__ void kVelInputs(BandView b, LowerViews lw, const uint32_t *alist, con
       for (int d = 0; d < 3; ++d) r[3 + d] = gn[d];
       r[6] = (double)lo;
       r[7] = (double)alphaMaterial(lv, lw.n + 1, phi0);
-    } else if (scheme == S_LLF || scheme == S_LLFA) {
+    } else if (scheme == S_LLF || scheme == S_LLFA || scheme == S_SLLF) {
+      double box[(D == 3) ? 27 : 9];
+      if (scheme == S_SLLF) gatherBox<D>(b, c, box);
       for (int q = 0; q < R - 1; ++q) {
         const int ni[3] = {q % 3 - 1, (q / 3) % 3 - 1, q / 9 - 1};
         double bn[3];
         if (scheme == S_LLF) llfBoxNormal<D>(b, c, q, bn);
-        else llfaBoxDiffs<D>(b, c, q, bn);
+        else if (scheme == S_LLFA) llfaBoxDiffs<D>(b, c, q, bn);
+        else sllfNormal<D>(box, b.g.dx, ni[0], ni[1], D == 3 ? ni[2] : 0, bn);
         double *r = o + 8 * (1 + q);
         for (int d = 0; d < 3; ++d) r[d] = (d < D) ? coord[d] + (double)ni[d] * b.g.dx : 0.0;
         for (int d = 0; d < 3; ++d) r[3 + d] = bn[d];
@@ -1530,6 +1535,63 @@ __global__ void kVelInputs(BandView b, LowerViews lw, const uint32_t *alist, con
       }
     }
   }
+}
+
+// Second round of the stencil local Lax-Friedrichs host calls: for active
+// point j, material m and box point q, the scalar velocity at the box normal
+// moved by -DN and +DN along each axis k (lsStencilLocalLaxFriedrichsScalar.
+// hpp:318-338), DN from the box point's velocity of the first round (the
+// simple adaptive epsilon, :62-74), the axis moves replayed in the
+// reference's order (each undone before the next axis).  Record
+// ((j L + m) 3^D + q) 2 D + 2 k + s, 8 doubles: coordinate, normal, point id,
+// material m -- or -2 where the reference makes no call: the point's scalar
+// velocity is 0 (:264-266) or an earlier box point's normal was corrupted
+// (:283-288).  `in` and `vel` are the first round's records and results.
+template <int D>
+__global__ void kSllfInputs(const uint32_t *nact, int L, const double *in, const double *vel, double baseEps,
+                            double *out) {
+  constexpr int NB = (D == 3) ? 27 : 9, R = 1 + NB;
+  const uint32_t n = *nact;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n * (uint32_t)L; t += gridDim.x * blockDim.x) {
+    const uint32_t j = t / (uint32_t)L;
+    const int m = (int)(t % (uint32_t)L);
+    const double *c0 = in + 8 * (size_t)j * R;
+    const double *h0 = vel + ((size_t)j * R * L + m) * 4;
+    double sv = h0[0];
+    for (int i = 0; i < D; ++i) sv += h0[1 + i] * c0[3 + i];
+    bool alive = sv != 0.;
+    double *o = out + 8 * (size_t)t * NB * 2 * D;  // 8 doubles per record
+    for (int q = 0; q < NB; ++q) {
+      const double *r = c0 + 8 * (1 + q);
+      const double *h = vel + (((size_t)j * R + 1 + q) * L + m) * 4;
+      const double ln[3] = {r[3], r[4], r[5]};
+      alive = alive && !sllfCorrupt(ln);
+      double lsv = h[0];
+      for (int d = 0; d < D; ++d) lsv += h[1 + d] * ln[d];
+      const double DN = sllfEpsilon(baseEps, lsv);
+      double np[3] = {ln[0], ln[1], ln[2]}, nn[3] = {ln[0], ln[1], ln[2]};
+      for (int k = 0; k < D; ++k) {
+        np[k] -= DN;
+        nn[k] += DN;
+        for (int sgn = 0; sgn < 2; ++sgn) {
+          double *w = o + 8 * ((q * D + k) * 2 + sgn);
+          const double *nv = sgn ? nn : np;
+          for (int d = 0; d < 3; ++d) w[d] = r[d];
+          for (int d = 0; d < 3; ++d) w[3 + d] = nv[d];
+          w[6] = r[6];
+          w[7] = alive ? (double)m : -2.0;
+        }
+        np[k] += DN;
+        nn[k] -= DN;
+      }
+    }
+  }
+}
+
+void launchSllfInputs(hipStream_t s, int D, const uint32_t *nact, int L, const double *in, const double *vel,
+                      double baseEps, double *out) {
+  if (D == 2) kSllfInputs<2><<<rateBlocks(), TPB, 0, s>>>(nact, L, in, vel, baseEps, out);
+  else kSllfInputs<3><<<rateBlocks(), TPB, 0, s>>>(nact, L, in, vel, baseEps, out);
 }
 
 // Hamiltonian + dissipation per active entry of the expanded band
@@ -1582,10 +1644,11 @@ template <int D, int SCHEME, bool SEG> struct RatePointFn {
       ve.vec[0] = h[1];
       ve.vec[1] = h[2];
       ve.vec[2] = h[3];
-      if (SCHEME == S_LLF || SCHEME == S_LLFA) {  // the alpha box follows the scheme's call
+      if (SCHEME == S_LLF || SCHEME == S_LLFA || SCHEME == S_SLLF) {  // the alpha box follows the scheme's call
         ve.box = h + 4;
         ve.boxStride = 4;
       }
+      if (SCHEME == S_SLLF) ve.sllf = rp.hostSllf + (size_t)j * ((D == 3) ? 27 : 9) * 2 * D;
     }
     if (SEG) {
       const uint32_t k = sa.iseg[i];
@@ -2022,10 +2085,11 @@ __global__ __launch_bounds__(256) void kRatesMM(BandView b, LowerViews lw, const
           vm.vec[0] = h[1];
           vm.vec[1] = h[2];
           vm.vec[2] = h[3];
-          if (SCHEME == S_LLF || SCHEME == S_LLFA) {
+          if (SCHEME == S_LLF || SCHEME == S_LLFA || SCHEME == S_SLLF) {
             vm.box = h + 4 * L;
             vm.boxStride = 4 * L;
           }
+          if (SCHEME == S_SLLF) vm.sllf = rp.hostSllf + ((size_t)j * L + mat) * ((D == 3) ? 27 : 9) * 2 * D;
         }
         bool used = false;
         v = ratePoint<D, SCHEME>(b, c, phi0, rp, vm, ga, la, velocity, nullptr, nullptr, &used, &dn);

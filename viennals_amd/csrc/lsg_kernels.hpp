@@ -136,6 +136,11 @@ int hostVelocityRecords(int spatial, int D);
 void launchVelInputs(hipStream_t s, int D, const BandBufs &b, const Frame &f, const GridInfo &g, const RateParams &rp,
                      const LowerViews &lw, Workspace &ws, uint32_t *dlist, uint32_t *ndef, double *out,
                      DevScalars *sc);
+// Second round of the stencil local Lax-Friedrichs host calls: records at
+// the perturbed box normals (2 D per box point, 8 doubles each) from the
+// first round's records `in` and results `vel`, for L materials.
+void launchSllfInputs(hipStream_t s, int D, const uint32_t *nact, int L, const double *in, const double *vel,
+                      double baseEps, double *out);
 void launchSetFlag(hipStream_t s, uint32_t *flag, uint32_t v);
 
 // Band operations (all asynchronous on `s`; sizes live in device memory).
