@@ -130,11 +130,26 @@ __device__ inline void blockChunk(uint32_t n, uint32_t &lo, uint32_t &hi, uint32
   if (hi > n) hi = n;
 }
 
-// Sum of partials[0 .. logical block) computed by the whole block.
+// Sum of partials[0 .. logical block) computed by the whole block.  The
+// partials (SCAN_BLOCKS_MAX + 1 counts, 256-byte aligned) are read four at a
+// time, every thread's loads issued before the first is used (a loop that
+// adds each count as it arrives waits once per load).
 __device__ inline uint32_t blockOffset(const uint32_t *partials) {
-  uint32_t s = 0;
+  static_assert(SCAN_BLOCKS_MAX <= 4 * 4 * TPB, "four uint4 loads per thread cover the partials");
   const uint32_t me = logicalBlock();
-  for (uint32_t k = threadIdx.x; k < me; k += TPB) s += partials[k];
+  const uint4 *p4 = reinterpret_cast<const uint4 *>(partials);
+  uint4 v[4];
+  LSG_UNROLL
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t q = threadIdx.x + (uint32_t)r * TPB;
+    v[r] = (4u * q < me) ? p4[q] : make_uint4(0u, 0u, 0u, 0u);
+  }
+  uint32_t s = 0;
+  LSG_UNROLL
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t k = 4u * (threadIdx.x + (uint32_t)r * TPB);
+    s += (k < me ? v[r].x : 0u) + (k + 1 < me ? v[r].y : 0u) + (k + 2 < me ? v[r].z : 0u) + (k + 3 < me ? v[r].w : 0u);
+  }
   return blockSum(s);
 }
 
