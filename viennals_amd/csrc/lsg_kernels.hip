@@ -240,35 +240,52 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t
         }
       }
     }
-    // each wave writes the positions of its own non-empty words (read from
-    // the owning lanes), one word per iteration with lane = bit, so
-    // consecutive lanes write consecutive slots.  Slot arithmetic is 32-bit:
-    // candidates are distinct frame bits (< 2^32) and the candidate capacity
-    // is below 2^30, so q < ccap also bounds the byte offset.
+    // the tile's words are dealt to the waves round robin (slot wid + 4 lane:
+    // a chunk shorter than the tile leaves no wave with all of them); each
+    // wave writes the positions of its non-empty words one word per
+    // iteration (read from the owning lanes) with lane = bit, so consecutive
+    // lanes write consecutive slots.  Slot arithmetic is 32-bit: candidates
+    // are distinct frame bits (< 2^32) and the candidate capacity is below
+    // 2^30, so q < ccap also bounds the byte offset.
+    __shared__ uint64_t tileWord[TPB];
+    __shared__ uint32_t tilePre[TPB];
+    tileWord[threadIdx.x] = (w < hi) ? c : 0ull;
+    tilePre[threadIdx.x] = pl;
+    __syncthreads();
+    const uint32_t slot = (uint32_t)(threadIdx.x >> 6) + (uint32_t)(TPB / 64) * (uint32_t)lane;
+    const uint64_t cs = tileWord[slot];
+    const uint32_t ps = tilePre[slot];
+    uint32_t shs = 0, lws = t + slot, kss = 0;
+    if (SEG && cs) {
+      kss = sa.wseg[t + slot];
+      const SegInfo &S = sa.segs[kss];
+      shs = S.baseIncl;
+      lws = t + slot - S.wordOff;  // positions are level-set local
+    }
     const uint32_t q0 = (SEG ? 0u : (uint32_t)base) + off;
     const uint32_t cap32 = (uint32_t)ccap;
     const uint32_t lsh = (uint32_t)lane & 31u;
     const uint32_t belowLo = lane < 32 ? ((1u << lsh) - 1u) : 0xFFFFFFFFu;
     const uint32_t belowHi = lane < 32 ? 0u : ((1u << lsh) - 1u);
-    uint64_t todo = (uint64_t)__ballot(c != 0ull);
+    uint64_t todo = (uint64_t)__ballot(cs != 0ull);
     while (todo) {
       const int j = __builtin_ctzll(todo);
       todo &= todo - 1ull;
-      const uint32_t cwLo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
-      const uint32_t cwHi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j);
-      const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)pl, j);
-      const uint32_t word = SEG ? (uint32_t)__builtin_amdgcn_readlane((int)lw, j)
-                                : t + (uint32_t)(threadIdx.x & ~63) + (uint32_t)j;
-      const uint32_t shj = SEG ? (uint32_t)__builtin_amdgcn_readlane((int)sh, j) : 0u;
+      const uint32_t cwLo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)cs, j);
+      const uint32_t cwHi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(cs >> 32), j);
+      const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)ps, j);
+      const uint32_t word = (uint32_t)__builtin_amdgcn_readlane((int)lws, j);
+      const uint32_t shj = SEG ? (uint32_t)__builtin_amdgcn_readlane((int)shs, j) : 0u;
       if ((((lane < 32) ? cwLo : cwHi) >> lsh) & 1u) {
         const uint32_t q = q0 + pj + (uint32_t)__builtin_popcount(cwLo & belowLo) +
                            (uint32_t)__builtin_popcount(cwHi & belowHi) + shj;
         if (q < cap32) {
           *(uint32_t *)((char *)cpos + (q << 2)) = (word << 6) | (uint32_t)lane;
-          if (SEG) *(uint16_t *)((char *)cseg + (q << 1)) = (uint16_t)__builtin_amdgcn_readlane((int)k, j);
+          if (SEG) *(uint16_t *)((char *)cseg + (q << 1)) = (uint16_t)__builtin_amdgcn_readlane((int)kss, j);
         }
       }
     }
+    __syncthreads();  // the next tile rewrites tileWord / tilePre
     off += tot;
   }
   if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) {
