@@ -240,21 +240,31 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t
         }
       }
     }
-    // the tile's words are dealt to the waves round robin (slot wid + 4 lane:
-    // a chunk shorter than the tile leaves no wave with all of them); each
-    // wave writes the positions of its non-empty words one word per
-    // iteration (read from the owning lanes) with lane = bit, so consecutive
-    // lanes write consecutive slots.  Slot arithmetic is 32-bit: candidates
-    // are distinct frame bits (< 2^32) and the candidate capacity is below
-    // 2^30, so q < ccap also bounds the byte offset.
+    // a full tile gives each wave its own 64 words; a partial one (a chunk
+    // shorter than the tile: 65 words per block at the trench256 size) is
+    // dealt to the waves through LDS as contiguous ranges of ceil(valid / 4)
+    // words, so no wave is left with all of them.  Each wave then writes the
+    // positions of its non-empty words one word per iteration (read from the
+    // owning lanes) with lane = bit, so consecutive lanes write consecutive
+    // slots.  Slot arithmetic is 32-bit: candidates are distinct frame bits
+    // (< 2^32) and the candidate capacity is below 2^30, so q < ccap also
+    // bounds the byte offset.
     __shared__ uint64_t tileWord[TPB];
     __shared__ uint32_t tilePre[TPB];
-    tileWord[threadIdx.x] = (w < hi) ? c : 0ull;
-    tilePre[threadIdx.x] = pl;
-    __syncthreads();
-    const uint32_t slot = (uint32_t)(threadIdx.x >> 6) + (uint32_t)(TPB / 64) * (uint32_t)lane;
-    const uint64_t cs = tileWord[slot];
-    const uint32_t ps = tilePre[slot];
+    const uint32_t valid = (hi - t < (uint32_t)TPB) ? hi - t : (uint32_t)TPB;  // uniform
+    uint64_t cs = c;
+    uint32_t ps = pl, slot = threadIdx.x;
+    if (valid < (uint32_t)TPB) {
+      tileWord[threadIdx.x] = (w < hi) ? c : 0ull;
+      tilePre[threadIdx.x] = pl;
+      __syncthreads();
+      const uint32_t per = (valid + TPB / 64 - 1) / (TPB / 64);
+      slot = (uint32_t)(threadIdx.x >> 6) * per + (uint32_t)lane;
+      const bool mine = (uint32_t)lane < per && slot < valid;
+      cs = mine ? tileWord[slot] : 0ull;
+      ps = mine ? tilePre[slot] : 0u;
+      __syncthreads();  // the next tile rewrites tileWord / tilePre
+    }
     uint32_t shs = 0, lws = t + slot, kss = 0;
     if (SEG && cs) {
       kss = sa.wseg[t + slot];
@@ -285,7 +295,6 @@ __global__ __launch_bounds__(256) void kCandList(const uint64_t *cbits, uint32_t
         }
       }
     }
-    __syncthreads();  // the next tile rewrites tileWord / tilePre
     off += tot;
   }
   if (logicalBlock() == gridDim.x - 1 && threadIdx.x == 0) {
