@@ -1656,15 +1656,16 @@ static int advectApply(lsg_advect *a) {
   if (!d->framed) return fail(-7, "Advect: level set has no data");
   // scan grid by band size for this apply (LSG_SCAN_BLOCKS overrides it):
   // the default up to band stores of 8M entries (about 1M band entries),
-  // twice it beyond (rough1024 LLF2: 11.70 ms per step at 3072 blocks
-  // against 12.39 ms at 1536; trench256 WENO5 is fastest at 1536, 1.44 ms
-  // against 1.72 ms at 3072).  Every step graph keeps the grid it was
-  // captured with, so its kernels agree on the per-block counts.
+  // the largest grid beyond (rough1024 LLF2: 11.47 ms per step at 4096
+  // blocks, 11.70 at 3072, 12.17 at 2304, 12.39 at 1536; trench256 WENO5
+  // is fastest at 1536, 1.44 ms against 1.72 ms at 3072).  Every step graph
+  // keeps the grid it was captured with, so its kernels agree on the
+  // per-block counts.
   struct ScanGridScope {
     ~ScanGridScope() { setScanBlocks(defaultScanBlocks()); }
   } scanGridScope;
   if (!getenv("LSG_SCAN_BLOCKS"))
-    setScanBlocks(d->cur.b.cap >= ((size_t)8 << 20) ? 2 * SCAN_BLOCKS : SCAN_BLOCKS);
+    setScanBlocks(d->cur.b.cap >= ((size_t)8 << 20) ? SCAN_BLOCKS_MAX : SCAN_BLOCKS);
   if (a->cfl >= 0.5) fprintf(stderr, "[viennals_gpu] Integration time step ratio should be smaller than 0.5. Advection might fail!\n");
   a->rp.spatial = a->spatial;
   a->rp.order = schemeOrder(a->spatial);
